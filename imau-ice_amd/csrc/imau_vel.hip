@@ -1,0 +1,996 @@
+// imau_vel.hip -- host side of libimauvel.so: the C ABI of include/imau_vel.h,
+// device-resident problems (one grid or an uneven-size ensemble), and the
+// Picard driver of solve_DIVA / solve_SSA / solve_SIA
+// (src/ice_velocity_module.f90:36-374).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/imau_vel.h"
+#include "dev_common.h"
+#include "kernels.h"
+
+namespace {
+
+struct Err {
+  int code;
+  std::string msg;
+};
+
+#define HIPCHK(x)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) throw Err{IMAU_ERR_DEVICE, std::string(#x ": ") + hipGetErrorString(e_)}; \
+  } while (0)
+
+[[noreturn]] void fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  throw Err{code, buf};
+}
+
+// vertical_integrate, src/utilities_module.f90:73-96
+double vint(int nz, const double *z, const double *f) {
+  double s = 0.0;
+  for (int k = 1; k < nz; ++k) s += 0.5 * (f[k] + f[k - 1]) * (z[k] - z[k - 1]);
+  return s;
+}
+
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    release();
+    n = count;
+    if (count) {
+      HIPCHK(hipMalloc((void **)&p, count * sizeof(T)));
+      HIPCHK(hipMemset(p, 0, count * sizeof(T)));
+    }
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DBuf() { release(); }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct imau_problem {
+  int ngrids = 0, nz = 0;
+  std::vector<Member> mem;
+  std::vector<Tile> tiles;
+  std::vector<int> tstart;
+  std::vector<int> sheet_any;  // member has grounded ice (else trivial solve)
+  long long na = 0;            // total 2D slab size
+
+  DBuf<Member> mem_d;
+  DBuf<Tile> tiles_d;
+  DBuf<int> tstart_d;
+  DBuf<double> xg, yg;
+  DBuf<double> f2;    // all double 2D fields (arena)
+  DBuf<int32_t> i2;   // int 2D fields
+  DBuf<uint8_t> rt;   // row types
+  DBuf<double> f3;    // 3D fields (arena)
+  DBuf<double> stage;  // host-layout staging for 3D transfers
+  DBuf<double> partial;
+  DBuf<double> msum;
+  DBuf<unsigned long long> resmax;
+  DBuf<int> act_pic, act_lin;
+
+  Fields F{};
+  OutF O{};
+  double *Acx3 = nullptr, *Acy3 = nullptr;
+  double *visc_cur = nullptr, *visc_oth = nullptr;
+  // uploaded state, for reset_state
+  double *u0 = nullptr, *v0 = nullptr, *tbx0 = nullptr, *tby0 = nullptr, *visc0 = nullptr;
+  double *u3a0 = nullptr, *v3a0 = nullptr;
+  bool has_ti = false;
+  int dyn_uploaded = 0;
+  // per-member results
+  std::vector<int> picard_its;
+  std::vector<long long> sweeps;
+  std::vector<double> resid_uv, nl_resid;
+  double seconds = 0.0;
+  // per-call linear-solver settings (ice%DIVA_SOR_*, ice%DIVA_PETSc_*) of member 0
+  int sor_nit_ = 10000;
+  double sor_tol_ = 2.5, sor_omega_ = 1.3, petsc_rtol_ = 0.01, petsc_abstol_ = 2.5;
+};
+
+struct imau_ctx {
+  int device = 0;
+  hipStream_t s = nullptr;
+  std::string err;
+  imau_problem *cached = nullptr;
+};
+
+static std::string g_init_err;
+
+// ---------------------------------------------------------------------------
+// problem construction
+// ---------------------------------------------------------------------------
+namespace {
+
+// named 2D double fields in the arena
+enum F2 {
+  F_Hi, F_Hb, F_SL, F_Hs, F_TAF, F_dHs_dx_a, F_dHs_dy_a, F_dHi_dx_a, F_dHi_dy_a, F_dHb_dt, F_A_vav,
+  F_Hi_cx, F_dHs_dx_cx, F_dHs_dy_cx, F_Hi_cy, F_dHs_dx_cy, F_dHs_dy_cy, F_Hi_b,
+  F_beta_sq, F_alpha_sq, F_phi_fric, F_tauc,
+  F_u, F_v, F_taub_cx, F_taub_cy,
+  F_N_a, F_N_b, F_N_cx, F_N_cy, F_visc_int, F_F2, F_beta_a, F_beta_eff_a, F_beta_eff_cx, F_beta_eff_cy,
+  F_taudx, F_taudy, F_fg_a, F_fg_cx, F_fg_cy, F_fg_b, F_overb, F_pore, F_Neff, F_front_rhs,
+  F_qNW, F_qNE, F_qSW, F_qSE,
+  F_xu0, F_xv0, F_xu1, F_xv1, F_u_prev, F_v_prev, F_u_base, F_v_base,
+  F_uvav, F_vvav, F_u_vav_a, F_v_vav_a, F_u_surf_a, F_v_surf_a, F_u_base_a, F_v_base_a,
+  F_uabs_vav, F_uabs_surf, F_uabs_base, F_R_shear, F_w_surf, F_w_base,
+  F_u0, F_v0, F_tbx0, F_tby0,
+  F2_COUNT
+};
+enum I2 { I_ice, I_sheet, I_ocean, I_land, I_shelf, I_combi, I_fin, I_fout, I2_COUNT };
+enum F3 { G_A3, G_B3, G_visc_a, G_visc_b, G_visc0, G_F1, G_u3, G_v3, G_u3sia, G_v3sia, G_w3, G_u3a, G_v3a, G_u3a0, G_v3a0,
+          G_Ti, G_Tpmp, G_Acx, G_Acy, F3_COUNT };
+
+enum Kind { KA, KCX, KCY, KB };
+
+struct HostMember {
+  int nx, ny;
+};
+
+inline double *f2p(imau_problem *P, int f) { return P->f2.p + (size_t)f * P->na; }
+inline int32_t *i2p(imau_problem *P, int f) { return P->i2.p + (size_t)f * P->na; }
+inline double *f3p(imau_problem *P, int f) { return P->f3.p + (size_t)f * P->na * P->nz; }
+
+// copy a host Fortran 2D field of kind k into member g's slab (ld ny)
+void up2(imau_ctx *c, imau_problem *P, int g, Kind k, const void *src, void *dst_base, size_t esz) {
+  if (!src) return;
+  const Member &m = P->mem[g];
+  int ni = m.nx, nj = m.ny;
+  if (k == KCX || k == KB) ni = m.nx - 1;
+  if (k == KCY || k == KB) nj = m.ny - 1;
+  char *dst = (char *)dst_base + (size_t)m.off * esz;
+  HIPCHK(hipMemcpy2DAsync(dst, (size_t)m.ny * esz, src, (size_t)nj * esz, (size_t)nj * esz, ni, hipMemcpyHostToDevice, c->s));
+}
+void down2(imau_ctx *c, imau_problem *P, int g, Kind k, void *dst, const void *src_base, size_t esz) {
+  if (!dst) return;
+  const Member &m = P->mem[g];
+  int ni = m.nx, nj = m.ny;
+  if (k == KCX || k == KB) ni = m.nx - 1;
+  if (k == KCY || k == KB) nj = m.ny - 1;
+  const char *src = (const char *)src_base + (size_t)m.off * esz;
+  HIPCHK(hipMemcpy2DAsync(dst, (size_t)nj * esz, src, (size_t)m.ny * esz, (size_t)nj * esz, ni, hipMemcpyDeviceToHost, c->s));
+}
+void up3(imau_ctx *c, imau_problem *P, int g, Kind k, const double *src, double *dst) {
+  if (!src) return;
+  const Member &m = P->mem[g];
+  int ni = m.nx, nj = m.ny;
+  if (k == KCX) ni = m.nx - 1;
+  if (k == KCY) nj = m.ny - 1;
+  size_t n = (size_t)ni * nj * P->nz;
+  HIPCHK(hipMemcpyAsync(P->stage.p, src, n * sizeof(double), hipMemcpyHostToDevice, c->s));
+  imau::h2d_3d(c->s, P->mem_d.p, g, P->stage.p, dst, ni, nj, nj, P->nz);
+  HIPCHK(hipGetLastError());
+}
+void down3(imau_ctx *c, imau_problem *P, int g, Kind k, double *dst, const double *src) {
+  if (!dst) return;
+  const Member &m = P->mem[g];
+  int ni = m.nx, nj = m.ny;
+  if (k == KCX) ni = m.nx - 1;
+  if (k == KCY) nj = m.ny - 1;
+  size_t n = (size_t)ni * nj * P->nz;
+  imau::d2h_3d(c->s, P->mem_d.p, g, src, P->stage.p, ni, nj, nj, P->nz);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(dst, P->stage.p, n * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+}
+
+void check_cfg(const imau_cfg *cfg) {
+  if (!cfg) fail(IMAU_ERR_ARG, "cfg is NULL");
+  if (cfg->nz < 2 || cfg->nz > IMAU_NZ_MAX) fail(IMAU_ERR_CONFIG, "nz=%d outside [2,%d]", cfg->nz, IMAU_NZ_MAX);
+  if (cfg->choice_ice_margin != IMAU_MARGIN_BC && cfg->choice_ice_margin != IMAU_MARGIN_INFINITE_SLAB)
+    fail(IMAU_ERR_CONFIG, "unknown choice_ice_margin %d", cfg->choice_ice_margin);
+  if (cfg->choice_sliding_law < 0 || cfg->choice_sliding_law > 7)
+    fail(IMAU_ERR_CONFIG, "unknown choice_sliding_law %d", cfg->choice_sliding_law);
+  if (cfg->choice_sliding_law == IMAU_SLID_IDEALISED && cfg->choice_idealised_sliding_law != IMAU_IDEAL_ISMIP_HOM_C &&
+      cfg->choice_idealised_sliding_law != IMAU_IDEAL_ISMIP_HOM_D && cfg->choice_idealised_sliding_law != IMAU_IDEAL_ISMIP_HOM_F)
+    fail(IMAU_ERR_CONFIG, "unsupported choice_idealised_sliding_law %d (ISMIP_HOM_E reads a data file)",
+         cfg->choice_idealised_sliding_law);
+  if (cfg->choice_basal_hydrology < 0 || cfg->choice_basal_hydrology > 2)
+    fail(IMAU_ERR_CONFIG, "unknown choice_basal_hydrology %d", cfg->choice_basal_hydrology);
+  const int *bcs = &cfg->DIVA_boundary_BC_u_west;
+  for (int q = 0; q < 8; ++q)
+    if (bcs[q] < 0 || bcs[q] > 2) fail(IMAU_ERR_CONFIG, "unknown DIVA boundary condition %d", bcs[q]);
+  if (cfg->DIVA_choice_matrix_solver != IMAU_SOLVER_SOR && cfg->DIVA_choice_matrix_solver != IMAU_SOLVER_PETSC)
+    fail(IMAU_ERR_CONFIG, "unknown DIVA_choice_matrix_solver %d", cfg->DIVA_choice_matrix_solver);
+  for (int k = 1; k < cfg->nz; ++k)
+    if (!(cfg->zeta[k] > cfg->zeta[k - 1])) fail(IMAU_ERR_CONFIG, "zeta must increase");
+}
+
+KP make_kp(const imau_cfg *c) {
+  KP k{};
+  k.nz = c->nz;
+  for (int q = 0; q < c->nz; ++q) k.zeta[q] = c->zeta[q];
+  k.dyn = c->choice_ice_dynamics;
+  k.margin = c->choice_ice_margin;
+  k.slid = c->choice_sliding_law;
+  k.ideal = c->choice_idealised_sliding_law;
+  k.hydro = c->choice_basal_hydrology;
+  k.rough = c->choice_basal_roughness;
+  k.prough = c->choice_param_basal_roughness;
+  k.bc_u[0] = c->DIVA_boundary_BC_u_west; k.bc_u[1] = c->DIVA_boundary_BC_u_east;
+  k.bc_u[2] = c->DIVA_boundary_BC_u_south; k.bc_u[3] = c->DIVA_boundary_BC_u_north;
+  k.bc_v[0] = c->DIVA_boundary_BC_v_west; k.bc_v[1] = c->DIVA_boundary_BC_v_east;
+  k.bc_v[2] = c->DIVA_boundary_BC_v_south; k.bc_v[3] = c->DIVA_boundary_BC_v_north;
+  k.cross = c->include_SSADIVA_crossterms;
+  k.glsub = c->do_GL_subgrid_friction;
+  k.freeze = c->include_basal_freezing;
+  k.n_flow = c->n_flow;
+  k.eps0 = c->DIVA_epsilon_sq_0;
+  k.visc_min = c->DIVA_visc_eff_min;
+  k.beta_max = c->DIVA_beta_max;
+  k.vel_max = c->DIVA_vel_max;
+  k.relax = c->DIVA_visc_it_relax;
+  k.delta_v = c->slid_delta_v;
+  k.wm = c->slid_Weertman_m;
+  k.q_plastic = c->slid_Coulomb_reg_q_plastic;
+  k.u_threshold = c->slid_Coulomb_reg_u_threshold;
+  k.zi_ut = c->slid_ZI_ut;
+  k.zi_p = c->slid_ZI_p;
+  k.dT_freeze = c->deltaT_basal_freezing;
+  k.hydro_min = c->Martin2011_hydro_Hb_min;
+  k.hydro_max = c->Martin2011_hydro_Hb_max;
+  k.phi_hb_min = c->Martin2011till_phi_Hb_min;
+  k.phi_hb_max = c->Martin2011till_phi_Hb_max;
+  k.phi_min = c->Martin2011till_phi_min;
+  k.phi_max = c->Martin2011till_phi_max;
+  k.ssa_L = c->SSA_icestream_L;
+  k.ssa_m = c->SSA_icestream_m;
+  k.ismip_L = c->ISMIP_HOM_L;
+  // F_int_min, src/ice_velocity_module.f90:980
+  double f[IMAU_NZ_MAX];
+  for (int q = 0; q < c->nz; ++q) f[q] = (1.0 / c->DIVA_visc_eff_min) * (c->zeta[q] * c->zeta[q]);
+  k.F_int_min = vint(c->nz, c->zeta, f);
+  return k;
+}
+
+void wire(imau_problem *P) {
+  Fields &F = P->F;
+  F.Hi = f2p(P, F_Hi); F.Hb = f2p(P, F_Hb); F.SL = f2p(P, F_SL); F.Hs = f2p(P, F_Hs); F.TAF = f2p(P, F_TAF);
+  F.dHs_dx_a = f2p(P, F_dHs_dx_a); F.dHs_dy_a = f2p(P, F_dHs_dy_a);
+  F.dHi_dx_a = f2p(P, F_dHi_dx_a); F.dHi_dy_a = f2p(P, F_dHi_dy_a);
+  F.dHb_dt = f2p(P, F_dHb_dt); F.A_vav = f2p(P, F_A_vav);
+  F.m_ice = i2p(P, I_ice); F.m_sheet = i2p(P, I_sheet); F.m_ocean = i2p(P, I_ocean);
+  F.Hi_cx = f2p(P, F_Hi_cx); F.dHs_dx_cx = f2p(P, F_dHs_dx_cx); F.dHs_dy_cx = f2p(P, F_dHs_dy_cx);
+  F.Hi_cy = f2p(P, F_Hi_cy); F.dHs_dx_cy = f2p(P, F_dHs_dx_cy); F.dHs_dy_cy = f2p(P, F_dHs_dy_cy);
+  F.Hi_b = f2p(P, F_Hi_b);
+  F.Ti3 = f3p(P, G_Ti); F.Tpmp3 = f3p(P, G_Tpmp);
+  F.A3 = f3p(P, G_A3);
+  F.B3 = f3p(P, G_B3);
+  F.beta_sq = f2p(P, F_beta_sq); F.alpha_sq = f2p(P, F_alpha_sq); F.phi_fric = f2p(P, F_phi_fric); F.tauc = f2p(P, F_tauc);
+  F.u = f2p(P, F_u); F.v = f2p(P, F_v); F.taub_cx = f2p(P, F_taub_cx); F.taub_cy = f2p(P, F_taub_cy);
+  F.visc_a = f3p(P, G_visc_a); F.visc_b = f3p(P, G_visc_b);
+  F.N_a = f2p(P, F_N_a); F.N_b = f2p(P, F_N_b); F.N_cx = f2p(P, F_N_cx); F.N_cy = f2p(P, F_N_cy);
+  F.visc_int = f2p(P, F_visc_int); F.F2 = f2p(P, F_F2); F.beta_a = f2p(P, F_beta_a); F.beta_eff_a = f2p(P, F_beta_eff_a);
+  F.beta_eff_cx = f2p(P, F_beta_eff_cx); F.beta_eff_cy = f2p(P, F_beta_eff_cy);
+  F.taudx = f2p(P, F_taudx); F.taudy = f2p(P, F_taudy);
+  F.fg_a = f2p(P, F_fg_a); F.fg_cx = f2p(P, F_fg_cx); F.fg_cy = f2p(P, F_fg_cy); F.fg_b = f2p(P, F_fg_b);
+  F.overb = f2p(P, F_overb); F.pore = f2p(P, F_pore); F.Neff = f2p(P, F_Neff); F.front_rhs = f2p(P, F_front_rhs);
+  F.qNW = f2p(P, F_qNW); F.qNE = f2p(P, F_qNE); F.qSW = f2p(P, F_qSW); F.qSE = f2p(P, F_qSE);
+  F.combi = i2p(P, I_combi); F.fin = i2p(P, I_fin); F.fout = i2p(P, I_fout);
+  F.rt_u = P->rt.p; F.rt_v = P->rt.p + P->na;
+  F.xu0 = f2p(P, F_xu0); F.xv0 = f2p(P, F_xv0); F.xu1 = f2p(P, F_xu1); F.xv1 = f2p(P, F_xv1);
+  F.u_prev = f2p(P, F_u_prev); F.v_prev = f2p(P, F_v_prev); F.u_base = f2p(P, F_u_base); F.v_base = f2p(P, F_v_base);
+  OutF &O = P->O;
+  O.F1 = f3p(P, G_F1); O.u3 = f3p(P, G_u3); O.v3 = f3p(P, G_v3); O.u3sia = f3p(P, G_u3sia); O.v3sia = f3p(P, G_v3sia);
+  O.w3 = f3p(P, G_w3); O.u3a = f3p(P, G_u3a); O.v3a = f3p(P, G_v3a);
+  O.uvav = f2p(P, F_uvav); O.vvav = f2p(P, F_vvav);
+  O.u_vav_a = f2p(P, F_u_vav_a); O.v_vav_a = f2p(P, F_v_vav_a); O.u_surf_a = f2p(P, F_u_surf_a);
+  O.v_surf_a = f2p(P, F_v_surf_a); O.u_base_a = f2p(P, F_u_base_a); O.v_base_a = f2p(P, F_v_base_a);
+  O.uabs_vav = f2p(P, F_uabs_vav); O.uabs_surf = f2p(P, F_uabs_surf); O.uabs_base = f2p(P, F_uabs_base);
+  O.R_shear = f2p(P, F_R_shear); O.w_surf = f2p(P, F_w_surf); O.w_base = f2p(P, F_w_base);
+  P->Acx3 = f3p(P, G_Acx); P->Acy3 = f3p(P, G_Acy);
+  P->u0 = f2p(P, F_u0); P->v0 = f2p(P, F_v0); P->tbx0 = f2p(P, F_tbx0); P->tby0 = f2p(P, F_tby0);
+  P->visc0 = f3p(P, G_visc0); P->u3a0 = f3p(P, G_u3a0); P->v3a0 = f3p(P, G_v3a0);
+}
+
+bool uses_ssa_field(int dyn) { return dyn == IMAU_DYN_SSA || dyn == IMAU_DYN_SIA_SSA; }
+
+void upload(imau_ctx *c, imau_problem *P, const imau_cfg *cfg, const imau_ice_io *ios) {
+  const int nz = P->nz;
+  for (int g = 0; g < P->ngrids; ++g) {
+    const imau_ice_io &io = ios[g];
+    if (!io.Hi_a || !io.Hb_a || !io.SL_a || !io.Hs_a || !io.TAF_a || !io.mask_ice_a || !io.mask_sheet_a ||
+        !io.mask_ocean_a || !io.Hi_cx || !io.Hi_cy || !io.Hi_b || !io.dHs_dx_cx || !io.dHs_dy_cy || !io.dHs_dx_cy ||
+        !io.dHs_dy_cx || !io.dHs_dx_a || !io.dHs_dy_a || !io.dHi_dx_a || !io.dHi_dy_a)
+      fail(IMAU_ERR_ARG, "member %d: a required geometry input is NULL", g);
+    if ((cfg->choice_ice_dynamics == IMAU_DYN_DIVA || uses_ssa_field(cfg->choice_ice_dynamics)) && !io.A_flow_3D_a)
+      fail(IMAU_ERR_ARG, "member %d: A_flow_3D_a is NULL", g);
+    if (cfg->include_basal_freezing && (!io.Ti_a || !io.Ti_pmp_a))
+      fail(IMAU_ERR_ARG, "member %d: include_basal_freezing needs Ti_a and Ti_pmp_a", g);
+    const size_t d = sizeof(double), q = sizeof(int32_t);
+    up2(c, P, g, KA, io.Hi_a, f2p(P, F_Hi), d);
+    up2(c, P, g, KA, io.Hb_a, f2p(P, F_Hb), d);
+    up2(c, P, g, KA, io.SL_a, f2p(P, F_SL), d);
+    up2(c, P, g, KA, io.Hs_a, f2p(P, F_Hs), d);
+    up2(c, P, g, KA, io.TAF_a, f2p(P, F_TAF), d);
+    up2(c, P, g, KA, io.dHs_dx_a, f2p(P, F_dHs_dx_a), d);
+    up2(c, P, g, KA, io.dHs_dy_a, f2p(P, F_dHs_dy_a), d);
+    up2(c, P, g, KA, io.dHi_dx_a, f2p(P, F_dHi_dx_a), d);
+    up2(c, P, g, KA, io.dHi_dy_a, f2p(P, F_dHi_dy_a), d);
+    up2(c, P, g, KA, io.dHb_dt_a, f2p(P, F_dHb_dt), d);
+    up2(c, P, g, KA, io.A_flow_vav_a, f2p(P, F_A_vav), d);
+    up2(c, P, g, KA, io.mask_ice_a, i2p(P, I_ice), q);
+    up2(c, P, g, KA, io.mask_sheet_a, i2p(P, I_sheet), q);
+    up2(c, P, g, KA, io.mask_ocean_a, i2p(P, I_ocean), q);
+    up2(c, P, g, KCX, io.Hi_cx, f2p(P, F_Hi_cx), d);
+    up2(c, P, g, KCX, io.dHs_dx_cx, f2p(P, F_dHs_dx_cx), d);
+    up2(c, P, g, KCX, io.dHs_dy_cx, f2p(P, F_dHs_dy_cx), d);
+    up2(c, P, g, KCY, io.Hi_cy, f2p(P, F_Hi_cy), d);
+    up2(c, P, g, KCY, io.dHs_dx_cy, f2p(P, F_dHs_dx_cy), d);
+    up2(c, P, g, KCY, io.dHs_dy_cy, f2p(P, F_dHs_dy_cy), d);
+    up2(c, P, g, KB, io.Hi_b, f2p(P, F_Hi_b), d);
+    up2(c, P, g, KA, io.beta_sq_a, f2p(P, F_beta_sq), d);
+    up2(c, P, g, KA, io.alpha_sq_a, f2p(P, F_alpha_sq), d);
+    up2(c, P, g, KA, io.phi_fric_a, f2p(P, F_phi_fric), d);
+    up2(c, P, g, KA, io.tauc_a, f2p(P, F_tauc), d);
+    const bool ssa = uses_ssa_field(cfg->choice_ice_dynamics);
+    up2(c, P, g, KCX, ssa ? io.u_SSA_cx : io.u_vav_cx, P->u0, d);
+    up2(c, P, g, KCY, ssa ? io.v_SSA_cy : io.v_vav_cy, P->v0, d);
+    up2(c, P, g, KCX, io.taub_cx, P->tbx0, d);
+    up2(c, P, g, KCY, io.taub_cy, P->tby0, d);
+    up2(c, P, g, KA, io.N_a, f2p(P, F_N_a), d);              // BC margin keeps stale N_a of ice-free cells
+    up2(c, P, g, KA, io.visc_eff_int_a, f2p(P, F_visc_int), d);
+    up3(c, P, g, KA, io.A_flow_3D_a, f3p(P, G_A3));
+    up3(c, P, g, KA, io.visc_eff_3D_a, P->visc0);
+    up3(c, P, g, KA, io.u_3D_a, P->u3a0);                  // previous a-grid velocities (read by w_3D)
+    up3(c, P, g, KA, io.v_3D_a, P->v3a0);
+    up3(c, P, g, KCX, io.A_flow_3D_cx, P->Acx3);
+    up3(c, P, g, KCY, io.A_flow_3D_cy, P->Acy3);
+    if (cfg->choice_ice_dynamics == IMAU_DYN_SIA_SSA) {  // SIA columns feed the hybrid SSA (:444-451)
+      up3(c, P, g, KCX, io.u_3D_SIA_cx, f3p(P, G_u3sia));
+      up3(c, P, g, KCY, io.v_3D_SIA_cy, f3p(P, G_v3sia));
+    }
+    if (g == 0) {
+      P->sor_nit_ = io.DIVA_SOR_nit > 0 ? io.DIVA_SOR_nit : 10000;
+      P->sor_tol_ = io.DIVA_SOR_tol;
+      P->sor_omega_ = io.DIVA_SOR_omega > 0.0 ? io.DIVA_SOR_omega : 1.3;
+      P->petsc_rtol_ = io.DIVA_PETSc_rtol;
+      P->petsc_abstol_ = io.DIVA_PETSc_abstol;
+    }
+    if (cfg->include_basal_freezing) {
+      up3(c, P, g, KA, io.Ti_a, f3p(P, G_Ti));
+      up3(c, P, g, KA, io.Ti_pmp_a, f3p(P, G_Tpmp));
+    }
+    // trivial members (no grounded ice): src/ice_velocity_module.f90:276-285
+    long long s = 0;
+    const Member &m = P->mem[g];
+    for (long long p = 0; p < (long long)m.nx * m.ny; ++p) s += io.mask_sheet_a[p];
+    P->sheet_any[g] = s > 0;
+  }
+  (void)nz;
+  HIPCHK(hipStreamSynchronize(c->s));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+#define ABI_TRY try {
+#define ABI_CATCH(ctx)                                        \
+  }                                                           \
+  catch (const Err &e) {                                      \
+    if (ctx) (ctx)->err = e.msg; else g_init_err = e.msg;     \
+    return e.code;                                            \
+  }                                                           \
+  catch (const std::exception &e) {                           \
+    if (ctx) (ctx)->err = e.what(); else g_init_err = e.what(); \
+    return IMAU_ERR_DEVICE;                                   \
+  }
+
+extern "C" {
+
+void imau_vel_default_config(imau_cfg *c) {
+  static const double zeta[15] = {0.00, 0.10, 0.20, 0.30, 0.40, 0.50, 0.60, 0.70, 0.80, 0.90, 0.925, 0.95, 0.975, 0.99, 1.00};
+  std::memset(c, 0, sizeof *c);
+  c->choice_ice_dynamics = IMAU_DYN_DIVA;
+  c->choice_ice_margin = IMAU_MARGIN_INFINITE_SLAB;
+  c->choice_sliding_law = IMAU_SLID_COULOMB_REGULARISED;
+  c->choice_basal_hydrology = IMAU_HYDRO_MARTIN2011;
+  c->choice_basal_roughness = IMAU_ROUGH_PARAMETERISED;
+  c->choice_param_basal_roughness = IMAU_PROUGH_MARTIN2011;
+  c->DIVA_choice_matrix_solver = IMAU_SOLVER_PETSC;
+  c->include_SSADIVA_crossterms = 1;
+  c->do_GL_subgrid_friction = 1;
+  c->include_basal_freezing = 1;
+  c->nz = 15;
+  for (int k = 0; k < 15; ++k) c->zeta[k] = zeta[k];
+  c->n_flow = 3.0;
+  c->DIVA_visc_it_norm_dUV_tol = 1e-2;
+  c->DIVA_visc_it_nit = 50;
+  c->DIVA_visc_it_relax = 0.4;
+  c->DIVA_epsilon_sq_0 = 1e-15;
+  c->DIVA_visc_eff_min = 1e3;
+  c->DIVA_beta_max = 1e20;
+  c->DIVA_vel_max = 5000.0;
+  c->slid_delta_v = 1.0e-3;
+  c->slid_Weertman_m = 3.0;
+  c->slid_Coulomb_reg_q_plastic = 0.3;
+  c->slid_Coulomb_reg_u_threshold = 100.0;
+  c->slid_ZI_ut = 200.0;
+  c->slid_ZI_p = 5.0;
+  c->deltaT_basal_freezing = 2.0;
+  c->Martin2011_hydro_Hb_min = 0.0;
+  c->Martin2011_hydro_Hb_max = 1000.0;
+  c->Martin2011till_phi_Hb_min = -1000.0;
+  c->Martin2011till_phi_Hb_max = 0.0;
+  c->Martin2011till_phi_min = 5.0;
+  c->Martin2011till_phi_max = 20.0;
+  c->SSA_icestream_L = 100000.0;
+  c->SSA_icestream_m = 0.5;
+  c->ISMIP_HOM_L = 160000.0;
+  c->precision = IMAU_PREC_FP64;
+  c->stop_rule = IMAU_STOP_PICARD;
+  c->nonlinear_rtol = 1e-4;
+}
+
+int imau_vel_init(const imau_grid *grid, const imau_cfg *cfg, imau_ctx **out) {
+  imau_ctx *ctx = nullptr;
+  ABI_TRY
+  if (!out) fail(IMAU_ERR_ARG, "ctx output pointer is NULL");
+  *out = nullptr;
+  if (grid && (grid->nx < 4 || grid->ny < 4)) fail(IMAU_ERR_ARG, "grid must be at least 4x4 (got %dx%d)", grid->nx, grid->ny);
+  if (cfg) check_cfg(cfg);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) fail(IMAU_ERR_NO_DEVICE, "no HIP device visible");
+  ctx = new imau_ctx();
+  HIPCHK(hipGetDevice(&ctx->device));
+  HIPCHK(hipStreamCreateWithFlags(&ctx->s, hipStreamNonBlocking));
+  *out = ctx;
+  return IMAU_OK;
+  }
+  catch (const Err &e) {
+    g_init_err = e.msg;
+    delete ctx;
+    return e.code;
+  }
+}
+
+void imau_vel_finalize(imau_ctx *ctx) {
+  if (!ctx) return;
+  imau_vel_problem_destroy(ctx->cached);
+  if (ctx->s) (void)hipStreamDestroy(ctx->s);
+  delete ctx;
+}
+
+const char *imau_vel_last_error(const imau_ctx *ctx) { return ctx ? ctx->err.c_str() : g_init_err.c_str(); }
+
+int imau_vel_problem_create(imau_ctx *ctx, int32_t ngrids, const imau_grid *grids, const imau_cfg *cfg,
+                            const imau_ice_io *ios, imau_problem **prob) {
+  imau_problem *P = nullptr;
+  ABI_TRY
+  if (!ctx || !grids || !ios || !prob) fail(IMAU_ERR_ARG, "NULL argument");
+  if (ngrids < 1) fail(IMAU_ERR_ARG, "ngrids=%d", ngrids);
+  check_cfg(cfg);
+  P = new imau_problem();
+  P->ngrids = ngrids;
+  P->nz = cfg->nz;
+  long long off = 0, offx = 0, offy = 0;
+  std::vector<double> xs, ys;
+  size_t max3 = 0;
+  for (int g = 0; g < ngrids; ++g) {
+    const imau_grid &gr = grids[g];
+    if (gr.nx < 4 || gr.ny < 4) fail(IMAU_ERR_ARG, "member %d: grid must be at least 4x4 (got %dx%d)", g, gr.nx, gr.ny);
+    if (!(gr.dx > 0.0)) fail(IMAU_ERR_ARG, "member %d: dx must be positive", g);
+    if (!gr.x || !gr.y) fail(IMAU_ERR_ARG, "member %d: grid coordinates are NULL", g);
+    Member m{};
+    m.nx = gr.nx; m.ny = gr.ny; m.dx = gr.dx;
+    m.off = off; m.off3 = off * cfg->nz; m.offx = offx; m.offy = offy;
+    P->mem.push_back(m);
+    P->tstart.push_back((int)P->tiles.size());
+    for (int i0 = 0; i0 < gr.nx; i0 += IMAU_TI)
+      for (int j0 = 0; j0 < gr.ny; j0 += IMAU_TJ) P->tiles.push_back(Tile{g, i0, j0, 0});
+    // pad each slab to a multiple of 64 doubles (512 B) for aligned member starts
+    long long na = (long long)gr.nx * gr.ny;
+    off += (na + 63) / 64 * 64;
+    xs.insert(xs.end(), gr.x, gr.x + gr.nx);
+    ys.insert(ys.end(), gr.y, gr.y + gr.ny);
+    offx += gr.nx; offy += gr.ny;
+    max3 = std::max(max3, (size_t)na * cfg->nz);
+  }
+  P->tstart.push_back((int)P->tiles.size());
+  P->na = off;
+  P->sheet_any.assign(ngrids, 0);
+  P->picard_its.assign(ngrids, 0);
+  P->sweeps.assign(ngrids, 0);
+  P->resid_uv.assign(ngrids, 0.0);
+  P->nl_resid.assign(ngrids, NAN);
+  P->mem_d.alloc(ngrids);
+  P->tiles_d.alloc(P->tiles.size());
+  P->tstart_d.alloc(P->tstart.size());
+  P->xg.alloc(xs.size());
+  P->yg.alloc(ys.size());
+  P->f2.alloc((size_t)F2_COUNT * P->na);
+  P->i2.alloc((size_t)I2_COUNT * P->na);
+  P->rt.alloc((size_t)2 * P->na);
+  P->f3.alloc((size_t)F3_COUNT * P->na * cfg->nz);
+  P->stage.alloc(max3);
+  P->partial.alloc((size_t)4 * P->tiles.size());
+  P->msum.alloc((size_t)4 * ngrids);
+  P->resmax.alloc(ngrids);
+  P->act_pic.alloc(ngrids);
+  P->act_lin.alloc(ngrids);
+  HIPCHK(hipMemcpy(P->mem_d.p, P->mem.data(), ngrids * sizeof(Member), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(P->tiles_d.p, P->tiles.data(), P->tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(P->tstart_d.p, P->tstart.data(), P->tstart.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(P->xg.p, xs.data(), xs.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(P->yg.p, ys.data(), ys.size() * sizeof(double), hipMemcpyHostToDevice));
+  wire(P);
+  upload(ctx, P, cfg, ios);
+  P->dyn_uploaded = cfg->choice_ice_dynamics;
+  P->has_ti = cfg->include_basal_freezing != 0;
+  *prob = P;
+  return imau_vel_problem_reset_state(ctx, P);
+  }
+  catch (const Err &e) {
+    if (ctx) ctx->err = e.msg;
+    delete P;
+    return e.code;
+  }
+}
+
+void imau_vel_problem_destroy(imau_problem *P) { delete P; }
+
+int imau_vel_problem_reset_state(imau_ctx *ctx, imau_problem *P) {
+  ABI_TRY
+  if (!ctx || !P) fail(IMAU_ERR_ARG, "NULL argument");
+  const size_t b2 = (size_t)P->na * sizeof(double), b3 = b2 * P->nz;
+  HIPCHK(hipMemcpyAsync(P->F.u, P->u0, b2, hipMemcpyDeviceToDevice, ctx->s));
+  HIPCHK(hipMemcpyAsync(P->F.v, P->v0, b2, hipMemcpyDeviceToDevice, ctx->s));
+  HIPCHK(hipMemcpyAsync(P->F.taub_cx, P->tbx0, b2, hipMemcpyDeviceToDevice, ctx->s));
+  HIPCHK(hipMemcpyAsync(P->F.taub_cy, P->tby0, b2, hipMemcpyDeviceToDevice, ctx->s));
+  HIPCHK(hipMemcpyAsync(P->F.visc_a, P->visc0, b3, hipMemcpyDeviceToDevice, ctx->s));
+  HIPCHK(hipMemcpyAsync(P->O.u3a, P->u3a0, b3, hipMemcpyDeviceToDevice, ctx->s));
+  HIPCHK(hipMemcpyAsync(P->O.v3a, P->v3a0, b3, hipMemcpyDeviceToDevice, ctx->s));
+  P->visc_cur = P->F.visc_a;
+  P->visc_oth = P->F.visc_b;
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// the solves
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Solver {
+  imau_ctx *c;
+  imau_problem *P;
+  const imau_cfg *cfg;
+  KP kp;
+  imau::Launch L;
+  std::vector<int> act_pic, act_lin;
+  std::vector<double> host4;
+
+  Solver(imau_ctx *c_, imau_problem *P_, const imau_cfg *cfg_) : c(c_), P(P_), cfg(cfg_) {
+    kp = make_kp(cfg);
+    L = imau::Launch{c->s, (int)P->tiles.size(), P->mem_d.p, P->tiles_d.p};
+    host4.resize(4 * (size_t)P->ngrids);
+  }
+  imau::Coef coef() const {
+    imau::Coef k;
+    k.Na = P->F.N_a; k.Nb = P->F.N_b; k.Ncx = P->F.N_cx; k.Ncy = P->F.N_cy;
+    k.bcx = P->F.beta_eff_cx; k.bcy = P->F.beta_eff_cy; k.taudx = P->F.taudx; k.taudy = P->F.taudy;
+    k.frhs = P->F.front_rhs; k.rt_u = P->F.rt_u; k.rt_v = P->F.rt_v;
+    return k;
+  }
+  void push_active(int *dev, const std::vector<int> &h) {
+    HIPCHK(hipMemcpyAsync(dev, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, c->s));
+  }
+  void kcheck() { HIPCHK(hipGetLastError()); }
+
+  // per-member relative FP64 residual at the current coefficients and state
+  void residual(std::vector<double> &r) {
+    imau::resid(L, kp, coef(), P->F.u, P->F.v, P->partial.p);
+    imau::member_sum2(c->s, P->ngrids, P->tstart_d.p, P->partial.p, P->msum.p);
+    kcheck();
+    HIPCHK(hipMemcpyAsync(host4.data(), P->msum.p, 2 * P->ngrids * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    r.resize(P->ngrids);
+    for (int g = 0; g < P->ngrids; ++g) {
+      const double rr = host4[2 * g], bb = host4[2 * g + 1];
+      r[g] = bb > 0.0 ? std::sqrt(rr) / std::sqrt(bb) : std::sqrt(rr);
+    }
+  }
+
+  // linear solve of every Picard-active member (src/utilities_module.f90:600-718)
+  void linear_solve() {
+    const bool petsc = cfg->DIVA_choice_matrix_solver == IMAU_SOLVER_PETSC;
+    // the reference's per-call SOR settings apply to the SOR mode; the KSP mode
+    // uses its tolerances with the same smoother as the iteration
+    const int nit = sor_nit;
+    const double tol = sor_tol, omega = sor_omega, rtol = petsc_rtol, atol = petsc_abstol;
+    const int K = cfg->check_interval > 0 ? cfg->check_interval : 8;
+    act_lin = act_pic;
+    push_active(P->act_lin.p, act_lin);
+    imau::init_x(L, P->F, P->F.xu0, P->F.xv0);
+    double *xin_u = P->F.xu0, *xin_v = P->F.xv0, *xout_u = P->F.xu1, *xout_v = P->F.xv1;
+    int it = 0;
+    double om = omega;
+    std::vector<double> rnorm;
+    for (;;) {
+      if (it >= nit) break;
+      const bool checking = ((it + 1) % K == 0) || (it + 1 == nit);
+      if (checking && !petsc) imau::fill_u64(c->s, P->resmax.p, P->ngrids, 0ull);
+      imau::sweep(L, kp, coef(), xin_u, xin_v, xout_u, xout_v, P->act_lin.p, om, checking && !petsc, P->resmax.p);
+      kcheck();
+      std::swap(xin_u, xout_u);
+      std::swap(xin_v, xout_v);
+      ++it;
+      for (int g = 0; g < P->ngrids; ++g)
+        if (act_lin[g]) P->sweeps[g] += 1;
+      if (!checking) continue;
+      bool any = false;
+      if (!petsc) {
+        std::vector<unsigned long long> rm(P->ngrids);
+        HIPCHK(hipMemcpyAsync(rm.data(), P->resmax.p, P->ngrids * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+        bool diverged = false;
+        for (int g = 0; g < P->ngrids; ++g) {
+          if (!act_lin[g]) continue;
+          double r;
+          std::memcpy(&r, &rm[g], sizeof r);
+          if (!std::isfinite(r) || (it > 100 && r > 1e3)) diverged = true;
+          if (r <= tol) act_lin[g] = 0;
+          else any = true;
+        }
+        if (diverged) {
+          // src/utilities_module.f90:697-712: lower omega, restart from zero
+          om -= 0.1;
+          if (om <= 0.1) fail(IMAU_ERR_DIVERGED, "linear solver diverged even with omega <= 0.1");
+          it = 0;
+          act_lin = act_pic;
+          HIPCHK(hipMemsetAsync(xin_u, 0, (size_t)P->na * sizeof(double), c->s));
+          HIPCHK(hipMemsetAsync(xin_v, 0, (size_t)P->na * sizeof(double), c->s));
+          any = true;
+        }
+      } else {
+        // KSP-style stop: ||A x - b|| <= max(rtol ||b||, abstol) on the true FP64 residual
+        imau::resid(L, kp, coef(), xin_u, xin_v, P->partial.p);
+        imau::member_sum2(c->s, P->ngrids, P->tstart_d.p, P->partial.p, P->msum.p);
+        kcheck();
+        HIPCHK(hipMemcpyAsync(host4.data(), P->msum.p, 2 * P->ngrids * sizeof(double), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+        for (int g = 0; g < P->ngrids; ++g) {
+          if (!act_lin[g]) continue;
+          const double rr = std::sqrt(host4[2 * g]), bb = std::sqrt(host4[2 * g + 1]);
+          if (!std::isfinite(rr)) fail(IMAU_ERR_NAN, "member %d: non-finite linear residual", g);
+          if (rr <= std::max(rtol * bb, atol)) act_lin[g] = 0;
+          else any = true;
+        }
+      }
+      if (!any) break;
+      push_active(P->act_lin.p, act_lin);
+    }
+    // the final iterate is in xin_*: make it the canonical x0 buffers
+    if (xin_u != P->F.xu0) {
+      HIPCHK(hipMemcpyAsync(P->F.xu0, xin_u, (size_t)P->na * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+      HIPCHK(hipMemcpyAsync(P->F.xv0, xin_v, (size_t)P->na * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    }
+  }
+
+  int sor_nit = 10000;
+  double sor_tol = 2.5, sor_omega = 1.3, petsc_rtol = 0.01, petsc_abstol = 2.5;
+
+  void picard(bool diva) {
+    const int G = P->ngrids;
+    act_pic.assign(G, 0);
+    for (int g = 0; g < G; ++g) act_pic[g] = P->sheet_any[g];
+    if (!diva && cfg->choice_sliding_law == IMAU_SLID_NO_SLIDING) std::fill(act_pic.begin(), act_pic.end(), 0);
+    std::fill(P->picard_its.begin(), P->picard_its.end(), 0);
+    std::fill(P->sweeps.begin(), P->sweeps.end(), 0);
+    std::fill(P->nl_resid.begin(), P->nl_resid.end(), NAN);
+    imau::setup1(L, kp, P->F, P->yg.p);
+    imau::setup2(L, kp, P->F);
+    kcheck();
+    std::vector<double> r;
+    for (int it = 1;; ++it) {
+      bool any = false;
+      for (int g = 0; g < G; ++g) any = any || act_pic[g];
+      if (!any) break;
+      push_active(P->act_pic.p, act_pic);
+      imau::visc(L, kp, P->F, diva, P->visc_cur, P->visc_oth, P->xg.p, P->yg.p, P->act_pic.p);
+      std::swap(P->visc_cur, P->visc_oth);
+      imau::coef(L, kp, P->F, diva, P->act_pic.p);
+      kcheck();
+      if (cfg->stop_rule == IMAU_STOP_RESIDUAL) {
+        residual(r);
+        bool left = false;
+        for (int g = 0; g < G; ++g) {
+          if (!act_pic[g]) continue;
+          P->nl_resid[g] = r[g];
+          if (r[g] <= cfg->nonlinear_rtol || it > cfg->DIVA_visc_it_nit) act_pic[g] = 0;
+          else left = true;
+        }
+        if (!left) break;
+        push_active(P->act_pic.p, act_pic);
+      }
+      linear_solve();
+      imau::post(L, kp, P->F, P->F.xu0, P->F.xv0, P->act_pic.p, diva, P->partial.p);
+      imau::member_sum3(c->s, G, P->tstart_d.p, P->partial.p, P->msum.p);
+      kcheck();
+      HIPCHK(hipMemcpyAsync(host4.data(), P->msum.p, 3 * G * sizeof(double), hipMemcpyDeviceToHost, c->s));
+      HIPCHK(hipStreamSynchronize(c->s));
+      for (int g = 0; g < G; ++g) {
+        if (!act_pic[g]) continue;
+        P->picard_its[g] = it;
+        const double r1 = host4[3 * g], r2 = host4[3 * g + 1], n = host4[3 * g + 2];
+        double res = 0.0;
+        if (n > 0.0) res = 2.0 * std::sqrt(r1) / std::max(std::sqrt(r2), 1e-8);
+        if (!std::isfinite(r1) || !std::isfinite(r2)) fail(IMAU_ERR_NAN, "member %d: non-finite velocity in Picard iteration %d", g, it);
+        P->resid_uv[g] = res;
+        if (cfg->stop_rule == IMAU_STOP_PICARD && (res < cfg->DIVA_visc_it_norm_dUV_tol || it >= cfg->DIVA_visc_it_nit))
+          act_pic[g] = 0;
+      }
+    }
+  }
+
+  void finish(int dyn) {
+    if (dyn == IMAU_DYN_DIVA) {
+      imau::F1(L, kp, P->F, P->visc_cur, P->O);
+      imau::uv3d(L, kp, P->F, P->O);
+      imau::w3d(L, kp, P->F, P->O);
+      imau::secondary(L, kp, P->F, P->O, dyn, P->F.u, P->F.v);
+    } else {
+      imau::compose(L, kp, P->F, P->O, dyn == IMAU_DYN_SIA ? 1 : (dyn == IMAU_DYN_SSA ? 2 : 3));
+      if (dyn != IMAU_DYN_SSA) imau::w3d(L, kp, P->F, P->O);
+      imau::secondary(L, kp, P->F, P->O, dyn, P->O.uvav, P->O.vvav);
+    }
+    kcheck();
+  }
+};
+
+}  // namespace
+
+namespace {
+enum Kind3 { SOLVE_SIA = 1, SOLVE_SSA = 2, SOLVE_DIVA = 4 };
+
+int solve_kind(imau_ctx *ctx, imau_problem *P, const imau_cfg *cfg, int kinds) {
+  ABI_TRY
+  if (!ctx || !P) fail(IMAU_ERR_ARG, "NULL argument");
+  check_cfg(cfg);
+  if (cfg->nz != P->nz) fail(IMAU_ERR_CONFIG, "cfg nz=%d differs from the problem's nz=%d", cfg->nz, P->nz);
+  if (cfg->include_basal_freezing && !P->has_ti) fail(IMAU_ERR_CONFIG, "include_basal_freezing needs Ti_a/Ti_pmp_a at create");
+  const int dyn = cfg->choice_ice_dynamics;
+  auto t0 = std::chrono::steady_clock::now();
+  Solver S(ctx, P, cfg);
+  S.sor_nit = P->sor_nit_;
+  S.sor_tol = P->sor_tol_;
+  S.sor_omega = P->sor_omega_;
+  S.petsc_rtol = P->petsc_rtol_;
+  S.petsc_abstol = P->petsc_abstol_;
+  if (kinds & SOLVE_SIA) {
+    if (dyn != IMAU_DYN_SIA && dyn != IMAU_DYN_SIA_SSA)
+      fail(IMAU_ERR_CONFIG, "solve_SIA: choice_ice_dynamics is not SIA or SIA/SSA");
+    imau::sia(S.L, S.kp, P->F, P->Acx3, P->Acy3, P->O);
+    S.kcheck();
+    S.finish(dyn);
+  }
+  if (kinds & SOLVE_SSA) {
+    if (dyn != IMAU_DYN_SSA && dyn != IMAU_DYN_SIA_SSA)
+      fail(IMAU_ERR_CONFIG, "solve_SSA: choice_ice_dynamics is not SSA or SIA/SSA");
+    S.picard(false);
+    S.finish(dyn);
+  }
+  if (kinds & SOLVE_DIVA) {
+    if (dyn != IMAU_DYN_DIVA) fail(IMAU_ERR_CONFIG, "solve_DIVA: choice_ice_dynamics is not DIVA");
+    S.picard(true);
+    S.finish(dyn);
+  }
+  HIPCHK(hipStreamSynchronize(ctx->s));
+  P->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+// download a k-plane (k = 0 surface / nz-1 base) of a cx/cy 3D field as a 2D field
+void down_plane(imau_ctx *c, imau_problem *P, int g, Kind k, double *dst, const double *src3, int kk) {
+  if (!dst) return;
+  const Member &m = P->mem[g];
+  int ni = m.nx, nj = m.ny;
+  if (k == KCX) ni = m.nx - 1;
+  if (k == KCY) nj = m.ny - 1;
+  const double *src = src3 + m.off3 + (long long)kk * m.nx * m.ny;
+  HIPCHK(hipMemcpy2DAsync(dst, (size_t)nj * sizeof(double), src, (size_t)m.ny * sizeof(double), (size_t)nj * sizeof(double),
+                          ni, hipMemcpyDeviceToHost, c->s));
+}
+}  // namespace
+
+extern "C" {
+
+int imau_vel_problem_solve(imau_ctx *ctx, imau_problem *P, const imau_cfg *cfg) {
+  if (!cfg) return IMAU_ERR_ARG;
+  const int dyn = cfg->choice_ice_dynamics;
+  int kinds = dyn == IMAU_DYN_DIVA ? SOLVE_DIVA
+            : dyn == IMAU_DYN_SSA ? SOLVE_SSA
+            : dyn == IMAU_DYN_SIA ? SOLVE_SIA
+            : dyn == IMAU_DYN_SIA_SSA ? (SOLVE_SIA | SOLVE_SSA) : 0;
+  if (!kinds) {
+    if (ctx) ctx->err = "unknown choice_ice_dynamics";
+    return IMAU_ERR_CONFIG;
+  }
+  return solve_kind(ctx, P, cfg, kinds);
+}
+
+int imau_vel_problem_download(imau_ctx *ctx, imau_problem *P, imau_ice_io *ios) {
+  ABI_TRY
+  if (!ctx || !P || !ios) fail(IMAU_ERR_ARG, "NULL argument");
+  const size_t d = sizeof(double), q = sizeof(int32_t);
+  const int dyn = P->dyn_uploaded;
+  const bool ssa = uses_ssa_field(dyn);
+  for (int g = 0; g < P->ngrids; ++g) {
+    imau_ice_io &io = ios[g];
+    if (dyn == IMAU_DYN_DIVA) {
+      down2(ctx, P, g, KCX, io.u_vav_cx, P->F.u, d);
+      down2(ctx, P, g, KCY, io.v_vav_cy, P->F.v, d);
+    } else {
+      down2(ctx, P, g, KCX, io.u_vav_cx, P->O.uvav, d);
+      down2(ctx, P, g, KCY, io.v_vav_cy, P->O.vvav, d);
+    }
+    if (ssa) {
+      down2(ctx, P, g, KCX, io.u_SSA_cx, P->F.u, d);
+      down2(ctx, P, g, KCY, io.v_SSA_cy, P->F.v, d);
+    }
+    if (dyn == IMAU_DYN_DIVA) {
+      down2(ctx, P, g, KCX, io.taub_cx, P->F.taub_cx, d);
+      down2(ctx, P, g, KCY, io.taub_cy, P->F.taub_cy, d);
+    }
+    if (dyn != IMAU_DYN_SIA) down3(ctx, P, g, KA, io.visc_eff_3D_a, P->visc_cur);
+    down3(ctx, P, g, KCX, io.u_3D_cx, P->O.u3);
+    down3(ctx, P, g, KCY, io.v_3D_cy, P->O.v3);
+    if (dyn == IMAU_DYN_SIA || dyn == IMAU_DYN_SIA_SSA) {
+      down3(ctx, P, g, KCX, io.u_3D_SIA_cx, P->O.u3sia);
+      down3(ctx, P, g, KCY, io.v_3D_SIA_cy, P->O.v3sia);
+    }
+    down3(ctx, P, g, KA, io.u_3D_a, P->O.u3a);
+    down3(ctx, P, g, KA, io.v_3D_a, P->O.v3a);
+    down3(ctx, P, g, KA, io.w_3D_a, P->O.w3);
+    if (dyn == IMAU_DYN_DIVA) down3(ctx, P, g, KA, io.F1_3D_a, P->O.F1);
+    down2(ctx, P, g, KA, io.u_vav_a, P->O.u_vav_a, d);
+    down2(ctx, P, g, KA, io.v_vav_a, P->O.v_vav_a, d);
+    down2(ctx, P, g, KA, io.u_surf_a, P->O.u_surf_a, d);
+    down2(ctx, P, g, KA, io.v_surf_a, P->O.v_surf_a, d);
+    down2(ctx, P, g, KA, io.u_base_a, P->O.u_base_a, d);
+    down2(ctx, P, g, KA, io.v_base_a, P->O.v_base_a, d);
+    down2(ctx, P, g, KA, io.uabs_vav_a, P->O.uabs_vav, d);
+    down2(ctx, P, g, KA, io.uabs_surf_a, P->O.uabs_surf, d);
+    down2(ctx, P, g, KA, io.uabs_base_a, P->O.uabs_base, d);
+    down2(ctx, P, g, KA, io.w_surf_a, P->O.w_surf, d);
+    down2(ctx, P, g, KA, io.w_base_a, P->O.w_base, d);
+    down2(ctx, P, g, KA, io.R_shear, P->O.R_shear, d);
+    down_plane(ctx, P, g, KCX, io.u_surf_cx, P->O.u3, 0);
+    down_plane(ctx, P, g, KCY, io.v_surf_cy, P->O.v3, 0);
+    down2(ctx, P, g, KCX, io.u_base_cx, P->F.u_base, d);
+    down2(ctx, P, g, KCY, io.v_base_cy, P->F.v_base, d);
+    if (dyn != IMAU_DYN_SIA) {
+      down2(ctx, P, g, KA, io.beta_a, P->F.beta_a, d);
+      down2(ctx, P, g, KA, io.beta_eff_a, P->F.beta_eff_a, d);
+      down2(ctx, P, g, KCX, io.beta_eff_cx, P->F.beta_eff_cx, d);
+      down2(ctx, P, g, KCY, io.beta_eff_cy, P->F.beta_eff_cy, d);
+      down2(ctx, P, g, KCX, io.taudx_cx, P->F.taudx, d);
+      down2(ctx, P, g, KCY, io.taudy_cy, P->F.taudy, d);
+      down2(ctx, P, g, KA, io.N_a, P->F.N_a, d);
+      down2(ctx, P, g, KB, io.N_b, P->F.N_b, d);
+      down2(ctx, P, g, KCX, io.N_cx, P->F.N_cx, d);
+      down2(ctx, P, g, KCY, io.N_cy, P->F.N_cy, d);
+      down2(ctx, P, g, KA, io.visc_eff_int_a, P->F.visc_int, d);
+      down2(ctx, P, g, KA, io.F2_a, P->F.F2, d);
+      down2(ctx, P, g, KA, io.f_grnd_a, P->F.fg_a, d);
+      down2(ctx, P, g, KCX, io.f_grnd_cx, P->F.fg_cx, d);
+      down2(ctx, P, g, KCY, io.f_grnd_cy, P->F.fg_cy, d);
+      down2(ctx, P, g, KB, io.f_grnd_b, P->F.fg_b, d);
+      down2(ctx, P, g, KA, io.overburden_pressure_a, P->F.overb, d);
+      down2(ctx, P, g, KA, io.pore_water_pressure_a, P->F.pore, d);
+      down2(ctx, P, g, KA, io.Neff_a, P->F.Neff, d);
+      down2(ctx, P, g, KA, io.beta_sq_a, P->F.beta_sq, d);
+      down2(ctx, P, g, KA, io.alpha_sq_a, P->F.alpha_sq, d);
+      down2(ctx, P, g, KA, io.phi_fric_a, P->F.phi_fric, d);
+      down2(ctx, P, g, KA, io.tauc_a, P->F.tauc, d);
+      down2(ctx, P, g, KA, io.DIVA_mask_combi, P->F.combi, q);
+      down2(ctx, P, g, KA, io.DIVA_isfront_inner, P->F.fin, q);
+      down2(ctx, P, g, KA, io.DIVA_isfront_outer, P->F.fout, q);
+      down2(ctx, P, g, KCX, io.u_cx_prev, P->F.u_prev, d);
+      down2(ctx, P, g, KCY, io.v_cy_prev, P->F.v_prev, d);
+    }
+    io.stat_picard_iterations = P->picard_its[g];
+    io.stat_linear_sweeps = P->sweeps[g];
+    io.stat_resid_UV = P->resid_uv[g];
+    io.stat_nonlinear_residual = P->nl_resid[g];
+    io.stat_seconds = P->seconds;
+  }
+  HIPCHK(hipStreamSynchronize(ctx->s));
+  // the reference sets u, v = 0 and returns for grids without grounded ice (:276-285)
+  for (int g = 0; g < P->ngrids; ++g) {
+    if (P->sheet_any[g]) continue;
+    const Member &m = P->mem[g];
+    imau_ice_io &io = ios[g];
+    double *u = (dyn == IMAU_DYN_DIVA) ? io.u_vav_cx : io.u_SSA_cx;
+    double *v = (dyn == IMAU_DYN_DIVA) ? io.v_vav_cy : io.v_SSA_cy;
+    if (u) std::memset(u, 0, (size_t)m.ny * (m.nx - 1) * sizeof(double));
+    if (v) std::memset(v, 0, (size_t)(m.ny - 1) * m.nx * sizeof(double));
+  }
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+static int solve_convenience(imau_ctx *ctx, int ngrids, const imau_grid *grids, const imau_cfg *cfg, imau_ice_io *ios,
+                             int kinds) {
+  if (!ctx) return IMAU_ERR_ARG;
+  imau_problem *P = nullptr;
+  int rc = imau_vel_problem_create(ctx, ngrids, grids, cfg, ios, &P);
+  if (rc) return rc;
+  rc = solve_kind(ctx, P, cfg, kinds);
+  if (!rc) rc = imau_vel_problem_download(ctx, P, ios);
+  imau_vel_problem_destroy(P);
+  return rc;
+}
+
+int imau_vel_solve_diva(imau_ctx *ctx, const imau_grid *grid, const imau_cfg *cfg, imau_ice_io *io) {
+  return solve_convenience(ctx, 1, grid, cfg, io, SOLVE_DIVA);
+}
+int imau_vel_solve_ssa(imau_ctx *ctx, const imau_grid *grid, const imau_cfg *cfg, imau_ice_io *io) {
+  return solve_convenience(ctx, 1, grid, cfg, io, SOLVE_SSA);
+}
+int imau_vel_solve_sia(imau_ctx *ctx, const imau_grid *grid, const imau_cfg *cfg, imau_ice_io *io) {
+  return solve_convenience(ctx, 1, grid, cfg, io, SOLVE_SIA);
+}
+int imau_vel_solve_diva_batch(imau_ctx *ctx, int32_t ngrids, const imau_grid *grids, const imau_cfg *cfg,
+                              imau_ice_io *ios) {
+  return solve_convenience(ctx, ngrids, grids, cfg, ios, SOLVE_DIVA);
+}
+
+int imau_vel_nonlinear_residual(imau_ctx *ctx, const imau_grid *grid, const imau_cfg *cfg, imau_ice_io *io,
+                                double *residual) {
+  imau_problem *P = nullptr;
+  ABI_TRY
+  if (!ctx || !grid || !cfg || !io || !residual) fail(IMAU_ERR_ARG, "NULL argument");
+  int rc = imau_vel_problem_create(ctx, 1, grid, cfg, io, &P);
+  if (rc) return rc;
+  const bool diva = cfg->choice_ice_dynamics == IMAU_DYN_DIVA;
+  Solver S(ctx, P, cfg);
+  imau::setup1(S.L, S.kp, P->F, P->yg.p);
+  imau::setup2(S.L, S.kp, P->F);
+  std::vector<int> all(1, 1);
+  S.push_active(P->act_pic.p, all);
+  imau::visc(S.L, S.kp, P->F, diva, P->visc_cur, P->visc_oth, P->xg.p, P->yg.p, P->act_pic.p);
+  std::swap(P->visc_cur, P->visc_oth);
+  imau::coef(S.L, S.kp, P->F, diva, P->act_pic.p);
+  S.kcheck();
+  std::vector<double> r;
+  S.residual(r);
+  *residual = r[0];
+  imau_vel_problem_destroy(P);
+  return IMAU_OK;
+  }
+  catch (const Err &e) {
+    ctx->err = e.msg;
+    imau_vel_problem_destroy(P);
+    return e.code;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,1329 @@
+// kernels.hip -- HIP/gfx950 kernels of the ice-velocity hot path.
+//
+// Every kernel takes the tile work-list: workgroup b processes tile
+// tiles[b] = (member g, i0, j0) of IMAU_TI x IMAU_TJ a-grid cells with a
+// 64 x 4 thread block (lane = j, wavefront = i).  The same launch therefore
+// covers one grid or an ensemble of uneven grids.
+#include "dev_common.h"
+#include "kernels.h"
+
+#define TILE_SETUP                                  \
+  const Tile t = tiles[blockIdx.x];                 \
+  const Member m = mem[t.g];                        \
+  const int j = t.j0 + (int)threadIdx.x;
+
+#define FOR_TILE_ROWS for (int ii = threadIdx.y; ii < IMAU_TI; ii += IMAU_WG_I)
+
+// ---------------------------------------------------------------------------
+// block reductions (wave64 shuffles, then LDS across the 4 wavefronts)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  return v;
+}
+template <int NV>
+__device__ void block_sum_store(double (&v)[NV], double *out) {
+  __shared__ double red[IMAU_WG_I][NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double s = wave_sum(v[q]);
+    if (threadIdx.x == 0) red[threadIdx.y][q] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double s = 0.0;
+      for (int w = 0; w < IMAU_WG_I; ++w) s += red[w][q];
+      out[q] = s;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// sliding laws, src/basal_conditions_and_sliding_module.f90:908-1432
+// ---------------------------------------------------------------------------
+__device__ double sliding_beta(const KP &kp, double ua, double va, double beta_sq, double alpha_sq, double phi,
+                               double tauc, double Neff, double x, double y, double A_vav) {
+  const double uabs = sqrt(kp.delta_v * kp.delta_v + ua * ua + va * va);
+  switch (kp.slid) {
+    case 0: return 0.0;
+    case 4: return beta_sq * pow(uabs, 1.0 / kp.wm - 1.0);                                   // Weertman :984-987
+    case 2: return tauc / uabs;                                                               // Coulomb :1039-1041
+    case 3: return tauc * pow(uabs, kp.q_plastic - 1.0) / pow(kp.u_threshold, kp.q_plastic);  // :1078-1083
+    case 5: return fmin(alpha_sq * Neff, beta_sq * pow(uabs, 1.0 / kp.wm)) * pow(uabs, -1.0);  // Tsai :1128-1131
+    case 6: {                                                                                 // Schoof :1175-1183
+      const double mm = kp.wm;
+      return ((beta_sq * pow(uabs, 1.0 / mm) * alpha_sq * Neff) /
+              pow(pow(beta_sq, mm) * uabs + pow(alpha_sq * Neff, mm), 1.0 / mm)) * pow(uabs, -1.0);
+    }
+    case 7:                                                                                   // Zoet-Iverson :1220-1223
+      return Neff * tan((D_PI / 180.0) * phi) * pow(uabs, 1.0 / kp.zi_p - 1.0) * pow(uabs + kp.zi_ut, -1.0 / kp.zi_p);
+    case 1:                                                                                   // idealised :1288-1432
+      if (kp.ideal == 0) return 1000.0 + 1000.0 * sin(2.0 * D_PI * x / kp.ismip_L) * sin(2.0 * D_PI * y / kp.ismip_L);
+      if (kp.ideal == 1) return 1000.0 + 1000.0 * sin(2.0 * D_PI * x / kp.ismip_L);
+      if (kp.ideal == 3) return pow(A_vav * 1000.0, -1.0);
+      return __builtin_nan("");
+    default: return __builtin_nan("");
+  }
+}
+
+// Schoof (2006) till yield stress (src/utilities_module.f90:836-877), tauc part
+__device__ double schoof_tauc(double tantheta, double h0, double L, double mexp, double y) {
+  const double f = D_ICE_DENSITY * D_GRAV * h0 * tantheta;
+  return f * pow(fabs(y / L), mexp);
+}
+
+// ---------------------------------------------------------------------------
+// grounded fraction of a bilinear quad, src/general_ice_model_data_module.f90:521-781
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double gf_perturb(double f, double s) {
+  const double ftol = 1e-4;
+  if (f == 0.0) return ftol * s;
+  if (f > 0.0) return fmax(ftol * s, f);
+  if (f < 0.0) return fmin(-ftol * s, f);
+  return f;
+}
+__device__ __forceinline__ void gf_rot(double &NW, double &NE, double &SW, double &SE) {
+  const double a = NW, b = NE, c = SE, d = SW;
+  NW = b; NE = c; SE = d; SW = a;
+}
+__device__ __forceinline__ double gf_corner(double aa, double bb, double cc, double dd) {
+  return ((bb * cc - aa * dd) * log(fabs(1.0 - (aa * dd) / (bb * cc))) + aa * dd) / (dd * dd);
+}
+__device__ __forceinline__ double gf_strip(double aa, double bb, double cc, double dd, double x) {
+  return ((bb * cc - aa * dd) * log(fabs(cc + dd * x)) - bb * dd * x) / (dd * dd);
+}
+__device__ double fraction_above_zero(double fNW, double fNE, double fSW, double fSE) {
+  double NW = gf_perturb(fNW, 1.1), NE = gf_perturb(fNE, 1.21), SW = gf_perturb(fSW, 1.13), SE = gf_perturb(fSE, 0.97);
+  double phi;
+  if (NW >= 0 && NE >= 0 && SW >= 0 && SE >= 0) return 1.0;
+  if (NW <= 0 && NE <= 0 && SW <= 0 && SE <= 0) return 0.0;
+  int scen = 0;
+  for (int it = 0; it < 5 && scen == 0; ++it) {
+    gf_rot(NW, NE, SW, SE);
+    if (SW > 0 && SE < 0 && NE < 0 && NW < 0) scen = 1;
+    else if (SW < 0 && SE > 0 && NE > 0 && NW > 0) scen = 2;
+    else if (SW > 0 && SE > 0 && NE < 0 && NW < 0) scen = 3;
+    else if (SW > 0 && SE < 0 && NE > 0 && NW < 0) scen = 4;
+  }
+  double aa, bb, cc, dd;
+  if (scen == 1) {
+    aa = SW; bb = SE - SW; cc = NW - SW; dd = NE + SW - NW - SE;
+    phi = gf_corner(aa, bb, cc, dd);
+    if (fabs(dd) < 1e-4) {
+      SW = (SW > 0) ? SW + 0.1 : SW - 0.1;
+      aa = SW; bb = SE - SW; cc = NW - SW; dd = NE + SW - NW - SE;
+      phi = gf_corner(aa, bb, cc, dd);
+    }
+  } else if (scen == 2) {
+    aa = -SW; bb = -(SE - SW); cc = -(NW - SW); dd = -(NE + SW - NW - SE);
+    phi = 1.0 - gf_corner(aa, bb, cc, dd);
+    if (fabs(dd) < 1e-4) {
+      SW = (SW > 0) ? SW + 0.1 : SW - 0.1;
+      aa = -SW; bb = -(SE - SW); cc = -(NW - SW); dd = -(NE + SW - NW - SE);
+      phi = 1.0 - gf_corner(aa, bb, cc, dd);
+    }
+  } else if (scen == 3) {
+    if (fabs(1.0 - NW / NE) < 1e-6 && fabs(1.0 - SW / SE) < 1e-6) {
+      phi = SW / (SW - NW);
+    } else {
+      aa = SW; bb = SE - SW; cc = NW - SW; dd = NE + SW - NW - SE;
+      phi = gf_strip(aa, bb, cc, dd, 1.0) - gf_strip(aa, bb, cc, dd, 0.0);
+      if (fabs(dd) < 1e-4) {
+        SW = (SW > 0) ? SW + 0.1 : SW - 0.1;
+        aa = SW; bb = SE - SW; cc = NW - SW; dd = NE + SW - NW - SE;
+        phi = gf_strip(aa, bb, cc, dd, 1.0) - gf_strip(aa, bb, cc, dd, 0.0);
+      }
+    }
+  } else if (scen == 4) {
+    aa = SW; bb = SE - SW; cc = NW - SW; dd = NE + SW - NW - SE;
+    phi = gf_corner(aa, bb, cc, dd);
+    gf_rot(NW, NE, SW, SE);
+    gf_rot(NW, NE, SW, SE);
+    aa = SW; bb = SE - SW; cc = NW - SW; dd = NE + SW - NW - SE;
+    phi += gf_corner(aa, bb, cc, dd);
+  } else {
+    phi = __builtin_nan("");
+  }
+  return fmax(0.0, fmin(1.0, phi));
+}
+
+// ---------------------------------------------------------------------------
+// per-solve setup 1: driving stress, basal conditions, grounded-fraction
+// quadrants, SICOPOLIS masks, calving-front rhs, B = 0.5 A^(-1/n)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(IMAU_WG) k_setup1(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                    const KP kp, const Fields f, const double *__restrict__ yg) {
+  TILE_SETUP
+  if (j >= m.ny) return;
+  const int nx = m.nx, ny = m.ny;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= nx) break;
+    const long long p = p2(m, j, i);
+    // taudx/taudy, src/ice_velocity_module.f90:288-294
+    if (i < nx - 1) f.taudx[p] = -D_ICE_DENSITY * D_GRAV * f.Hi_cx[p] * f.dHs_dx_cx[p];
+    if (j < ny - 1) f.taudy[p] = -D_ICE_DENSITY * D_GRAV * f.Hi_cy[p] * f.dHs_dy_cy[p];
+    // basal hydrology, src/basal_conditions_and_sliding_module.f90:90-238
+    const double Hi = f.Hi[p], Hb = f.Hb[p], SL = f.SL[p];
+    double pore;
+    if (kp.hydro == 0) pore = 0.0;
+    else if (kp.hydro == 1) pore = -D_SEAWATER_DENSITY * D_GRAV * (SL - Hb);
+    else {
+      const double lp = fmin(1.0, fmax(0.0, 1.0 - (Hb - SL - kp.hydro_min) / (kp.hydro_max - kp.hydro_min)));
+      pore = 0.96 * D_ICE_DENSITY * D_GRAV * Hi * lp;
+    }
+    const double ob = D_ICE_DENSITY * D_GRAV * Hi;
+    f.pore[p] = pore;
+    f.overb[p] = ob;
+    f.Neff[p] = fmax(0.0, ob - pore);
+    // bed roughness parameterisations, :243-294, :769-903
+    if (kp.slid != 0 && kp.slid != 1 && kp.rough == 1) {
+      if (kp.prough == 0) {
+        const double w = fmin(1.0, fmax(0.0, (Hb - kp.phi_hb_min) / (kp.phi_hb_max - kp.phi_hb_min)));
+        f.phi_fric[p] = (1.0 - w) * kp.phi_min + w * kp.phi_max;
+      } else if (kp.prough == 1) {
+        f.tauc[p] = schoof_tauc(fabs(f.dHs_dx_a[p]), Hi, kp.ssa_L, kp.ssa_m, yg[m.offy + j]);
+      } else {
+        f.beta_sq[p] = 1.0e4;
+        f.alpha_sq[p] = 0.5;
+      }
+    }
+    // grounded-fraction quadrants from the edge-clamped TAF field, :420-520
+    {
+      const int im = iclamp(i - 1, 0, nx - 1), ip = iclamp(i + 1, 0, nx - 1);
+      const int jm = iclamp(j - 1, 0, ny - 1), jp = iclamp(j + 1, 0, ny - 1);
+      const double T_nw = f.TAF[p2(m, jp, im)], T_n = f.TAF[p2(m, jp, i)], T_ne = f.TAF[p2(m, jp, ip)];
+      const double T_w = f.TAF[p2(m, j, im)], T_m = f.TAF[p], T_e = f.TAF[p2(m, j, ip)];
+      const double T_sw = f.TAF[p2(m, jm, im)], T_s = f.TAF[p2(m, jm, i)], T_se = f.TAF[p2(m, jm, ip)];
+      const double f_NW = 0.25 * (T_nw + T_n + T_w + T_m);
+      const double f_N = 0.5 * (T_n + T_m);
+      const double f_NE = 0.25 * (T_n + T_ne + T_m + T_e);
+      const double f_W = 0.5 * (T_w + T_m);
+      const double f_m = T_m;
+      const double f_E = 0.5 * (T_m + T_e);
+      const double f_SW = 0.25 * (T_w + T_m + T_sw + T_s);
+      const double f_S = 0.5 * (T_m + T_s);
+      const double f_SE = 0.25 * (T_m + T_e + T_s + T_se);
+      f.qNW[p] = fraction_above_zero(f_NW, f_N, f_W, f_m);
+      f.qNE[p] = fraction_above_zero(f_N, f_NE, f_m, f_E);
+      f.qSW[p] = fraction_above_zero(f_W, f_m, f_SW, f_S);
+      f.qSE[p] = fraction_above_zero(f_m, f_E, f_S, f_SE);
+    }
+    // set_sico_masks, src/ice_velocity_module.f90:1322-1431
+    {
+      const bool flt = f.TAF[p] < 0.0;
+      const int combi = (Hi == 0.0) ? (flt ? 2 : 1) : (flt ? 3 : 0);
+      f.combi[p] = combi;
+      const int i1 = i > 0 ? i - 1 : 0, i2 = i < nx - 1 ? i + 1 : nx - 1;
+      const int j1 = j > 0 ? j - 1 : 0, j2 = j < ny - 1 ? j + 1 : ny - 1;
+      const double Hw = f.Hi[p2(m, j, i1)], He = f.Hi[p2(m, j, i2)], Hs_ = f.Hi[p2(m, j1, i)], Hn = f.Hi[p2(m, j2, i)];
+      int fin = (Hi > 0.0 && (Hw == 0.0 || He == 0.0 || Hs_ == 0.0 || Hn == 0.0)) ? 1 : 0;
+      const int fout = (Hi == 0.0 && (Hw > 0.0 || He > 0.0 || Hs_ > 0.0 || Hn > 0.0)) ? 1 : 0;
+      if (fin == 1 && combi == 0) fin = 0;
+      f.fin[p] = fin;
+      f.fout[p] = fout;
+      // calving-front rhs, :2250-2261
+      double H_ocn = 0.0;
+      if (SL > Hb) H_ocn = fmin(D_ICE_DENSITY / D_SEAWATER_DENSITY * Hi, SL - Hb);
+      f.front_rhs[p] = 0.5 * D_ICE_DENSITY * D_GRAV * Hi * Hi - 0.5 * D_SEAWATER_DENSITY * D_GRAV * H_ocn * H_ocn;
+    }
+    // B = 0.5 * A^(-1/n) (first factor of src/ice_velocity_module.f90:799)
+    for (int k = 0; k < kp.nz; ++k) {
+      const long long q = p3(m, k, j, i);
+      const double A = f.A3[q];
+      f.B3[q] = (kp.n_flow == 3.0) ? 0.5 / cbrt(A) : 0.5 * pow(A, -1.0 / kp.n_flow);
+    }
+  }
+}
+
+__device__ __forceinline__ int corner_bc(int b1, int b2) {
+  if (b1 == 1 || b2 == 1) return 1;  // zero
+  if (b1 == 0 || b2 == 0) return 0;  // infinite
+  return 2;                          // periodic
+}
+// BC code of a domain-boundary row at (j,i) with last indices imax/jmax, :1954-2004
+__device__ __forceinline__ int boundary_bc(const int *bc, int i, int j, int imax, int jmax) {
+  const int bw = bc[0], be = bc[1], bs = bc[2], bn = bc[3];
+  if (i == 0 && j == 0) return corner_bc(bs, bw);
+  if (i == imax && j == 0) return corner_bc(bs, be);
+  if (i == 0 && j == jmax) return corner_bc(bn, bw);
+  if (i == imax && j == jmax) return corner_bc(bn, be);
+  if (i == 0) return bw;
+  if (i == imax) return be;
+  if (j == 0) return bs;
+  return bn;
+}
+__device__ __forceinline__ uint8_t bc_rowtype(int bc) { return bc == 0 ? RT_BND_INF : (bc == 1 ? RT_BND_ZERO : RT_BND_PER); }
+
+// per-solve setup 2: grounded fractions on all grids, row types
+__global__ void __launch_bounds__(IMAU_WG) k_setup2(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                    const KP kp, const Fields f) {
+  TILE_SETUP
+  if (j >= m.ny) return;
+  const int nx = m.nx, ny = m.ny;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= nx) break;
+    const long long p = p2(m, j, i);
+    // src/general_ice_model_data_module.f90:380-400
+    f.fg_a[p] = 0.25 * (f.qNW[p] + f.qNE[p] + f.qSW[p] + f.qSE[p]);
+    if (i < nx - 1) f.fg_cx[p] = 0.25 * (f.qNE[p] + f.qSE[p] + f.qNW[p2(m, j, i + 1)] + f.qSW[p2(m, j, i + 1)]);
+    if (j < ny - 1) f.fg_cy[p] = 0.25 * (f.qNE[p] + f.qNW[p] + f.qSE[p2(m, j + 1, i)] + f.qSW[p2(m, j + 1, i)]);
+    if (i < nx - 1 && j < ny - 1)
+      f.fg_b[p] = 0.25 * (f.qNE[p] + f.qNW[p2(m, j, i + 1)] + f.qSE[p2(m, j + 1, i)] + f.qSW[p2(m, j + 1, i + 1)]);
+    // row types, src/ice_velocity_module.f90:1762-1931
+    if (i < nx - 1) {
+      uint8_t rt = RT_FREE;
+      if (i == 0 || i == nx - 2 || j == 0 || j == ny - 1) rt = bc_rowtype(boundary_bc(kp.bc_u, i, j, nx - 2, ny - 1));
+      else if (kp.margin == 0) {
+        const int c0 = f.combi[p], c1 = f.combi[p2(m, j, i + 1)];
+        if ((c0 == 3 && c1 == 1) || (c0 == 1 && c1 == 3)) rt = RT_ZERO;
+        else if ((f.fin[p] == 1 && f.fout[p2(m, j, i + 1)] == 1) || (f.fout[p] == 1 && f.fin[p2(m, j, i + 1)] == 1)) {
+          const int i1 = (f.fin[p] == 1) ? i : i + 1;
+          if (f.fout[p2(m, j, i1 - 1)] == 1 && f.fout[p2(m, j, i1 + 1)] == 1) rt = RT_ZERO;
+          else rt = (i1 == i) ? RT_MARGIN_I : RT_MARGIN_O;
+        }
+      }
+      f.rt_u[p] = rt;
+    }
+    if (j < ny - 1) {
+      uint8_t rt = RT_FREE;
+      if (i == 0 || i == nx - 1 || j == 0 || j == ny - 2) rt = bc_rowtype(boundary_bc(kp.bc_v, i, j, nx - 1, ny - 2));
+      else if (kp.margin == 0) {
+        const int c0 = f.combi[p], c1 = f.combi[p2(m, j + 1, i)];
+        if ((c0 == 3 && c1 == 1) || (c0 == 1 && c1 == 3)) rt = RT_ZERO;
+        else if ((f.fin[p] == 1 && f.fout[p2(m, j + 1, i)] == 1) || (f.fout[p] == 1 && f.fin[p2(m, j + 1, i)] == 1)) {
+          const int j1 = (f.fin[p] == 1) ? j : j + 1;
+          if (f.fout[p2(m, j1 - 1, i)] == 1 && f.fout[p2(m, j1 + 1, i)] == 1) rt = RT_ZERO;
+          else rt = (j1 == j) ? RT_MARGIN_I : RT_MARGIN_O;
+        }
+      }
+      f.rt_v[p] = rt;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Picard step 1: vertical shear strain rates + effective viscosity + N_a +
+// F2 + beta + beta_eff on the a-grid (fused A4, A5, A6, A7, A8-a).
+// Reads visc (previous iteration), writes visc_out.
+// src/ice_velocity_module.f90:664-1086
+// ---------------------------------------------------------------------------
+template <bool DIVA>
+__global__ void __launch_bounds__(IMAU_WG) k_visc(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                  const KP kp, const Fields f, const double *__restrict__ visc_in,
+                                                  double *__restrict__ visc_out, const double *__restrict__ xg,
+                                                  const double *__restrict__ yg, const int *__restrict__ active) {
+  TILE_SETUP
+  if (j >= m.ny) return;
+  const int nx = m.nx, ny = m.ny, nz = kp.nz;
+  if (!active[t.g]) {  // finished member: keep its viscosity in the buffer that becomes current
+    FOR_TILE_ROWS {
+      const int i = t.i0 + ii;
+      if (i >= nx) break;
+      for (int k = 0; k < nz; ++k) visc_out[p3(m, k, j, i)] = visc_in[p3(m, k, j, i)];
+    }
+    return;
+  }
+  const double dx = m.dx;
+  const double *__restrict__ u = f.u;
+  const double *__restrict__ v = f.v;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= nx) break;
+    const long long p = p2(m, j, i);
+    const double Hi = f.Hi[p];
+    const int mi = f.m_ice[p];
+    // horizontal strain rates on the a-grid (ddx_cx_to_a, ddy_cy_to_a, ddy_cx_to_a,
+    // ddx_cy_to_a; src/derivatives_and_grids_module.f90:755-938)
+#define U(jj, ii_) u[p2(m, (jj), (ii_))]
+#define V(jj, ii_) v[p2(m, (jj), (ii_))]
+    double dudx, dvdy, dudy, dvdx;
+    {
+      const int ic = iclamp(i, 1, nx - 2);
+      dudx = (U(j, ic) - U(j, ic - 1)) / dx;
+      const int jc = iclamp(j, 1, ny - 2);
+      dvdy = (V(jc, i) - V(jc - 1, i)) / dx;
+    }
+    const bool iint = (i > 0 && i < nx - 1), jint = (j > 0 && j < ny - 1);
+    if (iint && jint) {
+      dudy = (U(j + 1, i - 1) + U(j + 1, i) - U(j - 1, i - 1) - U(j - 1, i)) / (4.0 * dx);
+      dvdx = (V(j - 1, i + 1) + V(j, i + 1) - V(j - 1, i - 1) - V(j, i - 1)) / (4.0 * dx);
+    } else if (iint) {  // j == 0 or ny-1
+      if (j == 0) {
+        dudy = (U(1, i - 1) + U(1, i) - U(0, i - 1) - U(0, i)) / (4.0 * dx);
+        dvdx = (V(0, i + 1) - V(0, i - 1)) / (2.0 * dx);
+      } else {
+        dudy = (U(ny - 1, i - 1) + U(ny - 1, i) - U(ny - 2, i - 1) - U(ny - 2, i)) / (4.0 * dx);
+        dvdx = (V(ny - 2, i + 1) - V(ny - 2, i - 1)) / (2.0 * dx);
+      }
+    } else if (jint) {  // i == 0 or nx-1
+      if (i == 0) {
+        dudy = (U(j + 1, 0) - U(j - 1, 0)) / (2.0 * dx);
+        dvdx = (V(j - 1, 1) + V(j, 1) - V(j - 1, 0) - V(j, 0)) / (4.0 * dx);
+      } else {
+        dudy = (U(j + 1, nx - 2) - U(j - 1, nx - 2)) / (2.0 * dx);
+        dvdx = (V(j - 1, nx - 1) + V(j, nx - 1) - V(j - 1, nx - 2) - V(j, nx - 2)) / (4.0 * dx);
+      }
+    } else {  // corners
+      const int ic = (i == 0) ? 0 : nx - 2;
+      const int jv = (j == 0) ? 0 : ny - 2;
+      dudy = (j == 0) ? (U(1, ic) - U(0, ic)) / dx : (U(ny - 1, ic) - U(ny - 2, ic)) / dx;
+      dvdx = (i == 0) ? (V(jv, 1) - V(jv, 0)) / dx : (V(jv, nx - 1) - V(jv, nx - 2)) / dx;
+    }
+    // velocities on the a-grid for the sliding law (map_cx_to_a / map_cy_to_a, :1579-1644)
+    const double ua = (i == 0) ? U(j, 0) : (i == nx - 1 ? U(j, nx - 2) : 0.5 * (U(j, i - 1) + U(j, i)));
+    const double va = (j == 0) ? V(0, i) : (j == ny - 1 ? V(ny - 2, i) : 0.5 * (V(j - 1, i) + V(j, i)));
+#undef U
+#undef V
+    const double eps_h = dudx * dudx + dvdy * dvdy + dudx * dvdy + 0.25 * (dudy + dvdx) * (dudy + dvdx) + kp.eps0;
+
+    const bool bc_skip = (kp.margin == 0 && mi == 0);
+    double vint = 0.0, F2int = 0.0;
+    if (bc_skip) {
+      for (int k = 0; k < nz; ++k) visc_out[p3(m, k, j, i)] = 0.0;
+    } else {
+      // neighbour data for the vertical shear terms (:691-692, :704-705, :775-788)
+      const int icl = (i == 0) ? 0 : (i == nx - 1 ? nx - 2 : i - 1);  // left cx index used
+      const int icr = (i == 0) ? 0 : (i == nx - 1 ? nx - 2 : i);      // right cx index used
+      const int jcl = (j == 0) ? 0 : (j == ny - 1 ? ny - 2 : j - 1);
+      const int jcr = (j == 0) ? 0 : (j == ny - 1 ? ny - 2 : j);
+      const bool use_avg_x = (i > 0 && i < nx - 1), use_avg_y = (j > 0 && j < ny - 1);
+      double tbl = 0, tbr = 0, tcl = 0, tcr = 0;
+      double Hl0 = 0, Hl1 = 0, Hr0 = 0, Hr1 = 0, Hb0 = 0, Hb1 = 0, Ht0 = 0, Ht1 = 0;
+      bool noice_l = true, noice_r = true, noice_b = true, noice_t = true;
+      if (DIVA) {
+        tbl = f.taub_cx[p2(m, j, icl)]; tbr = f.taub_cx[p2(m, j, icr)];
+        tcl = f.taub_cy[p2(m, jcl, i)]; tcr = f.taub_cy[p2(m, jcr, i)];
+        Hl0 = f.Hi[p2(m, j, icl)]; Hl1 = f.Hi[p2(m, j, icl + 1)];
+        Hr0 = f.Hi[p2(m, j, icr)]; Hr1 = f.Hi[p2(m, j, icr + 1)];
+        Hb0 = f.Hi[p2(m, jcl, i)]; Hb1 = f.Hi[p2(m, jcl + 1, i)];
+        Ht0 = f.Hi[p2(m, jcr, i)]; Ht1 = f.Hi[p2(m, jcr + 1, i)];
+        noice_l = f.m_ice[p2(m, j, icl)] == 0 && f.m_ice[p2(m, j, icl + 1)] == 0;
+        noice_r = f.m_ice[p2(m, j, icr)] == 0 && f.m_ice[p2(m, j, icr + 1)] == 0;
+        noice_b = f.m_ice[p2(m, jcl, i)] == 0 && f.m_ice[p2(m, jcl + 1, i)] == 0;
+        noice_t = f.m_ice[p2(m, jcr, i)] == 0 && f.m_ice[p2(m, jcr + 1, i)] == 0;
+      }
+      const double Hxl = use_avg_x ? f.Hi[p2(m, j, i - 1)] : 0.0;
+      const double Hyl = use_avg_y ? f.Hi[p2(m, j - 1, i)] : 0.0;
+      const bool third = (kp.n_flow == 3.0);
+      const double expo = (1.0 - kp.n_flow) / (2.0 * kp.n_flow);
+      double eta_prev_k = 0.0, g_prev_k = 0.0;
+      for (int k = 0; k < nz; ++k) {
+        const double z = kp.zeta[k];
+        double du_dz_a = 0.0, dv_dz_a = 0.0;
+        if (DIVA) {
+          // du/dz on the two cx neighbours (calc_vertical_shear_strain_rates)
+          double dl = 0.0, dr = 0.0, db = 0.0, dt = 0.0;
+          if (!noice_l) dl = (tbl / stag_margin(visc_in[p3(m, k, j, icl)], visc_in[p3(m, k, j, icl + 1)], Hl0, Hl1)) * z;
+          if (!noice_r) dr = (tbr / stag_margin(visc_in[p3(m, k, j, icr)], visc_in[p3(m, k, j, icr + 1)], Hr0, Hr1)) * z;
+          if (!noice_b) db = (tcl / stag_margin(visc_in[p3(m, k, jcl, i)], visc_in[p3(m, k, jcl + 1, i)], Hb0, Hb1)) * z;
+          if (!noice_t) dt = (tcr / stag_margin(visc_in[p3(m, k, jcr, i)], visc_in[p3(m, k, jcr + 1, i)], Ht0, Ht1)) * z;
+          du_dz_a = use_avg_x ? stag_margin(dl, dr, Hxl, Hi) : dl;
+          dv_dz_a = use_avg_y ? stag_margin(db, dt, Hyl, Hi) : db;
+        }
+        const double eps_sq = eps_h + 0.25 * (du_dz_a * du_dz_a + dv_dz_a * dv_dz_a);
+        const double B = f.B3[p3(m, k, j, i)];
+        double eta = third ? B / cbrt(eps_sq) : B * pow(eps_sq, expo);
+        eta = fmax(eta, kp.visc_min);  // NaN (cold start 0/0) -> visc_eff_min
+        visc_out[p3(m, k, j, i)] = eta;
+        const double gk = (Hi / eta) * (z * z);
+        if (k > 0) {
+          const double dz = z - kp.zeta[k - 1];
+          vint += 0.5 * (eta + eta_prev_k) * dz;
+          F2int += 0.5 * (gk + g_prev_k) * dz;
+        }
+        eta_prev_k = eta;
+        g_prev_k = gk;
+      }
+      f.visc_int[p] = vint;
+      f.N_a[p] = vint * ((kp.margin == 0) ? Hi : fmax(0.1, Hi));
+    }
+    // F2 integral (calc_F_integral, :960-1013)
+    double F2 = 0.0;
+    if (DIVA) {
+      if (mi == 1) F2 = fmax(kp.F_int_min, F2int);
+      else F2 = (kp.margin == 0) ? 0.0 : kp.F_int_min;
+      f.F2[p] = F2;
+    }
+    // basal friction (calc_sliding_term_beta, :900-959)
+    double tauc = f.tauc[p];
+    if (kp.slid == 3 || (kp.slid == 2 && !(kp.rough == 1 && kp.prough == 1))) {
+      tauc = tan((D_PI / 180.0) * f.phi_fric[p]) * f.Neff[p];
+      f.tauc[p] = tauc;
+    }
+    double beta = sliding_beta(kp, ua, va, f.beta_sq[p], f.alpha_sq[p], f.phi_fric[p], tauc, f.Neff[p],
+                               xg[m.offx + i], yg[m.offy + j], f.A_vav ? f.A_vav[p] : 0.0);
+    beta = fmin(beta, kp.beta_max);
+    if (!kp.glsub && f.m_ocean[p] == 1) beta = 0.0;
+    f.beta_a[p] = beta;
+    // beta_eff (calc_beta_eff, :1014-1062); SSA uses beta itself (:193)
+    double be;
+    if (DIVA) {
+      if (kp.slid == 0) be = 1.0 / F2;
+      else {
+        be = beta / (1.0 + beta * F2);
+        if (kp.freeze) {
+          const long long qb = p3(m, nz - 1, j, i);
+          if (f.Tpmp3[qb] - f.Ti3[qb] > kp.dT_freeze) be = 1.0 / F2;
+        }
+      }
+    } else {
+      be = beta;
+    }
+    f.beta_eff_a[p] = be;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Picard step 2: staggered coefficients and right-hand sides
+// N_b (:820-874), beta_eff on cx/cy with f_grnd^2 (:1063-1076, SSA :195-206),
+// N_cx/N_cy for the sans-cross-terms form (:876-880), rhs of every row.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(IMAU_WG) k_coef(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                  const KP kp, const Fields f, int diva,
+                                                  const int *__restrict__ active) {
+  TILE_SETUP
+  if (j >= m.ny || !active[t.g]) return;
+  const int nx = m.nx, ny = m.ny;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= nx) break;
+    const long long p = p2(m, j, i);
+    if (i < nx - 1 && j < ny - 1) {
+      const long long pe = p2(m, j, i + 1), pn = p2(m, j + 1, i), pne = p2(m, j + 1, i + 1);
+      double vb;
+      if (kp.margin == 0) {
+        double s = 0.0, w = 0.0;
+        if (f.m_ice[p] == 1) { s += f.visc_int[p]; w += 1.0; }
+        if (f.m_ice[pe] == 1) { s += f.visc_int[pe]; w += 1.0; }
+        if (f.m_ice[pn] == 1) { s += f.visc_int[pn]; w += 1.0; }
+        if (f.m_ice[pne] == 1) { s += f.visc_int[pne]; w += 1.0; }
+        vb = (w > 0.0) ? s / w : 0.0;
+        f.N_b[p] = vb * f.Hi_b[p];
+      } else {
+        vb = (f.visc_int[p] + f.visc_int[pe] + f.visc_int[pn] + f.visc_int[pne]) / 4.0;
+        f.N_b[p] = vb * fmax(0.1, f.Hi_b[p]);
+      }
+    }
+    const bool use_fg = diva ? (kp.glsub != 0) : true;
+    if (i < nx - 1) {
+      const long long pe = p2(m, j, i + 1);
+      double b = (f.beta_eff_a[p] + f.beta_eff_a[pe]) / 2.0;
+      if (use_fg) b *= f.fg_cx[p] * f.fg_cx[p];
+      f.beta_eff_cx[p] = b;
+      if (!kp.cross) f.N_cx[p] = (f.N_a[p] + f.N_a[pe]) / 2.0;
+    }
+    if (j < ny - 1) {
+      const long long pn = p2(m, j + 1, i);
+      double b = (f.beta_eff_a[p] + f.beta_eff_a[pn]) / 2.0;
+      if (use_fg) b *= f.fg_cy[p] * f.fg_cy[p];
+      f.beta_eff_cy[p] = b;
+      if (!kp.cross) f.N_cy[p] = (f.N_a[p] + f.N_a[pn]) / 2.0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The linearised staggered DIVA/SSA rows (src/ice_velocity_module.f90:1762-2589),
+// matrix-free.  For the unknown at (j,i) they return the off-diagonal sum S,
+// the diagonal D and the right-hand side b, so that the row reads S + D x = b.
+// ---------------------------------------------------------------------------
+struct CoefP {
+  const double *Na, *Nb, *Ncx, *Ncy, *bcx, *bcy, *taudx, *taudy, *frhs;
+  const uint8_t *rt_u, *rt_v;
+};
+
+// LDS tile of the sweep (u in [0], v in [1]) with a one-cell halo
+__shared__ double g_sx[2][IMAU_TI + 2][IMAU_TJ + 2];
+
+// LDS-tile accessor (the rows evaluated through it never leave the tile halo;
+// periodic boundary rows, which do, are handled separately in the sweep)
+template <int COMP>
+struct LAcc {
+  int i0, j0;
+  __device__ __forceinline__ double operator()(int jj, int ii) const { return g_sx[COMP][ii - i0 + 1][jj - j0 + 1]; }
+};
+
+// source cell of a domain-boundary row (src/ice_velocity_module.f90:2006-2030)
+__device__ __forceinline__ void bnd_src(uint8_t rt, int i, int j, int imax, int jmax, int &jo, int &io) {
+  if (rt == RT_BND_INF) {
+    io = (i == 0) ? 1 : (i == imax ? imax - 1 : i);
+    jo = (j == 0) ? 1 : (j == jmax ? jmax - 1 : j);
+  } else {
+    io = (i == 0) ? imax - 1 : (i == imax ? 1 : i);
+    jo = (j == 0) ? jmax - 1 : (j == jmax ? 1 : j);
+  }
+}
+
+template <class AU, class AV>
+__device__ __forceinline__ void u_row(const Member &m, const KP &kp, const CoefP &c, int j, int i, uint8_t rt,
+                                      const AU &U, const AV &V, double &S, double &D, double &b) {
+  const int nx = m.nx, ny = m.ny;
+  if (rt == RT_FREE) {
+    const double dx2 = m.dx * m.dx;
+    if (kp.cross) {
+      const double Na0 = c.Na[p2(m, j, i)], Na1 = c.Na[p2(m, j, i + 1)];
+      const double Nb0 = c.Nb[p2(m, j, i)], Nbm = c.Nb[p2(m, j - 1, i)];
+      S = 4.0 * Na1 * U(j, i + 1) + 4.0 * Na0 * U(j, i - 1) + Nb0 * U(j + 1, i) + Nbm * U(j - 1, i) +
+          (-2.0 * Na0 - Nb0) * V(j, i) + (2.0 * Na1 + Nb0) * V(j, i + 1) + (2.0 * Na0 + Nbm) * V(j - 1, i) +
+          (-2.0 * Na1 - Nbm) * V(j - 1, i + 1);
+      D = -4.0 * (Na1 + Na0) + -1.0 * (Nb0 + Nbm) - c.bcx[p2(m, j, i)] * dx2;
+      b = -c.taudx[p2(m, j, i)] * dx2;
+    } else {
+      const double Ncx = c.Ncx[p2(m, j, i)];
+      S = 4.0 * U(j, i + 1) + 4.0 * U(j, i - 1) + U(j + 1, i) + U(j - 1, i) - 3.0 * V(j, i) + 3.0 * V(j, i + 1) +
+          3.0 * V(j - 1, i) - 3.0 * V(j - 1, i + 1);
+      D = -10.0 - dx2 * c.bcx[p2(m, j, i)] / Ncx;
+      b = -c.taudx[p2(m, j, i)] * dx2 / Ncx;
+    }
+    return;
+  }
+  if (rt == RT_ZERO || rt == RT_BND_ZERO) { S = 0.0; D = 1.0; b = 0.0; return; }
+  if (rt == RT_BND_INF || rt == RT_BND_PER) {
+    int io_, jo_;
+    bnd_src(rt, i, j, nx - 2, ny - 1, jo_, io_);
+    S = -U(jo_, io_); D = 1.0; b = 0.0;
+    return;
+  }
+  // calving-front stress row, :2192-2266
+  const int i1 = (rt == RT_MARGIN_I) ? i : i + 1;
+  const double Na = c.Na[p2(m, j, i1)] / m.dx;
+  const double cl = -4.0 * Na, cr = 4.0 * Na;
+  const double sv = -2.0 * Na * V(j - 1, i1) + 2.0 * Na * V(j, i1);
+  if (i1 == i) { S = cl * U(j, i - 1) + sv; D = cr; }
+  else { S = cr * U(j, i + 1) + sv; D = cl; }
+  b = c.frhs[p2(m, j, i1)];
+}
+
+template <class AU, class AV>
+__device__ __forceinline__ void v_row(const Member &m, const KP &kp, const CoefP &c, int j, int i, uint8_t rt,
+                                      const AU &U, const AV &V, double &S, double &D, double &b) {
+  const int nx = m.nx, ny = m.ny;
+  if (rt == RT_FREE) {
+    const double dx2 = m.dx * m.dx;
+    if (kp.cross) {
+      const double Na0 = c.Na[p2(m, j, i)], Na1 = c.Na[p2(m, j + 1, i)];
+      const double Nb0 = c.Nb[p2(m, j, i)], Nbm = c.Nb[p2(m, j, i - 1)];
+      S = 4.0 * Na1 * V(j + 1, i) + 4.0 * Na0 * V(j - 1, i) + Nb0 * V(j, i + 1) + Nbm * V(j, i - 1) +
+          (-2.0 * Na1 - Nbm) * U(j + 1, i - 1) + (2.0 * Na1 + Nb0) * U(j + 1, i) + (2.0 * Na0 + Nbm) * U(j, i - 1) +
+          (-2.0 * Na0 - Nb0) * U(j, i);
+      D = -4.0 * (Na1 + Na0) + -1.0 * (Nb0 + Nbm) - c.bcy[p2(m, j, i)] * dx2;
+      b = -c.taudy[p2(m, j, i)] * dx2;
+    } else {
+      const double Ncy = c.Ncy[p2(m, j, i)];
+      S = 4.0 * V(j + 1, i) + 4.0 * V(j - 1, i) + V(j, i + 1) + V(j, i - 1) - 3.0 * U(j + 1, i - 1) + 3.0 * U(j + 1, i) +
+          3.0 * U(j, i - 1) - 3.0 * U(j, i);
+      D = -10.0 - dx2 * c.bcy[p2(m, j, i)] / Ncy;
+      b = -c.taudy[p2(m, j, i)] * dx2 / Ncy;
+    }
+    return;
+  }
+  if (rt == RT_ZERO || rt == RT_BND_ZERO) { S = 0.0; D = 1.0; b = 0.0; return; }
+  if (rt == RT_BND_INF || rt == RT_BND_PER) {
+    int io_, jo_;
+    bnd_src(rt, i, j, nx - 1, ny - 2, jo_, io_);
+    S = -V(jo_, io_); D = 1.0; b = 0.0;
+    return;
+  }
+  const int j1 = (rt == RT_MARGIN_I) ? j : j + 1;
+  const double Na = c.Na[p2(m, j1, i)] / m.dx;
+  const double cl = -4.0 * Na, cr = 4.0 * Na;
+  const double su = -2.0 * Na * U(j1, i - 1) + 2.0 * Na * U(j1, i);
+  if (j1 == j) { S = cl * V(j - 1, i) + su; D = cr; }
+  else { S = cr * V(j + 1, i) + su; D = cl; }
+  b = c.frhs[p2(m, j1, i)];
+}
+
+struct GAcc {  // global-memory accessor
+  const double *x;
+  Member m;
+  __device__ __forceinline__ double operator()(int jj, int ii) const { return x[p2(m, jj, ii)]; }
+};
+
+__device__ __forceinline__ bool is_bnd(uint8_t rt) { return rt == RT_BND_INF || rt == RT_BND_ZERO || rt == RT_BND_PER; }
+
+// ---------------------------------------------------------------------------
+// One SOR sweep over every tile: four colour phases (u red, u black, v red,
+// v black) with the tile and a one-cell halo of u and v staged in LDS, then
+// the domain-boundary rows.  Tile halos come from the previous iterate (xin),
+// the result goes to xout, so the sweep is deterministic (Gauss-Seidel inside
+// a tile, Jacobi across tile edges).  With track != 0 the per-member maximum
+// |(S + D x - b)/D| (the reference's res_max, src/utilities_module.f90:671-695)
+// is folded into resmax[g] with an atomic max on its bit pattern.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(IMAU_WG) k_sweep(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                   const KP kp, const CoefP c, const double *__restrict__ xu_in,
+                                                   const double *__restrict__ xv_in, double *__restrict__ xu_out,
+                                                   double *__restrict__ xv_out, const int *__restrict__ active,
+                                                   double omega, int track, unsigned long long *__restrict__ resmax) {
+  __shared__ double red[IMAU_WG_I];
+#define su g_sx[0]
+#define sv g_sx[1]
+  const Tile t = tiles[blockIdx.x];
+  const Member m = mem[t.g];
+  const int nx = m.nx, ny = m.ny;
+  const int tid = threadIdx.y * IMAU_TJ + threadIdx.x;
+  if (!active[t.g]) {  // converged member: carry the iterate through
+    for (int q = tid; q < IMAU_TI * IMAU_TJ; q += IMAU_WG) {
+      const int i = t.i0 + q / IMAU_TJ, j = t.j0 + q % IMAU_TJ;
+      if (i < nx && j < ny) {
+        const long long p = p2(m, j, i);
+        xu_out[p] = xu_in[p];
+        xv_out[p] = xv_in[p];
+      }
+    }
+    return;
+  }
+  // stage tile + halo
+  for (int q = tid; q < (IMAU_TI + 2) * (IMAU_TJ + 2); q += IMAU_WG) {
+    const int li = q / (IMAU_TJ + 2), lj = q % (IMAU_TJ + 2);
+    const int i = t.i0 + li - 1, j = t.j0 + lj - 1;
+    double a = 0.0, b = 0.0;
+    if (i >= 0 && i < nx && j >= 0 && j < ny) {
+      const long long p = p2(m, j, i);
+      a = xu_in[p];
+      b = xv_in[p];
+    }
+    su[li][lj] = a;
+    sv[li][lj] = b;
+  }
+  __syncthreads();
+  const LAcc<0> U{t.i0, t.j0};
+  const LAcc<1> V{t.i0, t.j0};
+  const int j = t.j0 + (int)threadIdx.x;
+  const int lj = threadIdx.x + 1;
+  double rmax = 0.0;
+  // colour phases: 0 u-red, 1 u-black, 2 v-red, 3 v-black, 4 boundary rows
+  for (int ph = 0; ph < 5; ++ph) {
+    FOR_TILE_ROWS {
+      const int i = t.i0 + ii;
+      if (i >= nx || j >= ny) continue;
+      const long long p = p2(m, j, i);
+      const int li = ii + 1;
+      if (ph < 4) {
+        const bool isu = ph < 2;
+        if (((i + j) & 1) != (ph & 1)) continue;
+        if (isu && i >= nx - 1) continue;
+        if (!isu && j >= ny - 1) continue;
+        const uint8_t rt = isu ? c.rt_u[p] : c.rt_v[p];
+        if (is_bnd(rt)) continue;
+        double S, D, b;
+        if (isu) {
+          u_row(m, kp, c, j, i, rt, U, V, S, D, b);
+          const double x = su[li][lj];
+          const double res = (S + D * x - b) / D;
+          rmax = fmax(rmax, fabs(res));
+          su[li][lj] = (rt == RT_FREE) ? x - omega * res : x - res;
+        } else {
+          v_row(m, kp, c, j, i, rt, U, V, S, D, b);
+          const double x = sv[li][lj];
+          const double res = (S + D * x - b) / D;
+          rmax = fmax(rmax, fabs(res));
+          sv[li][lj] = (rt == RT_FREE) ? x - omega * res : x - res;
+        }
+      } else {
+        // domain-boundary rows: x = x(source) ('infinite': inner neighbour,
+        // staged in LDS; 'periodic': opposite side, previous iterate), or 0
+        if (i < nx - 1) {
+          const uint8_t rt = c.rt_u[p];
+          if (is_bnd(rt)) {
+            double val = 0.0;
+            if (rt != RT_BND_ZERO) {
+              int jo, io;
+              bnd_src(rt, i, j, nx - 2, ny - 1, jo, io);
+              val = (rt == RT_BND_INF) ? su[io - t.i0 + 1][jo - t.j0 + 1] : xu_in[p2(m, jo, io)];
+            }
+            rmax = fmax(rmax, fabs(su[li][lj] - val));
+            su[li][lj] = val;
+          }
+        }
+        if (j < ny - 1) {
+          const uint8_t rt = c.rt_v[p];
+          if (is_bnd(rt)) {
+            double val = 0.0;
+            if (rt != RT_BND_ZERO) {
+              int jo, io;
+              bnd_src(rt, i, j, nx - 1, ny - 2, jo, io);
+              val = (rt == RT_BND_INF) ? sv[io - t.i0 + 1][jo - t.j0 + 1] : xv_in[p2(m, jo, io)];
+            }
+            rmax = fmax(rmax, fabs(sv[li][lj] - val));
+            sv[li][lj] = val;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // write back the tile interior
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= nx || j >= ny) continue;
+    const long long p = p2(m, j, i);
+    xu_out[p] = su[ii + 1][lj];
+    xv_out[p] = sv[ii + 1][lj];
+  }
+  if (track) {
+    const double w = wave_max(rmax);
+    if (threadIdx.x == 0) red[threadIdx.y] = w;
+    __syncthreads();
+    if (tid == 0) {
+      double r = red[0];
+      for (int q = 1; q < IMAU_WG_I; ++q) r = fmax(r, red[q]);
+      // the slot is global memory: address it as such so that no generic-pointer
+      // LDS test is generated around the 64-bit atomic
+      typedef __attribute__((address_space(1))) unsigned long long gu64;
+      __hip_atomic_fetch_max((gu64 *)(resmax + t.g), (unsigned long long)__double_as_longlong(r), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+#undef su
+#undef sv
+
+// ---------------------------------------------------------------------------
+// Picard step 3 (after the linear solve): velocity limits (:1432-1498),
+// relaxation (:1499-1529), Picard residual partial sums (:1530-1601),
+// basal stress (:1087-1120) and basal velocity (:1121-1199).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(IMAU_WG) k_post(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                  const KP kp, const Fields f, const double *__restrict__ xu,
+                                                  const double *__restrict__ xv, const int *__restrict__ active,
+                                                  int diva, double *__restrict__ partial) {
+  TILE_SETUP
+  double acc[3] = {0.0, 0.0, 0.0};
+  const int nx = m.nx, ny = m.ny;
+  const double rel = kp.relax, vmax = kp.vel_max;
+  if (j < ny && active[t.g]) {
+    FOR_TILE_ROWS {
+      const int i = t.i0 + ii;
+      if (i >= nx) break;
+      const long long p = p2(m, j, i);
+      if (i < nx - 1) {
+        double x = xu[p];
+        // map_cy_to_cx of the solved v, :1757-1797
+        double vcx;
+        if (j == 0) vcx = (xv[p2(m, 0, i)] + xv[p2(m, 0, i + 1)]) / 2.0;
+        else if (j == ny - 1) vcx = (xv[p2(m, ny - 2, i)] + xv[p2(m, ny - 2, i + 1)]) / 2.0;
+        else vcx = (xv[p2(m, j - 1, i)] + xv[p2(m, j - 1, i + 1)] + xv[p2(m, j, i)] + xv[p2(m, j, i + 1)]) / 4.0;
+        const double ua = sqrt(x * x + vcx * vcx);
+        if (ua > vmax) x = x * vmax / ua;
+        const double uo = f.u[p];
+        const double un = rel * x + (1.0 - rel) * uo;
+        f.u[p] = un;
+        f.u_prev[p] = uo;
+        if (fabs(un) > 1e-6) {
+          acc[0] += (un - uo) * (un - uo);
+          acc[1] += (un + uo) * (un + uo);
+          acc[2] += 1.0;
+        }
+        if (diva) {
+          const double tb = f.beta_eff_cx[p] * un;
+          f.taub_cx[p] = tb;
+          double ub;
+          if (kp.slid == 0) ub = 0.0;
+          else {
+            const double F2s = stag_margin(f.F2[p], f.F2[p2(m, j, i + 1)], f.Hi[p], f.Hi[p2(m, j, i + 1)]);
+            ub = un - tb * F2s;
+            if (kp.freeze) {
+              const long long qb = p3(m, kp.nz - 1, j, i);
+              if (f.Tpmp3[qb] - f.Ti3[qb] > kp.dT_freeze) ub = 0.0;
+            }
+          }
+          f.u_base[p] = ub;
+        }
+      }
+      if (j < ny - 1) {
+        double x = xv[p];
+        // map_cx_to_cy of the solved u, :1718-1756
+        double ucy;
+        if (i == 0) ucy = (xu[p2(m, j, 0)] + xu[p2(m, j + 1, 0)]) / 2.0;
+        else if (i == nx - 1) ucy = (xu[p2(m, j, nx - 2)] + xu[p2(m, j + 1, nx - 2)]) / 2.0;
+        else ucy = (xu[p2(m, j, i - 1)] + xu[p2(m, j, i)] + xu[p2(m, j + 1, i - 1)] + xu[p2(m, j + 1, i)]) / 4.0;
+        const double ua = sqrt(ucy * ucy + x * x);
+        if (ua > vmax) x = x * vmax / ua;
+        const double vo = f.v[p];
+        const double vn = rel * x + (1.0 - rel) * vo;
+        f.v[p] = vn;
+        f.v_prev[p] = vo;
+        if (fabs(vn) > 1e-6) {
+          acc[0] += (vn - vo) * (vn - vo);
+          acc[1] += (vn + vo) * (vn + vo);
+          acc[2] += 1.0;
+        }
+        if (diva) {
+          const double tb = f.beta_eff_cy[p] * vn;
+          f.taub_cy[p] = tb;
+          double vb;
+          if (kp.slid == 0) vb = 0.0;
+          else {
+            const double F2s = stag_margin(f.F2[p], f.F2[p2(m, j + 1, i)], f.Hi[p], f.Hi[p2(m, j + 1, i)]);
+            vb = vn - tb * F2s;
+            if (kp.freeze) {
+              const long long qb = p3(m, kp.nz - 1, j, i);
+              if (f.Tpmp3[qb] - f.Ti3[qb] > kp.dT_freeze) vb = 0.0;
+            }
+          }
+          f.v_base[p] = vb;
+        }
+      }
+    }
+  }
+  block_sum_store<3>(acc, partial + 3 * (long long)blockIdx.x);
+}
+
+// FP64 residual partial sums: sum (S + D x - b)^2 and sum b^2 over every row
+// of the system assembled at the current coefficients.
+__global__ void __launch_bounds__(IMAU_WG) k_resid(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                   const KP kp, const CoefP c, const double *__restrict__ xu,
+                                                   const double *__restrict__ xv, double *__restrict__ partial) {
+  TILE_SETUP
+  double acc[2] = {0.0, 0.0};
+  const int nx = m.nx, ny = m.ny;
+  const GAcc U{xu, m}, V{xv, m};
+  if (j < ny) {
+    FOR_TILE_ROWS {
+      const int i = t.i0 + ii;
+      if (i >= nx) break;
+      const long long p = p2(m, j, i);
+      double S, D, b;
+      if (i < nx - 1) {
+        u_row(m, kp, c, j, i, c.rt_u[p], U, V, S, D, b);
+        const double r = S + D * xu[p] - b;
+        acc[0] += r * r;
+        acc[1] += b * b;
+      }
+      if (j < ny - 1) {
+        v_row(m, kp, c, j, i, c.rt_v[p], U, V, S, D, b);
+        const double r = S + D * xv[p] - b;
+        acc[0] += r * r;
+        acc[1] += b * b;
+      }
+    }
+  }
+  block_sum_store<2>(acc, partial + 2 * (long long)blockIdx.x);
+}
+
+// Sum NV partials per member over its tile range [tstart[g], tstart[g+1])
+// in a fixed order (deterministic).  One workgroup per member.
+template <int NV>
+__global__ void __launch_bounds__(256) k_member_sum(const int *__restrict__ tstart, const double *__restrict__ partial,
+                                                    double *__restrict__ out) {
+  const int g = blockIdx.x;
+  const int a = tstart[g], b = tstart[g + 1];
+  __shared__ double red[NV][256];
+  double s[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) s[q] = 0.0;
+  for (int k = a + (int)threadIdx.x; k < b; k += 256)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) s[q] += partial[(long long)NV * k + q];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) red[q][threadIdx.x] = s[q];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int q = 0; q < NV; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) out[NV * g + q] = red[q][0];
+}
+
+__global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) p[q] = v;
+}
+
+// copy the Picard state into the linear-solve iterate (initial guess, the
+// reference's x(n) = u(j,i); zero rows start from 0)
+__global__ void __launch_bounds__(IMAU_WG) k_init_x(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                    const Fields f, double *__restrict__ xu, double *__restrict__ xv) {
+  TILE_SETUP
+  if (j >= m.ny) return;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= m.nx) break;
+    const long long p = p2(m, j, i);
+    const uint8_t ru = (i < m.nx - 1) ? f.rt_u[p] : RT_ZERO;
+    const uint8_t rv = (j < m.ny - 1) ? f.rt_v[p] : RT_ZERO;
+    xu[p] = (ru == RT_ZERO || ru == RT_BND_ZERO) ? 0.0 : f.u[p];
+    xv[p] = (rv == RT_ZERO || rv == RT_BND_ZERO) ? 0.0 : f.v[p];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// final 3D velocities and secondary fields
+// ---------------------------------------------------------------------------
+// F1 column (src/ice_velocity_module.f90:1222-1227)
+__global__ void __launch_bounds__(IMAU_WG) k_F1(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                const KP kp, const Fields f, const double *__restrict__ visc,
+                                                OutF o) {
+  TILE_SETUP
+  if (j >= m.ny) return;
+  const int nz = kp.nz;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= m.nx) break;
+    const double H = f.Hi[p2(m, j, i)];
+    double Fk1 = 0.0;
+    double gk1 = (-H / visc[p3(m, nz - 1, j, i)]) * kp.zeta[nz - 1];
+    o.F1[p3(m, nz - 1, j, i)] = 0.0;
+    for (int k = nz - 2; k >= 0; --k) {
+      const double gk = (-H / visc[p3(m, k, j, i)]) * kp.zeta[k];
+      const double Fk = Fk1 - 0.5 * (gk1 + gk) * (kp.zeta[k + 1] - kp.zeta[k]);
+      o.F1[p3(m, k, j, i)] = Fk;
+      Fk1 = Fk;
+      gk1 = gk;
+    }
+  }
+}
+
+// u_3D_cx / v_3D_cy of the DIVA (:1230-1289)
+__global__ void __launch_bounds__(IMAU_WG) k_uv3d(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                  const KP kp, const Fields f, OutF o) {
+  TILE_SETUP
+  if (j >= m.ny) return;
+  const int nz = kp.nz;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= m.nx) break;
+    const long long p = p2(m, j, i);
+    const double H0 = f.Hi[p];
+    for (int dir = 0; dir < 2; ++dir) {
+      if (dir == 0 && i >= m.nx - 1) continue;
+      if (dir == 1 && j >= m.ny - 1) continue;
+      const int j1 = dir ? j + 1 : j, i1 = dir ? i : i + 1;
+      const double H1 = f.Hi[p2(m, j1, i1)];
+      const double base = dir ? f.v_base[p] : f.u_base[p];
+      const double tb = dir ? f.taub_cy[p] : f.taub_cx[p];
+      double *out = dir ? o.v3 : o.u3;
+      for (int k = 0; k < nz; ++k) {
+        const double F0 = o.F1[p3(m, k, j, i)], Fn = o.F1[p3(m, k, j1, i1)];
+        double Fs;
+        if (kp.margin == 0) {
+          if (H0 > 0.0 && H1 == 0.0) Fs = F0;
+          else if (H0 == 0.0 && H1 > 0.0) Fs = Fn;
+          else if (H0 > 0.0 && H1 > 0.0) Fs = 0.5 * (F0 + Fn);
+          else Fs = 0.0;
+        } else {
+          Fs = 0.5 * (F0 + Fn);
+        }
+        out[p3(m, k, j, i)] = base + tb * Fs;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ double vavg_col(const double *a, const Member &m, const KP &kp, int j, int i) {
+  double s = 0.0;
+  for (int k = 0; k < kp.nz - 1; ++k)
+    s += 0.5 * (a[p3(m, k + 1, j, i)] + a[p3(m, k, j, i)]) * (kp.zeta[k + 1] - kp.zeta[k]);
+  return s;
+}
+
+// 3D columns, basal and depth-averaged velocity for the non-DIVA modes
+// (calc_secondary_velocities branches, :393-475).  mode: 1 SIA, 2 SSA, 3 SIA/SSA
+__global__ void __launch_bounds__(IMAU_WG) k_compose(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                     const KP kp, const Fields f, OutF o, int mode) {
+  TILE_SETUP
+  if (j >= m.ny) return;
+  const int nz = kp.nz;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= m.nx) break;
+    const long long p = p2(m, j, i);
+    for (int dir = 0; dir < 2; ++dir) {
+      if (dir == 0 && i >= m.nx - 1) continue;
+      if (dir == 1 && j >= m.ny - 1) continue;
+      double *u3 = dir ? o.v3 : o.u3;
+      const double *s3 = dir ? o.v3sia : o.u3sia;
+      const double us = dir ? f.v[p] : f.u[p];
+      double *ub = dir ? f.v_base : f.u_base;
+      double *uv = dir ? o.vvav : o.uvav;
+      for (int k = 0; k < nz; ++k) {
+        const long long q = p3(m, k, j, i);
+        u3[q] = (mode == 1) ? s3[q] : (mode == 2 ? us : s3[q] + us);
+      }
+      ub[p] = (mode == 1) ? 0.0 : us;
+      uv[p] = (mode == 2) ? us : vavg_col(u3, m, kp, j, i);
+    }
+  }
+}
+
+// w_3D_a by mass conservation (:534-661); u3a/v3a hold the a-grid 3D
+// velocities of the previous call, as in the reference
+__global__ void __launch_bounds__(IMAU_WG) k_w3d(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                 const KP kp, const Fields f, OutF o) {
+  TILE_SETUP
+  if (j >= m.ny) return;
+  const int nx = m.nx, ny = m.ny, nz = kp.nz;
+  const double dx = m.dx;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= nx) break;
+    const long long p = p2(m, j, i);
+    // local ddx_a_to_a / ddy_a_to_a of Hs and Hi (:563-566)
+    double dHs_dx, dHs_dy, dHi_dx, dHi_dy;
+    if (i == 0) { dHs_dx = (f.Hs[p2(m, j, 1)] - f.Hs[p]) / dx; dHi_dx = (f.Hi[p2(m, j, 1)] - f.Hi[p]) / dx; }
+    else if (i == nx - 1) { dHs_dx = (f.Hs[p] - f.Hs[p2(m, j, nx - 2)]) / dx; dHi_dx = (f.Hi[p] - f.Hi[p2(m, j, nx - 2)]) / dx; }
+    else { dHs_dx = (f.Hs[p2(m, j, i + 1)] - f.Hs[p2(m, j, i - 1)]) / (2 * dx); dHi_dx = (f.Hi[p2(m, j, i + 1)] - f.Hi[p2(m, j, i - 1)]) / (2 * dx); }
+    if (j == 0) { dHs_dy = (f.Hs[p2(m, 1, i)] - f.Hs[p]) / dx; dHi_dy = (f.Hi[p2(m, 1, i)] - f.Hi[p]) / dx; }
+    else if (j == ny - 1) { dHs_dy = (f.Hs[p] - f.Hs[p2(m, ny - 2, i)]) / dx; dHi_dy = (f.Hi[p] - f.Hi[p2(m, ny - 2, i)]) / dx; }
+    else { dHs_dy = (f.Hs[p2(m, j + 1, i)] - f.Hs[p2(m, j - 1, i)]) / (2 * dx); dHi_dy = (f.Hi[p2(m, j + 1, i)] - f.Hi[p2(m, j - 1, i)]) / (2 * dx); }
+    const double dHbase_dx = dHs_dx - dHi_dx, dHbase_dy = dHs_dy - dHi_dy;
+    const int iL = (i == 0) ? 0 : (i == nx - 1 ? nx - 3 : i - 1);
+    const int iR = (i == 0) ? 1 : (i == nx - 1 ? nx - 2 : i);
+    const int jL = (j == 0) ? 0 : (j == ny - 1 ? ny - 3 : j - 1);
+    const int jR = (j == 0) ? 1 : (j == ny - 1 ? ny - 2 : j);
+    const int kb = nz - 1;
+    double ub, vb;
+    if (i == 0) ub = o.u3[p3(m, kb, j, 0)];
+    else if (i == nx - 1) ub = o.u3[p3(m, kb, j, nx - 2)];
+    else ub = 0.5 * (o.u3[p3(m, kb, j, i - 1)] + o.u3[p3(m, kb, j, i)]);
+    if (j == 0) vb = o.v3[p3(m, kb, 0, i)];
+    else if (j == ny - 1) vb = o.v3[p3(m, kb, ny - 2, i)];
+    else vb = 0.5 * (o.v3[p3(m, kb, j - 1, i)] + o.v3[p3(m, kb, j, i)]);
+    double w = ub * dHbase_dx + vb * dHbase_dy;
+    if (f.m_sheet[p] == 1 && f.dHb_dt) w += f.dHb_dt[p];
+    o.w3[p3(m, kb, j, i)] = w;
+    const double Hm = fmax(0.1, f.Hi[p]);
+    const double dHsx = f.dHs_dx_a[p], dHix = f.dHi_dx_a[p], dHsy = f.dHs_dy_a[p], dHiy = f.dHi_dy_a[p];
+    for (int k = nz - 2; k >= 0; --k) {
+      const double dudx_k = (o.u3[p3(m, k, j, iR)] - o.u3[p3(m, k, j, iL)]) / dx;
+      const double dudx_k1 = (o.u3[p3(m, k + 1, j, iR)] - o.u3[p3(m, k + 1, j, iL)]) / dx;
+      const double dvdy_k = (o.v3[p3(m, k, jR, i)] - o.v3[p3(m, k, jL, i)]) / dx;
+      const double dvdy_k1 = (o.v3[p3(m, k + 1, jR, i)] - o.v3[p3(m, k + 1, jL, i)]) / dx;
+      const double w1 = (dudx_k + dudx_k1) / 2.0, w2 = (dvdy_k + dvdy_k1) / 2.0;
+      const double zm = 0.5 * (kp.zeta[k + 1] + kp.zeta[k]), dz = kp.zeta[k + 1] - kp.zeta[k];
+      const double w3 = ((dHsx - zm * dHix) / Hm) * ((o.u3a[p3(m, k + 1, j, i)] - o.u3a[p3(m, k, j, i)]) / dz);
+      const double w4 = ((dHsy - zm * dHiy) / Hm) * ((o.v3a[p3(m, k + 1, j, i)] - o.v3a[p3(m, k, j, i)]) / dz);
+      w = w - f.Hi[p] * (w1 + w2 + w3 + w4) * dz;
+      o.w3[p3(m, k, j, i)] = w;
+    }
+  }
+}
+
+// map_velocity_from_cx/cy_to_a (:2592-2811) value at a-cell (j,i) from a cx/cy field
+__device__ __forceinline__ double mapvel_x(const double *a, const Member &m, const Fields &f, int j, int i, int k, bool is3d) {
+  const int nx = m.nx;
+#define AT(jj, ii_) (is3d ? a[p3(m, k, (jj), (ii_))] : a[p2(m, (jj), (ii_))])
+  const int mi = f.m_ice[p2(m, j, i)];
+  if (i == 0) return mi == 1 ? AT(j, 0) : 0.0;
+  if (i == nx - 1) return mi == 1 ? AT(j, nx - 2) : 0.0;
+  const int a0 = f.m_ice[p2(m, j, i - 1)], a2 = f.m_ice[p2(m, j, i + 1)];
+  if (a0 == 1 && mi == 1 && a2 == 1) return 0.5 * (AT(j, i - 1) + AT(j, i));
+  if (a0 == 0 && mi == 1 && a2 == 1) return AT(j, i);
+  if (a0 == 1 && mi == 1 && a2 == 0) return AT(j, i - 1);
+  return 0.0;
+#undef AT
+}
+__device__ __forceinline__ double mapvel_y(const double *a, const Member &m, const Fields &f, int j, int i, int k, bool is3d) {
+  const int ny = m.ny;
+#define AT(jj, ii_) (is3d ? a[p3(m, k, (jj), (ii_))] : a[p2(m, (jj), (ii_))])
+  const int mi = f.m_ice[p2(m, j, i)];
+  if (j == 0) return mi == 1 ? AT(0, i) : 0.0;
+  if (j == ny - 1) return mi == 1 ? AT(ny - 2, i) : 0.0;
+  const int a0 = f.m_ice[p2(m, j - 1, i)], a2 = f.m_ice[p2(m, j + 1, i)];
+  if (a0 == 1 && mi == 1 && a2 == 1) return 0.5 * (AT(j - 1, i) + AT(j, i));
+  if (a0 == 0 && mi == 1 && a2 == 1) return AT(j, i);
+  if (a0 == 1 && mi == 1 && a2 == 0) return AT(j - 1, i);
+  return 0.0;
+#undef AT
+}
+
+// a-grid copies, |u|, shear ratio, surface and basal w (:490-533, :642-650)
+__global__ void __launch_bounds__(IMAU_WG) k_secondary(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                       const KP kp, const Fields f, OutF o, int dyn,
+                                                       const double *__restrict__ uvav, const double *__restrict__ vvav) {
+  TILE_SETUP
+  if (j >= m.ny) return;
+  const int nz = kp.nz;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= m.nx) break;
+    const long long p = p2(m, j, i);
+    for (int k = 0; k < nz; ++k) {
+      o.u3a[p3(m, k, j, i)] = mapvel_x(o.u3, m, f, j, i, k, true);
+      o.v3a[p3(m, k, j, i)] = mapvel_y(o.v3, m, f, j, i, k, true);
+    }
+    const double uva = mapvel_x(uvav, m, f, j, i, 0, false), vva = mapvel_y(vvav, m, f, j, i, 0, false);
+    const double usa = mapvel_x(o.u3, m, f, j, i, 0, true), vsa = mapvel_y(o.v3, m, f, j, i, 0, true);
+    const double uba = mapvel_x(f.u_base, m, f, j, i, 0, false), vba = mapvel_y(f.v_base, m, f, j, i, 0, false);
+    o.u_vav_a[p] = uva; o.v_vav_a[p] = vva;
+    o.u_surf_a[p] = usa; o.v_surf_a[p] = vsa;
+    o.u_base_a[p] = uba; o.v_base_a[p] = vba;
+    const double av = sqrt(uva * uva + vva * vva), as = sqrt(usa * usa + vsa * vsa), ab = sqrt(uba * uba + vba * vba);
+    o.uabs_vav[p] = av; o.uabs_surf[p] = as; o.uabs_base[p] = ab;
+    double R;
+    if (dyn == 1) R = 1.0;
+    else if (dyn == 2) R = 0.0;
+    else R = (f.m_ice[p] == 1) ? fmax(0.0, fmin(1.0, 1.0 - ab / as)) : 0.0;
+    o.R_shear[p] = R;
+    o.w_surf[p] = o.w3[p3(m, 0, j, i)];
+    o.w_base[p] = o.w3[p3(m, nz - 1, j, i)];
+  }
+}
+
+// SIA columns (solve_SIA, src/ice_velocity_module.f90:61-105)
+__global__ void __launch_bounds__(IMAU_WG) k_sia(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                 const KP kp, const Fields f, const double *__restrict__ Acx,
+                                                 const double *__restrict__ Acy, OutF o) {
+  TILE_SETUP
+  if (j >= m.ny) return;
+  const int nz = kp.nz;
+  const double n = kp.n_flow;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= m.nx) break;
+    const long long p = p2(m, j, i);
+    for (int dir = 0; dir < 2; ++dir) {
+      if (dir == 0 && i >= m.nx - 1) continue;
+      if (dir == 1 && j >= m.ny - 1) continue;
+      const int j1 = dir ? j + 1 : j, i1 = dir ? i : i + 1;
+      double *out = dir ? o.v3sia : o.u3sia;
+      if (f.m_sheet[p] == 0 && f.m_sheet[p2(m, j1, i1)] == 0) {
+        for (int k = 0; k < nz; ++k) out[p3(m, k, j, i)] = 0.0;
+        continue;
+      }
+      const double H = dir ? f.Hi_cy[p] : f.Hi_cx[p];
+      const double sx = dir ? f.dHs_dx_cy[p] : f.dHs_dx_cx[p];
+      const double sy = dir ? f.dHs_dy_cy[p] : f.dHs_dy_cx[p];
+      const double slope = dir ? sy : sx;
+      const double *A = dir ? Acy : Acx;
+      const double D0 = pow(D_ICE_DENSITY * D_GRAV * H, n) * pow(sx * sx + sy * sy, (n - 1.0) / 2.0);
+      // integral from the bed to zeta of A zeta^n (vertical_integration_from_bottom_to_zeta)
+      double Dk1 = 0.0, gk1 = A[p3(m, nz - 1, j, i)] * pow(kp.zeta[nz - 1], n);
+      out[p3(m, nz - 1, j, i)] = fmax(D0 * (2.0 * H * 0.0), -1e5) * slope;
+      for (int k = nz - 2; k >= 0; --k) {
+        const double gk = A[p3(m, k, j, i)] * pow(kp.zeta[k], n);
+        const double Dk = Dk1 - 0.5 * (gk1 + gk) * (kp.zeta[k + 1] - kp.zeta[k]);
+        out[p3(m, k, j, i)] = fmax(D0 * (2.0 * H * Dk), -1e5) * slope;
+        Dk1 = Dk;
+        gk1 = gk;
+      }
+    }
+  }
+}
+
+// layout transposes between the host (k fastest, leading dimension ldj)
+// and the device (k-plane major, leading dimension ny) for ni x nj columns
+__global__ void k_h2d_3d(const Member *__restrict__ mem, int g, const double *__restrict__ src, double *__restrict__ dst,
+                         int ni, int nj, int ldj, int nz) {
+  const Member m = mem[g];
+  const long long n = (long long)ni * nj * nz;
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x) {
+    const int k = (int)(q % nz);
+    const long long col = q / nz;
+    const int j = (int)(col % nj), i = (int)(col / nj);
+    dst[p3(m, k, j, i)] = src[((long long)i * ldj + j) * nz + k];
+  }
+}
+__global__ void k_d2h_3d(const Member *__restrict__ mem, int g, const double *__restrict__ src, double *__restrict__ dst,
+                         int ni, int nj, int ldj, int nz) {
+  const Member m = mem[g];
+  const long long n = (long long)ni * nj * nz;
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x) {
+    const int k = (int)(q % nz);
+    const long long col = q / nz;
+    const int j = (int)(col % nj), i = (int)(col / nj);
+    dst[((long long)i * ldj + j) * nz + k] = src[p3(m, k, j, i)];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+namespace imau {
+static inline dim3 wg() { return dim3(IMAU_TJ, IMAU_WG_I, 1); }
+static inline CoefP to_p(const Coef &c) {
+  CoefP p;
+  p.Na = c.Na; p.Nb = c.Nb; p.Ncx = c.Ncx; p.Ncy = c.Ncy; p.bcx = c.bcx; p.bcy = c.bcy;
+  p.taudx = c.taudx; p.taudy = c.taudy; p.frhs = c.frhs; p.rt_u = c.rt_u; p.rt_v = c.rt_v;
+  return p;
+}
+void setup1(const Launch &L, const KP &kp, const Fields &f, const double *yg) {
+  hipLaunchKernelGGL(k_setup1, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, yg);
+}
+void setup2(const Launch &L, const KP &kp, const Fields &f) {
+  hipLaunchKernelGGL(k_setup2, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f);
+}
+void visc(const Launch &L, const KP &kp, const Fields &f, bool diva, const double *visc_in, double *visc_out,
+          const double *xg, const double *yg, const int *active) {
+  if (diva)
+    hipLaunchKernelGGL(k_visc<true>, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, visc_in, visc_out, xg, yg, active);
+  else
+    hipLaunchKernelGGL(k_visc<false>, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, visc_in, visc_out, xg, yg, active);
+}
+void coef(const Launch &L, const KP &kp, const Fields &f, bool diva, const int *active) {
+  hipLaunchKernelGGL(k_coef, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, diva ? 1 : 0, active);
+}
+void init_x(const Launch &L, const Fields &f, double *xu, double *xv) {
+  hipLaunchKernelGGL(k_init_x, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, f, xu, xv);
+}
+void sweep(const Launch &L, const KP &kp, const Coef &c, const double *xu_in, const double *xv_in, double *xu_out,
+           double *xv_out, const int *active, double omega, bool track, unsigned long long *resmax) {
+  hipLaunchKernelGGL(k_sweep, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, to_p(c), xu_in, xv_in, xu_out, xv_out,
+                     active, omega, track ? 1 : 0, resmax);
+}
+void post(const Launch &L, const KP &kp, const Fields &f, const double *xu, const double *xv, const int *active,
+          bool diva, double *partial) {
+  hipLaunchKernelGGL(k_post, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, xu, xv, active, diva ? 1 : 0, partial);
+}
+void resid(const Launch &L, const KP &kp, const Coef &c, const double *xu, const double *xv, double *partial) {
+  hipLaunchKernelGGL(k_resid, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, to_p(c), xu, xv, partial);
+}
+void member_sum3(hipStream_t s, int ngrids, const int *tstart, const double *partial, double *out) {
+  hipLaunchKernelGGL(k_member_sum<3>, dim3(ngrids), dim3(256), 0, s, tstart, partial, out);
+}
+void member_sum2(hipStream_t s, int ngrids, const int *tstart, const double *partial, double *out) {
+  hipLaunchKernelGGL(k_member_sum<2>, dim3(ngrids), dim3(256), 0, s, tstart, partial, out);
+}
+void fill_u64(hipStream_t s, unsigned long long *p, int n, unsigned long long v) {
+  hipLaunchKernelGGL(k_fill_u64, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+}
+void F1(const Launch &L, const KP &kp, const Fields &f, const double *visc, const OutF &o) {
+  hipLaunchKernelGGL(k_F1, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, visc, o);
+}
+void uv3d(const Launch &L, const KP &kp, const Fields &f, const OutF &o) {
+  hipLaunchKernelGGL(k_uv3d, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, o);
+}
+void compose(const Launch &L, const KP &kp, const Fields &f, const OutF &o, int mode) {
+  hipLaunchKernelGGL(k_compose, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, o, mode);
+}
+void w3d(const Launch &L, const KP &kp, const Fields &f, const OutF &o) {
+  hipLaunchKernelGGL(k_w3d, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, o);
+}
+void secondary(const Launch &L, const KP &kp, const Fields &f, const OutF &o, int dyn, const double *uvav,
+               const double *vvav) {
+  hipLaunchKernelGGL(k_secondary, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, o, dyn, uvav, vvav);
+}
+void sia(const Launch &L, const KP &kp, const Fields &f, const double *Acx, const double *Acy, const OutF &o) {
+  hipLaunchKernelGGL(k_sia, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, Acx, Acy, o);
+}
+void h2d_3d(hipStream_t s, const Member *mem, int g, const double *src, double *dst, int ni, int nj, int ldj, int nz) {
+  long long n = (long long)ni * nj * nz;
+  int blocks = (int)((n + 255) / 256);
+  if (blocks > 4096) blocks = 4096;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_h2d_3d, dim3(blocks), dim3(256), 0, s, mem, g, src, dst, ni, nj, ldj, nz);
+}
+void d2h_3d(hipStream_t s, const Member *mem, int g, const double *src, double *dst, int ni, int nj, int ldj, int nz) {
+  long long n = (long long)ni * nj * nz;
+  int blocks = (int)((n + 255) / 256);
+  if (blocks > 4096) blocks = 4096;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_d2h_3d, dim3(blocks), dim3(256), 0, s, mem, g, src, dst, ni, nj, ldj, nz);
+}
+}  // namespace imau

@@ -1,0 +1,257 @@
+"""Synthetic ice-sheet geometries and the host-side input producers.
+
+`update_general_ice_model_data` restates the producer the reference runs
+before every velocity solve (src/general_ice_model_data_module.f90:30-84,
+masks :87-346 without ocean flood-fill; flotation helpers
+src/utilities_module.f90:126-158; A vertical average
+src/thermodynamics_module.f90:737-742).  The generators build the workloads
+the velocity path is measured on:
+
+  dome      29x29 at 50 km, flat bed, H = 3000 (1-(r/750 km)^(4/3))^(3/8)
+  channel   401x41 at 2 km marine channel with Gaussian side walls
+  random    n_x x n_y random geometries (Matern bed, dome + fractal noise)
+  ensemble  uneven-size members drawn from `random`
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .config import (UNIFORM_COULOMB_PHI_FRIC, UNIFORM_SCHOOF2005_ALPHA_SQ, UNIFORM_SCHOOF2005_BETA_SQ,
+                     UNIFORM_TSAI2015_ALPHA_SQ, UNIFORM_TSAI2015_BETA_SQ, UNIFORM_WEERTMAN_BETA_SQ,
+                     default_config, zeta_of)
+from .state import Grid, IceState
+
+ICE_DENSITY = 910.0        # src/parameters_module.f90:20
+SEAWATER_DENSITY = 1028.0  # src/parameters_module.f90:21
+TINY = np.finfo(np.float64).tiny
+
+
+# --------------------------------------------------------------------------
+# staggered-grid operators used by the producer (derivatives_and_grids_module)
+# --------------------------------------------------------------------------
+def _ddx_a_to_a(d, dx):
+    o = np.empty_like(d)
+    o[:, 1:-1] = (d[:, 2:] - d[:, :-2]) / (2 * dx)
+    o[:, 0] = (d[:, 1] - d[:, 0]) / dx
+    o[:, -1] = (d[:, -1] - d[:, -2]) / dx
+    return o
+
+
+def _ddy_a_to_a(d, dx):
+    o = np.empty_like(d)
+    o[1:-1, :] = (d[2:, :] - d[:-2, :]) / (2 * dx)
+    o[0, :] = (d[1, :] - d[0, :]) / dx
+    o[-1, :] = (d[-1, :] - d[-2, :]) / dx
+    return o
+
+
+def _ddx_a_to_cx(d, dx):
+    return (d[:, 1:] - d[:, :-1]) / dx
+
+
+def _ddy_a_to_cy(d, dx):
+    return (d[1:, :] - d[:-1, :]) / dx
+
+
+def _ddy_a_to_cx(d, dx):
+    ny = d.shape[0]
+    o = np.empty((ny, d.shape[1] - 1))
+    o[1:-1, :] = (d[2:, :-1] + d[2:, 1:] - d[:-2, :-1] - d[:-2, 1:]) / (4.0 * dx)
+    o[0, :] = (d[1, :-1] + d[1, 1:] - d[0, :-1] - d[0, 1:]) / (2.0 * dx)
+    o[-1, :] = (d[-1, :-1] + d[-1, 1:] - d[-2, :-1] - d[-2, 1:]) / (2.0 * dx)
+    return o
+
+
+def _ddx_a_to_cy(d, dx):
+    nx = d.shape[1]
+    o = np.empty((d.shape[0] - 1, nx))
+    o[:, 1:-1] = (d[:-1, 2:] + d[1:, 2:] - d[:-1, :-2] - d[1:, :-2]) / (4.0 * dx)
+    o[:, 0] = (d[:-1, 1] + d[1:, 1] - d[:-1, 0] - d[1:, 0]) / (2.0 * dx)
+    o[:, -1] = (d[:-1, -1] + d[1:, -1] - d[:-1, -2] - d[1:, -2]) / (2.0 * dx)
+    return o
+
+
+def _vertical_average(f3, zeta):
+    z = np.asarray(zeta)
+    s = np.zeros(f3.shape[1:])
+    for k in range(len(z) - 1):
+        s = s + 0.5 * (f3[k + 1] + f3[k]) * (z[k + 1] - z[k])
+    return s
+
+
+def update_general_ice_model_data(st: IceState, zeta) -> None:
+    """Fill Hs, TAF, masks, staggered H and A, and the surface slopes from
+    Hi_a, Hb_a, SL_a and A_flow_3D_a (src/general_ice_model_data_module.f90:30-84)."""
+    dx = st.grid.dx
+    Hi, Hb, SL = st["Hi_a"], st["Hb_a"], st["SL_a"]
+    st["Hs_a"] = Hi + np.maximum(SL - ICE_DENSITY / SEAWATER_DENSITY * Hi, Hb)
+    st["TAF_a"] = Hi - np.maximum(0.0, (SL - Hb) * (SEAWATER_DENSITY / ICE_DENSITY))
+    floating = Hi < (SL - Hb) * SEAWATER_DENSITY / ICE_DENSITY
+    st["mask_land_a"] = np.where(floating, 0, 1)
+    st["mask_ocean_a"] = np.where(floating, 1, 0)
+    ice = Hi > TINY
+    st["mask_ice_a"] = ice.astype(np.int32)
+    st["mask_sheet_a"] = (ice & ~floating).astype(np.int32)
+    st["mask_shelf_a"] = (ice & floating).astype(np.int32)
+    st["Hi_cx"] = (Hi[:, :-1] + Hi[:, 1:]) / 2.0
+    st["Hi_cy"] = (Hi[:-1, :] + Hi[1:, :]) / 2.0
+    st["Hi_b"] = (Hi[:-1, :-1] + Hi[:-1, 1:] + Hi[1:, :-1] + Hi[1:, 1:]) / 4.0
+    A3 = st["A_flow_3D_a"]
+    st["A_flow_3D_cx"] = (A3[:, :, :-1] + A3[:, :, 1:]) / 2.0
+    st["A_flow_3D_cy"] = (A3[:, :-1, :] + A3[:, 1:, :]) / 2.0
+    st["A_flow_vav_a"] = _vertical_average(A3, zeta)
+    Hs = st["Hs_a"]
+    st["dHi_dx_a"] = _ddx_a_to_a(Hi, dx)
+    st["dHi_dy_a"] = _ddy_a_to_a(Hi, dx)
+    st["dHs_dx_a"] = _ddx_a_to_a(Hs, dx)
+    st["dHs_dy_a"] = _ddy_a_to_a(Hs, dx)
+    st["dHs_dx_cx"] = _ddx_a_to_cx(Hs, dx)
+    st["dHs_dy_cx"] = _ddy_a_to_cx(Hs, dx)
+    st["dHs_dx_cy"] = _ddx_a_to_cy(Hs, dx)
+    st["dHs_dy_cy"] = _ddy_a_to_cy(Hs, dx)
+
+
+def fill_uniform_roughness(st: IceState, cfg) -> None:
+    """initialise_bed_roughness 'uniform' (src/basal_conditions_and_sliding_module.f90:336-356)."""
+    st["beta_sq_a"] = UNIFORM_WEERTMAN_BETA_SQ
+    st["phi_fric_a"] = UNIFORM_COULOMB_PHI_FRIC
+    if cfg.choice_sliding_law == 5:  # Tsai2015
+        st["alpha_sq_a"] = UNIFORM_TSAI2015_ALPHA_SQ
+        st["beta_sq_a"] = UNIFORM_TSAI2015_BETA_SQ
+    else:
+        st["alpha_sq_a"] = UNIFORM_SCHOOF2005_ALPHA_SQ
+        if cfg.choice_sliding_law == 6:
+            st["beta_sq_a"] = UNIFORM_SCHOOF2005_BETA_SQ
+
+
+def _finish(st: IceState, cfg, A) -> IceState:
+    A = np.asarray(A, dtype=np.float64)
+    if A.ndim == 0:
+        st["A_flow_3D_a"] = float(A)
+    else:
+        st["A_flow_3D_a"] = np.broadcast_to(A, st["A_flow_3D_a"].shape[1:])[None, :, :]
+    fill_uniform_roughness(st, cfg)
+    update_general_ice_model_data(st, zeta_of(cfg))
+    return st
+
+
+def synthetic_config(**overrides):
+    """Shared settings of the synthetic workloads: DIVA, uniform rheology,
+    Weertman sliding with uniform beta_sq = 1e4, no basal freezing."""
+    kw = dict(choice_ice_dynamics="DIVA", choice_sliding_law="Weertman", choice_basal_roughness="uniform",
+              choice_basal_hydrology="dry", include_basal_freezing=0)
+    kw.update(overrides)
+    return default_config(**kw)
+
+
+def dome(nx: int = 29, ny: int = 29, dx: float = 50e3, A: float = 1e-16, cfg=None):
+    """Radially symmetric dome on a flat bed at 0 m (centre cell at the origin)."""
+    cfg = cfg or synthetic_config()
+    cx, cy = (nx - 1) // 2, (ny - 1) // 2
+    g = Grid(nx, ny, dx, xmin=-cx * dx, ymin=-cy * dx)
+    st = IceState(g, cfg.nz)
+    X, Y = np.meshgrid(g.x, g.y)          # (ny,nx)
+    r = np.sqrt(X ** 2 + Y ** 2)
+    st["Hi_a"] = 3000.0 * np.maximum(0.0, 1.0 - (r / 750e3) ** (4.0 / 3.0)) ** (3.0 / 8.0)
+    st["Hb_a"] = 0.0
+    st["SL_a"] = 0.0
+    return g, cfg, _finish(st, cfg, A)
+
+
+def channel(nx: int = 401, ny: int = 41, dx: float = 2e3, seed: int = 0, A: float = 2e-17, cfg=None):
+    """Marine channel 800x80 km: bed -150 - 0.5 x[km] plus Gaussian side walls
+    500 m high with e-folding width 10 km (exp(-(d/10 km)^2)) centred on the
+    walls at y = +-40 km; thickness a seeded smooth profile P(x) from 1500 m
+    to 300 m along flow, thinned by the wall height, H = max(0, P(x) - wall(y)),
+    so that the grounded surface is level across flow; MISMIP+ style 'zero'
+    boundaries for u_west, v_west, v_south, v_north."""
+    if cfg is None:
+        cfg = synthetic_config(DIVA_boundary_BC_u_west="zero", DIVA_boundary_BC_v_west="zero",
+                               DIVA_boundary_BC_v_south="zero", DIVA_boundary_BC_v_north="zero")
+    g = Grid(nx, ny, dx, xmin=0.0, ymin=-(ny - 1) // 2 * dx)
+    st = IceState(g, cfg.nz)
+    X, Y = np.meshgrid(g.x, g.y)
+    xkm = X / 1e3
+    wall = 500.0 * (np.exp(-((Y - 40e3) / 10e3) ** 2) + np.exp(-((Y + 40e3) / 10e3) ** 2))
+    st["Hb_a"] = -150.0 - 0.5 * xkm + wall
+    rng = np.random.default_rng(seed)
+    xi = (g.x - g.x[0]) / (g.x[-1] - g.x[0])
+    amp = rng.uniform(-60.0, 60.0, size=3)
+    prof = 1500.0 + (300.0 - 1500.0) * xi
+    for k, a in enumerate(amp, start=1):
+        prof = prof + a * np.sin(k * np.pi * xi)
+    st["Hi_a"] = np.maximum(0.0, prof[None, :] - wall)
+    st["SL_a"] = 0.0
+    return g, cfg, _finish(st, cfg, A)
+
+
+def _matern_field(ny, nx, dx, sigma, ell, rng, nu=1.5):
+    ky = np.fft.fftfreq(ny, d=dx) * 2 * np.pi
+    kx = np.fft.fftfreq(nx, d=dx) * 2 * np.pi
+    K2 = ky[:, None] ** 2 + kx[None, :] ** 2
+    spec = (1.0 + K2 * ell ** 2) ** (-(nu + 1.0) / 2.0)
+    w = rng.standard_normal((ny, nx))
+    f = np.real(np.fft.ifft2(np.fft.fft2(w) * spec))
+    f = f - f.mean()
+    s = f.std()
+    return sigma * f / s if s > 0 else f
+
+
+def _fractal_field(ny, nx, rng, slope=2.0):
+    ky = np.fft.fftfreq(ny)
+    kx = np.fft.fftfreq(nx)
+    K = np.sqrt(ky[:, None] ** 2 + kx[None, :] ** 2)
+    K[0, 0] = 1.0
+    amp = K ** (-slope / 2.0 - 1.0)
+    amp[0, 0] = 0.0
+    w = rng.standard_normal((ny, nx)) + 1j * rng.standard_normal((ny, nx))
+    f = np.real(np.fft.ifft2(w * amp))
+    s = f.std()
+    return f / s if s > 0 else f
+
+
+def random_geometry(nx: int, ny: int, dx: float = 4e3, seed: int = 0, A=None, beta_sq=None,
+                    beta_sigma: float = 0.5, cfg=None):
+    """Seeded random geometry: Matern bed (sigma 400 m, correlation 200 km)
+    around 0 m, a smoothed dome covering ~70% of the domain with fractal
+    thickness noise (H >= 0), A log-uniform in [1e-17, 1e-16] unless given,
+    lognormal beta_sq field around 1e4 unless a scalar is given."""
+    cfg = cfg or synthetic_config()
+    rng = np.random.default_rng(seed)
+    g = Grid(nx, ny, dx, xmin=-(nx - 1) / 2 * dx, ymin=-(ny - 1) / 2 * dx)
+    st = IceState(g, cfg.nz)
+    X, Y = np.meshgrid(g.x, g.y)
+    st["Hb_a"] = _matern_field(ny, nx, dx, 400.0, 200e3, rng) - 100.0
+    Rx, Ry = 0.42 * (nx - 1) * dx, 0.42 * (ny - 1) * dx
+    r = np.sqrt((X / Rx) ** 2 + (Y / Ry) ** 2)
+    H0 = 2500.0 + 1000.0 * rng.random()
+    dome_h = H0 * np.maximum(0.0, 1.0 - r ** 2) ** 0.5
+    noise = 150.0 * _fractal_field(ny, nx, rng)
+    H = np.where(dome_h > 0.0, np.maximum(0.0, dome_h + noise * np.minimum(1.0, dome_h / 300.0)), 0.0)
+    st["Hi_a"] = H
+    st["SL_a"] = 0.0
+    if A is None:
+        A = 10.0 ** rng.uniform(-17.0, -16.0)
+    _finish(st, cfg, A)
+    if beta_sq is None:
+        st["beta_sq_a"] = 1.0e4 * np.exp(beta_sigma * _matern_field(ny, nx, dx, 1.0, 100e3, rng))
+    else:
+        st["beta_sq_a"] = beta_sq
+    st.meta = dict(seed=seed, A=float(np.asarray(A).mean()))
+    return g, cfg, st
+
+
+def ensemble(n_members: int = 64, nmin: int = 96, nmax: int = 512, seed: int = 0, dx: float = 4e3, cfg=None):
+    """Uneven-size ensemble: member sizes n ~ U[nmin, nmax] rounded to odd,
+    A and beta_sq scale log-uniform over one decade."""
+    cfg = cfg or synthetic_config()
+    rng = np.random.default_rng(seed)
+    members = []
+    for m in range(n_members):
+        n = int(rng.integers(nmin, nmax + 1)) | 1
+        A = 10.0 ** rng.uniform(-17.0, -16.0)
+        bscale = 10.0 ** rng.uniform(3.5, 4.5)
+        g, c, st = random_geometry(n, n, dx=dx, seed=seed * 1000 + m + 1, A=A, cfg=cfg)
+        st["beta_sq_a"] = st["beta_sq_a"] * (bscale / 1.0e4)
+        members.append((g, st))
+    return cfg, members

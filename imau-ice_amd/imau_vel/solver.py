@@ -1,0 +1,156 @@
+"""Python mirror of ice_velocity_module's call surface over libimauvel.so.
+
+`Context.solve_diva(grid, cfg, state)` is the counterpart of
+solve_DIVA(grid, ice) (src/ice_velocity_module.f90:249-374), likewise
+solve_ssa / solve_sia.  A non-zero status raises ImauError with the library's
+message, as the reference's crash() stops the run
+(src/configuration_module.f90:3566-3638).  There is no fallback: without the
+HIP library or a visible GPU every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+from . import abi
+from .abi import ImauCfg, ImauGrid, ImauIceIO
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libimauvel.so")
+
+EXPORTED = (
+    "imau_vel_init", "imau_vel_solve_sia", "imau_vel_solve_ssa", "imau_vel_solve_diva",
+    "imau_vel_nonlinear_residual", "imau_vel_finalize", "imau_vel_last_error", "imau_vel_default_config",
+    "imau_vel_problem_create", "imau_vel_problem_solve", "imau_vel_problem_reset_state",
+    "imau_vel_problem_download", "imau_vel_problem_destroy", "imau_vel_solve_diva_batch",
+)
+
+
+class ImauError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("imau_vel error %d: %s" % (code, msg))
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libimauvel.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError("libimauvel.so not built (%s); run __graft_entry__.build()" % path)
+    L = ctypes.CDLL(path)
+    P = ctypes.POINTER
+    vp = ctypes.c_void_p
+    L.imau_vel_init.argtypes = [P(ImauGrid), P(ImauCfg), P(vp)]
+    L.imau_vel_finalize.argtypes = [vp]
+    L.imau_vel_finalize.restype = None
+    L.imau_vel_last_error.argtypes = [vp]
+    L.imau_vel_last_error.restype = ctypes.c_char_p
+    L.imau_vel_default_config.argtypes = [P(ImauCfg)]
+    L.imau_vel_default_config.restype = None
+    for name in ("imau_vel_solve_sia", "imau_vel_solve_ssa", "imau_vel_solve_diva"):
+        getattr(L, name).argtypes = [vp, P(ImauGrid), P(ImauCfg), P(ImauIceIO)]
+    L.imau_vel_nonlinear_residual.argtypes = [vp, P(ImauGrid), P(ImauCfg), P(ImauIceIO), P(ctypes.c_double)]
+    L.imau_vel_problem_create.argtypes = [vp, ctypes.c_int32, P(ImauGrid), P(ImauCfg), P(ImauIceIO), P(vp)]
+    L.imau_vel_problem_solve.argtypes = [vp, vp, P(ImauCfg)]
+    L.imau_vel_problem_reset_state.argtypes = [vp, vp]
+    L.imau_vel_problem_download.argtypes = [vp, vp, P(ImauIceIO)]
+    L.imau_vel_problem_destroy.argtypes = [vp]
+    L.imau_vel_problem_destroy.restype = None
+    L.imau_vel_solve_diva_batch.argtypes = [vp, ctypes.c_int32, P(ImauGrid), P(ImauCfg), P(ImauIceIO)]
+    _lib = L
+    return L
+
+
+class Context:
+    """One velocity-solver context (device stream and workspaces)."""
+
+    def __init__(self):
+        self.L = load_library()
+        self.h = ctypes.c_void_p()
+        rc = self.L.imau_vel_init(None, None, ctypes.byref(self.h))
+        if rc != 0:
+            raise ImauError(rc, self.L.imau_vel_last_error(None).decode())
+
+    def close(self):
+        if self.h:
+            self.L.imau_vel_finalize(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise ImauError(rc, self.L.imau_vel_last_error(self.h).decode())
+
+    def _solve(self, fn, grid, cfg, state, outputs):
+        g = grid.struct()
+        io = state.io(outputs)
+        self._check(fn(self.h, ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(io)))
+        state.take_stats(io)
+        return state
+
+    def solve_diva(self, grid, cfg, state, outputs=None):
+        return self._solve(self.L.imau_vel_solve_diva, grid, cfg, state, outputs)
+
+    def solve_ssa(self, grid, cfg, state, outputs=None):
+        return self._solve(self.L.imau_vel_solve_ssa, grid, cfg, state, outputs)
+
+    def solve_sia(self, grid, cfg, state, outputs=None):
+        return self._solve(self.L.imau_vel_solve_sia, grid, cfg, state, outputs)
+
+    def nonlinear_residual(self, grid, cfg, state):
+        g = grid.struct()
+        io = state.io(set())
+        r = ctypes.c_double()
+        self._check(self.L.imau_vel_nonlinear_residual(self.h, ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(io),
+                                                       ctypes.byref(r)))
+        return r.value
+
+    def problem(self, grids, cfg, states):
+        return Problem(self, grids, cfg, states)
+
+
+class Problem:
+    """Device-resident grid or ensemble (inputs and state stay in HBM)."""
+
+    def __init__(self, ctx: Context, grids, cfg, states):
+        self.ctx, self.cfg = ctx, cfg
+        self.grids, self.states = list(grids), list(states)
+        n = len(self.grids)
+        self._g = (ImauGrid * n)(*[g.struct() for g in self.grids])
+        self._io = (ImauIceIO * n)(*[s.io() for s in self.states])
+        self.h = ctypes.c_void_p()
+        ctx._check(ctx.L.imau_vel_problem_create(ctx.h, n, self._g, ctypes.byref(cfg), self._io, ctypes.byref(self.h)))
+
+    def solve(self, cfg=None):
+        self.ctx._check(self.ctx.L.imau_vel_problem_solve(self.ctx.h, self.h, ctypes.byref(cfg or self.cfg)))
+
+    def reset_state(self):
+        self.ctx._check(self.ctx.L.imau_vel_problem_reset_state(self.ctx.h, self.h))
+
+    def download(self, outputs=None):
+        n = len(self.states)
+        io = (ImauIceIO * n)(*[s.io(outputs) for s in self.states])
+        self.ctx._check(self.ctx.L.imau_vel_problem_download(self.ctx.h, self.h, io))
+        for s, one in zip(self.states, io):
+            s.take_stats(one)
+        return self.states
+
+    def close(self):
+        if self.h:
+            self.ctx.L.imau_vel_problem_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,144 @@
+"""Host-side mirror of the velocity fields of type_ice_model.
+
+Arrays are numpy, Fortran order, with the reference's shapes
+(src/ice_dynamics_module.f90:1052-1196): a-grid (ny,nx), cx (ny,nx-1),
+cy (ny-1,nx), b (ny-1,nx-1), 3D (nz,ny,nx) etc.  `IceState.io()` builds the
+imau_ice_io struct that points at them.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import abi
+from .abi import ImauGrid, ImauIceIO
+
+# name -> (grid, is3d, dtype)
+_F64, _I32 = np.float64, np.int32
+FIELDS = {
+    # inputs
+    "Hi_a": ("a", 0, _F64), "Hb_a": ("a", 0, _F64), "SL_a": ("a", 0, _F64), "Hs_a": ("a", 0, _F64),
+    "TAF_a": ("a", 0, _F64), "dHs_dx_a": ("a", 0, _F64), "dHs_dy_a": ("a", 0, _F64),
+    "dHi_dx_a": ("a", 0, _F64), "dHi_dy_a": ("a", 0, _F64), "dHb_dt_a": ("a", 0, _F64),
+    "A_flow_vav_a": ("a", 0, _F64),
+    "mask_land_a": ("a", 0, _I32), "mask_ocean_a": ("a", 0, _I32), "mask_ice_a": ("a", 0, _I32),
+    "mask_sheet_a": ("a", 0, _I32), "mask_shelf_a": ("a", 0, _I32),
+    "Hi_cx": ("cx", 0, _F64), "dHs_dx_cx": ("cx", 0, _F64), "dHs_dy_cx": ("cx", 0, _F64),
+    "Hi_cy": ("cy", 0, _F64), "dHs_dx_cy": ("cy", 0, _F64), "dHs_dy_cy": ("cy", 0, _F64),
+    "Hi_b": ("b", 0, _F64),
+    "A_flow_3D_a": ("a", 1, _F64), "A_flow_3D_cx": ("cx", 1, _F64), "A_flow_3D_cy": ("cy", 1, _F64),
+    "Ti_a": ("a", 1, _F64), "Ti_pmp_a": ("a", 1, _F64),
+    "beta_sq_a": ("a", 0, _F64), "alpha_sq_a": ("a", 0, _F64), "phi_fric_a": ("a", 0, _F64), "tauc_a": ("a", 0, _F64),
+    # in/out state
+    "u_vav_cx": ("cx", 0, _F64), "v_vav_cy": ("cy", 0, _F64), "u_SSA_cx": ("cx", 0, _F64), "v_SSA_cy": ("cy", 0, _F64),
+    "taub_cx": ("cx", 0, _F64), "taub_cy": ("cy", 0, _F64), "visc_eff_3D_a": ("a", 1, _F64),
+    # outputs
+    "u_3D_cx": ("cx", 1, _F64), "v_3D_cy": ("cy", 1, _F64), "u_3D_SIA_cx": ("cx", 1, _F64), "v_3D_SIA_cy": ("cy", 1, _F64),
+    "u_3D_a": ("a", 1, _F64), "v_3D_a": ("a", 1, _F64), "w_3D_a": ("a", 1, _F64), "F1_3D_a": ("a", 1, _F64),
+    "du_dz_3D_cx": ("cx", 1, _F64), "dv_dz_3D_cy": ("cy", 1, _F64),
+    "u_vav_a": ("a", 0, _F64), "v_vav_a": ("a", 0, _F64), "u_surf_a": ("a", 0, _F64), "v_surf_a": ("a", 0, _F64),
+    "u_base_a": ("a", 0, _F64), "v_base_a": ("a", 0, _F64), "uabs_vav_a": ("a", 0, _F64),
+    "uabs_surf_a": ("a", 0, _F64), "uabs_base_a": ("a", 0, _F64), "w_surf_a": ("a", 0, _F64),
+    "w_base_a": ("a", 0, _F64), "R_shear": ("a", 0, _F64),
+    "u_surf_cx": ("cx", 0, _F64), "v_surf_cy": ("cy", 0, _F64), "u_base_cx": ("cx", 0, _F64), "v_base_cy": ("cy", 0, _F64),
+    "beta_a": ("a", 0, _F64), "beta_eff_a": ("a", 0, _F64), "beta_eff_cx": ("cx", 0, _F64), "beta_eff_cy": ("cy", 0, _F64),
+    "taudx_cx": ("cx", 0, _F64), "taudy_cy": ("cy", 0, _F64),
+    "N_a": ("a", 0, _F64), "N_b": ("b", 0, _F64), "N_cx": ("cx", 0, _F64), "N_cy": ("cy", 0, _F64),
+    "visc_eff_int_a": ("a", 0, _F64), "F2_a": ("a", 0, _F64),
+    "f_grnd_a": ("a", 0, _F64), "f_grnd_cx": ("cx", 0, _F64), "f_grnd_cy": ("cy", 0, _F64), "f_grnd_b": ("b", 0, _F64),
+    "overburden_pressure_a": ("a", 0, _F64), "pore_water_pressure_a": ("a", 0, _F64), "Neff_a": ("a", 0, _F64),
+    "DIVA_mask_combi": ("a", 0, _I32), "DIVA_isfront_inner": ("a", 0, _I32), "DIVA_isfront_outer": ("a", 0, _I32),
+    "u_cx_prev": ("cx", 0, _F64), "v_cy_prev": ("cy", 0, _F64),
+}
+
+STATE_FIELDS = ("u_vav_cx", "v_vav_cy", "u_SSA_cx", "v_SSA_cy", "taub_cx", "taub_cy", "visc_eff_3D_a")
+
+
+def grid_shape(kind: str, nx: int, ny: int):
+    return {"a": (ny, nx), "cx": (ny, nx - 1), "cy": (ny - 1, nx), "b": (ny - 1, nx - 1)}[kind]
+
+
+class Grid:
+    """type_grid subset: a-grid coordinates x(nx), y(ny) and spacing dx."""
+
+    def __init__(self, nx: int, ny: int, dx: float, xmin: float = 0.0, ymin: float = 0.0):
+        if nx < 4 or ny < 4:
+            raise ValueError("grid must be at least 4x4")
+        self.nx, self.ny, self.dx = int(nx), int(ny), float(dx)
+        self.x = np.ascontiguousarray(xmin + dx * np.arange(nx, dtype=np.float64))
+        self.y = np.ascontiguousarray(ymin + dx * np.arange(ny, dtype=np.float64))
+
+    def struct(self) -> ImauGrid:
+        g = ImauGrid()
+        g.nx, g.ny, g.dx = self.nx, self.ny, self.dx
+        g.x = self.x.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        g.y = self.y.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        return g
+
+
+class IceState:
+    """All velocity-path fields of one grid, zero-initialised like the reference's shared windows."""
+
+    def __init__(self, grid: Grid, nz: int):
+        self.grid, self.nz = grid, int(nz)
+        self.f = {}
+        for name, (kind, is3d, dt) in FIELDS.items():
+            shp = grid_shape(kind, grid.nx, grid.ny)
+            if is3d:
+                shp = (self.nz,) + shp
+            self.f[name] = np.zeros(shp, dtype=dt, order="F")
+        self.DIVA_SOR_nit = 10000
+        self.DIVA_SOR_tol = 2.5
+        self.DIVA_SOR_omega = 1.3
+        self.DIVA_PETSc_rtol = 0.01
+        self.DIVA_PETSc_abstol = 2.5
+        self.stats = {}
+
+    def __getitem__(self, k):
+        return self.f[k]
+
+    def __setitem__(self, k, v):
+        a = self.f[k]
+        a[...] = v
+
+    def copy(self) -> "IceState":
+        s = IceState(self.grid, self.nz)
+        for k, v in self.f.items():
+            s.f[k][...] = v
+        for k in ("DIVA_SOR_nit", "DIVA_SOR_tol", "DIVA_SOR_omega", "DIVA_PETSc_rtol", "DIVA_PETSc_abstol"):
+            setattr(s, k, getattr(self, k))
+        return s
+
+    def io(self, outputs=None) -> ImauIceIO:
+        """imau_ice_io pointing at this state's arrays.  `outputs` (a set of
+        field names) restricts which output pointers are passed; inputs and
+        in/out state are always passed."""
+        io = ImauIceIO()
+        for name in abi.IO_POINTER_FIELDS:
+            if name not in self.f:
+                continue
+            if outputs is not None and name not in outputs and not _is_input(name):
+                continue
+            arr = self.f[name]
+            ct = ctypes.c_int32 if arr.dtype == np.int32 else ctypes.c_double
+            setattr(io, name, arr.ctypes.data_as(ctypes.POINTER(ct)))
+        io.DIVA_SOR_nit = self.DIVA_SOR_nit
+        io.DIVA_SOR_tol = self.DIVA_SOR_tol
+        io.DIVA_SOR_omega = self.DIVA_SOR_omega
+        io.DIVA_PETSc_rtol = self.DIVA_PETSc_rtol
+        io.DIVA_PETSc_abstol = self.DIVA_PETSc_abstol
+        return io
+
+    def take_stats(self, io: ImauIceIO):
+        self.stats = dict(picard_iterations=io.stat_picard_iterations, linear_sweeps=io.stat_linear_sweeps,
+                          resid_UV=io.stat_resid_UV, nonlinear_residual=io.stat_nonlinear_residual,
+                          seconds=io.stat_seconds)
+        return self.stats
+
+
+_INPUTS = set(list(FIELDS)[: list(FIELDS).index("u_vav_cx")]) | set(STATE_FIELDS)
+
+
+def _is_input(name: str) -> bool:
+    return name in _INPUTS

@@ -1,0 +1,244 @@
+/*
+ * imau_vel.h -- C ABI of the MI355X ice-velocity library (libimauvel.so).
+ *
+ * Drop-in replacement for the velocity solvers of IMAU-ICE's
+ * ice_velocity_module (src/ice_velocity_module.f90).  A Fortran host keeps its
+ * own module of that name whose routines solve_SIA / solve_SSA / solve_DIVA
+ * (grid, ice) call the functions below through ISO_C_BINDING.
+ *
+ * Conventions (identical to the reference data types,
+ * src/data_types_module.f90:13-325):
+ *   - every array is the host's own storage, borrowed for the duration of the
+ *     call, Fortran column-major with 1-based (j,i) mapped to 0-based offsets:
+ *       a-grid  (ny  ,nx  )  offset = i*ny     + j
+ *       cx-grid (ny  ,nx-1)  offset = i*ny     + j      (u between a(j,i),a(j,i+1))
+ *       cy-grid (ny-1,nx  )  offset = i*(ny-1) + j      (v between a(j,i),a(j+1,i))
+ *       b-grid  (ny-1,nx-1)  offset = i*(ny-1) + j
+ *       3D      (nz,ny,nx)   offset = (i*ny + j)*nz + k (k=0 surface, k=nz-1 base)
+ *   - reals are double, masks are int32 0/1;
+ *   - a NULL output pointer means "not requested"; a NULL input pointer is an
+ *     error unless the configuration does not read that field.
+ *   - units as in the reference: m, yr, Pa, Pa^-n yr^-1 (src/data_types_module.f90:93,151,210).
+ *
+ * Error behaviour: every entry point returns IMAU_OK (0) or a negative status;
+ * imau_vel_last_error() gives the message.  The reference calls crash() in the
+ * same situations (src/configuration_module.f90:3566-3638); the Fortran shim
+ * maps a non-zero status onto crash().
+ */
+#ifndef IMAU_VEL_H
+#define IMAU_VEL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IMAU_VEL_ABI_VERSION 1
+#define IMAU_NZ_MAX 32
+
+/* status codes */
+enum {
+  IMAU_OK              =  0,
+  IMAU_ERR_CONFIG      = -1,  /* unknown choice / inconsistent configuration   */
+  IMAU_ERR_ARG         = -2,  /* NULL required pointer, bad grid size          */
+  IMAU_ERR_NAN         = -3,  /* NaN / Inf detected in a solver field          */
+  IMAU_ERR_DIVERGED    = -4,  /* linear solver diverged (reference: SOR guard) */
+  IMAU_ERR_STALLED     = -5,  /* FP64 residual stalled, fallback also failed   */
+  IMAU_ERR_DEVICE      = -6,  /* HIP runtime error                             */
+  IMAU_ERR_NO_DEVICE   = -7   /* library built without / cannot see a GPU      */
+};
+
+/* choice_ice_dynamics (src/configuration_module.f90:271) */
+enum { IMAU_DYN_NONE = 0, IMAU_DYN_SIA = 1, IMAU_DYN_SSA = 2, IMAU_DYN_SIA_SSA = 3, IMAU_DYN_DIVA = 4 };
+/* choice_ice_margin (src/configuration_module.f90:275) */
+enum { IMAU_MARGIN_BC = 0, IMAU_MARGIN_INFINITE_SLAB = 1 };
+/* choice_sliding_law (src/configuration_module.f90:349) */
+enum {
+  IMAU_SLID_NO_SLIDING = 0, IMAU_SLID_IDEALISED = 1, IMAU_SLID_COULOMB = 2,
+  IMAU_SLID_COULOMB_REGULARISED = 3, IMAU_SLID_WEERTMAN = 4, IMAU_SLID_TSAI2015 = 5,
+  IMAU_SLID_SCHOOF2005 = 6, IMAU_SLID_ZOET_IVERSON = 7
+};
+/* choice_idealised_sliding_law (src/configuration_module.f90:350) */
+enum { IMAU_IDEAL_ISMIP_HOM_C = 0, IMAU_IDEAL_ISMIP_HOM_D = 1, IMAU_IDEAL_ISMIP_HOM_F = 3 };
+/* choice_basal_hydrology (src/configuration_module.f90:361) */
+enum { IMAU_HYDRO_DRY = 0, IMAU_HYDRO_SATURATED = 1, IMAU_HYDRO_MARTIN2011 = 2 };
+/* choice_basal_roughness / choice_param_basal_roughness (src/configuration_module.f90:366,373) */
+enum { IMAU_ROUGH_UNIFORM = 0, IMAU_ROUGH_PARAMETERISED = 1, IMAU_ROUGH_PRESCRIBED = 2 };
+enum { IMAU_PROUGH_MARTIN2011 = 0, IMAU_PROUGH_SSA_ICESTREAM = 1, IMAU_PROUGH_MISMIPPLUS = 2 };
+/* DIVA_boundary_BC_* (src/configuration_module.f90:288-295) */
+enum { IMAU_BC_INFINITE = 0, IMAU_BC_ZERO = 1, IMAU_BC_PERIODIC = 2 };
+/* DIVA_choice_matrix_solver (src/configuration_module.f90:296).  The GPU
+ * replaces both reference solvers; IMAU_SOLVER_SOR keeps the SOR stop rule
+ * (max-norm of the Gauss-Seidel update <= DIVA_SOR_tol), IMAU_SOLVER_PETSC
+ * keeps the KSP stop rule (||r|| <= max(rtol*||b||, abstol)). */
+enum { IMAU_SOLVER_SOR = 0, IMAU_SOLVER_PETSC = 1 };
+/* arithmetic of the linearised solve (no reference counterpart: the
+ * reference is FP64 throughout) */
+enum { IMAU_PREC_FP64 = 0, IMAU_PREC_MIXED = 1 };
+/* Picard stop rule: reference rule (relative velocity change, ice_velocity_module.f90:346-351)
+ * or the FP64 relative nonlinear residual ||A(u)u-b(u)||/||b(u)|| <= nonlinear_rtol */
+enum { IMAU_STOP_PICARD = 0, IMAU_STOP_RESIDUAL = 1 };
+
+/* type_grid (src/data_types_module.f90:13-29) -- the fields the path reads */
+typedef struct imau_grid {
+  int32_t nx, ny;
+  double  dx;
+  const double *x;   /* (nx) a-grid x coordinates [m] */
+  const double *y;   /* (ny) a-grid y coordinates [m] */
+} imau_grid;
+
+/* The velocity-solver part of the reference's global config struct C
+ * (src/configuration_module.f90:271-302, 349-377, 168-171, 425-426, 70-85),
+ * with every CHARACTER choice mapped onto an enum. */
+typedef struct imau_cfg {
+  int32_t choice_ice_dynamics;
+  int32_t choice_ice_margin;
+  int32_t choice_sliding_law;
+  int32_t choice_idealised_sliding_law;
+  int32_t choice_basal_hydrology;
+  int32_t choice_basal_roughness;
+  int32_t choice_param_basal_roughness;
+  int32_t DIVA_boundary_BC_u_west, DIVA_boundary_BC_u_east, DIVA_boundary_BC_u_south, DIVA_boundary_BC_u_north;
+  int32_t DIVA_boundary_BC_v_west, DIVA_boundary_BC_v_east, DIVA_boundary_BC_v_south, DIVA_boundary_BC_v_north;
+  int32_t DIVA_choice_matrix_solver;
+  int32_t include_SSADIVA_crossterms;
+  int32_t do_GL_subgrid_friction;
+  int32_t include_basal_freezing;
+  int32_t nz;
+  double  zeta[IMAU_NZ_MAX];
+  double  n_flow;
+  double  DIVA_visc_it_norm_dUV_tol;
+  int32_t DIVA_visc_it_nit;
+  double  DIVA_visc_it_relax;
+  double  DIVA_epsilon_sq_0;
+  double  DIVA_visc_eff_min;
+  double  DIVA_beta_max;
+  double  DIVA_vel_max;
+  double  slid_delta_v;
+  double  slid_Weertman_m;
+  double  slid_Coulomb_reg_q_plastic;
+  double  slid_Coulomb_reg_u_threshold;
+  double  slid_ZI_ut;
+  double  slid_ZI_p;
+  double  deltaT_basal_freezing;
+  double  Martin2011_hydro_Hb_min, Martin2011_hydro_Hb_max;
+  double  Martin2011till_phi_Hb_min, Martin2011till_phi_Hb_max;
+  double  Martin2011till_phi_min, Martin2011till_phi_max;
+  double  SSA_icestream_L, SSA_icestream_m;
+  double  ISMIP_HOM_L;
+  /* GPU-only knobs (no reference counterpart) */
+  int32_t precision;       /* IMAU_PREC_*                                     */
+  int32_t stop_rule;       /* IMAU_STOP_*                                     */
+  double  nonlinear_rtol;  /* target of IMAU_STOP_RESIDUAL                    */
+  int32_t mg_levels;       /* 0 = automatic, 1 = smoother only                */
+  int32_t check_interval;  /* sweeps between linear-solver convergence checks */
+} imau_cfg;
+
+/* The fields of type_ice_model (src/data_types_module.f90:56-325) that the
+ * velocity path reads or writes. */
+typedef struct imau_ice_io {
+  /* ---- inputs: geometry and derived data (update_general_ice_model_data) ---- */
+  const double  *Hi_a, *Hb_a, *SL_a, *Hs_a, *TAF_a;       /* (ny,nx) */
+  const double  *dHs_dx_a, *dHs_dy_a, *dHi_dx_a, *dHi_dy_a, *dHb_dt_a;
+  const double  *A_flow_vav_a;                             /* (ny,nx) */
+  const int32_t *mask_land_a, *mask_ocean_a, *mask_ice_a, *mask_sheet_a, *mask_shelf_a;
+  const double  *Hi_cx, *dHs_dx_cx, *dHs_dy_cx;            /* (ny,nx-1) */
+  const double  *Hi_cy, *dHs_dx_cy, *dHs_dy_cy;            /* (ny-1,nx) */
+  const double  *Hi_b;                                     /* (ny-1,nx-1) */
+  const double  *A_flow_3D_a;                              /* (nz,ny,nx) */
+  const double  *A_flow_3D_cx, *A_flow_3D_cy;              /* SIA only */
+  const double  *Ti_a, *Ti_pmp_a;                          /* only if include_basal_freezing */
+  /* ---- bed roughness (in; updated when choice_basal_roughness=parameterised) ---- */
+  double *beta_sq_a, *alpha_sq_a, *phi_fric_a, *tauc_a;    /* (ny,nx) */
+  /* ---- in/out state (warm start and result) ---- */
+  double *u_vav_cx, *v_vav_cy;                             /* DIVA */
+  double *u_SSA_cx, *v_SSA_cy;                             /* SSA, SIA/SSA */
+  double *taub_cx, *taub_cy;
+  double *visc_eff_3D_a;                                   /* (nz,ny,nx) */
+  /* ---- outputs (NULL = not requested) ---- */
+  double *u_3D_cx, *v_3D_cy, *u_3D_SIA_cx, *v_3D_SIA_cy;   /* (nz,..) */
+  double *u_3D_a, *v_3D_a, *w_3D_a, *F1_3D_a;              /* (nz,ny,nx) */
+  double *du_dz_3D_cx, *dv_dz_3D_cy;
+  double *u_vav_a, *v_vav_a, *u_surf_a, *v_surf_a, *u_base_a, *v_base_a;
+  double *uabs_vav_a, *uabs_surf_a, *uabs_base_a, *w_surf_a, *w_base_a, *R_shear;
+  double *u_surf_cx, *v_surf_cy, *u_base_cx, *v_base_cy;
+  double *beta_a, *beta_eff_a, *beta_eff_cx, *beta_eff_cy;
+  double *taudx_cx, *taudy_cy;
+  double *N_a, *N_b, *N_cx, *N_cy, *visc_eff_int_a, *F2_a;
+  double *f_grnd_a, *f_grnd_cx, *f_grnd_cy, *f_grnd_b;
+  double *overburden_pressure_a, *pore_water_pressure_a, *Neff_a;
+  int32_t *DIVA_mask_combi, *DIVA_isfront_inner, *DIVA_isfront_outer;
+  double *u_cx_prev, *v_cy_prev;
+  /* ---- per-call solver tolerances (ice%DIVA_SOR_*, ice%DIVA_PETSc_*,
+   *      src/data_types_module.f90:256-260; set by the caller as in
+   *      src/ice_dynamics_module.f90:94-99) ---- */
+  int32_t DIVA_SOR_nit;
+  double  DIVA_SOR_tol, DIVA_SOR_omega;
+  double  DIVA_PETSc_rtol, DIVA_PETSc_abstol;
+  /* ---- statistics written by the solve (no reference counterpart) ---- */
+  int32_t stat_picard_iterations;
+  int64_t stat_linear_sweeps;
+  double  stat_resid_UV;            /* last Picard relative change           */
+  double  stat_nonlinear_residual;  /* FP64 ||A(u)u-b(u)||/||b(u)|| at return */
+  double  stat_seconds;             /* wall time of the device solve          */
+} imau_ice_io;
+
+typedef struct imau_ctx imau_ctx;
+
+/* Replaces initialise_SSADIVA_solution_matrix (src/ice_velocity_module.f90:2814-2860,
+ * called at src/ice_dynamics_module.f90:1197): creates the per-process
+ * context with device workspaces for this grid. */
+int  imau_vel_init(const imau_grid *grid, const imau_cfg *cfg, imau_ctx **ctx);
+
+/* Replaces solve_SIA (src/ice_velocity_module.f90:36-114). */
+int  imau_vel_solve_sia(imau_ctx *ctx, const imau_grid *grid, const imau_cfg *cfg, imau_ice_io *io);
+
+/* Replaces solve_SSA (src/ice_velocity_module.f90:115-248). */
+int  imau_vel_solve_ssa(imau_ctx *ctx, const imau_grid *grid, const imau_cfg *cfg, imau_ice_io *io);
+
+/* Replaces solve_DIVA (src/ice_velocity_module.f90:249-374). */
+int  imau_vel_solve_diva(imau_ctx *ctx, const imau_grid *grid, const imau_cfg *cfg, imau_ice_io *io);
+
+/* FP64 relative nonlinear residual ||A(u)u-b(u)||_2/||b(u)||_2 of the DIVA
+ * (or SSA) system at the state held in io (u, taub, visc_eff_3D_a), with A
+ * assembled as solve_DIVA_stag_linearised does (src/ice_velocity_module.f90:1656-2589).
+ * No reference counterpart: it is the stop rule of the GPU solves. */
+int  imau_vel_nonlinear_residual(imau_ctx *ctx, const imau_grid *grid, const imau_cfg *cfg,
+                                 imau_ice_io *io, double *residual);
+
+/* Replaces finalise (no reference counterpart beyond freeing the CSR windows). */
+void imau_vel_finalize(imau_ctx *ctx);
+
+/* Text of the last error on this context (or of the last failed init). */
+const char *imau_vel_last_error(const imau_ctx *ctx);
+
+/* Fills cfg with the reference defaults (src/configuration_module.f90). */
+void imau_vel_default_config(imau_cfg *cfg);
+
+/* ---- device-resident problems (no reference counterpart) ----
+ * A problem holds one or more grids (an ensemble) in device memory: create
+ * uploads inputs and state, solve runs the velocity solve on the device only,
+ * download copies the requested outputs back, reset_state restores the state
+ * that was uploaded (for repeated cold/warm-start timing).  The convenience
+ * entry points imau_vel_solve_* are create + solve + download. */
+typedef struct imau_problem imau_problem;
+int  imau_vel_problem_create(imau_ctx *ctx, int32_t ngrids, const imau_grid *grids, const imau_cfg *cfg,
+                             const imau_ice_io *ios, imau_problem **prob);
+int  imau_vel_problem_solve(imau_ctx *ctx, imau_problem *prob, const imau_cfg *cfg);
+int  imau_vel_problem_reset_state(imau_ctx *ctx, imau_problem *prob);
+int  imau_vel_problem_download(imau_ctx *ctx, imau_problem *prob, imau_ice_io *ios);
+void imau_vel_problem_destroy(imau_problem *prob);
+
+/* ---- ensembles (no reference counterpart: the reference solves one region
+ *      grid per call, src/IMAU_ICE_program.f90:247-250) ----
+ * Solves ngrids independent DIVA problems of possibly different sizes in one
+ * batched device solve.  grids[g] / ios[g] describe member g; the
+ * configuration is shared. */
+int  imau_vel_solve_diva_batch(imau_ctx *ctx, int32_t ngrids, const imau_grid *grids,
+                               const imau_cfg *cfg, imau_ice_io *ios);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IMAU_VEL_H */

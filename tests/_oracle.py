@@ -1,0 +1,118 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) -- the checker."""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+
+LIN_SOR, LIN_BANDED_LU = 0, 1
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            import subprocess
+            subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+        _lib = ctypes.CDLL(LIB)
+        _lib.orc_last_error.restype = ctypes.c_char_p
+        _lib.orc_vertical_integrate.restype = ctypes.c_double
+        _lib.orc_vertical_average.restype = ctypes.c_double
+        _lib.orc_calc_fraction_above_zero.restype = ctypes.c_double
+        _lib.orc_calc_fraction_above_zero.argtypes = [ctypes.c_double] * 4
+        _lib.orc_sliding_beta.restype = ctypes.c_double
+        _lib.orc_sliding_beta.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 10
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def solve(kind, grid, cfg, st, lin=LIN_BANDED_LU):
+    """Run the oracle's solve_DIVA / solve_SSA / solve_SIA on IceState st in place."""
+    L = lib()
+    g = grid.struct()
+    io = st.io()
+    if kind == "diva":
+        rc = L.orc_solve_diva(ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(io), lin)
+    elif kind == "ssa":
+        rc = L.orc_solve_ssa(ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(io), lin)
+    elif kind == "sia":
+        rc = L.orc_solve_sia(ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(io))
+    else:
+        raise ValueError(kind)
+    if rc != 0:
+        raise RuntimeError("oracle %s failed (%d): %s" % (kind, rc, L.orc_last_error().decode()))
+    st.take_stats(io)
+    return st
+
+
+def nonlinear_residual(grid, cfg, st):
+    L = lib()
+    g = grid.struct()
+    io = st.io()
+    r = ctypes.c_double()
+    L.orc_nonlinear_residual(ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(io), ctypes.byref(r))
+    return r.value
+
+
+def vertical_integrate(zeta, f):
+    z = np.ascontiguousarray(zeta, dtype=np.float64)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    return lib().orc_vertical_integrate(len(z), _ptr(z), _ptr(f))
+
+
+def vertical_average(zeta, f):
+    z = np.ascontiguousarray(zeta, dtype=np.float64)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    return lib().orc_vertical_average(len(z), _ptr(z), _ptr(f))
+
+
+def integrate_bottom_to_zeta(zeta, f):
+    z = np.ascontiguousarray(zeta, dtype=np.float64)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.zeros_like(z)
+    lib().orc_vertical_integration_from_bottom_to_zeta(len(z), _ptr(z), _ptr(f), _ptr(out))
+    return out
+
+
+def fraction_above_zero(nw, ne, sw, se):
+    return lib().orc_calc_fraction_above_zero(nw, ne, sw, se)
+
+
+def sliding_beta(cfg, u, v, beta_sq=1e4, alpha_sq=0.5, phi=15.0, tauc=0.0, neff=0.0, x=0.0, y=0.0, a_vav=1e-16):
+    return lib().orc_sliding_beta(ctypes.addressof(cfg), u, v, beta_sq, alpha_sq, phi, tauc, neff, x, y, a_vav)
+
+
+def assemble_csr(grid, cfg, st, u, v):
+    m = (grid.nx - 1) * grid.ny + grid.nx * (grid.ny - 1)
+    ptr = np.zeros(m + 1, np.int32)
+    idx = np.zeros(9 * m, np.int32)
+    val = np.zeros(9 * m)
+    b = np.zeros(m)
+    x = np.zeros(m)
+    g = grid.struct()
+    io = st.io()
+    u = np.asfortranarray(u, dtype=np.float64)
+    v = np.asfortranarray(v, dtype=np.float64)
+    nnz = lib().orc_assemble_csr(ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(io), _ptr(u), _ptr(v),
+                                 _ptr(ptr), _ptr(idx), _ptr(val), _ptr(b), _ptr(x))
+    return ptr, idx[:nnz], val[:nnz], b, x
+
+
+def sor(ptr, idx, val, b, x0, nit, tol, omega):
+    x = np.array(x0, dtype=np.float64)
+    it = ctypes.c_int()
+    rc = lib().orc_sor(len(b), _ptr(ptr), _ptr(idx), _ptr(val), _ptr(b), _ptr(x), nit,
+                       ctypes.c_double(tol), ctypes.c_double(omega), ctypes.byref(it))
+    return rc, x, it.value
+
+
+def banded_lu(ptr, idx, val, b):
+    x = np.zeros(len(b))
+    rc = lib().orc_banded_lu_solve(len(b), _ptr(ptr), _ptr(idx), _ptr(val), _ptr(b), _ptr(x))
+    return rc, x

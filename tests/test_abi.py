@@ -1,0 +1,55 @@
+"""The C ABI boundary without a GPU: struct layout, exported symbols,
+default configuration, and loud failure when no device is visible."""
+import ctypes
+import re
+
+import numpy as np
+import pytest
+
+import _oracle as O
+from imau_vel import abi, config, solver
+
+
+def test_python_structs_match_c_layout():
+    out = np.zeros(8, np.int64)
+    O.lib().orc_abi_layout(O._ptr(out))
+    assert out[0] == ctypes.sizeof(abi.ImauGrid)
+    assert out[1] == ctypes.sizeof(abi.ImauCfg)
+    assert out[2] == ctypes.sizeof(abi.ImauIceIO)
+    assert out[3] == abi.ImauCfg.zeta.offset
+    assert out[4] == abi.ImauCfg.check_interval.offset
+    assert out[5] == abi.ImauIceIO.u_vav_cx.offset
+    assert out[6] == abi.ImauIceIO.DIVA_SOR_nit.offset
+    assert out[7] == abi.ImauIceIO.stat_seconds.offset
+
+
+def test_library_exports_every_declared_entry_point():
+    L = solver.load_library()
+    txt = open(abi.HEADER).read()
+    declared = set(re.findall(r"\b(imau_vel_\w+)\s*\(", txt))
+    assert declared, "no entry points parsed"
+    for name in sorted(declared):
+        assert hasattr(L, name), name
+    assert declared == set(solver.EXPORTED)
+
+
+def test_c_default_config_equals_python_defaults():
+    L = solver.load_library()
+    c = abi.ImauCfg()
+    L.imau_vel_default_config(ctypes.byref(c))
+    p = config.default_config()
+    for name, typ in abi.ImauCfg._fields_:
+        a, b = getattr(c, name), getattr(p, name)
+        if name == "zeta":
+            assert list(a) == list(b)
+        else:
+            assert a == b, name
+
+
+def test_context_fails_loudly_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    with pytest.raises(solver.ImauError) as e:
+        solver.Context()
+    assert e.value.code == abi.IMAU_ERR_NO_DEVICE

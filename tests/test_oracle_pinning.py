@@ -1,0 +1,334 @@
+"""Pins the CPU oracle (the checker) to facts it does not restate itself.
+
+The reference ships no test suite or golden vectors for this path, and its
+Fortran cannot be built here (it needs MPI, NetCDF and PETSc).  Each oracle
+function is therefore checked against something independent of its own
+formula: closed forms, physical limits, brute force, a library routine
+(scipy's sparse direct solver), symmetries and invariants.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+import _oracle as O
+from imau_vel import config, geometry
+from imau_vel.state import Grid, IceState
+
+ZETA = config.DEFAULT_ZETA
+
+
+# ---------------------------------------------------------------- vertical integrals
+def test_vertical_integrals_exact_for_linear_profiles():
+    # the trapezoid rule is exact for f linear in zeta on the non-uniform grid
+    z = np.array(ZETA)
+    f = 3.0 - 2.0 * z
+    assert O.vertical_integrate(z, f) == pytest.approx(3.0 - 1.0, abs=1e-15)
+    assert O.vertical_average(z, f) == pytest.approx(2.0, abs=1e-15)
+    # bottom-to-zeta integral is negative: int_1^zeta f = -(F(1) - F(zeta))
+    out = O.integrate_bottom_to_zeta(z, f)
+    expect = -((3.0 * 1 - 1.0) - (3.0 * z - z ** 2))
+    np.testing.assert_allclose(out, expect, atol=1e-14)
+
+
+def test_vertical_integral_converges_to_closed_form():
+    # int_0^1 zeta^4 = 1/5 ; the trapezoid error decreases as dz^2
+    errs = []
+    for n in (8, 16, 32):
+        z = np.linspace(0.0, 1.0, n)
+        errs.append(abs(O.vertical_integrate(z, z ** 4) - 0.2))
+    assert errs[1] < errs[0] / 3.5 and errs[2] < errs[1] / 3.5
+
+
+# ---------------------------------------------------------------- grounded fractions
+def test_fraction_above_zero_trivial_cases():
+    assert O.fraction_above_zero(1.0, 2.0, 3.0, 4.0) == 1.0
+    assert O.fraction_above_zero(-1.0, -2.0, -3.0, -4.0) == 0.0
+
+
+@pytest.mark.parametrize("a,b", [(1.0, 0.3), (2.0, -1.0), (0.5, 1.7)])
+def test_fraction_above_zero_half_plane(a, b):
+    # f(x,y) = a*(x - x0) with corners in x only: exact fraction is the area right of x0
+    x0 = 0.37
+    f = lambda x, y: a * (x - x0) + 0.0 * y + 0.0 * b
+    phi = O.fraction_above_zero(f(0, 1), f(1, 1), f(0, 0), f(1, 0))
+    assert phi == pytest.approx(1.0 - x0, abs=2e-3)
+
+
+def _mc_fraction(nw, ne, sw, se, n=400):
+    xs = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(xs, xs)
+    f = sw * (1 - X) * (1 - Y) + se * X * (1 - Y) + nw * (1 - X) * Y + ne * X * Y
+    return float((f > 0).mean())
+
+
+def test_fraction_above_zero_brute_force():
+    rng = np.random.default_rng(7)
+    worst = 0.0
+    for _ in range(300):
+        c = rng.uniform(-1.0, 1.0, 4)
+        c = np.where(np.abs(c) < 0.05, np.sign(c) * 0.05 + 0.0, c)
+        phi = O.fraction_above_zero(*c)
+        worst = max(worst, abs(phi - _mc_fraction(*c)))
+    assert worst < 5e-3, worst
+
+
+# ---------------------------------------------------------------- sliding laws
+def _cfg(**kw):
+    return config.default_config(**kw)
+
+
+@pytest.mark.parametrize("u", [10.0, 200.0, 3000.0])
+def test_weertman_stress_velocity_relation(u):
+    # tau_b = beta |u| = beta_sq |u|^(1/m)  (power law)
+    c = _cfg(choice_sliding_law="Weertman")
+    beta = O.sliding_beta(c, u, 0.0, beta_sq=1e4)
+    assert beta * u == pytest.approx(1e4 * u ** (1.0 / 3.0), rel=1e-7)  # delta_v = 1e-3 regularisation
+
+
+def test_coulomb_laws_plastic_limits():
+    c = _cfg(choice_sliding_law="Coulomb")
+    for u in (1.0, 50.0, 900.0):
+        # |u| is regularised as sqrt(delta_v^2 + u^2), delta_v = 1e-3
+        assert O.sliding_beta(c, u, 0.0, tauc=7e4) * u == pytest.approx(7e4, rel=1e-6)
+    c = _cfg(choice_sliding_law="Coulomb_regularised")
+    # tau = tauc (|u|/u_t)^q ; at |u| = u_t the stress is tauc
+    assert O.sliding_beta(c, 100.0, 0.0, tauc=5e4) * 100.0 == pytest.approx(5e4, rel=1e-9)
+    assert O.sliding_beta(c, 0.0, 400.0, tauc=5e4) * 400.0 == pytest.approx(5e4 * 4 ** 0.3, rel=1e-9)
+
+
+def test_tsai_and_schoof_limits():
+    N, a2, b2 = 2e6, 0.5, 1e4
+    for law in ("Tsai2015", "Schoof2005"):
+        c = _cfg(choice_sliding_law=law)
+        slow = 1e-2
+        assert O.sliding_beta(c, slow, 0.0, beta_sq=b2, alpha_sq=a2, neff=N) * slow == pytest.approx(
+            b2 * slow ** (1 / 3), rel=1e-2)
+        fast = 1e9
+        assert O.sliding_beta(c, fast, 0.0, beta_sq=b2, alpha_sq=a2, neff=N) * fast == pytest.approx(a2 * N, rel=1e-2)
+
+
+def test_zoet_iverson_limit():
+    c = _cfg(choice_sliding_law="Zoet-Iverson")
+    N, phi = 1e6, 20.0
+    u = 1e9
+    assert O.sliding_beta(c, u, 0.0, phi=phi, neff=N) * u == pytest.approx(N * math.tan(math.radians(phi)), rel=1e-6)
+
+
+# ---------------------------------------------------------------- producers
+def test_numpy_producer_matches_oracle_producer():
+    import ctypes
+    g, cfg, st = geometry.channel(nx=41, ny=13, dx=20e3, seed=1)
+    L = O.lib()
+    out = {k: np.zeros_like(st[k]) for k in ("Hs_a", "TAF_a", "mask_land_a", "mask_ocean_a", "mask_ice_a",
+                                             "mask_sheet_a", "mask_shelf_a", "Hi_cx", "Hi_cy", "Hi_b",
+                                             "A_flow_3D_cx", "A_flow_3D_cy", "A_flow_vav_a", "dHi_dx_a", "dHi_dy_a",
+                                             "dHs_dx_a", "dHs_dy_a", "dHs_dx_cx", "dHs_dy_cx", "dHs_dx_cy", "dHs_dy_cy")}
+    p = O._ptr
+    L.orc_update_general_ice_model_data(ctypes.byref(g.struct()), ctypes.byref(cfg), p(st["Hi_a"]), p(st["Hb_a"]),
+                                        p(st["SL_a"]), p(st["A_flow_3D_a"]),
+                                        *[p(out[k]) for k in out])
+    for k, v in out.items():
+        np.testing.assert_array_equal(v, st[k], err_msg=k)
+
+
+def test_producer_slopes_exact_on_planar_surface():
+    g = Grid(9, 7, 1e3)
+    cfg = geometry.synthetic_config()
+    st = IceState(g, cfg.nz)
+    X, Y = np.meshgrid(g.x, g.y)
+    st["Hb_a"] = 100.0 + 0.002 * X - 0.003 * Y
+    st["Hi_a"] = 500.0
+    st["A_flow_3D_a"] = 1e-16
+    geometry.update_general_ice_model_data(st, ZETA)
+    for k, val in (("dHs_dx_a", 0.002), ("dHs_dy_a", -0.003), ("dHs_dx_cx", 0.002), ("dHs_dy_cx", -0.003),
+                   ("dHs_dx_cy", 0.002), ("dHs_dy_cy", -0.003)):
+        np.testing.assert_allclose(st[k], val, rtol=1e-9, err_msg=k)
+
+
+# ---------------------------------------------------------------- linearised system
+def _coeff_state(seed=0):
+    g, cfg, st = geometry.channel(nx=21, ny=11, dx=8e3, seed=seed)
+    c2 = geometry.synthetic_config()
+    config.set_config(c2, DIVA_visc_it_nit=3, DIVA_visc_it_norm_dUV_tol=0.0)
+    O.solve("diva", g, c2, st, O.LIN_BANDED_LU)  # fills N_a, N_b, beta_eff, taud
+    return g, cfg, st
+
+
+def test_free_rows_form_a_symmetric_block():
+    """The interior (free) rows of the staggered DIVA operator are the
+    Hessian of a convex functional: their mutual block is symmetric."""
+    g, cfg, st = _coeff_state()
+    ptr, idx, val, b, x = O.assemble_csr(g, cfg, st, st["u_vav_cx"], st["v_vav_cy"])
+    m = len(b)
+    A = sp.csr_matrix((val, idx, ptr), shape=(m, m)).toarray()
+    nrows = np.diff(ptr)
+    free = np.where(nrows == 9)[0]
+    Af = A[np.ix_(free, free)]
+    assert np.abs(Af - Af.T).max() <= 1e-12 * np.abs(Af).max()
+
+
+def test_constant_mode_of_free_rows():
+    """A uniform velocity field only feels the friction term: row sums of the
+    u (v) coefficients equal -beta_eff dx^2, and the cross terms cancel."""
+    g, cfg, st = _coeff_state(1)
+    u1 = np.ones_like(st["u_vav_cx"])
+    v0 = np.zeros_like(st["v_vav_cy"])
+    ptr, idx, val, b, x = O.assemble_csr(g, cfg, st, u1, v0)
+    m = len(b)
+    A = sp.csr_matrix((val, idx, ptr), shape=(m, m))
+    isu = np.zeros(m, bool)
+    n = 0
+    nx, ny = g.nx, g.ny
+    bcx = np.zeros(m)
+    for i in range(nx):
+        for j in range(ny):
+            if i < nx - 1:
+                isu[n] = True
+                bcx[n] = st["beta_eff_cx"][j, i]
+                n += 1
+            if j < ny - 1:
+                n += 1
+    free = np.diff(ptr) == 9
+    ones_u = isu.astype(float)
+    r = A @ ones_u
+    dx2 = g.dx ** 2
+    np.testing.assert_allclose(r[free & isu], -bcx[free & isu] * dx2, rtol=1e-9, atol=1e-3)
+    np.testing.assert_allclose(r[free & ~isu], 0.0, atol=1e-3 * np.abs(A).max())
+
+
+def test_banded_lu_and_sor_match_scipy_direct_solve():
+    g, cfg, st = _coeff_state(2)
+    ptr, idx, val, b, x0 = O.assemble_csr(g, cfg, st, st["u_vav_cx"], st["v_vav_cy"])
+    m = len(b)
+    A = sp.csr_matrix((val, idx, ptr), shape=(m, m)).tocsc()
+    D = sp.diags(1.0 / np.abs(A).max(axis=1).toarray().ravel())
+    xs = spl.spsolve((D @ A).tocsc(), D @ b)
+    rc, xl = O.banded_lu(ptr, idx, val, b)
+    assert rc == 0
+    scale = np.abs(xs).max()
+    assert np.abs(xl - xs).max() <= 1e-10 * scale
+    rc, xo, its = O.sor(ptr, idx, val, b, x0, 200000, 1e-10, 1.3)
+    assert rc == 0
+    assert np.abs(xo - xs).max() <= 1e-6 * scale
+
+
+# ---------------------------------------------------------------- solves against closed forms
+def test_diva_uniform_slab_closed_form():
+    """Uniform grounded slab with Weertman sliding.  With no membrane stresses
+    tau_b = tau_d = rho g H |s'|; the shear-only viscosity gives
+    F2 = int H/eta zeta^2 = 2 A H tau_b^2 / 5 (Lipscomb et al. 2019 Eq. 30,
+    the SIA depth-averaged deformation 2 A tau^3 H / 5); and, the reference
+    evaluating the sliding law with the depth-averaged velocity
+    (src/ice_velocity_module.f90:320), u_vav solves
+    tau = beta_sq u^(1/m - 1) (u - tau F2)."""
+    from scipy.optimize import brentq
+    nz = 32
+    zeta = list(np.linspace(0.0, 1.0, nz))
+    cfg = geometry.synthetic_config(zeta=zeta, DIVA_visc_it_nit=600, stop_rule=1, nonlinear_rtol=1e-12)
+    g = Grid(8, 8, 10e3)
+    st = IceState(g, nz)
+    H, slope, A = 2000.0, 5e-3, 1e-16
+    st["Hi_a"] = H
+    st["Hb_a"] = -slope * np.meshgrid(g.x, g.y)[0]
+    st["SL_a"] = -1e4
+    st["A_flow_3D_a"] = A
+    geometry.fill_uniform_roughness(st, cfg)
+    geometry.update_general_ice_model_data(st, zeta)
+    O.solve("diva", g, cfg, st, O.LIN_BANDED_LU)
+    assert np.abs(st["u_vav_cx"]).max() < 0.5 * cfg.DIVA_vel_max   # velocity limiter inactive
+    tau = 910.0 * 9.81 * H * slope
+    F2 = 2.0 * A * H * tau ** 2 / 5.0
+    assert F2 > 1.0 / (3.0 * 1e3)          # above the visc_eff_min floor of F2
+    np.testing.assert_allclose(st["F2_a"], F2, rtol=3e-3)
+    np.testing.assert_allclose(st["taub_cx"], tau, rtol=1e-8)
+    F2n = float(st["F2_a"][3, 3])
+    u_exp = brentq(lambda u: 1e4 * u ** (-2.0 / 3.0) * (u - tau * F2n) - tau, tau * F2n * 1.0001, 1e6)
+    np.testing.assert_allclose(st["u_vav_cx"], u_exp, rtol=1e-8)
+    np.testing.assert_allclose(st["u_base_cx"], u_exp - tau * F2n, rtol=1e-7)
+    assert np.abs(st["v_vav_cy"]).max() < 1e-9 * u_exp
+
+
+def test_ssa_schoof2006_ice_stream():
+    """SSA with the Schoof (2006) till yield stress reproduces the closed-form
+    ice-stream velocity (src/utilities_module.f90:836-877) up to the grid error."""
+    L, mexp, tant, H, A = 100e3, 0.5, 3e-4, 2000.0, 1e-17
+    ny, nx, dx = 121, 5, 5e3
+    cfg = config.default_config(choice_ice_dynamics="SSA", choice_sliding_law="Coulomb",
+                                choice_basal_roughness="parameterised", choice_param_basal_roughness="SSA_icestream",
+                                choice_basal_hydrology="dry", include_basal_freezing=0, SSA_icestream_L=L,
+                                SSA_icestream_m=mexp, DIVA_visc_it_nit=300, DIVA_visc_it_norm_dUV_tol=1e-9)
+    g = Grid(nx, ny, dx, xmin=0.0, ymin=-(ny - 1) / 2 * dx)
+    st = IceState(g, cfg.nz)
+    st["Hi_a"] = H
+    st["Hb_a"] = -tant * np.meshgrid(g.x, g.y)[0]
+    st["SL_a"] = -1e5
+    st["A_flow_3D_a"] = A
+    geometry.update_general_ice_model_data(st, config.zeta_of(cfg))
+    ua = np.array([O.lib() and _schoof(y, L, mexp, tant, H, A) for y in g.y])
+    st["u_SSA_cx"] = 0.5 * ua[:, None]
+    O.solve("ssa", g, cfg, st, O.LIN_BANDED_LU)
+    u = st["u_SSA_cx"][:, 2]
+    assert np.abs(u - ua).max() <= 0.03 * ua.max()
+
+
+def _schoof(y, L, m, tant, h0, A):
+    import ctypes
+    U, tc = ctypes.c_double(), ctypes.c_double()
+    O.lib().orc_ssa_schoof2006(ctypes.c_double(3.0), ctypes.c_double(L), ctypes.c_double(m), ctypes.c_double(tant),
+                               ctypes.c_double(h0), ctypes.c_double(A), ctypes.c_double(y), ctypes.byref(U),
+                               ctypes.byref(tc))
+    return U.value
+
+
+def test_sia_column_matches_closed_form():
+    """SIA on a planar slab: u(zeta) = -2 A (rho g H)^3 |s'|^2 s' H (1 - zeta^4)/4."""
+    nz = 32
+    zeta = list(np.linspace(0.0, 1.0, nz))
+    cfg = geometry.synthetic_config(choice_ice_dynamics="SIA", zeta=zeta)
+    g = Grid(6, 6, 10e3)
+    st = IceState(g, nz)
+    H, slope, A = 1500.0, 2e-3, 1e-16
+    st["Hi_a"] = H
+    st["Hb_a"] = -slope * np.meshgrid(g.x, g.y)[0]
+    st["SL_a"] = -1e4
+    st["A_flow_3D_a"] = A
+    geometry.update_general_ice_model_data(st, zeta)
+    O.solve("sia", g, cfg, st)
+    z = np.array(zeta)
+    expect = 2 * A * (910 * 9.81 * H) ** 3 * slope ** 3 * H * (1 - z ** 4) / 4
+    np.testing.assert_allclose(st["u_3D_SIA_cx"][:, 3, 2], expect, rtol=2e-3, atol=1e-9)
+
+
+# ---------------------------------------------------------------- invariants
+def test_no_grounded_ice_gives_zero_velocity():
+    g, cfg, st = geometry.dome()
+    st["Hi_a"] = 0.0
+    geometry.update_general_ice_model_data(st, ZETA)
+    st["u_vav_cx"] = 5.0
+    O.solve("diva", g, cfg, st)
+    assert np.all(st["u_vav_cx"] == 0.0) and np.all(st["v_vav_cy"] == 0.0)
+
+
+def test_dome_fixed_point_is_transpose_symmetric():
+    g, cfg, st = geometry.dome()
+    config.set_config(cfg, stop_rule=1, nonlinear_rtol=1e-11, DIVA_visc_it_nit=600)
+    O.solve("diva", g, cfg, st, O.LIN_BANDED_LU)
+    u, v = st["u_vav_cx"], st["v_vav_cy"]
+    assert np.abs(u - v.T).max() <= 1e-9 * np.abs(u).max()
+
+
+def test_fixed_point_independent_of_relaxation_and_linear_solver():
+    g, cfg, st = geometry.channel(nx=41, ny=11, dx=10e3, seed=5)
+    a, b = st.copy(), st.copy()
+    config.set_config(cfg, stop_rule=1, nonlinear_rtol=1e-11, DIVA_visc_it_nit=1500)
+    O.solve("diva", g, cfg, a, O.LIN_BANDED_LU)
+    c2 = geometry.synthetic_config(DIVA_boundary_BC_u_west="zero", DIVA_boundary_BC_v_west="zero",
+                                   DIVA_boundary_BC_v_south="zero", DIVA_boundary_BC_v_north="zero",
+                                   stop_rule=1, nonlinear_rtol=1e-11, DIVA_visc_it_nit=1500, DIVA_visc_it_relax=0.7)
+    b.DIVA_SOR_tol = 1e-10
+    b.DIVA_SOR_nit = 500000
+    O.solve("diva", g, c2, b, O.LIN_SOR)
+    assert np.abs(a["u_vav_cx"] - b["u_vav_cx"]).max() <= 1e-7 * np.abs(a["u_vav_cx"]).max()
