@@ -89,6 +89,26 @@ struct Fields {
   double *u_prev, *v_prev, *u_base, *v_base;
 };
 
+// one member of a coarse multigrid level: u unknowns on (nyu, nxu), v on
+// (nyv, nxv), stored in a slab of ny = max(nyu,nyv) by nx = max(nxu,nxv)
+struct CMember {
+  int nxu, nyu, nxv, nyv, nx, ny;
+  long long off;
+};
+
+// a coarse level: explicit staggered 9-point stencils.  u-row slots:
+// 0 C, 1 U(J,I-1), 2 U(J,I+1), 3 U(J-1,I), 4 U(J+1,I), 5 V(J-1,I), 6 V(J-1,I+1),
+// 7 V(J,I), 8 V(J,I+1).  v-row slots: 0 C, 1 V(J,I-1), 2 V(J,I+1), 3 V(J-1,I),
+// 4 V(J+1,I), 5 U(J,I-1), 6 U(J,I), 7 U(J+1,I-1), 8 U(J+1,I).  cu[k*na + p].
+struct CLevel {
+  const CMember *mem;
+  const Tile *tiles;
+  int ntiles;
+  long long na;
+  double *cu, *cv;
+  double *xu, *xv, *bu, *bv;
+};
+
 // buffers of the final (once per solve) phase
 struct OutF {
   double *F1, *u3, *v3, *u3sia, *v3sia, *w3, *u3a, *v3a;        // 3D

@@ -90,6 +90,7 @@ struct imau_problem {
   DBuf<double> msum;
   DBuf<unsigned long long> resmax;
   DBuf<int> act_pic, act_lin;
+  DBuf<double> sc;    // per-member Krylov scalars
 
   Fields F{};
   OutF O{};
@@ -115,6 +116,12 @@ struct imau_ctx {
   hipStream_t s = nullptr;
   std::string err;
   imau_problem *cached = nullptr;
+  // optional HIP-event timing of the sweep kernel on this context's stream
+  int ktime = 0;
+  std::vector<hipEvent_t> ev;      // pairs (start, stop)
+  size_t ev_used = 0;
+  long long sweep_launches = 0;
+  double sweep_ms = 0.0;
 };
 
 static std::string g_init_err;
@@ -137,6 +144,7 @@ enum F2 {
   F_uvav, F_vvav, F_u_vav_a, F_v_vav_a, F_u_surf_a, F_v_surf_a, F_u_base_a, F_v_base_a,
   F_uabs_vav, F_uabs_surf, F_uabs_base, F_R_shear, F_w_surf, F_w_base,
   F_u0, F_v0, F_tbx0, F_tby0,
+  F_ru, F_rv, F_rhu, F_rhv, F_pu, F_pv, F_vu, F_vv, F_phu, F_phv, F_shu, F_shv, F_tu, F_tv,
   F2_COUNT
 };
 enum I2 { I_ice, I_sheet, I_ocean, I_land, I_shelf, I_combi, I_fin, I_fout, I2_COUNT };
@@ -470,12 +478,23 @@ int imau_vel_init(const imau_grid *grid, const imau_cfg *cfg, imau_ctx **out) {
 
 void imau_vel_finalize(imau_ctx *ctx) {
   if (!ctx) return;
+  for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
   imau_vel_problem_destroy(ctx->cached);
   if (ctx->s) (void)hipStreamDestroy(ctx->s);
   delete ctx;
 }
 
 const char *imau_vel_last_error(const imau_ctx *ctx) { return ctx ? ctx->err.c_str() : g_init_err.c_str(); }
+
+int imau_vel_kernel_timing(imau_ctx *ctx, int32_t enable, int64_t *sweep_launches, double *sweep_ms) {
+  if (!ctx) return IMAU_ERR_ARG;
+  if (sweep_launches) *sweep_launches = ctx->sweep_launches;
+  if (sweep_ms) *sweep_ms = ctx->sweep_ms;
+  ctx->ktime = enable ? 1 : 0;
+  ctx->sweep_launches = 0;
+  ctx->sweep_ms = 0.0;
+  return IMAU_OK;
+}
 
 int imau_vel_problem_create(imau_ctx *ctx, int32_t ngrids, const imau_grid *grids, const imau_cfg *cfg,
                             const imau_ice_io *ios, imau_problem **prob) {
@@ -532,6 +551,7 @@ int imau_vel_problem_create(imau_ctx *ctx, int32_t ngrids, const imau_grid *grid
   P->resmax.alloc(ngrids);
   P->act_pic.alloc(ngrids);
   P->act_lin.alloc(ngrids);
+  P->sc.alloc((size_t)8 * ngrids);
   HIPCHK(hipMemcpy(P->mem_d.p, P->mem.data(), ngrids * sizeof(Member), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(P->tiles_d.p, P->tiles.data(), P->tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(P->tstart_d.p, P->tstart.data(), P->tstart.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -598,10 +618,22 @@ struct Solver {
     k.frhs = P->F.front_rhs; k.rt_u = P->F.rt_u; k.rt_v = P->F.rt_v;
     return k;
   }
+  // device copies of the activity flags are refreshed only when they change;
+  // the staging vectors must outlive the asynchronous copy, hence members
+  std::vector<int> last_pic_, last_lin_, stage_pic_, stage_lin_;
   void push_active(int *dev, const std::vector<int> &h) {
-    HIPCHK(hipMemcpyAsync(dev, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, c->s));
+    const bool pic = dev == P->act_pic.p;
+    std::vector<int> &last = pic ? last_pic_ : last_lin_;
+    std::vector<int> &stage = pic ? stage_pic_ : stage_lin_;
+    if (last == h) return;
+    HIPCHK(hipStreamSynchronize(c->s));   // the previous copy from `stage` has completed
+    stage = h;
+    last = h;
+    HIPCHK(hipMemcpyAsync(dev, stage.data(), stage.size() * sizeof(int), hipMemcpyHostToDevice, c->s));
   }
   void kcheck() { HIPCHK(hipGetLastError()); }
+
+  std::vector<double> r_abs, b_abs;  // last residual evaluation: ||r||, ||b|| per member
 
   // per-member relative FP64 residual at the current coefficients and state
   void residual(std::vector<double> &r) {
@@ -611,85 +643,164 @@ struct Solver {
     HIPCHK(hipMemcpyAsync(host4.data(), P->msum.p, 2 * P->ngrids * sizeof(double), hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     r.resize(P->ngrids);
+    r_abs.resize(P->ngrids);
+    b_abs.resize(P->ngrids);
     for (int g = 0; g < P->ngrids; ++g) {
       const double rr = host4[2 * g], bb = host4[2 * g + 1];
+      r_abs[g] = std::sqrt(rr);
+      b_abs[g] = std::sqrt(bb);
       r[g] = bb > 0.0 ? std::sqrt(rr) / std::sqrt(bb) : std::sqrt(rr);
     }
   }
 
+  void time_begin() {
+    if (!c->ktime) return;
+    if (c->ev_used + 2 > c->ev.size()) {
+      for (int q = 0; q < 256; ++q) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        c->ev.push_back(e);
+      }
+    }
+    HIPCHK(hipEventRecord(c->ev[c->ev_used], c->s));
+  }
+  void time_end() {
+    if (!c->ktime) return;
+    HIPCHK(hipEventRecord(c->ev[c->ev_used + 1], c->s));
+    c->ev_used += 2;
+  }
+  void time_collect() {
+    if (!c->ktime || !c->ev_used) return;
+    HIPCHK(hipStreamSynchronize(c->s));
+    for (size_t q = 0; q + 1 < c->ev_used; q += 2) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev[q], c->ev[q + 1]));
+      c->sweep_ms += ms;
+      c->sweep_launches += 1;
+    }
+    c->ev_used = 0;
+  }
+
   // linear solve of every Picard-active member (src/utilities_module.f90:600-718)
   void linear_solve() {
-    const bool petsc = cfg->DIVA_choice_matrix_solver == IMAU_SOLVER_PETSC;
-    // the reference's per-call SOR settings apply to the SOR mode; the KSP mode
-    // uses its tolerances with the same smoother as the iteration
-    const int nit = sor_nit;
-    const double tol = sor_tol, omega = sor_omega, rtol = petsc_rtol, atol = petsc_abstol;
-    const int K = cfg->check_interval > 0 ? cfg->check_interval : 8;
     act_lin = act_pic;
     push_active(P->act_lin.p, act_lin);
     imau::init_x(L, P->F, P->F.xu0, P->F.xv0);
-    double *xin_u = P->F.xu0, *xin_v = P->F.xv0, *xout_u = P->F.xu1, *xout_v = P->F.xv1;
+    if (cfg->DIVA_choice_matrix_solver == IMAU_SOLVER_SOR) sor_solve();
+    else krylov_solve();
+    time_collect();
+  }
+
+  // reference SOR semantics (stop when the largest Gauss-Seidel update is
+  // <= DIVA_SOR_tol, at most DIVA_SOR_nit sweeps, divergence guard), in a
+  // multicolour ordering: colours u-red, u-black, v-red, v-black, boundary rows
+  void sor_solve() {
+    const int K = cfg->check_interval > 0 ? cfg->check_interval : 8;
+    double om = sor_omega;
     int it = 0;
-    double om = omega;
-    std::vector<double> rnorm;
-    for (;;) {
-      if (it >= nit) break;
-      const bool checking = ((it + 1) % K == 0) || (it + 1 == nit);
-      if (checking && !petsc) imau::fill_u64(c->s, P->resmax.p, P->ngrids, 0ull);
-      imau::sweep(L, kp, coef(), xin_u, xin_v, xout_u, xout_v, P->act_lin.p, om, checking && !petsc, P->resmax.p);
+    std::vector<unsigned long long> rm(P->ngrids);
+    while (it < sor_nit) {
+      const bool checking = ((it + 1) % K == 0) || (it + 1 == sor_nit);
+      if (checking) imau::fill_u64(c->s, P->resmax.p, P->ngrids, 0ull);
+      time_begin();
+      for (int col = 0; col < 5; ++col)
+        imau::sor_color(L, kp, coef(), P->F.xu0, P->F.xv0, col, om, P->act_lin.p, checking, P->resmax.p);
+      time_end();
       kcheck();
-      std::swap(xin_u, xout_u);
-      std::swap(xin_v, xout_v);
       ++it;
       for (int g = 0; g < P->ngrids; ++g)
         if (act_lin[g]) P->sweeps[g] += 1;
       if (!checking) continue;
-      bool any = false;
-      if (!petsc) {
-        std::vector<unsigned long long> rm(P->ngrids);
-        HIPCHK(hipMemcpyAsync(rm.data(), P->resmax.p, P->ngrids * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->s));
-        HIPCHK(hipStreamSynchronize(c->s));
-        bool diverged = false;
-        for (int g = 0; g < P->ngrids; ++g) {
-          if (!act_lin[g]) continue;
-          double r;
-          std::memcpy(&r, &rm[g], sizeof r);
-          if (!std::isfinite(r) || (it > 100 && r > 1e3)) diverged = true;
-          if (r <= tol) act_lin[g] = 0;
-          else any = true;
-        }
-        if (diverged) {
-          // src/utilities_module.f90:697-712: lower omega, restart from zero
-          om -= 0.1;
-          if (om <= 0.1) fail(IMAU_ERR_DIVERGED, "linear solver diverged even with omega <= 0.1");
-          it = 0;
-          act_lin = act_pic;
-          HIPCHK(hipMemsetAsync(xin_u, 0, (size_t)P->na * sizeof(double), c->s));
-          HIPCHK(hipMemsetAsync(xin_v, 0, (size_t)P->na * sizeof(double), c->s));
-          any = true;
-        }
-      } else {
-        // KSP-style stop: ||A x - b|| <= max(rtol ||b||, abstol) on the true FP64 residual
-        imau::resid(L, kp, coef(), xin_u, xin_v, P->partial.p);
-        imau::member_sum2(c->s, P->ngrids, P->tstart_d.p, P->partial.p, P->msum.p);
-        kcheck();
-        HIPCHK(hipMemcpyAsync(host4.data(), P->msum.p, 2 * P->ngrids * sizeof(double), hipMemcpyDeviceToHost, c->s));
-        HIPCHK(hipStreamSynchronize(c->s));
-        for (int g = 0; g < P->ngrids; ++g) {
-          if (!act_lin[g]) continue;
-          const double rr = std::sqrt(host4[2 * g]), bb = std::sqrt(host4[2 * g + 1]);
-          if (!std::isfinite(rr)) fail(IMAU_ERR_NAN, "member %d: non-finite linear residual", g);
-          if (rr <= std::max(rtol * bb, atol)) act_lin[g] = 0;
-          else any = true;
-        }
+      HIPCHK(hipMemcpyAsync(rm.data(), P->resmax.p, P->ngrids * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->s));
+      HIPCHK(hipStreamSynchronize(c->s));
+      bool any = false, diverged = false;
+      for (int g = 0; g < P->ngrids; ++g) {
+        if (!act_lin[g]) continue;
+        double r;
+        std::memcpy(&r, &rm[g], sizeof r);
+        if (!std::isfinite(r) || (it > 100 && r > 1e3)) diverged = true;
+        if (r <= sor_tol) act_lin[g] = 0;
+        else any = true;
+      }
+      if (diverged) {
+        // src/utilities_module.f90:697-712: lower omega, restart from zero
+        om -= 0.1;
+        if (om <= 0.1) fail(IMAU_ERR_DIVERGED, "SOR diverged even with omega <= 0.1");
+        it = 0;
+        act_lin = act_pic;
+        HIPCHK(hipMemsetAsync(P->F.xu0, 0, (size_t)P->na * sizeof(double), c->s));
+        HIPCHK(hipMemsetAsync(P->F.xv0, 0, (size_t)P->na * sizeof(double), c->s));
+        any = true;
       }
       if (!any) break;
       push_active(P->act_lin.p, act_lin);
     }
-    // the final iterate is in xin_*: make it the canonical x0 buffers
-    if (xin_u != P->F.xu0) {
-      HIPCHK(hipMemcpyAsync(P->F.xu0, xin_u, (size_t)P->na * sizeof(double), hipMemcpyDeviceToDevice, c->s));
-      HIPCHK(hipMemcpyAsync(P->F.xv0, xin_v, (size_t)P->na * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  }
+
+  // KSP semantics (||A x - b|| <= max(rtol * ref, abstol), ref = ||b|| under
+  // the Picard stop rule, the initial residual under the residual stop rule):
+  // BiCGStab, right-preconditioned by tile-wise symmetric Gauss-Seidel
+  void krylov_solve() {
+    const int G = P->ngrids;
+    double *X[2] = {P->F.xu0, P->F.xv0};
+    double *R[2] = {f2p(P, F_ru), f2p(P, F_rv)}, *RH[2] = {f2p(P, F_rhu), f2p(P, F_rhv)};
+    double *Pp[2] = {f2p(P, F_pu), f2p(P, F_pv)}, *Vv[2] = {f2p(P, F_vu), f2p(P, F_vv)};
+    double *PH[2] = {f2p(P, F_phu), f2p(P, F_phv)}, *SH[2] = {f2p(P, F_shu), f2p(P, F_shv)};
+    double *T[2] = {f2p(P, F_tu), f2p(P, F_tv)};
+    const imau::Coef cf = coef();
+    const int nsw = 1;
+    std::vector<double> out(2 * (size_t)G), target(G, 0.0);
+    auto fetch = [&]() {
+      HIPCHK(hipMemcpyAsync(out.data(), P->msum.p, 2 * G * sizeof(double), hipMemcpyDeviceToHost, c->s));
+      HIPCHK(hipStreamSynchronize(c->s));
+    };
+    imau::bicg_init(L, kp, cf, X[0], X[1], R[0], R[1], RH[0], RH[1], Pp[0], Pp[1], Vv[0], Vv[1], P->act_lin.p,
+                    P->partial.p);
+    imau::bicg_scalar(c->s, G, P->tstart_d.p, P->partial.p, P->sc.p, P->msum.p, P->act_lin.p, 0);
+    kcheck();
+    fetch();
+    bool any = false;
+    for (int g = 0; g < G; ++g) {
+      if (!act_lin[g]) continue;
+      const double r0 = std::sqrt(out[2 * g]), bn = std::sqrt(out[2 * g + 1]);
+      if (!std::isfinite(r0)) fail(IMAU_ERR_NAN, "member %d: non-finite initial linear residual", g);
+      const double ref = (cfg->stop_rule == IMAU_STOP_RESIDUAL) ? r0 : bn;
+      target[g] = std::max(petsc_rtol * ref, petsc_abstol);
+      if (r0 <= target[g]) act_lin[g] = 0;
+      else any = true;
+    }
+    if (!any) return;
+    push_active(P->act_lin.p, act_lin);
+    const int maxit = sor_nit > 0 ? sor_nit : 10000;
+    for (int it = 0; it < maxit; ++it) {
+      imau::bicg_p(L, R[0], R[1], Pp[0], Pp[1], Vv[0], Vv[1], P->sc.p, P->act_lin.p);
+      time_begin();
+      imau::prec_tile(L, kp, cf, Pp[0], Pp[1], PH[0], PH[1], P->act_lin.p, nsw);
+      time_end();
+      imau::apply(L, kp, cf, PH[0], PH[1], Vv[0], Vv[1], RH[0], RH[1], P->act_lin.p, P->partial.p);
+      imau::bicg_scalar(c->s, G, P->tstart_d.p, P->partial.p, P->sc.p, P->msum.p, P->act_lin.p, 1);
+      imau::bicg_s(L, R[0], R[1], Vv[0], Vv[1], P->sc.p, P->act_lin.p);
+      time_begin();
+      imau::prec_tile(L, kp, cf, R[0], R[1], SH[0], SH[1], P->act_lin.p, nsw);
+      time_end();
+      imau::apply(L, kp, cf, SH[0], SH[1], T[0], T[1], R[0], R[1], P->act_lin.p, P->partial.p);
+      imau::bicg_scalar(c->s, G, P->tstart_d.p, P->partial.p, P->sc.p, P->msum.p, P->act_lin.p, 2);
+      imau::bicg_xr(L, X[0], X[1], R[0], R[1], PH[0], PH[1], SH[0], SH[1], T[0], T[1], RH[0], RH[1], P->sc.p,
+                    P->act_lin.p, P->partial.p);
+      imau::bicg_scalar(c->s, G, P->tstart_d.p, P->partial.p, P->sc.p, P->msum.p, P->act_lin.p, 3);
+      kcheck();
+      fetch();
+      any = false;
+      for (int g = 0; g < G; ++g) {
+        if (!act_lin[g]) continue;
+        P->sweeps[g] += 2;
+        const double rn = std::sqrt(out[2 * g + 1]);
+        if (!std::isfinite(rn)) fail(IMAU_ERR_NAN, "member %d: non-finite linear residual", g);
+        if (rn <= target[g]) act_lin[g] = 0;
+        else any = true;
+      }
+      if (!any) break;
+      push_active(P->act_lin.p, act_lin);
     }
   }
 

@@ -42,4 +42,34 @@ void sia(const Launch &L, const KP &kp, const Fields &f, const double *Acx, cons
 void h2d_3d(hipStream_t s, const Member *mem, int g, const double *src, double *dst, int ni, int nj, int ldj, int nz);
 void d2h_3d(hipStream_t s, const Member *mem, int g, const double *src, double *dst, int ni, int nj, int ldj, int nz);
 
+// Krylov solver and multicolour SOR
+void apply(const Launch &L, const KP &kp, const Coef &c, const double *xu, const double *xv, double *yu, double *yv,
+           const double *wu, const double *wv, const int *active, double *partial);
+void bicg_init(const Launch &L, const KP &kp, const Coef &c, const double *xu, const double *xv, double *ru, double *rv,
+               double *rhu, double *rhv, double *pu, double *pv, double *vu, double *vv, const int *active,
+               double *partial);
+void bicg_p(const Launch &L, const double *ru, const double *rv, double *pu, double *pv, const double *vu,
+            const double *vv, const double *sc, const int *active);
+void bicg_s(const Launch &L, double *ru, double *rv, const double *vu, const double *vv, const double *sc,
+            const int *active);
+void bicg_xr(const Launch &L, double *xu, double *xv, double *ru, double *rv, const double *phu, const double *phv,
+             const double *shu, const double *shv, const double *tu, const double *tv, const double *rhu,
+             const double *rhv, const double *sc, const int *active, double *partial);
+void bicg_scalar(hipStream_t s, int ngrids, const int *tstart, const double *partial, double *sc, double *out,
+                 const int *active, int stage);
+void prec_tile(const Launch &L, const KP &kp, const Coef &c, const double *yu, const double *yv, double *zu, double *zv,
+               const int *active, int nsweeps);
+void sor_color(const Launch &L, const KP &kp, const Coef &c, double *xu, double *xv, int color, double omega,
+               const int *active, bool track, unsigned long long *resmax);
+// aggregation multigrid
+void gal0(const Launch &F, const KP &kp, const Coef &c, const CLevel &L, const int *active);
+void galc(hipStream_t s, const CLevel &Fl, const CLevel &L, const int *active);
+void cgs(hipStream_t s, const CLevel &L, int color, const int *active);
+void crestrict(hipStream_t s, const CLevel &Fl, const CLevel &L, const int *active);
+void cprolong(hipStream_t s, const CLevel &Fl, const CLevel &L, const int *active);
+void restrict0(const Launch &F, const KP &kp, const Coef &c, const double *yu, const double *yv, const double *zu,
+               const double *zv, const CLevel &L, const int *active);
+void prolong0(const Launch &F, const Coef &c, double *zu, double *zv, const CLevel &L, const int *active);
+void coarsest(hipStream_t s, int ngrids, const CLevel &L, const int *active, int nsweeps);
+void fill(hipStream_t s, double *p, long long n, double v);
 }  // namespace imau

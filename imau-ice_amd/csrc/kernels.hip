@@ -1327,3 +1327,789 @@ void d2h_3d(hipStream_t s, const Member *mem, int g, const double *src, double *
   hipLaunchKernelGGL(k_d2h_3d, dim3(blocks), dim3(256), 0, s, mem, g, src, dst, ni, nj, ldj, nz);
 }
 }  // namespace imau
+
+// ===========================================================================
+// Krylov linear solver (BiCGStab, right-preconditioned) and global
+// multicolour SOR.  Vectors are (u-slab, v-slab) pairs in the member slabs;
+// per-member scalars live on the device (sc[g*NSC + k]) so that an
+// iteration needs no host round trip.
+// ===========================================================================
+namespace bicg {
+enum { RHO = 0, ALPHA = 1, OMEGA = 2, BETA = 3, RR = 4, NSC = 8 };
+}
+
+struct Vec2 {
+  double *u, *v;
+};
+struct CVec2 {
+  const double *u, *v;
+};
+
+// y = A x over every row; with w != null also partial sums (w, y) and (y, y)
+__global__ void __launch_bounds__(IMAU_WG) k_apply(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                   const KP kp, const CoefP c, CVec2 x, Vec2 y, CVec2 w,
+                                                   const int *__restrict__ active, double *__restrict__ partial) {
+  TILE_SETUP
+  double acc[2] = {0.0, 0.0};
+  const GAcc U{x.u, m}, V{x.v, m};
+  if (j < m.ny && active[t.g]) {
+    FOR_TILE_ROWS {
+      const int i = t.i0 + ii;
+      if (i >= m.nx) break;
+      const long long p = p2(m, j, i);
+      double S, D, b;
+      if (i < m.nx - 1) {
+        u_row(m, kp, c, j, i, c.rt_u[p], U, V, S, D, b);
+        const double yv = S + D * x.u[p];
+        y.u[p] = yv;
+        if (w.u) { acc[0] += w.u[p] * yv; acc[1] += yv * yv; }
+      }
+      if (j < m.ny - 1) {
+        v_row(m, kp, c, j, i, c.rt_v[p], U, V, S, D, b);
+        const double yv = S + D * x.v[p];
+        y.v[p] = yv;
+        if (w.u) { acc[0] += w.v[p] * yv; acc[1] += yv * yv; }
+      }
+    }
+  }
+  if (partial) block_sum_store<2>(acc, partial + 2 * (long long)blockIdx.x);
+}
+
+// r = b - A x, rhat = r, p = v = 0; partials (r, r), (b, b)
+__global__ void __launch_bounds__(IMAU_WG) k_bicg_init(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                       const KP kp, const CoefP c, CVec2 x, Vec2 r, Vec2 rh, Vec2 pp,
+                                                       Vec2 vv, const int *__restrict__ active,
+                                                       double *__restrict__ partial) {
+  TILE_SETUP
+  double acc[2] = {0.0, 0.0};
+  const GAcc U{x.u, m}, V{x.v, m};
+  if (j < m.ny && active[t.g]) {
+    FOR_TILE_ROWS {
+      const int i = t.i0 + ii;
+      if (i >= m.nx) break;
+      const long long p = p2(m, j, i);
+      double S, D, b;
+      if (i < m.nx - 1) {
+        u_row(m, kp, c, j, i, c.rt_u[p], U, V, S, D, b);
+        const double rv = b - (S + D * x.u[p]);
+        r.u[p] = rv; rh.u[p] = rv; pp.u[p] = 0.0; vv.u[p] = 0.0;
+        acc[0] += rv * rv; acc[1] += b * b;
+      }
+      if (j < m.ny - 1) {
+        v_row(m, kp, c, j, i, c.rt_v[p], U, V, S, D, b);
+        const double rv = b - (S + D * x.v[p]);
+        r.v[p] = rv; rh.v[p] = rv; pp.v[p] = 0.0; vv.v[p] = 0.0;
+        acc[0] += rv * rv; acc[1] += b * b;
+      }
+    }
+  }
+  block_sum_store<2>(acc, partial + 2 * (long long)blockIdx.x);
+}
+
+#define VEC_LOOP(...)                                               \
+  TILE_SETUP                                                       \
+  if (j < m.ny && active[t.g]) {                                   \
+    FOR_TILE_ROWS {                                                \
+      const int i = t.i0 + ii;                                     \
+      if (i >= m.nx) break;                                        \
+      const long long p = p2(m, j, i);                             \
+      const bool hu = i < m.nx - 1, hv = j < m.ny - 1;             \
+      __VA_ARGS__                                                  \
+    }                                                              \
+  }
+
+// p = r + beta (p - omega v)
+__global__ void __launch_bounds__(IMAU_WG) k_bicg_p(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                    CVec2 r, Vec2 pp, CVec2 vv, const double *__restrict__ sc,
+                                                    const int *__restrict__ active) {
+  VEC_LOOP({
+    const double beta = sc[t.g * bicg::NSC + bicg::BETA], om = sc[t.g * bicg::NSC + bicg::OMEGA];
+    if (hu) pp.u[p] = r.u[p] + beta * (pp.u[p] - om * vv.u[p]);
+    if (hv) pp.v[p] = r.v[p] + beta * (pp.v[p] - om * vv.v[p]);
+  })
+}
+
+// s = r - alpha v (s overwrites r)
+__global__ void __launch_bounds__(IMAU_WG) k_bicg_s(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                    Vec2 r, CVec2 vv, const double *__restrict__ sc,
+                                                    const int *__restrict__ active) {
+  VEC_LOOP({
+    const double al = sc[t.g * bicg::NSC + bicg::ALPHA];
+    if (hu) r.u[p] -= al * vv.u[p];
+    if (hv) r.v[p] -= al * vv.v[p];
+  })
+}
+
+// x += alpha phat + omega shat ; r = s - omega t ; partials (rhat, r), (r, r)
+__global__ void __launch_bounds__(IMAU_WG) k_bicg_xr(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                     Vec2 x, Vec2 r, CVec2 ph, CVec2 sh, CVec2 tt, CVec2 rh,
+                                                     const double *__restrict__ sc, const int *__restrict__ active,
+                                                     double *__restrict__ partial) {
+  double acc[2] = {0.0, 0.0};
+  VEC_LOOP({
+    const double al = sc[t.g * bicg::NSC + bicg::ALPHA], om = sc[t.g * bicg::NSC + bicg::OMEGA];
+    if (hu) {
+      x.u[p] += al * ph.u[p] + om * sh.u[p];
+      const double rv = r.u[p] - om * tt.u[p];
+      r.u[p] = rv;
+      acc[0] += rh.u[p] * rv; acc[1] += rv * rv;
+    }
+    if (hv) {
+      x.v[p] += al * ph.v[p] + om * sh.v[p];
+      const double rv = r.v[p] - om * tt.v[p];
+      r.v[p] = rv;
+      acc[0] += rh.v[p] * rv; acc[1] += rv * rv;
+    }
+  })
+  block_sum_store<2>(acc, partial + 2 * (long long)blockIdx.x);
+}
+
+// per-member reduction of two partials and the BiCGStab scalar update.
+// stage 0 (after init):  rho = (r,r), rr = (r,r), alpha = omega = 1, beta = 0
+// stage 1 (after v=Ap):  alpha = rho / (rhat, v)
+// stage 2 (after t=As):  omega = (t,s) / (t,t)
+// stage 3 (after x,r):   beta = (rho_new/rho)(alpha/omega), rho = rho_new, rr = (r,r)
+// out[2g], out[2g+1] receive the two sums (for the host's convergence test)
+__global__ void __launch_bounds__(256) k_bicg_scalar(const int *__restrict__ tstart, const double *__restrict__ partial,
+                                                     double *__restrict__ sc, double *__restrict__ out,
+                                                     const int *__restrict__ active, int stage) {
+  const int g = blockIdx.x;
+  const int a = tstart[g], b = tstart[g + 1];
+  __shared__ double red[2][256];
+  double s0 = 0.0, s1 = 0.0;
+  for (int k = a + (int)threadIdx.x; k < b; k += 256) {
+    s0 += partial[2 * (long long)k];
+    s1 += partial[2 * (long long)k + 1];
+  }
+  red[0][threadIdx.x] = s0;
+  red[1][threadIdx.x] = s1;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + w];
+      red[1][threadIdx.x] += red[1][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  s0 = red[0][0];
+  s1 = red[1][0];
+  out[2 * g] = s0;
+  out[2 * g + 1] = s1;
+  if (!active[g]) return;
+  double *S = sc + g * bicg::NSC;
+  if (stage == 0) {
+    S[bicg::RHO] = s0; S[bicg::RR] = s0; S[bicg::ALPHA] = 1.0; S[bicg::OMEGA] = 1.0; S[bicg::BETA] = 0.0;
+  } else if (stage == 1) {
+    S[bicg::ALPHA] = (s0 != 0.0) ? S[bicg::RHO] / s0 : 0.0;
+  } else if (stage == 2) {
+    S[bicg::OMEGA] = (s1 != 0.0) ? s0 / s1 : 0.0;
+  } else {
+    const double rho = S[bicg::RHO], om = S[bicg::OMEGA];
+    S[bicg::BETA] = (rho != 0.0 && om != 0.0) ? (s0 / rho) * (S[bicg::ALPHA] / om) : 0.0;
+    S[bicg::RHO] = s0;
+    S[bicg::RR] = s1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Preconditioner z = M^-1 y: symmetric Gauss-Seidel on every tile (forward
+// colours u-red, u-black, v-red, v-black, boundary rows; then backward), from
+// a zero start, with the tile halo held at zero (block Jacobi across tiles).
+// nsweeps symmetric sweeps per application.  The row right-hand side is y.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(IMAU_WG) k_prec_tile(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                       const KP kp, const CoefP c, CVec2 y, Vec2 z,
+                                                       const int *__restrict__ active, int nsweeps) {
+  __shared__ double red_unused;
+  (void)red_unused;
+  const Tile t = tiles[blockIdx.x];
+  const Member m = mem[t.g];
+  const int nx = m.nx, ny = m.ny;
+  const int tid = threadIdx.y * IMAU_TJ + threadIdx.x;
+  if (!active[t.g]) return;
+  for (int q = tid; q < (IMAU_TI + 2) * (IMAU_TJ + 2); q += IMAU_WG) {
+    g_sx[0][q / (IMAU_TJ + 2)][q % (IMAU_TJ + 2)] = 0.0;
+    g_sx[1][q / (IMAU_TJ + 2)][q % (IMAU_TJ + 2)] = 0.0;
+  }
+  __syncthreads();
+  const LAcc<0> U{t.i0, t.j0};
+  const LAcc<1> V{t.i0, t.j0};
+  const int j = t.j0 + (int)threadIdx.x;
+  const int lj = threadIdx.x + 1;
+  for (int sw = 0; sw < 2 * nsweeps; ++sw) {
+    const bool fwd = (sw & 1) == 0;
+    for (int q = 0; q < 5; ++q) {
+      // forward: 0 1 2 3 B ; backward: 3 2 1 0 B
+      const int ph = (q == 4) ? 4 : (fwd ? q : 3 - q);
+      FOR_TILE_ROWS {
+        const int i = t.i0 + ii;
+        if (i >= nx || j >= ny) continue;
+        const long long p = p2(m, j, i);
+        const int li = ii + 1;
+        if (ph < 4) {
+          const bool isu = ph < 2;
+          if (((i + j) & 1) != (ph & 1)) continue;
+          if (isu && i >= nx - 1) continue;
+          if (!isu && j >= ny - 1) continue;
+          const uint8_t rt = isu ? c.rt_u[p] : c.rt_v[p];
+          if (is_bnd(rt)) continue;
+          double S, D, b;
+          if (isu) {
+            u_row(m, kp, c, j, i, rt, U, V, S, D, b);
+            g_sx[0][li][lj] = (y.u[p] - S) / D;
+          } else {
+            v_row(m, kp, c, j, i, rt, U, V, S, D, b);
+            g_sx[1][li][lj] = (y.v[p] - S) / D;
+          }
+        } else {
+          if (i < nx - 1) {
+            const uint8_t rt = c.rt_u[p];
+            if (is_bnd(rt)) {
+              double val = y.u[p];
+              if (rt == RT_BND_INF) {
+                int jo, io;
+                bnd_src(rt, i, j, nx - 2, ny - 1, jo, io);
+                val += g_sx[0][io - t.i0 + 1][jo - t.j0 + 1];
+              }
+              g_sx[0][li][lj] = val;
+            }
+          }
+          if (j < ny - 1) {
+            const uint8_t rt = c.rt_v[p];
+            if (is_bnd(rt)) {
+              double val = y.v[p];
+              if (rt == RT_BND_INF) {
+                int jo, io;
+                bnd_src(rt, i, j, nx - 1, ny - 2, jo, io);
+                val += g_sx[1][io - t.i0 + 1][jo - t.j0 + 1];
+              }
+              g_sx[1][li][lj] = val;
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= nx || j >= ny) continue;
+    const long long p = p2(m, j, i);
+    if (i < nx - 1) z.u[p] = g_sx[0][ii + 1][lj];
+    if (j < ny - 1) z.v[p] = g_sx[1][ii + 1][lj];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Global multicolour SOR (reference SOR semantics, src/utilities_module.f90:644-718,
+// in a parallel ordering): one launch updates one colour in place.
+// colour 0 u-red, 1 u-black, 2 v-red, 3 v-black, 4 boundary rows.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(IMAU_WG) k_sor_color(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                       const KP kp, const CoefP c, Vec2 x, int color, double omega,
+                                                       const int *__restrict__ active, int track,
+                                                       unsigned long long *__restrict__ resmax) {
+  __shared__ double red[IMAU_WG_I];
+  TILE_SETUP
+  double rmax = 0.0;
+  const GAcc U{x.u, m}, V{x.v, m};
+  const int nx = m.nx, ny = m.ny;
+  if (j < ny && active[t.g]) {
+    FOR_TILE_ROWS {
+      const int i = t.i0 + ii;
+      if (i >= nx) break;
+      const long long p = p2(m, j, i);
+      if (color < 4) {
+        const bool isu = color < 2;
+        if (((i + j) & 1) != (color & 1)) continue;
+        if (isu && i >= nx - 1) continue;
+        if (!isu && j >= ny - 1) continue;
+        const uint8_t rt = isu ? c.rt_u[p] : c.rt_v[p];
+        if (is_bnd(rt)) continue;
+        double S, D, b;
+        if (isu) u_row(m, kp, c, j, i, rt, U, V, S, D, b);
+        else v_row(m, kp, c, j, i, rt, U, V, S, D, b);
+        double *xx = isu ? x.u : x.v;
+        const double xo = xx[p];
+        const double res = (S + D * xo - b) / D;
+        rmax = fmax(rmax, fabs(res));
+        xx[p] = (rt == RT_FREE) ? xo - omega * res : xo - res;
+      } else {
+        if (i < nx - 1 && is_bnd(c.rt_u[p])) {
+          const uint8_t rt = c.rt_u[p];
+          double val = 0.0;
+          if (rt != RT_BND_ZERO) {
+            int jo, io;
+            bnd_src(rt, i, j, nx - 2, ny - 1, jo, io);
+            val = x.u[p2(m, jo, io)];
+          }
+          rmax = fmax(rmax, fabs(x.u[p] - val));
+          x.u[p] = val;
+        }
+        if (j < ny - 1 && is_bnd(c.rt_v[p])) {
+          const uint8_t rt = c.rt_v[p];
+          double val = 0.0;
+          if (rt != RT_BND_ZERO) {
+            int jo, io;
+            bnd_src(rt, i, j, nx - 1, ny - 2, jo, io);
+            val = x.v[p2(m, jo, io)];
+          }
+          rmax = fmax(rmax, fabs(x.v[p] - val));
+          x.v[p] = val;
+        }
+      }
+    }
+  }
+  if (track) {
+    const double w = wave_max(rmax);
+    if (threadIdx.x == 0) red[threadIdx.y] = w;
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      double r = red[0];
+      for (int q = 1; q < IMAU_WG_I; ++q) r = fmax(r, red[q]);
+      typedef __attribute__((address_space(1))) unsigned long long gu64;
+      __hip_atomic_fetch_max((gu64 *)(resmax + t.g), (unsigned long long)__double_as_longlong(r), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+namespace imau {
+void apply(const Launch &L, const KP &kp, const Coef &c, const double *xu, const double *xv, double *yu, double *yv,
+           const double *wu, const double *wv, const int *active, double *partial) {
+  hipLaunchKernelGGL(k_apply, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, to_p(c), CVec2{xu, xv}, Vec2{yu, yv},
+                     CVec2{wu, wv}, active, partial);
+}
+void bicg_init(const Launch &L, const KP &kp, const Coef &c, const double *xu, const double *xv, double *ru, double *rv,
+               double *rhu, double *rhv, double *pu, double *pv, double *vu, double *vv, const int *active,
+               double *partial) {
+  hipLaunchKernelGGL(k_bicg_init, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, to_p(c), CVec2{xu, xv}, Vec2{ru, rv},
+                     Vec2{rhu, rhv}, Vec2{pu, pv}, Vec2{vu, vv}, active, partial);
+}
+void bicg_p(const Launch &L, const double *ru, const double *rv, double *pu, double *pv, const double *vu,
+            const double *vv, const double *sc, const int *active) {
+  hipLaunchKernelGGL(k_bicg_p, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, CVec2{ru, rv}, Vec2{pu, pv}, CVec2{vu, vv},
+                     sc, active);
+}
+void bicg_s(const Launch &L, double *ru, double *rv, const double *vu, const double *vv, const double *sc,
+            const int *active) {
+  hipLaunchKernelGGL(k_bicg_s, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, Vec2{ru, rv}, CVec2{vu, vv}, sc, active);
+}
+void bicg_xr(const Launch &L, double *xu, double *xv, double *ru, double *rv, const double *phu, const double *phv,
+             const double *shu, const double *shv, const double *tu, const double *tv, const double *rhu,
+             const double *rhv, const double *sc, const int *active, double *partial) {
+  hipLaunchKernelGGL(k_bicg_xr, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, Vec2{xu, xv}, Vec2{ru, rv},
+                     CVec2{phu, phv}, CVec2{shu, shv}, CVec2{tu, tv}, CVec2{rhu, rhv}, sc, active, partial);
+}
+void bicg_scalar(hipStream_t s, int ngrids, const int *tstart, const double *partial, double *sc, double *out,
+                 const int *active, int stage) {
+  hipLaunchKernelGGL(k_bicg_scalar, dim3(ngrids), dim3(256), 0, s, tstart, partial, sc, out, active, stage);
+}
+void prec_tile(const Launch &L, const KP &kp, const Coef &c, const double *yu, const double *yv, double *zu, double *zv,
+               const int *active, int nsweeps) {
+  hipLaunchKernelGGL(k_prec_tile, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, to_p(c), CVec2{yu, yv},
+                     Vec2{zu, zv}, active, nsweeps);
+}
+void sor_color(const Launch &L, const KP &kp, const Coef &c, double *xu, double *xv, int color, double omega,
+               const int *active, bool track, unsigned long long *resmax) {
+  hipLaunchKernelGGL(k_sor_color, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, to_p(c), Vec2{xu, xv}, color, omega,
+                     active, track ? 1 : 0, resmax);
+}
+}  // namespace imau
+
+// ===========================================================================
+// Aggregation multigrid preconditioner.  Level 0 is the matrix-free fine
+// operator; coarse levels hold explicit staggered 9-point stencils built on
+// the device as Galerkin products R A P, with P piecewise constant on 2 x 2
+// aggregates of each component (and 'infinite' boundary unknowns folded into
+// the aggregate of their interior source), R = P^T over the non-boundary rows.
+// ===========================================================================
+__device__ __forceinline__ int agg_slot(int rowc, int colc, int dJ, int dI) {
+  if (rowc == colc) {
+    if (dJ == 0 && dI == 0) return 0;
+    if (dJ == 0 && dI == -1) return 1;
+    if (dJ == 0 && dI == 1) return 2;
+    if (dJ == -1 && dI == 0) return 3;
+    if (dJ == 1 && dI == 0) return 4;
+    return -1;
+  }
+  if (rowc == 0) {  // u row, v column
+    if (dJ == -1 && dI == 0) return 5;
+    if (dJ == -1 && dI == 1) return 6;
+    if (dJ == 0 && dI == 0) return 7;
+    if (dJ == 0 && dI == 1) return 8;
+    return -1;
+  }
+  // v row, u column
+  if (dJ == 0 && dI == -1) return 5;
+  if (dJ == 0 && dI == 0) return 6;
+  if (dJ == 1 && dI == -1) return 7;
+  if (dJ == 1 && dI == 0) return 8;
+  return -1;
+}
+
+// nonzeros of the fine u-row (free or calving-front) at (j,i): f(comp, jj, ii, value)
+template <class F>
+__device__ __forceinline__ void u_couplings(const Member &m, const KP &kp, const CoefP &c, int j, int i, uint8_t rt,
+                                            F &&f) {
+  const double dx2 = m.dx * m.dx;
+  if (rt == RT_FREE) {
+    if (kp.cross) {
+      const double Na0 = c.Na[p2(m, j, i)], Na1 = c.Na[p2(m, j, i + 1)];
+      const double Nb0 = c.Nb[p2(m, j, i)], Nbm = c.Nb[p2(m, j - 1, i)];
+      f(0, j, i + 1, 4.0 * Na1); f(0, j, i - 1, 4.0 * Na0); f(0, j + 1, i, Nb0); f(0, j - 1, i, Nbm);
+      f(0, j, i, -4.0 * (Na1 + Na0) + -1.0 * (Nb0 + Nbm) - c.bcx[p2(m, j, i)] * dx2);
+      f(1, j, i, -2.0 * Na0 - Nb0); f(1, j, i + 1, 2.0 * Na1 + Nb0);
+      f(1, j - 1, i, 2.0 * Na0 + Nbm); f(1, j - 1, i + 1, -2.0 * Na1 - Nbm);
+    } else {
+      const double Ncx = c.Ncx[p2(m, j, i)];
+      f(0, j, i + 1, 4.0); f(0, j, i - 1, 4.0); f(0, j + 1, i, 1.0); f(0, j - 1, i, 1.0);
+      f(0, j, i, -10.0 - dx2 * c.bcx[p2(m, j, i)] / Ncx);
+      f(1, j, i, -3.0); f(1, j, i + 1, 3.0); f(1, j - 1, i, 3.0); f(1, j - 1, i + 1, -3.0);
+    }
+  } else {  // RT_MARGIN_I / RT_MARGIN_O
+    const int i1 = (rt == RT_MARGIN_I) ? i : i + 1;
+    const double Na = c.Na[p2(m, j, i1)] / m.dx;
+    f(0, j, i1 - 1, -4.0 * Na); f(0, j, i1, 4.0 * Na); f(1, j - 1, i1, -2.0 * Na); f(1, j, i1, 2.0 * Na);
+  }
+}
+template <class F>
+__device__ __forceinline__ void v_couplings(const Member &m, const KP &kp, const CoefP &c, int j, int i, uint8_t rt,
+                                            F &&f) {
+  const double dx2 = m.dx * m.dx;
+  if (rt == RT_FREE) {
+    if (kp.cross) {
+      const double Na0 = c.Na[p2(m, j, i)], Na1 = c.Na[p2(m, j + 1, i)];
+      const double Nb0 = c.Nb[p2(m, j, i)], Nbm = c.Nb[p2(m, j, i - 1)];
+      f(1, j + 1, i, 4.0 * Na1); f(1, j - 1, i, 4.0 * Na0); f(1, j, i + 1, Nb0); f(1, j, i - 1, Nbm);
+      f(1, j, i, -4.0 * (Na1 + Na0) + -1.0 * (Nb0 + Nbm) - c.bcy[p2(m, j, i)] * dx2);
+      f(0, j + 1, i - 1, -2.0 * Na1 - Nbm); f(0, j + 1, i, 2.0 * Na1 + Nb0);
+      f(0, j, i - 1, 2.0 * Na0 + Nbm); f(0, j, i, -2.0 * Na0 - Nb0);
+    } else {
+      const double Ncy = c.Ncy[p2(m, j, i)];
+      f(1, j + 1, i, 4.0); f(1, j - 1, i, 4.0); f(1, j, i + 1, 1.0); f(1, j, i - 1, 1.0);
+      f(1, j, i, -10.0 - dx2 * c.bcy[p2(m, j, i)] / Ncy);
+      f(0, j + 1, i - 1, -3.0); f(0, j + 1, i, 3.0); f(0, j, i - 1, 3.0); f(0, j, i, -3.0);
+    }
+  } else {
+    const int j1 = (rt == RT_MARGIN_I) ? j : j + 1;
+    const double Na = c.Na[p2(m, j1, i)] / m.dx;
+    f(1, j1 - 1, i, -4.0 * Na); f(1, j1, i, 4.0 * Na); f(0, j1, i - 1, -2.0 * Na); f(0, j1, i, 2.0 * Na);
+  }
+}
+
+// coarse aggregate of the fine unknown (comp, jj, ii); false if it has no coarse counterpart
+__device__ __forceinline__ bool fine_agg(const Member &m, const CoefP &c, int comp, int jj, int ii, int &J, int &I) {
+  uint8_t rt = comp == 0 ? c.rt_u[p2(m, jj, ii)] : c.rt_v[p2(m, jj, ii)];
+  if (rt == RT_BND_INF) {
+    int jo, io;
+    bnd_src(rt, ii, jj, comp == 0 ? m.nx - 2 : m.nx - 1, comp == 0 ? m.ny - 1 : m.ny - 2, jo, io);
+    jj = jo;
+    ii = io;
+    rt = comp == 0 ? c.rt_u[p2(m, jj, ii)] : c.rt_v[p2(m, jj, ii)];
+  }
+  if (rt != RT_FREE && rt != RT_MARGIN_I && rt != RT_MARGIN_O) return false;
+  J = jj >> 1;
+  I = ii >> 1;
+  return true;
+}
+
+__device__ __forceinline__ bool in_row(uint8_t rt) { return rt == RT_FREE || rt == RT_MARGIN_I || rt == RT_MARGIN_O; }
+
+#define CTILE_SETUP                                  \
+  const Tile t = L.tiles[blockIdx.x];                \
+  const CMember cm = L.mem[t.g];                     \
+  const int J = t.j0 + (int)threadIdx.x;
+
+__device__ __forceinline__ long long cp2(const CMember &m, int J, int I) { return m.off + (long long)I * m.ny + J; }
+
+// Galerkin product fine (level 0) -> level 1
+__global__ void __launch_bounds__(IMAU_WG) k_gal0(const Member *__restrict__ mem, const KP kp, const CoefP c, CLevel L,
+                                                  const int *__restrict__ active) {
+  CTILE_SETUP
+  if (J >= cm.ny || !active[t.g]) return;
+  const Member m = mem[t.g];
+  FOR_TILE_ROWS {
+    const int I = t.i0 + ii;
+    if (I >= cm.nx) break;
+    const long long pc = cp2(cm, J, I);
+    for (int comp = 0; comp < 2; ++comp) {
+      const bool valid = comp == 0 ? (J < cm.nyu && I < cm.nxu) : (J < cm.nyv && I < cm.nxv);
+      if (!valid) continue;
+      double cf[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int dj = 0; dj < 2; ++dj)
+        for (int di = 0; di < 2; ++di) {
+          const int j = 2 * J + dj, i = 2 * I + di;
+          if (comp == 0 && (i > m.nx - 2 || j > m.ny - 1)) continue;
+          if (comp == 1 && (i > m.nx - 1 || j > m.ny - 2)) continue;
+          const uint8_t rt = comp == 0 ? c.rt_u[p2(m, j, i)] : c.rt_v[p2(m, j, i)];
+          if (!in_row(rt)) continue;
+          auto acc = [&](int cc, int jj, int i2, double val) {
+            int Jc, Ic;
+            if (!fine_agg(m, c, cc, jj, i2, Jc, Ic)) return;
+            const int s = agg_slot(comp, cc, Jc - J, Ic - I);
+            if (s >= 0) cf[s] += val;
+          };
+          if (comp == 0) u_couplings(m, kp, c, j, i, rt, acc);
+          else v_couplings(m, kp, c, j, i, rt, acc);
+        }
+      double *dst = comp == 0 ? L.cu : L.cv;
+      for (int k = 0; k < 9; ++k) dst[k * L.na + pc] = cf[k];
+    }
+  }
+}
+
+// neighbour position of coarse slot k for a row of component rowc at (J,I)
+__device__ __forceinline__ void slot_pos(int rowc, int k, int J, int I, int &cc, int &JJ, int &II) {
+  const int du[9][3] = {{0, 0, 0}, {0, 0, -1}, {0, 0, 1}, {0, -1, 0}, {0, 1, 0}, {1, -1, 0}, {1, -1, 1}, {1, 0, 0}, {1, 0, 1}};
+  const int dv[9][3] = {{1, 0, 0}, {1, 0, -1}, {1, 0, 1}, {1, -1, 0}, {1, 1, 0}, {0, 0, -1}, {0, 0, 0}, {0, 1, -1}, {0, 1, 0}};
+  const int(*d)[3] = rowc == 0 ? du : dv;
+  cc = d[k][0];
+  JJ = J + d[k][1];
+  II = I + d[k][2];
+}
+__device__ __forceinline__ bool cvalid(const CMember &m, int comp, int J, int I) {
+  return comp == 0 ? (J >= 0 && I >= 0 && J < m.nyu && I < m.nxu) : (J >= 0 && I >= 0 && J < m.nyv && I < m.nxv);
+}
+
+// Galerkin product level l (F) -> level l+1 (L)
+__global__ void __launch_bounds__(IMAU_WG) k_galc(CLevel Fl, CLevel L, const int *__restrict__ active) {
+  CTILE_SETUP
+  if (J >= cm.ny || !active[t.g]) return;
+  const CMember fm = Fl.mem[t.g];
+  FOR_TILE_ROWS {
+    const int I = t.i0 + ii;
+    if (I >= cm.nx) break;
+    const long long pc = cp2(cm, J, I);
+    for (int comp = 0; comp < 2; ++comp) {
+      if (!cvalid(cm, comp, J, I)) continue;
+      double cf[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int dj = 0; dj < 2; ++dj)
+        for (int di = 0; di < 2; ++di) {
+          const int j = 2 * J + dj, i = 2 * I + di;
+          if (!cvalid(fm, comp, j, i)) continue;
+          const long long pf = cp2(fm, j, i);
+          const double *src = comp == 0 ? Fl.cu : Fl.cv;
+          for (int k = 0; k < 9; ++k) {
+            int cc, jj, i2;
+            slot_pos(comp, k, j, i, cc, jj, i2);
+            if (!cvalid(fm, cc, jj, i2)) continue;
+            const int s = agg_slot(comp, cc, (jj >> 1) - J, (i2 >> 1) - I);
+            if (s >= 0) cf[s] += src[k * Fl.na + pf];
+          }
+        }
+      double *dst = comp == 0 ? L.cu : L.cv;
+      for (int k = 0; k < 9; ++k) dst[k * L.na + pc] = cf[k];
+    }
+  }
+}
+
+// row of a coarse level: off-diagonal sum S and diagonal D at (J,I)
+__device__ __forceinline__ void crow(const CLevel &L, const CMember &m, int comp, int J, int I, const double *xu,
+                                     const double *xv, double &S, double &D) {
+  const long long p = cp2(m, J, I);
+  const double *cf = comp == 0 ? L.cu : L.cv;
+  D = cf[p];
+  S = 0.0;
+#pragma unroll
+  for (int k = 1; k < 9; ++k) {
+    int cc, JJ, II;
+    slot_pos(comp, k, J, I, cc, JJ, II);
+    if (!cvalid(m, cc, JJ, II)) continue;
+    S += cf[k * L.na + p] * (cc == 0 ? xu : xv)[cp2(m, JJ, II)];
+  }
+}
+
+// one colour of Gauss-Seidel on a coarse level (x <- (b - S)/D), in place
+__global__ void __launch_bounds__(IMAU_WG) k_cgs(CLevel L, int color, const int *__restrict__ active) {
+  CTILE_SETUP
+  if (J >= cm.ny || !active[t.g]) return;
+  const int comp = color >> 1;
+  FOR_TILE_ROWS {
+    const int I = t.i0 + ii;
+    if (I >= cm.nx) break;
+    if (((I + J) & 1) != (color & 1) || !cvalid(cm, comp, J, I)) continue;
+    double S, D;
+    crow(L, cm, comp, J, I, L.xu, L.xv, S, D);
+    const long long p = cp2(cm, J, I);
+    const double b = (comp == 0 ? L.bu : L.bv)[p];
+    (comp == 0 ? L.xu : L.xv)[p] = (D != 0.0) ? (b - S) / D : 0.0;
+  }
+}
+
+// b(level l+1) = R (b - A x)(level l), coarse thread per aggregate
+__global__ void __launch_bounds__(IMAU_WG) k_crestrict(CLevel Fl, CLevel L, const int *__restrict__ active) {
+  CTILE_SETUP
+  if (J >= cm.ny || !active[t.g]) return;
+  const CMember fm = Fl.mem[t.g];
+  FOR_TILE_ROWS {
+    const int I = t.i0 + ii;
+    if (I >= cm.nx) break;
+    for (int comp = 0; comp < 2; ++comp) {
+      if (!cvalid(cm, comp, J, I)) continue;
+      double s = 0.0;
+      for (int dj = 0; dj < 2; ++dj)
+        for (int di = 0; di < 2; ++di) {
+          const int j = 2 * J + dj, i = 2 * I + di;
+          if (!cvalid(fm, comp, j, i)) continue;
+          double S, D;
+          crow(Fl, fm, comp, j, i, Fl.xu, Fl.xv, S, D);
+          const long long pf = cp2(fm, j, i);
+          s += (comp == 0 ? Fl.bu : Fl.bv)[pf] - S - D * (comp == 0 ? Fl.xu : Fl.xv)[pf];
+        }
+      (comp == 0 ? L.bu : L.bv)[cp2(cm, J, I)] = s;
+    }
+  }
+}
+
+// x(level l) += P x(level l+1), fine thread per cell
+__global__ void __launch_bounds__(IMAU_WG) k_cprolong(CLevel Fl, CLevel L, const int *__restrict__ active) {
+  const Tile t = Fl.tiles[blockIdx.x];
+  const CMember fm = Fl.mem[t.g];
+  const int j = t.j0 + (int)threadIdx.x;
+  if (j >= fm.ny || !active[t.g]) return;
+  const CMember cm = L.mem[t.g];
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= fm.nx) break;
+    const long long pf = cp2(fm, j, i), pc = cp2(cm, j >> 1, i >> 1);
+    if (cvalid(fm, 0, j, i)) Fl.xu[pf] += L.xu[pc];
+    if (cvalid(fm, 1, j, i)) Fl.xv[pf] += L.xv[pc];
+  }
+}
+
+// level-0 -> level-1 restriction of the residual y - A z (non-boundary rows)
+__global__ void __launch_bounds__(IMAU_WG) k_restrict0(const Member *__restrict__ mem, const KP kp, const CoefP c,
+                                                       CVec2 y, CVec2 z, CLevel L, const int *__restrict__ active) {
+  CTILE_SETUP
+  if (J >= cm.ny || !active[t.g]) return;
+  const Member m = mem[t.g];
+  const GAcc U{z.u, m}, V{z.v, m};
+  FOR_TILE_ROWS {
+    const int I = t.i0 + ii;
+    if (I >= cm.nx) break;
+    for (int comp = 0; comp < 2; ++comp) {
+      if (!cvalid(cm, comp, J, I)) continue;
+      double s = 0.0;
+      for (int dj = 0; dj < 2; ++dj)
+        for (int di = 0; di < 2; ++di) {
+          const int j = 2 * J + dj, i = 2 * I + di;
+          if (comp == 0 && (i > m.nx - 2 || j > m.ny - 1)) continue;
+          if (comp == 1 && (i > m.nx - 1 || j > m.ny - 2)) continue;
+          const long long p = p2(m, j, i);
+          const uint8_t rt = comp == 0 ? c.rt_u[p] : c.rt_v[p];
+          if (!in_row(rt)) continue;
+          double S, D, b;
+          if (comp == 0) {
+            u_row(m, kp, c, j, i, rt, U, V, S, D, b);
+            s += y.u[p] - S - D * z.u[p];
+          } else {
+            v_row(m, kp, c, j, i, rt, U, V, S, D, b);
+            s += y.v[p] - S - D * z.v[p];
+          }
+        }
+      (comp == 0 ? L.bu : L.bv)[cp2(cm, J, I)] = s;
+    }
+  }
+}
+
+// z += P x(level 1) on the fine non-boundary rows and the 'infinite' boundary rows
+__global__ void __launch_bounds__(IMAU_WG) k_prolong0(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                      const CoefP c, Vec2 z, CLevel L, const int *__restrict__ active) {
+  TILE_SETUP
+  if (j >= m.ny || !active[t.g]) return;
+  const CMember cm = L.mem[t.g];
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= m.nx) break;
+    const long long p = p2(m, j, i);
+    int Jc, Ic;
+    if (i < m.nx - 1 && fine_agg(m, c, 0, j, i, Jc, Ic)) z.u[p] += L.xu[cp2(cm, Jc, Ic)];
+    if (j < m.ny - 1 && fine_agg(m, c, 1, j, i, Jc, Ic)) z.v[p] += L.xv[cp2(cm, Jc, Ic)];
+  }
+}
+
+// coarsest level: one workgroup per member, nsweeps symmetric Gauss-Seidel
+// sweeps from zero with the iterate in LDS (coarsest slabs are <= 32 x 32)
+#define IMAU_CMAX 32
+__global__ void __launch_bounds__(256) k_coarsest(CLevel L, const int *__restrict__ active, int nsweeps) {
+  __shared__ double sx[2][IMAU_CMAX * IMAU_CMAX];
+  const int g = blockIdx.x;
+  if (!active[g]) return;
+  const CMember cm = L.mem[g];
+  const int n = cm.nx * cm.ny;
+  for (int q = threadIdx.x; q < IMAU_CMAX * IMAU_CMAX; q += 256) {
+    sx[0][q] = 0.0;
+    sx[1][q] = 0.0;
+  }
+  __syncthreads();
+  for (int sw = 0; sw < 2 * nsweeps; ++sw) {
+    for (int q4 = 0; q4 < 4; ++q4) {
+      const int color = (sw & 1) ? 3 - q4 : q4;
+      const int comp = color >> 1;
+      for (int q = threadIdx.x; q < n; q += 256) {
+        const int J = q % cm.ny, I = q / cm.ny;
+        if (((I + J) & 1) != (color & 1) || !cvalid(cm, comp, J, I)) continue;
+        const long long p = cp2(cm, J, I);
+        const double *cf = comp == 0 ? L.cu : L.cv;
+        const double D = cf[p];
+        double S = 0.0;
+        for (int k = 1; k < 9; ++k) {
+          int cc, JJ, II;
+          slot_pos(comp, k, J, I, cc, JJ, II);
+          if (!cvalid(cm, cc, JJ, II)) continue;
+          S += cf[k * L.na + p] * sx[cc][II * cm.ny + JJ];
+        }
+        const double b = (comp == 0 ? L.bu : L.bv)[p];
+        sx[comp][I * cm.ny + J] = (D != 0.0) ? (b - S) / D : 0.0;
+      }
+      __syncthreads();
+    }
+  }
+  for (int q = threadIdx.x; q < n; q += 256) {
+    const int J = q % cm.ny, I = q / cm.ny;
+    const long long p = cp2(cm, J, I);
+    if (cvalid(cm, 0, J, I)) L.xu[p] = sx[0][q];
+    if (cvalid(cm, 1, J, I)) L.xv[p] = sx[1][q];
+  }
+}
+
+__global__ void k_fill_f64(double *p, long long n, double v) {
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
+    p[q] = v;
+}
+
+namespace imau {
+void gal0(const Launch &F, const KP &kp, const Coef &c, const CLevel &L, const int *active) {
+  hipLaunchKernelGGL(k_gal0, dim3(L.ntiles), wg(), 0, F.s, F.mem, kp, to_p(c), L, active);
+}
+void galc(hipStream_t s, const CLevel &Fl, const CLevel &L, const int *active) {
+  hipLaunchKernelGGL(k_galc, dim3(L.ntiles), wg(), 0, s, Fl, L, active);
+}
+void cgs(hipStream_t s, const CLevel &L, int color, const int *active) {
+  hipLaunchKernelGGL(k_cgs, dim3(L.ntiles), wg(), 0, s, L, color, active);
+}
+void crestrict(hipStream_t s, const CLevel &Fl, const CLevel &L, const int *active) {
+  hipLaunchKernelGGL(k_crestrict, dim3(L.ntiles), wg(), 0, s, Fl, L, active);
+}
+void cprolong(hipStream_t s, const CLevel &Fl, const CLevel &L, const int *active) {
+  hipLaunchKernelGGL(k_cprolong, dim3(Fl.ntiles), wg(), 0, s, Fl, L, active);
+}
+void restrict0(const Launch &F, const KP &kp, const Coef &c, const double *yu, const double *yv, const double *zu,
+               const double *zv, const CLevel &L, const int *active) {
+  hipLaunchKernelGGL(k_restrict0, dim3(L.ntiles), wg(), 0, F.s, F.mem, kp, to_p(c), CVec2{yu, yv}, CVec2{zu, zv}, L,
+                     active);
+}
+void prolong0(const Launch &F, const Coef &c, double *zu, double *zv, const CLevel &L, const int *active) {
+  hipLaunchKernelGGL(k_prolong0, dim3(F.ntiles), wg(), 0, F.s, F.mem, F.tiles, to_p(c), Vec2{zu, zv}, L, active);
+}
+void coarsest(hipStream_t s, int ngrids, const CLevel &L, const int *active, int nsweeps) {
+  hipLaunchKernelGGL(k_coarsest, dim3(ngrids), dim3(256), 0, s, L, active, nsweeps);
+}
+void fill(hipStream_t s, double *p, long long n, double v) {
+  int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_fill_f64, dim3(blocks), dim3(256), 0, s, p, n, v);
+}
+}  // namespace imau

@@ -22,6 +22,7 @@ EXPORTED = (
     "imau_vel_nonlinear_residual", "imau_vel_finalize", "imau_vel_last_error", "imau_vel_default_config",
     "imau_vel_problem_create", "imau_vel_problem_solve", "imau_vel_problem_reset_state",
     "imau_vel_problem_download", "imau_vel_problem_destroy", "imau_vel_solve_diva_batch",
+    "imau_vel_kernel_timing",
 )
 
 
@@ -61,6 +62,7 @@ def load_library(path: str = LIB_PATH):
     L.imau_vel_problem_destroy.argtypes = [vp]
     L.imau_vel_problem_destroy.restype = None
     L.imau_vel_solve_diva_batch.argtypes = [vp, ctypes.c_int32, P(ImauGrid), P(ImauCfg), P(ImauIceIO)]
+    L.imau_vel_kernel_timing.argtypes = [vp, ctypes.c_int32, P(ctypes.c_int64), P(ctypes.c_double)]
     _lib = L
     return L
 
@@ -113,6 +115,12 @@ class Context:
         self._check(self.L.imau_vel_nonlinear_residual(self.h, ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(io),
                                                        ctypes.byref(r)))
         return r.value
+
+    def kernel_timing(self, enable: bool):
+        """(sweep launches, summed ms) since the last call; enables/disables event timing."""
+        n, ms = ctypes.c_int64(), ctypes.c_double()
+        self._check(self.L.imau_vel_kernel_timing(self.h, 1 if enable else 0, ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value
 
     def problem(self, grids, cfg, states):
         return Problem(self, grids, cfg, states)

@@ -213,6 +213,12 @@ void imau_vel_finalize(imau_ctx *ctx);
 /* Text of the last error on this context (or of the last failed init). */
 const char *imau_vel_last_error(const imau_ctx *ctx);
 
+/* Kernel timing (no reference counterpart): returns the number of SOR-sweep
+ * launches and their summed duration measured with HIP events on the
+ * context's stream since the previous call, then resets the counters and
+ * turns the event timing on (enable != 0) or off. */
+int  imau_vel_kernel_timing(imau_ctx *ctx, int32_t enable, int64_t *sweep_launches, double *sweep_ms);
+
 /* Fills cfg with the reference defaults (src/configuration_module.f90). */
 void imau_vel_default_config(imau_cfg *cfg);
 

@@ -90,3 +90,46 @@ def test_nonlinear_residual_matches_oracle(ctx):
     r_gpu = ctx.nonlinear_residual(g, cfg, st)
     r_ref = O.nonlinear_residual(g, cfg, st)
     assert abs(r_gpu - r_ref) <= 1e-9 * r_ref, (r_gpu, r_ref)
+
+
+def test_sor_mode_fixed_point(ctx):
+    """SOR mode (reference stop rule on the Gauss-Seidel update) reaches the same fixed point."""
+    g, cfg, st = geometry.dome()
+    tight(cfg, st)
+    config.set_config(cfg, DIVA_choice_matrix_solver="SOR")
+    gpu, ref = st.copy(), st.copy()
+    ctx.solve_diva(g, cfg, gpu)
+    O.solve("diva", g, cfg, ref, O.LIN_BANDED_LU)
+    assert relerr(gpu["u_vav_cx"], ref["u_vav_cx"]) < 1e-6
+
+
+def test_batch_equals_single_solves(ctx):
+    """Uneven-size members solved in one batch give exactly the single-grid results."""
+    from imau_vel import solver
+    cases = [geometry.dome(), geometry.channel(nx=61, ny=21, dx=8e3, seed=3), geometry.random_geometry(45, 37, seed=4)]
+    cfg = cases[0][1]
+    config.set_config(cfg, stop_rule=1, nonlinear_rtol=1e-6, DIVA_visc_it_nit=300)
+    singles = []
+    for g, _, st in cases:
+        s = st.copy()
+        ctx.solve_diva(g, cfg, s)
+        singles.append(s)
+    batch = [st.copy() for _, _, st in cases]
+    prob = solver.Problem(ctx, [c[0] for c in cases], cfg, batch)
+    prob.solve()
+    prob.download()
+    prob.close()
+    for s1, sb in zip(singles, batch):
+        assert s1.stats["picard_iterations"] == sb.stats["picard_iterations"]
+        np.testing.assert_array_equal(s1["u_vav_cx"], sb["u_vav_cx"])
+        np.testing.assert_array_equal(s1["w_3D_a"], sb["w_3D_a"])
+
+
+def test_repeat_solves_are_deterministic(ctx):
+    g, cfg, st = geometry.channel(nx=61, ny=21, dx=8e3, seed=3)
+    config.set_config(cfg, stop_rule=1, nonlinear_rtol=1e-5, DIVA_visc_it_nit=300)
+    a, b = st.copy(), st.copy()
+    ctx.solve_diva(g, cfg, a)
+    ctx.solve_diva(g, cfg, b)
+    np.testing.assert_array_equal(a["u_vav_cx"], b["u_vav_cx"])
+    np.testing.assert_array_equal(a["visc_eff_3D_a"], b["visc_eff_3D_a"])
