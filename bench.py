@@ -22,7 +22,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "imau-ice_amd"))
 
 HBM_PEAK_GBS = 8000.0
-SWEEP_BYTES_PER_CELL_FP64 = 10 * 8 + 2   # u,v in+out, N_a, N_b, beta_cx, beta_cy, taud_x, taud_y; 2 row-type bytes
+# one fine-level smoother sweep (5 colour launches of k_sor_color): read u, v,
+# N_a, N_b, beta_eff_cx, beta_eff_cy and the two right-hand sides, write u, v
+# (10 FP64 words per cell) plus the two row-type bytes
+SWEEP_BYTES_PER_CELL_FP64 = 10 * 8 + 2
 
 
 def workload():
@@ -99,7 +102,7 @@ def main():
         picard_iterations=stats["picard_iterations"], sweeps_per_solve=stats["linear_sweeps"],
         final_residual=stats["nonlinear_residual"],
         roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                      traffic=None, kernel="k_sweep", avg_launch_us=avg_us, launches=n_sweeps,
+                      traffic=None, kernel="k_sor_color (one 5-colour fine-level sweep)", avg_launch_us=avg_us, launches=n_sweeps,
                       bytes_per_launch=bytes_per_launch),
     )
     if not args.no_cpu_baseline:

@@ -91,6 +91,9 @@ struct Fields {
 
 // one member of a coarse multigrid level: u unknowns on (nyu, nxu), v on
 // (nyv, nxv), stored in a slab of ny = max(nyu,nyv) by nx = max(nxu,nxv)
+// coarsest multigrid level: every member at most IMAU_CMAX x IMAU_CMAX (one workgroup, LDS)
+#define IMAU_CMAX 32
+
 struct CMember {
   int nxu, nyu, nxv, nyv, nx, ny;
   long long off;
@@ -103,6 +106,8 @@ struct CMember {
 struct CLevel {
   const CMember *mem;
   const Tile *tiles;
+  const int *depth;  // per member: index of its coarsest level (>= 1)
+  int lev;           // index of this level (1 = first coarse level)
   int ntiles;
   long long na;
   double *cu, *cv;

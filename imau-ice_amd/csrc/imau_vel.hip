@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -66,6 +67,17 @@ struct DBuf {
   ~DBuf() { release(); }
 };
 
+// one coarse level of the aggregation multigrid (explicit staggered stencils)
+struct MGLevel {
+  std::vector<CMember> mem;
+  std::vector<Tile> tiles;
+  long long na = 0;
+  DBuf<CMember> mem_d;
+  DBuf<Tile> tiles_d;
+  DBuf<double> data;  // [cu 9na][cv 9na][xu na][xv na][bu na][bv na]
+  CLevel L{};
+};
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -91,6 +103,9 @@ struct imau_problem {
   DBuf<unsigned long long> resmax;
   DBuf<int> act_pic, act_lin;
   DBuf<double> sc;    // per-member Krylov scalars
+  std::vector<std::unique_ptr<MGLevel>> mg;  // coarse levels 1.. of the multigrid preconditioner
+  DBuf<int> mg_depth;                        // per member: its coarsest level
+  std::vector<char> mg_ends;                 // mg_ends[l]: some member's coarsest level is l + 1
 
   Fields F{};
   OutF O{};
@@ -128,6 +143,65 @@ static std::string g_init_err;
 
 // ---------------------------------------------------------------------------
 // problem construction
+
+// Coarse levels: every level halves each component's extent (2 x 2
+// aggregates, rounded up).  Member g is coarsened until it fits
+// IMAU_CMAX x IMAU_CMAX (at least once); that level, depth[g], is solved by
+// one workgroup in LDS, so each member's cycle is the same alone or in a batch.
+static void build_mg(imau_problem *P) {
+  std::vector<CMember> cur(P->ngrids);
+  std::vector<int> depth(P->ngrids, 0);
+  for (int g = 0; g < P->ngrids; ++g) {
+    const Member &m = P->mem[g];
+    cur[g] = CMember{m.nx - 1, m.ny, m.nx, m.ny - 1, m.nx, m.ny, 0};
+  }
+  auto fits = [](const CMember &m) { return m.nx <= IMAU_CMAX && m.ny <= IMAU_CMAX; };
+  for (int lev = 1;; ++lev) {
+    bool need = false;
+    for (int g = 0; g < P->ngrids; ++g) need = need || depth[g] == 0;
+    if (!need) break;
+    auto lv = std::make_unique<MGLevel>();
+    long long off = 0;
+    for (int g = 0; g < P->ngrids; ++g) {
+      CMember c = cur[g];
+      CMember n{(c.nxu + 1) / 2, (c.nyu + 1) / 2, (c.nxv + 1) / 2, (c.nyv + 1) / 2, 0, 0, off};
+      n.nx = std::max(n.nxu, n.nxv);
+      n.ny = std::max(n.nyu, n.nyv);
+      lv->mem.push_back(n);
+      if (depth[g] == 0)
+        for (int i0 = 0; i0 < n.nx; i0 += IMAU_TI)
+          for (int j0 = 0; j0 < n.ny; j0 += IMAU_TJ) lv->tiles.push_back(Tile{g, i0, j0, 0});
+      off += ((long long)n.nx * n.ny + 63) / 64 * 64;
+      cur[g] = n;
+      if (depth[g] == 0 && fits(n)) depth[g] = lev;
+    }
+    lv->na = off;
+    lv->mem_d.alloc(lv->mem.size());
+    lv->tiles_d.alloc(lv->tiles.size());
+    lv->data.alloc((size_t)22 * off);
+    HIPCHK(hipMemcpy(lv->mem_d.p, lv->mem.data(), lv->mem.size() * sizeof(CMember), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(lv->tiles_d.p, lv->tiles.data(), lv->tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    CLevel &L = lv->L;
+    L.mem = lv->mem_d.p;
+    L.tiles = lv->tiles_d.p;
+    L.lev = lev;
+    L.ntiles = (int)lv->tiles.size();
+    L.na = off;
+    L.cu = lv->data.p;
+    L.cv = L.cu + 9 * off;
+    L.xu = L.cv + 9 * off;
+    L.xv = L.xu + off;
+    L.bu = L.xv + off;
+    L.bv = L.bu + off;
+    P->mg.push_back(std::move(lv));
+  }
+  P->mg_depth.alloc(P->ngrids);
+  HIPCHK(hipMemcpy(P->mg_depth.p, depth.data(), P->ngrids * sizeof(int), hipMemcpyHostToDevice));
+  P->mg_ends.assign(P->mg.size(), 0);
+  for (int d : depth) P->mg_ends[d - 1] = 1;
+  for (auto &lv : P->mg) lv->L.depth = P->mg_depth.p;
+}
+
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -222,6 +296,9 @@ void check_cfg(const imau_cfg *cfg) {
     if (bcs[q] < 0 || bcs[q] > 2) fail(IMAU_ERR_CONFIG, "unknown DIVA boundary condition %d", bcs[q]);
   if (cfg->DIVA_choice_matrix_solver != IMAU_SOLVER_SOR && cfg->DIVA_choice_matrix_solver != IMAU_SOLVER_PETSC)
     fail(IMAU_ERR_CONFIG, "unknown DIVA_choice_matrix_solver %d", cfg->DIVA_choice_matrix_solver);
+  if (cfg->mg_levels != 0 && cfg->mg_levels != 1)
+    fail(IMAU_ERR_CONFIG, "mg_levels must be 0 (multigrid, automatic depth) or 1 (no multigrid), got %d",
+         cfg->mg_levels);
   for (int k = 1; k < cfg->nz; ++k)
     if (!(cfg->zeta[k] > cfg->zeta[k - 1])) fail(IMAU_ERR_CONFIG, "zeta must increase");
 }
@@ -557,6 +634,7 @@ int imau_vel_problem_create(imau_ctx *ctx, int32_t ngrids, const imau_grid *grid
   HIPCHK(hipMemcpy(P->tstart_d.p, P->tstart.data(), P->tstart.size() * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(P->xg.p, xs.data(), xs.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(P->yg.p, ys.data(), ys.size() * sizeof(double), hipMemcpyHostToDevice));
+  build_mg(P);
   wire(P);
   upload(ctx, P, cfg, ios);
   P->dyn_uploaded = cfg->choice_ice_dynamics;
@@ -748,7 +826,7 @@ struct Solver {
     double *PH[2] = {f2p(P, F_phu), f2p(P, F_phv)}, *SH[2] = {f2p(P, F_shu), f2p(P, F_shv)};
     double *T[2] = {f2p(P, F_tu), f2p(P, F_tv)};
     const imau::Coef cf = coef();
-    const int nsw = 1;
+    const bool use_mg = cfg->mg_levels != 1 && !P->mg.empty();
     std::vector<double> out(2 * (size_t)G), target(G, 0.0);
     auto fetch = [&]() {
       HIPCHK(hipMemcpyAsync(out.data(), P->msum.p, 2 * G * sizeof(double), hipMemcpyDeviceToHost, c->s));
@@ -771,18 +849,23 @@ struct Solver {
     }
     if (!any) return;
     push_active(P->act_lin.p, act_lin);
+    if (use_mg) mg_setup(cf);
+    auto precondition = [&](double *const *y, double *const *z) {
+      if (use_mg) mg_vcycle(cf, y[0], y[1], z[0], z[1]);
+      else {
+        time_begin();
+        imau::prec_tile(L, kp, cf, y[0], y[1], z[0], z[1], P->act_lin.p, 1);
+        time_end();
+      }
+    };
     const int maxit = sor_nit > 0 ? sor_nit : 10000;
     for (int it = 0; it < maxit; ++it) {
       imau::bicg_p(L, R[0], R[1], Pp[0], Pp[1], Vv[0], Vv[1], P->sc.p, P->act_lin.p);
-      time_begin();
-      imau::prec_tile(L, kp, cf, Pp[0], Pp[1], PH[0], PH[1], P->act_lin.p, nsw);
-      time_end();
+      precondition(Pp, PH);
       imau::apply(L, kp, cf, PH[0], PH[1], Vv[0], Vv[1], RH[0], RH[1], P->act_lin.p, P->partial.p);
       imau::bicg_scalar(c->s, G, P->tstart_d.p, P->partial.p, P->sc.p, P->msum.p, P->act_lin.p, 1);
       imau::bicg_s(L, R[0], R[1], Vv[0], Vv[1], P->sc.p, P->act_lin.p);
-      time_begin();
-      imau::prec_tile(L, kp, cf, R[0], R[1], SH[0], SH[1], P->act_lin.p, nsw);
-      time_end();
+      precondition(R, SH);
       imau::apply(L, kp, cf, SH[0], SH[1], T[0], T[1], R[0], R[1], P->act_lin.p, P->partial.p);
       imau::bicg_scalar(c->s, G, P->tstart_d.p, P->partial.p, P->sc.p, P->msum.p, P->act_lin.p, 2);
       imau::bicg_xr(L, X[0], X[1], R[0], R[1], PH[0], PH[1], SH[0], SH[1], T[0], T[1], RH[0], RH[1], P->sc.p,
@@ -802,6 +885,47 @@ struct Solver {
       if (!any) break;
       push_active(P->act_lin.p, act_lin);
     }
+  }
+
+  // Galerkin coarse operators for the current coefficients (once per linear solve)
+  void mg_setup(const imau::Coef &cf) {
+    const int *act = P->act_lin.p;
+    imau::gal0(L, kp, cf, P->mg[0]->L, act);
+    for (size_t l = 1; l < P->mg.size(); ++l) imau::galc(c->s, P->mg[l - 1]->L, P->mg[l]->L, act);
+    kcheck();
+  }
+
+  static constexpr int kCoarsestSweeps = 10;
+
+  // z = V(y): one V-cycle from z = 0 with one multicolour Gauss-Seidel sweep
+  // before and after each coarse-grid correction (reverse colour order after);
+  // a member's coarsest level is solved by kCoarsestSweeps symmetric sweeps
+  void mg_vcycle(const imau::Coef &cf, const double *yu, const double *yv, double *zu, double *zv) {
+    const int *act = P->act_lin.p;
+    hipStream_t s = c->s;
+    const int nl = (int)P->mg.size();
+    imau::fill(s, zu, P->na, 0.0);
+    imau::fill(s, zv, P->na, 0.0);
+    time_begin();
+    for (int col = 0; col < 5; ++col) imau::sor_color(L, kp, cf, zu, zv, col, 1.0, act, false, nullptr, yu, yv);
+    time_end();
+    imau::restrict0(L, kp, cf, yu, yv, zu, zv, P->mg[0]->L, act);
+    for (int l = 0; l < nl; ++l) {
+      const CLevel &C = P->mg[l]->L;
+      imau::fill(s, C.xu, 2 * C.na, 0.0);
+      if (P->mg_ends[l]) imau::coarsest(s, P->ngrids, C, act, kCoarsestSweeps);
+      if (l + 1 == nl) break;
+      for (int col = 0; col < 4; ++col) imau::cgs(s, C, col, act);
+      imau::crestrict(s, C, P->mg[l + 1]->L, act);
+    }
+    for (int l = nl - 2; l >= 0; --l) {
+      const CLevel &C = P->mg[l]->L;
+      imau::cprolong(s, C, P->mg[l + 1]->L, act);
+      for (int col = 3; col >= 0; --col) imau::cgs(s, C, col, act);
+    }
+    imau::prolong0(L, cf, zu, zv, P->mg[0]->L, act);
+    for (int col = 3; col >= 0; --col) imau::sor_color(L, kp, cf, zu, zv, col, 1.0, act, false, nullptr, yu, yv);
+    imau::sor_color(L, kp, cf, zu, zv, 4, 1.0, act, false, nullptr, yu, yv);
   }
 
   int sor_nit = 10000;

@@ -60,7 +60,8 @@ void bicg_scalar(hipStream_t s, int ngrids, const int *tstart, const double *par
 void prec_tile(const Launch &L, const KP &kp, const Coef &c, const double *yu, const double *yv, double *zu, double *zv,
                const int *active, int nsweeps);
 void sor_color(const Launch &L, const KP &kp, const Coef &c, double *xu, double *xv, int color, double omega,
-               const int *active, bool track, unsigned long long *resmax);
+               const int *active, bool track, unsigned long long *resmax, const double *ru = nullptr,
+               const double *rv = nullptr);
 // aggregation multigrid
 void gal0(const Launch &F, const KP &kp, const Coef &c, const CLevel &L, const int *active);
 void galc(hipStream_t s, const CLevel &Fl, const CLevel &L, const int *active);

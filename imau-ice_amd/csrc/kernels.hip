@@ -1605,11 +1605,13 @@ __global__ void __launch_bounds__(IMAU_WG) k_prec_tile(const Member *__restrict_
 // Global multicolour SOR (reference SOR semantics, src/utilities_module.f90:644-718,
 // in a parallel ordering): one launch updates one colour in place.
 // colour 0 u-red, 1 u-black, 2 v-red, 3 v-black, 4 boundary rows.
+// With rhs.u != nullptr the right-hand side is rhs (boundary rows included)
+// instead of the assembled b: the multigrid smoother on A z = y.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(IMAU_WG) k_sor_color(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
                                                        const KP kp, const CoefP c, Vec2 x, int color, double omega,
                                                        const int *__restrict__ active, int track,
-                                                       unsigned long long *__restrict__ resmax) {
+                                                       unsigned long long *__restrict__ resmax, CVec2 rhs) {
   __shared__ double red[IMAU_WG_I];
   TILE_SETUP
   double rmax = 0.0;
@@ -1630,6 +1632,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_sor_color(const Member *__restrict_
         double S, D, b;
         if (isu) u_row(m, kp, c, j, i, rt, U, V, S, D, b);
         else v_row(m, kp, c, j, i, rt, U, V, S, D, b);
+        if (rhs.u) b = isu ? rhs.u[p] : rhs.v[p];
         double *xx = isu ? x.u : x.v;
         const double xo = xx[p];
         const double res = (S + D * xo - b) / D;
@@ -1638,22 +1641,22 @@ __global__ void __launch_bounds__(IMAU_WG) k_sor_color(const Member *__restrict_
       } else {
         if (i < nx - 1 && is_bnd(c.rt_u[p])) {
           const uint8_t rt = c.rt_u[p];
-          double val = 0.0;
+          double val = rhs.u ? rhs.u[p] : 0.0;
           if (rt != RT_BND_ZERO) {
             int jo, io;
             bnd_src(rt, i, j, nx - 2, ny - 1, jo, io);
-            val = x.u[p2(m, jo, io)];
+            val += x.u[p2(m, jo, io)];
           }
           rmax = fmax(rmax, fabs(x.u[p] - val));
           x.u[p] = val;
         }
         if (j < ny - 1 && is_bnd(c.rt_v[p])) {
           const uint8_t rt = c.rt_v[p];
-          double val = 0.0;
+          double val = rhs.v ? rhs.v[p] : 0.0;
           if (rt != RT_BND_ZERO) {
             int jo, io;
             bnd_src(rt, i, j, nx - 1, ny - 2, jo, io);
-            val = x.v[p2(m, jo, io)];
+            val += x.v[p2(m, jo, io)];
           }
           rmax = fmax(rmax, fabs(x.v[p] - val));
           x.v[p] = val;
@@ -1712,9 +1715,9 @@ void prec_tile(const Launch &L, const KP &kp, const Coef &c, const double *yu, c
                      Vec2{zu, zv}, active, nsweeps);
 }
 void sor_color(const Launch &L, const KP &kp, const Coef &c, double *xu, double *xv, int color, double omega,
-               const int *active, bool track, unsigned long long *resmax) {
+               const int *active, bool track, unsigned long long *resmax, const double *ru, const double *rv) {
   hipLaunchKernelGGL(k_sor_color, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, to_p(c), Vec2{xu, xv}, color, omega,
-                     active, track ? 1 : 0, resmax);
+                     active, track ? 1 : 0, resmax, CVec2{ru, rv});
 }
 }  // namespace imau
 
@@ -1876,7 +1879,7 @@ __device__ __forceinline__ bool cvalid(const CMember &m, int comp, int J, int I)
 // Galerkin product level l (F) -> level l+1 (L)
 __global__ void __launch_bounds__(IMAU_WG) k_galc(CLevel Fl, CLevel L, const int *__restrict__ active) {
   CTILE_SETUP
-  if (J >= cm.ny || !active[t.g]) return;
+  if (J >= cm.ny || !active[t.g] || L.depth[t.g] < L.lev) return;
   const CMember fm = Fl.mem[t.g];
   FOR_TILE_ROWS {
     const int I = t.i0 + ii;
@@ -1924,7 +1927,7 @@ __device__ __forceinline__ void crow(const CLevel &L, const CMember &m, int comp
 // one colour of Gauss-Seidel on a coarse level (x <- (b - S)/D), in place
 __global__ void __launch_bounds__(IMAU_WG) k_cgs(CLevel L, int color, const int *__restrict__ active) {
   CTILE_SETUP
-  if (J >= cm.ny || !active[t.g]) return;
+  if (J >= cm.ny || !active[t.g] || L.depth[t.g] <= L.lev) return;
   const int comp = color >> 1;
   FOR_TILE_ROWS {
     const int I = t.i0 + ii;
@@ -1941,7 +1944,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_cgs(CLevel L, int color, const int 
 // b(level l+1) = R (b - A x)(level l), coarse thread per aggregate
 __global__ void __launch_bounds__(IMAU_WG) k_crestrict(CLevel Fl, CLevel L, const int *__restrict__ active) {
   CTILE_SETUP
-  if (J >= cm.ny || !active[t.g]) return;
+  if (J >= cm.ny || !active[t.g] || Fl.depth[t.g] <= Fl.lev) return;
   const CMember fm = Fl.mem[t.g];
   FOR_TILE_ROWS {
     const int I = t.i0 + ii;
@@ -1968,7 +1971,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_cprolong(CLevel Fl, CLevel L, const
   const Tile t = Fl.tiles[blockIdx.x];
   const CMember fm = Fl.mem[t.g];
   const int j = t.j0 + (int)threadIdx.x;
-  if (j >= fm.ny || !active[t.g]) return;
+  if (j >= fm.ny || !active[t.g] || Fl.depth[t.g] <= Fl.lev) return;
   const CMember cm = L.mem[t.g];
   FOR_TILE_ROWS {
     const int i = t.i0 + ii;
@@ -2030,48 +2033,67 @@ __global__ void __launch_bounds__(IMAU_WG) k_prolong0(const Member *__restrict__
   }
 }
 
-// coarsest level: one workgroup per member, nsweeps symmetric Gauss-Seidel
-// sweeps from zero with the iterate in LDS (coarsest slabs are <= 32 x 32)
-#define IMAU_CMAX 32
-__global__ void __launch_bounds__(256) k_coarsest(CLevel L, const int *__restrict__ active, int nsweeps) {
-  __shared__ double sx[2][IMAU_CMAX * IMAU_CMAX];
+// coarsest level of each member (the level where it fits IMAU_CMAX x IMAU_CMAX):
+// one 1024-thread workgroup per member, thread q owns cell (J, I) = (q % ny, q / ny)
+// with its u and v rows held in registers and the iterate in LDS; nsweeps
+// symmetric four-colour Gauss-Seidel sweeps from zero.
+__device__ __forceinline__ void coarse_row_load(const CLevel &L, const CMember &m, int comp, int J, int I, bool valid,
+                                                double (&cf)[9], int (&nb)[9], double &b) {
+  const long long p = cp2(m, J, I);
+  const double *src = comp == 0 ? L.cu : L.cv;
+  b = valid ? (comp == 0 ? L.bu : L.bv)[p] : 0.0;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    int cc, JJ, II;
+    slot_pos(comp, k, J, I, cc, JJ, II);
+    const bool ok = valid && (k == 0 || cvalid(m, cc, JJ, II));
+    cf[k] = ok ? src[k * L.na + p] : 0.0;
+    nb[k] = ok ? cc * (IMAU_CMAX * IMAU_CMAX) + II * m.ny + JJ : 0;
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_coarsest(CLevel L, const int *__restrict__ active, int nsweeps) {
+  __shared__ double sx[2 * IMAU_CMAX * IMAU_CMAX];
   const int g = blockIdx.x;
-  if (!active[g]) return;
+  if (!active[g] || L.depth[g] != L.lev) return;
   const CMember cm = L.mem[g];
   const int n = cm.nx * cm.ny;
-  for (int q = threadIdx.x; q < IMAU_CMAX * IMAU_CMAX; q += 256) {
-    sx[0][q] = 0.0;
-    sx[1][q] = 0.0;
-  }
+  const int q = threadIdx.x;
+  const int J = q % cm.ny, I = q / cm.ny;
+  const bool in = q < n;
+  const bool vu = in && cvalid(cm, 0, J, I), vv = in && cvalid(cm, 1, J, I);
+  double cu[9], cv[9], bu, bv;
+  int nu[9], nv[9];
+  coarse_row_load(L, cm, 0, J, I, vu, cu, nu, bu);
+  coarse_row_load(L, cm, 1, J, I, vv, cv, nv, bv);
+  sx[q] = 0.0;
+  sx[IMAU_CMAX * IMAU_CMAX + q] = 0.0;
   __syncthreads();
+  const int par = (I + J) & 1;
   for (int sw = 0; sw < 2 * nsweeps; ++sw) {
+#pragma unroll
     for (int q4 = 0; q4 < 4; ++q4) {
       const int color = (sw & 1) ? 3 - q4 : q4;
-      const int comp = color >> 1;
-      for (int q = threadIdx.x; q < n; q += 256) {
-        const int J = q % cm.ny, I = q / cm.ny;
-        if (((I + J) & 1) != (color & 1) || !cvalid(cm, comp, J, I)) continue;
-        const long long p = cp2(cm, J, I);
-        const double *cf = comp == 0 ? L.cu : L.cv;
-        const double D = cf[p];
-        double S = 0.0;
-        for (int k = 1; k < 9; ++k) {
-          int cc, JJ, II;
-          slot_pos(comp, k, J, I, cc, JJ, II);
-          if (!cvalid(cm, cc, JJ, II)) continue;
-          S += cf[k * L.na + p] * sx[cc][II * cm.ny + JJ];
+      if (par == (color & 1)) {
+        if (color < 2 && vu) {
+          double S = 0.0;
+#pragma unroll
+          for (int k = 1; k < 9; ++k) S += cu[k] * sx[nu[k]];
+          sx[q] = (cu[0] != 0.0) ? (bu - S) / cu[0] : 0.0;
+        } else if (color >= 2 && vv) {
+          double S = 0.0;
+#pragma unroll
+          for (int k = 1; k < 9; ++k) S += cv[k] * sx[nv[k]];
+          sx[IMAU_CMAX * IMAU_CMAX + q] = (cv[0] != 0.0) ? (bv - S) / cv[0] : 0.0;
         }
-        const double b = (comp == 0 ? L.bu : L.bv)[p];
-        sx[comp][I * cm.ny + J] = (D != 0.0) ? (b - S) / D : 0.0;
       }
       __syncthreads();
     }
   }
-  for (int q = threadIdx.x; q < n; q += 256) {
-    const int J = q % cm.ny, I = q / cm.ny;
+  if (in) {
     const long long p = cp2(cm, J, I);
-    if (cvalid(cm, 0, J, I)) L.xu[p] = sx[0][q];
-    if (cvalid(cm, 1, J, I)) L.xv[p] = sx[1][q];
+    if (vu) L.xu[p] = sx[q];
+    if (vv) L.xv[p] = sx[IMAU_CMAX * IMAU_CMAX + q];
   }
 }
 
@@ -2105,7 +2127,7 @@ void prolong0(const Launch &F, const Coef &c, double *zu, double *zv, const CLev
   hipLaunchKernelGGL(k_prolong0, dim3(F.ntiles), wg(), 0, F.s, F.mem, F.tiles, to_p(c), Vec2{zu, zv}, L, active);
 }
 void coarsest(hipStream_t s, int ngrids, const CLevel &L, const int *active, int nsweeps) {
-  hipLaunchKernelGGL(k_coarsest, dim3(ngrids), dim3(256), 0, s, L, active, nsweeps);
+  hipLaunchKernelGGL(k_coarsest, dim3(ngrids), dim3(1024), 0, s, L, active, nsweeps);
 }
 void fill(hipStream_t s, double *p, long long n, double v) {
   int blocks = (int)std::min<long long>((n + 255) / 256, 4096);

@@ -211,33 +211,52 @@ def _fractal_field(ny, nx, rng, slope=2.0):
 
 
 def random_geometry(nx: int, ny: int, dx: float = 4e3, seed: int = 0, A=None, beta_sq=None,
-                    beta_sigma: float = 0.5, cfg=None):
-    """Seeded random geometry: Matern bed (sigma 400 m, correlation 200 km)
-    around 0 m, a smoothed dome covering ~70% of the domain with fractal
-    thickness noise (H >= 0), A log-uniform in [1e-17, 1e-16] unless given,
-    lognormal beta_sq field around 1e4 unless a scalar is given."""
+                    beta_sigma: float = 0.3, beta_scale: float = 1.0, u_slide_max: float = 1500.0,
+                    tau0: float = 5e4, cfg=None):
+    """Seeded random land-terminating ice cap on a rough bed.
+
+    * bed: Matern field (nu 3/2, sigma 400 m, correlation 200 km) around +100 m;
+    * surface: a parabolic cap s = 100 m + H0 (1 - r^2) over an ellipse with
+      semi-axes 0.42 x the domain, H0 = sqrt(2 tau0 R / (rho_i g)) (R the shorter
+      semi-axis) so the driving stress stays near tau0 whatever the domain
+      size, plus seeded fractal noise of 1% of H0; H = max(0, s - bed);
+    * no floating ice and no ice-free ocean: cells whose ice would float are
+      made ice-free and ice-free cells get a bed of at least 10 m, so every
+      margin is a land margin (ice-free ocean cells next to a marine cliff carry
+      no friction and run into the 5000 m/yr velocity limiter, which leaves a
+      nonlinear residual that cannot vanish);
+    * A log-uniform in [1e-17, 1e-16] unless given;
+    * beta_sq: beta_scale x 1e4 x exp(beta_sigma x Matern(100 km)), floored at
+      tau_d / u_slide_max^(1/3) where tau_d = rho_i g H |grad s| on the a-grid,
+      so that Weertman sliding under the local driving stress stays below
+      u_slide_max (a calibrated friction field, as a basal inversion would give).
+      A scalar beta_sq replaces the field (no floor)."""
     cfg = cfg or synthetic_config()
     rng = np.random.default_rng(seed)
     g = Grid(nx, ny, dx, xmin=-(nx - 1) / 2 * dx, ymin=-(ny - 1) / 2 * dx)
     st = IceState(g, cfg.nz)
     X, Y = np.meshgrid(g.x, g.y)
-    st["Hb_a"] = _matern_field(ny, nx, dx, 400.0, 200e3, rng) - 100.0
+    bed = _matern_field(ny, nx, dx, 400.0, 200e3, rng) + 100.0
     Rx, Ry = 0.42 * (nx - 1) * dx, 0.42 * (ny - 1) * dx
     r = np.sqrt((X / Rx) ** 2 + (Y / Ry) ** 2)
-    H0 = 2500.0 + 1000.0 * rng.random()
-    dome_h = H0 * np.maximum(0.0, 1.0 - r ** 2) ** 0.5
-    noise = 150.0 * _fractal_field(ny, nx, rng)
-    H = np.where(dome_h > 0.0, np.maximum(0.0, dome_h + noise * np.minimum(1.0, dome_h / 300.0)), 0.0)
+    H0 = np.sqrt(2.0 * tau0 * min(Rx, Ry) / (910.0 * 9.81))
+    prof = H0 * np.maximum(0.0, 1.0 - r * r)
+    surf = 100.0 + prof + 0.01 * H0 * _fractal_field(ny, nx, rng) * np.minimum(1.0, prof / 300.0)
+    H = np.where(prof > 0.0, np.maximum(0.0, surf - bed), 0.0)
+    H = np.where(H * 910.0 < -bed * 1028.0 * 1.05, 0.0, H)
+    st["Hb_a"] = np.where(H > 0.0, bed, np.maximum(bed, 10.0))
     st["Hi_a"] = H
     st["SL_a"] = 0.0
     if A is None:
         A = 10.0 ** rng.uniform(-17.0, -16.0)
     _finish(st, cfg, A)
+    field = beta_scale * 1.0e4 * np.exp(beta_sigma * _matern_field(ny, nx, dx, 1.0, 100e3, rng))
     if beta_sq is None:
-        st["beta_sq_a"] = 1.0e4 * np.exp(beta_sigma * _matern_field(ny, nx, dx, 1.0, 100e3, rng))
+        taud = 910.0 * 9.81 * st["Hi_a"] * np.hypot(st["dHs_dx_a"], st["dHs_dy_a"])
+        st["beta_sq_a"] = np.maximum(field, taud / u_slide_max ** (1.0 / 3.0))
     else:
         st["beta_sq_a"] = beta_sq
-    st.meta = dict(seed=seed, A=float(np.asarray(A).mean()))
+    st.meta = dict(seed=seed, A=float(np.asarray(A).mean()), beta_scale=beta_scale)
     return g, cfg, st
 
 
@@ -250,8 +269,7 @@ def ensemble(n_members: int = 64, nmin: int = 96, nmax: int = 512, seed: int = 0
     for m in range(n_members):
         n = int(rng.integers(nmin, nmax + 1)) | 1
         A = 10.0 ** rng.uniform(-17.0, -16.0)
-        bscale = 10.0 ** rng.uniform(3.5, 4.5)
-        g, c, st = random_geometry(n, n, dx=dx, seed=seed * 1000 + m + 1, A=A, cfg=cfg)
-        st["beta_sq_a"] = st["beta_sq_a"] * (bscale / 1.0e4)
+        bscale = 10.0 ** rng.uniform(0.0, 1.0)
+        g, c, st = random_geometry(n, n, dx=dx, seed=seed * 1000 + m + 1, A=A, beta_scale=bscale, cfg=cfg)
         members.append((g, st))
     return cfg, members

@@ -131,7 +131,8 @@ typedef struct imau_cfg {
   int32_t precision;       /* IMAU_PREC_*                                     */
   int32_t stop_rule;       /* IMAU_STOP_*                                     */
   double  nonlinear_rtol;  /* target of IMAU_STOP_RESIDUAL                    */
-  int32_t mg_levels;       /* 0 = automatic, 1 = smoother only                */
+  int32_t mg_levels;       /* KSP mode preconditioner: 0 = aggregation multigrid
+                              (automatic depth), 1 = tile Gauss-Seidel only     */
   int32_t check_interval;  /* sweeps between linear-solver convergence checks */
 } imau_cfg;
 
