@@ -12,12 +12,14 @@ solve (state reset + Picard loop until every member has
 
 The line also carries:
   roofline      the fine-level smoother sweep (5 colour launches of
-                k_sor_color): algorithmic bytes per sweep / its average
+                k_smooth0): algorithmic bytes per sweep / its average
                 duration from HIP events on the library stream;
   cpu_baseline  the CPU oracle (FP64, banded-LU linear solves, 1 thread) on a
                 bounded sample: the smallest ensemble member solved alone;
-  plain_fp64    the same workload with the plain FP64 method (no multigrid:
-                BiCGStab with tile Gauss-Seidel preconditioning).
+  fp64_multigrid, plain_fp64
+                the same workload with the FP64 multigrid method and with the
+                plain FP64 method (BiCGStab with tile Gauss-Seidel
+                preconditioning, no multigrid).
 """
 from __future__ import annotations
 
@@ -31,11 +33,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "imau-ice_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# one fine-level smoother sweep (5 colour launches of k_sor_color): read u, v,
+# one fine-level smoother sweep (5 colour launches of k_smooth0): read u, v,
 # N_a, N_b, beta_eff_cx, beta_eff_cy and the two right-hand sides, write u, v
-# (10 words per cell) plus the two row-type bytes
+# (10 words per cell, 4 bytes each in the mixed mode, 8 in FP64) plus the two
+# row-type bytes
 SWEEP_WORDS_PER_CELL = 10
 SWEEP_BYTES_EXTRA = 2
+METHODS = {
+    "mixed": dict(precision="mixed", mg_levels=0,
+                  text="BiCGStab in FP64 (operator, residuals, update) preconditioned by an FP32 aggregation-"
+                       "multigrid V-cycle; FP64 viscosity/beta; reference Picard loop"),
+    "fp64": dict(precision="fp64", mg_levels=0,
+                 text="FP64 BiCGStab preconditioned by an FP64 aggregation-multigrid V-cycle; reference Picard loop"),
+    "plain": dict(precision="fp64", mg_levels=1,
+                  text="FP64 BiCGStab preconditioned by tile Gauss-Seidel (no multigrid); reference Picard loop"),
+}
 
 
 def workload(name):
@@ -94,7 +106,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=("ensemble", "channel"), default="ensemble")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison leg")
+    ap.add_argument("--method", choices=tuple(METHODS), default="mixed")
+    ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -111,8 +124,9 @@ def main():
     ctx = solver.Context()
     prob = ctx.problem(grids, cfg, states)
 
-    # headline: FP64 with the multigrid-preconditioned Krylov solve
-    config.set_config(cfg, mg_levels=0)
+    # headline method
+    meth = METHODS[args.method]
+    config.set_config(cfg, precision=meth["precision"], mg_levels=meth["mg_levels"])
     dt = timed_solves(prob, cfg, args.steps, args.warmup)
     out = prob.download()
     its = [o.stats["picard_iterations"] for o in out]
@@ -126,34 +140,39 @@ def main():
     n_sweeps, sweep_ms = ctx.kernel_timing(False)
     avg_us = 1e3 * sweep_ms / max(n_sweeps, 1)
     cells = sum(g.nx * g.ny for g in grids)
-    bytes_per_launch = cells * (SWEEP_WORDS_PER_CELL * 8 + SWEEP_BYTES_EXTRA)
+    word = 4 if meth["precision"] == "mixed" else 8
+    bytes_per_launch = cells * (SWEEP_WORDS_PER_CELL * word + SWEEP_BYTES_EXTRA)
     achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9
 
     result = dict(
         metric="DIVA solves/s at 1e-4 rel residual: learned+FP32 vs plain FP64 GPU",
         value=nsolve * args.steps / dt, unit="solves/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
         ms_per_step=1e3 * dt / args.steps, higher_is_better=True, scaling="weak", vs_baseline=None,
-        dtype="f64", data="synthetic",
-        config=dict(workload=desc, members=nsolve, cells=cells,
-                    method="FP64, BiCGStab + aggregation-multigrid preconditioner, reference Picard loop",
-                    stop="FP64 nonlinear residual <= 1e-4 for every member"),
+        dtype="f32 preconditioner / f64 residual" if meth["precision"] == "mixed" else "f64", data="synthetic",
+        config=dict(workload=desc, members=nsolve, cells=cells, method=meth["text"],
+                    stop="FP64 nonlinear residual <= 1e-4 for every member", learned_warm_start=False),
         picard_iterations=dict(min=min(its), max=max(its), mean=float(np.mean(its))),
         preconditioner_applications=dict(min=min(apps), max=max(apps)),
         max_final_residual=max(res),
         roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                      traffic=None, kernel="k_sor_color x5 (one fine-level smoother sweep)", avg_launch_us=avg_us,
+                      traffic=None, kernel="k_smooth0 x5 (one fine-level smoother sweep)", avg_launch_us=avg_us,
                       launches=n_sweeps, bytes_per_launch=bytes_per_launch),
     )
-    if not args.no_plain:
-        config.set_config(cfg, mg_levels=1)
-        dtp = timed_solves(prob, cfg, 1, 0)
-        outp = prob.download()
-        result["plain_fp64"] = dict(value=nsolve / dtp, unit="solves/s",
-                                    method="FP64, BiCGStab + tile Gauss-Seidel preconditioner",
-                                    picard_max=max(o.stats["picard_iterations"] for o in outp),
-                                    max_final_residual=max(o.stats["nonlinear_residual"] for o in outp))
-        result["speedup_vs_plain_fp64"] = result["value"] / result["plain_fp64"]["value"]
-        config.set_config(cfg, mg_levels=0)
+    if not args.no_compare:
+        for key in ("fp64", "plain"):
+            if key == args.method:
+                continue
+            m2 = METHODS[key]
+            config.set_config(cfg, precision=m2["precision"], mg_levels=m2["mg_levels"])
+            dtp = timed_solves(prob, cfg, 1, 0)
+            outp = prob.download()
+            name = "plain_fp64" if key == "plain" else "fp64_multigrid"
+            result[name] = dict(value=nsolve / dtp, unit="solves/s", method=m2["text"],
+                                picard_max=max(o.stats["picard_iterations"] for o in outp),
+                                max_final_residual=max(o.stats["nonlinear_residual"] for o in outp))
+        if "plain_fp64" in result:
+            result["speedup_vs_plain_fp64"] = result["value"] / result["plain_fp64"]["value"]
+        config.set_config(cfg, precision=meth["precision"], mg_levels=meth["mg_levels"])
     if not args.no_cpu_baseline:
         k, ref, cb = cpu_baseline(cfg, members)
         result["cpu_baseline"] = cb

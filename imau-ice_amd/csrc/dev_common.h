@@ -103,15 +103,16 @@ struct CMember {
 // 0 C, 1 U(J,I-1), 2 U(J,I+1), 3 U(J-1,I), 4 U(J+1,I), 5 V(J-1,I), 6 V(J-1,I+1),
 // 7 V(J,I), 8 V(J,I+1).  v-row slots: 0 C, 1 V(J,I-1), 2 V(J,I+1), 3 V(J-1,I),
 // 4 V(J+1,I), 5 U(J,I-1), 6 U(J,I), 7 U(J+1,I-1), 8 U(J+1,I).  cu[k*na + p].
-struct CLevel {
+template <class T>
+struct CLevelT {
   const CMember *mem;
   const Tile *tiles;
   const int *depth;  // per member: index of its coarsest level (>= 1)
   int lev;           // index of this level (1 = first coarse level)
   int ntiles;
   long long na;
-  double *cu, *cv;
-  double *xu, *xv, *bu, *bv;
+  T *cu, *cv;        // 9 coefficient planes per component: c[k * na + p]
+  T *xu, *xv, *bu, *bv;
 };
 
 // buffers of the final (once per solve) phase

@@ -17,6 +17,7 @@
 #include "../../include/imau_vel.h"
 #include "dev_common.h"
 #include "kernels.h"
+#include "mg.h"
 
 namespace {
 
@@ -74,8 +75,7 @@ struct MGLevel {
   long long na = 0;
   DBuf<CMember> mem_d;
   DBuf<Tile> tiles_d;
-  DBuf<double> data;  // [cu 9na][cv 9na][xu na][xv na][bu na][bv na]
-  CLevel L{};
+  DBuf<double> data;  // [cu 9na][cv 9na][xu na][xv na][bu na][bv na], FP64 or FP32 elements
 };
 
 }  // namespace
@@ -105,7 +105,8 @@ struct imau_problem {
   DBuf<double> sc;    // per-member Krylov scalars
   std::vector<std::unique_ptr<MGLevel>> mg;  // coarse levels 1.. of the multigrid preconditioner
   DBuf<int> mg_depth;                        // per member: its coarsest level
-  std::vector<char> mg_ends;                 // mg_ends[l]: some member's coarsest level is l + 1
+  DBuf<float> mg_f32;                        // FP32 fine-level coefficients, rhs and iterate
+  imau::MGPlan plan;                         // device view of the levels
 
   Fields F{};
   OutF O{};
@@ -181,25 +182,21 @@ static void build_mg(imau_problem *P) {
     lv->data.alloc((size_t)22 * off);
     HIPCHK(hipMemcpy(lv->mem_d.p, lv->mem.data(), lv->mem.size() * sizeof(CMember), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(lv->tiles_d.p, lv->tiles.data(), lv->tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
-    CLevel &L = lv->L;
-    L.mem = lv->mem_d.p;
-    L.tiles = lv->tiles_d.p;
-    L.lev = lev;
-    L.ntiles = (int)lv->tiles.size();
-    L.na = off;
-    L.cu = lv->data.p;
-    L.cv = L.cu + 9 * off;
-    L.xu = L.cv + 9 * off;
-    L.xv = L.xu + off;
-    L.bu = L.xv + off;
-    L.bv = L.bu + off;
+    P->plan.levels.push_back(imau::MGLevelDev{lv->mem_d.p, lv->tiles_d.p, (int)lv->tiles.size(), lev, off,
+                                              lv->data.p, false});
     P->mg.push_back(std::move(lv));
   }
   P->mg_depth.alloc(P->ngrids);
   HIPCHK(hipMemcpy(P->mg_depth.p, depth.data(), P->ngrids * sizeof(int), hipMemcpyHostToDevice));
-  P->mg_ends.assign(P->mg.size(), 0);
-  for (int d : depth) P->mg_ends[d - 1] = 1;
-  for (auto &lv : P->mg) lv->L.depth = P->mg_depth.p;
+  for (int d : depth) P->plan.levels[d - 1].ends = true;
+  P->plan.depth = P->mg_depth.p;
+  P->plan.ngrids = P->ngrids;
+  P->plan.coarsest_sweeps = 10;
+  const long long na = P->na;
+  P->mg_f32.alloc((size_t)10 * na);
+  float *f = P->mg_f32.p;
+  P->plan.f32 = imau::MGFine32{f, f + na, f + 2 * na, f + 3 * na, f + 4 * na,
+                               f + 5 * na, f + 6 * na, f + 7 * na, f + 8 * na, f + 9 * na};
 }
 
 // ---------------------------------------------------------------------------
@@ -849,9 +846,10 @@ struct Solver {
     }
     if (!any) return;
     push_active(P->act_lin.p, act_lin);
-    if (use_mg) mg_setup(cf);
+    const bool fp32 = cfg->precision == IMAU_PREC_MIXED;
+    if (use_mg) imau::mg_setup(L, kp, cf, P->plan, fp32, P->act_lin.p);
     auto precondition = [&](double *const *y, double *const *z) {
-      if (use_mg) mg_vcycle(cf, y[0], y[1], z[0], z[1]);
+      if (use_mg) imau::mg_vcycle(L, kp, cf, P->plan, fp32, y[0], y[1], z[0], z[1], P->act_lin.p, &Solver::mg_mark, this);
       else {
         time_begin();
         imau::prec_tile(L, kp, cf, y[0], y[1], z[0], z[1], P->act_lin.p, 1);
@@ -887,45 +885,10 @@ struct Solver {
     }
   }
 
-  // Galerkin coarse operators for the current coefficients (once per linear solve)
-  void mg_setup(const imau::Coef &cf) {
-    const int *act = P->act_lin.p;
-    imau::gal0(L, kp, cf, P->mg[0]->L, act);
-    for (size_t l = 1; l < P->mg.size(); ++l) imau::galc(c->s, P->mg[l - 1]->L, P->mg[l]->L, act);
-    kcheck();
-  }
-
-  static constexpr int kCoarsestSweeps = 10;
-
-  // z = V(y): one V-cycle from z = 0 with one multicolour Gauss-Seidel sweep
-  // before and after each coarse-grid correction (reverse colour order after);
-  // a member's coarsest level is solved by kCoarsestSweeps symmetric sweeps
-  void mg_vcycle(const imau::Coef &cf, const double *yu, const double *yv, double *zu, double *zv) {
-    const int *act = P->act_lin.p;
-    hipStream_t s = c->s;
-    const int nl = (int)P->mg.size();
-    imau::fill(s, zu, P->na, 0.0);
-    imau::fill(s, zv, P->na, 0.0);
-    time_begin();
-    for (int col = 0; col < 5; ++col) imau::sor_color(L, kp, cf, zu, zv, col, 1.0, act, false, nullptr, yu, yv);
-    time_end();
-    imau::restrict0(L, kp, cf, yu, yv, zu, zv, P->mg[0]->L, act);
-    for (int l = 0; l < nl; ++l) {
-      const CLevel &C = P->mg[l]->L;
-      imau::fill(s, C.xu, 2 * C.na, 0.0);
-      if (P->mg_ends[l]) imau::coarsest(s, P->ngrids, C, act, kCoarsestSweeps);
-      if (l + 1 == nl) break;
-      for (int col = 0; col < 4; ++col) imau::cgs(s, C, col, act);
-      imau::crestrict(s, C, P->mg[l + 1]->L, act);
-    }
-    for (int l = nl - 2; l >= 0; --l) {
-      const CLevel &C = P->mg[l]->L;
-      imau::cprolong(s, C, P->mg[l + 1]->L, act);
-      for (int col = 3; col >= 0; --col) imau::cgs(s, C, col, act);
-    }
-    imau::prolong0(L, cf, zu, zv, P->mg[0]->L, act);
-    for (int col = 3; col >= 0; --col) imau::sor_color(L, kp, cf, zu, zv, col, 1.0, act, false, nullptr, yu, yv);
-    imau::sor_color(L, kp, cf, zu, zv, 4, 1.0, act, false, nullptr, yu, yv);
+  static void mg_mark(void *self, int which) {
+    Solver *S = static_cast<Solver *>(self);
+    if (which == 0) S->time_begin();
+    else S->time_end();
   }
 
   int sor_nit = 10000;

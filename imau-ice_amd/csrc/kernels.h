@@ -2,7 +2,6 @@
 #pragma once
 #include "dev_common.h"
 
-struct CoefP;
 namespace imau {
 
 struct Launch {
@@ -62,15 +61,4 @@ void prec_tile(const Launch &L, const KP &kp, const Coef &c, const double *yu, c
 void sor_color(const Launch &L, const KP &kp, const Coef &c, double *xu, double *xv, int color, double omega,
                const int *active, bool track, unsigned long long *resmax, const double *ru = nullptr,
                const double *rv = nullptr);
-// aggregation multigrid
-void gal0(const Launch &F, const KP &kp, const Coef &c, const CLevel &L, const int *active);
-void galc(hipStream_t s, const CLevel &Fl, const CLevel &L, const int *active);
-void cgs(hipStream_t s, const CLevel &L, int color, const int *active);
-void crestrict(hipStream_t s, const CLevel &Fl, const CLevel &L, const int *active);
-void cprolong(hipStream_t s, const CLevel &Fl, const CLevel &L, const int *active);
-void restrict0(const Launch &F, const KP &kp, const Coef &c, const double *yu, const double *yv, const double *zu,
-               const double *zv, const CLevel &L, const int *active);
-void prolong0(const Launch &F, const Coef &c, double *zu, double *zv, const CLevel &L, const int *active);
-void coarsest(hipStream_t s, int ngrids, const CLevel &L, const int *active, int nsweeps);
-void fill(hipStream_t s, double *p, long long n, double v);
 }  // namespace imau

@@ -6,6 +6,7 @@
 // covers one grid or an ensemble of uneven grids.
 #include "dev_common.h"
 #include "kernels.h"
+#include "stencil.h"
 
 #define TILE_SETUP                                  \
   const Tile t = tiles[blockIdx.x];                 \
@@ -537,10 +538,8 @@ __global__ void __launch_bounds__(IMAU_WG) k_coef(const Member *__restrict__ mem
 // matrix-free.  For the unknown at (j,i) they return the off-diagonal sum S,
 // the diagonal D and the right-hand side b, so that the row reads S + D x = b.
 // ---------------------------------------------------------------------------
-struct CoefP {
-  const double *Na, *Nb, *Ncx, *Ncy, *bcx, *bcy, *taudx, *taudy, *frhs;
-  const uint8_t *rt_u, *rt_v;
-};
+using CoefP = CoefT<double>;
+using GAcc = GAccT<double>;
 
 // LDS tile of the sweep (u in [0], v in [1]) with a one-cell halo
 __shared__ double g_sx[2][IMAU_TI + 2][IMAU_TJ + 2];
@@ -553,103 +552,34 @@ struct LAcc {
   __device__ __forceinline__ double operator()(int jj, int ii) const { return g_sx[COMP][ii - i0 + 1][jj - j0 + 1]; }
 };
 
-// source cell of a domain-boundary row (src/ice_velocity_module.f90:2006-2030)
-__device__ __forceinline__ void bnd_src(uint8_t rt, int i, int j, int imax, int jmax, int &jo, int &io) {
-  if (rt == RT_BND_INF) {
-    io = (i == 0) ? 1 : (i == imax ? imax - 1 : i);
-    jo = (j == 0) ? 1 : (j == jmax ? jmax - 1 : j);
-  } else {
-    io = (i == 0) ? imax - 1 : (i == imax ? 1 : i);
-    jo = (j == 0) ? jmax - 1 : (j == jmax ? 1 : j);
-  }
-}
-
 template <class AU, class AV>
 __device__ __forceinline__ void u_row(const Member &m, const KP &kp, const CoefP &c, int j, int i, uint8_t rt,
                                       const AU &U, const AV &V, double &S, double &D, double &b) {
-  const int nx = m.nx, ny = m.ny;
+  u_sd<double>(m, kp, c, j, i, rt, U, V, S, D);
   if (rt == RT_FREE) {
     const double dx2 = m.dx * m.dx;
-    if (kp.cross) {
-      const double Na0 = c.Na[p2(m, j, i)], Na1 = c.Na[p2(m, j, i + 1)];
-      const double Nb0 = c.Nb[p2(m, j, i)], Nbm = c.Nb[p2(m, j - 1, i)];
-      S = 4.0 * Na1 * U(j, i + 1) + 4.0 * Na0 * U(j, i - 1) + Nb0 * U(j + 1, i) + Nbm * U(j - 1, i) +
-          (-2.0 * Na0 - Nb0) * V(j, i) + (2.0 * Na1 + Nb0) * V(j, i + 1) + (2.0 * Na0 + Nbm) * V(j - 1, i) +
-          (-2.0 * Na1 - Nbm) * V(j - 1, i + 1);
-      D = -4.0 * (Na1 + Na0) + -1.0 * (Nb0 + Nbm) - c.bcx[p2(m, j, i)] * dx2;
-      b = -c.taudx[p2(m, j, i)] * dx2;
-    } else {
-      const double Ncx = c.Ncx[p2(m, j, i)];
-      S = 4.0 * U(j, i + 1) + 4.0 * U(j, i - 1) + U(j + 1, i) + U(j - 1, i) - 3.0 * V(j, i) + 3.0 * V(j, i + 1) +
-          3.0 * V(j - 1, i) - 3.0 * V(j - 1, i + 1);
-      D = -10.0 - dx2 * c.bcx[p2(m, j, i)] / Ncx;
-      b = -c.taudx[p2(m, j, i)] * dx2 / Ncx;
-    }
-    return;
+    b = kp.cross ? -c.taudx[p2(m, j, i)] * dx2 : -c.taudx[p2(m, j, i)] * dx2 / c.Ncx[p2(m, j, i)];
+  } else if (rt == RT_MARGIN_I || rt == RT_MARGIN_O) {
+    b = c.frhs[p2(m, j, (rt == RT_MARGIN_I) ? i : i + 1)];
+  } else {
+    b = 0.0;
   }
-  if (rt == RT_ZERO || rt == RT_BND_ZERO) { S = 0.0; D = 1.0; b = 0.0; return; }
-  if (rt == RT_BND_INF || rt == RT_BND_PER) {
-    int io_, jo_;
-    bnd_src(rt, i, j, nx - 2, ny - 1, jo_, io_);
-    S = -U(jo_, io_); D = 1.0; b = 0.0;
-    return;
-  }
-  // calving-front stress row, :2192-2266
-  const int i1 = (rt == RT_MARGIN_I) ? i : i + 1;
-  const double Na = c.Na[p2(m, j, i1)] / m.dx;
-  const double cl = -4.0 * Na, cr = 4.0 * Na;
-  const double sv = -2.0 * Na * V(j - 1, i1) + 2.0 * Na * V(j, i1);
-  if (i1 == i) { S = cl * U(j, i - 1) + sv; D = cr; }
-  else { S = cr * U(j, i + 1) + sv; D = cl; }
-  b = c.frhs[p2(m, j, i1)];
 }
 
 template <class AU, class AV>
 __device__ __forceinline__ void v_row(const Member &m, const KP &kp, const CoefP &c, int j, int i, uint8_t rt,
                                       const AU &U, const AV &V, double &S, double &D, double &b) {
-  const int nx = m.nx, ny = m.ny;
+  v_sd<double>(m, kp, c, j, i, rt, U, V, S, D);
   if (rt == RT_FREE) {
     const double dx2 = m.dx * m.dx;
-    if (kp.cross) {
-      const double Na0 = c.Na[p2(m, j, i)], Na1 = c.Na[p2(m, j + 1, i)];
-      const double Nb0 = c.Nb[p2(m, j, i)], Nbm = c.Nb[p2(m, j, i - 1)];
-      S = 4.0 * Na1 * V(j + 1, i) + 4.0 * Na0 * V(j - 1, i) + Nb0 * V(j, i + 1) + Nbm * V(j, i - 1) +
-          (-2.0 * Na1 - Nbm) * U(j + 1, i - 1) + (2.0 * Na1 + Nb0) * U(j + 1, i) + (2.0 * Na0 + Nbm) * U(j, i - 1) +
-          (-2.0 * Na0 - Nb0) * U(j, i);
-      D = -4.0 * (Na1 + Na0) + -1.0 * (Nb0 + Nbm) - c.bcy[p2(m, j, i)] * dx2;
-      b = -c.taudy[p2(m, j, i)] * dx2;
-    } else {
-      const double Ncy = c.Ncy[p2(m, j, i)];
-      S = 4.0 * V(j + 1, i) + 4.0 * V(j - 1, i) + V(j, i + 1) + V(j, i - 1) - 3.0 * U(j + 1, i - 1) + 3.0 * U(j + 1, i) +
-          3.0 * U(j, i - 1) - 3.0 * U(j, i);
-      D = -10.0 - dx2 * c.bcy[p2(m, j, i)] / Ncy;
-      b = -c.taudy[p2(m, j, i)] * dx2 / Ncy;
-    }
-    return;
+    b = kp.cross ? -c.taudy[p2(m, j, i)] * dx2 : -c.taudy[p2(m, j, i)] * dx2 / c.Ncy[p2(m, j, i)];
+  } else if (rt == RT_MARGIN_I || rt == RT_MARGIN_O) {
+    b = c.frhs[p2(m, (rt == RT_MARGIN_I) ? j : j + 1, i)];
+  } else {
+    b = 0.0;
   }
-  if (rt == RT_ZERO || rt == RT_BND_ZERO) { S = 0.0; D = 1.0; b = 0.0; return; }
-  if (rt == RT_BND_INF || rt == RT_BND_PER) {
-    int io_, jo_;
-    bnd_src(rt, i, j, nx - 1, ny - 2, jo_, io_);
-    S = -V(jo_, io_); D = 1.0; b = 0.0;
-    return;
-  }
-  const int j1 = (rt == RT_MARGIN_I) ? j : j + 1;
-  const double Na = c.Na[p2(m, j1, i)] / m.dx;
-  const double cl = -4.0 * Na, cr = 4.0 * Na;
-  const double su = -2.0 * Na * U(j1, i - 1) + 2.0 * Na * U(j1, i);
-  if (j1 == j) { S = cl * V(j - 1, i) + su; D = cr; }
-  else { S = cr * V(j + 1, i) + su; D = cl; }
-  b = c.frhs[p2(m, j1, i)];
 }
 
-struct GAcc {  // global-memory accessor
-  const double *x;
-  Member m;
-  __device__ __forceinline__ double operator()(int jj, int ii) const { return x[p2(m, jj, ii)]; }
-};
-
-__device__ __forceinline__ bool is_bnd(uint8_t rt) { return rt == RT_BND_INF || rt == RT_BND_ZERO || rt == RT_BND_PER; }
 
 // ---------------------------------------------------------------------------
 // One SOR sweep over every tile: four colour phases (u red, u black, v red,
@@ -1718,420 +1648,5 @@ void sor_color(const Launch &L, const KP &kp, const Coef &c, double *xu, double 
                const int *active, bool track, unsigned long long *resmax, const double *ru, const double *rv) {
   hipLaunchKernelGGL(k_sor_color, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, to_p(c), Vec2{xu, xv}, color, omega,
                      active, track ? 1 : 0, resmax, CVec2{ru, rv});
-}
-}  // namespace imau
-
-// ===========================================================================
-// Aggregation multigrid preconditioner.  Level 0 is the matrix-free fine
-// operator; coarse levels hold explicit staggered 9-point stencils built on
-// the device as Galerkin products R A P, with P piecewise constant on 2 x 2
-// aggregates of each component (and 'infinite' boundary unknowns folded into
-// the aggregate of their interior source), R = P^T over the non-boundary rows.
-// ===========================================================================
-__device__ __forceinline__ int agg_slot(int rowc, int colc, int dJ, int dI) {
-  if (rowc == colc) {
-    if (dJ == 0 && dI == 0) return 0;
-    if (dJ == 0 && dI == -1) return 1;
-    if (dJ == 0 && dI == 1) return 2;
-    if (dJ == -1 && dI == 0) return 3;
-    if (dJ == 1 && dI == 0) return 4;
-    return -1;
-  }
-  if (rowc == 0) {  // u row, v column
-    if (dJ == -1 && dI == 0) return 5;
-    if (dJ == -1 && dI == 1) return 6;
-    if (dJ == 0 && dI == 0) return 7;
-    if (dJ == 0 && dI == 1) return 8;
-    return -1;
-  }
-  // v row, u column
-  if (dJ == 0 && dI == -1) return 5;
-  if (dJ == 0 && dI == 0) return 6;
-  if (dJ == 1 && dI == -1) return 7;
-  if (dJ == 1 && dI == 0) return 8;
-  return -1;
-}
-
-// nonzeros of the fine u-row (free or calving-front) at (j,i): f(comp, jj, ii, value)
-template <class F>
-__device__ __forceinline__ void u_couplings(const Member &m, const KP &kp, const CoefP &c, int j, int i, uint8_t rt,
-                                            F &&f) {
-  const double dx2 = m.dx * m.dx;
-  if (rt == RT_FREE) {
-    if (kp.cross) {
-      const double Na0 = c.Na[p2(m, j, i)], Na1 = c.Na[p2(m, j, i + 1)];
-      const double Nb0 = c.Nb[p2(m, j, i)], Nbm = c.Nb[p2(m, j - 1, i)];
-      f(0, j, i + 1, 4.0 * Na1); f(0, j, i - 1, 4.0 * Na0); f(0, j + 1, i, Nb0); f(0, j - 1, i, Nbm);
-      f(0, j, i, -4.0 * (Na1 + Na0) + -1.0 * (Nb0 + Nbm) - c.bcx[p2(m, j, i)] * dx2);
-      f(1, j, i, -2.0 * Na0 - Nb0); f(1, j, i + 1, 2.0 * Na1 + Nb0);
-      f(1, j - 1, i, 2.0 * Na0 + Nbm); f(1, j - 1, i + 1, -2.0 * Na1 - Nbm);
-    } else {
-      const double Ncx = c.Ncx[p2(m, j, i)];
-      f(0, j, i + 1, 4.0); f(0, j, i - 1, 4.0); f(0, j + 1, i, 1.0); f(0, j - 1, i, 1.0);
-      f(0, j, i, -10.0 - dx2 * c.bcx[p2(m, j, i)] / Ncx);
-      f(1, j, i, -3.0); f(1, j, i + 1, 3.0); f(1, j - 1, i, 3.0); f(1, j - 1, i + 1, -3.0);
-    }
-  } else {  // RT_MARGIN_I / RT_MARGIN_O
-    const int i1 = (rt == RT_MARGIN_I) ? i : i + 1;
-    const double Na = c.Na[p2(m, j, i1)] / m.dx;
-    f(0, j, i1 - 1, -4.0 * Na); f(0, j, i1, 4.0 * Na); f(1, j - 1, i1, -2.0 * Na); f(1, j, i1, 2.0 * Na);
-  }
-}
-template <class F>
-__device__ __forceinline__ void v_couplings(const Member &m, const KP &kp, const CoefP &c, int j, int i, uint8_t rt,
-                                            F &&f) {
-  const double dx2 = m.dx * m.dx;
-  if (rt == RT_FREE) {
-    if (kp.cross) {
-      const double Na0 = c.Na[p2(m, j, i)], Na1 = c.Na[p2(m, j + 1, i)];
-      const double Nb0 = c.Nb[p2(m, j, i)], Nbm = c.Nb[p2(m, j, i - 1)];
-      f(1, j + 1, i, 4.0 * Na1); f(1, j - 1, i, 4.0 * Na0); f(1, j, i + 1, Nb0); f(1, j, i - 1, Nbm);
-      f(1, j, i, -4.0 * (Na1 + Na0) + -1.0 * (Nb0 + Nbm) - c.bcy[p2(m, j, i)] * dx2);
-      f(0, j + 1, i - 1, -2.0 * Na1 - Nbm); f(0, j + 1, i, 2.0 * Na1 + Nb0);
-      f(0, j, i - 1, 2.0 * Na0 + Nbm); f(0, j, i, -2.0 * Na0 - Nb0);
-    } else {
-      const double Ncy = c.Ncy[p2(m, j, i)];
-      f(1, j + 1, i, 4.0); f(1, j - 1, i, 4.0); f(1, j, i + 1, 1.0); f(1, j, i - 1, 1.0);
-      f(1, j, i, -10.0 - dx2 * c.bcy[p2(m, j, i)] / Ncy);
-      f(0, j + 1, i - 1, -3.0); f(0, j + 1, i, 3.0); f(0, j, i - 1, 3.0); f(0, j, i, -3.0);
-    }
-  } else {
-    const int j1 = (rt == RT_MARGIN_I) ? j : j + 1;
-    const double Na = c.Na[p2(m, j1, i)] / m.dx;
-    f(1, j1 - 1, i, -4.0 * Na); f(1, j1, i, 4.0 * Na); f(0, j1, i - 1, -2.0 * Na); f(0, j1, i, 2.0 * Na);
-  }
-}
-
-// coarse aggregate of the fine unknown (comp, jj, ii); false if it has no coarse counterpart
-__device__ __forceinline__ bool fine_agg(const Member &m, const CoefP &c, int comp, int jj, int ii, int &J, int &I) {
-  uint8_t rt = comp == 0 ? c.rt_u[p2(m, jj, ii)] : c.rt_v[p2(m, jj, ii)];
-  if (rt == RT_BND_INF) {
-    int jo, io;
-    bnd_src(rt, ii, jj, comp == 0 ? m.nx - 2 : m.nx - 1, comp == 0 ? m.ny - 1 : m.ny - 2, jo, io);
-    jj = jo;
-    ii = io;
-    rt = comp == 0 ? c.rt_u[p2(m, jj, ii)] : c.rt_v[p2(m, jj, ii)];
-  }
-  if (rt != RT_FREE && rt != RT_MARGIN_I && rt != RT_MARGIN_O) return false;
-  J = jj >> 1;
-  I = ii >> 1;
-  return true;
-}
-
-__device__ __forceinline__ bool in_row(uint8_t rt) { return rt == RT_FREE || rt == RT_MARGIN_I || rt == RT_MARGIN_O; }
-
-#define CTILE_SETUP                                  \
-  const Tile t = L.tiles[blockIdx.x];                \
-  const CMember cm = L.mem[t.g];                     \
-  const int J = t.j0 + (int)threadIdx.x;
-
-__device__ __forceinline__ long long cp2(const CMember &m, int J, int I) { return m.off + (long long)I * m.ny + J; }
-
-// Galerkin product fine (level 0) -> level 1
-__global__ void __launch_bounds__(IMAU_WG) k_gal0(const Member *__restrict__ mem, const KP kp, const CoefP c, CLevel L,
-                                                  const int *__restrict__ active) {
-  CTILE_SETUP
-  if (J >= cm.ny || !active[t.g]) return;
-  const Member m = mem[t.g];
-  FOR_TILE_ROWS {
-    const int I = t.i0 + ii;
-    if (I >= cm.nx) break;
-    const long long pc = cp2(cm, J, I);
-    for (int comp = 0; comp < 2; ++comp) {
-      const bool valid = comp == 0 ? (J < cm.nyu && I < cm.nxu) : (J < cm.nyv && I < cm.nxv);
-      if (!valid) continue;
-      double cf[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-      for (int dj = 0; dj < 2; ++dj)
-        for (int di = 0; di < 2; ++di) {
-          const int j = 2 * J + dj, i = 2 * I + di;
-          if (comp == 0 && (i > m.nx - 2 || j > m.ny - 1)) continue;
-          if (comp == 1 && (i > m.nx - 1 || j > m.ny - 2)) continue;
-          const uint8_t rt = comp == 0 ? c.rt_u[p2(m, j, i)] : c.rt_v[p2(m, j, i)];
-          if (!in_row(rt)) continue;
-          auto acc = [&](int cc, int jj, int i2, double val) {
-            int Jc, Ic;
-            if (!fine_agg(m, c, cc, jj, i2, Jc, Ic)) return;
-            const int s = agg_slot(comp, cc, Jc - J, Ic - I);
-            if (s >= 0) cf[s] += val;
-          };
-          if (comp == 0) u_couplings(m, kp, c, j, i, rt, acc);
-          else v_couplings(m, kp, c, j, i, rt, acc);
-        }
-      double *dst = comp == 0 ? L.cu : L.cv;
-      for (int k = 0; k < 9; ++k) dst[k * L.na + pc] = cf[k];
-    }
-  }
-}
-
-// neighbour position of coarse slot k for a row of component rowc at (J,I)
-__device__ __forceinline__ void slot_pos(int rowc, int k, int J, int I, int &cc, int &JJ, int &II) {
-  const int du[9][3] = {{0, 0, 0}, {0, 0, -1}, {0, 0, 1}, {0, -1, 0}, {0, 1, 0}, {1, -1, 0}, {1, -1, 1}, {1, 0, 0}, {1, 0, 1}};
-  const int dv[9][3] = {{1, 0, 0}, {1, 0, -1}, {1, 0, 1}, {1, -1, 0}, {1, 1, 0}, {0, 0, -1}, {0, 0, 0}, {0, 1, -1}, {0, 1, 0}};
-  const int(*d)[3] = rowc == 0 ? du : dv;
-  cc = d[k][0];
-  JJ = J + d[k][1];
-  II = I + d[k][2];
-}
-__device__ __forceinline__ bool cvalid(const CMember &m, int comp, int J, int I) {
-  return comp == 0 ? (J >= 0 && I >= 0 && J < m.nyu && I < m.nxu) : (J >= 0 && I >= 0 && J < m.nyv && I < m.nxv);
-}
-
-// Galerkin product level l (F) -> level l+1 (L)
-__global__ void __launch_bounds__(IMAU_WG) k_galc(CLevel Fl, CLevel L, const int *__restrict__ active) {
-  CTILE_SETUP
-  if (J >= cm.ny || !active[t.g] || L.depth[t.g] < L.lev) return;
-  const CMember fm = Fl.mem[t.g];
-  FOR_TILE_ROWS {
-    const int I = t.i0 + ii;
-    if (I >= cm.nx) break;
-    const long long pc = cp2(cm, J, I);
-    for (int comp = 0; comp < 2; ++comp) {
-      if (!cvalid(cm, comp, J, I)) continue;
-      double cf[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-      for (int dj = 0; dj < 2; ++dj)
-        for (int di = 0; di < 2; ++di) {
-          const int j = 2 * J + dj, i = 2 * I + di;
-          if (!cvalid(fm, comp, j, i)) continue;
-          const long long pf = cp2(fm, j, i);
-          const double *src = comp == 0 ? Fl.cu : Fl.cv;
-          for (int k = 0; k < 9; ++k) {
-            int cc, jj, i2;
-            slot_pos(comp, k, j, i, cc, jj, i2);
-            if (!cvalid(fm, cc, jj, i2)) continue;
-            const int s = agg_slot(comp, cc, (jj >> 1) - J, (i2 >> 1) - I);
-            if (s >= 0) cf[s] += src[k * Fl.na + pf];
-          }
-        }
-      double *dst = comp == 0 ? L.cu : L.cv;
-      for (int k = 0; k < 9; ++k) dst[k * L.na + pc] = cf[k];
-    }
-  }
-}
-
-// row of a coarse level: off-diagonal sum S and diagonal D at (J,I)
-__device__ __forceinline__ void crow(const CLevel &L, const CMember &m, int comp, int J, int I, const double *xu,
-                                     const double *xv, double &S, double &D) {
-  const long long p = cp2(m, J, I);
-  const double *cf = comp == 0 ? L.cu : L.cv;
-  D = cf[p];
-  S = 0.0;
-#pragma unroll
-  for (int k = 1; k < 9; ++k) {
-    int cc, JJ, II;
-    slot_pos(comp, k, J, I, cc, JJ, II);
-    if (!cvalid(m, cc, JJ, II)) continue;
-    S += cf[k * L.na + p] * (cc == 0 ? xu : xv)[cp2(m, JJ, II)];
-  }
-}
-
-// one colour of Gauss-Seidel on a coarse level (x <- (b - S)/D), in place
-__global__ void __launch_bounds__(IMAU_WG) k_cgs(CLevel L, int color, const int *__restrict__ active) {
-  CTILE_SETUP
-  if (J >= cm.ny || !active[t.g] || L.depth[t.g] <= L.lev) return;
-  const int comp = color >> 1;
-  FOR_TILE_ROWS {
-    const int I = t.i0 + ii;
-    if (I >= cm.nx) break;
-    if (((I + J) & 1) != (color & 1) || !cvalid(cm, comp, J, I)) continue;
-    double S, D;
-    crow(L, cm, comp, J, I, L.xu, L.xv, S, D);
-    const long long p = cp2(cm, J, I);
-    const double b = (comp == 0 ? L.bu : L.bv)[p];
-    (comp == 0 ? L.xu : L.xv)[p] = (D != 0.0) ? (b - S) / D : 0.0;
-  }
-}
-
-// b(level l+1) = R (b - A x)(level l), coarse thread per aggregate
-__global__ void __launch_bounds__(IMAU_WG) k_crestrict(CLevel Fl, CLevel L, const int *__restrict__ active) {
-  CTILE_SETUP
-  if (J >= cm.ny || !active[t.g] || Fl.depth[t.g] <= Fl.lev) return;
-  const CMember fm = Fl.mem[t.g];
-  FOR_TILE_ROWS {
-    const int I = t.i0 + ii;
-    if (I >= cm.nx) break;
-    for (int comp = 0; comp < 2; ++comp) {
-      if (!cvalid(cm, comp, J, I)) continue;
-      double s = 0.0;
-      for (int dj = 0; dj < 2; ++dj)
-        for (int di = 0; di < 2; ++di) {
-          const int j = 2 * J + dj, i = 2 * I + di;
-          if (!cvalid(fm, comp, j, i)) continue;
-          double S, D;
-          crow(Fl, fm, comp, j, i, Fl.xu, Fl.xv, S, D);
-          const long long pf = cp2(fm, j, i);
-          s += (comp == 0 ? Fl.bu : Fl.bv)[pf] - S - D * (comp == 0 ? Fl.xu : Fl.xv)[pf];
-        }
-      (comp == 0 ? L.bu : L.bv)[cp2(cm, J, I)] = s;
-    }
-  }
-}
-
-// x(level l) += P x(level l+1), fine thread per cell
-__global__ void __launch_bounds__(IMAU_WG) k_cprolong(CLevel Fl, CLevel L, const int *__restrict__ active) {
-  const Tile t = Fl.tiles[blockIdx.x];
-  const CMember fm = Fl.mem[t.g];
-  const int j = t.j0 + (int)threadIdx.x;
-  if (j >= fm.ny || !active[t.g] || Fl.depth[t.g] <= Fl.lev) return;
-  const CMember cm = L.mem[t.g];
-  FOR_TILE_ROWS {
-    const int i = t.i0 + ii;
-    if (i >= fm.nx) break;
-    const long long pf = cp2(fm, j, i), pc = cp2(cm, j >> 1, i >> 1);
-    if (cvalid(fm, 0, j, i)) Fl.xu[pf] += L.xu[pc];
-    if (cvalid(fm, 1, j, i)) Fl.xv[pf] += L.xv[pc];
-  }
-}
-
-// level-0 -> level-1 restriction of the residual y - A z (non-boundary rows)
-__global__ void __launch_bounds__(IMAU_WG) k_restrict0(const Member *__restrict__ mem, const KP kp, const CoefP c,
-                                                       CVec2 y, CVec2 z, CLevel L, const int *__restrict__ active) {
-  CTILE_SETUP
-  if (J >= cm.ny || !active[t.g]) return;
-  const Member m = mem[t.g];
-  const GAcc U{z.u, m}, V{z.v, m};
-  FOR_TILE_ROWS {
-    const int I = t.i0 + ii;
-    if (I >= cm.nx) break;
-    for (int comp = 0; comp < 2; ++comp) {
-      if (!cvalid(cm, comp, J, I)) continue;
-      double s = 0.0;
-      for (int dj = 0; dj < 2; ++dj)
-        for (int di = 0; di < 2; ++di) {
-          const int j = 2 * J + dj, i = 2 * I + di;
-          if (comp == 0 && (i > m.nx - 2 || j > m.ny - 1)) continue;
-          if (comp == 1 && (i > m.nx - 1 || j > m.ny - 2)) continue;
-          const long long p = p2(m, j, i);
-          const uint8_t rt = comp == 0 ? c.rt_u[p] : c.rt_v[p];
-          if (!in_row(rt)) continue;
-          double S, D, b;
-          if (comp == 0) {
-            u_row(m, kp, c, j, i, rt, U, V, S, D, b);
-            s += y.u[p] - S - D * z.u[p];
-          } else {
-            v_row(m, kp, c, j, i, rt, U, V, S, D, b);
-            s += y.v[p] - S - D * z.v[p];
-          }
-        }
-      (comp == 0 ? L.bu : L.bv)[cp2(cm, J, I)] = s;
-    }
-  }
-}
-
-// z += P x(level 1) on the fine non-boundary rows and the 'infinite' boundary rows
-__global__ void __launch_bounds__(IMAU_WG) k_prolong0(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
-                                                      const CoefP c, Vec2 z, CLevel L, const int *__restrict__ active) {
-  TILE_SETUP
-  if (j >= m.ny || !active[t.g]) return;
-  const CMember cm = L.mem[t.g];
-  FOR_TILE_ROWS {
-    const int i = t.i0 + ii;
-    if (i >= m.nx) break;
-    const long long p = p2(m, j, i);
-    int Jc, Ic;
-    if (i < m.nx - 1 && fine_agg(m, c, 0, j, i, Jc, Ic)) z.u[p] += L.xu[cp2(cm, Jc, Ic)];
-    if (j < m.ny - 1 && fine_agg(m, c, 1, j, i, Jc, Ic)) z.v[p] += L.xv[cp2(cm, Jc, Ic)];
-  }
-}
-
-// coarsest level of each member (the level where it fits IMAU_CMAX x IMAU_CMAX):
-// one 1024-thread workgroup per member, thread q owns cell (J, I) = (q % ny, q / ny)
-// with its u and v rows held in registers and the iterate in LDS; nsweeps
-// symmetric four-colour Gauss-Seidel sweeps from zero.
-__device__ __forceinline__ void coarse_row_load(const CLevel &L, const CMember &m, int comp, int J, int I, bool valid,
-                                                double (&cf)[9], int (&nb)[9], double &b) {
-  const long long p = cp2(m, J, I);
-  const double *src = comp == 0 ? L.cu : L.cv;
-  b = valid ? (comp == 0 ? L.bu : L.bv)[p] : 0.0;
-#pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    int cc, JJ, II;
-    slot_pos(comp, k, J, I, cc, JJ, II);
-    const bool ok = valid && (k == 0 || cvalid(m, cc, JJ, II));
-    cf[k] = ok ? src[k * L.na + p] : 0.0;
-    nb[k] = ok ? cc * (IMAU_CMAX * IMAU_CMAX) + II * m.ny + JJ : 0;
-  }
-}
-
-__global__ void __launch_bounds__(1024) k_coarsest(CLevel L, const int *__restrict__ active, int nsweeps) {
-  __shared__ double sx[2 * IMAU_CMAX * IMAU_CMAX];
-  const int g = blockIdx.x;
-  if (!active[g] || L.depth[g] != L.lev) return;
-  const CMember cm = L.mem[g];
-  const int n = cm.nx * cm.ny;
-  const int q = threadIdx.x;
-  const int J = q % cm.ny, I = q / cm.ny;
-  const bool in = q < n;
-  const bool vu = in && cvalid(cm, 0, J, I), vv = in && cvalid(cm, 1, J, I);
-  double cu[9], cv[9], bu, bv;
-  int nu[9], nv[9];
-  coarse_row_load(L, cm, 0, J, I, vu, cu, nu, bu);
-  coarse_row_load(L, cm, 1, J, I, vv, cv, nv, bv);
-  sx[q] = 0.0;
-  sx[IMAU_CMAX * IMAU_CMAX + q] = 0.0;
-  __syncthreads();
-  const int par = (I + J) & 1;
-  for (int sw = 0; sw < 2 * nsweeps; ++sw) {
-#pragma unroll
-    for (int q4 = 0; q4 < 4; ++q4) {
-      const int color = (sw & 1) ? 3 - q4 : q4;
-      if (par == (color & 1)) {
-        if (color < 2 && vu) {
-          double S = 0.0;
-#pragma unroll
-          for (int k = 1; k < 9; ++k) S += cu[k] * sx[nu[k]];
-          sx[q] = (cu[0] != 0.0) ? (bu - S) / cu[0] : 0.0;
-        } else if (color >= 2 && vv) {
-          double S = 0.0;
-#pragma unroll
-          for (int k = 1; k < 9; ++k) S += cv[k] * sx[nv[k]];
-          sx[IMAU_CMAX * IMAU_CMAX + q] = (cv[0] != 0.0) ? (bv - S) / cv[0] : 0.0;
-        }
-      }
-      __syncthreads();
-    }
-  }
-  if (in) {
-    const long long p = cp2(cm, J, I);
-    if (vu) L.xu[p] = sx[q];
-    if (vv) L.xv[p] = sx[IMAU_CMAX * IMAU_CMAX + q];
-  }
-}
-
-__global__ void k_fill_f64(double *p, long long n, double v) {
-  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
-    p[q] = v;
-}
-
-namespace imau {
-void gal0(const Launch &F, const KP &kp, const Coef &c, const CLevel &L, const int *active) {
-  hipLaunchKernelGGL(k_gal0, dim3(L.ntiles), wg(), 0, F.s, F.mem, kp, to_p(c), L, active);
-}
-void galc(hipStream_t s, const CLevel &Fl, const CLevel &L, const int *active) {
-  hipLaunchKernelGGL(k_galc, dim3(L.ntiles), wg(), 0, s, Fl, L, active);
-}
-void cgs(hipStream_t s, const CLevel &L, int color, const int *active) {
-  hipLaunchKernelGGL(k_cgs, dim3(L.ntiles), wg(), 0, s, L, color, active);
-}
-void crestrict(hipStream_t s, const CLevel &Fl, const CLevel &L, const int *active) {
-  hipLaunchKernelGGL(k_crestrict, dim3(L.ntiles), wg(), 0, s, Fl, L, active);
-}
-void cprolong(hipStream_t s, const CLevel &Fl, const CLevel &L, const int *active) {
-  hipLaunchKernelGGL(k_cprolong, dim3(Fl.ntiles), wg(), 0, s, Fl, L, active);
-}
-void restrict0(const Launch &F, const KP &kp, const Coef &c, const double *yu, const double *yv, const double *zu,
-               const double *zv, const CLevel &L, const int *active) {
-  hipLaunchKernelGGL(k_restrict0, dim3(L.ntiles), wg(), 0, F.s, F.mem, kp, to_p(c), CVec2{yu, yv}, CVec2{zu, zv}, L,
-                     active);
-}
-void prolong0(const Launch &F, const Coef &c, double *zu, double *zv, const CLevel &L, const int *active) {
-  hipLaunchKernelGGL(k_prolong0, dim3(F.ntiles), wg(), 0, F.s, F.mem, F.tiles, to_p(c), Vec2{zu, zv}, L, active);
-}
-void coarsest(hipStream_t s, int ngrids, const CLevel &L, const int *active, int nsweeps) {
-  hipLaunchKernelGGL(k_coarsest, dim3(ngrids), dim3(1024), 0, s, L, active, nsweeps);
-}
-void fill(hipStream_t s, double *p, long long n, double v) {
-  int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
-  if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(k_fill_f64, dim3(blocks), dim3(256), 0, s, p, n, v);
 }
 }  // namespace imau

@@ -1,0 +1,50 @@
+// mg.h -- host interface of the aggregation multigrid preconditioner (mg.hip).
+//
+// The preconditioner approximates A^-1 y for the linearised staggered DIVA/SSA
+// operator by one V-cycle.  It runs either in FP64 or, for the mixed-precision
+// mode, entirely in FP32 (FP32 copies of the fine coefficients, FP32 coarse
+// stencils, FP32 iterate) while the Krylov loop around it keeps its vectors,
+// operator applications and residual norms in FP64.
+#pragma once
+#include <vector>
+
+#include "dev_common.h"
+#include "kernels.h"
+
+namespace imau {
+
+// one coarse level; data holds [cu 9na][cv 9na][xu na][xv na][bu na][bv na]
+// elements of the working precision (sized for FP64, FP32 uses the first half)
+struct MGLevelDev {
+  const CMember *mem;
+  const Tile *tiles;
+  int ntiles;
+  int lev;
+  long long na;
+  void *data;
+  bool ends;  // some member's coarsest level is this one
+};
+
+// FP32 copies of the fine-level coefficients and the FP32 right-hand side / iterate
+struct MGFine32 {
+  float *Na, *Nb, *Ncx, *Ncy, *bcx, *bcy, *yu, *yv, *zu, *zv;
+};
+
+struct MGPlan {
+  std::vector<MGLevelDev> levels;
+  const int *depth;  // per member: its coarsest level
+  int ngrids;
+  MGFine32 f32;
+  int coarsest_sweeps;
+};
+
+// marker callback around the first fine-level smoother sweep (which = 0 before, 1 after)
+typedef void (*MGMark)(void *ctx, int which);
+
+// Galerkin coarse operators for the current fine coefficients (once per linear solve)
+void mg_setup(const Launch &F, const KP &kp, const Coef &c, const MGPlan &plan, bool fp32, const int *active);
+// z = V-cycle(y) for every active member
+void mg_vcycle(const Launch &F, const KP &kp, const Coef &c, const MGPlan &plan, bool fp32, const double *yu,
+               const double *yv, double *zu, double *zv, const int *active, MGMark mark, void *mark_ctx);
+
+}  // namespace imau

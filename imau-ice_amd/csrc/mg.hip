@@ -1,0 +1,661 @@
+// mg.hip -- aggregation multigrid preconditioner for the linearised staggered
+// DIVA/SSA system (src/ice_velocity_module.f90:1762-2589), in FP64 or FP32.
+//
+// Level 0 is the matrix-free fine operator (stencil.h).  Coarse levels hold
+// explicit staggered 9-point stencils built on the device as Galerkin products
+// R A P, with P piecewise constant on 2 x 2 aggregates of each velocity
+// component ('infinite' boundary unknowns follow the aggregate of their
+// interior source; 'zero' and 'periodic' boundary unknowns get no coarse
+// counterpart) and R = P^T over the interior and calving-front rows.  A 2 x 2
+// aggregation of the staggered u (cx) and v (cy) points keeps the u-v coupling
+// pattern of the fine rows, so every coarse level has the same 9-point form.
+// Smoothing is one four-colour Gauss-Seidel sweep (u red, u black, v red,
+// v black, then the boundary rows at level 0) before and after the coarse
+// correction; each member's coarsest level (at most IMAU_CMAX x IMAU_CMAX)
+// is solved by symmetric Gauss-Seidel sweeps in LDS by one workgroup.
+#include <type_traits>
+
+#include "mg.h"
+#include "stencil.h"
+
+#define TILE_SETUP                                  \
+  const Tile t = tiles[blockIdx.x];                 \
+  const Member m = mem[t.g];                        \
+  const int j = t.j0 + (int)threadIdx.x;
+
+#define CTILE_SETUP                                  \
+  const Tile t = L.tiles[blockIdx.x];                \
+  const CMember cm = L.mem[t.g];                     \
+  const int J = t.j0 + (int)threadIdx.x;
+
+#define FOR_TILE_ROWS for (int ii = threadIdx.y; ii < IMAU_TI; ii += IMAU_WG_I)
+
+namespace {
+
+template <class T>
+struct VecT {
+  T *u, *v;
+};
+template <class T>
+struct CVecT {
+  const T *u, *v;
+};
+
+__device__ __forceinline__ long long cp2(const CMember &m, int J, int I) { return m.off + (long long)I * m.ny + J; }
+
+__device__ __forceinline__ bool cvalid(const CMember &m, int comp, int J, int I) {
+  return comp == 0 ? (J >= 0 && I >= 0 && J < m.nyu && I < m.nxu) : (J >= 0 && I >= 0 && J < m.nyv && I < m.nxv);
+}
+
+// coarse slot of the coupling from a row of component rowc to an unknown of
+// component colc at aggregate offset (dJ, dI); -1 outside the 9-point pattern
+__device__ __forceinline__ int agg_slot(int rowc, int colc, int dJ, int dI) {
+  if (rowc == colc) {
+    if (dJ == 0 && dI == 0) return 0;
+    if (dJ == 0 && dI == -1) return 1;
+    if (dJ == 0 && dI == 1) return 2;
+    if (dJ == -1 && dI == 0) return 3;
+    if (dJ == 1 && dI == 0) return 4;
+    return -1;
+  }
+  if (rowc == 0) {  // u row, v column
+    if (dJ == -1 && dI == 0) return 5;
+    if (dJ == -1 && dI == 1) return 6;
+    if (dJ == 0 && dI == 0) return 7;
+    if (dJ == 0 && dI == 1) return 8;
+    return -1;
+  }
+  if (dJ == 0 && dI == -1) return 5;  // v row, u column
+  if (dJ == 0 && dI == 0) return 6;
+  if (dJ == 1 && dI == -1) return 7;
+  if (dJ == 1 && dI == 0) return 8;
+  return -1;
+}
+
+// unknown (component cc, JJ, II) addressed by slot k of a row of component rowc at (J, I)
+__device__ __forceinline__ void slot_pos(int rowc, int k, int J, int I, int &cc, int &JJ, int &II) {
+  if (k < 5) {
+    cc = rowc;
+    JJ = J + (k == 3 ? -1 : (k == 4 ? 1 : 0));
+    II = I + (k == 1 ? -1 : (k == 2 ? 1 : 0));
+  } else if (rowc == 0) {
+    cc = 1;
+    JJ = J + (k < 7 ? -1 : 0);
+    II = I + ((k & 1) ? 0 : 1);
+  } else {
+    cc = 0;
+    JJ = J + (k < 7 ? 0 : 1);
+    II = I + ((k & 1) ? -1 : 0);
+  }
+}
+
+// nonzeros of the fine u row (interior or calving front) at (j,i): f(comp, jj, ii, value)
+template <class F>
+__device__ __forceinline__ void u_couplings(const Member &m, const KP &kp, const CoefT<double> &c, int j, int i,
+                                            uint8_t rt, F &&f) {
+  const double dx2 = m.dx * m.dx;
+  if (rt == RT_FREE) {
+    if (kp.cross) {
+      const double Na0 = c.Na[p2(m, j, i)], Na1 = c.Na[p2(m, j, i + 1)];
+      const double Nb0 = c.Nb[p2(m, j, i)], Nbm = c.Nb[p2(m, j - 1, i)];
+      f(0, j, i + 1, 4.0 * Na1); f(0, j, i - 1, 4.0 * Na0); f(0, j + 1, i, Nb0); f(0, j - 1, i, Nbm);
+      f(0, j, i, -4.0 * (Na1 + Na0) + -1.0 * (Nb0 + Nbm) - c.bcx[p2(m, j, i)] * dx2);
+      f(1, j, i, -2.0 * Na0 - Nb0); f(1, j, i + 1, 2.0 * Na1 + Nb0);
+      f(1, j - 1, i, 2.0 * Na0 + Nbm); f(1, j - 1, i + 1, -2.0 * Na1 - Nbm);
+    } else {
+      const double Ncx = c.Ncx[p2(m, j, i)];
+      f(0, j, i + 1, 4.0); f(0, j, i - 1, 4.0); f(0, j + 1, i, 1.0); f(0, j - 1, i, 1.0);
+      f(0, j, i, -10.0 - dx2 * c.bcx[p2(m, j, i)] / Ncx);
+      f(1, j, i, -3.0); f(1, j, i + 1, 3.0); f(1, j - 1, i, 3.0); f(1, j - 1, i + 1, -3.0);
+    }
+  } else {  // RT_MARGIN_I / RT_MARGIN_O
+    const int i1 = (rt == RT_MARGIN_I) ? i : i + 1;
+    const double Na = c.Na[p2(m, j, i1)] / m.dx;
+    f(0, j, i1 - 1, -4.0 * Na); f(0, j, i1, 4.0 * Na); f(1, j - 1, i1, -2.0 * Na); f(1, j, i1, 2.0 * Na);
+  }
+}
+template <class F>
+__device__ __forceinline__ void v_couplings(const Member &m, const KP &kp, const CoefT<double> &c, int j, int i,
+                                            uint8_t rt, F &&f) {
+  const double dx2 = m.dx * m.dx;
+  if (rt == RT_FREE) {
+    if (kp.cross) {
+      const double Na0 = c.Na[p2(m, j, i)], Na1 = c.Na[p2(m, j + 1, i)];
+      const double Nb0 = c.Nb[p2(m, j, i)], Nbm = c.Nb[p2(m, j, i - 1)];
+      f(1, j + 1, i, 4.0 * Na1); f(1, j - 1, i, 4.0 * Na0); f(1, j, i + 1, Nb0); f(1, j, i - 1, Nbm);
+      f(1, j, i, -4.0 * (Na1 + Na0) + -1.0 * (Nb0 + Nbm) - c.bcy[p2(m, j, i)] * dx2);
+      f(0, j + 1, i - 1, -2.0 * Na1 - Nbm); f(0, j + 1, i, 2.0 * Na1 + Nb0);
+      f(0, j, i - 1, 2.0 * Na0 + Nbm); f(0, j, i, -2.0 * Na0 - Nb0);
+    } else {
+      const double Ncy = c.Ncy[p2(m, j, i)];
+      f(1, j + 1, i, 4.0); f(1, j - 1, i, 4.0); f(1, j, i + 1, 1.0); f(1, j, i - 1, 1.0);
+      f(1, j, i, -10.0 - dx2 * c.bcy[p2(m, j, i)] / Ncy);
+      f(0, j + 1, i - 1, -3.0); f(0, j + 1, i, 3.0); f(0, j, i - 1, 3.0); f(0, j, i, -3.0);
+    }
+  } else {
+    const int j1 = (rt == RT_MARGIN_I) ? j : j + 1;
+    const double Na = c.Na[p2(m, j1, i)] / m.dx;
+    f(1, j1 - 1, i, -4.0 * Na); f(1, j1, i, 4.0 * Na); f(0, j1, i - 1, -2.0 * Na); f(0, j1, i, 2.0 * Na);
+  }
+}
+
+// level-1 aggregate of the fine unknown (comp, jj, ii); false if it has none
+__device__ __forceinline__ bool fine_agg(const Member &m, const uint8_t *rt_u, const uint8_t *rt_v, int comp, int jj,
+                                         int ii, int &J, int &I) {
+  uint8_t rt = comp == 0 ? rt_u[p2(m, jj, ii)] : rt_v[p2(m, jj, ii)];
+  if (rt == RT_BND_INF) {
+    int jo, io;
+    bnd_src(rt, ii, jj, comp == 0 ? m.nx - 2 : m.nx - 1, comp == 0 ? m.ny - 1 : m.ny - 2, jo, io);
+    jj = jo;
+    ii = io;
+    rt = comp == 0 ? rt_u[p2(m, jj, ii)] : rt_v[p2(m, jj, ii)];
+  }
+  if (!in_row(rt)) return false;
+  J = jj >> 1;
+  I = ii >> 1;
+  return true;
+}
+
+__device__ __forceinline__ bool fine_valid(const Member &m, int comp, int j, int i) {
+  return comp == 0 ? (i <= m.nx - 2 && j <= m.ny - 1) : (i <= m.nx - 1 && j <= m.ny - 2);
+}
+
+// ---------------------------------------------------------------------------
+// Galerkin products
+// ---------------------------------------------------------------------------
+// fine (level 0, FP64 coefficients) -> level 1
+template <class T>
+__global__ void __launch_bounds__(IMAU_WG) k_gal0(const Member *__restrict__ mem, const KP kp, const CoefT<double> c,
+                                                  CLevelT<T> L, const int *__restrict__ active) {
+  CTILE_SETUP
+  if (J >= cm.ny || !active[t.g]) return;
+  const Member m = mem[t.g];
+  FOR_TILE_ROWS {
+    const int I = t.i0 + ii;
+    if (I >= cm.nx) break;
+    const long long pc = cp2(cm, J, I);
+    for (int comp = 0; comp < 2; ++comp) {
+      if (!cvalid(cm, comp, J, I)) continue;
+      double cf[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int dj = 0; dj < 2; ++dj)
+        for (int di = 0; di < 2; ++di) {
+          const int j = 2 * J + dj, i = 2 * I + di;
+          if (!fine_valid(m, comp, j, i)) continue;
+          const uint8_t rt = comp == 0 ? c.rt_u[p2(m, j, i)] : c.rt_v[p2(m, j, i)];
+          if (!in_row(rt)) continue;
+          auto acc = [&](int cc, int jj, int i2, double val) {
+            int Jc, Ic;
+            if (!fine_agg(m, c.rt_u, c.rt_v, cc, jj, i2, Jc, Ic)) return;
+            const int s = agg_slot(comp, cc, Jc - J, Ic - I);
+            if (s >= 0) cf[s] += val;
+          };
+          if (comp == 0) u_couplings(m, kp, c, j, i, rt, acc);
+          else v_couplings(m, kp, c, j, i, rt, acc);
+        }
+      T *dst = comp == 0 ? L.cu : L.cv;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) dst[k * L.na + pc] = (T)cf[k];
+    }
+  }
+}
+
+// level l (Fl) -> level l+1 (L), accumulated in FP64
+template <class T>
+__global__ void __launch_bounds__(IMAU_WG) k_galc(CLevelT<T> Fl, CLevelT<T> L, const int *__restrict__ active) {
+  CTILE_SETUP
+  if (J >= cm.ny || !active[t.g] || L.depth[t.g] < L.lev) return;
+  const CMember fm = Fl.mem[t.g];
+  FOR_TILE_ROWS {
+    const int I = t.i0 + ii;
+    if (I >= cm.nx) break;
+    const long long pc = cp2(cm, J, I);
+    for (int comp = 0; comp < 2; ++comp) {
+      if (!cvalid(cm, comp, J, I)) continue;
+      double cf[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      const T *src = comp == 0 ? Fl.cu : Fl.cv;
+      for (int dj = 0; dj < 2; ++dj)
+        for (int di = 0; di < 2; ++di) {
+          const int j = 2 * J + dj, i = 2 * I + di;
+          if (!cvalid(fm, comp, j, i)) continue;
+          const long long pf = cp2(fm, j, i);
+#pragma unroll
+          for (int k = 0; k < 9; ++k) {
+            int cc, jj, i2;
+            slot_pos(comp, k, j, i, cc, jj, i2);
+            if (!cvalid(fm, cc, jj, i2)) continue;
+            const int s = agg_slot(comp, cc, (jj >> 1) - J, (i2 >> 1) - I);
+            if (s >= 0) cf[s] += (double)src[k * Fl.na + pf];
+          }
+        }
+      T *dst = comp == 0 ? L.cu : L.cv;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) dst[k * L.na + pc] = (T)cf[k];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// coarse-level smoothing and transfers
+// ---------------------------------------------------------------------------
+// off-diagonal sum S and diagonal D of a coarse row at (J,I)
+template <class T>
+__device__ __forceinline__ void crow(const CLevelT<T> &L, const CMember &m, int comp, int J, int I, const T *xu,
+                                     const T *xv, T &S, T &D) {
+  const long long p = cp2(m, J, I);
+  const T *cf = comp == 0 ? L.cu : L.cv;
+  D = cf[p];
+  S = (T)0.0;
+#pragma unroll
+  for (int k = 1; k < 9; ++k) {
+    int cc, JJ, II;
+    slot_pos(comp, k, J, I, cc, JJ, II);
+    if (!cvalid(m, cc, JJ, II)) continue;
+    S += cf[k * L.na + p] * (cc == 0 ? xu : xv)[cp2(m, JJ, II)];
+  }
+}
+
+// one colour (0 u red, 1 u black, 2 v red, 3 v black) of Gauss-Seidel, in place
+template <class T>
+__global__ void __launch_bounds__(IMAU_WG) k_cgs(CLevelT<T> L, int color, const int *__restrict__ active) {
+  CTILE_SETUP
+  if (J >= cm.ny || !active[t.g] || L.depth[t.g] <= L.lev) return;
+  const int comp = color >> 1;
+  FOR_TILE_ROWS {
+    const int I = t.i0 + ii;
+    if (I >= cm.nx) break;
+    if (((I + J) & 1) != (color & 1) || !cvalid(cm, comp, J, I)) continue;
+    T S, D;
+    crow(L, cm, comp, J, I, L.xu, L.xv, S, D);
+    const long long p = cp2(cm, J, I);
+    const T b = (comp == 0 ? L.bu : L.bv)[p];
+    (comp == 0 ? L.xu : L.xv)[p] = (D != (T)0.0) ? (b - S) / D : (T)0.0;
+  }
+}
+
+// b(l+1) = R (b - A x)(l); one thread per coarse cell
+template <class T>
+__global__ void __launch_bounds__(IMAU_WG) k_crestrict(CLevelT<T> Fl, CLevelT<T> L, const int *__restrict__ active) {
+  CTILE_SETUP
+  if (J >= cm.ny || !active[t.g] || Fl.depth[t.g] <= Fl.lev) return;
+  const CMember fm = Fl.mem[t.g];
+  FOR_TILE_ROWS {
+    const int I = t.i0 + ii;
+    if (I >= cm.nx) break;
+    for (int comp = 0; comp < 2; ++comp) {
+      if (!cvalid(cm, comp, J, I)) continue;
+      T s = (T)0.0;
+      for (int dj = 0; dj < 2; ++dj)
+        for (int di = 0; di < 2; ++di) {
+          const int j = 2 * J + dj, i = 2 * I + di;
+          if (!cvalid(fm, comp, j, i)) continue;
+          T S, D;
+          crow(Fl, fm, comp, j, i, Fl.xu, Fl.xv, S, D);
+          const long long pf = cp2(fm, j, i);
+          s += (comp == 0 ? Fl.bu : Fl.bv)[pf] - S - D * (comp == 0 ? Fl.xu : Fl.xv)[pf];
+        }
+      (comp == 0 ? L.bu : L.bv)[cp2(cm, J, I)] = s;
+    }
+  }
+}
+
+// x(l) += P x(l+1); one thread per fine cell
+template <class T>
+__global__ void __launch_bounds__(IMAU_WG) k_cprolong(CLevelT<T> Fl, CLevelT<T> L, const int *__restrict__ active) {
+  const Tile t = Fl.tiles[blockIdx.x];
+  const CMember fm = Fl.mem[t.g];
+  const int j = t.j0 + (int)threadIdx.x;
+  if (j >= fm.ny || !active[t.g] || Fl.depth[t.g] <= Fl.lev) return;
+  const CMember cm = L.mem[t.g];
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= fm.nx) break;
+    const long long pf = cp2(fm, j, i), pc = cp2(cm, j >> 1, i >> 1);
+    if (cvalid(fm, 0, j, i)) Fl.xu[pf] += L.xu[pc];
+    if (cvalid(fm, 1, j, i)) Fl.xv[pf] += L.xv[pc];
+  }
+}
+
+// the coarsest level of each member: one 1024-thread workgroup, thread q owns
+// cell (J, I) = (q % ny, q / ny) with its u and v rows in registers and the
+// iterate in LDS; nsweeps symmetric four-colour Gauss-Seidel sweeps from zero
+template <class T>
+__device__ __forceinline__ void coarse_row_load(const CLevelT<T> &L, const CMember &m, int comp, int J, int I,
+                                                bool valid, T (&cf)[9], int (&nb)[9], T &b) {
+  const long long p = cp2(m, J, I);
+  const T *src = comp == 0 ? L.cu : L.cv;
+  b = valid ? (comp == 0 ? L.bu : L.bv)[p] : (T)0.0;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    int cc, JJ, II;
+    slot_pos(comp, k, J, I, cc, JJ, II);
+    const bool ok = valid && (k == 0 || cvalid(m, cc, JJ, II));
+    cf[k] = ok ? src[k * L.na + p] : (T)0.0;
+    nb[k] = ok ? cc * (IMAU_CMAX * IMAU_CMAX) + II * m.ny + JJ : 0;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(1024) k_coarsest(CLevelT<T> L, const int *__restrict__ active, int nsweeps) {
+  __shared__ T sx[2 * IMAU_CMAX * IMAU_CMAX];
+  const int g = blockIdx.x;
+  if (!active[g] || L.depth[g] != L.lev) return;
+  const CMember cm = L.mem[g];
+  const int n = cm.nx * cm.ny;
+  const int q = threadIdx.x;
+  const int J = q % cm.ny, I = q / cm.ny;
+  const bool in = q < n;
+  const bool vu = in && cvalid(cm, 0, J, I), vv = in && cvalid(cm, 1, J, I);
+  T cu[9], cv[9], bu, bv;
+  int nu[9], nv[9];
+  coarse_row_load(L, cm, 0, J, I, vu, cu, nu, bu);
+  coarse_row_load(L, cm, 1, J, I, vv, cv, nv, bv);
+  sx[q] = (T)0.0;
+  sx[IMAU_CMAX * IMAU_CMAX + q] = (T)0.0;
+  __syncthreads();
+  const int par = (I + J) & 1;
+  for (int sw = 0; sw < 2 * nsweeps; ++sw) {
+#pragma unroll
+    for (int q4 = 0; q4 < 4; ++q4) {
+      const int color = (sw & 1) ? 3 - q4 : q4;
+      if (par == (color & 1)) {
+        if (color < 2 && vu) {
+          T S = (T)0.0;
+#pragma unroll
+          for (int k = 1; k < 9; ++k) S += cu[k] * sx[nu[k]];
+          sx[q] = (cu[0] != (T)0.0) ? (bu - S) / cu[0] : (T)0.0;
+        } else if (color >= 2 && vv) {
+          T S = (T)0.0;
+#pragma unroll
+          for (int k = 1; k < 9; ++k) S += cv[k] * sx[nv[k]];
+          sx[IMAU_CMAX * IMAU_CMAX + q] = (cv[0] != (T)0.0) ? (bv - S) / cv[0] : (T)0.0;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (in) {
+    const long long p = cp2(cm, J, I);
+    if (vu) L.xu[p] = sx[q];
+    if (vv) L.xv[p] = sx[IMAU_CMAX * IMAU_CMAX + q];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// level 0 (matrix-free fine operator)
+// ---------------------------------------------------------------------------
+// one colour of Gauss-Seidel on A z = y (colour 4: the domain-boundary rows)
+template <class T>
+__global__ void __launch_bounds__(IMAU_WG) k_smooth0(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                     const KP kp, const CoefT<T> c, VecT<T> x, CVecT<T> y, int color,
+                                                     const int *__restrict__ active) {
+  TILE_SETUP
+  if (j >= m.ny || !active[t.g]) return;
+  const GAccT<T> U{x.u, m}, V{x.v, m};
+  const int nx = m.nx, ny = m.ny;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= nx) break;
+    const long long p = p2(m, j, i);
+    if (color < 4) {
+      const bool isu = color < 2;
+      if (((i + j) & 1) != (color & 1)) continue;
+      if (isu && i >= nx - 1) continue;
+      if (!isu && j >= ny - 1) continue;
+      const uint8_t rt = isu ? c.rt_u[p] : c.rt_v[p];
+      if (is_bnd(rt)) continue;
+      T S, D;
+      if (isu) u_sd<T>(m, kp, c, j, i, rt, U, V, S, D);
+      else v_sd<T>(m, kp, c, j, i, rt, U, V, S, D);
+      (isu ? x.u : x.v)[p] = ((isu ? y.u : y.v)[p] - S) / D;
+    } else {
+      if (i < nx - 1 && is_bnd(c.rt_u[p])) {
+        const uint8_t rt = c.rt_u[p];
+        T val = y.u[p];
+        if (rt != RT_BND_ZERO) {
+          int jo, io;
+          bnd_src(rt, i, j, nx - 2, ny - 1, jo, io);
+          val += x.u[p2(m, jo, io)];
+        }
+        x.u[p] = val;
+      }
+      if (j < ny - 1 && is_bnd(c.rt_v[p])) {
+        const uint8_t rt = c.rt_v[p];
+        T val = y.v[p];
+        if (rt != RT_BND_ZERO) {
+          int jo, io;
+          bnd_src(rt, i, j, nx - 1, ny - 2, jo, io);
+          val += x.v[p2(m, jo, io)];
+        }
+        x.v[p] = val;
+      }
+    }
+  }
+}
+
+// level-1 right-hand side: R (y - A z) over the interior and calving-front rows
+template <class T>
+__global__ void __launch_bounds__(IMAU_WG) k_restrict0(const Member *__restrict__ mem, const KP kp, const CoefT<T> c,
+                                                       CVecT<T> y, CVecT<T> z, CLevelT<T> L,
+                                                       const int *__restrict__ active) {
+  CTILE_SETUP
+  if (J >= cm.ny || !active[t.g]) return;
+  const Member m = mem[t.g];
+  const GAccT<T> U{z.u, m}, V{z.v, m};
+  FOR_TILE_ROWS {
+    const int I = t.i0 + ii;
+    if (I >= cm.nx) break;
+    for (int comp = 0; comp < 2; ++comp) {
+      if (!cvalid(cm, comp, J, I)) continue;
+      T s = (T)0.0;
+      for (int dj = 0; dj < 2; ++dj)
+        for (int di = 0; di < 2; ++di) {
+          const int j = 2 * J + dj, i = 2 * I + di;
+          if (!fine_valid(m, comp, j, i)) continue;
+          const long long p = p2(m, j, i);
+          const uint8_t rt = comp == 0 ? c.rt_u[p] : c.rt_v[p];
+          if (!in_row(rt)) continue;
+          T S, D;
+          if (comp == 0) {
+            u_sd<T>(m, kp, c, j, i, rt, U, V, S, D);
+            s += y.u[p] - S - D * z.u[p];
+          } else {
+            v_sd<T>(m, kp, c, j, i, rt, U, V, S, D);
+            s += y.v[p] - S - D * z.v[p];
+          }
+        }
+      (comp == 0 ? L.bu : L.bv)[cp2(cm, J, I)] = s;
+    }
+  }
+}
+
+// z += P x(1) on the interior, calving-front and 'infinite' boundary unknowns
+template <class T>
+__global__ void __launch_bounds__(IMAU_WG) k_prolong0(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                      const uint8_t *__restrict__ rt_u, const uint8_t *__restrict__ rt_v,
+                                                      VecT<T> z, CLevelT<T> L, const int *__restrict__ active) {
+  TILE_SETUP
+  if (j >= m.ny || !active[t.g]) return;
+  const CMember cm = L.mem[t.g];
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= m.nx) break;
+    const long long p = p2(m, j, i);
+    int Jc, Ic;
+    if (i < m.nx - 1 && fine_agg(m, rt_u, rt_v, 0, j, i, Jc, Ic)) z.u[p] += L.xu[cp2(cm, Jc, Ic)];
+    if (j < m.ny - 1 && fine_agg(m, rt_u, rt_v, 1, j, i, Jc, Ic)) z.v[p] += L.xv[cp2(cm, Jc, Ic)];
+  }
+}
+
+// z = 0 (and, for FP32, y32 = y) on every cell of the active members
+template <class T>
+__global__ void __launch_bounds__(IMAU_WG) k_prep(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                  CVecT<double> y64, VecT<T> y, VecT<T> z,
+                                                  const int *__restrict__ active) {
+  TILE_SETUP
+  if (j >= m.ny || !active[t.g]) return;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= m.nx) break;
+    const long long p = p2(m, j, i);
+    z.u[p] = (T)0.0;
+    z.v[p] = (T)0.0;
+    if constexpr (!std::is_same<T, double>::value) {
+      y.u[p] = (T)y64.u[p];
+      y.v[p] = (T)y64.v[p];
+    }
+  }
+}
+
+// FP64 result of the FP32 cycle
+__global__ void __launch_bounds__(IMAU_WG) k_fin(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                 CVecT<float> z32, VecT<double> z, const int *__restrict__ active) {
+  TILE_SETUP
+  if (j >= m.ny || !active[t.g]) return;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= m.nx) break;
+    const long long p = p2(m, j, i);
+    z.u[p] = (double)z32.u[p];
+    z.v[p] = (double)z32.v[p];
+  }
+}
+
+// FP32 copies of the fine coefficients the smoother reads
+__global__ void __launch_bounds__(IMAU_WG) k_coef32(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                    const CoefT<double> c, imau::MGFine32 o, int sans,
+                                                    const int *__restrict__ active) {
+  TILE_SETUP
+  if (j >= m.ny || !active[t.g]) return;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= m.nx) break;
+    const long long p = p2(m, j, i);
+    o.Na[p] = (float)c.Na[p];
+    o.Nb[p] = (float)c.Nb[p];
+    o.bcx[p] = (float)c.bcx[p];
+    o.bcy[p] = (float)c.bcy[p];
+    if (sans) {
+      o.Ncx[p] = (float)c.Ncx[p];
+      o.Ncy[p] = (float)c.Ncy[p];
+    }
+  }
+}
+
+template <class T>
+__global__ void k_fill(T *p, long long n) {
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
+    p[q] = (T)0.0;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+inline dim3 wg() { return dim3(IMAU_TJ, IMAU_WG_I, 1); }
+
+inline CoefT<double> coef64(const imau::Coef &c) {
+  return CoefT<double>{c.Na, c.Nb, c.Ncx, c.Ncy, c.bcx, c.bcy, c.taudx, c.taudy, c.frhs, c.rt_u, c.rt_v};
+}
+inline CoefT<float> coef32(const imau::MGFine32 &f, const imau::Coef &c) {
+  return CoefT<float>{f.Na, f.Nb, f.Ncx, f.Ncy, f.bcx, f.bcy, nullptr, nullptr, nullptr, c.rt_u, c.rt_v};
+}
+
+template <class T>
+CLevelT<T> view(const imau::MGPlan &P, int l) {
+  const imau::MGLevelDev &d = P.levels[l];
+  CLevelT<T> L{};
+  L.mem = d.mem;
+  L.tiles = d.tiles;
+  L.depth = P.depth;
+  L.lev = d.lev;
+  L.ntiles = d.ntiles;
+  L.na = d.na;
+  L.cu = (T *)d.data;
+  L.cv = L.cu + 9 * d.na;
+  L.xu = L.cv + 9 * d.na;
+  L.xv = L.xu + d.na;
+  L.bu = L.xv + d.na;
+  L.bv = L.bu + d.na;
+  return L;
+}
+
+template <class T>
+void fill(hipStream_t s, T *p, long long n) {
+  int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_fill<T>, dim3(blocks), dim3(256), 0, s, p, n);
+}
+
+template <class T>
+void setup_t(const imau::Launch &F, const KP &kp, const imau::Coef &c, const imau::MGPlan &P, const int *act) {
+  const CLevelT<T> L1 = view<T>(P, 0);
+  hipLaunchKernelGGL(k_gal0<T>, dim3(L1.ntiles), wg(), 0, F.s, F.mem, kp, coef64(c), L1, act);
+  for (size_t l = 1; l < P.levels.size(); ++l) {
+    const CLevelT<T> Fl = view<T>(P, (int)l - 1), L = view<T>(P, (int)l);
+    hipLaunchKernelGGL(k_galc<T>, dim3(L.ntiles), wg(), 0, F.s, Fl, L, act);
+  }
+}
+
+template <class T>
+void vcycle_t(const imau::Launch &F, const KP &kp, const CoefT<T> &cf, const imau::MGPlan &P, CVecT<T> y, VecT<T> z,
+              const int *act, imau::MGMark mark, void *mctx) {
+  hipStream_t s = F.s;
+  const int nl = (int)P.levels.size();
+  if (mark) mark(mctx, 0);
+  for (int col = 0; col < 5; ++col)
+    hipLaunchKernelGGL(k_smooth0<T>, dim3(F.ntiles), wg(), 0, s, F.mem, F.tiles, kp, cf, z, y, col, act);
+  if (mark) mark(mctx, 1);
+  const CLevelT<T> L1 = view<T>(P, 0);
+  hipLaunchKernelGGL(k_restrict0<T>, dim3(L1.ntiles), wg(), 0, s, F.mem, kp, cf, y, CVecT<T>{z.u, z.v}, L1, act);
+  for (int l = 0; l < nl; ++l) {
+    const CLevelT<T> C = view<T>(P, l);
+    fill<T>(s, C.xu, 2 * C.na);
+    if (P.levels[l].ends)
+      hipLaunchKernelGGL(k_coarsest<T>, dim3(P.ngrids), dim3(1024), 0, s, C, act, P.coarsest_sweeps);
+    if (l + 1 == nl) break;
+    for (int col = 0; col < 4; ++col) hipLaunchKernelGGL(k_cgs<T>, dim3(C.ntiles), wg(), 0, s, C, col, act);
+    const CLevelT<T> N = view<T>(P, l + 1);
+    hipLaunchKernelGGL(k_crestrict<T>, dim3(N.ntiles), wg(), 0, s, C, N, act);
+  }
+  for (int l = nl - 2; l >= 0; --l) {
+    const CLevelT<T> C = view<T>(P, l), N = view<T>(P, l + 1);
+    hipLaunchKernelGGL(k_cprolong<T>, dim3(C.ntiles), wg(), 0, s, C, N, act);
+    for (int col = 3; col >= 0; --col) hipLaunchKernelGGL(k_cgs<T>, dim3(C.ntiles), wg(), 0, s, C, col, act);
+  }
+  hipLaunchKernelGGL(k_prolong0<T>, dim3(F.ntiles), wg(), 0, s, F.mem, F.tiles, cf.rt_u, cf.rt_v, z, L1, act);
+  for (int col = 3; col >= 0; --col)
+    hipLaunchKernelGGL(k_smooth0<T>, dim3(F.ntiles), wg(), 0, s, F.mem, F.tiles, kp, cf, z, y, col, act);
+  hipLaunchKernelGGL(k_smooth0<T>, dim3(F.ntiles), wg(), 0, s, F.mem, F.tiles, kp, cf, z, y, 4, act);
+}
+
+}  // namespace
+
+namespace imau {
+
+void mg_setup(const Launch &F, const KP &kp, const Coef &c, const MGPlan &P, bool fp32, const int *active) {
+  if (fp32) {
+    hipLaunchKernelGGL(k_coef32, dim3(F.ntiles), wg(), 0, F.s, F.mem, F.tiles, coef64(c), P.f32, kp.cross ? 0 : 1,
+                       active);
+    setup_t<float>(F, kp, c, P, active);
+  } else {
+    setup_t<double>(F, kp, c, P, active);
+  }
+}
+
+void mg_vcycle(const Launch &F, const KP &kp, const Coef &c, const MGPlan &P, bool fp32, const double *yu,
+               const double *yv, double *zu, double *zv, const int *active, MGMark mark, void *mark_ctx) {
+  if (fp32) {
+    VecT<float> y32{P.f32.yu, P.f32.yv}, z32{P.f32.zu, P.f32.zv};
+    hipLaunchKernelGGL(k_prep<float>, dim3(F.ntiles), wg(), 0, F.s, F.mem, F.tiles, CVecT<double>{yu, yv}, y32, z32,
+                       active);
+    vcycle_t<float>(F, kp, coef32(P.f32, c), P, CVecT<float>{y32.u, y32.v}, z32, active, mark, mark_ctx);
+    hipLaunchKernelGGL(k_fin, dim3(F.ntiles), wg(), 0, F.s, F.mem, F.tiles, CVecT<float>{z32.u, z32.v},
+                       VecT<double>{zu, zv}, active);
+  } else {
+    VecT<double> z{zu, zv};
+    hipLaunchKernelGGL(k_prep<double>, dim3(F.ntiles), wg(), 0, F.s, F.mem, F.tiles, CVecT<double>{yu, yv},
+                       VecT<double>{nullptr, nullptr}, z, active);
+    vcycle_t<double>(F, kp, coef64(c), P, CVecT<double>{yu, yv}, z, active, mark, mark_ctx);
+  }
+}
+
+}  // namespace imau

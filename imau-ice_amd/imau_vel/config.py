@@ -96,6 +96,7 @@ _STRING_MAPS = {
     "choice_idealised_sliding_law": IDEALISED, "choice_basal_hydrology": HYDROLOGY,
     "choice_basal_roughness": ROUGHNESS, "choice_param_basal_roughness": PARAM_ROUGHNESS,
     "DIVA_choice_matrix_solver": SOLVER,
+    "precision": {"fp64": abi.IMAU_PREC_FP64, "mixed": abi.IMAU_PREC_MIXED},
 }
 
 

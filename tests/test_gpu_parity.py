@@ -168,3 +168,45 @@ def test_multigrid_needs_fewer_applications(ctx):
     assert out[0].stats["nonlinear_residual"] <= 1e-4 and out[1].stats["nonlinear_residual"] <= 1e-4
     assert out[0].stats["linear_sweeps"] * 3 < out[1].stats["linear_sweeps"]
     assert relerr(out[0]["u_vav_cx"], out[1]["u_vav_cx"]) < 1e-2
+
+
+@pytest.mark.parametrize("case", ["dome", "random65"])
+def test_mixed_precision_fixed_point(ctx, case):
+    """FP32 multigrid preconditioner inside the FP64 Krylov loop reaches the
+    same fixed point as the oracle (FP64 residual, operator and update)."""
+    if case == "dome":
+        g, cfg, st = geometry.dome()
+    else:
+        g, cfg, st = geometry.random_geometry(65, 65, seed=7)
+    tight(cfg, st, rtol=1e-10)
+    config.set_config(cfg, mg_levels=0, precision="mixed")
+    gpu, ref = st.copy(), st.copy()
+    ctx.solve_diva(g, cfg, gpu)
+    O.solve("diva", g, cfg, ref, O.LIN_BANDED_LU)
+    print(case, "picard gpu/oracle", gpu.stats["picard_iterations"], ref.stats["picard_iterations"],
+          "precond applications", gpu.stats["linear_sweeps"], "residual", gpu.stats["nonlinear_residual"])
+    assert gpu.stats["nonlinear_residual"] <= 1e-10
+    for name in ("u_vav_cx", "v_vav_cy", "u_3D_cx", "w_3D_a", "taub_cx"):
+        e = relerr(gpu[name], ref[name])
+        assert e < 1e-6, (name, e)
+
+
+def test_mixed_batch_equals_single_solves(ctx):
+    """The FP32 cycle keeps members independent: batch results equal single solves bitwise."""
+    from imau_vel import solver
+    cases = [geometry.dome(), geometry.random_geometry(45, 37, seed=4)]
+    cfg = cases[0][1]
+    config.set_config(cfg, stop_rule=1, nonlinear_rtol=1e-6, DIVA_visc_it_nit=300, precision="mixed")
+    singles = []
+    for g, _, st in cases:
+        s = st.copy()
+        ctx.solve_diva(g, cfg, s)
+        singles.append(s)
+    batch = [st.copy() for _, _, st in cases]
+    prob = solver.Problem(ctx, [c[0] for c in cases], cfg, batch)
+    prob.solve()
+    prob.download()
+    prob.close()
+    for s1, sb in zip(singles, batch):
+        assert s1.stats["picard_iterations"] == sb.stats["picard_iterations"]
+        np.testing.assert_array_equal(s1["u_vav_cx"], sb["u_vav_cx"])
