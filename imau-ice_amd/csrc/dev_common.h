@@ -135,4 +135,14 @@ __device__ __forceinline__ double stag_margin(double v0, double v1, double H0, d
   return 0.5 * (v0 + v1);
 }
 
+// the same choice made once from the thicknesses, then applied to values
+__device__ __forceinline__ int stag_sel(double H0, double H1) {
+  if (H0 > 0.0 && H1 == 0.0) return 0;
+  if (H0 == 0.0 && H1 > 0.0) return 1;
+  return 2;
+}
+__device__ __forceinline__ double stag_pick(double v0, double v1, int sel) {
+  return sel == 0 ? v0 : (sel == 1 ? v1 : 0.5 * (v0 + v1));
+}
+
 __device__ __forceinline__ int iclamp(int a, int lo, int hi) { return a < lo ? lo : (a > hi ? hi : a); }

@@ -14,6 +14,7 @@
   const int j = t.j0 + (int)threadIdx.x;
 
 #define FOR_TILE_ROWS for (int ii = threadIdx.y; ii < IMAU_TI; ii += IMAU_WG_I)
+#define FOR_TILE_ROWS_SERIAL _Pragma("unroll 1") FOR_TILE_ROWS
 
 // ---------------------------------------------------------------------------
 // block reductions (wave64 shuffles, then LDS across the 4 wavefronts)
@@ -318,7 +319,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_setup2(const Member *__restrict__ m
 // Reads visc (previous iteration), writes visc_out.
 // src/ice_velocity_module.f90:664-1086
 // ---------------------------------------------------------------------------
-template <bool DIVA>
+template <bool DIVA, bool THIRD>
 __global__ void __launch_bounds__(IMAU_WG) k_visc(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
                                                   const KP kp, const Fields f, const double *__restrict__ visc_in,
                                                   double *__restrict__ visc_out, const double *__restrict__ xg,
@@ -337,7 +338,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_visc(const Member *__restrict__ mem
   const double dx = m.dx;
   const double *__restrict__ u = f.u;
   const double *__restrict__ v = f.v;
-  FOR_TILE_ROWS {
+  FOR_TILE_ROWS_SERIAL {
     const int i = t.i0 + ii;
     if (i >= nx) break;
     const long long p = p2(m, j, i);
@@ -380,9 +381,6 @@ __global__ void __launch_bounds__(IMAU_WG) k_visc(const Member *__restrict__ mem
       dudy = (j == 0) ? (U(1, ic) - U(0, ic)) / dx : (U(ny - 1, ic) - U(ny - 2, ic)) / dx;
       dvdx = (i == 0) ? (V(jv, 1) - V(jv, 0)) / dx : (V(jv, nx - 1) - V(jv, nx - 2)) / dx;
     }
-    // velocities on the a-grid for the sliding law (map_cx_to_a / map_cy_to_a, :1579-1644)
-    const double ua = (i == 0) ? U(j, 0) : (i == nx - 1 ? U(j, nx - 2) : 0.5 * (U(j, i - 1) + U(j, i)));
-    const double va = (j == 0) ? V(0, i) : (j == ny - 1 ? V(ny - 2, i) : 0.5 * (V(j - 1, i) + V(j, i)));
 #undef U
 #undef V
     const double eps_h = dudx * dudx + dvdy * dvdy + dudx * dvdy + 0.25 * (dudy + dvdx) * (dudy + dvdx) + kp.eps0;
@@ -415,25 +413,35 @@ __global__ void __launch_bounds__(IMAU_WG) k_visc(const Member *__restrict__ mem
       }
       const double Hxl = use_avg_x ? f.Hi[p2(m, j, i - 1)] : 0.0;
       const double Hyl = use_avg_y ? f.Hi[p2(m, j - 1, i)] : 0.0;
-      const bool third = (kp.n_flow == 3.0);
+      // calc_staggered_margin choices, fixed per column: 0 first, 1 second, 2 mean
+      const int sl = stag_sel(Hl0, Hl1), sr = stag_sel(Hr0, Hr1), sb = stag_sel(Hb0, Hb1), st = stag_sel(Ht0, Ht1);
+      const int sxa = stag_sel(Hxl, Hi), sya = stag_sel(Hyl, Hi);
       const double expo = (1.0 - kp.n_flow) / (2.0 * kp.n_flow);
       double eta_prev_k = 0.0, g_prev_k = 0.0;
+#pragma unroll 1
       for (int k = 0; k < nz; ++k) {
         const double z = kp.zeta[k];
         double du_dz_a = 0.0, dv_dz_a = 0.0;
         if (DIVA) {
-          // du/dz on the two cx neighbours (calc_vertical_shear_strain_rates)
+          // du/dz on the two cx and cy neighbours (calc_vertical_shear_strain_rates);
+          // the pairs (icl, icl+1), (icr, icr+1), (jcl, jcl+1), (jcr, jcr+1) are
+          // drawn from the centre and its four neighbours
+          const double vc = visc_in[p3(m, k, j, i)];
+          const double vw = (i > 0) ? visc_in[p3(m, k, j, i - 1)] : 0.0;
+          const double ve = (i < nx - 1) ? visc_in[p3(m, k, j, i + 1)] : 0.0;
+          const double vs = (j > 0) ? visc_in[p3(m, k, j - 1, i)] : 0.0;
+          const double vn = (j < ny - 1) ? visc_in[p3(m, k, j + 1, i)] : 0.0;
           double dl = 0.0, dr = 0.0, db = 0.0, dt = 0.0;
-          if (!noice_l) dl = (tbl / stag_margin(visc_in[p3(m, k, j, icl)], visc_in[p3(m, k, j, icl + 1)], Hl0, Hl1)) * z;
-          if (!noice_r) dr = (tbr / stag_margin(visc_in[p3(m, k, j, icr)], visc_in[p3(m, k, j, icr + 1)], Hr0, Hr1)) * z;
-          if (!noice_b) db = (tcl / stag_margin(visc_in[p3(m, k, jcl, i)], visc_in[p3(m, k, jcl + 1, i)], Hb0, Hb1)) * z;
-          if (!noice_t) dt = (tcr / stag_margin(visc_in[p3(m, k, jcr, i)], visc_in[p3(m, k, jcr + 1, i)], Ht0, Ht1)) * z;
-          du_dz_a = use_avg_x ? stag_margin(dl, dr, Hxl, Hi) : dl;
-          dv_dz_a = use_avg_y ? stag_margin(db, dt, Hyl, Hi) : db;
+          if (!noice_l) dl = (tbl / stag_pick((i == 0) ? vc : vw, (i == 0) ? ve : vc, sl)) * z;
+          if (!noice_r) dr = (tbr / stag_pick((i == nx - 1) ? vw : vc, (i == nx - 1) ? vc : ve, sr)) * z;
+          if (!noice_b) db = (tcl / stag_pick((j == 0) ? vc : vs, (j == 0) ? vn : vc, sb)) * z;
+          if (!noice_t) dt = (tcr / stag_pick((j == ny - 1) ? vs : vc, (j == ny - 1) ? vc : vn, st)) * z;
+          du_dz_a = use_avg_x ? stag_pick(dl, dr, sxa) : dl;
+          dv_dz_a = use_avg_y ? stag_pick(db, dt, sya) : db;
         }
         const double eps_sq = eps_h + 0.25 * (du_dz_a * du_dz_a + dv_dz_a * dv_dz_a);
         const double B = f.B3[p3(m, k, j, i)];
-        double eta = third ? B / cbrt(eps_sq) : B * pow(eps_sq, expo);
+        double eta = THIRD ? B / cbrt(eps_sq) : B * pow(eps_sq, expo);
         eta = fmax(eta, kp.visc_min);  // NaN (cold start 0/0) -> visc_eff_min
         visc_out[p3(m, k, j, i)] = eta;
         const double gk = (Hi / eta) * (z * z);
@@ -449,12 +457,37 @@ __global__ void __launch_bounds__(IMAU_WG) k_visc(const Member *__restrict__ mem
       f.N_a[p] = vint * ((kp.margin == 0) ? Hi : fmax(0.1, Hi));
     }
     // F2 integral (calc_F_integral, :960-1013)
-    double F2 = 0.0;
-    if (DIVA) {
-      if (mi == 1) F2 = fmax(kp.F_int_min, F2int);
-      else F2 = (kp.margin == 0) ? 0.0 : kp.F_int_min;
-      f.F2[p] = F2;
-    }
+    if (DIVA) f.F2[p] = (mi == 1) ? fmax(kp.F_int_min, F2int) : ((kp.margin == 0) ? 0.0 : kp.F_int_min);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Picard step 1b: basal friction on the a-grid (calc_sliding_term_beta,
+// :900-959) and beta_eff (calc_beta_eff, :1014-1062), after k_visc has written
+// F2.  A separate kernel so the register-hungry sliding laws do not limit the
+// occupancy of the 3D viscosity kernel.
+// ---------------------------------------------------------------------------
+template <bool DIVA>
+__global__ void __launch_bounds__(IMAU_WG) k_beta(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                  const KP kp, const Fields f, const double *__restrict__ xg,
+                                                  const double *__restrict__ yg, const int *__restrict__ active) {
+  TILE_SETUP
+  if (j >= m.ny || !active[t.g]) return;
+  const int nx = m.nx, ny = m.ny, nz = kp.nz;
+  const double *__restrict__ u = f.u;
+  const double *__restrict__ v = f.v;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= nx) break;
+    const long long p = p2(m, j, i);
+    // velocities on the a-grid for the sliding law (map_cx_to_a / map_cy_to_a, :1579-1644)
+#define U(jj, ii_) u[p2(m, (jj), (ii_))]
+#define V(jj, ii_) v[p2(m, (jj), (ii_))]
+    const double ua = (i == 0) ? U(j, 0) : (i == nx - 1 ? U(j, nx - 2) : 0.5 * (U(j, i - 1) + U(j, i)));
+    const double va = (j == 0) ? V(0, i) : (j == ny - 1 ? V(ny - 2, i) : 0.5 * (V(j - 1, i) + V(j, i)));
+#undef U
+#undef V
+    const double F2 = DIVA ? f.F2[p] : 0.0;
     // basal friction (calc_sliding_term_beta, :900-959)
     double tauc = f.tauc[p];
     if (kp.slid == 3 || (kp.slid == 2 && !(kp.rough == 1 && kp.prough == 1))) {
@@ -1191,10 +1224,12 @@ void setup2(const Launch &L, const KP &kp, const Fields &f) {
 }
 void visc(const Launch &L, const KP &kp, const Fields &f, bool diva, const double *visc_in, double *visc_out,
           const double *xg, const double *yg, const int *active) {
-  if (diva)
-    hipLaunchKernelGGL(k_visc<true>, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, visc_in, visc_out, xg, yg, active);
-  else
-    hipLaunchKernelGGL(k_visc<false>, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, visc_in, visc_out, xg, yg, active);
+  // n = 3 (every shipped configuration) takes the cube-root form of the viscosity
+  const bool third = kp.n_flow == 3.0;
+  auto kv = diva ? (third ? k_visc<true, true> : k_visc<true, false>) : (third ? k_visc<false, true> : k_visc<false, false>);
+  hipLaunchKernelGGL(kv, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, visc_in, visc_out, xg, yg, active);
+  if (diva) hipLaunchKernelGGL(k_beta<true>, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, xg, yg, active);
+  else hipLaunchKernelGGL(k_beta<false>, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, xg, yg, active);
 }
 void coef(const Launch &L, const KP &kp, const Fields &f, bool diva, const int *active) {
   hipLaunchKernelGGL(k_coef, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, diva ? 1 : 0, active);
