@@ -40,14 +40,23 @@ HBM_PEAK_GBS = 8000.0
 SWEEP_WORDS_PER_CELL = 10
 SWEEP_BYTES_EXTRA = 2
 METHODS = {
-    "mixed": dict(precision="mixed", mg_levels=0,
+    "mixed": dict(precision="mixed", mg_levels=0, picard_relax_max=1.0,
                   text="BiCGStab in FP64 (operator, residuals, update) preconditioned by an FP32 aggregation-"
-                       "multigrid V-cycle; FP64 viscosity/beta; reference Picard loop"),
-    "fp64": dict(precision="fp64", mg_levels=0,
-                 text="FP64 BiCGStab preconditioned by an FP64 aggregation-multigrid V-cycle; reference Picard loop"),
-    "plain": dict(precision="fp64", mg_levels=1,
-                  text="FP64 BiCGStab preconditioned by tile Gauss-Seidel (no multigrid); reference Picard loop"),
+                       "multigrid V-cycle; FP64 viscosity/beta; Picard loop with adaptive relaxation "
+                       "(0.4 raised x1.25 per residual decrease up to 1.0, reset on an increase)"),
+    "fp64": dict(precision="fp64", mg_levels=0, picard_relax_max=0.0,
+                 text="FP64 BiCGStab preconditioned by an FP64 aggregation-multigrid V-cycle; reference Picard "
+                      "loop (fixed relaxation 0.4)"),
+    "plain": dict(precision="fp64", mg_levels=1, picard_relax_max=0.0,
+                  text="FP64 BiCGStab preconditioned by tile Gauss-Seidel (no multigrid); reference Picard loop "
+                       "(fixed relaxation 0.4)"),
 }
+
+
+def use_method(cfg, meth, relax_max=None):
+    from imau_vel import config
+    config.set_config(cfg, precision=meth["precision"], mg_levels=meth["mg_levels"],
+                      picard_relax_max=meth["picard_relax_max"] if relax_max is None else relax_max)
 
 
 def workload(name):
@@ -108,6 +117,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--method", choices=tuple(METHODS), default="mixed")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison legs")
+    ap.add_argument("--relax-max", type=float, default=None, help="override the headline's picard_relax_max")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -126,7 +136,7 @@ def main():
 
     # headline method
     meth = METHODS[args.method]
-    config.set_config(cfg, precision=meth["precision"], mg_levels=meth["mg_levels"])
+    use_method(cfg, meth, args.relax_max)
     dt = timed_solves(prob, cfg, args.steps, args.warmup)
     out = prob.download()
     its = [o.stats["picard_iterations"] for o in out]
@@ -163,7 +173,7 @@ def main():
             if key == args.method:
                 continue
             m2 = METHODS[key]
-            config.set_config(cfg, precision=m2["precision"], mg_levels=m2["mg_levels"])
+            use_method(cfg, m2)
             dtp = timed_solves(prob, cfg, 1, 0)
             outp = prob.download()
             name = "plain_fp64" if key == "plain" else "fp64_multigrid"
@@ -172,7 +182,7 @@ def main():
                                 max_final_residual=max(o.stats["nonlinear_residual"] for o in outp))
         if "plain_fp64" in result:
             result["speedup_vs_plain_fp64"] = result["value"] / result["plain_fp64"]["value"]
-        config.set_config(cfg, precision=meth["precision"], mg_levels=meth["mg_levels"])
+        use_method(cfg, meth, args.relax_max)
     if not args.no_cpu_baseline:
         k, ref, cb = cpu_baseline(cfg, members)
         result["cpu_baseline"] = cb

@@ -103,6 +103,7 @@ struct imau_problem {
   DBuf<unsigned long long> resmax;
   DBuf<int> act_pic, act_lin;
   DBuf<double> sc;    // per-member Krylov scalars
+  DBuf<double> relax_d;  // per-member Picard relaxation (adaptive mode)
   std::vector<std::unique_ptr<MGLevel>> mg;  // coarse levels 1.. of the multigrid preconditioner
   DBuf<int> mg_depth;                        // per member: its coarsest level
   DBuf<float> mg_f32;                        // FP32 fine-level coefficients, rhs and iterate
@@ -293,6 +294,10 @@ void check_cfg(const imau_cfg *cfg) {
     if (bcs[q] < 0 || bcs[q] > 2) fail(IMAU_ERR_CONFIG, "unknown DIVA boundary condition %d", bcs[q]);
   if (cfg->DIVA_choice_matrix_solver != IMAU_SOLVER_SOR && cfg->DIVA_choice_matrix_solver != IMAU_SOLVER_PETSC)
     fail(IMAU_ERR_CONFIG, "unknown DIVA_choice_matrix_solver %d", cfg->DIVA_choice_matrix_solver);
+  if (!(cfg->picard_relax_max >= 0.0 && cfg->picard_relax_max <= 1.0))
+    fail(IMAU_ERR_CONFIG, "picard_relax_max must lie in [0, 1], got %g", cfg->picard_relax_max);
+  if (cfg->picard_relax_max > 0.0 && cfg->stop_rule != IMAU_STOP_RESIDUAL)
+    fail(IMAU_ERR_CONFIG, "adaptive Picard relaxation (picard_relax_max > 0) needs stop_rule IMAU_STOP_RESIDUAL");
   if (cfg->mg_levels != 0 && cfg->mg_levels != 1)
     fail(IMAU_ERR_CONFIG, "mg_levels must be 0 (multigrid, automatic depth) or 1 (no multigrid), got %d",
          cfg->mg_levels);
@@ -526,6 +531,7 @@ void imau_vel_default_config(imau_cfg *c) {
   c->precision = IMAU_PREC_FP64;
   c->stop_rule = IMAU_STOP_PICARD;
   c->nonlinear_rtol = 1e-4;
+  c->picard_relax_max = 0.0;
 }
 
 int imau_vel_init(const imau_grid *grid, const imau_cfg *cfg, imau_ctx **out) {
@@ -626,6 +632,7 @@ int imau_vel_problem_create(imau_ctx *ctx, int32_t ngrids, const imau_grid *grid
   P->act_pic.alloc(ngrids);
   P->act_lin.alloc(ngrids);
   P->sc.alloc((size_t)8 * ngrids);
+  P->relax_d.alloc(ngrids);
   HIPCHK(hipMemcpy(P->mem_d.p, P->mem.data(), ngrids * sizeof(Member), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(P->tiles_d.p, P->tiles.data(), P->tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(P->tstart_d.p, P->tstart.data(), P->tstart.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -893,6 +900,7 @@ struct Solver {
 
   int sor_nit = 10000;
   double sor_tol = 2.5, sor_omega = 1.3, petsc_rtol = 0.01, petsc_abstol = 2.5;
+  std::vector<double> relax_h_;
 
   void picard(bool diva) {
     const int G = P->ngrids;
@@ -906,6 +914,10 @@ struct Solver {
     imau::setup2(L, kp, P->F);
     kcheck();
     std::vector<double> r;
+    // adaptive relaxation: per-member factor, raised while the FP64 residual falls
+    const bool adaptive = cfg->picard_relax_max > 0.0 && cfg->stop_rule == IMAU_STOP_RESIDUAL;
+    relax_h_.assign(G, cfg->DIVA_visc_it_relax);
+    std::vector<double> prev_r(G, -1.0);
     for (int it = 1;; ++it) {
       bool any = false;
       for (int g = 0; g < G; ++g) any = any || act_pic[g];
@@ -923,12 +935,20 @@ struct Solver {
           P->nl_resid[g] = r[g];
           if (r[g] <= cfg->nonlinear_rtol || it > cfg->DIVA_visc_it_nit) act_pic[g] = 0;
           else left = true;
+          if (adaptive) {
+            if (prev_r[g] >= 0.0)
+              relax_h_[g] = (r[g] < prev_r[g]) ? std::min(cfg->picard_relax_max, 1.25 * relax_h_[g])
+                                               : cfg->DIVA_visc_it_relax;
+            prev_r[g] = r[g];
+          }
         }
         if (!left) break;
         push_active(P->act_pic.p, act_pic);
       }
       linear_solve();
-      imau::post(L, kp, P->F, P->F.xu0, P->F.xv0, P->act_pic.p, diva, P->partial.p);
+      if (adaptive)  // relax_h_ is not modified before the next stream synchronisation
+        HIPCHK(hipMemcpyAsync(P->relax_d.p, relax_h_.data(), G * sizeof(double), hipMemcpyHostToDevice, c->s));
+      imau::post(L, kp, P->F, P->F.xu0, P->F.xv0, P->act_pic.p, diva, P->partial.p, adaptive ? P->relax_d.p : nullptr);
       imau::member_sum3(c->s, G, P->tstart_d.p, P->partial.p, P->msum.p);
       kcheck();
       HIPCHK(hipMemcpyAsync(host4.data(), P->msum.p, 3 * G * sizeof(double), hipMemcpyDeviceToHost, c->s));

@@ -25,8 +25,9 @@ void coef(const Launch &L, const KP &kp, const Fields &f, bool diva, const int *
 void init_x(const Launch &L, const Fields &f, double *xu, double *xv);
 void sweep(const Launch &L, const KP &kp, const Coef &c, const double *xu_in, const double *xv_in, double *xu_out,
            double *xv_out, const int *active, double omega, bool track, unsigned long long *resmax);
+// relax: per-member relaxation factors, or nullptr for the configured DIVA_visc_it_relax
 void post(const Launch &L, const KP &kp, const Fields &f, const double *xu, const double *xv, const int *active,
-          bool diva, double *partial);
+          bool diva, double *partial, const double *relax);
 void resid(const Launch &L, const KP &kp, const Coef &c, const double *xu, const double *xv, double *partial);
 void member_sum3(hipStream_t s, int ngrids, const int *tstart, const double *partial, double *out);
 void member_sum2(hipStream_t s, int ngrids, const int *tstart, const double *partial, double *out);

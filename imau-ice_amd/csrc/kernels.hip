@@ -761,11 +761,12 @@ __global__ void __launch_bounds__(IMAU_WG) k_sweep(const Member *__restrict__ me
 __global__ void __launch_bounds__(IMAU_WG) k_post(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
                                                   const KP kp, const Fields f, const double *__restrict__ xu,
                                                   const double *__restrict__ xv, const int *__restrict__ active,
-                                                  int diva, double *__restrict__ partial) {
+                                                  int diva, double *__restrict__ partial,
+                                                  const double *__restrict__ relax) {
   TILE_SETUP
   double acc[3] = {0.0, 0.0, 0.0};
   const int nx = m.nx, ny = m.ny;
-  const double rel = kp.relax, vmax = kp.vel_max;
+  const double rel = relax ? relax[t.g] : kp.relax, vmax = kp.vel_max;
   if (j < ny && active[t.g]) {
     FOR_TILE_ROWS {
       const int i = t.i0 + ii;
@@ -1243,8 +1244,9 @@ void sweep(const Launch &L, const KP &kp, const Coef &c, const double *xu_in, co
                      active, omega, track ? 1 : 0, resmax);
 }
 void post(const Launch &L, const KP &kp, const Fields &f, const double *xu, const double *xv, const int *active,
-          bool diva, double *partial) {
-  hipLaunchKernelGGL(k_post, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, xu, xv, active, diva ? 1 : 0, partial);
+          bool diva, double *partial, const double *relax) {
+  hipLaunchKernelGGL(k_post, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, xu, xv, active, diva ? 1 : 0, partial,
+                     relax);
 }
 void resid(const Launch &L, const KP &kp, const Coef &c, const double *xu, const double *xv, double *partial) {
   hipLaunchKernelGGL(k_resid, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, to_p(c), xu, xv, partial);

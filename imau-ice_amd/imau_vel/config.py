@@ -87,6 +87,7 @@ def default_config(**overrides) -> ImauCfg:
     c.nonlinear_rtol = 1e-4
     c.mg_levels = 0
     c.check_interval = 0
+    c.picard_relax_max = 0.0
     set_config(c, **overrides)
     return c
 

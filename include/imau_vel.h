@@ -134,6 +134,12 @@ typedef struct imau_cfg {
   int32_t mg_levels;       /* KSP mode preconditioner: 0 = aggregation multigrid
                               (automatic depth), 1 = tile Gauss-Seidel only     */
   int32_t check_interval;  /* sweeps between linear-solver convergence checks */
+  double  picard_relax_max; /* 0: the reference's fixed DIVA_visc_it_relax.  > 0:
+                              adaptive relaxation, per member: start at
+                              DIVA_visc_it_relax, grow x1.25 after every Picard
+                              iteration whose FP64 residual fell, back to
+                              DIVA_visc_it_relax after one that rose, capped at
+                              this value (needs IMAU_STOP_RESIDUAL)          */
 } imau_cfg;
 
 /* The fields of type_ice_model (src/data_types_module.f90:56-325) that the

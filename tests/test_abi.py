@@ -1,6 +1,7 @@
 """The C ABI boundary without a GPU: struct layout, exported symbols,
 default configuration, and loud failure when no device is visible."""
 import ctypes
+import os
 import re
 
 import numpy as np
@@ -8,6 +9,8 @@ import pytest
 
 import _oracle as O
 from imau_vel import abi, config, solver
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_python_structs_match_c_layout():
@@ -53,3 +56,14 @@ def test_context_fails_loudly_without_gpu():
     with pytest.raises(solver.ImauError) as e:
         solver.Context()
     assert e.value.code == abi.IMAU_ERR_NO_DEVICE
+
+
+def test_fortran_bindings_are_current():
+    """imau_vel_bindings.f90 is the generator's output for the current header."""
+    import subprocess
+    import sys
+    gen = os.path.join(ROOT, "imau-ice_amd", "fortran", "gen_bindings.py")
+    path = os.path.join(ROOT, "imau-ice_amd", "fortran", "imau_vel_bindings.f90")
+    before = open(path).read()
+    out = subprocess.run([sys.executable, gen, "--stdout"], capture_output=True, text=True, check=True).stdout
+    assert out == before

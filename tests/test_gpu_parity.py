@@ -210,3 +210,23 @@ def test_mixed_batch_equals_single_solves(ctx):
     for s1, sb in zip(singles, batch):
         assert s1.stats["picard_iterations"] == sb.stats["picard_iterations"]
         np.testing.assert_array_equal(s1["u_vav_cx"], sb["u_vav_cx"])
+
+
+def test_adaptive_relaxation_same_fixed_point(ctx):
+    """Adaptive Picard relaxation changes the path, not the fixed point: the
+    oracle's converged velocities within 1e-6, in fewer Picard iterations."""
+    g, cfg, st = geometry.random_geometry(65, 65, seed=7)
+    tight(cfg, st, rtol=1e-10)
+    config.set_config(cfg, precision="mixed")
+    fixed, adapt, ref = st.copy(), st.copy(), st.copy()
+    ctx.solve_diva(g, cfg, fixed)
+    config.set_config(cfg, picard_relax_max=1.0)
+    ctx.solve_diva(g, cfg, adapt)
+    config.set_config(cfg, picard_relax_max=0.0)
+    O.solve("diva", g, cfg, ref, O.LIN_BANDED_LU)
+    print("picard fixed/adaptive/oracle", fixed.stats["picard_iterations"], adapt.stats["picard_iterations"],
+          ref.stats["picard_iterations"])
+    assert adapt.stats["picard_iterations"] < fixed.stats["picard_iterations"]
+    for name in ("u_vav_cx", "v_vav_cy", "u_3D_cx", "taub_cx"):
+        e = relerr(adapt[name], ref[name])
+        assert e < 1e-6, (name, e)
