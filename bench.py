@@ -11,9 +11,13 @@ solve (state reset + Picard loop until every member has
 "--workload channel": the 401 x 41 marine channel at 2 km, one solve per step.
 
 The line also carries:
-  roofline      the fine-level smoother sweep (5 colour launches of
-                k_smooth0): algorithmic bytes per sweep / its average
-                duration from HIP events on the library stream;
+  roofline      the dominant kernel, k_visc (3D viscosity, one launch per
+                Picard iteration): algorithmic bytes / launch duration from
+                HIP events on the library stream, over the members each
+                launch processed;
+  roofline_smoother
+                the same for one fine-level smoother sweep (5 colour launches
+                of k_smooth0);
   cpu_baseline  the CPU oracle (FP64, banded-LU linear solves, 1 thread) on a
                 bounded sample: the smallest ensemble member solved alone;
   fp64_multigrid, plain_fp64
@@ -39,6 +43,21 @@ HBM_PEAK_GBS = 8000.0
 # row-type bytes
 SWEEP_WORDS_PER_CELL = 10
 SWEEP_BYTES_EXTRA = 2
+
+
+def VISC_BYTES_PER_CELL(nz):
+    """k_visc per a-grid cell: read the previous viscosity and the rate-factor
+    term B (nz words each), write the new viscosity (nz words); read u, v, H,
+    taub_cx, taub_cy (8 B each) and the ice mask (4 B); write visc_int, N_a,
+    F2 (8 B each)."""
+    return 24 * nz + 5 * 8 + 4 + 3 * 8
+
+
+# HBM bytes per processed cell measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE on the ensemble workload (FETCH_SIZE scaled by 2.0, WRITE_SIZE by
+# 1.0: the factors a 1 GiB device copy reads; profiles/k_visc_ensemble_traffic.md)
+VISC_TRAFFIC_BYTES_PER_CELL = None
+SWEEP_TRAFFIC_BYTES_PER_CELL = {}
 METHODS = {
     "mixed": dict(precision="mixed", mg_levels=0, picard_relax_max=1.0,
                   text="BiCGStab in FP64 (operator, residuals, update) preconditioned by an FP32 aggregation-"
@@ -95,6 +114,18 @@ def cpu_baseline(cfg, members):
     return k, s, cb
 
 
+def copy_probe(nbytes):
+    """Five copies of nbytes between two device buffers: the reference reading
+    for the FETCH_SIZE / WRITE_SIZE counters (each copy reads and writes nbytes)."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps(dict(copy_probe_bytes=nbytes, copies=5)))
+
+
 def timed_solves(prob, cfg, steps, warmup):
     """steps complete solves from the uploaded state; each solve returns only
     after its stream has drained, so wall time brackets the GPU work."""
@@ -118,7 +149,12 @@ def main():
     ap.add_argument("--method", choices=tuple(METHODS), default="mixed")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison legs")
     ap.add_argument("--relax-max", type=float, default=None, help="override the headline's picard_relax_max")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run 5 plain device-to-device copies of BYTES each (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(ensemble members would be sharded across GPUs by size)", file=sys.stderr)
@@ -142,17 +178,36 @@ def main():
     its = [o.stats["picard_iterations"] for o in out]
     apps = [o.stats["linear_sweeps"] for o in out]
     res = [o.stats["nonlinear_residual"] for o in out]
-    # kernel timing pass over the same solves (HIP events on the library stream)
-    ctx.kernel_timing(True)
-    for _ in range(args.steps):
-        prob.reset_state()
-        prob.solve(cfg)
-    n_sweeps, sweep_ms = ctx.kernel_timing(False)
-    avg_us = 1e3 * sweep_ms / max(n_sweeps, 1)
+    # kernel timing passes over the same solves (HIP events on the library stream):
+    # the 3D viscosity kernel, then one fine-level smoother sweep
+    timed = {}
+    for what in ("visc", "smoother"):
+        ctx.kernel_timing(what)
+        for _ in range(args.steps):
+            prob.reset_state()
+            prob.solve(cfg)
+        timed[what] = ctx.kernel_timing(False)
     cells = sum(g.nx * g.ny for g in grids)
     word = 4 if meth["precision"] == "mixed" else 8
-    bytes_per_launch = cells * (SWEEP_WORDS_PER_CELL * word + SWEEP_BYTES_EXTRA)
-    achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9
+    nz = cfg.nz
+
+    def roof(what, kernel, bytes_per_cell, traffic_per_cell):
+        n, ms, cells_t = timed[what]
+        avg_us = 1e3 * ms / max(n, 1)
+        per_launch = bytes_per_cell * cells_t / max(n, 1)
+        achieved = bytes_per_cell * cells_t / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        traffic = traffic_per_cell * cells_t / max(n, 1) if traffic_per_cell else None
+        return dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                    traffic=traffic, kernel=kernel, avg_launch_us=avg_us, launches=n,
+                    bytes_per_launch=per_launch, bytes_per_cell=bytes_per_cell)
+
+    visc_traffic = VISC_TRAFFIC_BYTES_PER_CELL if args.workload == "ensemble" else None
+    sweep_traffic = SWEEP_TRAFFIC_BYTES_PER_CELL.get(meth["precision"]) if args.workload == "ensemble" else None
+    roof_visc = roof("visc", "k_visc (3D viscosity, N, F2: one launch per Picard iteration)",
+                     VISC_BYTES_PER_CELL(nz), visc_traffic)
+    roof_sweep = roof("smoother", "k_smooth0 x5 (one fine-level smoother sweep)" if meth["mg_levels"] == 0
+                      else "k_prec_tile (one tile Gauss-Seidel application)",
+                      SWEEP_WORDS_PER_CELL * word + SWEEP_BYTES_EXTRA, sweep_traffic if meth["mg_levels"] == 0 else None)
 
     result = dict(
         metric="DIVA solves/s at 1e-4 rel residual: learned+FP32 vs plain FP64 GPU",
@@ -164,9 +219,8 @@ def main():
         picard_iterations=dict(min=min(its), max=max(its), mean=float(np.mean(its))),
         preconditioner_applications=dict(min=min(apps), max=max(apps)),
         max_final_residual=max(res),
-        roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                      traffic=None, kernel="k_smooth0 x5 (one fine-level smoother sweep)", avg_launch_us=avg_us,
-                      launches=n_sweeps, bytes_per_launch=bytes_per_launch),
+        roofline=roof_visc,
+        roofline_smoother=roof_sweep,
     )
     if not args.no_compare:
         for key in ("fp64", "plain"):

@@ -139,6 +139,7 @@ struct imau_ctx {
   size_t ev_used = 0;
   long long sweep_launches = 0;
   double sweep_ms = 0.0;
+  long long sweep_cells = 0;
 };
 
 static std::string g_init_err;
@@ -566,13 +567,16 @@ void imau_vel_finalize(imau_ctx *ctx) {
 
 const char *imau_vel_last_error(const imau_ctx *ctx) { return ctx ? ctx->err.c_str() : g_init_err.c_str(); }
 
-int imau_vel_kernel_timing(imau_ctx *ctx, int32_t enable, int64_t *sweep_launches, double *sweep_ms) {
+int imau_vel_kernel_timing(imau_ctx *ctx, int32_t what, int64_t *launches, double *ms, int64_t *cells) {
   if (!ctx) return IMAU_ERR_ARG;
-  if (sweep_launches) *sweep_launches = ctx->sweep_launches;
-  if (sweep_ms) *sweep_ms = ctx->sweep_ms;
-  ctx->ktime = enable ? 1 : 0;
+  if (what < IMAU_TIME_OFF || what > IMAU_TIME_VISC) return IMAU_ERR_ARG;
+  if (launches) *launches = ctx->sweep_launches;
+  if (ms) *ms = ctx->sweep_ms;
+  if (cells) *cells = ctx->sweep_cells;
+  ctx->ktime = what;
   ctx->sweep_launches = 0;
   ctx->sweep_ms = 0.0;
+  ctx->sweep_cells = 0;
   return IMAU_OK;
 }
 
@@ -735,8 +739,8 @@ struct Solver {
     }
   }
 
-  void time_begin() {
-    if (!c->ktime) return;
+  void time_begin(int what = IMAU_TIME_SMOOTHER) {
+    if (c->ktime != what) return;
     if (c->ev_used + 2 > c->ev.size()) {
       for (int q = 0; q < 256; ++q) {
         hipEvent_t e;
@@ -746,8 +750,11 @@ struct Solver {
     }
     HIPCHK(hipEventRecord(c->ev[c->ev_used], c->s));
   }
-  void time_end() {
-    if (!c->ktime) return;
+  void time_end(int what = IMAU_TIME_SMOOTHER) {
+    if (c->ktime != what) return;
+    const std::vector<int> &act = (what == IMAU_TIME_VISC) ? act_pic : act_lin;
+    for (int g = 0; g < P->ngrids; ++g)
+      if (act[g]) c->sweep_cells += (long long)P->mem[g].nx * P->mem[g].ny;
     HIPCHK(hipEventRecord(c->ev[c->ev_used + 1], c->s));
     c->ev_used += 2;
   }
@@ -923,7 +930,10 @@ struct Solver {
       for (int g = 0; g < G; ++g) any = any || act_pic[g];
       if (!any) break;
       push_active(P->act_pic.p, act_pic);
-      imau::visc(L, kp, P->F, diva, P->visc_cur, P->visc_oth, P->xg.p, P->yg.p, P->act_pic.p);
+      time_begin(IMAU_TIME_VISC);
+      imau::visc3d(L, kp, P->F, diva, P->visc_cur, P->visc_oth, P->act_pic.p);
+      time_end(IMAU_TIME_VISC);
+      imau::beta(L, kp, P->F, diva, P->xg.p, P->yg.p, P->act_pic.p);
       std::swap(P->visc_cur, P->visc_oth);
       imau::coef(L, kp, P->F, diva, P->act_pic.p);
       kcheck();
@@ -1194,7 +1204,8 @@ int imau_vel_nonlinear_residual(imau_ctx *ctx, const imau_grid *grid, const imau
   imau::setup2(S.L, S.kp, P->F);
   std::vector<int> all(1, 1);
   S.push_active(P->act_pic.p, all);
-  imau::visc(S.L, S.kp, P->F, diva, P->visc_cur, P->visc_oth, P->xg.p, P->yg.p, P->act_pic.p);
+  imau::visc3d(S.L, S.kp, P->F, diva, P->visc_cur, P->visc_oth, P->act_pic.p);
+  imau::beta(S.L, S.kp, P->F, diva, P->xg.p, P->yg.p, P->act_pic.p);
   std::swap(P->visc_cur, P->visc_oth);
   imau::coef(S.L, S.kp, P->F, diva, P->act_pic.p);
   S.kcheck();

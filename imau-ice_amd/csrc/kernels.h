@@ -19,12 +19,13 @@ struct Coef {
 
 void setup1(const Launch &L, const KP &kp, const Fields &f, const double *yg);
 void setup2(const Launch &L, const KP &kp, const Fields &f);
-void visc(const Launch &L, const KP &kp, const Fields &f, bool diva, const double *visc_in, double *visc_out,
-          const double *xg, const double *yg, const int *active);
+// 3D viscosity, N, F2 (k_visc) and then basal friction / beta_eff (k_beta)
+void visc3d(const Launch &L, const KP &kp, const Fields &f, bool diva, const double *visc_in, double *visc_out,
+            const int *active);
+void beta(const Launch &L, const KP &kp, const Fields &f, bool diva, const double *xg, const double *yg,
+          const int *active);
 void coef(const Launch &L, const KP &kp, const Fields &f, bool diva, const int *active);
 void init_x(const Launch &L, const Fields &f, double *xu, double *xv);
-void sweep(const Launch &L, const KP &kp, const Coef &c, const double *xu_in, const double *xv_in, double *xu_out,
-           double *xv_out, const int *active, double omega, bool track, unsigned long long *resmax);
 // relax: per-member relaxation factors, or nullptr for the configured DIVA_visc_it_relax
 void post(const Launch &L, const KP &kp, const Fields &f, const double *xu, const double *xv, const int *active,
           bool diva, double *partial, const double *relax);

@@ -322,8 +322,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_setup2(const Member *__restrict__ m
 template <bool DIVA, bool THIRD>
 __global__ void __launch_bounds__(IMAU_WG) k_visc(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
                                                   const KP kp, const Fields f, const double *__restrict__ visc_in,
-                                                  double *__restrict__ visc_out, const double *__restrict__ xg,
-                                                  const double *__restrict__ yg, const int *__restrict__ active) {
+                                                  double *__restrict__ visc_out, const int *__restrict__ active) {
   TILE_SETUP
   if (j >= m.ny) return;
   const int nx = m.nx, ny = m.ny, nz = kp.nz;
@@ -614,141 +613,6 @@ __device__ __forceinline__ void v_row(const Member &m, const KP &kp, const CoefP
 }
 
 
-// ---------------------------------------------------------------------------
-// One SOR sweep over every tile: four colour phases (u red, u black, v red,
-// v black) with the tile and a one-cell halo of u and v staged in LDS, then
-// the domain-boundary rows.  Tile halos come from the previous iterate (xin),
-// the result goes to xout, so the sweep is deterministic (Gauss-Seidel inside
-// a tile, Jacobi across tile edges).  With track != 0 the per-member maximum
-// |(S + D x - b)/D| (the reference's res_max, src/utilities_module.f90:671-695)
-// is folded into resmax[g] with an atomic max on its bit pattern.
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(IMAU_WG) k_sweep(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
-                                                   const KP kp, const CoefP c, const double *__restrict__ xu_in,
-                                                   const double *__restrict__ xv_in, double *__restrict__ xu_out,
-                                                   double *__restrict__ xv_out, const int *__restrict__ active,
-                                                   double omega, int track, unsigned long long *__restrict__ resmax) {
-  __shared__ double red[IMAU_WG_I];
-#define su g_sx[0]
-#define sv g_sx[1]
-  const Tile t = tiles[blockIdx.x];
-  const Member m = mem[t.g];
-  const int nx = m.nx, ny = m.ny;
-  const int tid = threadIdx.y * IMAU_TJ + threadIdx.x;
-  if (!active[t.g]) {  // converged member: carry the iterate through
-    for (int q = tid; q < IMAU_TI * IMAU_TJ; q += IMAU_WG) {
-      const int i = t.i0 + q / IMAU_TJ, j = t.j0 + q % IMAU_TJ;
-      if (i < nx && j < ny) {
-        const long long p = p2(m, j, i);
-        xu_out[p] = xu_in[p];
-        xv_out[p] = xv_in[p];
-      }
-    }
-    return;
-  }
-  // stage tile + halo
-  for (int q = tid; q < (IMAU_TI + 2) * (IMAU_TJ + 2); q += IMAU_WG) {
-    const int li = q / (IMAU_TJ + 2), lj = q % (IMAU_TJ + 2);
-    const int i = t.i0 + li - 1, j = t.j0 + lj - 1;
-    double a = 0.0, b = 0.0;
-    if (i >= 0 && i < nx && j >= 0 && j < ny) {
-      const long long p = p2(m, j, i);
-      a = xu_in[p];
-      b = xv_in[p];
-    }
-    su[li][lj] = a;
-    sv[li][lj] = b;
-  }
-  __syncthreads();
-  const LAcc<0> U{t.i0, t.j0};
-  const LAcc<1> V{t.i0, t.j0};
-  const int j = t.j0 + (int)threadIdx.x;
-  const int lj = threadIdx.x + 1;
-  double rmax = 0.0;
-  // colour phases: 0 u-red, 1 u-black, 2 v-red, 3 v-black, 4 boundary rows
-  for (int ph = 0; ph < 5; ++ph) {
-    FOR_TILE_ROWS {
-      const int i = t.i0 + ii;
-      if (i >= nx || j >= ny) continue;
-      const long long p = p2(m, j, i);
-      const int li = ii + 1;
-      if (ph < 4) {
-        const bool isu = ph < 2;
-        if (((i + j) & 1) != (ph & 1)) continue;
-        if (isu && i >= nx - 1) continue;
-        if (!isu && j >= ny - 1) continue;
-        const uint8_t rt = isu ? c.rt_u[p] : c.rt_v[p];
-        if (is_bnd(rt)) continue;
-        double S, D, b;
-        if (isu) {
-          u_row(m, kp, c, j, i, rt, U, V, S, D, b);
-          const double x = su[li][lj];
-          const double res = (S + D * x - b) / D;
-          rmax = fmax(rmax, fabs(res));
-          su[li][lj] = (rt == RT_FREE) ? x - omega * res : x - res;
-        } else {
-          v_row(m, kp, c, j, i, rt, U, V, S, D, b);
-          const double x = sv[li][lj];
-          const double res = (S + D * x - b) / D;
-          rmax = fmax(rmax, fabs(res));
-          sv[li][lj] = (rt == RT_FREE) ? x - omega * res : x - res;
-        }
-      } else {
-        // domain-boundary rows: x = x(source) ('infinite': inner neighbour,
-        // staged in LDS; 'periodic': opposite side, previous iterate), or 0
-        if (i < nx - 1) {
-          const uint8_t rt = c.rt_u[p];
-          if (is_bnd(rt)) {
-            double val = 0.0;
-            if (rt != RT_BND_ZERO) {
-              int jo, io;
-              bnd_src(rt, i, j, nx - 2, ny - 1, jo, io);
-              val = (rt == RT_BND_INF) ? su[io - t.i0 + 1][jo - t.j0 + 1] : xu_in[p2(m, jo, io)];
-            }
-            rmax = fmax(rmax, fabs(su[li][lj] - val));
-            su[li][lj] = val;
-          }
-        }
-        if (j < ny - 1) {
-          const uint8_t rt = c.rt_v[p];
-          if (is_bnd(rt)) {
-            double val = 0.0;
-            if (rt != RT_BND_ZERO) {
-              int jo, io;
-              bnd_src(rt, i, j, nx - 1, ny - 2, jo, io);
-              val = (rt == RT_BND_INF) ? sv[io - t.i0 + 1][jo - t.j0 + 1] : xv_in[p2(m, jo, io)];
-            }
-            rmax = fmax(rmax, fabs(sv[li][lj] - val));
-            sv[li][lj] = val;
-          }
-        }
-      }
-    }
-    __syncthreads();
-  }
-  // write back the tile interior
-  FOR_TILE_ROWS {
-    const int i = t.i0 + ii;
-    if (i >= nx || j >= ny) continue;
-    const long long p = p2(m, j, i);
-    xu_out[p] = su[ii + 1][lj];
-    xv_out[p] = sv[ii + 1][lj];
-  }
-  if (track) {
-    const double w = wave_max(rmax);
-    if (threadIdx.x == 0) red[threadIdx.y] = w;
-    __syncthreads();
-    if (tid == 0) {
-      double r = red[0];
-      for (int q = 1; q < IMAU_WG_I; ++q) r = fmax(r, red[q]);
-      // the slot is global memory: address it as such so that no generic-pointer
-      // LDS test is generated around the 64-bit atomic
-      typedef __attribute__((address_space(1))) unsigned long long gu64;
-      __hip_atomic_fetch_max((gu64 *)(resmax + t.g), (unsigned long long)__double_as_longlong(r), __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
-}
 
 #undef su
 #undef sv
@@ -1223,12 +1087,15 @@ void setup1(const Launch &L, const KP &kp, const Fields &f, const double *yg) {
 void setup2(const Launch &L, const KP &kp, const Fields &f) {
   hipLaunchKernelGGL(k_setup2, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f);
 }
-void visc(const Launch &L, const KP &kp, const Fields &f, bool diva, const double *visc_in, double *visc_out,
-          const double *xg, const double *yg, const int *active) {
+void visc3d(const Launch &L, const KP &kp, const Fields &f, bool diva, const double *visc_in, double *visc_out,
+            const int *active) {
   // n = 3 (every shipped configuration) takes the cube-root form of the viscosity
   const bool third = kp.n_flow == 3.0;
   auto kv = diva ? (third ? k_visc<true, true> : k_visc<true, false>) : (third ? k_visc<false, true> : k_visc<false, false>);
-  hipLaunchKernelGGL(kv, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, visc_in, visc_out, xg, yg, active);
+  hipLaunchKernelGGL(kv, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, visc_in, visc_out, active);
+}
+void beta(const Launch &L, const KP &kp, const Fields &f, bool diva, const double *xg, const double *yg,
+          const int *active) {
   if (diva) hipLaunchKernelGGL(k_beta<true>, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, xg, yg, active);
   else hipLaunchKernelGGL(k_beta<false>, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, xg, yg, active);
 }
@@ -1237,11 +1104,6 @@ void coef(const Launch &L, const KP &kp, const Fields &f, bool diva, const int *
 }
 void init_x(const Launch &L, const Fields &f, double *xu, double *xv) {
   hipLaunchKernelGGL(k_init_x, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, f, xu, xv);
-}
-void sweep(const Launch &L, const KP &kp, const Coef &c, const double *xu_in, const double *xv_in, double *xu_out,
-           double *xv_out, const int *active, double omega, bool track, unsigned long long *resmax) {
-  hipLaunchKernelGGL(k_sweep, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, to_p(c), xu_in, xv_in, xu_out, xv_out,
-                     active, omega, track ? 1 : 0, resmax);
 }
 void post(const Launch &L, const KP &kp, const Fields &f, const double *xu, const double *xv, const int *active,
           bool diva, double *partial, const double *relax) {

@@ -62,7 +62,7 @@ def load_library(path: str = LIB_PATH):
     L.imau_vel_problem_destroy.argtypes = [vp]
     L.imau_vel_problem_destroy.restype = None
     L.imau_vel_solve_diva_batch.argtypes = [vp, ctypes.c_int32, P(ImauGrid), P(ImauCfg), P(ImauIceIO)]
-    L.imau_vel_kernel_timing.argtypes = [vp, ctypes.c_int32, P(ctypes.c_int64), P(ctypes.c_double)]
+    L.imau_vel_kernel_timing.argtypes = [vp, ctypes.c_int32, P(ctypes.c_int64), P(ctypes.c_double), P(ctypes.c_int64)]
     _lib = L
     return L
 
@@ -116,11 +116,15 @@ class Context:
                                                        ctypes.byref(r)))
         return r.value
 
-    def kernel_timing(self, enable: bool):
-        """(sweep launches, summed ms) since the last call; enables/disables event timing."""
-        n, ms = ctypes.c_int64(), ctypes.c_double()
-        self._check(self.L.imau_vel_kernel_timing(self.h, 1 if enable else 0, ctypes.byref(n), ctypes.byref(ms)))
-        return n.value, ms.value
+    def kernel_timing(self, what):
+        """(launches, summed ms, summed cells of the processed members) timed since
+        the last call; then times `what` next: False/"off", True/"smoother" (one
+        fine-level smoother sweep) or "visc" (the 3D viscosity kernel)."""
+        code = {False: abi.IMAU_TIME_OFF, True: abi.IMAU_TIME_SMOOTHER, "off": abi.IMAU_TIME_OFF,
+                "smoother": abi.IMAU_TIME_SMOOTHER, "visc": abi.IMAU_TIME_VISC}[what]
+        n, ms, cells = ctypes.c_int64(), ctypes.c_double(), ctypes.c_int64()
+        self._check(self.L.imau_vel_kernel_timing(self.h, code, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(cells)))
+        return n.value, ms.value, cells.value
 
     def problem(self, grids, cfg, states):
         return Problem(self, grids, cfg, states)

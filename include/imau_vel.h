@@ -220,11 +220,17 @@ void imau_vel_finalize(imau_ctx *ctx);
 /* Text of the last error on this context (or of the last failed init). */
 const char *imau_vel_last_error(const imau_ctx *ctx);
 
-/* Kernel timing (no reference counterpart): returns the number of SOR-sweep
- * launches and their summed duration measured with HIP events on the
- * context's stream since the previous call, then resets the counters and
- * turns the event timing on (enable != 0) or off. */
-int  imau_vel_kernel_timing(imau_ctx *ctx, int32_t enable, int64_t *sweep_launches, double *sweep_ms);
+/* Kernel timing (no reference counterpart): returns the number of timed
+ * launches, their summed duration, measured with HIP events on the context's
+ * stream, and the summed a-grid cells of the members each timed launch
+ * processed (members that have converged are skipped), since the previous
+ * call; then resets the counters and
+ * selects what is timed next: IMAU_TIME_OFF, IMAU_TIME_SMOOTHER (one
+ * fine-level smoother sweep: the five colour launches of a multigrid sweep,
+ * a tile Gauss-Seidel application, or one SOR sweep) or IMAU_TIME_VISC (the
+ * 3D viscosity kernel of a Picard iteration). */
+enum { IMAU_TIME_OFF = 0, IMAU_TIME_SMOOTHER = 1, IMAU_TIME_VISC = 2 };
+int  imau_vel_kernel_timing(imau_ctx *ctx, int32_t what, int64_t *launches, double *ms, int64_t *cells);
 
 /* Fills cfg with the reference defaults (src/configuration_module.f90). */
 void imau_vel_default_config(imau_cfg *cfg);

@@ -11,6 +11,19 @@
  * linearised staggered DIVA/SSA and either the reference SOR
  * (src/utilities_module.f90:644-718) or an exact banded LU solve standing in
  * for PETSc's KSP (src/petsc_module.F90:24-88).
+ *
+ * Pinning (tests/test_oracle_pinning.py): closed forms, invariants and
+ * library routines, since the reference has no test suite, no golden vectors
+ * and cannot be built here.  PARITY UNPINNED (no closed form, invariant or
+ * fixture reaches these parts; they follow the cited reference lines only):
+ *   - calving-front and shelf-land rows of choice_ice_margin = 'BC'
+ *     (src/ice_velocity_module.f90:1762-1865, 2192-2341);
+ *   - the corner logic of 'periodic' / 'zero' domain boundaries (:1932-2191);
+ *   - the CISM quadrant rotations with their ftol perturbations
+ *     (src/general_ice_model_data_module.f90:539-574, 608-680);
+ *   - the cold start (0/0 in du/dz resolved to visc_eff_min through fmax);
+ *   - Picard trajectories and SOR iteration counts at reference tolerances;
+ *   - PETSc's own Krylov iterations (replaced by the exact banded LU).
  */
 #ifndef IMAU_ORACLE_H
 #define IMAU_ORACLE_H
