@@ -198,7 +198,7 @@ def main():
         achieved = bytes_per_cell * cells_t / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
         traffic = traffic_per_cell * cells_t / max(n, 1) if traffic_per_cell else None
         return dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                    traffic=traffic, kernel=kernel, avg_launch_us=avg_us, launches=n,
+                    traffic=traffic, kernel=kernel, avg_launch_us=avg_us, launches=n, cells_processed=cells_t,
                     bytes_per_launch=per_launch, bytes_per_cell=bytes_per_cell)
 
     visc_traffic = VISC_TRAFFIC_BYTES_PER_CELL if args.workload == "ensemble" else None

@@ -17,6 +17,11 @@
 #define IMAU_TI 8           // tile extent along i
 #define IMAU_WG_I 4         // wavefronts per workgroup along i
 #define IMAU_WG (IMAU_TJ * IMAU_WG_I)
+// fused smoother sweep: tiles of IMAU_FTI x IMAU_FTJ cells staged in LDS with a
+// recomputed halo of IMAU_FH cells (four colour phases + the boundary rows)
+#define IMAU_FTI 16
+#define IMAU_FTJ 64
+#define IMAU_FH 5
 #define IMAU_NZMAX 32
 
 // physical constants, src/parameters_module.f90:11-24

@@ -106,7 +106,9 @@ struct imau_problem {
   DBuf<double> relax_d;  // per-member Picard relaxation (adaptive mode)
   std::vector<std::unique_ptr<MGLevel>> mg;  // coarse levels 1.. of the multigrid preconditioner
   DBuf<int> mg_depth;                        // per member: its coarsest level
-  DBuf<float> mg_f32;                        // FP32 fine-level coefficients, rhs and iterate
+  DBuf<float> mg_f32;                        // FP32 fine-level coefficients, rhs and iterates
+  DBuf<double> mg_z64;                       // FP64 scratch iterate of the fused smoother
+  DBuf<Tile> mg_ftiles;                      // tiles of the fused smoother
   imau::MGPlan plan;                         // device view of the levels
 
   Fields F{};
@@ -195,10 +197,21 @@ static void build_mg(imau_problem *P) {
   P->plan.ngrids = P->ngrids;
   P->plan.coarsest_sweeps = 10;
   const long long na = P->na;
-  P->mg_f32.alloc((size_t)10 * na);
+  P->mg_f32.alloc((size_t)12 * na);
   float *f = P->mg_f32.p;
-  P->plan.f32 = imau::MGFine32{f, f + na, f + 2 * na, f + 3 * na, f + 4 * na,
-                               f + 5 * na, f + 6 * na, f + 7 * na, f + 8 * na, f + 9 * na};
+  P->plan.f32 = imau::MGFine32{f, f + na, f + 2 * na, f + 3 * na, f + 4 * na, f + 5 * na,
+                               f + 6 * na, f + 7 * na, f + 8 * na, f + 9 * na, f + 10 * na, f + 11 * na};
+  P->mg_z64.alloc((size_t)2 * na);
+  P->plan.zu64 = P->mg_z64.p;
+  P->plan.zv64 = P->mg_z64.p + na;
+  std::vector<Tile> ft;
+  for (int g = 0; g < P->ngrids; ++g)
+    for (int i0 = 0; i0 < P->mem[g].nx; i0 += IMAU_FTI)
+      for (int j0 = 0; j0 < P->mem[g].ny; j0 += IMAU_FTJ) ft.push_back(Tile{g, i0, j0, 0});
+  P->mg_ftiles.alloc(ft.size());
+  HIPCHK(hipMemcpy(P->mg_ftiles.p, ft.data(), ft.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  P->plan.ftiles = P->mg_ftiles.p;
+  P->plan.nftiles = (int)ft.size();
 }
 
 // ---------------------------------------------------------------------------
@@ -299,6 +312,8 @@ void check_cfg(const imau_cfg *cfg) {
     fail(IMAU_ERR_CONFIG, "picard_relax_max must lie in [0, 1], got %g", cfg->picard_relax_max);
   if (cfg->picard_relax_max > 0.0 && cfg->stop_rule != IMAU_STOP_RESIDUAL)
     fail(IMAU_ERR_CONFIG, "adaptive Picard relaxation (picard_relax_max > 0) needs stop_rule IMAU_STOP_RESIDUAL");
+  if (cfg->mg_smoother != 0 && cfg->mg_smoother != 1)
+    fail(IMAU_ERR_CONFIG, "mg_smoother must be 0 (fused sweep) or 1 (one launch per colour), got %d", cfg->mg_smoother);
   if (cfg->mg_levels != 0 && cfg->mg_levels != 1)
     fail(IMAU_ERR_CONFIG, "mg_levels must be 0 (multigrid, automatic depth) or 1 (no multigrid), got %d",
          cfg->mg_levels);
@@ -533,6 +548,7 @@ void imau_vel_default_config(imau_cfg *c) {
   c->stop_rule = IMAU_STOP_PICARD;
   c->nonlinear_rtol = 1e-4;
   c->picard_relax_max = 0.0;
+  c->mg_smoother = 0;
 }
 
 int imau_vel_init(const imau_grid *grid, const imau_cfg *cfg, imau_ctx **out) {
@@ -861,6 +877,9 @@ struct Solver {
     if (!any) return;
     push_active(P->act_lin.p, act_lin);
     const bool fp32 = cfg->precision == IMAU_PREC_MIXED;
+    P->plan.fused = cfg->mg_smoother == 0;
+    P->plan.periodic = false;
+    for (int q = 0; q < 8; ++q) P->plan.periodic = P->plan.periodic || (&cfg->DIVA_boundary_BC_u_west)[q] == IMAU_BC_PERIODIC;
     if (use_mg) imau::mg_setup(L, kp, cf, P->plan, fp32, P->act_lin.p);
     auto precondition = [&](double *const *y, double *const *z) {
       if (use_mg) imau::mg_vcycle(L, kp, cf, P->plan, fp32, y[0], y[1], z[0], z[1], P->act_lin.p, &Solver::mg_mark, this);

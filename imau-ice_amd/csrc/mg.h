@@ -27,7 +27,7 @@ struct MGLevelDev {
 
 // FP32 copies of the fine-level coefficients and the FP32 right-hand side / iterate
 struct MGFine32 {
-  float *Na, *Nb, *Ncx, *Ncy, *bcx, *bcy, *yu, *yv, *zu, *zv;
+  float *Na, *Nb, *Ncx, *Ncy, *bcx, *bcy, *yu, *yv, *zu, *zv, *zu2, *zv2;
 };
 
 struct MGPlan {
@@ -35,7 +35,12 @@ struct MGPlan {
   const int *depth;  // per member: its coarsest level
   int ngrids;
   MGFine32 f32;
+  double *zu64, *zv64;   // FP64 scratch iterate of the fused smoother
+  const Tile *ftiles;    // IMAU_FTI x IMAU_FTJ tiles of the fused smoother
+  int nftiles;
   int coarsest_sweeps;
+  bool fused;            // one fused launch per fine-level sweep
+  bool periodic;         // some domain boundary is periodic (needs a separate boundary-row pass)
 };
 
 // marker callback around the first fine-level smoother sweep (which = 0 before, 1 after)

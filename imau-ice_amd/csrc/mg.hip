@@ -432,6 +432,116 @@ __global__ void __launch_bounds__(IMAU_WG) k_smooth0(const Member *__restrict__ 
   }
 }
 
+// One fused fine-level Gauss-Seidel sweep: the tile plus a halo of IMAU_FH
+// cells is staged in LDS from zin (zero when zin is null), the four colour
+// phases (u red, u black, v red, v black; reversed when reverse != 0) update
+// shrinking regions -- after phase p the values are exact up to IMAU_FH-1-p
+// cells beyond the tile, because each row couples only neighbours at distance
+// one -- and then the domain-boundary rows of the tile are set from their
+// sources (exact one cell beyond the tile).  The tile is written to zout, a
+// different buffer, so the result equals the one-launch-per-colour sweep
+// exactly.  Periodic boundary rows, whose sources lie across the domain, are
+// completed by a separate colour-4 pass of k_smooth0.
+#define FRI (IMAU_FTI + 2 * IMAU_FH)
+#define FRJ (IMAU_FTJ + 2 * IMAU_FH)
+
+template <class T>
+struct LAccT {
+  const T *base;  // LDS region, element (0,0) at global (ib, jb)
+  int ib, jb;
+  __device__ __forceinline__ T operator()(int jj, int ii) const { return base[(ii - ib) * FRJ + (jj - jb)]; }
+};
+
+template <class T>
+__global__ void __launch_bounds__(IMAU_WG) k_smooth_fused(const Member *__restrict__ mem, const Tile *__restrict__ ftiles,
+                                                          const KP kp, const CoefT<T> c, CVecT<T> zin, VecT<T> zout,
+                                                          CVecT<T> y, int reverse, const int *__restrict__ active) {
+  __shared__ T su[FRI * FRJ], sv[FRI * FRJ];
+  const Tile t = ftiles[blockIdx.x];
+  if (!active[t.g]) return;
+  const Member m = mem[t.g];
+  const int nx = m.nx, ny = m.ny;
+  const int ib = t.i0 - IMAU_FH, jb = t.j0 - IMAU_FH;
+  const int tid = threadIdx.y * IMAU_TJ + threadIdx.x;
+  for (int q = tid; q < FRI * FRJ; q += IMAU_WG) {
+    const int li = q / FRJ, lj = q - li * FRJ;
+    const int i = ib + li, j = jb + lj;
+    T u = (T)0.0, v = (T)0.0;
+    if (zin.u && i >= 0 && j >= 0 && i < nx && j < ny) {
+      const long long p = p2(m, j, i);
+      if (i < nx - 1) u = zin.u[p];
+      if (j < ny - 1) v = zin.v[p];
+    }
+    su[q] = u;
+    sv[q] = v;
+  }
+  __syncthreads();
+  const LAccT<T> U{su, ib, jb}, V{sv, ib, jb};
+  for (int ph = 0; ph < 4; ++ph) {
+    const int color = reverse ? 3 - ph : ph;
+    const bool isu = color < 2;
+    const int r = IMAU_FH - 1 - ph;
+    const int li0 = IMAU_FH - r, lj0 = IMAU_FH - r;
+    const int RW = IMAU_FTJ + 2 * r, n = (IMAU_FTI + 2 * r) * RW;
+    for (int q = tid; q < n; q += IMAU_WG) {
+      const int li = li0 + q / RW, lj = lj0 + q % RW;
+      const int i = ib + li, j = jb + lj;
+      if (i < 0 || j < 0 || i >= nx || j >= ny) continue;
+      if (((i + j) & 1) != (color & 1)) continue;
+      if (isu && i >= nx - 1) continue;
+      if (!isu && j >= ny - 1) continue;
+      const long long p = p2(m, j, i);
+      const uint8_t rt = isu ? c.rt_u[p] : c.rt_v[p];
+      if (is_bnd(rt)) continue;
+      T S, D;
+      if (isu) u_sd<T>(m, kp, c, j, i, rt, U, V, S, D);
+      else v_sd<T>(m, kp, c, j, i, rt, U, V, S, D);
+      (isu ? su : sv)[li * FRJ + lj] = ((isu ? y.u : y.v)[p] - S) / D;
+    }
+    __syncthreads();
+  }
+  // domain-boundary rows of the tile ('infinite' and 'zero'; sources are interior rows)
+  for (int q = tid; q < IMAU_FTI * IMAU_FTJ; q += IMAU_WG) {
+    const int li = IMAU_FH + q / IMAU_FTJ, lj = IMAU_FH + q % IMAU_FTJ;
+    const int i = ib + li, j = jb + lj;
+    if (i >= nx || j >= ny) continue;
+    const long long p = p2(m, j, i);
+    if (i < nx - 1) {
+      const uint8_t rt = c.rt_u[p];
+      if (rt == RT_BND_ZERO || rt == RT_BND_INF) {
+        T val = y.u[p];
+        if (rt == RT_BND_INF) {
+          int jo, io;
+          bnd_src(rt, i, j, nx - 2, ny - 1, jo, io);
+          val += U(jo, io);
+        }
+        su[li * FRJ + lj] = val;
+      }
+    }
+    if (j < ny - 1) {
+      const uint8_t rt = c.rt_v[p];
+      if (rt == RT_BND_ZERO || rt == RT_BND_INF) {
+        T val = y.v[p];
+        if (rt == RT_BND_INF) {
+          int jo, io;
+          bnd_src(rt, i, j, nx - 1, ny - 2, jo, io);
+          val += V(jo, io);
+        }
+        sv[li * FRJ + lj] = val;
+      }
+    }
+  }
+  __syncthreads();
+  for (int q = tid; q < IMAU_FTI * IMAU_FTJ; q += IMAU_WG) {
+    const int li = IMAU_FH + q / IMAU_FTJ, lj = IMAU_FH + q % IMAU_FTJ;
+    const int i = ib + li, j = jb + lj;
+    if (i >= nx || j >= ny) continue;
+    const long long p = p2(m, j, i);
+    if (i < nx - 1) zout.u[p] = su[li * FRJ + lj];
+    if (j < ny - 1) zout.v[p] = sv[li * FRJ + lj];
+  }
+}
+
 // level-1 right-hand side: R (y - A z) over the interior and calving-front rows
 template <class T>
 __global__ void __launch_bounds__(IMAU_WG) k_restrict0(const Member *__restrict__ mem, const KP kp, const CoefT<T> c,
@@ -595,17 +705,10 @@ void setup_t(const imau::Launch &F, const KP &kp, const imau::Coef &c, const ima
   }
 }
 
+// coarse part of the cycle: b(1) is set; x(1) is returned
 template <class T>
-void vcycle_t(const imau::Launch &F, const KP &kp, const CoefT<T> &cf, const imau::MGPlan &P, CVecT<T> y, VecT<T> z,
-              const int *act, imau::MGMark mark, void *mctx) {
-  hipStream_t s = F.s;
+void coarse_t(hipStream_t s, const imau::MGPlan &P, const int *act) {
   const int nl = (int)P.levels.size();
-  if (mark) mark(mctx, 0);
-  for (int col = 0; col < 5; ++col)
-    hipLaunchKernelGGL(k_smooth0<T>, dim3(F.ntiles), wg(), 0, s, F.mem, F.tiles, kp, cf, z, y, col, act);
-  if (mark) mark(mctx, 1);
-  const CLevelT<T> L1 = view<T>(P, 0);
-  hipLaunchKernelGGL(k_restrict0<T>, dim3(L1.ntiles), wg(), 0, s, F.mem, kp, cf, y, CVecT<T>{z.u, z.v}, L1, act);
   for (int l = 0; l < nl; ++l) {
     const CLevelT<T> C = view<T>(P, l);
     fill<T>(s, C.xu, 2 * C.na);
@@ -621,10 +724,45 @@ void vcycle_t(const imau::Launch &F, const KP &kp, const CoefT<T> &cf, const ima
     hipLaunchKernelGGL(k_cprolong<T>, dim3(C.ntiles), wg(), 0, s, C, N, act);
     for (int col = 3; col >= 0; --col) hipLaunchKernelGGL(k_cgs<T>, dim3(C.ntiles), wg(), 0, s, C, col, act);
   }
+}
+
+// one launch per colour; the cycle runs in place on z (zeroed by the caller)
+template <class T>
+void vcycle_t(const imau::Launch &F, const KP &kp, const CoefT<T> &cf, const imau::MGPlan &P, CVecT<T> y, VecT<T> z,
+              const int *act, imau::MGMark mark, void *mctx) {
+  hipStream_t s = F.s;
+  if (mark) mark(mctx, 0);
+  for (int col = 0; col < 5; ++col)
+    hipLaunchKernelGGL(k_smooth0<T>, dim3(F.ntiles), wg(), 0, s, F.mem, F.tiles, kp, cf, z, y, col, act);
+  if (mark) mark(mctx, 1);
+  const CLevelT<T> L1 = view<T>(P, 0);
+  hipLaunchKernelGGL(k_restrict0<T>, dim3(L1.ntiles), wg(), 0, s, F.mem, kp, cf, y, CVecT<T>{z.u, z.v}, L1, act);
+  coarse_t<T>(s, P, act);
   hipLaunchKernelGGL(k_prolong0<T>, dim3(F.ntiles), wg(), 0, s, F.mem, F.tiles, cf.rt_u, cf.rt_v, z, L1, act);
   for (int col = 3; col >= 0; --col)
     hipLaunchKernelGGL(k_smooth0<T>, dim3(F.ntiles), wg(), 0, s, F.mem, F.tiles, kp, cf, z, y, col, act);
   hipLaunchKernelGGL(k_smooth0<T>, dim3(F.ntiles), wg(), 0, s, F.mem, F.tiles, kp, cf, z, y, 4, act);
+}
+
+// fused sweeps: pre-smoothing from zero into za, post-smoothing from za into zb
+template <class T>
+void vcycle_fused_t(const imau::Launch &F, const KP &kp, const CoefT<T> &cf, const imau::MGPlan &P, CVecT<T> y,
+                    VecT<T> za, VecT<T> zb, const int *act, imau::MGMark mark, void *mctx) {
+  hipStream_t s = F.s;
+  if (mark) mark(mctx, 0);
+  hipLaunchKernelGGL(k_smooth_fused<T>, dim3(P.nftiles), wg(), 0, s, F.mem, P.ftiles, kp, cf, CVecT<T>{nullptr, nullptr},
+                     za, y, 0, act);
+  if (P.periodic)
+    hipLaunchKernelGGL(k_smooth0<T>, dim3(F.ntiles), wg(), 0, s, F.mem, F.tiles, kp, cf, za, y, 4, act);
+  if (mark) mark(mctx, 1);
+  const CLevelT<T> L1 = view<T>(P, 0);
+  hipLaunchKernelGGL(k_restrict0<T>, dim3(L1.ntiles), wg(), 0, s, F.mem, kp, cf, y, CVecT<T>{za.u, za.v}, L1, act);
+  coarse_t<T>(s, P, act);
+  hipLaunchKernelGGL(k_prolong0<T>, dim3(F.ntiles), wg(), 0, s, F.mem, F.tiles, cf.rt_u, cf.rt_v, za, L1, act);
+  hipLaunchKernelGGL(k_smooth_fused<T>, dim3(P.nftiles), wg(), 0, s, F.mem, P.ftiles, kp, cf, CVecT<T>{za.u, za.v},
+                     zb, y, 1, act);
+  if (P.periodic)
+    hipLaunchKernelGGL(k_smooth0<T>, dim3(F.ntiles), wg(), 0, s, F.mem, F.tiles, kp, cf, zb, y, 4, act);
 }
 
 }  // namespace
@@ -644,12 +782,21 @@ void mg_setup(const Launch &F, const KP &kp, const Coef &c, const MGPlan &P, boo
 void mg_vcycle(const Launch &F, const KP &kp, const Coef &c, const MGPlan &P, bool fp32, const double *yu,
                const double *yv, double *zu, double *zv, const int *active, MGMark mark, void *mark_ctx) {
   if (fp32) {
-    VecT<float> y32{P.f32.yu, P.f32.yv}, z32{P.f32.zu, P.f32.zv};
+    VecT<float> y32{P.f32.yu, P.f32.yv}, z32{P.f32.zu, P.f32.zv}, z32b{P.f32.zu2, P.f32.zv2};
     hipLaunchKernelGGL(k_prep<float>, dim3(F.ntiles), wg(), 0, F.s, F.mem, F.tiles, CVecT<double>{yu, yv}, y32, z32,
                        active);
-    vcycle_t<float>(F, kp, coef32(P.f32, c), P, CVecT<float>{y32.u, y32.v}, z32, active, mark, mark_ctx);
+    const CoefT<float> cf = coef32(P.f32, c);
+    if (P.fused) {
+      vcycle_fused_t<float>(F, kp, cf, P, CVecT<float>{y32.u, y32.v}, z32, z32b, active, mark, mark_ctx);
+      z32 = z32b;
+    } else {
+      vcycle_t<float>(F, kp, cf, P, CVecT<float>{y32.u, y32.v}, z32, active, mark, mark_ctx);
+    }
     hipLaunchKernelGGL(k_fin, dim3(F.ntiles), wg(), 0, F.s, F.mem, F.tiles, CVecT<float>{z32.u, z32.v},
                        VecT<double>{zu, zv}, active);
+  } else if (P.fused) {
+    vcycle_fused_t<double>(F, kp, coef64(c), P, CVecT<double>{yu, yv}, VecT<double>{P.zu64, P.zv64},
+                           VecT<double>{zu, zv}, active, mark, mark_ctx);
   } else {
     VecT<double> z{zu, zv};
     hipLaunchKernelGGL(k_prep<double>, dim3(F.ntiles), wg(), 0, F.s, F.mem, F.tiles, CVecT<double>{yu, yv},

@@ -88,6 +88,7 @@ def default_config(**overrides) -> ImauCfg:
     c.mg_levels = 0
     c.check_interval = 0
     c.picard_relax_max = 0.0
+    c.mg_smoother = 0
     set_config(c, **overrides)
     return c
 

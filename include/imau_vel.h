@@ -140,6 +140,9 @@ typedef struct imau_cfg {
                               iteration whose FP64 residual fell, back to
                               DIVA_visc_it_relax after one that rose, capped at
                               this value (needs IMAU_STOP_RESIDUAL)          */
+  int32_t mg_smoother;     /* multigrid fine-level smoother: 0 = one fused launch
+                              per sweep (tiles with a recomputed halo in LDS),
+                              1 = one launch per colour; identical results     */
 } imau_cfg;
 
 /* The fields of type_ice_model (src/data_types_module.f90:56-325) that the

@@ -230,3 +230,28 @@ def test_adaptive_relaxation_same_fixed_point(ctx):
     for name in ("u_vav_cx", "v_vav_cy", "u_3D_cx", "taub_cx"):
         e = relerr(adapt[name], ref[name])
         assert e < 1e-6, (name, e)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("case", ["random65", "channel_small", "dome_periodic"])
+def test_fused_smoother_equals_colour_launches(ctx, case, precision):
+    """The fused LDS sweep (recomputed halo) gives bitwise the same solve as one
+    launch per colour, including 'zero' and 'periodic' domain boundaries."""
+    if case == "random65":
+        g, cfg, st = geometry.random_geometry(65, 65, seed=7)
+    elif case == "channel_small":
+        g, cfg, st = geometry.channel(nx=61, ny=21, dx=8e3, seed=3)
+    else:
+        g, cfg, st = geometry.dome()
+        config.set_config(cfg, **{"DIVA_boundary_BC_%s_%s" % (c, s): "periodic"
+                                  for c in ("u", "v") for s in ("west", "east", "south", "north")})
+    config.set_config(cfg, stop_rule=1, nonlinear_rtol=1e-6, DIVA_visc_it_nit=300, precision=precision, mg_levels=0)
+    out = []
+    for sm in (0, 1):
+        config.set_config(cfg, mg_smoother=sm)
+        s = st.copy()
+        ctx.solve_diva(g, cfg, s)
+        out.append(s)
+    assert out[0].stats["linear_sweeps"] == out[1].stats["linear_sweeps"]
+    np.testing.assert_array_equal(out[0]["u_vav_cx"], out[1]["u_vav_cx"])
+    np.testing.assert_array_equal(out[0]["v_vav_cy"], out[1]["v_vav_cy"])
