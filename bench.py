@@ -59,14 +59,14 @@ def VISC_BYTES_PER_CELL(nz):
 VISC_TRAFFIC_BYTES_PER_CELL = None
 SWEEP_TRAFFIC_BYTES_PER_CELL = {}
 METHODS = {
-    "mixed": dict(precision="mixed", mg_levels=0, picard_relax_max=1.0,
-                  text="BiCGStab in FP64 (operator, residuals, update) preconditioned by an FP32 aggregation-"
+    "mixed": dict(precision="mixed", mg_levels=0, picard_relax_max=1.0, ksp_type=1,
+                  text="GCR in FP64 (operator, residuals, update) right-preconditioned by an FP32 aggregation-"
                        "multigrid V-cycle; FP64 viscosity/beta; Picard loop with adaptive relaxation "
                        "(0.4 raised x1.25 per residual decrease up to 1.0, reset on an increase)"),
-    "fp64": dict(precision="fp64", mg_levels=0, picard_relax_max=0.0,
+    "fp64": dict(precision="fp64", mg_levels=0, picard_relax_max=0.0, ksp_type=0,
                  text="FP64 BiCGStab preconditioned by an FP64 aggregation-multigrid V-cycle; reference Picard "
                       "loop (fixed relaxation 0.4)"),
-    "plain": dict(precision="fp64", mg_levels=1, picard_relax_max=0.0,
+    "plain": dict(precision="fp64", mg_levels=1, picard_relax_max=0.0, ksp_type=0,
                   text="FP64 BiCGStab preconditioned by tile Gauss-Seidel (no multigrid); reference Picard loop "
                        "(fixed relaxation 0.4)"),
 }
@@ -74,7 +74,7 @@ METHODS = {
 
 def use_method(cfg, meth, relax_max=None):
     from imau_vel import config
-    config.set_config(cfg, precision=meth["precision"], mg_levels=meth["mg_levels"],
+    config.set_config(cfg, precision=meth["precision"], mg_levels=meth["mg_levels"], ksp_type=meth["ksp_type"],
                       picard_relax_max=meth["picard_relax_max"] if relax_max is None else relax_max)
 
 
@@ -149,6 +149,7 @@ def main():
     ap.add_argument("--method", choices=tuple(METHODS), default="mixed")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison legs")
     ap.add_argument("--relax-max", type=float, default=None, help="override the headline's picard_relax_max")
+    ap.add_argument("--ksp", choices=("bicgstab", "gcr"), default=None, help="override the headline's Krylov method")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run 5 plain device-to-device copies of BYTES each (counter calibration)")
     args = ap.parse_args()
@@ -171,7 +172,10 @@ def main():
     prob = ctx.problem(grids, cfg, states)
 
     # headline method
-    meth = METHODS[args.method]
+    meth = dict(METHODS[args.method])
+    if args.ksp:
+        meth["ksp_type"] = 1 if args.ksp == "gcr" else 0
+        meth["text"] = meth["text"].replace("GCR", "BiCGStab") if args.ksp == "bicgstab" else meth["text"]
     use_method(cfg, meth, args.relax_max)
     dt = timed_solves(prob, cfg, args.steps, args.warmup)
     out = prob.download()

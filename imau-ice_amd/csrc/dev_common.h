@@ -22,6 +22,14 @@
 #define IMAU_FTI 16
 #define IMAU_FTJ 64
 #define IMAU_FH 5
+
+// restart length of the GCR Krylov solver
+#define IMAU_GCR_M 4
+// pointers to the stored GCR directions (u and v components)
+struct GList {
+  const double *u[IMAU_GCR_M];
+  const double *v[IMAU_GCR_M];
+};
 #define IMAU_NZMAX 32
 
 // physical constants, src/parameters_module.f90:11-24

@@ -104,6 +104,7 @@ struct imau_problem {
   DBuf<int> act_pic, act_lin;
   DBuf<double> sc;    // per-member Krylov scalars
   DBuf<double> relax_d;  // per-member Picard relaxation (adaptive mode)
+  DBuf<double> gcr_dirs; // GCR directions Q_i, Z_i (allocated on first use)
   std::vector<std::unique_ptr<MGLevel>> mg;  // coarse levels 1.. of the multigrid preconditioner
   DBuf<int> mg_depth;                        // per member: its coarsest level
   DBuf<float> mg_f32;                        // FP32 fine-level coefficients, rhs and iterates
@@ -312,6 +313,8 @@ void check_cfg(const imau_cfg *cfg) {
     fail(IMAU_ERR_CONFIG, "picard_relax_max must lie in [0, 1], got %g", cfg->picard_relax_max);
   if (cfg->picard_relax_max > 0.0 && cfg->stop_rule != IMAU_STOP_RESIDUAL)
     fail(IMAU_ERR_CONFIG, "adaptive Picard relaxation (picard_relax_max > 0) needs stop_rule IMAU_STOP_RESIDUAL");
+  if (cfg->ksp_type != 0 && cfg->ksp_type != 1)
+    fail(IMAU_ERR_CONFIG, "ksp_type must be 0 (BiCGStab) or 1 (GCR), got %d", cfg->ksp_type);
   if (cfg->mg_smoother != 0 && cfg->mg_smoother != 1)
     fail(IMAU_ERR_CONFIG, "mg_smoother must be 0 (fused sweep) or 1 (one launch per colour), got %d", cfg->mg_smoother);
   if (cfg->mg_levels != 0 && cfg->mg_levels != 1)
@@ -549,6 +552,7 @@ void imau_vel_default_config(imau_cfg *c) {
   c->nonlinear_rtol = 1e-4;
   c->picard_relax_max = 0.0;
   c->mg_smoother = 0;
+  c->ksp_type = 0;
 }
 
 int imau_vel_init(const imau_grid *grid, const imau_cfg *cfg, imau_ctx **out) {
@@ -646,7 +650,7 @@ int imau_vel_problem_create(imau_ctx *ctx, int32_t ngrids, const imau_grid *grid
   P->rt.alloc((size_t)2 * P->na);
   P->f3.alloc((size_t)F3_COUNT * P->na * cfg->nz);
   P->stage.alloc(max3);
-  P->partial.alloc((size_t)4 * P->tiles.size());
+  P->partial.alloc((size_t)(2 * IMAU_GCR_M + 2) * P->tiles.size());
   P->msum.alloc((size_t)4 * ngrids);
   P->resmax.alloc(ngrids);
   P->act_pic.alloc(ngrids);
@@ -792,6 +796,7 @@ struct Solver {
     push_active(P->act_lin.p, act_lin);
     imau::init_x(L, P->F, P->F.xu0, P->F.xv0);
     if (cfg->DIVA_choice_matrix_solver == IMAU_SOLVER_SOR) sor_solve();
+    else if (cfg->ksp_type == 1) gcr_solve();
     else krylov_solve();
     time_collect();
   }
@@ -915,6 +920,94 @@ struct Solver {
       }
       if (!any) break;
       push_active(P->act_lin.p, act_lin);
+    }
+  }
+
+  // preconditioner application z = M^-1 y shared by the Krylov methods
+  void precondition(const imau::Coef &cf, bool use_mg, bool fp32, const double *yu, const double *yv, double *zu,
+                    double *zv) {
+    if (use_mg) {
+      imau::mg_vcycle(L, kp, cf, P->plan, fp32, yu, yv, zu, zv, P->act_lin.p, &Solver::mg_mark, this);
+    } else {
+      time_begin();
+      imau::prec_tile(L, kp, cf, yu, yv, zu, zv, P->act_lin.p, 1);
+      time_end();
+    }
+  }
+
+  // KSP semantics as in krylov_solve, with right-preconditioned GCR restarted
+  // every IMAU_GCR_M directions: per iteration z = M^-1 r, q = A z, q and z
+  // orthogonalised against the stored directions, x += alpha z, r -= alpha q
+  void gcr_solve() {
+    const int G = P->ngrids;
+    const long long na = P->na;
+    if (!P->gcr_dirs.p) P->gcr_dirs.alloc((size_t)4 * IMAU_GCR_M * na);
+    GList Q{}, Z{};
+    double *Qd[2][IMAU_GCR_M], *Zd[2][IMAU_GCR_M];
+    for (int k = 0; k < IMAU_GCR_M; ++k) {
+      for (int c2 = 0; c2 < 2; ++c2) {
+        Qd[c2][k] = P->gcr_dirs.p + (size_t)(2 * k + c2) * na;
+        Zd[c2][k] = P->gcr_dirs.p + (size_t)(2 * IMAU_GCR_M + 2 * k + c2) * na;
+      }
+      Q.u[k] = Qd[0][k]; Q.v[k] = Qd[1][k];
+      Z.u[k] = Zd[0][k]; Z.v[k] = Zd[1][k];
+    }
+    double *X[2] = {P->F.xu0, P->F.xv0};
+    double *R[2] = {f2p(P, F_ru), f2p(P, F_rv)}, *RH[2] = {f2p(P, F_rhu), f2p(P, F_rhv)};
+    double *Pp[2] = {f2p(P, F_pu), f2p(P, F_pv)}, *Vv[2] = {f2p(P, F_vu), f2p(P, F_vv)};
+    const imau::Coef cf = coef();
+    const bool use_mg = cfg->mg_levels != 1 && !P->mg.empty();
+    const bool fp32 = cfg->precision == IMAU_PREC_MIXED;
+    std::vector<double> out(2 * (size_t)G), target(G, 0.0);
+    auto fetch = [&]() {
+      HIPCHK(hipMemcpyAsync(out.data(), P->msum.p, 2 * G * sizeof(double), hipMemcpyDeviceToHost, c->s));
+      HIPCHK(hipStreamSynchronize(c->s));
+    };
+    // r = b - A x and the norms of r and b (the BiCGStab start kernel)
+    imau::bicg_init(L, kp, cf, X[0], X[1], R[0], R[1], RH[0], RH[1], Pp[0], Pp[1], Vv[0], Vv[1], P->act_lin.p,
+                    P->partial.p);
+    imau::member_sum2(c->s, G, P->tstart_d.p, P->partial.p, P->msum.p);
+    kcheck();
+    fetch();
+    bool any = false;
+    for (int g = 0; g < G; ++g) {
+      if (!act_lin[g]) continue;
+      const double r0 = std::sqrt(out[2 * g]), bn = std::sqrt(out[2 * g + 1]);
+      if (!std::isfinite(r0)) fail(IMAU_ERR_NAN, "member %d: non-finite initial linear residual", g);
+      const double ref = (cfg->stop_rule == IMAU_STOP_RESIDUAL) ? r0 : bn;
+      target[g] = std::max(petsc_rtol * ref, petsc_abstol);
+      if (r0 <= target[g]) act_lin[g] = 0;
+      else any = true;
+    }
+    if (!any) return;
+    push_active(P->act_lin.p, act_lin);
+    P->plan.fused = cfg->mg_smoother == 0;
+    P->plan.periodic = false;
+    for (int q = 0; q < 8; ++q) P->plan.periodic = P->plan.periodic || (&cfg->DIVA_boundary_BC_u_west)[q] == IMAU_BC_PERIODIC;
+    if (use_mg) imau::mg_setup(L, kp, cf, P->plan, fp32, P->act_lin.p);
+    const int maxit = sor_nit > 0 ? sor_nit : 10000;
+    int nj = 0;
+    for (int it = 0; it < maxit; ++it) {
+      precondition(cf, use_mg, fp32, R[0], R[1], Zd[0][nj], Zd[1][nj]);
+      imau::apply(L, kp, cf, Zd[0][nj], Zd[1][nj], Qd[0][nj], Qd[1][nj], nullptr, nullptr, P->act_lin.p, nullptr);
+      imau::gcr_dots(L, Qd[0][nj], Qd[1][nj], R[0], R[1], Q, nj, P->act_lin.p, P->partial.p);
+      imau::gcr_scalar(c->s, G, P->tstart_d.p, P->partial.p, P->sc.p, P->act_lin.p, nj);
+      imau::gcr_update(L, Q, Z, nj, X[0], X[1], R[0], R[1], P->sc.p, P->act_lin.p, P->partial.p);
+      imau::member_sum2(c->s, G, P->tstart_d.p, P->partial.p, P->msum.p);
+      kcheck();
+      fetch();
+      any = false;
+      for (int g = 0; g < G; ++g) {
+        if (!act_lin[g]) continue;
+        P->sweeps[g] += 1;
+        const double rn = std::sqrt(out[2 * g]);
+        if (!std::isfinite(rn)) fail(IMAU_ERR_NAN, "member %d: non-finite linear residual", g);
+        if (rn <= target[g]) act_lin[g] = 0;
+        else any = true;
+      }
+      if (!any) break;
+      push_active(P->act_lin.p, act_lin);
+      nj = (nj + 1 == IMAU_GCR_M) ? 0 : nj + 1;
     }
   }
 

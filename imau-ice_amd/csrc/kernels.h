@@ -63,4 +63,11 @@ void prec_tile(const Launch &L, const KP &kp, const Coef &c, const double *yu, c
 void sor_color(const Launch &L, const KP &kp, const Coef &c, double *xu, double *xv, int color, double omega,
                const int *active, bool track, unsigned long long *resmax, const double *ru = nullptr,
                const double *rv = nullptr);
+// GCR: partial dots, per-member scalars, orthogonalising update
+void gcr_dots(const Launch &L, const double *qu, const double *qv, const double *ru, const double *rv, const GList &Q,
+              int nj, const int *active, double *partial);
+void gcr_scalar(hipStream_t s, int ngrids, const int *tstart, const double *partial, double *sc, const int *active,
+                int nj);
+void gcr_update(const Launch &L, const GList &Q, const GList &Z, int nj, double *xu, double *xv, double *ru, double *rv,
+                const double *sc, const int *active, double *partial);
 }  // namespace imau

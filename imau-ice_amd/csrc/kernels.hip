@@ -1549,3 +1549,142 @@ void sor_color(const Launch &L, const KP &kp, const Coef &c, double *xu, double 
                      active, track ? 1 : 0, resmax, CVec2{ru, rv});
 }
 }  // namespace imau
+
+// ---------------------------------------------------------------------------
+// GCR (generalised conjugate residual, restarted every IMAU_GCR_M directions),
+// right-preconditioned: one preconditioner application and one operator
+// application per iteration.  For each member the search directions Q_i = A Z_i
+// are kept orthonormal; scalars live on the device (sc[g*8 + ...]).
+// ---------------------------------------------------------------------------
+namespace gcr {
+enum { BETA = 0, INVN = IMAU_GCR_M, ALPHA = IMAU_GCR_M + 1, NV = 2 * IMAU_GCR_M + 2 };
+}
+
+// partial sums per tile: (q,Q_i) and (r,Q_i) for i < nj, (q,q), (r,q)
+__global__ void __launch_bounds__(IMAU_WG) k_gcr_dots(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                      CVec2 q, CVec2 r, GList Q, int nj,
+                                                      const int *__restrict__ active, double *__restrict__ partial) {
+  TILE_SETUP
+  double acc[gcr::NV];
+#pragma unroll
+  for (int k = 0; k < gcr::NV; ++k) acc[k] = 0.0;
+  if (j < m.ny && active[t.g]) {
+    FOR_TILE_ROWS {
+      const int i = t.i0 + ii;
+      if (i >= m.nx) break;
+      const long long p = p2(m, j, i);
+      for (int comp = 0; comp < 2; ++comp) {
+        if (comp == 0 ? (i >= m.nx - 1) : (j >= m.ny - 1)) continue;
+        const double qv = (comp == 0 ? q.u : q.v)[p], rv = (comp == 0 ? r.u : r.v)[p];
+#pragma unroll
+        for (int k = 0; k < IMAU_GCR_M; ++k)
+          if (k < nj) {
+            const double w = (comp == 0 ? Q.u[k] : Q.v[k])[p];
+            acc[k] += qv * w;
+            acc[IMAU_GCR_M + k] += rv * w;
+          }
+        acc[2 * IMAU_GCR_M] += qv * qv;
+        acc[2 * IMAU_GCR_M + 1] += rv * qv;
+      }
+    }
+  }
+  block_sum_store<gcr::NV>(acc, partial + gcr::NV * (long long)blockIdx.x);
+}
+
+// per member: beta_i = (q,Q_i), q' = q - sum beta_i Q_i, |q'| from the Gram
+// identity, alpha = (r, q'/|q'|)
+__global__ void __launch_bounds__(256) k_gcr_scalar(const int *__restrict__ tstart, const double *__restrict__ partial,
+                                                    double *__restrict__ sc, const int *__restrict__ active, int nj) {
+  const int g = blockIdx.x;
+  if (!active[g]) return;
+  const int a = tstart[g], b = tstart[g + 1];
+  __shared__ double red[gcr::NV][256];
+  double s[gcr::NV];
+#pragma unroll
+  for (int q = 0; q < gcr::NV; ++q) s[q] = 0.0;
+  for (int k = a + (int)threadIdx.x; k < b; k += 256)
+#pragma unroll
+    for (int q = 0; q < gcr::NV; ++q) s[q] += partial[(long long)gcr::NV * k + q];
+#pragma unroll
+  for (int q = 0; q < gcr::NV; ++q) red[q][threadIdx.x] = s[q];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int q = 0; q < gcr::NV; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  double *S = sc + g * bicg::NSC;
+  double qq = red[2 * IMAU_GCR_M][0], rq = red[2 * IMAU_GCR_M + 1][0];
+  for (int k = 0; k < IMAU_GCR_M; ++k) {
+    const double beta = (k < nj) ? red[k][0] : 0.0;
+    S[gcr::BETA + k] = beta;
+    qq -= beta * beta;
+    rq -= beta * ((k < nj) ? red[IMAU_GCR_M + k][0] : 0.0);
+  }
+  const bool ok = qq > 0.0 && isfinite(qq);
+  const double inv = ok ? 1.0 / sqrt(qq) : 0.0;
+  S[gcr::INVN] = inv;
+  S[gcr::ALPHA] = ok ? rq * inv : 0.0;
+}
+
+// Q_nj = (q - sum beta_i Q_i)/|q'|, Z_nj likewise, x += alpha Z_nj, r -= alpha Q_nj;
+// q and z are read from, and replaced in, the slots Q_nj and Z_nj; partial (r,r)
+__global__ void __launch_bounds__(IMAU_WG) k_gcr_update(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                        GList Q, GList Z, int nj, Vec2 x, Vec2 r,
+                                                        const double *__restrict__ sc, const int *__restrict__ active,
+                                                        double *__restrict__ partial) {
+  TILE_SETUP
+  double acc[2] = {0.0, 0.0};
+  if (j < m.ny && active[t.g]) {
+    const double *S = sc + t.g * bicg::NSC;
+    const double inv = S[gcr::INVN], al = S[gcr::ALPHA];
+    double be[IMAU_GCR_M];
+#pragma unroll
+    for (int k = 0; k < IMAU_GCR_M; ++k) be[k] = S[gcr::BETA + k];
+    FOR_TILE_ROWS {
+      const int i = t.i0 + ii;
+      if (i >= m.nx) break;
+      const long long p = p2(m, j, i);
+      for (int comp = 0; comp < 2; ++comp) {
+        if (comp == 0 ? (i >= m.nx - 1) : (j >= m.ny - 1)) continue;
+        double *qn = const_cast<double *>(comp == 0 ? Q.u[nj] : Q.v[nj]);
+        double *zn = const_cast<double *>(comp == 0 ? Z.u[nj] : Z.v[nj]);
+        double qv = qn[p], zv = zn[p];
+#pragma unroll
+        for (int k = 0; k < IMAU_GCR_M; ++k)
+          if (k < nj) {
+            qv -= be[k] * (comp == 0 ? Q.u[k] : Q.v[k])[p];
+            zv -= be[k] * (comp == 0 ? Z.u[k] : Z.v[k])[p];
+          }
+        qv *= inv;
+        zv *= inv;
+        qn[p] = qv;
+        zn[p] = zv;
+        (comp == 0 ? x.u : x.v)[p] += al * zv;
+        double &rr = (comp == 0 ? r.u : r.v)[p];
+        rr -= al * qv;
+        acc[0] += rr * rr;
+      }
+    }
+  }
+  block_sum_store<2>(acc, partial + 2 * (long long)blockIdx.x);
+}
+
+namespace imau {
+void gcr_dots(const Launch &L, const double *qu, const double *qv, const double *ru, const double *rv, const GList &Q,
+              int nj, const int *active, double *partial) {
+  hipLaunchKernelGGL(k_gcr_dots, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, CVec2{qu, qv}, CVec2{ru, rv}, Q, nj,
+                     active, partial);
+}
+void gcr_scalar(hipStream_t s, int ngrids, const int *tstart, const double *partial, double *sc, const int *active,
+                int nj) {
+  hipLaunchKernelGGL(k_gcr_scalar, dim3(ngrids), dim3(256), 0, s, tstart, partial, sc, active, nj);
+}
+void gcr_update(const Launch &L, const GList &Q, const GList &Z, int nj, double *xu, double *xv, double *ru, double *rv,
+                const double *sc, const int *active, double *partial) {
+  hipLaunchKernelGGL(k_gcr_update, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, Q, Z, nj, Vec2{xu, xv},
+                     Vec2{ru, rv}, sc, active, partial);
+}
+}  // namespace imau

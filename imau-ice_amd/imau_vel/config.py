@@ -89,6 +89,7 @@ def default_config(**overrides) -> ImauCfg:
     c.check_interval = 0
     c.picard_relax_max = 0.0
     c.mg_smoother = 0
+    c.ksp_type = 0
     set_config(c, **overrides)
     return c
 
@@ -99,6 +100,7 @@ _STRING_MAPS = {
     "choice_basal_roughness": ROUGHNESS, "choice_param_basal_roughness": PARAM_ROUGHNESS,
     "DIVA_choice_matrix_solver": SOLVER,
     "precision": {"fp64": abi.IMAU_PREC_FP64, "mixed": abi.IMAU_PREC_MIXED},
+    "ksp_type": {"bicgstab": 0, "gcr": 1},
 }
 
 

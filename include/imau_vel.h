@@ -143,6 +143,9 @@ typedef struct imau_cfg {
   int32_t mg_smoother;     /* multigrid fine-level smoother: 0 = one fused launch
                               per sweep (tiles with a recomputed halo in LDS),
                               1 = one launch per colour; identical results     */
+  int32_t ksp_type;        /* KSP-mode Krylov method: 0 = BiCGStab, 1 = GCR
+                              restarted every 4 directions (one preconditioner
+                              application per iteration)                       */
 } imau_cfg;
 
 /* The fields of type_ice_model (src/data_types_module.f90:56-325) that the
