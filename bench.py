@@ -59,14 +59,16 @@ def VISC_BYTES_PER_CELL(nz):
 VISC_TRAFFIC_BYTES_PER_CELL = None
 SWEEP_TRAFFIC_BYTES_PER_CELL = {}
 METHODS = {
-    "mixed": dict(precision="mixed", mg_levels=0, picard_relax_max=1.0, ksp_type=1,
+    "mixed": dict(precision="mixed", mg_levels=0, picard_relax_max=0.0, ksp_type=1, anderson_m=3,
                   text="GCR in FP64 (operator, residuals, update) right-preconditioned by an FP32 aggregation-"
-                       "multigrid V-cycle; FP64 viscosity/beta; Picard loop with adaptive relaxation "
-                       "(0.4 raised x1.25 per residual decrease up to 1.0, reset on an increase)"),
-    "fp64": dict(precision="fp64", mg_levels=0, picard_relax_max=0.0, ksp_type=0,
+                       "multigrid V-cycle; FP64 viscosity/beta; Anderson-accelerated Picard loop (3 differences)"),
+    "adaptive": dict(precision="mixed", mg_levels=0, picard_relax_max=1.0, ksp_type=1, anderson_m=0,
+                     text="GCR in FP64 right-preconditioned by an FP32 aggregation-multigrid V-cycle; Picard loop "
+                          "with adaptive relaxation (0.4 raised x1.25 per residual decrease up to 1.0)"),
+    "fp64": dict(precision="fp64", mg_levels=0, picard_relax_max=0.0, ksp_type=0, anderson_m=0,
                  text="FP64 BiCGStab preconditioned by an FP64 aggregation-multigrid V-cycle; reference Picard "
                       "loop (fixed relaxation 0.4)"),
-    "plain": dict(precision="fp64", mg_levels=1, picard_relax_max=0.0, ksp_type=0,
+    "plain": dict(precision="fp64", mg_levels=1, picard_relax_max=0.0, ksp_type=0, anderson_m=0,
                   text="FP64 BiCGStab preconditioned by tile Gauss-Seidel (no multigrid); reference Picard loop "
                        "(fixed relaxation 0.4)"),
 }
@@ -75,6 +77,7 @@ METHODS = {
 def use_method(cfg, meth, relax_max=None):
     from imau_vel import config
     config.set_config(cfg, precision=meth["precision"], mg_levels=meth["mg_levels"], ksp_type=meth["ksp_type"],
+                      anderson_m=meth["anderson_m"],
                       picard_relax_max=meth["picard_relax_max"] if relax_max is None else relax_max)
 
 

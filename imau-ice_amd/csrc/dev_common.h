@@ -23,12 +23,28 @@
 #define IMAU_FTJ 64
 #define IMAU_FH 5
 
+// Anderson acceleration of the Picard loop: up to IMAU_AND_SLOTS stored
+// iterates (IMAU_AND_SLOTS - 1 differences)
+#define IMAU_AND_SLOTS 4
+
 // restart length of the GCR Krylov solver
 #define IMAU_GCR_M 4
 // pointers to the stored GCR directions (u and v components)
 struct GList {
   const double *u[IMAU_GCR_M];
   const double *v[IMAU_GCR_M];
+};
+struct SList {
+  const double *u[IMAU_AND_SLOTS];
+  const double *v[IMAU_AND_SLOTS];
+};
+// Anderson state for k_and_prep / k_post: slots U_s, F_s; ns valid slots;
+// w = per-member weights a_s then b_s (2 * IMAU_AND_SLOTS each) of
+// u_new = sum_s a_s U_s + b_s F_s; null = the configured relaxation
+struct AMix {
+  SList U, F;
+  int ns;
+  const double *w;
 };
 #define IMAU_NZMAX 32
 

@@ -105,6 +105,8 @@ struct imau_problem {
   DBuf<double> sc;    // per-member Krylov scalars
   DBuf<double> relax_d;  // per-member Picard relaxation (adaptive mode)
   DBuf<double> gcr_dirs; // GCR directions Q_i, Z_i (allocated on first use)
+  DBuf<double> and_slots;  // Anderson slots U_s, F_s (allocated on first use)
+  DBuf<double> and_w;      // Anderson weights per member
   std::vector<std::unique_ptr<MGLevel>> mg;  // coarse levels 1.. of the multigrid preconditioner
   DBuf<int> mg_depth;                        // per member: its coarsest level
   DBuf<float> mg_f32;                        // FP32 fine-level coefficients, rhs and iterates
@@ -313,6 +315,12 @@ void check_cfg(const imau_cfg *cfg) {
     fail(IMAU_ERR_CONFIG, "picard_relax_max must lie in [0, 1], got %g", cfg->picard_relax_max);
   if (cfg->picard_relax_max > 0.0 && cfg->stop_rule != IMAU_STOP_RESIDUAL)
     fail(IMAU_ERR_CONFIG, "adaptive Picard relaxation (picard_relax_max > 0) needs stop_rule IMAU_STOP_RESIDUAL");
+  if (cfg->anderson_m < 0 || cfg->anderson_m > IMAU_AND_SLOTS - 1)
+    fail(IMAU_ERR_CONFIG, "anderson_m must lie in [0, %d], got %d", IMAU_AND_SLOTS - 1, cfg->anderson_m);
+  if (cfg->anderson_m > 0 && cfg->stop_rule != IMAU_STOP_RESIDUAL)
+    fail(IMAU_ERR_CONFIG, "Anderson acceleration (anderson_m > 0) needs stop_rule IMAU_STOP_RESIDUAL");
+  if (cfg->anderson_m > 0 && cfg->picard_relax_max > 0.0)
+    fail(IMAU_ERR_CONFIG, "anderson_m and picard_relax_max exclude each other");
   if (cfg->ksp_type != 0 && cfg->ksp_type != 1)
     fail(IMAU_ERR_CONFIG, "ksp_type must be 0 (BiCGStab) or 1 (GCR), got %d", cfg->ksp_type);
   if (cfg->mg_smoother != 0 && cfg->mg_smoother != 1)
@@ -553,6 +561,7 @@ void imau_vel_default_config(imau_cfg *c) {
   c->picard_relax_max = 0.0;
   c->mg_smoother = 0;
   c->ksp_type = 0;
+  c->anderson_m = 0;
 }
 
 int imau_vel_init(const imau_grid *grid, const imau_cfg *cfg, imau_ctx **out) {
@@ -1011,6 +1020,111 @@ struct Solver {
     }
   }
 
+  // ----- Anderson acceleration of the Picard map (host side) -----
+  long long and_count_ = 0;              // Anderson steps taken in this solve
+  std::vector<long long> and_since_;     // per member: step of its last restart
+  std::vector<double> and_gram_;         // per member: Gram matrix (F_s, F_t) of the slots
+  std::vector<double> and_w_h_;          // per member: a_s, b_s
+
+  void and_init(AMix &mix) {
+    const long long na = P->na;
+    if (!P->and_slots.p) P->and_slots.alloc((size_t)4 * IMAU_AND_SLOTS * na);
+    if (!P->and_w.p) P->and_w.alloc((size_t)2 * IMAU_AND_SLOTS * P->ngrids);
+    for (int q = 0; q < IMAU_AND_SLOTS; ++q) {
+      mix.U.u[q] = P->and_slots.p + (size_t)(4 * q + 0) * na;
+      mix.U.v[q] = P->and_slots.p + (size_t)(4 * q + 1) * na;
+      mix.F.u[q] = P->and_slots.p + (size_t)(4 * q + 2) * na;
+      mix.F.v[q] = P->and_slots.p + (size_t)(4 * q + 3) * na;
+    }
+    mix.ns = 0;
+    mix.w = nullptr;
+  }
+
+  // Store slot k = (U_k, F_k), update the Gram matrices, solve the per-member
+  // least-squares problem min |F_k - sum_i gamma_i dF_i| over the member's
+  // valid slots and set u_new = G_k - sum_i gamma_i dG_i (G_s = U_s + F_s).
+  void and_step(AMix &mix, int m) {
+    const int G = P->ngrids, S = m + 1;
+    const int slot = (int)(and_count_ % S);
+    mix.ns = (int)std::min<long long>(and_count_ + 1, S);
+    mix.w = nullptr;
+    imau::and_prep(L, kp, P->F, P->F.xu0, P->F.xv0, mix, slot, P->act_pic.p, P->partial.p);
+    imau::member_sum_and(c->s, G, P->tstart_d.p, P->partial.p, P->msum.p);
+    kcheck();
+    std::vector<double> dots((size_t)IMAU_AND_SLOTS * G);
+    HIPCHK(hipMemcpyAsync(dots.data(), P->msum.p, dots.size() * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    and_w_h_.assign((size_t)2 * IMAU_AND_SLOTS * G, 0.0);
+    const double rel = cfg->DIVA_visc_it_relax;
+    for (int g = 0; g < G; ++g) {
+      double *W = &and_w_h_[(size_t)2 * IMAU_AND_SLOTS * g];
+      double *Gm = &and_gram_[(size_t)IMAU_AND_SLOTS * IMAU_AND_SLOTS * g];
+      for (int q = 0; q < mix.ns; ++q) {
+        Gm[slot * IMAU_AND_SLOTS + q] = dots[(size_t)IMAU_AND_SLOTS * g + q];
+        Gm[q * IMAU_AND_SLOTS + slot] = dots[(size_t)IMAU_AND_SLOTS * g + q];
+      }
+      // the member's slots, oldest first
+      const int nv = (int)std::min<long long>(and_count_ - and_since_[g] + 1, S);
+      int sl[IMAU_AND_SLOTS];
+      for (int i = 0; i < nv; ++i) sl[i] = (int)((and_count_ - (nv - 1) + i) % S);
+      bool mixed = false;
+      if (nv >= 2) {
+        const int n = nv - 1;
+        double M[IMAU_AND_SLOTS][IMAU_AND_SLOTS + 1];
+        auto gr = [&](int a, int b) { return Gm[sl[a] * IMAU_AND_SLOTS + sl[b]]; };
+        double tr = 0.0;
+        for (int a = 0; a < n; ++a) {
+          for (int b = 0; b < n; ++b) M[a][b] = gr(a + 1, b + 1) - gr(a + 1, b) - gr(a, b + 1) + gr(a, b);
+          M[a][n] = gr(a + 1, n) - gr(a, n);
+          tr += M[a][a];
+        }
+        for (int a = 0; a < n; ++a) M[a][a] += 1e-10 * tr;
+        bool ok = tr > 0.0 && std::isfinite(tr);
+        for (int a = 0; a < n && ok; ++a) {  // Gaussian elimination with partial pivoting
+          int piv = a;
+          for (int b = a + 1; b < n; ++b)
+            if (std::fabs(M[b][a]) > std::fabs(M[piv][a])) piv = b;
+          if (!(std::fabs(M[piv][a]) > 1e-14 * tr)) { ok = false; break; }
+          for (int cidx = 0; cidx <= n; ++cidx) std::swap(M[a][cidx], M[piv][cidx]);
+          for (int b = a + 1; b < n; ++b) {
+            const double f = M[b][a] / M[a][a];
+            for (int cidx = a; cidx <= n; ++cidx) M[b][cidx] -= f * M[a][cidx];
+          }
+        }
+        double gam[IMAU_AND_SLOTS] = {0, 0, 0, 0};
+        if (ok) {
+          for (int a = n - 1; a >= 0; --a) {
+            double v = M[a][n];
+            for (int b = a + 1; b < n; ++b) v -= M[a][b] * gam[b];
+            gam[a] = v / M[a][a];
+          }
+          for (int a = 0; a < n; ++a) ok = ok && std::isfinite(gam[a]);
+        }
+        if (ok) {
+          double w[IMAU_AND_SLOTS] = {0, 0, 0, 0};
+          w[n] = 1.0;
+          for (int a = 0; a < n; ++a) {
+            w[a + 1] -= gam[a];
+            w[a] += gam[a];
+          }
+          for (int i = 0; i < nv; ++i) {
+            W[sl[i]] = w[i];
+            W[IMAU_AND_SLOTS + sl[i]] = w[i];
+          }
+          mixed = true;
+        }
+      }
+      if (!mixed) {  // relaxation step u_k + rel (G(u_k) - u_k)
+        W[slot] = 1.0;
+        W[IMAU_AND_SLOTS + slot] = rel;
+        and_since_[g] = and_count_;
+      }
+    }
+    HIPCHK(hipMemcpyAsync(P->and_w.p, and_w_h_.data(), and_w_h_.size() * sizeof(double), hipMemcpyHostToDevice, c->s));
+    mix.w = P->and_w.p;
+    ++and_count_;
+  }
+
   static void mg_mark(void *self, int which) {
     Solver *S = static_cast<Solver *>(self);
     if (which == 0) S->time_begin();
@@ -1037,6 +1151,13 @@ struct Solver {
     const bool adaptive = cfg->picard_relax_max > 0.0 && cfg->stop_rule == IMAU_STOP_RESIDUAL;
     relax_h_.assign(G, cfg->DIVA_visc_it_relax);
     std::vector<double> prev_r(G, -1.0);
+    // Anderson acceleration state
+    const int and_m = (cfg->stop_rule == IMAU_STOP_RESIDUAL) ? cfg->anderson_m : 0;
+    AMix mix{};
+    if (and_m > 0) and_init(mix);
+    and_count_ = 0;
+    and_since_.assign(G, 0);
+    and_gram_.assign((size_t)G * IMAU_AND_SLOTS * IMAU_AND_SLOTS, 0.0);
     for (int it = 1;; ++it) {
       bool any = false;
       for (int g = 0; g < G; ++g) any = any || act_pic[g];
@@ -1063,6 +1184,10 @@ struct Solver {
                                                : cfg->DIVA_visc_it_relax;
             prev_r[g] = r[g];
           }
+          if (and_m > 0) {  // restart the member's history after a 1.5x residual rise
+            if (prev_r[g] >= 0.0 && r[g] > 1.5 * prev_r[g]) and_since_[g] = and_count_;
+            prev_r[g] = r[g];
+          }
         }
         if (!left) break;
         push_active(P->act_pic.p, act_pic);
@@ -1070,7 +1195,9 @@ struct Solver {
       linear_solve();
       if (adaptive)  // relax_h_ is not modified before the next stream synchronisation
         HIPCHK(hipMemcpyAsync(P->relax_d.p, relax_h_.data(), G * sizeof(double), hipMemcpyHostToDevice, c->s));
-      imau::post(L, kp, P->F, P->F.xu0, P->F.xv0, P->act_pic.p, diva, P->partial.p, adaptive ? P->relax_d.p : nullptr);
+      if (and_m > 0) and_step(mix, and_m);
+      imau::post(L, kp, P->F, P->F.xu0, P->F.xv0, P->act_pic.p, diva, P->partial.p, adaptive ? P->relax_d.p : nullptr,
+                 mix);
       imau::member_sum3(c->s, G, P->tstart_d.p, P->partial.p, P->msum.p);
       kcheck();
       HIPCHK(hipMemcpyAsync(host4.data(), P->msum.p, 3 * G * sizeof(double), hipMemcpyDeviceToHost, c->s));

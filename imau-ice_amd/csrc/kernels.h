@@ -28,7 +28,11 @@ void coef(const Launch &L, const KP &kp, const Fields &f, bool diva, const int *
 void init_x(const Launch &L, const Fields &f, double *xu, double *xv);
 // relax: per-member relaxation factors, or nullptr for the configured DIVA_visc_it_relax
 void post(const Launch &L, const KP &kp, const Fields &f, const double *xu, const double *xv, const int *active,
-          bool diva, double *partial, const double *relax);
+          bool diva, double *partial, const double *relax, const AMix &mix);
+// Anderson: store the slot and accumulate (F_k, F_s); member sums of the IMAU_AND_SLOTS dots
+void and_prep(const Launch &L, const KP &kp, const Fields &f, const double *xu, const double *xv, const AMix &mix,
+              int slot, const int *active, double *partial);
+void member_sum_and(hipStream_t s, int ngrids, const int *tstart, const double *partial, double *out);
 void resid(const Launch &L, const KP &kp, const Coef &c, const double *xu, const double *xv, double *partial);
 void member_sum3(hipStream_t s, int ngrids, const int *tstart, const double *partial, double *out);
 void member_sum2(hipStream_t s, int ngrids, const int *tstart, const double *partial, double *out);

@@ -626,11 +626,19 @@ __global__ void __launch_bounds__(IMAU_WG) k_post(const Member *__restrict__ mem
                                                   const KP kp, const Fields f, const double *__restrict__ xu,
                                                   const double *__restrict__ xv, const int *__restrict__ active,
                                                   int diva, double *__restrict__ partial,
-                                                  const double *__restrict__ relax) {
+                                                  const double *__restrict__ relax, AMix mix) {
   TILE_SETUP
   double acc[3] = {0.0, 0.0, 0.0};
   const int nx = m.nx, ny = m.ny;
   const double rel = relax ? relax[t.g] : kp.relax, vmax = kp.vel_max;
+  // Anderson mixing: u_new = sum_s (a_s U_s + b_s F_s) over the stored slots
+  double wa[IMAU_AND_SLOTS], wb[IMAU_AND_SLOTS];
+  if (mix.w)
+#pragma unroll
+    for (int q = 0; q < IMAU_AND_SLOTS; ++q) {
+      wa[q] = mix.w[t.g * 2 * IMAU_AND_SLOTS + q];
+      wb[q] = mix.w[t.g * 2 * IMAU_AND_SLOTS + IMAU_AND_SLOTS + q];
+    }
   if (j < ny && active[t.g]) {
     FOR_TILE_ROWS {
       const int i = t.i0 + ii;
@@ -646,7 +654,13 @@ __global__ void __launch_bounds__(IMAU_WG) k_post(const Member *__restrict__ mem
         const double ua = sqrt(x * x + vcx * vcx);
         if (ua > vmax) x = x * vmax / ua;
         const double uo = f.u[p];
-        const double un = rel * x + (1.0 - rel) * uo;
+        double un = rel * x + (1.0 - rel) * uo;
+        if (mix.w) {
+          un = 0.0;
+#pragma unroll
+          for (int q = 0; q < IMAU_AND_SLOTS; ++q)
+            if (q < mix.ns) un += wa[q] * mix.U.u[q][p] + wb[q] * mix.F.u[q][p];
+        }
         f.u[p] = un;
         f.u_prev[p] = uo;
         if (fabs(un) > 1e-6) {
@@ -680,7 +694,13 @@ __global__ void __launch_bounds__(IMAU_WG) k_post(const Member *__restrict__ mem
         const double ua = sqrt(ucy * ucy + x * x);
         if (ua > vmax) x = x * vmax / ua;
         const double vo = f.v[p];
-        const double vn = rel * x + (1.0 - rel) * vo;
+        double vn = rel * x + (1.0 - rel) * vo;
+        if (mix.w) {
+          vn = 0.0;
+#pragma unroll
+          for (int q = 0; q < IMAU_AND_SLOTS; ++q)
+            if (q < mix.ns) vn += wa[q] * mix.U.v[q][p] + wb[q] * mix.F.v[q][p];
+        }
         f.v[p] = vn;
         f.v_prev[p] = vo;
         if (fabs(vn) > 1e-6) {
@@ -1106,9 +1126,9 @@ void init_x(const Launch &L, const Fields &f, double *xu, double *xv) {
   hipLaunchKernelGGL(k_init_x, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, f, xu, xv);
 }
 void post(const Launch &L, const KP &kp, const Fields &f, const double *xu, const double *xv, const int *active,
-          bool diva, double *partial, const double *relax) {
+          bool diva, double *partial, const double *relax, const AMix &mix) {
   hipLaunchKernelGGL(k_post, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, xu, xv, active, diva ? 1 : 0, partial,
-                     relax);
+                     relax, mix);
 }
 void resid(const Launch &L, const KP &kp, const Coef &c, const double *xu, const double *xv, double *partial) {
   hipLaunchKernelGGL(k_resid, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, to_p(c), xu, xv, partial);
@@ -1686,5 +1706,74 @@ void gcr_update(const Launch &L, const GList &Q, const GList &Z, int nj, double 
                 const double *sc, const int *active, double *partial) {
   hipLaunchKernelGGL(k_gcr_update, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, Q, Z, nj, Vec2{xu, xv},
                      Vec2{ru, rv}, sc, active, partial);
+}
+}  // namespace imau
+
+// ---------------------------------------------------------------------------
+// Anderson acceleration of the Picard map u -> G(u) (the limited solution of
+// the linearised system): store U_k = u and F_k = G(u) - u in slot k and
+// accumulate (F_k, F_s) for every stored slot s; the per-member mixing
+// weights are solved on the host and applied by k_post.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(IMAU_WG) k_and_prep(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                      const KP kp, const Fields f, const double *__restrict__ xu,
+                                                      const double *__restrict__ xv, AMix mix, int slot,
+                                                      const int *__restrict__ active, double *__restrict__ partial) {
+  TILE_SETUP
+  double acc[IMAU_AND_SLOTS];
+#pragma unroll
+  for (int q = 0; q < IMAU_AND_SLOTS; ++q) acc[q] = 0.0;
+  const int nx = m.nx, ny = m.ny;
+  const double vmax = kp.vel_max;
+  if (j < ny && active[t.g]) {
+    double *Uu = const_cast<double *>(mix.U.u[slot]), *Uv = const_cast<double *>(mix.U.v[slot]);
+    double *Fu = const_cast<double *>(mix.F.u[slot]), *Fv = const_cast<double *>(mix.F.v[slot]);
+    FOR_TILE_ROWS {
+      const int i = t.i0 + ii;
+      if (i >= nx) break;
+      const long long p = p2(m, j, i);
+      if (i < nx - 1) {  // the velocity limit of k_post (:1432-1498)
+        double x = xu[p];
+        double vcx;
+        if (j == 0) vcx = (xv[p2(m, 0, i)] + xv[p2(m, 0, i + 1)]) / 2.0;
+        else if (j == ny - 1) vcx = (xv[p2(m, ny - 2, i)] + xv[p2(m, ny - 2, i + 1)]) / 2.0;
+        else vcx = (xv[p2(m, j - 1, i)] + xv[p2(m, j - 1, i + 1)] + xv[p2(m, j, i)] + xv[p2(m, j, i + 1)]) / 4.0;
+        const double ua = sqrt(x * x + vcx * vcx);
+        if (ua > vmax) x = x * vmax / ua;
+        const double uo = f.u[p], fk = x - uo;
+        Uu[p] = uo;
+        Fu[p] = fk;
+#pragma unroll
+        for (int q = 0; q < IMAU_AND_SLOTS; ++q)
+          if (q < mix.ns) acc[q] += fk * (q == slot ? fk : mix.F.u[q][p]);
+      }
+      if (j < ny - 1) {
+        double x = xv[p];
+        double ucy;
+        if (i == 0) ucy = (xu[p2(m, j, 0)] + xu[p2(m, j + 1, 0)]) / 2.0;
+        else if (i == nx - 1) ucy = (xu[p2(m, j, nx - 2)] + xu[p2(m, j + 1, nx - 2)]) / 2.0;
+        else ucy = (xu[p2(m, j, i - 1)] + xu[p2(m, j, i)] + xu[p2(m, j + 1, i - 1)] + xu[p2(m, j + 1, i)]) / 4.0;
+        const double ua = sqrt(ucy * ucy + x * x);
+        if (ua > vmax) x = x * vmax / ua;
+        const double vo = f.v[p], fk = x - vo;
+        Uv[p] = vo;
+        Fv[p] = fk;
+#pragma unroll
+        for (int q = 0; q < IMAU_AND_SLOTS; ++q)
+          if (q < mix.ns) acc[q] += fk * (q == slot ? fk : mix.F.v[q][p]);
+      }
+    }
+  }
+  block_sum_store<IMAU_AND_SLOTS>(acc, partial + IMAU_AND_SLOTS * (long long)blockIdx.x);
+}
+
+namespace imau {
+void and_prep(const Launch &L, const KP &kp, const Fields &f, const double *xu, const double *xv, const AMix &mix,
+              int slot, const int *active, double *partial) {
+  hipLaunchKernelGGL(k_and_prep, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, xu, xv, mix, slot, active,
+                     partial);
+}
+void member_sum_and(hipStream_t s, int ngrids, const int *tstart, const double *partial, double *out) {
+  hipLaunchKernelGGL(k_member_sum<IMAU_AND_SLOTS>, dim3(ngrids), dim3(256), 0, s, tstart, partial, out);
 }
 }  // namespace imau

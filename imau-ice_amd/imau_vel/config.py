@@ -90,6 +90,7 @@ def default_config(**overrides) -> ImauCfg:
     c.picard_relax_max = 0.0
     c.mg_smoother = 0
     c.ksp_type = 0
+    c.anderson_m = 0
     set_config(c, **overrides)
     return c
 

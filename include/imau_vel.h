@@ -146,6 +146,10 @@ typedef struct imau_cfg {
   int32_t ksp_type;        /* KSP-mode Krylov method: 0 = BiCGStab, 1 = GCR
                               restarted every 4 directions (one preconditioner
                               application per iteration)                       */
+  int32_t anderson_m;      /* 0: relaxation (fixed or adaptive).  1..3: Anderson
+                              acceleration of the Picard map with m stored
+                              differences, restarted for a member whose FP64
+                              residual grows 1.5x (needs IMAU_STOP_RESIDUAL) */
 } imau_cfg;
 
 /* The fields of type_ice_model (src/data_types_module.f90:56-325) that the

@@ -301,3 +301,49 @@ def test_gcr_channel_and_batch(ctx):
     prob.close()
     for s1, sb in zip(singles, batch):
         np.testing.assert_array_equal(s1["u_vav_cx"], sb["u_vav_cx"])
+
+
+@pytest.mark.parametrize("case", ["random65", "channel_small"])
+def test_anderson_same_fixed_point(ctx, case):
+    """Anderson-accelerated Picard reaches the oracle's fixed point within 1e-6
+    in fewer iterations than the reference relaxation."""
+    if case == "random65":
+        g, cfg, st = geometry.random_geometry(65, 65, seed=7)
+    else:
+        g, cfg, st = geometry.channel(nx=61, ny=21, dx=8e3, seed=3)
+    tight(cfg, st, rtol=1e-10)
+    config.set_config(cfg, precision="mixed", ksp_type="gcr")
+    plain, acc, ref = st.copy(), st.copy(), st.copy()
+    ctx.solve_diva(g, cfg, plain)
+    config.set_config(cfg, anderson_m=3)
+    ctx.solve_diva(g, cfg, acc)
+    config.set_config(cfg, anderson_m=0)
+    O.solve("diva", g, cfg, ref, O.LIN_BANDED_LU)
+    print(case, "picard reference/anderson/oracle", plain.stats["picard_iterations"], acc.stats["picard_iterations"],
+          ref.stats["picard_iterations"])
+    assert acc.stats["nonlinear_residual"] <= 1e-10
+    assert acc.stats["picard_iterations"] < plain.stats["picard_iterations"]
+    for name in ("u_vav_cx", "v_vav_cy", "u_3D_cx", "taub_cx"):
+        e = relerr(acc[name], ref[name])
+        assert e < 1e-6, (name, e)
+
+
+def test_anderson_batch_equals_single(ctx):
+    from imau_vel import solver
+    cases = [geometry.dome(), geometry.random_geometry(45, 37, seed=4)]
+    cfg = cases[0][1]
+    config.set_config(cfg, stop_rule=1, nonlinear_rtol=1e-6, DIVA_visc_it_nit=300, ksp_type="gcr", precision="mixed",
+                      anderson_m=3)
+    singles = []
+    for g, _, st in cases:
+        s = st.copy()
+        ctx.solve_diva(g, cfg, s)
+        singles.append(s)
+    batch = [c[2].copy() for c in cases]
+    prob = solver.Problem(ctx, [c[0] for c in cases], cfg, batch)
+    prob.solve()
+    prob.download()
+    prob.close()
+    for s1, sb in zip(singles, batch):
+        assert s1.stats["picard_iterations"] == sb.stats["picard_iterations"]
+        np.testing.assert_array_equal(s1["u_vav_cx"], sb["u_vav_cx"])
