@@ -16,8 +16,7 @@ The line also carries:
                 HIP events on the library stream, over the members each
                 launch processed;
   roofline_smoother
-                the same for one fine-level smoother sweep (5 colour launches
-                of k_smooth0);
+                the same for one fine-level smoother sweep (k_smooth_fused);
   cpu_baseline  the CPU oracle (FP64, banded-LU linear solves, 1 thread) on a
                 bounded sample: the smallest ensemble member solved alone;
   fp64_multigrid, plain_fp64
@@ -37,7 +36,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "imau-ice_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# one fine-level smoother sweep (5 colour launches of k_smooth0): read u, v,
+# one fine-level smoother sweep (one k_smooth_fused launch): read u, v,
 # N_a, N_b, beta_eff_cx, beta_eff_cy and the two right-hand sides, write u, v
 # (10 words per cell, 4 bytes each in the mixed mode, 8 in FP64) plus the two
 # row-type bytes
@@ -54,10 +53,12 @@ def VISC_BYTES_PER_CELL(nz):
 
 
 # HBM bytes per processed cell measured with rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE on the ensemble workload (FETCH_SIZE scaled by 2.0, WRITE_SIZE by
-# 1.0: the factors a 1 GiB device copy reads; profiles/k_visc_ensemble_traffic.md)
-VISC_TRAFFIC_BYTES_PER_CELL = None
-SWEEP_TRAFFIC_BYTES_PER_CELL = {}
+# WRITE_SIZE on the ensemble workload with the default (mixed) method
+# (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0: the factors a 1 GiB device
+# copy reads; profiles/ensemble_mixed_kernel_stats.md).  Re-measure after
+# changing either kernel.
+VISC_TRAFFIC_BYTES_PER_CELL = 659.3
+SWEEP_TRAFFIC_BYTES_PER_CELL = {"mixed": 146.3}
 METHODS = {
     "mixed": dict(precision="mixed", mg_levels=0, picard_relax_max=0.0, ksp_type=1, anderson_m=3,
                   text="GCR in FP64 (operator, residuals, update) right-preconditioned by an FP32 aggregation-"
@@ -208,11 +209,12 @@ def main():
                     traffic=traffic, kernel=kernel, avg_launch_us=avg_us, launches=n, cells_processed=cells_t,
                     bytes_per_launch=per_launch, bytes_per_cell=bytes_per_cell)
 
-    visc_traffic = VISC_TRAFFIC_BYTES_PER_CELL if args.workload == "ensemble" else None
-    sweep_traffic = SWEEP_TRAFFIC_BYTES_PER_CELL.get(meth["precision"]) if args.workload == "ensemble" else None
+    measured = args.workload == "ensemble" and args.method == "mixed" and args.ksp is None and args.relax_max is None
+    visc_traffic = VISC_TRAFFIC_BYTES_PER_CELL if measured else None
+    sweep_traffic = SWEEP_TRAFFIC_BYTES_PER_CELL.get(meth["precision"]) if measured else None
     roof_visc = roof("visc", "k_visc (3D viscosity, N, F2: one launch per Picard iteration)",
                      VISC_BYTES_PER_CELL(nz), visc_traffic)
-    roof_sweep = roof("smoother", "k_smooth0 x5 (one fine-level smoother sweep)" if meth["mg_levels"] == 0
+    roof_sweep = roof("smoother", "k_smooth_fused (one fine-level smoother sweep)" if meth["mg_levels"] == 0
                       else "k_prec_tile (one tile Gauss-Seidel application)",
                       SWEEP_WORDS_PER_CELL * word + SWEEP_BYTES_EXTRA, sweep_traffic if meth["mg_levels"] == 0 else None)
 

@@ -1217,6 +1217,7 @@ struct Solver {
   }
 
   void finish(int dyn) {
+    time_collect();  // event pairs of the last Picard iteration (it ends before a linear solve)
     if (dyn == IMAU_DYN_DIVA) {
       imau::F1(L, kp, P->F, P->visc_cur, P->O);
       imau::uv3d(L, kp, P->F, P->O);
