@@ -962,8 +962,7 @@ struct Solver {
       Z.u[k] = Zd[0][k]; Z.v[k] = Zd[1][k];
     }
     double *X[2] = {P->F.xu0, P->F.xv0};
-    double *R[2] = {f2p(P, F_ru), f2p(P, F_rv)}, *RH[2] = {f2p(P, F_rhu), f2p(P, F_rhv)};
-    double *Pp[2] = {f2p(P, F_pu), f2p(P, F_pv)}, *Vv[2] = {f2p(P, F_vu), f2p(P, F_vv)};
+    double *R[2] = {f2p(P, F_ru), f2p(P, F_rv)};
     const imau::Coef cf = coef();
     const bool use_mg = cfg->mg_levels != 1 && !P->mg.empty();
     const bool fp32 = cfg->precision == IMAU_PREC_MIXED;
@@ -972,9 +971,9 @@ struct Solver {
       HIPCHK(hipMemcpyAsync(out.data(), P->msum.p, 2 * G * sizeof(double), hipMemcpyDeviceToHost, c->s));
       HIPCHK(hipStreamSynchronize(c->s));
     };
-    // r = b - A x and the norms of r and b (the BiCGStab start kernel)
-    imau::bicg_init(L, kp, cf, X[0], X[1], R[0], R[1], RH[0], RH[1], Pp[0], Pp[1], Vv[0], Vv[1], P->act_lin.p,
-                    P->partial.p);
+    // r = b - A x and the norms of r and b (the BiCGStab start kernel, r only)
+    imau::bicg_init(L, kp, cf, X[0], X[1], R[0], R[1], nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                    P->act_lin.p, P->partial.p);
     imau::member_sum2(c->s, G, P->tstart_d.p, P->partial.p, P->msum.p);
     kcheck();
     fetch();

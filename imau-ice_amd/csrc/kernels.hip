@@ -1241,13 +1241,15 @@ __global__ void __launch_bounds__(IMAU_WG) k_bicg_init(const Member *__restrict_
       if (i < m.nx - 1) {
         u_row(m, kp, c, j, i, c.rt_u[p], U, V, S, D, b);
         const double rv = b - (S + D * x.u[p]);
-        r.u[p] = rv; rh.u[p] = rv; pp.u[p] = 0.0; vv.u[p] = 0.0;
+        r.u[p] = rv;
+        if (rh.u) { rh.u[p] = rv; pp.u[p] = 0.0; vv.u[p] = 0.0; }
         acc[0] += rv * rv; acc[1] += b * b;
       }
       if (j < m.ny - 1) {
         v_row(m, kp, c, j, i, c.rt_v[p], U, V, S, D, b);
         const double rv = b - (S + D * x.v[p]);
-        r.v[p] = rv; rh.v[p] = rv; pp.v[p] = 0.0; vv.v[p] = 0.0;
+        r.v[p] = rv;
+        if (rh.u) { rh.v[p] = rv; pp.v[p] = 0.0; vv.v[p] = 0.0; }
         acc[0] += rv * rv; acc[1] += b * b;
       }
     }
