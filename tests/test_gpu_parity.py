@@ -347,3 +347,47 @@ def test_anderson_batch_equals_single(ctx):
     for s1, sb in zip(singles, batch):
         assert s1.stats["picard_iterations"] == sb.stats["picard_iterations"]
         np.testing.assert_array_equal(s1["u_vav_cx"], sb["u_vav_cx"])
+
+
+def test_channel_full_size_matches_oracle(ctx):
+    """The 401 x 41 marine channel at 2 km (MISMIP+-style 'zero' boundaries,
+    floating ice): the accelerated mixed-precision path and the oracle's exact-LU
+    fixed point agree within 1e-6."""
+    g, cfg, st = geometry.channel()
+    tight(cfg, st, rtol=1e-10)
+    config.set_config(cfg, DIVA_choice_matrix_solver="PETSc", precision="mixed", ksp_type="gcr", anderson_m=3)
+    st.DIVA_PETSc_rtol = 0.1
+    st.DIVA_PETSc_abstol = 0.0
+    gpu, ref = st.copy(), st.copy()
+    ctx.solve_diva(g, cfg, gpu)
+    config.set_config(cfg, anderson_m=0)
+    O.solve("diva", g, cfg, ref, O.LIN_BANDED_LU)
+    print("channel picard gpu/oracle", gpu.stats["picard_iterations"], ref.stats["picard_iterations"])
+    for name in ("u_vav_cx", "v_vav_cy", "u_3D_cx", "w_3D_a", "taub_cx"):
+        e = relerr(gpu[name], ref[name])
+        assert e < 1e-6, (name, e)
+
+
+def test_large_grid_mixed_equals_fp64(ctx):
+    """A 426 x 726 random ice cap at 4 km (the mixed-precision configuration's
+    size, beyond the oracle's reach): the mixed-precision accelerated solve and
+    the plain FP64 solve reach the same fixed point, and an independent FP64
+    residual evaluation of the result confirms the stop criterion."""
+    g, cfg, st = geometry.random_geometry(426, 726, seed=3)
+    config.set_config(cfg, stop_rule=1, nonlinear_rtol=1e-8, DIVA_visc_it_nit=2000, DIVA_choice_matrix_solver="PETSc")
+    st.DIVA_PETSc_rtol = 0.1
+    st.DIVA_PETSc_abstol = 0.0
+    st.DIVA_SOR_nit = 20000
+    fast, plain = st.copy(), st.copy()
+    config.set_config(cfg, precision="mixed", ksp_type="gcr", anderson_m=3)
+    ctx.solve_diva(g, cfg, fast)
+    config.set_config(cfg, precision="fp64", ksp_type="bicgstab", anderson_m=0)
+    ctx.solve_diva(g, cfg, plain)
+    print("426x726 picard fast/plain", fast.stats["picard_iterations"], plain.stats["picard_iterations"],
+          "residuals", fast.stats["nonlinear_residual"], plain.stats["nonlinear_residual"])
+    assert fast.stats["nonlinear_residual"] <= 1e-8 and plain.stats["nonlinear_residual"] <= 1e-8
+    r = ctx.nonlinear_residual(g, cfg, fast)
+    assert r <= 2e-8, r
+    for name in ("u_vav_cx", "v_vav_cy", "taub_cx"):
+        e = relerr(fast[name], plain[name])
+        assert e < 1e-5, (name, e)
