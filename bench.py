@@ -44,12 +44,14 @@ SWEEP_WORDS_PER_CELL = 10
 SWEEP_BYTES_EXTRA = 2
 
 
-def VISC_BYTES_PER_CELL(nz):
-    """k_visc per a-grid cell: read the previous viscosity and the rate-factor
-    term B (nz words each), write the new viscosity (nz words); read u, v, H,
-    taub_cx, taub_cy (8 B each) and the ice mask (4 B); write visc_int, N_a,
-    F2 (8 B each)."""
-    return 24 * nz + 5 * 8 + 4 + 3 * 8
+def VISC_BYTES_PER_CELL(nz, word):
+    """k_strain + k_visc per a-grid cell: k_strain reads u, v and writes the
+    strain invariant (8 B each); k_visc reads the previous viscosity and the
+    rate-factor term B (nz words each), writes the new viscosity (nz words;
+    4 bytes each with FP32 viscosity, else 8), reads the strain invariant, H,
+    taub_cx, taub_cy (8 B each) and the ice mask (4 B) and writes visc_int,
+    N_a, F2 (8 B each)."""
+    return 3 * nz * word + 3 * 8 + 4 * 8 + 4 + 3 * 8
 
 
 # HBM bytes per processed cell measured with rocprofv3 --pmc FETCH_SIZE and
@@ -57,12 +59,15 @@ def VISC_BYTES_PER_CELL(nz):
 # (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0: the factors a 1 GiB device
 # copy reads; profiles/ensemble_mixed_kernel_stats.md).  Re-measure after
 # changing either kernel.
-VISC_TRAFFIC_BYTES_PER_CELL = 659.3
-SWEEP_TRAFFIC_BYTES_PER_CELL = {"mixed": 146.3}
+VISC_TRAFFIC_BYTES_PER_CELL = {}
+SWEEP_TRAFFIC_BYTES_PER_CELL = {"mixed_visc": 146.3}
 METHODS = {
-    "mixed": dict(precision="mixed", mg_levels=0, picard_relax_max=0.0, ksp_type=1, anderson_m=3,
+    "mixed": dict(precision="mixed_visc", mg_levels=0, picard_relax_max=0.0, ksp_type=1, anderson_m=3,
                   text="GCR in FP64 (operator, residuals, update) right-preconditioned by an FP32 aggregation-"
-                       "multigrid V-cycle; FP64 viscosity/beta; Anderson-accelerated Picard loop (3 differences)"),
+                       "multigrid V-cycle; FP32 3D viscosity; Anderson-accelerated Picard loop (3 differences)"),
+    "mixed64visc": dict(precision="mixed", mg_levels=0, picard_relax_max=0.0, ksp_type=1, anderson_m=3,
+                        text="GCR in FP64 right-preconditioned by an FP32 aggregation-multigrid V-cycle; FP64 "
+                             "viscosity/beta; Anderson-accelerated Picard loop (3 differences)"),
     "adaptive": dict(precision="mixed", mg_levels=0, picard_relax_max=1.0, ksp_type=1, anderson_m=0,
                      text="GCR in FP64 right-preconditioned by an FP32 aggregation-multigrid V-cycle; Picard loop "
                           "with adaptive relaxation (0.4 raised x1.25 per residual decrease up to 1.0)"),
@@ -196,7 +201,7 @@ def main():
             prob.solve(cfg)
         timed[what] = ctx.kernel_timing(False)
     cells = sum(g.nx * g.ny for g in grids)
-    word = 4 if meth["precision"] == "mixed" else 8
+    word = 4 if meth["precision"] in ("mixed", "mixed_visc") else 8
     nz = cfg.nz
 
     def roof(what, kernel, bytes_per_cell, traffic_per_cell):
@@ -210,10 +215,10 @@ def main():
                     bytes_per_launch=per_launch, bytes_per_cell=bytes_per_cell)
 
     measured = args.workload == "ensemble" and args.method == "mixed" and args.ksp is None and args.relax_max is None
-    visc_traffic = VISC_TRAFFIC_BYTES_PER_CELL if measured else None
+    visc_traffic = VISC_TRAFFIC_BYTES_PER_CELL.get(meth["precision"]) if measured else None
     sweep_traffic = SWEEP_TRAFFIC_BYTES_PER_CELL.get(meth["precision"]) if measured else None
-    roof_visc = roof("visc", "k_visc (3D viscosity, N, F2: one launch per Picard iteration)",
-                     VISC_BYTES_PER_CELL(nz), visc_traffic)
+    roof_visc = roof("visc", "k_strain + k_visc (strain rates, 3D viscosity, N, F2: once per Picard iteration)",
+                     VISC_BYTES_PER_CELL(nz, 4 if meth["precision"] == "mixed_visc" else 8), visc_traffic)
     roof_sweep = roof("smoother", "k_smooth_fused (one fine-level smoother sweep)" if meth["mg_levels"] == 0
                       else "k_prec_tile (one tile Gauss-Seidel application)",
                       SWEEP_WORDS_PER_CELL * word + SWEEP_BYTES_EXTRA, sweep_traffic if meth["mg_levels"] == 0 else None)
@@ -222,7 +227,8 @@ def main():
         metric="DIVA solves/s at 1e-4 rel residual: learned+FP32 vs plain FP64 GPU",
         value=nsolve * args.steps / dt, unit="solves/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
         ms_per_step=1e3 * dt / args.steps, higher_is_better=True, scaling="weak", vs_baseline=None,
-        dtype="f32 preconditioner / f64 residual" if meth["precision"] == "mixed" else "f64", data="synthetic",
+        dtype={"mixed": "f32 preconditioner / f64 residual", "mixed_visc": "f32 viscosity and preconditioner / f64 "
+               "residual and update"}.get(meth["precision"], "f64"), data="synthetic",
         config=dict(workload=desc, members=nsolve, cells=cells, method=meth["text"],
                     stop="FP64 nonlinear residual <= 1e-4 for every member", learned_warm_start=False),
         picard_iterations=dict(min=min(its), max=max(its), mean=float(np.mean(its))),

@@ -116,6 +116,7 @@ struct Fields {
   // linear solve
   double *xu0, *xv0, *xu1, *xv1;       // double-buffered iterate
   double *u_prev, *v_prev, *u_base, *v_base;
+  double *eps_h;                       // horizontal strain-rate invariant on the a-grid (k_strain -> k_visc)
 };
 
 // one member of a coarse multigrid level: u unknowns on (nyu, nxu), v on
@@ -170,8 +171,9 @@ __device__ __forceinline__ int stag_sel(double H0, double H1) {
   if (H0 == 0.0 && H1 > 0.0) return 1;
   return 2;
 }
-__device__ __forceinline__ double stag_pick(double v0, double v1, int sel) {
-  return sel == 0 ? v0 : (sel == 1 ? v1 : 0.5 * (v0 + v1));
+template <class T = double>
+__device__ __forceinline__ T stag_pick(T v0, T v1, int sel) {
+  return sel == 0 ? v0 : (sel == 1 ? v1 : (T)0.5 * (v0 + v1));
 }
 
 __device__ __forceinline__ int iclamp(int a, int lo, int hi) { return a < lo ? lo : (a > hi ? hi : a); }

@@ -107,6 +107,7 @@ struct imau_problem {
   DBuf<double> gcr_dirs; // GCR directions Q_i, Z_i (allocated on first use)
   DBuf<double> and_slots;  // Anderson slots U_s, F_s (allocated on first use)
   DBuf<double> and_w;      // Anderson weights per member
+  DBuf<float> visc32;      // mixed precision: two FP32 viscosity buffers and the FP32 rate-factor term
   std::vector<std::unique_ptr<MGLevel>> mg;  // coarse levels 1.. of the multigrid preconditioner
   DBuf<int> mg_depth;                        // per member: its coarsest level
   DBuf<float> mg_f32;                        // FP32 fine-level coefficients, rhs and iterates
@@ -234,6 +235,7 @@ enum F2 {
   F_uabs_vav, F_uabs_surf, F_uabs_base, F_R_shear, F_w_surf, F_w_base,
   F_u0, F_v0, F_tbx0, F_tby0,
   F_ru, F_rv, F_rhu, F_rhv, F_pu, F_pv, F_vu, F_vv, F_phu, F_phv, F_shu, F_shv, F_tu, F_tv,
+  F_eps_h,
   F2_COUNT
 };
 enum I2 { I_ice, I_sheet, I_ocean, I_land, I_shelf, I_combi, I_fin, I_fout, I2_COUNT };
@@ -321,6 +323,8 @@ void check_cfg(const imau_cfg *cfg) {
     fail(IMAU_ERR_CONFIG, "Anderson acceleration (anderson_m > 0) needs stop_rule IMAU_STOP_RESIDUAL");
   if (cfg->anderson_m > 0 && cfg->picard_relax_max > 0.0)
     fail(IMAU_ERR_CONFIG, "anderson_m and picard_relax_max exclude each other");
+  if (cfg->precision < IMAU_PREC_FP64 || cfg->precision > IMAU_PREC_MIXED_VISC)
+    fail(IMAU_ERR_CONFIG, "unknown precision %d", cfg->precision);
   if (cfg->ksp_type != 0 && cfg->ksp_type != 1)
     fail(IMAU_ERR_CONFIG, "ksp_type must be 0 (BiCGStab) or 1 (GCR), got %d", cfg->ksp_type);
   if (cfg->mg_smoother != 0 && cfg->mg_smoother != 1)
@@ -396,6 +400,7 @@ void wire(imau_problem *P) {
   F.u = f2p(P, F_u); F.v = f2p(P, F_v); F.taub_cx = f2p(P, F_taub_cx); F.taub_cy = f2p(P, F_taub_cy);
   F.visc_a = f3p(P, G_visc_a); F.visc_b = f3p(P, G_visc_b);
   F.N_a = f2p(P, F_N_a); F.N_b = f2p(P, F_N_b); F.N_cx = f2p(P, F_N_cx); F.N_cy = f2p(P, F_N_cy);
+  F.eps_h = f2p(P, F_eps_h);
   F.visc_int = f2p(P, F_visc_int); F.F2 = f2p(P, F_F2); F.beta_a = f2p(P, F_beta_a); F.beta_eff_a = f2p(P, F_beta_eff_a);
   F.beta_eff_cx = f2p(P, F_beta_eff_cx); F.beta_eff_cy = f2p(P, F_beta_eff_cy);
   F.taudx = f2p(P, F_taudx); F.taudy = f2p(P, F_taudy);
@@ -890,7 +895,7 @@ struct Solver {
     }
     if (!any) return;
     push_active(P->act_lin.p, act_lin);
-    const bool fp32 = cfg->precision == IMAU_PREC_MIXED;
+    const bool fp32 = cfg->precision == IMAU_PREC_MIXED || cfg->precision == IMAU_PREC_MIXED_VISC;
     P->plan.fused = cfg->mg_smoother == 0;
     P->plan.periodic = false;
     for (int q = 0; q < 8; ++q) P->plan.periodic = P->plan.periodic || (&cfg->DIVA_boundary_BC_u_west)[q] == IMAU_BC_PERIODIC;
@@ -965,7 +970,7 @@ struct Solver {
     double *R[2] = {f2p(P, F_ru), f2p(P, F_rv)};
     const imau::Coef cf = coef();
     const bool use_mg = cfg->mg_levels != 1 && !P->mg.empty();
-    const bool fp32 = cfg->precision == IMAU_PREC_MIXED;
+    const bool fp32 = cfg->precision == IMAU_PREC_MIXED || cfg->precision == IMAU_PREC_MIXED_VISC;
     std::vector<double> out(2 * (size_t)G), target(G, 0.0);
     auto fetch = [&]() {
       HIPCHK(hipMemcpyAsync(out.data(), P->msum.p, 2 * G * sizeof(double), hipMemcpyDeviceToHost, c->s));
@@ -1157,16 +1162,29 @@ struct Solver {
     and_count_ = 0;
     and_since_.assign(G, 0);
     and_gram_.assign((size_t)G * IMAU_AND_SLOTS * IMAU_AND_SLOTS, 0.0);
+    // mixed precision: the 3D viscosity and the rate-factor term in FP32
+    const bool v32 = cfg->precision == IMAU_PREC_MIXED_VISC;
+    const long long n3 = P->na * (long long)P->nz;
+    float *cur32 = nullptr, *oth32 = nullptr;
+    if (v32) {
+      if (!P->visc32.p) P->visc32.alloc((size_t)3 * n3);
+      cur32 = P->visc32.p;
+      oth32 = P->visc32.p + n3;
+      imau::to_f32(c->s, P->F.B3, P->visc32.p + 2 * n3, n3);
+      imau::to_f32(c->s, P->visc_cur, cur32, n3);
+    }
     for (int it = 1;; ++it) {
       bool any = false;
       for (int g = 0; g < G; ++g) any = any || act_pic[g];
       if (!any) break;
       push_active(P->act_pic.p, act_pic);
       time_begin(IMAU_TIME_VISC);
-      imau::visc3d(L, kp, P->F, diva, P->visc_cur, P->visc_oth, P->act_pic.p);
+      if (v32) imau::visc3d32(L, kp, P->F, diva, cur32, oth32, P->visc32.p + 2 * n3, P->act_pic.p);
+      else imau::visc3d(L, kp, P->F, diva, P->visc_cur, P->visc_oth, P->act_pic.p);
       time_end(IMAU_TIME_VISC);
       imau::beta(L, kp, P->F, diva, P->xg.p, P->yg.p, P->act_pic.p);
-      std::swap(P->visc_cur, P->visc_oth);
+      if (v32) std::swap(cur32, oth32);
+      else std::swap(P->visc_cur, P->visc_oth);
       imau::coef(L, kp, P->F, diva, P->act_pic.p);
       kcheck();
       if (cfg->stop_rule == IMAU_STOP_RESIDUAL) {
@@ -1213,6 +1231,7 @@ struct Solver {
           act_pic[g] = 0;
       }
     }
+    if (v32) imau::to_f64(c->s, cur32, P->visc_cur, n3);  // the solve's viscosity output in FP64
   }
 
   void finish(int dyn) {

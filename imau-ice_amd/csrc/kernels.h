@@ -22,8 +22,14 @@ void setup2(const Launch &L, const KP &kp, const Fields &f);
 // 3D viscosity, N, F2 (k_visc) and then basal friction / beta_eff (k_beta)
 void visc3d(const Launch &L, const KP &kp, const Fields &f, bool diva, const double *visc_in, double *visc_out,
             const int *active);
+// the same in FP32 (viscosity storage and arithmetic; N, F2 written in FP64)
+void visc3d32(const Launch &L, const KP &kp, const Fields &f, bool diva, const float *visc_in, float *visc_out,
+              const float *B3, const int *active);
 void beta(const Launch &L, const KP &kp, const Fields &f, bool diva, const double *xg, const double *yg,
           const int *active);
+// element-wise precision conversion of n values
+void to_f32(hipStream_t s, const double *a, float *b, long long n);
+void to_f64(hipStream_t s, const float *a, double *b, long long n);
 void coef(const Launch &L, const KP &kp, const Fields &f, bool diva, const int *active);
 void init_x(const Launch &L, const Fields &f, double *xu, double *xv);
 // relax: per-member relaxation factors, or nullptr for the configured DIVA_visc_it_relax

@@ -319,30 +319,22 @@ __global__ void __launch_bounds__(IMAU_WG) k_setup2(const Member *__restrict__ m
 // Reads visc (previous iteration), writes visc_out.
 // src/ice_velocity_module.f90:664-1086
 // ---------------------------------------------------------------------------
-template <bool DIVA, bool THIRD>
-__global__ void __launch_bounds__(IMAU_WG) k_visc(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
-                                                  const KP kp, const Fields f, const double *__restrict__ visc_in,
-                                                  double *__restrict__ visc_out, const int *__restrict__ active) {
+// Picard step 1a: the horizontal strain-rate invariant on the a-grid, from the
+// staggered velocities (A5, src/ice_velocity_module.f90:722-899, with the
+// staggered derivatives of src/derivatives_and_grids_module.f90:755-938).  A
+// separate kernel so the many boundary variants do not cost k_visc occupancy.
+__global__ void __launch_bounds__(IMAU_WG) k_strain(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                    const KP kp, const Fields f, const int *__restrict__ active) {
   TILE_SETUP
-  if (j >= m.ny) return;
-  const int nx = m.nx, ny = m.ny, nz = kp.nz;
-  if (!active[t.g]) {  // finished member: keep its viscosity in the buffer that becomes current
-    FOR_TILE_ROWS {
-      const int i = t.i0 + ii;
-      if (i >= nx) break;
-      for (int k = 0; k < nz; ++k) visc_out[p3(m, k, j, i)] = visc_in[p3(m, k, j, i)];
-    }
-    return;
-  }
+  if (j >= m.ny || !active[t.g]) return;
+  const int nx = m.nx, ny = m.ny;
   const double dx = m.dx;
   const double *__restrict__ u = f.u;
   const double *__restrict__ v = f.v;
-  FOR_TILE_ROWS_SERIAL {
+  FOR_TILE_ROWS {
     const int i = t.i0 + ii;
     if (i >= nx) break;
     const long long p = p2(m, j, i);
-    const double Hi = f.Hi[p];
-    const int mi = f.m_ice[p];
     // horizontal strain rates on the a-grid (ddx_cx_to_a, ddy_cy_to_a, ddy_cx_to_a,
     // ddx_cy_to_a; src/derivatives_and_grids_module.f90:755-938)
 #define U(jj, ii_) u[p2(m, (jj), (ii_))]
@@ -384,10 +376,41 @@ __global__ void __launch_bounds__(IMAU_WG) k_visc(const Member *__restrict__ mem
 #undef V
     const double eps_h = dudx * dudx + dvdy * dvdy + dudx * dvdy + 0.25 * (dudy + dvdx) * (dudy + dvdx) + kp.eps0;
 
+    f.eps_h[p] = eps_h;
+  }
+}
+
+template <bool DIVA, bool THIRD, class T>
+__global__ void __launch_bounds__(IMAU_WG) k_visc(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                  const KP kp, const Fields f, const T *__restrict__ visc_in,
+                                                  T *__restrict__ visc_out, const T *__restrict__ B3,
+                                                  const int *__restrict__ active) {
+  TILE_SETUP
+  if (j >= m.ny) return;
+  const int nx = m.nx, ny = m.ny, nz = kp.nz;
+  if (!active[t.g]) {  // finished member: keep its viscosity in the buffer that becomes current
+    FOR_TILE_ROWS {
+      const int i = t.i0 + ii;
+      if (i >= nx) break;
+      for (int k = 0; k < nz; ++k) visc_out[p3(m, k, j, i)] = visc_in[p3(m, k, j, i)];
+    }
+    return;
+  }
+  const double dx = m.dx;
+  const double *__restrict__ u = f.u;
+  const double *__restrict__ v = f.v;
+  FOR_TILE_ROWS_SERIAL {
+    const int i = t.i0 + ii;
+    if (i >= nx) break;
+    const long long p = p2(m, j, i);
+    const double Hi = f.Hi[p];
+    const int mi = f.m_ice[p];
+    const double eps_h = f.eps_h[p];  // k_strain
+
     const bool bc_skip = (kp.margin == 0 && mi == 0);
-    double vint = 0.0, F2int = 0.0;
+    T vint = (T)0.0, F2int = (T)0.0;
     if (bc_skip) {
-      for (int k = 0; k < nz; ++k) visc_out[p3(m, k, j, i)] = 0.0;
+      for (int k = 0; k < nz; ++k) visc_out[p3(m, k, j, i)] = (T)0.0;
     } else {
       // neighbour data for the vertical shear terms (:691-692, :704-705, :775-788)
       const int icl = (i == 0) ? 0 : (i == nx - 1 ? nx - 2 : i - 1);  // left cx index used
@@ -395,12 +418,12 @@ __global__ void __launch_bounds__(IMAU_WG) k_visc(const Member *__restrict__ mem
       const int jcl = (j == 0) ? 0 : (j == ny - 1 ? ny - 2 : j - 1);
       const int jcr = (j == 0) ? 0 : (j == ny - 1 ? ny - 2 : j);
       const bool use_avg_x = (i > 0 && i < nx - 1), use_avg_y = (j > 0 && j < ny - 1);
-      double tbl = 0, tbr = 0, tcl = 0, tcr = 0;
+      T tbl = 0, tbr = 0, tcl = 0, tcr = 0;
       double Hl0 = 0, Hl1 = 0, Hr0 = 0, Hr1 = 0, Hb0 = 0, Hb1 = 0, Ht0 = 0, Ht1 = 0;
       bool noice_l = true, noice_r = true, noice_b = true, noice_t = true;
       if (DIVA) {
-        tbl = f.taub_cx[p2(m, j, icl)]; tbr = f.taub_cx[p2(m, j, icr)];
-        tcl = f.taub_cy[p2(m, jcl, i)]; tcr = f.taub_cy[p2(m, jcr, i)];
+        tbl = (T)f.taub_cx[p2(m, j, icl)]; tbr = (T)f.taub_cx[p2(m, j, icr)];
+        tcl = (T)f.taub_cy[p2(m, jcl, i)]; tcr = (T)f.taub_cy[p2(m, jcr, i)];
         Hl0 = f.Hi[p2(m, j, icl)]; Hl1 = f.Hi[p2(m, j, icl + 1)];
         Hr0 = f.Hi[p2(m, j, icr)]; Hr1 = f.Hi[p2(m, j, icr + 1)];
         Hb0 = f.Hi[p2(m, jcl, i)]; Hb1 = f.Hi[p2(m, jcl + 1, i)];
@@ -415,48 +438,49 @@ __global__ void __launch_bounds__(IMAU_WG) k_visc(const Member *__restrict__ mem
       // calc_staggered_margin choices, fixed per column: 0 first, 1 second, 2 mean
       const int sl = stag_sel(Hl0, Hl1), sr = stag_sel(Hr0, Hr1), sb = stag_sel(Hb0, Hb1), st = stag_sel(Ht0, Ht1);
       const int sxa = stag_sel(Hxl, Hi), sya = stag_sel(Hyl, Hi);
-      const double expo = (1.0 - kp.n_flow) / (2.0 * kp.n_flow);
-      double eta_prev_k = 0.0, g_prev_k = 0.0;
+      const T expo = (T)((1.0 - kp.n_flow) / (2.0 * kp.n_flow));
+      const T HiT = (T)Hi, eps_hT = (T)eps_h, vmin = (T)kp.visc_min;
+      T eta_prev_k = (T)0.0, g_prev_k = (T)0.0;
 #pragma unroll 1
       for (int k = 0; k < nz; ++k) {
-        const double z = kp.zeta[k];
-        double du_dz_a = 0.0, dv_dz_a = 0.0;
+        const T z = (T)kp.zeta[k];
+        T du_dz_a = (T)0.0, dv_dz_a = (T)0.0;
         if (DIVA) {
           // du/dz on the two cx and cy neighbours (calc_vertical_shear_strain_rates);
           // the pairs (icl, icl+1), (icr, icr+1), (jcl, jcl+1), (jcr, jcr+1) are
           // drawn from the centre and its four neighbours
-          const double vc = visc_in[p3(m, k, j, i)];
-          const double vw = (i > 0) ? visc_in[p3(m, k, j, i - 1)] : 0.0;
-          const double ve = (i < nx - 1) ? visc_in[p3(m, k, j, i + 1)] : 0.0;
-          const double vs = (j > 0) ? visc_in[p3(m, k, j - 1, i)] : 0.0;
-          const double vn = (j < ny - 1) ? visc_in[p3(m, k, j + 1, i)] : 0.0;
-          double dl = 0.0, dr = 0.0, db = 0.0, dt = 0.0;
-          if (!noice_l) dl = (tbl / stag_pick((i == 0) ? vc : vw, (i == 0) ? ve : vc, sl)) * z;
-          if (!noice_r) dr = (tbr / stag_pick((i == nx - 1) ? vw : vc, (i == nx - 1) ? vc : ve, sr)) * z;
-          if (!noice_b) db = (tcl / stag_pick((j == 0) ? vc : vs, (j == 0) ? vn : vc, sb)) * z;
-          if (!noice_t) dt = (tcr / stag_pick((j == ny - 1) ? vs : vc, (j == ny - 1) ? vc : vn, st)) * z;
-          du_dz_a = use_avg_x ? stag_pick(dl, dr, sxa) : dl;
-          dv_dz_a = use_avg_y ? stag_pick(db, dt, sya) : db;
+          const T vc = visc_in[p3(m, k, j, i)];
+          const T vw = (i > 0) ? visc_in[p3(m, k, j, i - 1)] : (T)0.0;
+          const T ve = (i < nx - 1) ? visc_in[p3(m, k, j, i + 1)] : (T)0.0;
+          const T vs = (j > 0) ? visc_in[p3(m, k, j - 1, i)] : (T)0.0;
+          const T vn = (j < ny - 1) ? visc_in[p3(m, k, j + 1, i)] : (T)0.0;
+          T dl = (T)0.0, dr = (T)0.0, db = (T)0.0, dt = (T)0.0;
+          if (!noice_l) dl = (tbl / stag_pick<T>((i == 0) ? vc : vw, (i == 0) ? ve : vc, sl)) * z;
+          if (!noice_r) dr = (tbr / stag_pick<T>((i == nx - 1) ? vw : vc, (i == nx - 1) ? vc : ve, sr)) * z;
+          if (!noice_b) db = (tcl / stag_pick<T>((j == 0) ? vc : vs, (j == 0) ? vn : vc, sb)) * z;
+          if (!noice_t) dt = (tcr / stag_pick<T>((j == ny - 1) ? vs : vc, (j == ny - 1) ? vc : vn, st)) * z;
+          du_dz_a = use_avg_x ? stag_pick<T>(dl, dr, sxa) : dl;
+          dv_dz_a = use_avg_y ? stag_pick<T>(db, dt, sya) : db;
         }
-        const double eps_sq = eps_h + 0.25 * (du_dz_a * du_dz_a + dv_dz_a * dv_dz_a);
-        const double B = f.B3[p3(m, k, j, i)];
-        double eta = THIRD ? B / cbrt(eps_sq) : B * pow(eps_sq, expo);
-        eta = fmax(eta, kp.visc_min);  // NaN (cold start 0/0) -> visc_eff_min
+        const T eps_sq = eps_hT + (T)0.25 * (du_dz_a * du_dz_a + dv_dz_a * dv_dz_a);
+        const T B = B3[p3(m, k, j, i)];
+        T eta = THIRD ? B / cbrt(eps_sq) : B * pow(eps_sq, expo);
+        eta = fmax(eta, vmin);  // NaN (cold start 0/0) -> visc_eff_min
         visc_out[p3(m, k, j, i)] = eta;
-        const double gk = (Hi / eta) * (z * z);
+        const T gk = (HiT / eta) * (z * z);
         if (k > 0) {
-          const double dz = z - kp.zeta[k - 1];
-          vint += 0.5 * (eta + eta_prev_k) * dz;
-          F2int += 0.5 * (gk + g_prev_k) * dz;
+          const T dz = z - (T)kp.zeta[k - 1];
+          vint += (T)0.5 * (eta + eta_prev_k) * dz;
+          F2int += (T)0.5 * (gk + g_prev_k) * dz;
         }
         eta_prev_k = eta;
         g_prev_k = gk;
       }
-      f.visc_int[p] = vint;
-      f.N_a[p] = vint * ((kp.margin == 0) ? Hi : fmax(0.1, Hi));
+      f.visc_int[p] = (double)vint;
+      f.N_a[p] = (double)vint * ((kp.margin == 0) ? Hi : fmax(0.1, Hi));
     }
     // F2 integral (calc_F_integral, :960-1013)
-    if (DIVA) f.F2[p] = (mi == 1) ? fmax(kp.F_int_min, F2int) : ((kp.margin == 0) ? 0.0 : kp.F_int_min);
+    if (DIVA) f.F2[p] = (mi == 1) ? fmax(kp.F_int_min, (double)F2int) : ((kp.margin == 0) ? 0.0 : kp.F_int_min);
   }
 }
 
@@ -1111,8 +1135,32 @@ void visc3d(const Launch &L, const KP &kp, const Fields &f, bool diva, const dou
             const int *active) {
   // n = 3 (every shipped configuration) takes the cube-root form of the viscosity
   const bool third = kp.n_flow == 3.0;
-  auto kv = diva ? (third ? k_visc<true, true> : k_visc<true, false>) : (third ? k_visc<false, true> : k_visc<false, false>);
-  hipLaunchKernelGGL(kv, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, visc_in, visc_out, active);
+  auto kv = diva ? (third ? k_visc<true, true, double> : k_visc<true, false, double>)
+                 : (third ? k_visc<false, true, double> : k_visc<false, false, double>);
+  hipLaunchKernelGGL(k_strain, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, active);
+  hipLaunchKernelGGL(kv, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, visc_in, visc_out, f.B3, active);
+}
+void visc3d32(const Launch &L, const KP &kp, const Fields &f, bool diva, const float *visc_in, float *visc_out,
+              const float *B3, const int *active) {
+  const bool third = kp.n_flow == 3.0;
+  auto kv = diva ? (third ? k_visc<true, true, float> : k_visc<true, false, float>)
+                 : (third ? k_visc<false, true, float> : k_visc<false, false, float>);
+  hipLaunchKernelGGL(k_strain, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, active);
+  hipLaunchKernelGGL(kv, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, visc_in, visc_out, B3, active);
+}
+__global__ void k_d2f(const double *__restrict__ a, float *__restrict__ b, long long n) {
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
+    b[q] = (float)a[q];
+}
+__global__ void k_f2d(const float *__restrict__ a, double *__restrict__ b, long long n) {
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
+    b[q] = (double)a[q];
+}
+void to_f32(hipStream_t s, const double *a, float *b, long long n) {
+  hipLaunchKernelGGL(k_d2f, dim3((int)std::min<long long>((n + 255) / 256, 8192)), dim3(256), 0, s, a, b, n);
+}
+void to_f64(hipStream_t s, const float *a, double *b, long long n) {
+  hipLaunchKernelGGL(k_f2d, dim3((int)std::min<long long>((n + 255) / 256, 8192)), dim3(256), 0, s, a, b, n);
 }
 void beta(const Launch &L, const KP &kp, const Fields &f, bool diva, const double *xg, const double *yg,
           const int *active) {

@@ -100,7 +100,8 @@ _STRING_MAPS = {
     "choice_idealised_sliding_law": IDEALISED, "choice_basal_hydrology": HYDROLOGY,
     "choice_basal_roughness": ROUGHNESS, "choice_param_basal_roughness": PARAM_ROUGHNESS,
     "DIVA_choice_matrix_solver": SOLVER,
-    "precision": {"fp64": abi.IMAU_PREC_FP64, "mixed": abi.IMAU_PREC_MIXED},
+    "precision": {"fp64": abi.IMAU_PREC_FP64, "mixed": abi.IMAU_PREC_MIXED,
+                  "mixed_visc": abi.IMAU_PREC_MIXED_VISC},
     "ksp_type": {"bicgstab": 0, "gcr": 1},
 }
 

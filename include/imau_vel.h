@@ -75,7 +75,10 @@ enum { IMAU_BC_INFINITE = 0, IMAU_BC_ZERO = 1, IMAU_BC_PERIODIC = 2 };
 enum { IMAU_SOLVER_SOR = 0, IMAU_SOLVER_PETSC = 1 };
 /* arithmetic of the linearised solve (no reference counterpart: the
  * reference is FP64 throughout) */
-enum { IMAU_PREC_FP64 = 0, IMAU_PREC_MIXED = 1 };
+/* precision (GPU-only): FP64 throughout; MIXED = FP32 multigrid preconditioner
+ * inside the FP64 Krylov loop; MIXED_VISC = MIXED plus FP32 viscosity storage
+ * and arithmetic (N, F2, the operator, residual and update stay FP64) */
+enum { IMAU_PREC_FP64 = 0, IMAU_PREC_MIXED = 1, IMAU_PREC_MIXED_VISC = 2 };
 /* Picard stop rule: reference rule (relative velocity change, ice_velocity_module.f90:346-351)
  * or the FP64 relative nonlinear residual ||A(u)u-b(u)||/||b(u)|| <= nonlinear_rtol */
 enum { IMAU_STOP_PICARD = 0, IMAU_STOP_RESIDUAL = 1 };

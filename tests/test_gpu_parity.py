@@ -391,3 +391,25 @@ def test_large_grid_mixed_equals_fp64(ctx):
     for name in ("u_vav_cx", "v_vav_cy", "taub_cx"):
         e = relerr(fast[name], plain[name])
         assert e < 1e-5, (name, e)
+
+
+@pytest.mark.parametrize("case", ["random65", "channel_small"])
+def test_fp32_viscosity_fixed_point(ctx, case):
+    """FP32 viscosity (IMAU_PREC_MIXED_VISC): the FP64 residual still reaches
+    1e-8 and the converged velocities agree with the oracle's FP64 fixed point
+    within 1e-5 relative (the stated tolerance of this form)."""
+    if case == "random65":
+        g, cfg, st = geometry.random_geometry(65, 65, seed=7)
+    else:
+        g, cfg, st = geometry.channel(nx=61, ny=21, dx=8e3, seed=3)
+    tight(cfg, st, rtol=1e-8, nit=600)
+    config.set_config(cfg, precision="mixed_visc", ksp_type="gcr", anderson_m=3)
+    gpu, ref = st.copy(), st.copy()
+    ctx.solve_diva(g, cfg, gpu)
+    config.set_config(cfg, precision="fp64", anderson_m=0, nonlinear_rtol=1e-11, DIVA_visc_it_nit=3000)
+    O.solve("diva", g, cfg, ref, O.LIN_BANDED_LU)
+    errs = {name: relerr(gpu[name], ref[name]) for name in ("u_vav_cx", "v_vav_cy", "u_3D_cx", "taub_cx", "visc_eff_3D_a")}
+    print(case, "picard", gpu.stats["picard_iterations"], "residual", gpu.stats["nonlinear_residual"], errs)
+    assert gpu.stats["nonlinear_residual"] <= 1e-8
+    for name, e in errs.items():
+        assert e < 1e-5, (name, e)
