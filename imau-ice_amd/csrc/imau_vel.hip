@@ -75,7 +75,7 @@ struct MGLevel {
   long long na = 0;
   DBuf<CMember> mem_d;
   DBuf<Tile> tiles_d;
-  DBuf<double> data;  // [cu 9na][cv 9na][xu na][xv na][bu na][bv na], FP64 or FP32 elements
+  DBuf<double> data;  // [cu 9na][cv 9na][xu][xv][bu][bv][x2u][x2v] (na each), FP64 or FP32 elements
 };
 
 }  // namespace
@@ -187,7 +187,7 @@ static void build_mg(imau_problem *P) {
     lv->na = off;
     lv->mem_d.alloc(lv->mem.size());
     lv->tiles_d.alloc(lv->tiles.size());
-    lv->data.alloc((size_t)22 * off);
+    lv->data.alloc((size_t)24 * off);
     HIPCHK(hipMemcpy(lv->mem_d.p, lv->mem.data(), lv->mem.size() * sizeof(CMember), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(lv->tiles_d.p, lv->tiles.data(), lv->tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
     P->plan.levels.push_back(imau::MGLevelDev{lv->mem_d.p, lv->tiles_d.p, (int)lv->tiles.size(), lev, off,

@@ -13,7 +13,7 @@
 
 namespace imau {
 
-// one coarse level; data holds [cu 9na][cv 9na][xu na][xv na][bu na][bv na]
+// one coarse level; data holds [cu 9na][cv 9na][xu][xv][bu][bv][x2u][x2v] (na each)
 // elements of the working precision (sized for FP64, FP32 uses the first half)
 struct MGLevelDev {
   const CMember *mem;

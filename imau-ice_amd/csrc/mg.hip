@@ -272,6 +272,70 @@ __global__ void __launch_bounds__(IMAU_WG) k_cgs(CLevelT<T> L, int color, const 
   }
 }
 
+// One fused Gauss-Seidel sweep of a coarse level: the tile (IMAU_TI x IMAU_TJ)
+// plus a halo of 4 cells is staged in LDS from (xiu, xiv) (zero when xiu is
+// null), the four colour phases update shrinking regions (exact up to 3 - p
+// cells beyond the tile after phase p) and the tile is written to (xou, xov),
+// a different buffer, so the result equals four in-place colour launches.
+#define CH 4
+#define CRI (IMAU_TI + 2 * CH)
+#define CRJ (IMAU_TJ + 2 * CH)
+template <class T>
+__global__ void __launch_bounds__(IMAU_WG) k_cgs_fused(CLevelT<T> L, const T *__restrict__ xiu,
+                                                       const T *__restrict__ xiv, T *__restrict__ xou,
+                                                       T *__restrict__ xov, int reverse,
+                                                       const int *__restrict__ active) {
+  __shared__ T sx[2][CRI * CRJ];
+  const Tile t = L.tiles[blockIdx.x];
+  if (!active[t.g] || L.depth[t.g] <= L.lev) return;
+  const CMember cm = L.mem[t.g];
+  const int Ib = t.i0 - CH, Jb = t.j0 - CH;
+  const int tid = threadIdx.y * IMAU_TJ + threadIdx.x;
+  for (int q = tid; q < CRI * CRJ; q += IMAU_WG) {
+    const int li = q / CRJ, lj = q - li * CRJ;
+    const int I = Ib + li, J = Jb + lj;
+    T a = (T)0.0, b = (T)0.0;
+    if (xiu) {
+      if (cvalid(cm, 0, J, I)) a = xiu[cp2(cm, J, I)];
+      if (cvalid(cm, 1, J, I)) b = xiv[cp2(cm, J, I)];
+    }
+    sx[0][q] = a;
+    sx[1][q] = b;
+  }
+  __syncthreads();
+  for (int ph = 0; ph < 4; ++ph) {
+    const int color = reverse ? 3 - ph : ph;
+    const int comp = color >> 1;
+    const int r = CH - 1 - ph;
+    const int RW = IMAU_TJ + 2 * r, n = (IMAU_TI + 2 * r) * RW;
+    const T *cf = comp == 0 ? L.cu : L.cv;
+    for (int q = tid; q < n; q += IMAU_WG) {
+      const int li = CH - r + q / RW, lj = CH - r + q % RW;
+      const int I = Ib + li, J = Jb + lj;
+      if (((I + J) & 1) != (color & 1) || !cvalid(cm, comp, J, I)) continue;
+      const long long p = cp2(cm, J, I);
+      const T D = cf[p];
+      T S = (T)0.0;
+#pragma unroll
+      for (int k = 1; k < 9; ++k) {
+        int cc, JJ, II;
+        slot_pos(comp, k, J, I, cc, JJ, II);
+        if (!cvalid(cm, cc, JJ, II)) continue;
+        S += cf[k * L.na + p] * sx[cc][(II - Ib) * CRJ + (JJ - Jb)];
+      }
+      const T bb = (comp == 0 ? L.bu : L.bv)[p];
+      sx[comp][li * CRJ + lj] = (D != (T)0.0) ? (bb - S) / D : (T)0.0;
+    }
+    __syncthreads();
+  }
+  for (int q = tid; q < IMAU_TI * IMAU_TJ; q += IMAU_WG) {
+    const int li = CH + q / IMAU_TJ, lj = CH + q % IMAU_TJ;
+    const int I = Ib + li, J = Jb + lj;
+    if (cvalid(cm, 0, J, I)) xou[cp2(cm, J, I)] = sx[0][li * CRJ + lj];
+    if (cvalid(cm, 1, J, I)) xov[cp2(cm, J, I)] = sx[1][li * CRJ + lj];
+  }
+}
+
 // b(l+1) = R (b - A x)(l); one thread per coarse cell
 template <class T>
 __global__ void __launch_bounds__(IMAU_WG) k_crestrict(CLevelT<T> Fl, CLevelT<T> L, const int *__restrict__ active) {
@@ -688,6 +752,15 @@ CLevelT<T> view(const imau::MGPlan &P, int l) {
   return L;
 }
 
+// the level with its iterate pointers on the second buffer (after the fused post-smoothing)
+template <class T>
+CLevelT<T> view2(const imau::MGPlan &P, int l) {
+  CLevelT<T> L = view<T>(P, l);
+  L.xu = L.bv + L.na;
+  L.xv = L.xu + L.na;
+  return L;
+}
+
 template <class T>
 void fill(hipStream_t s, T *p, long long n) {
   int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
@@ -726,6 +799,29 @@ void coarse_t(hipStream_t s, const imau::MGPlan &P, const int *act) {
   }
 }
 
+// coarse part with fused sweeps: pre-smoothing from zero into x, the post-smoothed
+// (or coarsest-level) result of every level in the second buffer x2
+template <class T>
+void coarse_fused_t(hipStream_t s, const imau::MGPlan &P, const int *act) {
+  const int nl = (int)P.levels.size();
+  for (int l = 0; l < nl; ++l) {
+    const CLevelT<T> C = view<T>(P, l), C2 = view2<T>(P, l);
+    if (P.levels[l].ends)
+      hipLaunchKernelGGL(k_coarsest<T>, dim3(P.ngrids), dim3(1024), 0, s, C2, act, P.coarsest_sweeps);
+    if (l + 1 == nl) break;
+    hipLaunchKernelGGL(k_cgs_fused<T>, dim3(C.ntiles), wg(), 0, s, C, (const T *)nullptr, (const T *)nullptr, C.xu,
+                       C.xv, 0, act);
+    const CLevelT<T> N = view<T>(P, l + 1);
+    hipLaunchKernelGGL(k_crestrict<T>, dim3(N.ntiles), wg(), 0, s, C, N, act);
+  }
+  for (int l = nl - 2; l >= 0; --l) {
+    const CLevelT<T> C = view<T>(P, l), N2 = view2<T>(P, l + 1);
+    hipLaunchKernelGGL(k_cprolong<T>, dim3(C.ntiles), wg(), 0, s, C, N2, act);
+    hipLaunchKernelGGL(k_cgs_fused<T>, dim3(C.ntiles), wg(), 0, s, C, (const T *)C.xu, (const T *)C.xv,
+                       C.bv + C.na, C.bv + 2 * C.na, 1, act);
+  }
+}
+
 // one launch per colour; the cycle runs in place on z (zeroed by the caller)
 template <class T>
 void vcycle_t(const imau::Launch &F, const KP &kp, const CoefT<T> &cf, const imau::MGPlan &P, CVecT<T> y, VecT<T> z,
@@ -757,8 +853,9 @@ void vcycle_fused_t(const imau::Launch &F, const KP &kp, const CoefT<T> &cf, con
   if (mark) mark(mctx, 1);
   const CLevelT<T> L1 = view<T>(P, 0);
   hipLaunchKernelGGL(k_restrict0<T>, dim3(L1.ntiles), wg(), 0, s, F.mem, kp, cf, y, CVecT<T>{za.u, za.v}, L1, act);
-  coarse_t<T>(s, P, act);
-  hipLaunchKernelGGL(k_prolong0<T>, dim3(F.ntiles), wg(), 0, s, F.mem, F.tiles, cf.rt_u, cf.rt_v, za, L1, act);
+  coarse_fused_t<T>(s, P, act);
+  hipLaunchKernelGGL(k_prolong0<T>, dim3(F.ntiles), wg(), 0, s, F.mem, F.tiles, cf.rt_u, cf.rt_v, za, view2<T>(P, 0),
+                     act);
   hipLaunchKernelGGL(k_smooth_fused<T>, dim3(P.nftiles), wg(), 0, s, F.mem, P.ftiles, kp, cf, CVecT<T>{za.u, za.v},
                      zb, y, 1, act);
   if (P.periodic)

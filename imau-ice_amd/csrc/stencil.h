@@ -12,6 +12,13 @@ template <class T>
 struct CoefT {
   const T *Na, *Nb, *Ncx, *Ncy, *bcx, *bcy, *taudx, *taudy, *frhs;
   const uint8_t *rt_u, *rt_v;
+  // coefficient accessors used by the rows (an LDS-staged variant provides the same)
+  __device__ __forceinline__ T na(const Member &m, int j, int i) const { return Na[p2(m, j, i)]; }
+  __device__ __forceinline__ T nb(const Member &m, int j, int i) const { return Nb[p2(m, j, i)]; }
+  __device__ __forceinline__ T bx(const Member &m, int j, int i) const { return bcx[p2(m, j, i)]; }
+  __device__ __forceinline__ T by(const Member &m, int j, int i) const { return bcy[p2(m, j, i)]; }
+  __device__ __forceinline__ T ncx(const Member &m, int j, int i) const { return Ncx[p2(m, j, i)]; }
+  __device__ __forceinline__ T ncy(const Member &m, int j, int i) const { return Ncy[p2(m, j, i)]; }
 };
 
 template <class T>
@@ -39,24 +46,24 @@ __device__ __forceinline__ void bnd_src(uint8_t rt, int i, int j, int imax, int 
 // u row: free rows with cross terms (:2342-2467) or in the normalised 'sans'
 // form (:2468-2589); zero rows (:1787-1796, :2224-2230); domain boundary rows
 // (:1932-2191); calving-front stress rows (:2192-2266)
-template <class T, class AU, class AV>
-__device__ __forceinline__ void u_sd(const Member &m, const KP &kp, const CoefT<T> &c, int j, int i, uint8_t rt,
+template <class T, class AU, class AV, class C = CoefT<T>>
+__device__ __forceinline__ void u_sd(const Member &m, const KP &kp, const C &c, int j, int i, uint8_t rt,
                                      const AU &U, const AV &V, T &S, T &D) {
   const int nx = m.nx, ny = m.ny;
   if (rt == RT_FREE) {
     const T dx2 = (T)(m.dx * m.dx);
     if (kp.cross) {
-      const T Na0 = c.Na[p2(m, j, i)], Na1 = c.Na[p2(m, j, i + 1)];
-      const T Nb0 = c.Nb[p2(m, j, i)], Nbm = c.Nb[p2(m, j - 1, i)];
+      const T Na0 = c.na(m, j, i), Na1 = c.na(m, j, i + 1);
+      const T Nb0 = c.nb(m, j, i), Nbm = c.nb(m, j - 1, i);
       S = (T)4.0 * Na1 * U(j, i + 1) + (T)4.0 * Na0 * U(j, i - 1) + Nb0 * U(j + 1, i) + Nbm * U(j - 1, i) +
           ((T)-2.0 * Na0 - Nb0) * V(j, i) + ((T)2.0 * Na1 + Nb0) * V(j, i + 1) + ((T)2.0 * Na0 + Nbm) * V(j - 1, i) +
           ((T)-2.0 * Na1 - Nbm) * V(j - 1, i + 1);
-      D = (T)-4.0 * (Na1 + Na0) + (T)-1.0 * (Nb0 + Nbm) - c.bcx[p2(m, j, i)] * dx2;
+      D = (T)-4.0 * (Na1 + Na0) + (T)-1.0 * (Nb0 + Nbm) - c.bx(m, j, i) * dx2;
     } else {
-      const T Ncx = c.Ncx[p2(m, j, i)];
+      const T Ncx = c.ncx(m, j, i);
       S = (T)4.0 * U(j, i + 1) + (T)4.0 * U(j, i - 1) + U(j + 1, i) + U(j - 1, i) - (T)3.0 * V(j, i) +
           (T)3.0 * V(j, i + 1) + (T)3.0 * V(j - 1, i) - (T)3.0 * V(j - 1, i + 1);
-      D = (T)-10.0 - dx2 * c.bcx[p2(m, j, i)] / Ncx;
+      D = (T)-10.0 - dx2 * c.bx(m, j, i) / Ncx;
     }
     return;
   }
@@ -68,31 +75,31 @@ __device__ __forceinline__ void u_sd(const Member &m, const KP &kp, const CoefT<
     return;
   }
   const int i1 = (rt == RT_MARGIN_I) ? i : i + 1;
-  const T Na = c.Na[p2(m, j, i1)] / (T)m.dx;
+  const T Na = c.na(m, j, i1) / (T)m.dx;
   const T cl = (T)-4.0 * Na, cr = (T)4.0 * Na;
   const T sv = (T)-2.0 * Na * V(j - 1, i1) + (T)2.0 * Na * V(j, i1);
   if (i1 == i) { S = cl * U(j, i - 1) + sv; D = cr; }
   else { S = cr * U(j, i + 1) + sv; D = cl; }
 }
 
-template <class T, class AU, class AV>
-__device__ __forceinline__ void v_sd(const Member &m, const KP &kp, const CoefT<T> &c, int j, int i, uint8_t rt,
+template <class T, class AU, class AV, class C = CoefT<T>>
+__device__ __forceinline__ void v_sd(const Member &m, const KP &kp, const C &c, int j, int i, uint8_t rt,
                                      const AU &U, const AV &V, T &S, T &D) {
   const int nx = m.nx, ny = m.ny;
   if (rt == RT_FREE) {
     const T dx2 = (T)(m.dx * m.dx);
     if (kp.cross) {
-      const T Na0 = c.Na[p2(m, j, i)], Na1 = c.Na[p2(m, j + 1, i)];
-      const T Nb0 = c.Nb[p2(m, j, i)], Nbm = c.Nb[p2(m, j, i - 1)];
+      const T Na0 = c.na(m, j, i), Na1 = c.na(m, j + 1, i);
+      const T Nb0 = c.nb(m, j, i), Nbm = c.nb(m, j, i - 1);
       S = (T)4.0 * Na1 * V(j + 1, i) + (T)4.0 * Na0 * V(j - 1, i) + Nb0 * V(j, i + 1) + Nbm * V(j, i - 1) +
           ((T)-2.0 * Na1 - Nbm) * U(j + 1, i - 1) + ((T)2.0 * Na1 + Nb0) * U(j + 1, i) +
           ((T)2.0 * Na0 + Nbm) * U(j, i - 1) + ((T)-2.0 * Na0 - Nb0) * U(j, i);
-      D = (T)-4.0 * (Na1 + Na0) + (T)-1.0 * (Nb0 + Nbm) - c.bcy[p2(m, j, i)] * dx2;
+      D = (T)-4.0 * (Na1 + Na0) + (T)-1.0 * (Nb0 + Nbm) - c.by(m, j, i) * dx2;
     } else {
-      const T Ncy = c.Ncy[p2(m, j, i)];
+      const T Ncy = c.ncy(m, j, i);
       S = (T)4.0 * V(j + 1, i) + (T)4.0 * V(j - 1, i) + V(j, i + 1) + V(j, i - 1) - (T)3.0 * U(j + 1, i - 1) +
           (T)3.0 * U(j + 1, i) + (T)3.0 * U(j, i - 1) - (T)3.0 * U(j, i);
-      D = (T)-10.0 - dx2 * c.bcy[p2(m, j, i)] / Ncy;
+      D = (T)-10.0 - dx2 * c.by(m, j, i) / Ncy;
     }
     return;
   }
@@ -104,7 +111,7 @@ __device__ __forceinline__ void v_sd(const Member &m, const KP &kp, const CoefT<
     return;
   }
   const int j1 = (rt == RT_MARGIN_I) ? j : j + 1;
-  const T Na = c.Na[p2(m, j1, i)] / (T)m.dx;
+  const T Na = c.na(m, j1, i) / (T)m.dx;
   const T cl = (T)-4.0 * Na, cr = (T)4.0 * Na;
   const T su = (T)-2.0 * Na * U(j1, i - 1) + (T)2.0 * Na * U(j1, i);
   if (j1 == j) { S = cl * V(j - 1, i) + su; D = cr; }
