@@ -87,7 +87,7 @@ def use_method(cfg, meth, relax_max=None):
                       picard_relax_max=meth["picard_relax_max"] if relax_max is None else relax_max)
 
 
-def workload(name):
+def workload(name, lin_rtol=0.3):
     from imau_vel import config, geometry
     if name == "channel":
         g, cfg, st = geometry.channel(nx=401, ny=41, dx=2e3, seed=0)
@@ -99,7 +99,7 @@ def workload(name):
     config.set_config(cfg, stop_rule=1, nonlinear_rtol=1e-4, DIVA_visc_it_nit=500,
                       DIVA_choice_matrix_solver="PETSc", check_interval=8)
     for _, st in members:
-        st.DIVA_PETSc_rtol = 0.1      # inexact Picard: each linear solve reduces its residual 10x
+        st.DIVA_PETSc_rtol = lin_rtol  # inexact Picard: each linear solve reduces its residual by this factor
         st.DIVA_PETSc_abstol = 0.0
         st.DIVA_SOR_nit = 5000        # Krylov iteration cap per linear solve
     return cfg, members, desc
@@ -159,6 +159,8 @@ def main():
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison legs")
     ap.add_argument("--relax-max", type=float, default=None, help="override the headline's picard_relax_max")
     ap.add_argument("--ksp", choices=("bicgstab", "gcr"), default=None, help="override the headline's Krylov method")
+    ap.add_argument("--lin-rtol", type=float, default=0.3,
+                    help="relative tolerance of each linear solve (inexact Picard; DIVA_PETSc_rtol)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run 5 plain device-to-device copies of BYTES each (counter calibration)")
     args = ap.parse_args()
@@ -173,7 +175,7 @@ def main():
     import numpy as np
     from imau_vel import config, solver
 
-    cfg, members, desc = workload(args.workload)
+    cfg, members, desc = workload(args.workload, args.lin_rtol)
     grids = [g for g, _ in members]
     states = [st.copy() for _, st in members]    # download() writes into these
     nsolve = len(members)
@@ -214,7 +216,8 @@ def main():
                     traffic=traffic, kernel=kernel, avg_launch_us=avg_us, launches=n, cells_processed=cells_t,
                     bytes_per_launch=per_launch, bytes_per_cell=bytes_per_cell)
 
-    measured = args.workload == "ensemble" and args.method == "mixed" and args.ksp is None and args.relax_max is None
+    measured = (args.workload == "ensemble" and args.method == "mixed" and args.ksp is None and args.relax_max is None
+                and args.lin_rtol == 0.3)
     visc_traffic = VISC_TRAFFIC_BYTES_PER_CELL.get(meth["precision"]) if measured else None
     sweep_traffic = SWEEP_TRAFFIC_BYTES_PER_CELL.get(meth["precision"]) if measured else None
     roof_visc = roof("visc", "k_strain + k_visc (strain rates, 3D viscosity, N, F2: once per Picard iteration)",
