@@ -83,7 +83,7 @@ METHODS = {
 def use_method(cfg, meth, relax_max=None):
     from imau_vel import config
     config.set_config(cfg, precision=meth["precision"], mg_levels=meth["mg_levels"], ksp_type=meth["ksp_type"],
-                      anderson_m=meth["anderson_m"],
+                      anderson_m=meth["anderson_m"], mg_rebuild=meth.get("mg_rebuild", 1),
                       picard_relax_max=meth["picard_relax_max"] if relax_max is None else relax_max)
 
 
@@ -159,6 +159,7 @@ def main():
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison legs")
     ap.add_argument("--relax-max", type=float, default=None, help="override the headline's picard_relax_max")
     ap.add_argument("--ksp", choices=("bicgstab", "gcr"), default=None, help="override the headline's Krylov method")
+    ap.add_argument("--mg-rebuild", type=int, default=None, help="override the headline's mg_rebuild")
     ap.add_argument("--lin-rtol", type=float, default=0.3,
                     help="relative tolerance of each linear solve (inexact Picard; DIVA_PETSc_rtol)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -184,6 +185,8 @@ def main():
 
     # headline method
     meth = dict(METHODS[args.method])
+    if args.mg_rebuild:
+        meth["mg_rebuild"] = args.mg_rebuild
     if args.ksp:
         meth["ksp_type"] = 1 if args.ksp == "gcr" else 0
         meth["text"] = meth["text"].replace("GCR", "BiCGStab") if args.ksp == "bicgstab" else meth["text"]
@@ -217,7 +220,7 @@ def main():
                     bytes_per_launch=per_launch, bytes_per_cell=bytes_per_cell)
 
     measured = (args.workload == "ensemble" and args.method == "mixed" and args.ksp is None and args.relax_max is None
-                and args.lin_rtol == 0.3)
+                and args.lin_rtol == 0.3 and args.mg_rebuild is None)
     visc_traffic = VISC_TRAFFIC_BYTES_PER_CELL.get(meth["precision"]) if measured else None
     sweep_traffic = SWEEP_TRAFFIC_BYTES_PER_CELL.get(meth["precision"]) if measured else None
     roof_visc = roof("visc", "k_strain + k_visc (strain rates, 3D viscosity, N, F2: once per Picard iteration)",

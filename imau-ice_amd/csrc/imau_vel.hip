@@ -317,6 +317,7 @@ void check_cfg(const imau_cfg *cfg) {
     fail(IMAU_ERR_CONFIG, "picard_relax_max must lie in [0, 1], got %g", cfg->picard_relax_max);
   if (cfg->picard_relax_max > 0.0 && cfg->stop_rule != IMAU_STOP_RESIDUAL)
     fail(IMAU_ERR_CONFIG, "adaptive Picard relaxation (picard_relax_max > 0) needs stop_rule IMAU_STOP_RESIDUAL");
+  if (cfg->mg_rebuild < 1) fail(IMAU_ERR_CONFIG, "mg_rebuild must be >= 1, got %d", cfg->mg_rebuild);
   if (cfg->anderson_m < 0 || cfg->anderson_m > IMAU_AND_SLOTS - 1)
     fail(IMAU_ERR_CONFIG, "anderson_m must lie in [0, %d], got %d", IMAU_AND_SLOTS - 1, cfg->anderson_m);
   if (cfg->anderson_m > 0 && cfg->stop_rule != IMAU_STOP_RESIDUAL)
@@ -567,6 +568,7 @@ void imau_vel_default_config(imau_cfg *c) {
   c->mg_smoother = 0;
   c->ksp_type = 0;
   c->anderson_m = 0;
+  c->mg_rebuild = 1;
 }
 
 int imau_vel_init(const imau_grid *grid, const imau_cfg *cfg, imau_ctx **out) {
@@ -756,21 +758,36 @@ struct Solver {
   std::vector<double> r_abs, b_abs;  // last residual evaluation: ||r||, ||b|| per member
 
   // per-member relative FP64 residual at the current coefficients and state
+  // With GCR in KSP mode the same pass also prepares the linear solve's start
+  // vector and residual (lin_r0_), which gcr_solve then takes over.
+  bool r0_ready_ = false;
+  std::vector<double> lin_r0_;
   void residual(std::vector<double> &r) {
-    imau::resid(L, kp, coef(), P->F.u, P->F.v, P->partial.p);
-    imau::member_sum2(c->s, P->ngrids, P->tstart_d.p, P->partial.p, P->msum.p);
+    const bool fuse = cfg->DIVA_choice_matrix_solver != IMAU_SOLVER_SOR && cfg->ksp_type == 1;
+    const int NV = fuse ? 3 : 2;
+    if (fuse) {
+      imau::resid_init(L, kp, coef(), P->F.u, P->F.v, P->F.xu0, P->F.xv0, f2p(P, F_ru), f2p(P, F_rv), P->act_pic.p,
+                       P->partial.p);
+      imau::member_sum3(c->s, P->ngrids, P->tstart_d.p, P->partial.p, P->msum.p);
+    } else {
+      imau::resid(L, kp, coef(), P->F.u, P->F.v, P->partial.p);
+      imau::member_sum2(c->s, P->ngrids, P->tstart_d.p, P->partial.p, P->msum.p);
+    }
     kcheck();
-    HIPCHK(hipMemcpyAsync(host4.data(), P->msum.p, 2 * P->ngrids * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(host4.data(), P->msum.p, NV * P->ngrids * sizeof(double), hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     r.resize(P->ngrids);
     r_abs.resize(P->ngrids);
     b_abs.resize(P->ngrids);
+    lin_r0_.assign(P->ngrids, 0.0);
     for (int g = 0; g < P->ngrids; ++g) {
-      const double rr = host4[2 * g], bb = host4[2 * g + 1];
+      const double rr = host4[NV * g], bb = host4[NV * g + 1];
       r_abs[g] = std::sqrt(rr);
       b_abs[g] = std::sqrt(bb);
       r[g] = bb > 0.0 ? std::sqrt(rr) / std::sqrt(bb) : std::sqrt(rr);
+      if (fuse) lin_r0_[g] = std::sqrt(host4[NV * g + 2]);
     }
+    r0_ready_ = fuse;
   }
 
   void time_begin(int what = IMAU_TIME_SMOOTHER) {
@@ -808,10 +825,11 @@ struct Solver {
   void linear_solve() {
     act_lin = act_pic;
     push_active(P->act_lin.p, act_lin);
-    imau::init_x(L, P->F, P->F.xu0, P->F.xv0);
+    if (!r0_ready_) imau::init_x(L, P->F, P->F.xu0, P->F.xv0);
     if (cfg->DIVA_choice_matrix_solver == IMAU_SOLVER_SOR) sor_solve();
     else if (cfg->ksp_type == 1) gcr_solve();
     else krylov_solve();
+    r0_ready_ = false;
     time_collect();
   }
 
@@ -899,7 +917,11 @@ struct Solver {
     P->plan.fused = cfg->mg_smoother == 0;
     P->plan.periodic = false;
     for (int q = 0; q < 8; ++q) P->plan.periodic = P->plan.periodic || (&cfg->DIVA_boundary_BC_u_west)[q] == IMAU_BC_PERIODIC;
-    if (use_mg) imau::mg_setup(L, kp, cf, P->plan, fp32, P->act_lin.p);
+    if (use_mg) {
+      const int every = cfg->mg_rebuild > 0 ? cfg->mg_rebuild : 1;
+      imau::mg_setup(L, kp, cf, P->plan, fp32, mg_solves_ % every == 0, P->act_lin.p);
+      ++mg_solves_;
+    }
     auto precondition = [&](double *const *y, double *const *z) {
       if (use_mg) imau::mg_vcycle(L, kp, cf, P->plan, fp32, y[0], y[1], z[0], z[1], P->act_lin.p, &Solver::mg_mark, this);
       else {
@@ -976,12 +998,20 @@ struct Solver {
       HIPCHK(hipMemcpyAsync(out.data(), P->msum.p, 2 * G * sizeof(double), hipMemcpyDeviceToHost, c->s));
       HIPCHK(hipStreamSynchronize(c->s));
     };
-    // r = b - A x and the norms of r and b (the BiCGStab start kernel, r only)
-    imau::bicg_init(L, kp, cf, X[0], X[1], R[0], R[1], nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                    P->act_lin.p, P->partial.p);
-    imau::member_sum2(c->s, G, P->tstart_d.p, P->partial.p, P->msum.p);
-    kcheck();
-    fetch();
+    // r = b - A x and the norms of r and b (the BiCGStab start kernel, r only),
+    // unless the residual check of this Picard iteration prepared them
+    if (!r0_ready_) {
+      imau::bicg_init(L, kp, cf, X[0], X[1], R[0], R[1], nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                      P->act_lin.p, P->partial.p);
+      imau::member_sum2(c->s, G, P->tstart_d.p, P->partial.p, P->msum.p);
+      kcheck();
+      fetch();
+    } else {
+      for (int g = 0; g < G; ++g) {
+        out[2 * g] = lin_r0_[g] * lin_r0_[g];
+        out[2 * g + 1] = b_abs[g] * b_abs[g];
+      }
+    }
     bool any = false;
     for (int g = 0; g < G; ++g) {
       if (!act_lin[g]) continue;
@@ -997,7 +1027,11 @@ struct Solver {
     P->plan.fused = cfg->mg_smoother == 0;
     P->plan.periodic = false;
     for (int q = 0; q < 8; ++q) P->plan.periodic = P->plan.periodic || (&cfg->DIVA_boundary_BC_u_west)[q] == IMAU_BC_PERIODIC;
-    if (use_mg) imau::mg_setup(L, kp, cf, P->plan, fp32, P->act_lin.p);
+    if (use_mg) {
+      const int every = cfg->mg_rebuild > 0 ? cfg->mg_rebuild : 1;
+      imau::mg_setup(L, kp, cf, P->plan, fp32, mg_solves_ % every == 0, P->act_lin.p);
+      ++mg_solves_;
+    }
     const int maxit = sor_nit > 0 ? sor_nit : 10000;
     int nj = 0;
     for (int it = 0; it < maxit; ++it) {
@@ -1138,6 +1172,7 @@ struct Solver {
   int sor_nit = 10000;
   double sor_tol = 2.5, sor_omega = 1.3, petsc_rtol = 0.01, petsc_abstol = 2.5;
   std::vector<double> relax_h_;
+  long long mg_solves_ = 0;  // multigrid-preconditioned linear solves in this solve
 
   void picard(bool diva) {
     const int G = P->ngrids;

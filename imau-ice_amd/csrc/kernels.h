@@ -41,6 +41,10 @@ void and_prep(const Launch &L, const KP &kp, const Fields &f, const double *xu, 
 void member_sum_and(hipStream_t s, int ngrids, const int *tstart, const double *partial, double *out);
 void resid(const Launch &L, const KP &kp, const Coef &c, const double *xu, const double *xv, double *partial);
 void member_sum3(hipStream_t s, int ngrids, const int *tstart, const double *partial, double *out);
+// the nonlinear residual at the state fused with x0 (init_x) and r = b - A x0; partials
+// (|Au-b|^2, |b|^2, |r|^2) per tile, reduced with member_sum3
+void resid_init(const Launch &L, const KP &kp, const Coef &c, const double *xu, const double *xv, double *x0u,
+                double *x0v, double *ru, double *rv, const int *active, double *partial);
 void member_sum2(hipStream_t s, int ngrids, const int *tstart, const double *partial, double *out);
 void fill_u64(hipStream_t s, unsigned long long *p, int n, unsigned long long v);
 void F1(const Launch &L, const KP &kp, const Fields &f, const double *visc, const OutF &o);

@@ -597,6 +597,13 @@ __global__ void __launch_bounds__(IMAU_WG) k_coef(const Member *__restrict__ mem
 using CoefP = CoefT<double>;
 using GAcc = GAccT<double>;
 
+struct Vec2 {
+  double *u, *v;
+};
+struct CVec2 {
+  const double *u, *v;
+};
+
 // LDS tile of the sweep (u in [0], v in [1]) with a one-cell halo
 __shared__ double g_sx[2][IMAU_TI + 2][IMAU_TJ + 2];
 
@@ -783,6 +790,66 @@ __global__ void __launch_bounds__(IMAU_WG) k_resid(const Member *__restrict__ me
     }
   }
   block_sum_store<2>(acc, partial + 2 * (long long)blockIdx.x);
+}
+
+// The FP64 nonlinear residual at the current state (as k_resid) fused with the
+// start of the linear solve: x0 = u, v with the zero rows set to 0 (k_init_x)
+// and r = b - A x0.  Partial sums per tile: |A u - b|^2, |b|^2, |r|^2.
+struct X0Acc {  // the start vector seen through the state: zero rows read 0
+  const double *x;
+  const uint8_t *rt;
+  Member m;
+  __device__ __forceinline__ double operator()(int jj, int ii) const {
+    const long long p = p2(m, jj, ii);
+    const uint8_t t = rt[p];
+    return (t == RT_ZERO || t == RT_BND_ZERO) ? 0.0 : x[p];
+  }
+};
+
+__global__ void __launch_bounds__(IMAU_WG) k_resid_init(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                        const KP kp, const CoefP c, const double *__restrict__ xu,
+                                                        const double *__restrict__ xv, Vec2 x0, Vec2 r,
+                                                        const int *__restrict__ active, double *__restrict__ partial) {
+  TILE_SETUP
+  double acc[3] = {0.0, 0.0, 0.0};
+  const int nx = m.nx, ny = m.ny;
+  const GAcc U{xu, m}, V{xv, m};
+  const X0Acc U0{xu, c.rt_u, m}, V0{xv, c.rt_v, m};
+  if (j < ny && active[t.g]) {
+    FOR_TILE_ROWS {
+      const int i = t.i0 + ii;
+      if (i >= nx) break;
+      const long long p = p2(m, j, i);
+      double S, D, b;
+      if (i < nx - 1) {
+        const uint8_t rt = c.rt_u[p];
+        u_row(m, kp, c, j, i, rt, U, V, S, D, b);
+        const double rn = S + D * xu[p] - b;
+        acc[0] += rn * rn;
+        acc[1] += b * b;
+        const double x = (rt == RT_ZERO || rt == RT_BND_ZERO) ? 0.0 : xu[p];
+        u_row(m, kp, c, j, i, rt, U0, V0, S, D, b);
+        const double rl = b - (S + D * x);
+        x0.u[p] = x;
+        r.u[p] = rl;
+        acc[2] += rl * rl;
+      }
+      if (j < ny - 1) {
+        const uint8_t rt = c.rt_v[p];
+        v_row(m, kp, c, j, i, rt, U, V, S, D, b);
+        const double rn = S + D * xv[p] - b;
+        acc[0] += rn * rn;
+        acc[1] += b * b;
+        const double x = (rt == RT_ZERO || rt == RT_BND_ZERO) ? 0.0 : xv[p];
+        v_row(m, kp, c, j, i, rt, U0, V0, S, D, b);
+        const double rl = b - (S + D * x);
+        x0.v[p] = x;
+        r.v[p] = rl;
+        acc[2] += rl * rl;
+      }
+    }
+  }
+  block_sum_store<3>(acc, partial + 3 * (long long)blockIdx.x);
 }
 
 // Sum NV partials per member over its tile range [tstart[g], tstart[g+1])
@@ -1184,6 +1251,11 @@ void resid(const Launch &L, const KP &kp, const Coef &c, const double *xu, const
 void member_sum3(hipStream_t s, int ngrids, const int *tstart, const double *partial, double *out) {
   hipLaunchKernelGGL(k_member_sum<3>, dim3(ngrids), dim3(256), 0, s, tstart, partial, out);
 }
+void resid_init(const Launch &L, const KP &kp, const Coef &c, const double *xu, const double *xv, double *x0u,
+                double *x0v, double *ru, double *rv, const int *active, double *partial) {
+  hipLaunchKernelGGL(k_resid_init, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, to_p(c), xu, xv, Vec2{x0u, x0v},
+                     Vec2{ru, rv}, active, partial);
+}
 void member_sum2(hipStream_t s, int ngrids, const int *tstart, const double *partial, double *out) {
   hipLaunchKernelGGL(k_member_sum<2>, dim3(ngrids), dim3(256), 0, s, tstart, partial, out);
 }
@@ -1235,12 +1307,6 @@ namespace bicg {
 enum { RHO = 0, ALPHA = 1, OMEGA = 2, BETA = 3, RR = 4, NSC = 8 };
 }
 
-struct Vec2 {
-  double *u, *v;
-};
-struct CVec2 {
-  const double *u, *v;
-};
 
 // y = A x over every row; with w != null also partial sums (w, y) and (y, y)
 __global__ void __launch_bounds__(IMAU_WG) k_apply(const Member *__restrict__ mem, const Tile *__restrict__ tiles,

@@ -46,8 +46,10 @@ struct MGPlan {
 // marker callback around the first fine-level smoother sweep (which = 0 before, 1 after)
 typedef void (*MGMark)(void *ctx, int which);
 
-// Galerkin coarse operators for the current fine coefficients (once per linear solve)
-void mg_setup(const Launch &F, const KP &kp, const Coef &c, const MGPlan &plan, bool fp32, const int *active);
+// FP32 fine coefficients for the current operator and, with galerkin, the
+// Galerkin coarse operators (once per linear solve, or every few solves)
+void mg_setup(const Launch &F, const KP &kp, const Coef &c, const MGPlan &plan, bool fp32, bool galerkin,
+              const int *active);
 // z = V-cycle(y) for every active member
 void mg_vcycle(const Launch &F, const KP &kp, const Coef &c, const MGPlan &plan, bool fp32, const double *yu,
                const double *yv, double *zu, double *zv, const int *active, MGMark mark, void *mark_ctx);

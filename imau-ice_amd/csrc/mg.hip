@@ -866,12 +866,13 @@ void vcycle_fused_t(const imau::Launch &F, const KP &kp, const CoefT<T> &cf, con
 
 namespace imau {
 
-void mg_setup(const Launch &F, const KP &kp, const Coef &c, const MGPlan &P, bool fp32, const int *active) {
+void mg_setup(const Launch &F, const KP &kp, const Coef &c, const MGPlan &P, bool fp32, bool galerkin,
+              const int *active) {
   if (fp32) {
     hipLaunchKernelGGL(k_coef32, dim3(F.ntiles), wg(), 0, F.s, F.mem, F.tiles, coef64(c), P.f32, kp.cross ? 0 : 1,
                        active);
-    setup_t<float>(F, kp, c, P, active);
-  } else {
+    if (galerkin) setup_t<float>(F, kp, c, P, active);
+  } else if (galerkin) {
     setup_t<double>(F, kp, c, P, active);
   }
 }

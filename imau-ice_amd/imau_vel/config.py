@@ -91,6 +91,7 @@ def default_config(**overrides) -> ImauCfg:
     c.mg_smoother = 0
     c.ksp_type = 0
     c.anderson_m = 0
+    c.mg_rebuild = 1
     set_config(c, **overrides)
     return c
 

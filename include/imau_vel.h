@@ -149,6 +149,10 @@ typedef struct imau_cfg {
   int32_t ksp_type;        /* KSP-mode Krylov method: 0 = BiCGStab, 1 = GCR
                               restarted every 4 directions (one preconditioner
                               application per iteration)                       */
+  int32_t mg_rebuild;      /* rebuild the Galerkin coarse operators every
+                              mg_rebuild-th linear solve (1 = every solve; in
+                              between, the previous coarse levels precondition
+                              the current fine operator)                      */
   int32_t anderson_m;      /* 0: relaxation (fixed or adaptive).  1..3: Anderson
                               acceleration of the Picard map with m stored
                               differences, restarted for a member whose FP64
