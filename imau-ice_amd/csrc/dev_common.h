@@ -22,6 +22,9 @@
 #define IMAU_FTI 16
 #define IMAU_FTJ 64
 #define IMAU_FH 5
+// tile height of the 3D viscosity kernel (16 and 32 measured slower than 8 on
+// the 64-member ensemble: fewer workgroups in flight)
+#define IMAU_VTI 8
 
 // Anderson acceleration of the Picard loop: up to IMAU_AND_SLOTS stored
 // iterates (IMAU_AND_SLOTS - 1 differences)

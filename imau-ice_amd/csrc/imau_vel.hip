@@ -113,6 +113,8 @@ struct imau_problem {
   DBuf<float> mg_f32;                        // FP32 fine-level coefficients, rhs and iterates
   DBuf<double> mg_z64;                       // FP64 scratch iterate of the fused smoother
   DBuf<Tile> mg_ftiles;                      // tiles of the fused smoother
+  DBuf<Tile> vtiles;                         // IMAU_VTI-row tiles of the 3D viscosity kernel
+  int nvtiles = 0;
   imau::MGPlan plan;                         // device view of the levels
 
   Fields F{};
@@ -212,6 +214,13 @@ static void build_mg(imau_problem *P) {
   for (int g = 0; g < P->ngrids; ++g)
     for (int i0 = 0; i0 < P->mem[g].nx; i0 += IMAU_FTI)
       for (int j0 = 0; j0 < P->mem[g].ny; j0 += IMAU_FTJ) ft.push_back(Tile{g, i0, j0, 0});
+  std::vector<Tile> vt;
+  for (int g = 0; g < P->ngrids; ++g)
+    for (int i0 = 0; i0 < P->mem[g].nx; i0 += IMAU_VTI)
+      for (int j0 = 0; j0 < P->mem[g].ny; j0 += IMAU_TJ) vt.push_back(Tile{g, i0, j0, 0});
+  P->vtiles.alloc(vt.size());
+  HIPCHK(hipMemcpy(P->vtiles.p, vt.data(), vt.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  P->nvtiles = (int)vt.size();
   P->mg_ftiles.alloc(ft.size());
   HIPCHK(hipMemcpy(P->mg_ftiles.p, ft.data(), ft.size() * sizeof(Tile), hipMemcpyHostToDevice));
   P->plan.ftiles = P->mg_ftiles.p;
@@ -754,6 +763,7 @@ struct Solver {
     HIPCHK(hipMemcpyAsync(dev, stage.data(), stage.size() * sizeof(int), hipMemcpyHostToDevice, c->s));
   }
   void kcheck() { HIPCHK(hipGetLastError()); }
+  imau::Launch LV() const { return imau::Launch{c->s, P->nvtiles, P->mem_d.p, P->vtiles.p}; }
 
   std::vector<double> r_abs, b_abs;  // last residual evaluation: ||r||, ||b|| per member
 
@@ -1214,8 +1224,8 @@ struct Solver {
       if (!any) break;
       push_active(P->act_pic.p, act_pic);
       time_begin(IMAU_TIME_VISC);
-      if (v32) imau::visc3d32(L, kp, P->F, diva, cur32, oth32, P->visc32.p + 2 * n3, P->act_pic.p);
-      else imau::visc3d(L, kp, P->F, diva, P->visc_cur, P->visc_oth, P->act_pic.p);
+      if (v32) imau::visc3d32(L, LV(), kp, P->F, diva, cur32, oth32, P->visc32.p + 2 * n3, P->act_pic.p);
+      else imau::visc3d(L, LV(), kp, P->F, diva, P->visc_cur, P->visc_oth, P->act_pic.p);
       time_end(IMAU_TIME_VISC);
       imau::beta(L, kp, P->F, diva, P->xg.p, P->yg.p, P->act_pic.p);
       if (v32) std::swap(cur32, oth32);
@@ -1497,7 +1507,7 @@ int imau_vel_nonlinear_residual(imau_ctx *ctx, const imau_grid *grid, const imau
   imau::setup2(S.L, S.kp, P->F);
   std::vector<int> all(1, 1);
   S.push_active(P->act_pic.p, all);
-  imau::visc3d(S.L, S.kp, P->F, diva, P->visc_cur, P->visc_oth, P->act_pic.p);
+  imau::visc3d(S.L, S.LV(), S.kp, P->F, diva, P->visc_cur, P->visc_oth, P->act_pic.p);
   imau::beta(S.L, S.kp, P->F, diva, P->xg.p, P->yg.p, P->act_pic.p);
   std::swap(P->visc_cur, P->visc_oth);
   imau::coef(S.L, S.kp, P->F, diva, P->act_pic.p);

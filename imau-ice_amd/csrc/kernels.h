@@ -20,11 +20,12 @@ struct Coef {
 void setup1(const Launch &L, const KP &kp, const Fields &f, const double *yg);
 void setup2(const Launch &L, const KP &kp, const Fields &f);
 // 3D viscosity, N, F2 (k_visc) and then basal friction / beta_eff (k_beta)
-void visc3d(const Launch &L, const KP &kp, const Fields &f, bool diva, const double *visc_in, double *visc_out,
-            const int *active);
+// (L: the standard tiles, for k_strain; LV: IMAU_VTI-row tiles, for k_visc)
+void visc3d(const Launch &L, const Launch &LV, const KP &kp, const Fields &f, bool diva, const double *visc_in,
+            double *visc_out, const int *active);
 // the same in FP32 (viscosity storage and arithmetic; N, F2 written in FP64)
-void visc3d32(const Launch &L, const KP &kp, const Fields &f, bool diva, const float *visc_in, float *visc_out,
-              const float *B3, const int *active);
+void visc3d32(const Launch &L, const Launch &LV, const KP &kp, const Fields &f, bool diva, const float *visc_in,
+              float *visc_out, const float *B3, const int *active);
 void beta(const Launch &L, const KP &kp, const Fields &f, bool diva, const double *xg, const double *yg,
           const int *active);
 // element-wise precision conversion of n values

@@ -14,7 +14,7 @@
   const int j = t.j0 + (int)threadIdx.x;
 
 #define FOR_TILE_ROWS for (int ii = threadIdx.y; ii < IMAU_TI; ii += IMAU_WG_I)
-#define FOR_TILE_ROWS_SERIAL _Pragma("unroll 1") FOR_TILE_ROWS
+#define FOR_VTILE_ROWS _Pragma("unroll 1") for (int ii = threadIdx.y; ii < IMAU_VTI; ii += IMAU_WG_I)
 
 // ---------------------------------------------------------------------------
 // block reductions (wave64 shuffles, then LDS across the 4 wavefronts)
@@ -389,7 +389,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_visc(const Member *__restrict__ mem
   if (j >= m.ny) return;
   const int nx = m.nx, ny = m.ny, nz = kp.nz;
   if (!active[t.g]) {  // finished member: keep its viscosity in the buffer that becomes current
-    FOR_TILE_ROWS {
+    FOR_VTILE_ROWS {
       const int i = t.i0 + ii;
       if (i >= nx) break;
       for (int k = 0; k < nz; ++k) visc_out[p3(m, k, j, i)] = visc_in[p3(m, k, j, i)];
@@ -399,7 +399,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_visc(const Member *__restrict__ mem
   const double dx = m.dx;
   const double *__restrict__ u = f.u;
   const double *__restrict__ v = f.v;
-  FOR_TILE_ROWS_SERIAL {
+  FOR_VTILE_ROWS {
     const int i = t.i0 + ii;
     if (i >= nx) break;
     const long long p = p2(m, j, i);
@@ -1198,22 +1198,22 @@ void setup1(const Launch &L, const KP &kp, const Fields &f, const double *yg) {
 void setup2(const Launch &L, const KP &kp, const Fields &f) {
   hipLaunchKernelGGL(k_setup2, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f);
 }
-void visc3d(const Launch &L, const KP &kp, const Fields &f, bool diva, const double *visc_in, double *visc_out,
-            const int *active) {
+void visc3d(const Launch &L, const Launch &LV, const KP &kp, const Fields &f, bool diva, const double *visc_in,
+            double *visc_out, const int *active) {
   // n = 3 (every shipped configuration) takes the cube-root form of the viscosity
   const bool third = kp.n_flow == 3.0;
   auto kv = diva ? (third ? k_visc<true, true, double> : k_visc<true, false, double>)
                  : (third ? k_visc<false, true, double> : k_visc<false, false, double>);
   hipLaunchKernelGGL(k_strain, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, active);
-  hipLaunchKernelGGL(kv, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, visc_in, visc_out, f.B3, active);
+  hipLaunchKernelGGL(kv, dim3(LV.ntiles), wg(), 0, L.s, L.mem, LV.tiles, kp, f, visc_in, visc_out, f.B3, active);
 }
-void visc3d32(const Launch &L, const KP &kp, const Fields &f, bool diva, const float *visc_in, float *visc_out,
-              const float *B3, const int *active) {
+void visc3d32(const Launch &L, const Launch &LV, const KP &kp, const Fields &f, bool diva, const float *visc_in,
+              float *visc_out, const float *B3, const int *active) {
   const bool third = kp.n_flow == 3.0;
   auto kv = diva ? (third ? k_visc<true, true, float> : k_visc<true, false, float>)
                  : (third ? k_visc<false, true, float> : k_visc<false, false, float>);
   hipLaunchKernelGGL(k_strain, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, active);
-  hipLaunchKernelGGL(kv, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, visc_in, visc_out, B3, active);
+  hipLaunchKernelGGL(kv, dim3(LV.ntiles), wg(), 0, L.s, L.mem, LV.tiles, kp, f, visc_in, visc_out, B3, active);
 }
 __global__ void k_d2f(const double *__restrict__ a, float *__restrict__ b, long long n) {
   for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
