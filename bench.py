@@ -59,8 +59,8 @@ def VISC_BYTES_PER_CELL(nz, word):
 # (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0: the factors a 1 GiB device
 # copy reads; profiles/ensemble_mixed_kernel_stats.md).  Re-measure after
 # changing either kernel.
-VISC_TRAFFIC_BYTES_PER_CELL = {}
-SWEEP_TRAFFIC_BYTES_PER_CELL = {"mixed_visc": 146.3}
+VISC_TRAFFIC_BYTES_PER_CELL = {"mixed_visc": 442.4}
+SWEEP_TRAFFIC_BYTES_PER_CELL = {"mixed_visc": 153.5}
 METHODS = {
     "mixed": dict(precision="mixed_visc", mg_levels=0, picard_relax_max=0.0, ksp_type=1, anderson_m=3,
                   text="GCR in FP64 (operator, residuals, update) right-preconditioned by an FP32 aggregation-"
