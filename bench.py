@@ -3,14 +3,22 @@
 Default workload ("ensemble"): 64 DIVA solves of uneven sizes, n x n with
 n ~ U[96, 512] (rounded to odd), 4 km grids, seeded land-terminating random
 ice caps (Matern bed, parabolic surface with fractal noise), A and the beta_sq
-scale log-uniform over one decade, Weertman m = 3, cold start.  All members
-are device-resident in one batched problem; one step is one complete batched
-solve (state reset + Picard loop until every member has
-||A(u)u - b(u)|| / ||b(u)|| <= 1e-4 in FP64).  value = 64 x steps / seconds.
+scale log-uniform over one decade, Weertman m = 3.  All members are
+device-resident in one batched problem; one step is one complete batched solve
+from the uploaded cold state: the learned warm start (network features,
+forward pass, velocities onto the staggered grids, viscosity passes), then the
+Picard loop until every member has ||A(u)u - b(u)|| / ||b(u)|| <= 1e-4 in
+FP64.  value = 64 x steps / seconds.  "--method mixed" is the same solver from
+the cold start.
 
 "--workload channel": the 401 x 41 marine channel at 2 km, one solve per step.
 
 The line also carries:
+  roofline_net  the network's forward pass over the ensemble: its multiply-add
+                work / wall time around a drained stream, against the dense
+                BF16 matrix-core peak;
+  cold_start, learned_gain
+                the same solver from the cold start, and the ratio;
   roofline      the dominant kernel, k_visc (3D viscosity, one launch per
                 Picard iteration): algorithmic bytes / launch duration from
                 HIP events on the library stream, over the members each
@@ -36,6 +44,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "imau-ice_amd"))
 
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16 matrix-core peak
 # one fine-level smoother sweep (one k_smooth_fused launch): read u, v,
 # N_a, N_b, beta_eff_cx, beta_eff_cy and the two right-hand sides, write u, v
 # (10 words per cell, 4 bytes each in the mixed mode, 8 in FP64) plus the two
@@ -62,12 +71,15 @@ def VISC_BYTES_PER_CELL(nz, word):
 VISC_TRAFFIC_BYTES_PER_CELL = {"mixed_visc": 442.4}
 SWEEP_TRAFFIC_BYTES_PER_CELL = {"mixed_visc": 153.5}
 METHODS = {
-    "mixed": dict(precision="mixed_visc", mg_levels=0, picard_relax_max=0.0, ksp_type=1, anderson_m=3,
+    "learned": dict(precision="mixed", mg_levels=0, picard_relax_max=0.0, ksp_type=1, anderson_m=3, learned=True,
+                    text="learned warm start (BF16 conv net on the matrix cores, viscosity passes), then GCR in FP64 "
+                         "right-preconditioned by an FP32 aggregation-multigrid V-cycle; FP64 viscosity and beta; "
+                         "Anderson-accelerated Picard loop (3 differences); cold solve as fallback"),
+    "mixed": dict(precision="mixed", mg_levels=0, picard_relax_max=0.0, ksp_type=1, anderson_m=3,
                   text="GCR in FP64 (operator, residuals, update) right-preconditioned by an FP32 aggregation-"
-                       "multigrid V-cycle; FP32 3D viscosity; Anderson-accelerated Picard loop (3 differences)"),
-    "mixed64visc": dict(precision="mixed", mg_levels=0, picard_relax_max=0.0, ksp_type=1, anderson_m=3,
-                        text="GCR in FP64 right-preconditioned by an FP32 aggregation-multigrid V-cycle; FP64 "
-                             "viscosity/beta; Anderson-accelerated Picard loop (3 differences)"),
+                       "multigrid V-cycle; FP64 viscosity and beta; Anderson-accelerated Picard loop (3 differences)"),
+    "mixed_visc": dict(precision="mixed_visc", mg_levels=0, picard_relax_max=0.0, ksp_type=1, anderson_m=3,
+                       text="as mixed, with the 3D viscosity in FP32 (experimental: diverges on some geometries)"),
     "adaptive": dict(precision="mixed", mg_levels=0, picard_relax_max=1.0, ksp_type=1, anderson_m=0,
                      text="GCR in FP64 right-preconditioned by an FP32 aggregation-multigrid V-cycle; Picard loop "
                           "with adaptive relaxation (0.4 raised x1.25 per residual decrease up to 1.0)"),
@@ -135,16 +147,21 @@ def copy_probe(nbytes):
     print(json.dumps(dict(copy_probe_bytes=nbytes, copies=5)))
 
 
-def timed_solves(prob, cfg, steps, warmup):
-    """steps complete solves from the uploaded state; each solve returns only
-    after its stream has drained, so wall time brackets the GPU work."""
-    for _ in range(warmup):
+def timed_solves(prob, cfg, steps, warmup, ws=None):
+    """steps complete solves from the uploaded state (with the learned warm
+    start when ws is given); each solve returns only after its stream has
+    drained, so wall time brackets the GPU work."""
+    def step():
         prob.reset_state()
-        prob.solve(cfg)
+        if ws is not None:
+            ws.solve(cfg)
+        else:
+            prob.solve(cfg)
+    for _ in range(warmup):
+        step()
     t0 = time.perf_counter()
     for _ in range(steps):
-        prob.reset_state()
-        prob.solve(cfg)
+        step()
     return time.perf_counter() - t0
 
 
@@ -155,7 +172,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=("ensemble", "channel"), default="ensemble")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--method", choices=tuple(METHODS), default="mixed")
+    ap.add_argument("--method", choices=tuple(METHODS), default="learned")
+    ap.add_argument("--warm-iters", type=int, default=6,
+                    help="viscosity passes completing the learned warm start (warm_visc_iters)")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison legs")
     ap.add_argument("--relax-max", type=float, default=None, help="override the headline's picard_relax_max")
     ap.add_argument("--ksp", choices=("bicgstab", "gcr"), default=None, help="override the headline's Krylov method")
@@ -174,6 +193,9 @@ def main():
         sys.exit(2)
 
     import numpy as np
+    meth = dict(METHODS[args.method])
+    if meth.get("learned"):
+        from imau_vel import learned   # imports torch before the library initialises HIP
     from imau_vel import config, solver
 
     cfg, members, desc = workload(args.workload, args.lin_rtol)
@@ -182,16 +204,16 @@ def main():
     nsolve = len(members)
     ctx = solver.Context()
     prob = ctx.problem(grids, cfg, states)
+    ws = learned.WarmStartSolver(ctx, prob, warm_iters=args.warm_iters) if meth.get("learned") else None
 
     # headline method
-    meth = dict(METHODS[args.method])
     if args.mg_rebuild:
         meth["mg_rebuild"] = args.mg_rebuild
     if args.ksp:
         meth["ksp_type"] = 1 if args.ksp == "gcr" else 0
         meth["text"] = meth["text"].replace("GCR", "BiCGStab") if args.ksp == "bicgstab" else meth["text"]
     use_method(cfg, meth, args.relax_max)
-    dt = timed_solves(prob, cfg, args.steps, args.warmup)
+    dt = timed_solves(prob, cfg, args.steps, args.warmup, ws)
     out = prob.download()
     its = [o.stats["picard_iterations"] for o in out]
     apps = [o.stats["linear_sweeps"] for o in out]
@@ -201,9 +223,7 @@ def main():
     timed = {}
     for what in ("visc", "smoother"):
         ctx.kernel_timing(what)
-        for _ in range(args.steps):
-            prob.reset_state()
-            prob.solve(cfg)
+        timed_solves(prob, cfg, args.steps, 0, ws)
         timed[what] = ctx.kernel_timing(False)
     cells = sum(g.nx * g.ny for g in grids)
     word = 4 if meth["precision"] in ("mixed", "mixed_visc") else 8
@@ -219,7 +239,7 @@ def main():
                     traffic=traffic, kernel=kernel, avg_launch_us=avg_us, launches=n, cells_processed=cells_t,
                     bytes_per_launch=per_launch, bytes_per_cell=bytes_per_cell)
 
-    measured = (args.workload == "ensemble" and args.method == "mixed" and args.ksp is None and args.relax_max is None
+    measured = (args.workload == "ensemble" and args.method in ("learned", "mixed") and args.ksp is None and args.relax_max is None
                 and args.lin_rtol == 0.3 and args.mg_rebuild is None)
     visc_traffic = VISC_TRAFFIC_BYTES_PER_CELL.get(meth["precision"]) if measured else None
     sweep_traffic = SWEEP_TRAFFIC_BYTES_PER_CELL.get(meth["precision"]) if measured else None
@@ -229,20 +249,53 @@ def main():
                       else "k_prec_tile (one tile Gauss-Seidel application)",
                       SWEEP_WORDS_PER_CELL * word + SWEEP_BYTES_EXTRA, sweep_traffic if meth["mg_levels"] == 0 else None)
 
+    dtype = {"mixed": "f32 preconditioner / f64 residual", "mixed_visc": "f32 viscosity and preconditioner / f64 "
+             "residual and update"}.get(meth["precision"], "f64")
+    if ws is not None:
+        dtype = "bf16 network / " + dtype
     result = dict(
         metric="DIVA solves/s at 1e-4 rel residual: learned+FP32 vs plain FP64 GPU",
         value=nsolve * args.steps / dt, unit="solves/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
         ms_per_step=1e3 * dt / args.steps, higher_is_better=True, scaling="weak", vs_baseline=None,
-        dtype={"mixed": "f32 preconditioner / f64 residual", "mixed_visc": "f32 viscosity and preconditioner / f64 "
-               "residual and update"}.get(meth["precision"], "f64"), data="synthetic",
+        dtype=dtype, data="synthetic",
         config=dict(workload=desc, members=nsolve, cells=cells, method=meth["text"],
-                    stop="FP64 nonlinear residual <= 1e-4 for every member", learned_warm_start=False),
+                    stop="FP64 nonlinear residual <= 1e-4 for every member", learned_warm_start=ws is not None),
         picard_iterations=dict(min=min(its), max=max(its), mean=float(np.mean(its))),
         preconditioner_applications=dict(min=min(apps), max=max(apps)),
         max_final_residual=max(res),
         roofline=roof_visc,
         roofline_smoother=roof_sweep,
     )
+    if ws is not None:
+        # the network: one forward pass over the whole ensemble (features, 10
+        # convolutions, pooling, upsampling), host-timed around a drained stream
+        import torch
+        flops = learned.forward_flops(ws.io.plan)
+        for _ in range(2):
+            ws.net.forward(ws.io.plan, ws.io.features(cfg))
+        ws.net.sync()
+        reps = 10
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ws.net.forward(ws.io.plan, ws.io.features(cfg))
+        ws.net.sync()
+        t_fwd = (time.perf_counter() - t0) / reps
+        result["roofline_net"] = dict(bound="mfma", achieved=flops / t_fwd / 1e12, peak=MFMA_BF16_PEAK_TFLOPS,
+                                      unit="TFLOP/s", frac=flops / t_fwd / 1e12 / MFMA_BF16_PEAK_TFLOPS, traffic=None,
+                                      kernel="features + forward pass of the warm-start network (k_conv x10, "
+                                             "pool, upsample), one launch sequence over the ensemble",
+                                      ms_per_forward=1e3 * t_fwd, flop_per_forward=flops, pixels=ws.io.plan.npix[0])
+        result["warm_start"] = dict(network=os.path.relpath(learned.WEIGHTS, os.path.dirname(__file__)),
+                                    warm_visc_iters=args.warm_iters, fallbacks=ws.fallbacks,
+                                    fallback="cold solve of the whole ensemble when any member's FP64 residual "
+                                             "stays above 1e-4")
+        # the same solver from the cold start
+        dtc = timed_solves(prob, cfg, args.steps, 0)
+        itc, rc = prob.stats()
+        result["cold_start"] = dict(value=nsolve * args.steps / dtc, unit="solves/s",
+                                    picard_iterations=dict(min=min(itc), max=max(itc), mean=float(np.mean(itc))),
+                                    max_final_residual=max(rc))
+        result["learned_gain"] = result["value"] / result["cold_start"]["value"]
     if not args.no_compare:
         for key in ("fp64", "plain"):
             if key == args.method:

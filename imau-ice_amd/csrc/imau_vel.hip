@@ -14,10 +14,12 @@
 #include <string>
 #include <vector>
 
+#include "../../include/imau_net.h"
 #include "../../include/imau_vel.h"
 #include "dev_common.h"
 #include "kernels.h"
 #include "mg.h"
+#include "net.h"
 
 namespace {
 
@@ -327,6 +329,12 @@ void check_cfg(const imau_cfg *cfg) {
   if (cfg->picard_relax_max > 0.0 && cfg->stop_rule != IMAU_STOP_RESIDUAL)
     fail(IMAU_ERR_CONFIG, "adaptive Picard relaxation (picard_relax_max > 0) needs stop_rule IMAU_STOP_RESIDUAL");
   if (cfg->mg_rebuild < 1) fail(IMAU_ERR_CONFIG, "mg_rebuild must be >= 1, got %d", cfg->mg_rebuild);
+  if (!(cfg->anderson_beta > 0.0 && cfg->anderson_beta <= 1.0))
+    fail(IMAU_ERR_CONFIG, "anderson_beta must lie in (0, 1], got %g", cfg->anderson_beta);
+  if (!(cfg->anderson_restart >= 1.0))
+    fail(IMAU_ERR_CONFIG, "anderson_restart must be >= 1, got %g", cfg->anderson_restart);
+  if (cfg->warm_visc_iters < 0 || cfg->warm_visc_iters > 50)
+    fail(IMAU_ERR_CONFIG, "warm_visc_iters must lie in [0, 50], got %d", cfg->warm_visc_iters);
   if (cfg->anderson_m < 0 || cfg->anderson_m > IMAU_AND_SLOTS - 1)
     fail(IMAU_ERR_CONFIG, "anderson_m must lie in [0, %d], got %d", IMAU_AND_SLOTS - 1, cfg->anderson_m);
   if (cfg->anderson_m > 0 && cfg->stop_rule != IMAU_STOP_RESIDUAL)
@@ -578,6 +586,9 @@ void imau_vel_default_config(imau_cfg *c) {
   c->ksp_type = 0;
   c->anderson_m = 0;
   c->mg_rebuild = 1;
+  c->warm_visc_iters = 0;
+  c->anderson_beta = 1.0;
+  c->anderson_restart = 1.5;
 }
 
 int imau_vel_init(const imau_grid *grid, const imau_cfg *cfg, imau_ctx **out) {
@@ -703,6 +714,18 @@ int imau_vel_problem_create(imau_ctx *ctx, int32_t ngrids, const imau_grid *grid
 }
 
 void imau_vel_problem_destroy(imau_problem *P) { delete P; }
+
+int imau_vel_problem_stats(imau_ctx *ctx, const imau_problem *P, int32_t *picard_iterations,
+                           double *nonlinear_residual) {
+  ABI_TRY
+  if (!ctx || !P) fail(IMAU_ERR_ARG, "NULL argument");
+  for (int g = 0; g < P->ngrids; ++g) {
+    if (picard_iterations) picard_iterations[g] = P->picard_its.empty() ? 0 : P->picard_its[g];
+    if (nonlinear_residual) nonlinear_residual[g] = P->nl_resid.empty() ? NAN : P->nl_resid[g];
+  }
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
 
 int imau_vel_problem_reset_state(imau_ctx *ctx, imau_problem *P) {
   ABI_TRY
@@ -1157,7 +1180,7 @@ struct Solver {
           }
           for (int i = 0; i < nv; ++i) {
             W[sl[i]] = w[i];
-            W[IMAU_AND_SLOTS + sl[i]] = w[i];
+            W[IMAU_AND_SLOTS + sl[i]] = cfg->anderson_beta * w[i];
           }
           mixed = true;
         }
@@ -1218,18 +1241,32 @@ struct Solver {
       imau::to_f32(c->s, P->F.B3, P->visc32.p + 2 * n3, n3);
       imau::to_f32(c->s, P->visc_cur, cur32, n3);
     }
+    // viscosity, F2 and beta_eff at the current velocities (A4-A8)
+    auto visc_beta = [&](bool timed) {
+      if (timed) time_begin(IMAU_TIME_VISC);
+      if (v32) imau::visc3d32(L, LV(), kp, P->F, diva, cur32, oth32, P->visc32.p + 2 * n3, P->act_pic.p);
+      else imau::visc3d(L, LV(), kp, P->F, diva, P->visc_cur, P->visc_oth, P->act_pic.p);
+      if (timed) time_end(IMAU_TIME_VISC);
+      imau::beta(L, kp, P->F, diva, P->xg.p, P->yg.p, P->act_pic.p);
+      if (v32) std::swap(cur32, oth32);
+      else std::swap(P->visc_cur, P->visc_oth);
+    };
+    // warm start given as velocities only: bring viscosity and basal stress to them
+    if (diva && cfg->warm_visc_iters > 0) {
+      push_active(P->act_pic.p, act_pic);
+      for (int w = 0; w < cfg->warm_visc_iters; ++w) {
+        visc_beta(false);
+        imau::coef(L, kp, P->F, diva, P->act_pic.p);  // beta_eff on cx / cy
+        imau::taub(L, P->F, P->act_pic.p);
+      }
+      kcheck();
+    }
     for (int it = 1;; ++it) {
       bool any = false;
       for (int g = 0; g < G; ++g) any = any || act_pic[g];
       if (!any) break;
       push_active(P->act_pic.p, act_pic);
-      time_begin(IMAU_TIME_VISC);
-      if (v32) imau::visc3d32(L, LV(), kp, P->F, diva, cur32, oth32, P->visc32.p + 2 * n3, P->act_pic.p);
-      else imau::visc3d(L, LV(), kp, P->F, diva, P->visc_cur, P->visc_oth, P->act_pic.p);
-      time_end(IMAU_TIME_VISC);
-      imau::beta(L, kp, P->F, diva, P->xg.p, P->yg.p, P->act_pic.p);
-      if (v32) std::swap(cur32, oth32);
-      else std::swap(P->visc_cur, P->visc_oth);
+      visc_beta(true);
       imau::coef(L, kp, P->F, diva, P->act_pic.p);
       kcheck();
       if (cfg->stop_rule == IMAU_STOP_RESIDUAL) {
@@ -1247,7 +1284,7 @@ struct Solver {
             prev_r[g] = r[g];
           }
           if (and_m > 0) {  // restart the member's history after a 1.5x residual rise
-            if (prev_r[g] >= 0.0 && r[g] > 1.5 * prev_r[g]) and_since_[g] = and_count_;
+            if (prev_r[g] >= 0.0 && r[g] > cfg->anderson_restart * prev_r[g]) and_since_[g] = and_count_;
             prev_r[g] = r[g];
           }
         }
@@ -1523,6 +1560,281 @@ int imau_vel_nonlinear_residual(imau_ctx *ctx, const imau_grid *grid, const imau
     imau_vel_problem_destroy(P);
     return e.code;
   }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// the warm-start network (include/imau_net.h)
+// ---------------------------------------------------------------------------
+struct imau_net_plan {
+  int nimg = 0;
+  std::vector<int> rows0, cols0;
+  long long npix[IMAU_NET_LEVELS] = {0, 0, 0, 0};
+  int ntiles[IMAU_NET_LEVELS] = {0, 0, 0, 0};
+  DBuf<imau::net::NImg> img[IMAU_NET_LEVELS];
+  DBuf<imau::net::NTile> tiles[IMAU_NET_LEVELS];
+  DBuf<long long> poff;  // level-0 pixel offset per image
+  imau::net::LevelView view(int l) const {
+    return imau::net::LevelView{img[l].p, tiles[l].p, ntiles[l], npix[l]};
+  }
+};
+
+namespace {
+void net_plan_build(imau_net_plan *N, int nimg, const int32_t *rows, const int32_t *cols) {
+  N->nimg = nimg;
+  N->rows0.assign(rows, rows + nimg);
+  N->cols0.assign(cols, cols + nimg);
+  std::vector<long long> off0(nimg);
+  for (int l = 0; l < IMAU_NET_LEVELS; ++l) {
+    std::vector<imau::net::NImg> im(nimg);
+    std::vector<imau::net::NTile> tl;
+    long long off = 0;
+    for (int g = 0; g < nimg; ++g) {
+      int r = rows[g], c = cols[g];
+      for (int k = 0; k < l; ++k) { r = (r + 1) / 2; c = (c + 1) / 2; }
+      im[g] = imau::net::NImg{r, c, off};
+      if (l == 0) off0[g] = off;
+      off += (long long)r * c;
+      for (int r0 = 0; r0 < r; r0 += IMAU_NET_TR)
+        for (int c0 = 0; c0 < c; c0 += IMAU_NET_TC) tl.push_back(imau::net::NTile{g, r0, c0, 0});
+    }
+    N->npix[l] = off;
+    N->ntiles[l] = (int)tl.size();
+    N->img[l].alloc(nimg);
+    N->tiles[l].alloc(tl.size());
+    HIPCHK(hipMemcpy(N->img[l].p, im.data(), nimg * sizeof(imau::net::NImg), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(N->tiles[l].p, tl.data(), tl.size() * sizeof(imau::net::NTile), hipMemcpyHostToDevice));
+  }
+  N->poff.alloc(nimg);
+  HIPCHK(hipMemcpy(N->poff.p, off0.data(), nimg * sizeof(long long), hipMemcpyHostToDevice));
+}
+
+void net_check_level(const imau_net_plan *N, int level, bool coarser) {
+  if (!N) fail(IMAU_ERR_ARG, "NULL plan");
+  const int top = coarser ? IMAU_NET_LEVELS - 2 : IMAU_NET_LEVELS - 1;
+  if (level < 0 || level > top) fail(IMAU_ERR_ARG, "level %d outside [0, %d]", level, top);
+}
+
+void net_check_problem(const imau_problem *P, const imau_net_plan *N) {
+  if (!P || !N) fail(IMAU_ERR_ARG, "NULL argument");
+  if (N->nimg != P->ngrids) fail(IMAU_ERR_ARG, "plan has %d images, problem %d members", N->nimg, P->ngrids);
+  for (int g = 0; g < P->ngrids; ++g)
+    if (N->rows0[g] != P->mem[g].nx || N->cols0[g] != P->mem[g].ny)
+      fail(IMAU_ERR_ARG, "plan image %d is %d x %d, member grid is %d x %d", g, N->rows0[g], N->cols0[g],
+           P->mem[g].nx, P->mem[g].ny);
+}
+
+bool net_pair_ok(int cin, int cout) {
+  return (cin == 16 && cout == 32) || (cin == 32 && cout == 32) || (cin == 32 && cout == 16);
+}
+}  // namespace
+
+extern "C" {
+
+int imau_net_plan_create(imau_ctx *ctx, int32_t nimg, const int32_t *rows, const int32_t *cols,
+                         imau_net_plan **plan) {
+  imau_net_plan *N = nullptr;
+  ABI_TRY
+  if (!ctx || !rows || !cols || !plan) fail(IMAU_ERR_ARG, "NULL argument");
+  if (nimg < 1) fail(IMAU_ERR_ARG, "nimg=%d", nimg);
+  for (int g = 0; g < nimg; ++g)
+    if (rows[g] < 1 || cols[g] < 1) fail(IMAU_ERR_ARG, "image %d is %d x %d", g, rows[g], cols[g]);
+  N = new imau_net_plan();
+  net_plan_build(N, nimg, rows, cols);
+  *plan = N;
+  return IMAU_OK;
+  }
+  catch (const Err &e) {
+    if (ctx) ctx->err = e.msg;
+    delete N;
+    return e.code;
+  }
+}
+
+int imau_net_plan_for_problem(imau_ctx *ctx, const imau_problem *P, imau_net_plan **plan) {
+  ABI_TRY
+  if (!ctx || !P || !plan) fail(IMAU_ERR_ARG, "NULL argument");
+  std::vector<int32_t> r(P->ngrids), c(P->ngrids);
+  for (int g = 0; g < P->ngrids; ++g) { r[g] = P->mem[g].nx; c[g] = P->mem[g].ny; }
+  return imau_net_plan_create(ctx, P->ngrids, r.data(), c.data(), plan);
+  ABI_CATCH(ctx)
+}
+
+void imau_net_plan_destroy(imau_net_plan *plan) { delete plan; }
+
+int64_t imau_net_plan_pixels(const imau_net_plan *plan, int32_t level) {
+  if (!plan || level < 0 || level >= IMAU_NET_LEVELS) return -1;
+  return plan->npix[level];
+}
+
+int64_t imau_net_grad_w_work(const imau_net_plan *plan, int32_t level, int32_t cin, int32_t cout) {
+  if (!plan || level < 0 || level >= IMAU_NET_LEVELS || !net_pair_ok(cin, cout)) return -1;
+  return (int64_t)imau::net::dw_blocks(plan->view(level)) * (9 * cin * cout + cout);
+}
+
+int64_t imau_net_loss_work(const imau_net_plan *plan) {
+  if (!plan) return -1;
+  return imau::net::loss_blocks(plan->view(0));
+}
+
+int imau_net_conv(imau_ctx *ctx, const imau_net_plan *plan, int32_t level, int32_t cin, int32_t cout,
+                  const uint16_t *x, const uint16_t *w, const float *bias, int32_t relu, void *y, int32_t y_f32) {
+  ABI_TRY
+  if (!ctx || !x || !w || !y) fail(IMAU_ERR_ARG, "NULL argument");
+  net_check_level(plan, level, false);
+  if (!net_pair_ok(cin, cout)) fail(IMAU_ERR_ARG, "unsupported convolution %d -> %d channels", cin, cout);
+  if (y_f32 && cout != IMAU_NET_OUT) fail(IMAU_ERR_ARG, "FP32 output only for the %d-channel last layer", IMAU_NET_OUT);
+  if (!imau::net::conv(ctx->s, plan->view(level), cin, cout, x, w, bias, relu, y, y_f32))
+    fail(IMAU_ERR_ARG, "unsupported convolution %d -> %d channels", cin, cout);
+  HIPCHK(hipGetLastError());
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+int imau_net_conv_grad_w(imau_ctx *ctx, const imau_net_plan *plan, int32_t level, int32_t cin, int32_t cout,
+                         const uint16_t *g, const uint16_t *x, float *dw, float *db, float *work) {
+  ABI_TRY
+  if (!ctx || !g || !x || !dw || !db || !work) fail(IMAU_ERR_ARG, "NULL argument");
+  net_check_level(plan, level, false);
+  if (!net_pair_ok(cin, cout)) fail(IMAU_ERR_ARG, "unsupported convolution %d -> %d channels", cin, cout);
+  if (!imau::net::conv_dw(ctx->s, plan->view(level), cin, cout, g, x, dw, db, work))
+    fail(IMAU_ERR_ARG, "unsupported convolution %d -> %d channels", cin, cout);
+  HIPCHK(hipGetLastError());
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+int imau_net_pool(imau_ctx *ctx, const imau_net_plan *plan, int32_t level, const uint16_t *x, uint16_t *y) {
+  ABI_TRY
+  if (!ctx || !x || !y) fail(IMAU_ERR_ARG, "NULL argument");
+  net_check_level(plan, level, true);
+  imau::net::pool(ctx->s, plan->view(level), plan->view(level + 1), x, y);
+  HIPCHK(hipGetLastError());
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+int imau_net_upadd(imau_ctx *ctx, const imau_net_plan *plan, int32_t level, const uint16_t *a, const uint16_t *b,
+                   uint16_t *y) {
+  ABI_TRY
+  if (!ctx || !a || !b || !y) fail(IMAU_ERR_ARG, "NULL argument");
+  net_check_level(plan, level, true);
+  imau::net::upadd(ctx->s, plan->view(level), plan->view(level + 1), a, b, y);
+  HIPCHK(hipGetLastError());
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+int imau_net_grad_pool(imau_ctx *ctx, const imau_net_plan *plan, int32_t level, const uint16_t *a,
+                       const uint16_t *pc, const uint16_t *yact, uint16_t *g) {
+  ABI_TRY
+  if (!ctx || !yact || !g) fail(IMAU_ERR_ARG, "NULL argument");
+  net_check_level(plan, level, pc != nullptr);
+  imau::net::gpool(ctx->s, plan->view(level), pc ? plan->view(level + 1) : plan->view(level), a, pc, yact, g);
+  HIPCHK(hipGetLastError());
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+int imau_net_grad_up(imau_ctx *ctx, const imau_net_plan *plan, int32_t level, const uint16_t *a,
+                     const uint16_t *yact, uint16_t *g) {
+  ABI_TRY
+  if (!ctx || !a || !yact || !g) fail(IMAU_ERR_ARG, "NULL argument");
+  net_check_level(plan, level, true);
+  imau::net::gup(ctx->s, plan->view(level), plan->view(level + 1), a, yact, g);
+  HIPCHK(hipGetLastError());
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+int imau_net_loss(imau_ctx *ctx, const imau_net_plan *plan, const float *out, const float *target,
+                  const float *weight, float scale, uint16_t *g, double *work, double *loss) {
+  ABI_TRY
+  if (!ctx || !plan || !out || !target || !weight || !g || !work) fail(IMAU_ERR_ARG, "NULL argument");
+  const imau::net::LevelView V = plan->view(0);
+  imau::net::loss(ctx->s, V, out, target, weight, scale, g, work);
+  HIPCHK(hipGetLastError());
+  if (loss) {
+    const int nb = imau::net::loss_blocks(V);
+    std::vector<double> h(nb);
+    HIPCHK(hipMemcpyAsync(h.data(), work, nb * sizeof(double), hipMemcpyDeviceToHost, ctx->s));
+    HIPCHK(hipStreamSynchronize(ctx->s));
+    double sum = 0.0;
+    for (int b = 0; b < nb; ++b) sum += h[b];
+    *loss = scale * sum;
+  }
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+int imau_net_adam(imau_ctx *ctx, int64_t n, float *p, const float *g, float *m, float *v, float lr, float beta1,
+                  float beta2, float eps, int32_t step) {
+  ABI_TRY
+  if (!ctx || !p || !g || !m || !v) fail(IMAU_ERR_ARG, "NULL argument");
+  if (n < 0 || step < 1) fail(IMAU_ERR_ARG, "n=%lld step=%d", (long long)n, step);
+  const float bc1 = 1.0f - std::pow(beta1, (float)step), bc2 = 1.0f - std::pow(beta2, (float)step);
+  imau::net::adam(ctx->s, n, p, g, m, v, lr, beta1, beta2, eps, bc1, bc2);
+  HIPCHK(hipGetLastError());
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+int imau_net_pack(imau_ctx *ctx, int32_t cin, int32_t cout, const float *w, uint16_t *wf, uint16_t *wt) {
+  ABI_TRY
+  if (!ctx || !w || !wf || !wt) fail(IMAU_ERR_ARG, "NULL argument");
+  if (!net_pair_ok(cin, cout)) fail(IMAU_ERR_ARG, "unsupported convolution %d -> %d channels", cin, cout);
+  imau::net::pack(ctx->s, cin, cout, w, wf, wt);
+  HIPCHK(hipGetLastError());
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+int imau_net_features(imau_ctx *ctx, imau_problem *P, const imau_cfg *cfg, const imau_net_plan *plan,
+                      uint16_t *x) {
+  ABI_TRY
+  if (!ctx || !cfg || !x) fail(IMAU_ERR_ARG, "NULL argument");
+  net_check_problem(P, plan);
+  const KP kp = make_kp(cfg);
+  const imau::Launch L{ctx->s, (int)P->tiles.size(), P->mem_d.p, P->tiles_d.p};
+  imau::net_features(L, kp, P->F, plan->poff.p, x);
+  HIPCHK(hipGetLastError());
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+int imau_net_targets(imau_ctx *ctx, imau_problem *P, const imau_net_plan *plan, float *target, float *weight) {
+  ABI_TRY
+  if (!ctx || !target || !weight) fail(IMAU_ERR_ARG, "NULL argument");
+  net_check_problem(P, plan);
+  const imau::Launch L{ctx->s, (int)P->tiles.size(), P->mem_d.p, P->tiles_d.p};
+  imau::net_targets(L, P->F, plan->poff.p, target, weight);
+  HIPCHK(hipGetLastError());
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+int imau_net_warm_start(imau_ctx *ctx, imau_problem *P, const imau_cfg *cfg, const imau_net_plan *plan,
+                        const float *out) {
+  ABI_TRY
+  if (!ctx || !cfg || !out) fail(IMAU_ERR_ARG, "NULL argument");
+  net_check_problem(P, plan);
+  const KP kp = make_kp(cfg);
+  const imau::Launch L{ctx->s, (int)P->tiles.size(), P->mem_d.p, P->tiles_d.p};
+  HIPCHK(hipMemcpyAsync(P->act_lin.p, P->sheet_any.data(), P->ngrids * sizeof(int), hipMemcpyHostToDevice, ctx->s));
+  imau::net_warm(L, kp, P->F, plan->poff.p, out, P->act_lin.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->s));  // the host copy source is pageable
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+int imau_net_sync(imau_ctx *ctx) {
+  ABI_TRY
+  if (!ctx) fail(IMAU_ERR_ARG, "NULL argument");
+  HIPCHK(hipStreamSynchronize(ctx->s));
+  return IMAU_OK;
+  ABI_CATCH(ctx)
 }
 
 }  // extern "C"

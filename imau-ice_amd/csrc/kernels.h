@@ -32,6 +32,13 @@ void beta(const Launch &L, const KP &kp, const Fields &f, bool diva, const doubl
 void to_f32(hipStream_t s, const double *a, float *b, long long n);
 void to_f64(hipStream_t s, const float *a, double *b, long long n);
 void coef(const Launch &L, const KP &kp, const Fields &f, bool diva, const int *active);
+// warm-start network I/O (image g = member g's a-grid, pixel poff[g] + i ny + j)
+void net_features(const Launch &L, const KP &kp, const Fields &f, const long long *poff, unsigned short *x);
+void net_targets(const Launch &L, const Fields &f, const long long *poff, float *tg, float *wt);
+void net_warm(const Launch &L, const KP &kp, const Fields &f, const long long *poff, const float *o,
+              const int *active);
+// tau_b = beta_eff u on cx / cy (warm-start completion)
+void taub(const Launch &L, const Fields &f, const int *active);
 void init_x(const Launch &L, const Fields &f, double *xu, double *xv);
 // relax: per-member relaxation factors, or nullptr for the configured DIVA_visc_it_relax
 void post(const Launch &L, const KP &kp, const Fields &f, const double *xu, const double *xv, const int *active,

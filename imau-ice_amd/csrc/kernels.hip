@@ -760,6 +760,127 @@ __global__ void __launch_bounds__(IMAU_WG) k_post(const Member *__restrict__ mem
   block_sum_store<3>(acc, partial + 3 * (long long)blockIdx.x);
 }
 
+// Basal stress at the current velocities, tau_b = beta_eff u (:1087-1120),
+// without a linear solve: the viscosity passes that complete a warm start
+// (given velocities, cold viscosity and basal stress) before the Picard loop.
+// Reads beta_eff on cx / cy, which k_coef writes.
+__global__ void __launch_bounds__(IMAU_WG) k_taub(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                  const Fields f, const int *__restrict__ active) {
+  TILE_SETUP
+  if (j >= m.ny || !active[t.g]) return;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= m.nx) break;
+    const long long p = p2(m, j, i);
+    if (i < m.nx - 1) f.taub_cx[p] = f.beta_eff_cx[p] * f.u[p];
+    if (j < m.ny - 1) f.taub_cy[p] = f.beta_eff_cy[p] * f.v[p];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// warm-start network I/O: image g of the network is member g's a-grid with
+// rows i (x) and columns j (y), pixel off[g] + i ny + j
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned short f2bf_rne(float f) {
+  unsigned int u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+// input features, BF16 [pix][16]: H/1e3, Hs/1e3, Hb/1e3, 100 ds/dx, 100 ds/dy,
+// log10(beta^2) - 4, log10(A) + 16.5, ice mask, and asinh(u_est / 10 m/yr) for
+// both components of the local estimate u_est = -(u_b + u_d) grad s / |grad s|
+// with Weertman sliding u_b = (tau_d / beta^2)^m and the SIA deformation
+// velocity u_d = 2 A tau_d^n H / (n + 2) under tau_d = rho g H |grad s|
+__global__ void __launch_bounds__(IMAU_WG) k_net_features(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                          const KP kp, const Fields f,
+                                                          const long long *__restrict__ poff,
+                                                          unsigned short *__restrict__ x) {
+  TILE_SETUP
+  if (j >= m.ny) return;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= m.nx) break;
+    const long long p = p2(m, j, i);
+    const long long q = poff[t.g] + (long long)i * m.ny + j;
+    const double H = f.Hi[p], sx = f.dHs_dx_a[p], sy = f.dHs_dy_a[p];
+    const double gs = sqrt(sx * sx + sy * sy);
+    const double taud = D_ICE_DENSITY * D_GRAV * H * gs;
+    const double b2 = f.beta_sq ? f.beta_sq[p] : 0.0, A = f.A_vav[p];
+    double speed = 0.0;
+    if (H > 0.0 && gs > 0.0) {
+      if (b2 > 0.0) speed += pow(taud / b2, kp.wm);
+      if (A > 0.0) speed += 2.0 * A * pow(taud, kp.n_flow) * H / (kp.n_flow + 2.0);
+    }
+    speed = fmin(speed, kp.vel_max);
+    const double ex = gs > 0.0 ? -speed * sx / gs : 0.0, ey = gs > 0.0 ? -speed * sy / gs : 0.0;
+    float v[16];
+    v[0] = (float)(H * 1e-3);
+    v[1] = (float)(f.Hs[p] * 1e-3);
+    v[2] = (float)(f.Hb[p] * 1e-3);
+    v[3] = (float)(100.0 * sx);
+    v[4] = (float)(100.0 * sy);
+    v[5] = b2 > 0.0 ? (float)(log10(b2) - 4.0) : 0.0f;
+    v[6] = A > 0.0 ? (float)(log10(A) + 16.5) : 0.0f;
+    v[7] = H > 0.0 ? 1.0f : 0.0f;
+    v[8] = (float)asinh(ex / 10.0);
+    v[9] = (float)asinh(ey / 10.0);
+#pragma unroll
+    for (int k = 10; k < 16; ++k) v[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x[q * 16 + k] = f2bf_rne(v[k]);
+  }
+}
+
+// training targets: asinh(u_a / 10 m/yr), asinh(v_a / 10 m/yr) of the a-grid
+// velocities (mean of the two staggered neighbours, the one inside at the
+// domain edge), weight 1 on ice and 0 elsewhere
+__global__ void __launch_bounds__(IMAU_WG) k_net_targets(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                         const Fields f, const long long *__restrict__ poff,
+                                                         float *__restrict__ tg, float *__restrict__ wt) {
+  TILE_SETUP
+  if (j >= m.ny) return;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= m.nx) break;
+    const long long p = p2(m, j, i);
+    const long long q = poff[t.g] + (long long)i * m.ny + j;
+    const double ua = (i == 0) ? f.u[p2(m, j, 0)]
+                      : (i == m.nx - 1) ? f.u[p2(m, j, m.nx - 2)] : 0.5 * (f.u[p2(m, j, i - 1)] + f.u[p]);
+    const double va = (j == 0) ? f.v[p2(m, 0, i)]
+                      : (j == m.ny - 1) ? f.v[p2(m, m.ny - 2, i)] : 0.5 * (f.v[p2(m, j - 1, i)] + f.v[p]);
+    tg[2 * q] = (float)asinh(ua / 10.0);
+    tg[2 * q + 1] = (float)asinh(va / 10.0);
+    wt[q] = f.Hi[p] > 0.0 ? 1.0f : 0.0f;
+  }
+}
+
+// warm start from the network output o[pix][2] (asinh scale): a-grid
+// velocities 10 sinh(o), mapped to cx / cy by the mean of the two neighbours
+// and limited to vel_max
+__global__ void __launch_bounds__(IMAU_WG) k_net_warm(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                      const KP kp, const Fields f, const long long *__restrict__ poff,
+                                                      const float *__restrict__ o, const int *__restrict__ active) {
+  TILE_SETUP
+  if (j >= m.ny || !active[t.g]) return;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= m.nx) break;
+    const long long p = p2(m, j, i);
+    const long long q = poff[t.g] + (long long)i * m.ny + j;
+    const double vm = kp.vel_max;
+    if (i < m.nx - 1) {
+      const double u = 5.0 * (sinh((double)o[2 * q]) + sinh((double)o[2 * (q + m.ny)]));
+      f.u[p] = fmax(-vm, fmin(vm, u));
+    }
+    if (j < m.ny - 1) {
+      const double v = 5.0 * (sinh((double)o[2 * q + 1]) + sinh((double)o[2 * (q + 1) + 1]));
+      f.v[p] = fmax(-vm, fmin(vm, v));
+    }
+  }
+}
+
 // FP64 residual partial sums: sum (S + D x - b)^2 and sum b^2 over every row
 // of the system assembled at the current coefficients.
 __global__ void __launch_bounds__(IMAU_WG) k_resid(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
@@ -1236,6 +1357,19 @@ void beta(const Launch &L, const KP &kp, const Fields &f, bool diva, const doubl
 }
 void coef(const Launch &L, const KP &kp, const Fields &f, bool diva, const int *active) {
   hipLaunchKernelGGL(k_coef, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, diva ? 1 : 0, active);
+}
+void net_features(const Launch &L, const KP &kp, const Fields &f, const long long *poff, unsigned short *x) {
+  hipLaunchKernelGGL(k_net_features, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, poff, x);
+}
+void net_targets(const Launch &L, const Fields &f, const long long *poff, float *tg, float *wt) {
+  hipLaunchKernelGGL(k_net_targets, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, f, poff, tg, wt);
+}
+void net_warm(const Launch &L, const KP &kp, const Fields &f, const long long *poff, const float *o,
+              const int *active) {
+  hipLaunchKernelGGL(k_net_warm, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, poff, o, active);
+}
+void taub(const Launch &L, const Fields &f, const int *active) {
+  hipLaunchKernelGGL(k_taub, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, f, active);
 }
 void init_x(const Launch &L, const Fields &f, double *xu, double *xv) {
   hipLaunchKernelGGL(k_init_x, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, f, xu, xv);

@@ -22,7 +22,7 @@ EXPORTED = (
     "imau_vel_nonlinear_residual", "imau_vel_finalize", "imau_vel_last_error", "imau_vel_default_config",
     "imau_vel_problem_create", "imau_vel_problem_solve", "imau_vel_problem_reset_state",
     "imau_vel_problem_download", "imau_vel_problem_destroy", "imau_vel_solve_diva_batch",
-    "imau_vel_kernel_timing",
+    "imau_vel_kernel_timing", "imau_vel_problem_stats",
 )
 
 
@@ -63,6 +63,7 @@ def load_library(path: str = LIB_PATH):
     L.imau_vel_problem_destroy.restype = None
     L.imau_vel_solve_diva_batch.argtypes = [vp, ctypes.c_int32, P(ImauGrid), P(ImauCfg), P(ImauIceIO)]
     L.imau_vel_kernel_timing.argtypes = [vp, ctypes.c_int32, P(ctypes.c_int64), P(ctypes.c_double), P(ctypes.c_int64)]
+    L.imau_vel_problem_stats.argtypes = [vp, vp, P(ctypes.c_int32), P(ctypes.c_double)]
     _lib = L
     return L
 
@@ -155,6 +156,15 @@ class Problem:
         for s, one in zip(self.states, io):
             s.take_stats(one)
         return self.states
+
+    def stats(self):
+        """(Picard iterations, last FP64 nonlinear residual) per member of the
+        last solve, without downloading any field."""
+        n = len(self.grids)
+        its = (ctypes.c_int32 * n)()
+        res = (ctypes.c_double * n)()
+        self.ctx._check(self.ctx.L.imau_vel_problem_stats(self.ctx.h, self.h, its, res))
+        return list(its), list(res)
 
     def close(self):
         if self.h:

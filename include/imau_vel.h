@@ -156,7 +156,19 @@ typedef struct imau_cfg {
   int32_t anderson_m;      /* 0: relaxation (fixed or adaptive).  1..3: Anderson
                               acceleration of the Picard map with m stored
                               differences, restarted for a member whose FP64
-                              residual grows 1.5x (needs IMAU_STOP_RESIDUAL) */
+                              residual grows by anderson_restart (needs
+                              IMAU_STOP_RESIDUAL)                            */
+  int32_t warm_visc_iters; /* DIVA: before the Picard loop, this many passes of
+                              viscosity, beta_eff and tau_b = beta_eff u at the
+                              given velocities, without a linear solve (0 = none,
+                              the reference's start).  Completes a warm start
+                              that supplies velocities only.                  */
+  double  anderson_beta;   /* Anderson step u = sum_i w_i (U_i + beta F_i)
+                              (F = G(U) - U); 1 = undamped, in (0, 1]      */
+  double  anderson_restart; /* restart a member's Anderson history (and take
+                              one relaxation step) when its FP64 residual
+                              exceeds this factor times the previous one;
+                              >= 1                                           */
 } imau_cfg;
 
 /* The fields of type_ice_model (src/data_types_module.f90:56-325) that the
@@ -264,6 +276,10 @@ int  imau_vel_problem_create(imau_ctx *ctx, int32_t ngrids, const imau_grid *gri
 int  imau_vel_problem_solve(imau_ctx *ctx, imau_problem *prob, const imau_cfg *cfg);
 int  imau_vel_problem_reset_state(imau_ctx *ctx, imau_problem *prob);
 int  imau_vel_problem_download(imau_ctx *ctx, imau_problem *prob, imau_ice_io *ios);
+/* per member of the last solve: Picard iterations and the last FP64 relative
+ * nonlinear residual (NaN under IMAU_STOP_PICARD); either pointer may be NULL */
+int  imau_vel_problem_stats(imau_ctx *ctx, const imau_problem *prob, int32_t *picard_iterations,
+                            double *nonlinear_residual);
 void imau_vel_problem_destroy(imau_problem *prob);
 
 /* ---- ensembles (no reference counterpart: the reference solves one region

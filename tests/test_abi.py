@@ -36,6 +36,16 @@ def test_library_exports_every_declared_entry_point():
     assert declared == set(solver.EXPORTED)
 
 
+def test_library_exports_every_network_entry_point():
+    from imau_vel import learned
+    L = solver.load_library()
+    txt = open(os.path.join(ROOT, "include", "imau_net.h")).read()
+    declared = set(re.findall(r"\b(imau_net_\w+)\s*\(", txt))
+    assert declared == set(learned.NET_EXPORTED)
+    for name in sorted(declared):
+        assert hasattr(L, name), name
+
+
 def test_c_default_config_equals_python_defaults():
     L = solver.load_library()
     c = abi.ImauCfg()
