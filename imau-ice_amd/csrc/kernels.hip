@@ -828,8 +828,12 @@ __global__ void __launch_bounds__(IMAU_WG) k_net_features(const Member *__restri
     v[9] = (float)asinh(ey / 10.0);
 #pragma unroll
     for (int k = 10; k < 16; ++k) v[k] = 0.0f;
+    // four 8-byte stores per pixel (the 32 bytes of its channels)
+    typedef unsigned short us4 __attribute__((ext_vector_type(4)));
 #pragma unroll
-    for (int k = 0; k < 16; ++k) x[q * 16 + k] = f2bf_rne(v[k]);
+    for (int k = 0; k < 16; k += 4)
+      *reinterpret_cast<us4 *>(x + q * 16 + k) = us4{f2bf_rne(v[k]), f2bf_rne(v[k + 1]), f2bf_rne(v[k + 2]),
+                                                     f2bf_rne(v[k + 3])};
   }
 }
 

@@ -1,5 +1,6 @@
 // net.hip -- kernels of the warm-start network: 3x3 convolutions on the
-// matrix cores (v_mfma_f32_16x16x16_bf16: BF16 operands, FP32 accumulation),
+// matrix cores (v_mfma_f32_16x16x32_bf16 for 32 input channels,
+// v_mfma_f32_16x16x16_bf16 for 16: BF16 operands, FP32 accumulation),
 // their weight gradients, 2x2 pooling, upsampling with skip addition, the
 // backward passes of those, a weighted squared-error loss and Adam.
 //
@@ -88,6 +89,86 @@ __global__ void __launch_bounds__(256) k_conv(const NImg *__restrict__ img, cons
           for (int ct = 0; ct < NCT; ++ct)
             acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[tap][ch][ct], b, acc[r][ct], 0, 0, 0);
         }
+      }
+    }
+    const int c = t.c0 + lc;
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+      const int rr = t.r0 + r;
+      if (rr >= im.rows || c >= im.cols) continue;
+      const long long pix = im.off + (long long)rr * im.cols + c;
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct) {
+        float v[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          v[e] = acc[r][ct][e] + bs[ct][e];
+          if (relu) v[e] = fmaxf(v[e], 0.0f);
+        }
+        if constexpr (OUTF) {
+          if (ct == 0 && lk == 0) {
+            float *y = static_cast<float *>(yv);
+            y[2 * pix] = v[0];
+            y[2 * pix + 1] = v[1];
+          }
+        } else {
+          s4 o;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) o[e] = (short)f2bf(v[e]);
+          st4(static_cast<uint16_t *>(yv) + pix * COUT + ct * 16 + 4 * lk, o);
+        }
+      }
+    }
+  }
+}
+
+// The same convolution for 32 input channels on v_mfma_f32_16x16x32_bf16: one
+// instruction takes all 32 channels of a tap (lane l holds k = 8 (l / 16) + e,
+// e = 0..7, of its row / column), with 16-byte operand loads.
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ b8 ld8(const uint16_t *p) { return *reinterpret_cast<const b8 *>(p); }
+
+template <int COUT, bool OUTF>
+__global__ void __launch_bounds__(256) k_conv32(const NImg *__restrict__ img, const NTile *__restrict__ tiles,
+                                                int ntiles, const uint16_t *__restrict__ x,
+                                                const uint16_t *__restrict__ w, const float *__restrict__ bias,
+                                                int relu, void *__restrict__ yv) {
+  constexpr int CIN = 32, NCT = COUT / 16;
+  const int lane = threadIdx.x & 63, lc = lane & 15, lk = lane >> 4;
+  const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (wid >= ntiles) return;  // uniform per wavefront
+  b8 wa[9][NCT];
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) wa[tap][ct] = ld8(w + ((size_t)(tap * COUT + ct * 16 + lc) * CIN + 8 * lk));
+  float bs[NCT][4];
+#pragma unroll
+  for (int ct = 0; ct < NCT; ++ct)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) bs[ct][e] = bias ? bias[ct * 16 + 4 * lk + e] : 0.0f;
+
+  for (int tix = wid; tix < ntiles; tix += gridDim.x * 4) {
+    const NTile t = tiles[tix];
+    const NImg im = img[t.img];
+    f4 acc[TR][NCT];
+#pragma unroll
+    for (int r = 0; r < TR; ++r)
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct) acc[r][ct] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      const int dr = tap / 3 - 1, dc = tap % 3 - 1;
+      const int c = t.c0 + lc + dc;
+      const bool cok = c >= 0 && c < im.cols;
+#pragma unroll
+      for (int r = 0; r < TR; ++r) {
+        const int rr = t.r0 + r + dr;
+        b8 b = {};
+        if (cok && rr >= 0 && rr < im.rows) b = ld8(x + (im.off + (long long)rr * im.cols + c) * CIN + 8 * lk);
+#pragma unroll
+        for (int ct = 0; ct < NCT; ++ct)
+          acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[tap][ct], b, acc[r][ct], 0, 0, 0);
       }
     }
     const int c = t.c0 + lc;
@@ -225,113 +306,100 @@ __global__ void __launch_bounds__(256) k_sum_partials(int nblk, int n, const flo
 }
 
 // ---------------------------------------------------------------------------
-// pooling, upsampling and their backward passes (32 channels, one wavefront per tile)
+// pooling, upsampling and their backward passes (32 channels).  Two
+// workgroups per tile, eight lanes per pixel (four channels each), so a
+// wavefront reads and writes whole 64-byte channel vectors of 8 neighbouring
+// pixels.
 // ---------------------------------------------------------------------------
-#define NET_PIXEL(V, wid)                                   \
-  const int lane = threadIdx.x & 63;                        \
-  const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);      \
-  if (wid >= V.ntiles) return;                              \
-  const NTile t = V.tiles[wid];                             \
-  const int r = t.r0 + (lane >> 4), c = t.c0 + (lane & 15);
+#define NET_PIXEL8(V)                                                 \
+  const NTile t = V.tiles[blockIdx.x >> 1];                           \
+  const int px = (int)(threadIdx.x >> 3), k = 4 * (int)(threadIdx.x & 7); \
+  const int r = t.r0 + 2 * (int)(blockIdx.x & 1) + (px >> 4), c = t.c0 + (px & 15);
 
 __global__ void __launch_bounds__(256) k_pool(LevelView F, LevelView C, const uint16_t *__restrict__ x,
                                               uint16_t *__restrict__ y) {
-  NET_PIXEL(C, wid)
+  NET_PIXEL8(C)
   const NImg cm = C.img[t.img], fm = F.img[t.img];
   if (r >= cm.rows || c >= cm.cols) return;
   const long long q = cm.off + (long long)r * cm.cols + c;
+  float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-  for (int k = 0; k < 32; k += 4) {
-    float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      const int fr = 2 * r + a, fc = 2 * c + b;
+      if (fr < fm.rows && fc < fm.cols) {
+        const s4 v = ld4(x + (fm.off + (long long)fr * fm.cols + fc) * 32 + k);
 #pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        const int fr = 2 * r + a, fc = 2 * c + b;
-        if (fr < fm.rows && fc < fm.cols) {
-          const s4 v = ld4(x + (fm.off + (long long)fr * fm.cols + fc) * 32 + k);
-#pragma unroll
-          for (int e = 0; e < 4; ++e) s[e] += bf2f((uint16_t)v[e]);
-        }
+        for (int e = 0; e < 4; ++e) s[e] += bf2f((uint16_t)v[e]);
       }
-    s4 o;
+    }
+  s4 o;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) o[e] = (short)f2bf(0.25f * s[e]);
-    st4(y + q * 32 + k, o);
-  }
+  for (int e = 0; e < 4; ++e) o[e] = (short)f2bf(0.25f * s[e]);
+  st4(y + q * 32 + k, o);
 }
 
 __global__ void __launch_bounds__(256) k_upadd(LevelView F, LevelView C, const uint16_t *__restrict__ a,
                                                const uint16_t *__restrict__ b, uint16_t *__restrict__ y) {
-  NET_PIXEL(F, wid)
+  NET_PIXEL8(F)
   const NImg fm = F.img[t.img], cm = C.img[t.img];
   if (r >= fm.rows || c >= fm.cols) return;
   const long long p = fm.off + (long long)r * fm.cols + c;
   const long long q = cm.off + (long long)(r >> 1) * cm.cols + (c >> 1);
+  const s4 va = ld4(a + p * 32 + k), vb = ld4(b + q * 32 + k);
+  s4 o;
 #pragma unroll
-  for (int k = 0; k < 32; k += 4) {
-    const s4 va = ld4(a + p * 32 + k), vb = ld4(b + q * 32 + k);
-    s4 o;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) o[e] = (short)f2bf(bf2f((uint16_t)va[e]) + bf2f((uint16_t)vb[e]));
-    st4(y + p * 32 + k, o);
-  }
+  for (int e = 0; e < 4; ++e) o[e] = (short)f2bf(bf2f((uint16_t)va[e]) + bf2f((uint16_t)vb[e]));
+  st4(y + p * 32 + k, o);
 }
 
 __global__ void __launch_bounds__(256) k_gpool(LevelView F, LevelView C, const uint16_t *__restrict__ a,
                                                const uint16_t *__restrict__ pc, const uint16_t *__restrict__ yact,
                                                uint16_t *__restrict__ g) {
-  NET_PIXEL(F, wid)
+  NET_PIXEL8(F)
   const NImg fm = F.img[t.img];
   if (r >= fm.rows || c >= fm.cols) return;
   const long long p = fm.off + (long long)r * fm.cols + c;
-  long long q = 0;
+  const s4 ya = ld4(yact + p * 32 + k);
+  s4 va = s4{0, 0, 0, 0}, vc = s4{0, 0, 0, 0};
+  if (a) va = ld4(a + p * 32 + k);
   if (pc) {
     const NImg cm = C.img[t.img];
-    q = cm.off + (long long)(r >> 1) * cm.cols + (c >> 1);
+    vc = ld4(pc + (cm.off + (long long)(r >> 1) * cm.cols + (c >> 1)) * 32 + k);
   }
+  s4 o;
 #pragma unroll
-  for (int k = 0; k < 32; k += 4) {
-    const s4 ya = ld4(yact + p * 32 + k);
-    s4 va = s4{0, 0, 0, 0}, vc = s4{0, 0, 0, 0};
-    if (a) va = ld4(a + p * 32 + k);
-    if (pc) vc = ld4(pc + q * 32 + k);
-    s4 o;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const float s = bf2f((uint16_t)va[e]) + 0.25f * bf2f((uint16_t)vc[e]);
-      o[e] = (bf2f((uint16_t)ya[e]) > 0.0f) ? (short)f2bf(s) : (short)0;
-    }
-    st4(g + p * 32 + k, o);
+  for (int e = 0; e < 4; ++e) {
+    const float s = bf2f((uint16_t)va[e]) + 0.25f * bf2f((uint16_t)vc[e]);
+    o[e] = (bf2f((uint16_t)ya[e]) > 0.0f) ? (short)f2bf(s) : (short)0;
   }
+  st4(g + p * 32 + k, o);
 }
 
 __global__ void __launch_bounds__(256) k_gup(LevelView F, LevelView C, const uint16_t *__restrict__ a,
                                              const uint16_t *__restrict__ yact, uint16_t *__restrict__ g) {
-  NET_PIXEL(C, wid)
+  NET_PIXEL8(C)
   const NImg cm = C.img[t.img], fm = F.img[t.img];
   if (r >= cm.rows || c >= cm.cols) return;
   const long long q = cm.off + (long long)r * cm.cols + c;
+  float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-  for (int k = 0; k < 32; k += 4) {
-    float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int da = 0; da < 2; ++da)
 #pragma unroll
-    for (int da = 0; da < 2; ++da)
+    for (int db = 0; db < 2; ++db) {
+      const int fr = 2 * r + da, fc = 2 * c + db;
+      if (fr < fm.rows && fc < fm.cols) {
+        const s4 v = ld4(a + (fm.off + (long long)fr * fm.cols + fc) * 32 + k);
 #pragma unroll
-      for (int db = 0; db < 2; ++db) {
-        const int fr = 2 * r + da, fc = 2 * c + db;
-        if (fr < fm.rows && fc < fm.cols) {
-          const s4 v = ld4(a + (fm.off + (long long)fr * fm.cols + fc) * 32 + k);
-#pragma unroll
-          for (int e = 0; e < 4; ++e) s[e] += bf2f((uint16_t)v[e]);
-        }
+        for (int e = 0; e < 4; ++e) s[e] += bf2f((uint16_t)v[e]);
       }
-    const s4 ya = ld4(yact + q * 32 + k);
-    s4 o;
+    }
+  const s4 ya = ld4(yact + q * 32 + k);
+  s4 o;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) o[e] = (bf2f((uint16_t)ya[e]) > 0.0f) ? (short)f2bf(s[e]) : (short)0;
-    st4(g + q * 32 + k, o);
-  }
+  for (int e = 0; e < 4; ++e) o[e] = (bf2f((uint16_t)ya[e]) > 0.0f) ? (short)f2bf(s[e]) : (short)0;
+  st4(g + q * 32 + k, o);
 }
 
 // ---------------------------------------------------------------------------
@@ -408,11 +476,11 @@ bool conv(hipStream_t s, const LevelView &L, int cin, int cout, const uint16_t *
   if (cin == 16 && cout == 32 && !yf32)
     hipLaunchKernelGGL((k_conv<16, 32, false>), dim3(nb), dim3(256), 0, s, L.img, L.tiles, L.ntiles, x, w, bias, relu, y);
   else if (cin == 32 && cout == 32 && !yf32)
-    hipLaunchKernelGGL((k_conv<32, 32, false>), dim3(nb), dim3(256), 0, s, L.img, L.tiles, L.ntiles, x, w, bias, relu, y);
+    hipLaunchKernelGGL((k_conv32<32, false>), dim3(nb), dim3(256), 0, s, L.img, L.tiles, L.ntiles, x, w, bias, relu, y);
   else if (cin == 32 && cout == 16 && yf32)
-    hipLaunchKernelGGL((k_conv<32, 16, true>), dim3(nb), dim3(256), 0, s, L.img, L.tiles, L.ntiles, x, w, bias, relu, y);
+    hipLaunchKernelGGL((k_conv32<16, true>), dim3(nb), dim3(256), 0, s, L.img, L.tiles, L.ntiles, x, w, bias, relu, y);
   else if (cin == 32 && cout == 16 && !yf32)
-    hipLaunchKernelGGL((k_conv<32, 16, false>), dim3(nb), dim3(256), 0, s, L.img, L.tiles, L.ntiles, x, w, bias, relu, y);
+    hipLaunchKernelGGL((k_conv32<16, false>), dim3(nb), dim3(256), 0, s, L.img, L.tiles, L.ntiles, x, w, bias, relu, y);
   else
     return false;
   return true;
@@ -441,18 +509,19 @@ void sum_partials(hipStream_t s, int nblk, int n, const float *part, float *dw, 
   hipLaunchKernelGGL(k_sum_partials, dim3((n + 255) / 256), dim3(256), 0, s, nblk, n, part, dw, nw, db);
 }
 
+// two workgroups of 256 threads (32 pixels x 8 lanes) per 64-pixel tile
 void pool(hipStream_t s, const LevelView &F, const LevelView &C, const uint16_t *x, uint16_t *y) {
-  if (C.ntiles) hipLaunchKernelGGL(k_pool, dim3(nblk4(C.ntiles)), dim3(256), 0, s, F, C, x, y);
+  if (C.ntiles) hipLaunchKernelGGL(k_pool, dim3(2 * C.ntiles), dim3(256), 0, s, F, C, x, y);
 }
 void upadd(hipStream_t s, const LevelView &F, const LevelView &C, const uint16_t *a, const uint16_t *b, uint16_t *y) {
-  if (F.ntiles) hipLaunchKernelGGL(k_upadd, dim3(nblk4(F.ntiles)), dim3(256), 0, s, F, C, a, b, y);
+  if (F.ntiles) hipLaunchKernelGGL(k_upadd, dim3(2 * F.ntiles), dim3(256), 0, s, F, C, a, b, y);
 }
 void gpool(hipStream_t s, const LevelView &F, const LevelView &C, const uint16_t *a, const uint16_t *pc,
            const uint16_t *yact, uint16_t *g) {
-  if (F.ntiles) hipLaunchKernelGGL(k_gpool, dim3(nblk4(F.ntiles)), dim3(256), 0, s, F, C, a, pc, yact, g);
+  if (F.ntiles) hipLaunchKernelGGL(k_gpool, dim3(2 * F.ntiles), dim3(256), 0, s, F, C, a, pc, yact, g);
 }
 void gup(hipStream_t s, const LevelView &F, const LevelView &C, const uint16_t *a, const uint16_t *yact, uint16_t *g) {
-  if (C.ntiles) hipLaunchKernelGGL(k_gup, dim3(nblk4(C.ntiles)), dim3(256), 0, s, F, C, a, yact, g);
+  if (C.ntiles) hipLaunchKernelGGL(k_gup, dim3(2 * C.ntiles), dim3(256), 0, s, F, C, a, yact, g);
 }
 int loss_blocks(const LevelView &L) { return std::max(1, nblk4(L.ntiles)); }
 void loss(hipStream_t s, const LevelView &L, const float *out, const float *t, const float *w, float scale,
