@@ -64,12 +64,12 @@ def VISC_BYTES_PER_CELL(nz, word):
 
 
 # HBM bytes per processed cell measured with rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE on the ensemble workload with the default (mixed) method
-# (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0: the factors a 1 GiB device
-# copy reads; profiles/ensemble_mixed_kernel_stats.md).  Re-measure after
-# changing either kernel.
-VISC_TRAFFIC_BYTES_PER_CELL = {"mixed_visc": 442.4}
-SWEEP_TRAFFIC_BYTES_PER_CELL = {"mixed_visc": 153.5}
+# WRITE_SIZE on the ensemble workload with the default method (FP64 viscosity,
+# FP32 multigrid; FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0: the factors a
+# 1 GiB device copy reads; profiles/ensemble_learned_kernel_stats.md).
+# Re-measure after changing either kernel.
+VISC_TRAFFIC_BYTES_PER_CELL = {"mixed": 620.5}
+SWEEP_TRAFFIC_BYTES_PER_CELL = {"mixed": 165.2}
 METHODS = {
     "learned": dict(precision="mixed", mg_levels=0, picard_relax_max=0.0, ksp_type=1, anderson_m=3, learned=True,
                     text="learned warm start (BF16 conv net on the matrix cores, viscosity passes), then GCR in FP64 "
@@ -176,6 +176,7 @@ def main():
     ap.add_argument("--warm-iters", type=int, default=6,
                     help="viscosity passes completing the learned warm start (warm_visc_iters)")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison legs")
+    ap.add_argument("--no-cold", action="store_true", help="skip the cold-start leg of the learned method")
     ap.add_argument("--relax-max", type=float, default=None, help="override the headline's picard_relax_max")
     ap.add_argument("--ksp", choices=("bicgstab", "gcr"), default=None, help="override the headline's Krylov method")
     ap.add_argument("--mg-rebuild", type=int, default=None, help="override the headline's mg_rebuild")
@@ -289,6 +290,7 @@ def main():
                                     warm_visc_iters=args.warm_iters, fallbacks=ws.fallbacks,
                                     fallback="cold solve of the whole ensemble when any member's FP64 residual "
                                              "stays above 1e-4")
+    if ws is not None and not args.no_cold:
         # the same solver from the cold start
         dtc = timed_solves(prob, cfg, args.steps, 0)
         itc, rc = prob.stats()
