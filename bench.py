@@ -64,22 +64,22 @@ def VISC_BYTES_PER_CELL(nz, word):
 
 
 # HBM bytes per processed cell measured with rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE on the ensemble workload with the default method (FP64 viscosity,
-# FP32 multigrid; FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0: the factors a
-# 1 GiB device copy reads; profiles/ensemble_learned_kernel_stats.md).
-# Re-measure after changing either kernel.
+# WRITE_SIZE on the ensemble workload, keyed by the viscosity precision of the
+# method (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0: the factors a 1 GiB
+# device copy reads; profiles/ensemble_learned_kernel_stats.md).  Re-measure
+# after changing either kernel.
 VISC_TRAFFIC_BYTES_PER_CELL = {"mixed": 620.5}
 SWEEP_TRAFFIC_BYTES_PER_CELL = {"mixed": 165.2}
 METHODS = {
-    "learned": dict(precision="mixed", mg_levels=0, picard_relax_max=0.0, ksp_type=1, anderson_m=3, learned=True,
+    "learned": dict(precision="mixed_visc", mg_levels=0, picard_relax_max=0.0, ksp_type=1, anderson_m=3, learned=True,
                     text="learned warm start (BF16 conv net on the matrix cores, viscosity passes), then GCR in FP64 "
-                         "right-preconditioned by an FP32 aggregation-multigrid V-cycle; FP64 viscosity and beta; "
-                         "Anderson-accelerated Picard loop (3 differences); cold solve as fallback"),
-    "mixed": dict(precision="mixed", mg_levels=0, picard_relax_max=0.0, ksp_type=1, anderson_m=3,
+                         "(operator, residuals, update) right-preconditioned by an FP32 aggregation-multigrid V-cycle; "
+                         "FP32 3D viscosity; Anderson-accelerated Picard loop (3 differences); cold solve as fallback"),
+    "mixed": dict(precision="mixed_visc", mg_levels=0, picard_relax_max=0.0, ksp_type=1, anderson_m=3,
                   text="GCR in FP64 (operator, residuals, update) right-preconditioned by an FP32 aggregation-"
-                       "multigrid V-cycle; FP64 viscosity and beta; Anderson-accelerated Picard loop (3 differences)"),
-    "mixed_visc": dict(precision="mixed_visc", mg_levels=0, picard_relax_max=0.0, ksp_type=1, anderson_m=3,
-                       text="as mixed, with the 3D viscosity in FP32 (experimental: diverges on some geometries)"),
+                       "multigrid V-cycle; FP32 3D viscosity; Anderson-accelerated Picard loop (3 differences)"),
+    "mixed64visc": dict(precision="mixed", mg_levels=0, picard_relax_max=0.0, ksp_type=1, anderson_m=3,
+                        text="as mixed, with the viscosity in FP64"),
     "adaptive": dict(precision="mixed", mg_levels=0, picard_relax_max=1.0, ksp_type=1, anderson_m=0,
                      text="GCR in FP64 right-preconditioned by an FP32 aggregation-multigrid V-cycle; Picard loop "
                           "with adaptive relaxation (0.4 raised x1.25 per residual decrease up to 1.0)"),
@@ -240,7 +240,7 @@ def main():
                     traffic=traffic, kernel=kernel, avg_launch_us=avg_us, launches=n, cells_processed=cells_t,
                     bytes_per_launch=per_launch, bytes_per_cell=bytes_per_cell)
 
-    measured = (args.workload == "ensemble" and args.method in ("learned", "mixed") and args.ksp is None and args.relax_max is None
+    measured = (args.workload == "ensemble" and args.method in ("learned", "mixed", "mixed64visc") and args.ksp is None and args.relax_max is None
                 and args.lin_rtol == 0.3 and args.mg_rebuild is None)
     visc_traffic = VISC_TRAFFIC_BYTES_PER_CELL.get(meth["precision"]) if measured else None
     sweep_traffic = SWEEP_TRAFFIC_BYTES_PER_CELL.get(meth["precision"]) if measured else None

@@ -60,6 +60,11 @@ struct DBuf {
     if (count) {
       HIPCHK(hipMalloc((void **)&p, count * sizeof(T)));
       HIPCHK(hipMemset(p, 0, count * sizeof(T)));
+      // the zeroing runs on the null stream, which does not order with the
+      // context's non-blocking stream: wait for it, so that kernels enqueued
+      // next (buffers allocated on first use inside a solve) are not
+      // overwritten by it
+      HIPCHK(hipDeviceSynchronize());
     }
   }
   void release() {

@@ -31,11 +31,11 @@ from imau_vel import config, geometry, learned, solver  # noqa: E402
 
 
 def solver_config(lin_rtol=0.3):
-    """The headline solver: FP32 multigrid V-cycle inside FP64 GCR, FP64
+    """The headline solver: FP32 multigrid V-cycle inside FP64 GCR, FP32 3D
     viscosity, Anderson-accelerated Picard loop, stop at FP64 residual 1e-4."""
     cfg = geometry.synthetic_config()
     config.set_config(cfg, stop_rule=1, nonlinear_rtol=1e-4, DIVA_visc_it_nit=300, DIVA_choice_matrix_solver="PETSc",
-                      check_interval=8, precision="mixed", mg_levels=0, ksp_type="gcr", anderson_m=3)
+                      check_interval=8, precision="mixed_visc", mg_levels=0, ksp_type="gcr", anderson_m=3)
     return cfg
 
 
