@@ -68,8 +68,8 @@ def VISC_BYTES_PER_CELL(nz, word):
 # method (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0: the factors a 1 GiB
 # device copy reads; profiles/ensemble_learned_kernel_stats.md).  Re-measure
 # after changing either kernel.
-VISC_TRAFFIC_BYTES_PER_CELL = {"mixed": 620.5}
-SWEEP_TRAFFIC_BYTES_PER_CELL = {"mixed": 165.2}
+VISC_TRAFFIC_BYTES_PER_CELL = {"mixed_visc": 411.2, "mixed": 620.5}
+SWEEP_TRAFFIC_BYTES_PER_CELL = {"mixed_visc": 165.2, "mixed": 165.2}
 METHODS = {
     "learned": dict(precision="mixed_visc", mg_levels=0, picard_relax_max=0.0, ksp_type=1, anderson_m=3, learned=True,
                     text="learned warm start (BF16 conv net on the matrix cores, viscosity passes), then GCR in FP64 "
