@@ -173,7 +173,7 @@ def main():
     ap.add_argument("--workload", choices=("ensemble", "channel"), default="ensemble")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--method", choices=tuple(METHODS), default="learned")
-    ap.add_argument("--warm-iters", type=int, default=6,
+    ap.add_argument("--warm-iters", type=int, default=8,
                     help="viscosity passes completing the learned warm start (warm_visc_iters)")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison legs")
     ap.add_argument("--no-cold", action="store_true", help="skip the cold-start leg of the learned method")

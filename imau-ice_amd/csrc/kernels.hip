@@ -931,6 +931,21 @@ struct X0Acc {  // the start vector seen through the state: zero rows read 0
   }
 };
 
+// the state seen through the row, noting whether it read a zero row's value
+// (where the start vector x0 differs from the state)
+struct ZAcc {
+  const double *x;
+  const uint8_t *rt;
+  Member m;
+  bool *hit;
+  __device__ __forceinline__ double operator()(int jj, int ii) const {
+    const long long p = p2(m, jj, ii);
+    const uint8_t t = rt[p];
+    if (t == RT_ZERO || t == RT_BND_ZERO) *hit = true;
+    return x[p];
+  }
+};
+
 __global__ void __launch_bounds__(IMAU_WG) k_resid_init(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
                                                         const KP kp, const CoefP c, const double *__restrict__ xu,
                                                         const double *__restrict__ xv, Vec2 x0, Vec2 r,
@@ -938,7 +953,8 @@ __global__ void __launch_bounds__(IMAU_WG) k_resid_init(const Member *__restrict
   TILE_SETUP
   double acc[3] = {0.0, 0.0, 0.0};
   const int nx = m.nx, ny = m.ny;
-  const GAcc U{xu, m}, V{xv, m};
+  bool hit = false;
+  const ZAcc U{xu, c.rt_u, m, &hit}, V{xv, c.rt_v, m, &hit};
   const X0Acc U0{xu, c.rt_u, m}, V0{xv, c.rt_v, m};
   if (j < ny && active[t.g]) {
     FOR_TILE_ROWS {
@@ -948,12 +964,14 @@ __global__ void __launch_bounds__(IMAU_WG) k_resid_init(const Member *__restrict
       double S, D, b;
       if (i < nx - 1) {
         const uint8_t rt = c.rt_u[p];
+        hit = false;
         u_row(m, kp, c, j, i, rt, U, V, S, D, b);
         const double rn = S + D * xu[p] - b;
         acc[0] += rn * rn;
         acc[1] += b * b;
         const double x = (rt == RT_ZERO || rt == RT_BND_ZERO) ? 0.0 : xu[p];
-        u_row(m, kp, c, j, i, rt, U0, V0, S, D, b);
+        // the row at x0 equals the row at the state unless it reads a zero row
+        if (hit) u_row(m, kp, c, j, i, rt, U0, V0, S, D, b);
         const double rl = b - (S + D * x);
         x0.u[p] = x;
         r.u[p] = rl;
@@ -961,12 +979,13 @@ __global__ void __launch_bounds__(IMAU_WG) k_resid_init(const Member *__restrict
       }
       if (j < ny - 1) {
         const uint8_t rt = c.rt_v[p];
+        hit = false;
         v_row(m, kp, c, j, i, rt, U, V, S, D, b);
         const double rn = S + D * xv[p] - b;
         acc[0] += rn * rn;
         acc[1] += b * b;
         const double x = (rt == RT_ZERO || rt == RT_BND_ZERO) ? 0.0 : xv[p];
-        v_row(m, kp, c, j, i, rt, U0, V0, S, D, b);
+        if (hit) v_row(m, kp, c, j, i, rt, U0, V0, S, D, b);
         const double rl = b - (S + D * x);
         x0.v[p] = x;
         r.v[p] = rl;

@@ -353,7 +353,7 @@ class WarmStartSolver:
     state (the warm start replaces the velocities, the completion passes the
     viscosity and basal stress)."""
 
-    def __init__(self, ctx, problem, net=None, warm_iters=6):
+    def __init__(self, ctx, problem, net=None, warm_iters=8):
         self.net = net if net is not None else Net.load(ctx)
         self.problem = problem
         self.io = ProblemIO(self.net, problem)
