@@ -6,7 +6,8 @@ ice caps (Matern bed, parabolic surface with fractal noise), A and the beta_sq
 scale log-uniform over one decade, Weertman m = 3.  All members are
 device-resident in one batched problem; one step is one complete batched solve
 from the uploaded cold state: the learned warm start (network features,
-forward pass, velocities onto the staggered grids, viscosity passes), then the
+forward pass, velocities onto the staggered grids, column-local completion of
+viscosity and basal stress), then the
 Picard loop until every member has ||A(u)u - b(u)|| / ||b(u)|| <= 1e-4 in
 FP64.  value = 64 x steps / seconds.  "--method mixed" is the same solver from
 the cold start.
@@ -173,13 +174,16 @@ def main():
     ap.add_argument("--workload", choices=("ensemble", "channel"), default="ensemble")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--method", choices=tuple(METHODS), default="learned")
-    ap.add_argument("--warm-iters", type=int, default=8,
+    ap.add_argument("--warm-iters", type=int, default=0,
                     help="viscosity passes completing the learned warm start (warm_visc_iters)")
+    ap.add_argument("--warm-column", type=int, default=10,
+                    help="column-local completion iterations before those passes (warm_column_iters)")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison legs")
     ap.add_argument("--no-cold", action="store_true", help="skip the cold-start leg of the learned method")
     ap.add_argument("--relax-max", type=float, default=None, help="override the headline's picard_relax_max")
     ap.add_argument("--ksp", choices=("bicgstab", "gcr"), default=None, help="override the headline's Krylov method")
     ap.add_argument("--mg-rebuild", type=int, default=None, help="override the headline's mg_rebuild")
+    ap.add_argument("--anderson-m", type=int, default=None, help="override the headline's Anderson depth")
     ap.add_argument("--lin-rtol", type=float, default=0.3,
                     help="relative tolerance of each linear solve (inexact Picard; DIVA_PETSc_rtol)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -205,11 +209,14 @@ def main():
     nsolve = len(members)
     ctx = solver.Context()
     prob = ctx.problem(grids, cfg, states)
-    ws = learned.WarmStartSolver(ctx, prob, warm_iters=args.warm_iters) if meth.get("learned") else None
+    ws = (learned.WarmStartSolver(ctx, prob, warm_iters=args.warm_iters, column_iters=args.warm_column)
+          if meth.get("learned") else None)
 
     # headline method
     if args.mg_rebuild:
         meth["mg_rebuild"] = args.mg_rebuild
+    if args.anderson_m is not None:
+        meth["anderson_m"] = args.anderson_m
     if args.ksp:
         meth["ksp_type"] = 1 if args.ksp == "gcr" else 0
         meth["text"] = meth["text"].replace("GCR", "BiCGStab") if args.ksp == "bicgstab" else meth["text"]
@@ -241,7 +248,7 @@ def main():
                     bytes_per_launch=per_launch, bytes_per_cell=bytes_per_cell)
 
     measured = (args.workload == "ensemble" and args.method in ("learned", "mixed", "mixed64visc") and args.ksp is None and args.relax_max is None
-                and args.lin_rtol == 0.3 and args.mg_rebuild is None)
+                and args.lin_rtol == 0.3 and args.mg_rebuild is None and args.anderson_m is None)
     visc_traffic = VISC_TRAFFIC_BYTES_PER_CELL.get(meth["precision"]) if measured else None
     sweep_traffic = SWEEP_TRAFFIC_BYTES_PER_CELL.get(meth["precision"]) if measured else None
     roof_visc = roof("visc", "k_strain + k_visc (strain rates, 3D viscosity, N, F2: once per Picard iteration)",
@@ -287,7 +294,8 @@ def main():
                                              "pool, upsample), one launch sequence over the ensemble",
                                       ms_per_forward=1e3 * t_fwd, flop_per_forward=flops, pixels=ws.io.plan.npix[0])
         result["warm_start"] = dict(network=os.path.relpath(learned.WEIGHTS, os.path.dirname(__file__)),
-                                    warm_visc_iters=args.warm_iters, fallbacks=ws.fallbacks,
+                                    warm_visc_iters=args.warm_iters, warm_column_iters=args.warm_column,
+                                    fallbacks=ws.fallbacks,
                                     fallback="cold solve of the whole ensemble when any member's FP64 residual "
                                              "stays above 1e-4")
     if ws is not None and not args.no_cold:

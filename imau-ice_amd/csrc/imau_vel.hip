@@ -338,6 +338,8 @@ void check_cfg(const imau_cfg *cfg) {
     fail(IMAU_ERR_CONFIG, "anderson_beta must lie in (0, 1], got %g", cfg->anderson_beta);
   if (!(cfg->anderson_restart >= 1.0))
     fail(IMAU_ERR_CONFIG, "anderson_restart must be >= 1, got %g", cfg->anderson_restart);
+  if (cfg->warm_column_iters < 0 || cfg->warm_column_iters > 50)
+    fail(IMAU_ERR_CONFIG, "warm_column_iters must lie in [0, 50], got %d", cfg->warm_column_iters);
   if (cfg->warm_visc_iters < 0 || cfg->warm_visc_iters > 50)
     fail(IMAU_ERR_CONFIG, "warm_visc_iters must lie in [0, 50], got %d", cfg->warm_visc_iters);
   if (cfg->anderson_m < 0 || cfg->anderson_m > IMAU_AND_SLOTS - 1)
@@ -594,6 +596,7 @@ void imau_vel_default_config(imau_cfg *c) {
   c->warm_visc_iters = 0;
   c->anderson_beta = 1.0;
   c->anderson_restart = 1.5;
+  c->warm_column_iters = 0;
 }
 
 int imau_vel_init(const imau_grid *grid, const imau_cfg *cfg, imau_ctx **out) {
@@ -1257,8 +1260,18 @@ struct Solver {
       else std::swap(P->visc_cur, P->visc_oth);
     };
     // warm start given as velocities only: bring viscosity and basal stress to them
-    if (diva && cfg->warm_visc_iters > 0) {
+    if (diva && (cfg->warm_visc_iters > 0 || cfg->warm_column_iters > 0)) {
       push_active(P->act_pic.p, act_pic);
+      if (cfg->warm_column_iters > 0) {
+        if (v32)
+          imau::visc_column32(L, LV(), kp, P->F, cur32, P->visc32.p + 2 * n3, P->xg.p, P->yg.p,
+                              cfg->warm_column_iters, P->act_pic.p);
+        else
+          imau::visc_column(L, LV(), kp, P->F, P->visc_cur, P->xg.p, P->yg.p, cfg->warm_column_iters,
+                            P->act_pic.p);
+        imau::coef(L, kp, P->F, diva, P->act_pic.p);  // beta_eff on cx / cy
+        imau::taub(L, P->F, P->act_pic.p);
+      }
       for (int w = 0; w < cfg->warm_visc_iters; ++w) {
         visc_beta(false);
         imau::coef(L, kp, P->F, diva, P->act_pic.p);  // beta_eff on cx / cy

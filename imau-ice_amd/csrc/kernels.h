@@ -37,6 +37,11 @@ void net_features(const Launch &L, const KP &kp, const Fields &f, const long lon
 void net_targets(const Launch &L, const Fields &f, const long long *poff, float *tg, float *wt);
 void net_warm(const Launch &L, const KP &kp, const Fields &f, const long long *poff, const float *o,
               const int *active);
+// column-local completion of a velocity-only warm start (k_strain + k_visc_column)
+void visc_column(const Launch &L, const Launch &LV, const KP &kp, const Fields &f, double *visc, const double *xg,
+                 const double *yg, int iters, const int *active);
+void visc_column32(const Launch &L, const Launch &LV, const KP &kp, const Fields &f, float *visc, const float *B3,
+                   const double *xg, const double *yg, int iters, const int *active);
 // tau_b = beta_eff u on cx / cy (warm-start completion)
 void taub(const Launch &L, const Fields &f, const int *active);
 void init_x(const Launch &L, const Fields &f, double *xu, double *xv);

@@ -541,6 +541,90 @@ __global__ void __launch_bounds__(IMAU_WG) k_beta(const Member *__restrict__ mem
 }
 
 // ---------------------------------------------------------------------------
+// Column-local completion of a velocity-only warm start (DIVA).  Per a-grid
+// column, at the given velocities: viscosity with the vertical shear of the
+// column's own basal stress (A4, A5), F2 (A7), the sliding law and beta_eff
+// (A6, A8) and |tau_b| = beta_eff |u_a| (A9), iterated `iters` times in
+// registers from the shear-free viscosity.  The Picard loop's staggered forms
+// (tau_b and viscosity averaged onto cx / cy, :664-721) are replaced by the
+// column's own values; its first iteration restores them.  Writes the
+// viscosity (in place), visc_int, N_a, F2, beta_a and beta_eff_a.
+// ---------------------------------------------------------------------------
+template <bool THIRD, class T>
+__global__ void __launch_bounds__(IMAU_WG) k_visc_column(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                         const KP kp, const Fields f, T *__restrict__ visc,
+                                                         const T *__restrict__ B3, const double *__restrict__ xg,
+                                                         const double *__restrict__ yg, int iters,
+                                                         const int *__restrict__ active) {
+  TILE_SETUP
+  if (j >= m.ny || !active[t.g]) return;
+  const int nx = m.nx, ny = m.ny, nz = kp.nz;
+  FOR_VTILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= nx) break;
+    const long long p = p2(m, j, i);
+    const double Hi = f.Hi[p];
+    const int mi = f.m_ice[p];
+    const double ua = (i == 0) ? f.u[p2(m, j, 0)]
+                      : (i == nx - 1 ? f.u[p2(m, j, nx - 2)] : 0.5 * (f.u[p2(m, j, i - 1)] + f.u[p]));
+    const double va = (j == 0) ? f.v[p2(m, 0, i)]
+                      : (j == ny - 1 ? f.v[p2(m, ny - 2, i)] : 0.5 * (f.v[p2(m, j - 1, i)] + f.v[p]));
+    double tauc = f.tauc[p];
+    if (kp.slid == 3 || (kp.slid == 2 && !(kp.rough == 1 && kp.prough == 1)))
+      tauc = tan((D_PI / 180.0) * f.phi_fric[p]) * f.Neff[p];
+    double beta = sliding_beta(kp, ua, va, f.beta_sq[p], f.alpha_sq[p], f.phi_fric[p], tauc, f.Neff[p],
+                               xg[m.offx + i], yg[m.offy + j], f.A_vav ? f.A_vav[p] : 0.0);
+    beta = fmin(beta, kp.beta_max);
+    if (!kp.glsub && f.m_ocean[p] == 1) beta = 0.0;
+    bool frozen = false;
+    if (kp.freeze) {
+      const long long qb = p3(m, nz - 1, j, i);
+      frozen = f.Tpmp3[qb] - f.Ti3[qb] > kp.dT_freeze;
+    }
+    const double speed = sqrt(ua * ua + va * va);
+    const bool bc_skip = (kp.margin == 0 && mi == 0);
+    const T expo = (T)((1.0 - kp.n_flow) / (2.0 * kp.n_flow));
+    const T HiT = (T)Hi, eps_h = (T)f.eps_h[p], vmin = (T)kp.visc_min;
+    T vk[IMAU_NZMAX];
+    double F2 = 0.0, be = 0.0, vint_out = 0.0;
+    T tb = (T)0.0;
+    for (int it = 0; it < iters; ++it) {
+      T vint = (T)0.0, F2int = (T)0.0, eta_prev = (T)0.0, g_prev = (T)0.0;
+      for (int k = 0; k < nz; ++k) {
+        const T z = (T)kp.zeta[k];
+        if (bc_skip) { vk[k] = (T)0.0; continue; }
+        const T du = (it == 0) ? (T)0.0 : tb * z / vk[k];
+        const T eps_sq = eps_h + (T)0.25 * du * du;
+        const T B = B3[p3(m, k, j, i)];
+        T eta = THIRD ? B / cbrt(eps_sq) : B * pow(eps_sq, expo);
+        eta = fmax(eta, vmin);
+        const T gk = (HiT / eta) * (z * z);
+        if (k > 0) {
+          const T dz = z - (T)kp.zeta[k - 1];
+          vint += (T)0.5 * (eta + eta_prev) * dz;
+          F2int += (T)0.5 * (gk + g_prev) * dz;
+        }
+        eta_prev = eta;
+        g_prev = gk;
+        vk[k] = eta;
+      }
+      vint_out = (double)vint;
+      F2 = (mi == 1) ? fmax(kp.F_int_min, (double)F2int) : ((kp.margin == 0) ? 0.0 : kp.F_int_min);
+      be = (kp.slid == 0 || frozen) ? 1.0 / F2 : beta / (1.0 + beta * F2);
+      tb = (T)(be * speed);
+    }
+    for (int k = 0; k < nz; ++k) visc[p3(m, k, j, i)] = vk[k];
+    if (!bc_skip) {
+      f.visc_int[p] = vint_out;
+      f.N_a[p] = vint_out * ((kp.margin == 0) ? Hi : fmax(0.1, Hi));
+    }
+    f.F2[p] = F2;
+    f.beta_a[p] = beta;
+    f.beta_eff_a[p] = be;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Picard step 2: staggered coefficients and right-hand sides
 // N_b (:820-874), beta_eff on cx/cy with f_grnd^2 (:1063-1076, SSA :195-206),
 // N_cx/N_cy for the sans-cross-terms form (:876-880), rhs of every row.
@@ -1390,6 +1474,18 @@ void net_targets(const Launch &L, const Fields &f, const long long *poff, float 
 void net_warm(const Launch &L, const KP &kp, const Fields &f, const long long *poff, const float *o,
               const int *active) {
   hipLaunchKernelGGL(k_net_warm, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, poff, o, active);
+}
+void visc_column(const Launch &L, const Launch &LV, const KP &kp, const Fields &f, double *visc, const double *xg,
+                 const double *yg, int iters, const int *active) {
+  hipLaunchKernelGGL(k_strain, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, active);
+  auto kv = kp.n_flow == 3.0 ? k_visc_column<true, double> : k_visc_column<false, double>;
+  hipLaunchKernelGGL(kv, dim3(LV.ntiles), wg(), 0, L.s, L.mem, LV.tiles, kp, f, visc, f.B3, xg, yg, iters, active);
+}
+void visc_column32(const Launch &L, const Launch &LV, const KP &kp, const Fields &f, float *visc, const float *B3,
+                   const double *xg, const double *yg, int iters, const int *active) {
+  hipLaunchKernelGGL(k_strain, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, active);
+  auto kv = kp.n_flow == 3.0 ? k_visc_column<true, float> : k_visc_column<false, float>;
+  hipLaunchKernelGGL(kv, dim3(LV.ntiles), wg(), 0, L.s, L.mem, LV.tiles, kp, f, visc, B3, xg, yg, iters, active);
 }
 void taub(const Launch &L, const Fields &f, const int *active) {
   hipLaunchKernelGGL(k_taub, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, f, active);

@@ -95,6 +95,7 @@ def default_config(**overrides) -> ImauCfg:
     c.warm_visc_iters = 0
     c.anderson_beta = 1.0
     c.anderson_restart = 1.5
+    c.warm_column_iters = 0
     set_config(c, **overrides)
     return c
 

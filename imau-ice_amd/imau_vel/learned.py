@@ -353,24 +353,25 @@ class WarmStartSolver:
     state (the warm start replaces the velocities, the completion passes the
     viscosity and basal stress)."""
 
-    def __init__(self, ctx, problem, net=None, warm_iters=8):
+    def __init__(self, ctx, problem, net=None, warm_iters=0, column_iters=10):
         self.net = net if net is not None else Net.load(ctx)
         self.problem = problem
         self.io = ProblemIO(self.net, problem)
         self.warm_iters = warm_iters
+        self.column_iters = column_iters
         self.fallbacks = 0
 
     def warm_start(self, cfg):
         self.io.warm_start(cfg)
 
     def solve(self, cfg):
-        saved = cfg.warm_visc_iters
-        cfg.warm_visc_iters = self.warm_iters
+        saved = cfg.warm_visc_iters, cfg.warm_column_iters
+        cfg.warm_visc_iters, cfg.warm_column_iters = self.warm_iters, self.column_iters
         try:
             self.io.warm_start(cfg)
             self.problem.solve(cfg)
         finally:
-            cfg.warm_visc_iters = saved
+            cfg.warm_visc_iters, cfg.warm_column_iters = saved
         its, res = self.problem.stats()
         if cfg.stop_rule == 1 and any(not r <= cfg.nonlinear_rtol for r in res):
             self.fallbacks += 1

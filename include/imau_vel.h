@@ -169,6 +169,12 @@ typedef struct imau_cfg {
                               one relaxation step) when its FP64 residual
                               exceeds this factor times the previous one;
                               >= 1                                           */
+  int32_t warm_column_iters; /* DIVA: before the Picard loop (and before the
+                              warm_visc_iters passes), complete a
+                              velocity-only warm start column by column:
+                              viscosity, F2, beta_eff and |tau_b| iterated
+                              this many times in one kernel with each
+                              column's own basal stress (0 = none)          */
 } imau_cfg;
 
 /* The fields of type_ice_model (src/data_types_module.f90:56-325) that the

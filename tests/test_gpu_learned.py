@@ -100,14 +100,16 @@ def test_learned_warm_start_cuts_picard_iterations(env):
     prob, _ = problem(solver, tw, ctx, cfg, mem)
     prob.solve(cfg)
     its_cold, res_cold = prob.stats()
-    ws = learned.WarmStartSolver(ctx, prob, warm_iters=6)
-    prob.reset_state()
-    its_w, res_w = ws.solve(cfg)
-    print("cold", np.mean(its_cold), its_cold, "learned", np.mean(its_w), its_w)
-    assert max(res_cold) <= 1e-4 and max(res_w) <= 1e-4
-    assert ws.fallbacks == 0
-    assert np.mean(its_w) <= 0.8 * np.mean(its_cold)
-    ws.close()
+    # both completions of the warm state: column-local (the default) and staggered passes
+    for kw in (dict(), dict(warm_iters=6, column_iters=0)):
+        ws = learned.WarmStartSolver(ctx, prob, **kw)
+        prob.reset_state()
+        its_w, res_w = ws.solve(cfg)
+        print(kw, "cold", np.mean(its_cold), its_cold, "learned", np.mean(its_w), its_w)
+        assert max(res_cold) <= 1e-4 and max(res_w) <= 1e-4
+        assert ws.fallbacks == 0
+        assert np.mean(its_w) <= 0.8 * np.mean(its_cold)
+        ws.close()
     prob.close()
 
 
@@ -121,7 +123,7 @@ def test_warm_and_cold_starts_reach_the_same_fixed_point(env):
     prob, _ = problem(solver, tw, ctx, cfg, mem)
     prob.solve(cfg)
     cold = [(o["u_vav_cx"].copy(), o["v_vav_cy"].copy()) for o in prob.download()]
-    ws = learned.WarmStartSolver(ctx, prob, warm_iters=6)
+    ws = learned.WarmStartSolver(ctx, prob)
     prob.reset_state()
     its, res = ws.solve(cfg)
     assert max(res) <= 1e-8 and ws.fallbacks == 0
