@@ -178,12 +178,16 @@ def main():
                     help="viscosity passes completing the learned warm start (warm_visc_iters)")
     ap.add_argument("--warm-column", type=int, default=10,
                     help="column-local completion iterations before those passes (warm_column_iters)")
+    ap.add_argument("--warm-relax", type=float, default=0.7,
+                    help="relaxation of the unmixed Picard steps of a warm-started solve")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison legs")
     ap.add_argument("--no-cold", action="store_true", help="skip the cold-start leg of the learned method")
     ap.add_argument("--relax-max", type=float, default=None, help="override the headline's picard_relax_max")
     ap.add_argument("--ksp", choices=("bicgstab", "gcr"), default=None, help="override the headline's Krylov method")
     ap.add_argument("--mg-rebuild", type=int, default=None, help="override the headline's mg_rebuild")
     ap.add_argument("--anderson-m", type=int, default=None, help="override the headline's Anderson depth")
+    ap.add_argument("--relax", type=float, default=None,
+                    help="DIVA_visc_it_relax of the headline (its unmixed Picard steps)")
     ap.add_argument("--lin-rtol", type=float, default=0.3,
                     help="relative tolerance of each linear solve (inexact Picard; DIVA_PETSc_rtol)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -209,7 +213,8 @@ def main():
     nsolve = len(members)
     ctx = solver.Context()
     prob = ctx.problem(grids, cfg, states)
-    ws = (learned.WarmStartSolver(ctx, prob, warm_iters=args.warm_iters, column_iters=args.warm_column)
+    ws = (learned.WarmStartSolver(ctx, prob, warm_iters=args.warm_iters, column_iters=args.warm_column,
+                                  relax=args.warm_relax)
           if meth.get("learned") else None)
 
     # headline method
@@ -221,6 +226,8 @@ def main():
         meth["ksp_type"] = 1 if args.ksp == "gcr" else 0
         meth["text"] = meth["text"].replace("GCR", "BiCGStab") if args.ksp == "bicgstab" else meth["text"]
     use_method(cfg, meth, args.relax_max)
+    if args.relax is not None:
+        cfg.DIVA_visc_it_relax = args.relax
     dt = timed_solves(prob, cfg, args.steps, args.warmup, ws)
     out = prob.download()
     its = [o.stats["picard_iterations"] for o in out]
@@ -248,7 +255,8 @@ def main():
                     bytes_per_launch=per_launch, bytes_per_cell=bytes_per_cell)
 
     measured = (args.workload == "ensemble" and args.method in ("learned", "mixed", "mixed64visc") and args.ksp is None and args.relax_max is None
-                and args.lin_rtol == 0.3 and args.mg_rebuild is None and args.anderson_m is None)
+                and args.lin_rtol == 0.3 and args.mg_rebuild is None and args.anderson_m is None
+                and args.relax is None)
     visc_traffic = VISC_TRAFFIC_BYTES_PER_CELL.get(meth["precision"]) if measured else None
     sweep_traffic = SWEEP_TRAFFIC_BYTES_PER_CELL.get(meth["precision"]) if measured else None
     roof_visc = roof("visc", "k_strain + k_visc (strain rates, 3D viscosity, N, F2: once per Picard iteration)",
@@ -295,6 +303,7 @@ def main():
                                       ms_per_forward=1e3 * t_fwd, flop_per_forward=flops, pixels=ws.io.plan.npix[0])
         result["warm_start"] = dict(network=os.path.relpath(learned.WEIGHTS, os.path.dirname(__file__)),
                                     warm_visc_iters=args.warm_iters, warm_column_iters=args.warm_column,
+                                    warm_relax=args.warm_relax,
                                     fallbacks=ws.fallbacks,
                                     fallback="cold solve of the whole ensemble when any member's FP64 residual "
                                              "stays above 1e-4")
@@ -324,9 +333,12 @@ def main():
     if not args.no_cpu_baseline:
         k, ref, cb = cpu_baseline(cfg, members)
         result["cpu_baseline"] = cb
-        # the headline method's solve of member k against the oracle's (both stopped at 1e-4)
+        # the headline's solve of member k against the oracle's (both stopped at 1e-4)
         prob.reset_state()
-        prob.solve(cfg)
+        if ws is not None:
+            ws.solve(cfg)
+        else:
+            prob.solve(cfg)
         out = prob.download()
         u, ur = out[k]["u_vav_cx"], ref["u_vav_cx"]
         result["err_vs_oracle_at_1e-4"] = float(np.abs(u - ur).max() / np.abs(ur).max())

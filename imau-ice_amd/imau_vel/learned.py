@@ -353,25 +353,30 @@ class WarmStartSolver:
     state (the warm start replaces the velocities, the completion passes the
     viscosity and basal stress)."""
 
-    def __init__(self, ctx, problem, net=None, warm_iters=0, column_iters=10):
+    def __init__(self, ctx, problem, net=None, warm_iters=0, column_iters=10, relax=0.7):
         self.net = net if net is not None else Net.load(ctx)
         self.problem = problem
         self.io = ProblemIO(self.net, problem)
         self.warm_iters = warm_iters
         self.column_iters = column_iters
+        # relaxation of the unmixed Picard steps (the first, and any after an
+        # Anderson restart) from the warm start; the cold fallback keeps cfg's
+        self.relax = relax
         self.fallbacks = 0
 
     def warm_start(self, cfg):
         self.io.warm_start(cfg)
 
     def solve(self, cfg):
-        saved = cfg.warm_visc_iters, cfg.warm_column_iters
+        saved = cfg.warm_visc_iters, cfg.warm_column_iters, cfg.DIVA_visc_it_relax
         cfg.warm_visc_iters, cfg.warm_column_iters = self.warm_iters, self.column_iters
+        if self.relax is not None:
+            cfg.DIVA_visc_it_relax = self.relax
         try:
             self.io.warm_start(cfg)
             self.problem.solve(cfg)
         finally:
-            cfg.warm_visc_iters, cfg.warm_column_iters = saved
+            cfg.warm_visc_iters, cfg.warm_column_iters, cfg.DIVA_visc_it_relax = saved
         its, res = self.problem.stats()
         if cfg.stop_rule == 1 and any(not r <= cfg.nonlinear_rtol for r in res):
             self.fallbacks += 1
