@@ -4,8 +4,9 @@ The network (four levels, 3x3 convolutions with 32 channels, BF16 on the
 matrix cores) maps ten per-cell features of the geometry, friction and rate
 factor to the a-grid velocities.  Its forward and backward passes, the loss and
 the Adam optimiser are kernels of libimauvel.so; this module only sequences
-them and owns the device buffers (allocated through torch, which is used for
-memory and copies, never for the arithmetic).
+them and owns the device buffers (torch allocates them, gathers training
+batches and sums the loss weights; the network's arithmetic is the
+library's).
 
 IMAU-ICE has no counterpart: its Picard loop starts from the previous call's
 velocities (src/ice_velocity_module.f90:249-374).  For independent solves the
