@@ -100,15 +100,15 @@ def use_method(cfg, meth, relax_max=None):
                       picard_relax_max=meth["picard_relax_max"] if relax_max is None else relax_max)
 
 
-def workload(name, lin_rtol=0.3):
+def workload(name, lin_rtol=0.3, seed=0):
     from imau_vel import config, geometry
     if name == "channel":
         g, cfg, st = geometry.channel(nx=401, ny=41, dx=2e3, seed=0)
         members = [(g, st)]
         desc = "marine channel 401x41 at 2 km, DIVA cold start"
     else:
-        cfg, members = geometry.ensemble(64, 96, 512, seed=0)
-        desc = "ensemble of 64 DIVA solves, n x n with n ~ U[96,512] at 4 km (seed 0), cold start"
+        cfg, members = geometry.ensemble(64, 96, 512, seed=seed)
+        desc = "ensemble of 64 DIVA solves, n x n with n ~ U[96,512] at 4 km (seed %d), cold start" % seed
     config.set_config(cfg, stop_rule=1, nonlinear_rtol=1e-4, DIVA_visc_it_nit=500,
                       DIVA_choice_matrix_solver="PETSc", check_interval=8)
     for _, st in members:
@@ -172,6 +172,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=("ensemble", "channel"), default="ensemble")
+    ap.add_argument("--seed", type=int, default=0, help="seed of the ensemble")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--method", choices=tuple(METHODS), default="learned")
     ap.add_argument("--warm-iters", type=int, default=0,
@@ -207,7 +208,7 @@ def main():
         from imau_vel import learned   # imports torch before the library initialises HIP
     from imau_vel import config, solver
 
-    cfg, members, desc = workload(args.workload, args.lin_rtol)
+    cfg, members, desc = workload(args.workload, args.lin_rtol, args.seed)
     grids = [g for g, _ in members]
     states = [st.copy() for _, st in members]    # download() writes into these
     nsolve = len(members)
@@ -254,7 +255,7 @@ def main():
                     traffic=traffic, kernel=kernel, avg_launch_us=avg_us, launches=n, cells_processed=cells_t,
                     bytes_per_launch=per_launch, bytes_per_cell=bytes_per_cell)
 
-    measured = (args.workload == "ensemble" and args.method in ("learned", "mixed", "mixed64visc") and args.ksp is None and args.relax_max is None
+    measured = (args.workload == "ensemble" and args.seed == 0 and args.method in ("learned", "mixed", "mixed64visc") and args.ksp is None and args.relax_max is None
                 and args.lin_rtol == 0.3 and args.mg_rebuild is None and args.anderson_m is None
                 and args.relax is None)
     visc_traffic = VISC_TRAFFIC_BYTES_PER_CELL.get(meth["precision"]) if measured else None
