@@ -134,9 +134,13 @@ __global__ void __launch_bounds__(256) k_conv32(const NImg *__restrict__ img, co
                                                 const uint16_t *__restrict__ w, const float *__restrict__ bias,
                                                 int relu, void *__restrict__ yv) {
   constexpr int CIN = 32, NCT = COUT / 16;
-  const int lane = threadIdx.x & 63, lc = lane & 15, lk = lane >> 4;
-  const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (wid >= ntiles) return;  // uniform per wavefront
+  // each wavefront stages its tile plus a one-pixel halo ((TR+2) x (TC+2)
+  // pixels, 32 channels, rows padded to 40 channels against bank conflicts)
+  // in LDS once, instead of nine overlapping global loads per tap
+  constexpr int XR = TR + 2, XC = TC + 2, CP = 40, WSZ = XR * XC * CP;
+  __shared__ __align__(16) uint16_t sx[4 * WSZ];
+  const int lane = threadIdx.x & 63, lc = lane & 15, lk = lane >> 4, wv = threadIdx.x >> 6;
+  uint16_t *my = sx + wv * WSZ;
   b8 wa[9][NCT];
 #pragma unroll
   for (int tap = 0; tap < 9; ++tap)
@@ -148,57 +152,74 @@ __global__ void __launch_bounds__(256) k_conv32(const NImg *__restrict__ img, co
 #pragma unroll
     for (int e = 0; e < 4; ++e) bs[ct][e] = bias ? bias[ct * 16 + 4 * lk + e] : 0.0f;
 
-  for (int tix = wid; tix < ntiles; tix += gridDim.x * 4) {
-    const NTile t = tiles[tix];
-    const NImg im = img[t.img];
-    f4 acc[TR][NCT];
+  // the same number of rounds for every wavefront of the block (barriers)
+  for (int base = blockIdx.x * 4; base < ntiles; base += gridDim.x * 4) {
+    const int tix = base + wv;
+    const bool have = tix < ntiles;
+    NTile t = NTile{0, 0, 0, 0};
+    NImg im = NImg{0, 0, 0};
+    if (have) {
+      t = tiles[tix];
+      im = img[t.img];
+    }
+    if (have)
+      for (int q = lane; q < XR * XC * 4; q += 64) {
+        const int px = q >> 2, ch = q & 3;
+        const int hr = px / XC, hc = px - hr * XC;
+        const int rr = t.r0 + hr - 1, cc = t.c0 + hc - 1;
+        b8 v = {};
+        if (rr >= 0 && rr < im.rows && cc >= 0 && cc < im.cols)
+          v = ld8(x + (im.off + (long long)rr * im.cols + cc) * CIN + 8 * ch);
+        *reinterpret_cast<b8 *>(my + px * CP + 8 * ch) = v;
+      }
+    __syncthreads();
+    if (have) {
+      f4 acc[TR][NCT];
 #pragma unroll
-    for (int r = 0; r < TR; ++r)
+      for (int r = 0; r < TR; ++r)
 #pragma unroll
-      for (int ct = 0; ct < NCT; ++ct) acc[r][ct] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int ct = 0; ct < NCT; ++ct) acc[r][ct] = f4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int tap = 0; tap < 9; ++tap) {
-      const int dr = tap / 3 - 1, dc = tap % 3 - 1;
-      const int c = t.c0 + lc + dc;
-      const bool cok = c >= 0 && c < im.cols;
+      for (int tap = 0; tap < 9; ++tap) {
+        const int dr = tap / 3 - 1, dc = tap % 3 - 1;
+#pragma unroll
+        for (int r = 0; r < TR; ++r) {
+          const b8 b = *reinterpret_cast<const b8 *>(my + ((r + 1 + dr) * XC + lc + 1 + dc) * CP + 8 * lk);
+#pragma unroll
+          for (int ct = 0; ct < NCT; ++ct)
+            acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[tap][ct], b, acc[r][ct], 0, 0, 0);
+        }
+      }
+      const int c = t.c0 + lc;
 #pragma unroll
       for (int r = 0; r < TR; ++r) {
-        const int rr = t.r0 + r + dr;
-        b8 b = {};
-        if (cok && rr >= 0 && rr < im.rows) b = ld8(x + (im.off + (long long)rr * im.cols + c) * CIN + 8 * lk);
+        const int rr = t.r0 + r;
+        if (rr >= im.rows || c >= im.cols) continue;
+        const long long pix = im.off + (long long)rr * im.cols + c;
 #pragma unroll
-        for (int ct = 0; ct < NCT; ++ct)
-          acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[tap][ct], b, acc[r][ct], 0, 0, 0);
-      }
-    }
-    const int c = t.c0 + lc;
+        for (int ct = 0; ct < NCT; ++ct) {
+          float v[4];
 #pragma unroll
-    for (int r = 0; r < TR; ++r) {
-      const int rr = t.r0 + r;
-      if (rr >= im.rows || c >= im.cols) continue;
-      const long long pix = im.off + (long long)rr * im.cols + c;
-#pragma unroll
-      for (int ct = 0; ct < NCT; ++ct) {
-        float v[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          v[e] = acc[r][ct][e] + bs[ct][e];
-          if (relu) v[e] = fmaxf(v[e], 0.0f);
-        }
-        if constexpr (OUTF) {
-          if (ct == 0 && lk == 0) {
-            float *y = static_cast<float *>(yv);
-            y[2 * pix] = v[0];
-            y[2 * pix + 1] = v[1];
+          for (int e = 0; e < 4; ++e) {
+            v[e] = acc[r][ct][e] + bs[ct][e];
+            if (relu) v[e] = fmaxf(v[e], 0.0f);
           }
-        } else {
-          s4 o;
+          if constexpr (OUTF) {
+            if (ct == 0 && lk == 0) {
+              float *y = static_cast<float *>(yv);
+              y[2 * pix] = v[0];
+              y[2 * pix + 1] = v[1];
+            }
+          } else {
+            s4 o;
 #pragma unroll
-          for (int e = 0; e < 4; ++e) o[e] = (short)f2bf(v[e]);
-          st4(static_cast<uint16_t *>(yv) + pix * COUT + ct * 16 + 4 * lk, o);
+            for (int e = 0; e < 4; ++e) o[e] = (short)f2bf(v[e]);
+            st4(static_cast<uint16_t *>(yv) + pix * COUT + ct * 16 + 4 * lk, o);
+          }
         }
       }
     }
+    __syncthreads();
   }
 }
 
