@@ -51,10 +51,13 @@ __device__ void block_sum_store(double (&v)[NV], double *out) {
 // ---------------------------------------------------------------------------
 // sliding laws, src/basal_conditions_and_sliding_module.f90:908-1432
 // ---------------------------------------------------------------------------
-__device__ double sliding_beta(const KP &kp, double ua, double va, double beta_sq, double alpha_sq, double phi,
-                               double tauc, double Neff, double x, double y, double A_vav) {
+// `slid` is kp.slid, or the law as a compile-time constant so that a kernel
+// specialised on it carries only that law
+__device__ __forceinline__ double sliding_beta(const KP &kp, int slid, double ua, double va, double beta_sq,
+                                               double alpha_sq, double phi, double tauc, double Neff, double x,
+                                               double y, double A_vav) {
   const double uabs = sqrt(kp.delta_v * kp.delta_v + ua * ua + va * va);
-  switch (kp.slid) {
+  switch (slid) {
     case 0: return 0.0;
     case 4: return beta_sq * pow(uabs, 1.0 / kp.wm - 1.0);                                   // Weertman :984-987
     case 2: return tauc / uabs;                                                               // Coulomb :1039-1041
@@ -490,13 +493,15 @@ __global__ void __launch_bounds__(IMAU_WG) k_visc(const Member *__restrict__ mem
 // F2.  A separate kernel so the register-hungry sliding laws do not limit the
 // occupancy of the 3D viscosity kernel.
 // ---------------------------------------------------------------------------
-template <bool DIVA>
+// SLID >= 0: the kernel carries only that sliding law (kp.slid == SLID)
+template <bool DIVA, int SLID>
 __global__ void __launch_bounds__(IMAU_WG) k_beta(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
                                                   const KP kp, const Fields f, const double *__restrict__ xg,
                                                   const double *__restrict__ yg, const int *__restrict__ active) {
   TILE_SETUP
   if (j >= m.ny || !active[t.g]) return;
   const int nx = m.nx, ny = m.ny, nz = kp.nz;
+  const int slid = SLID >= 0 ? SLID : kp.slid;
   const double *__restrict__ u = f.u;
   const double *__restrict__ v = f.v;
   FOR_TILE_ROWS {
@@ -513,11 +518,11 @@ __global__ void __launch_bounds__(IMAU_WG) k_beta(const Member *__restrict__ mem
     const double F2 = DIVA ? f.F2[p] : 0.0;
     // basal friction (calc_sliding_term_beta, :900-959)
     double tauc = f.tauc[p];
-    if (kp.slid == 3 || (kp.slid == 2 && !(kp.rough == 1 && kp.prough == 1))) {
+    if (slid == 3 || (slid == 2 && !(kp.rough == 1 && kp.prough == 1))) {
       tauc = tan((D_PI / 180.0) * f.phi_fric[p]) * f.Neff[p];
       f.tauc[p] = tauc;
     }
-    double beta = sliding_beta(kp, ua, va, f.beta_sq[p], f.alpha_sq[p], f.phi_fric[p], tauc, f.Neff[p],
+    double beta = sliding_beta(kp, slid, ua, va, f.beta_sq[p], f.alpha_sq[p], f.phi_fric[p], tauc, f.Neff[p],
                                xg[m.offx + i], yg[m.offy + j], f.A_vav ? f.A_vav[p] : 0.0);
     beta = fmin(beta, kp.beta_max);
     if (!kp.glsub && f.m_ocean[p] == 1) beta = 0.0;
@@ -525,7 +530,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_beta(const Member *__restrict__ mem
     // beta_eff (calc_beta_eff, :1014-1062); SSA uses beta itself (:193)
     double be;
     if (DIVA) {
-      if (kp.slid == 0) be = 1.0 / F2;
+      if (slid == 0) be = 1.0 / F2;
       else {
         be = beta / (1.0 + beta * F2);
         if (kp.freeze) {
@@ -572,7 +577,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_visc_column(const Member *__restric
     double tauc = f.tauc[p];
     if (kp.slid == 3 || (kp.slid == 2 && !(kp.rough == 1 && kp.prough == 1)))
       tauc = tan((D_PI / 180.0) * f.phi_fric[p]) * f.Neff[p];
-    double beta = sliding_beta(kp, ua, va, f.beta_sq[p], f.alpha_sq[p], f.phi_fric[p], tauc, f.Neff[p],
+    double beta = sliding_beta(kp, kp.slid, ua, va, f.beta_sq[p], f.alpha_sq[p], f.phi_fric[p], tauc, f.Neff[p],
                                xg[m.offx + i], yg[m.offy + j], f.A_vav ? f.A_vav[p] : 0.0);
     beta = fmin(beta, kp.beta_max);
     if (!kp.glsub && f.m_ocean[p] == 1) beta = 0.0;
@@ -1459,8 +1464,10 @@ void to_f64(hipStream_t s, const float *a, double *b, long long n) {
 }
 void beta(const Launch &L, const KP &kp, const Fields &f, bool diva, const double *xg, const double *yg,
           const int *active) {
-  if (diva) hipLaunchKernelGGL(k_beta<true>, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, xg, yg, active);
-  else hipLaunchKernelGGL(k_beta<false>, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, xg, yg, active);
+  // Weertman (the common law) gets its own specialisation
+  auto k = diva ? (kp.slid == 4 ? k_beta<true, 4> : k_beta<true, -1>)
+                : (kp.slid == 4 ? k_beta<false, 4> : k_beta<false, -1>);
+  hipLaunchKernelGGL(k, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, xg, yg, active);
 }
 void coef(const Launch &L, const KP &kp, const Fields &f, bool diva, const int *active) {
   hipLaunchKernelGGL(k_coef, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, diva ? 1 : 0, active);
