@@ -1569,6 +1569,7 @@ enum { RHO = 0, ALPHA = 1, OMEGA = 2, BETA = 3, RR = 4, NSC = 8 };
 
 
 // y = A x over every row; with w != null also partial sums (w, y) and (y, y)
+template <int CROSS>
 __global__ void __launch_bounds__(IMAU_WG) k_apply(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
                                                    const KP kp, const CoefP c, CVec2 x, Vec2 y, CVec2 w,
                                                    const int *__restrict__ active, double *__restrict__ partial) {
@@ -1580,15 +1581,15 @@ __global__ void __launch_bounds__(IMAU_WG) k_apply(const Member *__restrict__ me
       const int i = t.i0 + ii;
       if (i >= m.nx) break;
       const long long p = p2(m, j, i);
-      double S, D, b;
+      double S, D;
       if (i < m.nx - 1) {
-        u_row(m, kp, c, j, i, c.rt_u[p], U, V, S, D, b);
+        u_sd_x<CROSS, double>(m, kp, c, j, i, c.rt_u[p], U, V, S, D);
         const double yv = S + D * x.u[p];
         y.u[p] = yv;
         if (w.u) { acc[0] += w.u[p] * yv; acc[1] += yv * yv; }
       }
       if (j < m.ny - 1) {
-        v_row(m, kp, c, j, i, c.rt_v[p], U, V, S, D, b);
+        v_sd_x<CROSS, double>(m, kp, c, j, i, c.rt_v[p], U, V, S, D);
         const double yv = S + D * x.v[p];
         y.v[p] = yv;
         if (w.u) { acc[0] += w.v[p] * yv; acc[1] += yv * yv; }
@@ -1906,7 +1907,8 @@ __global__ void __launch_bounds__(IMAU_WG) k_sor_color(const Member *__restrict_
 namespace imau {
 void apply(const Launch &L, const KP &kp, const Coef &c, const double *xu, const double *xv, double *yu, double *yv,
            const double *wu, const double *wv, const int *active, double *partial) {
-  hipLaunchKernelGGL(k_apply, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, to_p(c), CVec2{xu, xv}, Vec2{yu, yv},
+  auto k = kp.cross ? k_apply<1> : k_apply<0>;
+  hipLaunchKernelGGL(k, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, to_p(c), CVec2{xu, xv}, Vec2{yu, yv},
                      CVec2{wu, wv}, active, partial);
 }
 void bicg_init(const Launch &L, const KP &kp, const Coef &c, const double *xu, const double *xv, double *ru, double *rv,

@@ -607,7 +607,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_smooth_fused(const Member *__restri
 }
 
 // level-1 right-hand side: R (y - A z) over the interior and calving-front rows
-template <class T>
+template <class T, int CROSS>
 __global__ void __launch_bounds__(IMAU_WG) k_restrict0(const Member *__restrict__ mem, const KP kp, const CoefT<T> c,
                                                        CVecT<T> y, CVecT<T> z, CLevelT<T> L,
                                                        const int *__restrict__ active) {
@@ -630,10 +630,10 @@ __global__ void __launch_bounds__(IMAU_WG) k_restrict0(const Member *__restrict_
           if (!in_row(rt)) continue;
           T S, D;
           if (comp == 0) {
-            u_sd<T>(m, kp, c, j, i, rt, U, V, S, D);
+            u_sd_x<CROSS, T>(m, kp, c, j, i, rt, U, V, S, D);
             s += y.u[p] - S - D * z.u[p];
           } else {
-            v_sd<T>(m, kp, c, j, i, rt, U, V, S, D);
+            v_sd_x<CROSS, T>(m, kp, c, j, i, rt, U, V, S, D);
             s += y.v[p] - S - D * z.v[p];
           }
         }
@@ -832,7 +832,8 @@ void vcycle_t(const imau::Launch &F, const KP &kp, const CoefT<T> &cf, const ima
     hipLaunchKernelGGL(k_smooth0<T>, dim3(F.ntiles), wg(), 0, s, F.mem, F.tiles, kp, cf, z, y, col, act);
   if (mark) mark(mctx, 1);
   const CLevelT<T> L1 = view<T>(P, 0);
-  hipLaunchKernelGGL(k_restrict0<T>, dim3(L1.ntiles), wg(), 0, s, F.mem, kp, cf, y, CVecT<T>{z.u, z.v}, L1, act);
+  auto kr = kp.cross ? k_restrict0<T, 1> : k_restrict0<T, 0>;
+  hipLaunchKernelGGL(kr, dim3(L1.ntiles), wg(), 0, s, F.mem, kp, cf, y, CVecT<T>{z.u, z.v}, L1, act);
   coarse_t<T>(s, P, act);
   hipLaunchKernelGGL(k_prolong0<T>, dim3(F.ntiles), wg(), 0, s, F.mem, F.tiles, cf.rt_u, cf.rt_v, z, L1, act);
   for (int col = 3; col >= 0; --col)
@@ -852,7 +853,8 @@ void vcycle_fused_t(const imau::Launch &F, const KP &kp, const CoefT<T> &cf, con
     hipLaunchKernelGGL(k_smooth0<T>, dim3(F.ntiles), wg(), 0, s, F.mem, F.tiles, kp, cf, za, y, 4, act);
   if (mark) mark(mctx, 1);
   const CLevelT<T> L1 = view<T>(P, 0);
-  hipLaunchKernelGGL(k_restrict0<T>, dim3(L1.ntiles), wg(), 0, s, F.mem, kp, cf, y, CVecT<T>{za.u, za.v}, L1, act);
+  auto kr = kp.cross ? k_restrict0<T, 1> : k_restrict0<T, 0>;
+  hipLaunchKernelGGL(kr, dim3(L1.ntiles), wg(), 0, s, F.mem, kp, cf, y, CVecT<T>{za.u, za.v}, L1, act);
   coarse_fused_t<T>(s, P, act);
   hipLaunchKernelGGL(k_prolong0<T>, dim3(F.ntiles), wg(), 0, s, F.mem, F.tiles, cf.rt_u, cf.rt_v, za, view2<T>(P, 0),
                      act);

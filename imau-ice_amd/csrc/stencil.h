@@ -46,13 +46,14 @@ __device__ __forceinline__ void bnd_src(uint8_t rt, int i, int j, int imax, int 
 // u row: free rows with cross terms (:2342-2467) or in the normalised 'sans'
 // form (:2468-2589); zero rows (:1787-1796, :2224-2230); domain boundary rows
 // (:1932-2191); calving-front stress rows (:2192-2266)
-template <class T, class AU, class AV, class C = CoefT<T>>
-__device__ __forceinline__ void u_sd(const Member &m, const KP &kp, const C &c, int j, int i, uint8_t rt,
-                                     const AU &U, const AV &V, T &S, T &D) {
+// CROSS: 1 / 0 fixes the cross-term choice at compile time, -1 reads kp.cross
+template <int CROSS, class T, class AU, class AV, class C>
+__device__ __forceinline__ void u_sd_x(const Member &m, const KP &kp, const C &c, int j, int i, uint8_t rt,
+                                       const AU &U, const AV &V, T &S, T &D) {
   const int nx = m.nx, ny = m.ny;
   if (rt == RT_FREE) {
     const T dx2 = (T)(m.dx * m.dx);
-    if (kp.cross) {
+    if (CROSS >= 0 ? CROSS != 0 : kp.cross != 0) {
       const T Na0 = c.na(m, j, i), Na1 = c.na(m, j, i + 1);
       const T Nb0 = c.nb(m, j, i), Nbm = c.nb(m, j - 1, i);
       S = (T)4.0 * Na1 * U(j, i + 1) + (T)4.0 * Na0 * U(j, i - 1) + Nb0 * U(j + 1, i) + Nbm * U(j - 1, i) +
@@ -82,13 +83,13 @@ __device__ __forceinline__ void u_sd(const Member &m, const KP &kp, const C &c, 
   else { S = cr * U(j, i + 1) + sv; D = cl; }
 }
 
-template <class T, class AU, class AV, class C = CoefT<T>>
-__device__ __forceinline__ void v_sd(const Member &m, const KP &kp, const C &c, int j, int i, uint8_t rt,
-                                     const AU &U, const AV &V, T &S, T &D) {
+template <int CROSS, class T, class AU, class AV, class C>
+__device__ __forceinline__ void v_sd_x(const Member &m, const KP &kp, const C &c, int j, int i, uint8_t rt,
+                                       const AU &U, const AV &V, T &S, T &D) {
   const int nx = m.nx, ny = m.ny;
   if (rt == RT_FREE) {
     const T dx2 = (T)(m.dx * m.dx);
-    if (kp.cross) {
+    if (CROSS >= 0 ? CROSS != 0 : kp.cross != 0) {
       const T Na0 = c.na(m, j, i), Na1 = c.na(m, j + 1, i);
       const T Nb0 = c.nb(m, j, i), Nbm = c.nb(m, j, i - 1);
       S = (T)4.0 * Na1 * V(j + 1, i) + (T)4.0 * Na0 * V(j - 1, i) + Nb0 * V(j, i + 1) + Nbm * V(j, i - 1) +
@@ -116,4 +117,15 @@ __device__ __forceinline__ void v_sd(const Member &m, const KP &kp, const C &c, 
   const T su = (T)-2.0 * Na * U(j1, i - 1) + (T)2.0 * Na * U(j1, i);
   if (j1 == j) { S = cl * V(j - 1, i) + su; D = cr; }
   else { S = cr * V(j + 1, i) + su; D = cl; }
+}
+
+template <class T, class AU, class AV, class C = CoefT<T>>
+__device__ __forceinline__ void u_sd(const Member &m, const KP &kp, const C &c, int j, int i, uint8_t rt,
+                                     const AU &U, const AV &V, T &S, T &D) {
+  u_sd_x<-1, T>(m, kp, c, j, i, rt, U, V, S, D);
+}
+template <class T, class AU, class AV, class C = CoefT<T>>
+__device__ __forceinline__ void v_sd(const Member &m, const KP &kp, const C &c, int j, int i, uint8_t rt,
+                                     const AU &U, const AV &V, T &S, T &D) {
+  v_sd_x<-1, T>(m, kp, c, j, i, rt, U, V, S, D);
 }
