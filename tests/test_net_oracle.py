@@ -81,3 +81,21 @@ def test_bf16_rounding_matches_torch():
     assert np.array_equal(NO.bf16(a), ref)
     assert NO.bf16(np.float32(1.0 + 2.0 ** -8)) == 1.0          # tie to even (down)
     assert NO.bf16(np.float32(1.0 + 3 * 2.0 ** -8)) == 1.0 + 2.0 ** -6  # tie to even (up)
+
+
+def test_package_network_matches_the_oracle_contract():
+    """The layer table, parameter count and packed-parameter order of
+    imau_vel.learned are the oracle's; the committed weights have that shape."""
+    from imau_vel import learned
+    assert [tuple(l) for l in learned.LAYERS] == [tuple(l) for l in NO.LAYERS]
+    assert learned.n_params() == NO.n_params()
+    p = learned.init_params(0)
+    parts = NO.split_params(p)
+    assert len(parts) == len(NO.LAYERS)
+    assert np.all(parts[0][0][:, :, learned.USED_FEATURES:] == 0.0)   # unused inputs
+    assert np.all(parts[-1][0][:, 2:, :] == 0.0)                       # unused outputs
+    if os.path.exists(learned.WEIGHTS):
+        with np.load(learned.WEIGHTS, allow_pickle=False) as z:
+            assert z["params"].shape == (NO.n_params(),)
+            assert np.array_equal(z["layers"], np.array(NO.LAYERS, np.int32))
+            assert np.all(np.isfinite(z["params"]))
