@@ -26,12 +26,18 @@ The line also carries:
                 launch processed;
   roofline_smoother
                 the same for one fine-level smoother sweep (k_smooth_fused);
-  cpu_baseline  the CPU oracle (FP64, banded-LU linear solves, 1 thread) on a
-                bounded sample: the smallest ensemble member solved alone;
+  cpu_baseline  the CPU oracle (FP64, banded-LU linear solves) on bounded
+                samples: the smallest ensemble member on one thread (value),
+                the dome and the marine channel on one thread and on every
+                usable core, with the core count and CPU model;
   fp64_multigrid, plain_fp64
                 the same workload with the FP64 multigrid method and with the
                 plain FP64 method (BiCGStab with tile Gauss-Seidel
-                preconditioning, no multigrid).
+                preconditioning, no multigrid), timed with the headline's
+                warm-up and step count;
+  err_vs_oracle_at_1e-4
+                the headline's and the plain method's velocities against the
+                oracle's own solves of the three smallest members.
 """
 from __future__ import annotations
 
@@ -118,22 +124,85 @@ def workload(name, lin_rtol=0.3, seed=0):
     return cfg, members, desc
 
 
-def cpu_baseline(cfg, members):
-    """The CPU oracle (FP64, exact banded-LU linear solves standing in for
-    PETSc, 1 thread) on one full solve of the smallest member."""
+def cpu_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return dict(nproc=os.cpu_count(), usable=usable, model=model)
+
+
+def _oracle_concurrent(jobs):
+    """Run oracle DIVA solves [(grid, cfg, state)] concurrently, one thread each
+    (the oracle's C call releases the interpreter lock); returns the wall time
+    of the whole set and of each solve."""
+    import threading
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
-    k = min(range(len(members)), key=lambda q: members[q][0].nx * members[q][0].ny)
-    g, st = members[k]
-    s = st.copy()
+    errs, each = [], [0.0] * len(jobs)
+
+    def run(q, g, c, st):
+        t = time.perf_counter()
+        try:
+            O.solve("diva", g, c, st, O.LIN_BANDED_LU)
+        except Exception as e:  # reported below
+            errs.append(e)
+        each[q] = time.perf_counter() - t
+    threads = [threading.Thread(target=run, args=(q,) + tuple(job)) for q, job in enumerate(jobs)]
     t0 = time.perf_counter()
-    O.solve("diva", g, cfg, s, O.LIN_BANDED_LU)
-    dt = time.perf_counter() - t0
-    cb = dict(value=1.0 / dt, unit="solves/s", cores=1, kind="port",
-              sample="member %d (%dx%d) solved alone to the 1e-4 FP64 residual: %d Picard iterations, %.1f s "
-                     "(the smallest member; larger members cost more)" %
-                     (k, g.nx, g.ny, s.stats["picard_iterations"], dt))
-    return k, s, cb
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if errs:
+        raise errs[0]
+    return time.perf_counter() - t0, each
+
+
+def cpu_baseline(cfg, members, n_err=3):
+    """The CPU oracle (FP64, exact banded-LU linear solves standing in for
+    PETSc, one thread per solve):
+      * the headline sample: the n_err smallest ensemble members solved to the
+        1e-4 residual at once (one thread each); `value` is the smallest
+        member's rate, and the solutions are kept for the error of the GPU
+        results;
+      * the dome (29 x 29) and the marine channel (401 x 41): one solve on one
+        thread, and one solve per usable core concurrently (the reference's
+        parallelism is MPI over columns; independent solves on every core give
+        the machine's throughput)."""
+    from imau_vel import config, geometry
+    info = cpu_info()
+    order = sorted(range(len(members)), key=lambda q: members[q][0].nx * members[q][0].ny)[:n_err]
+    refs = [members[k][1].copy() for k in order]
+    _, each = _oracle_concurrent([(members[k][0], cfg, r) for k, r in zip(order, refs)])
+    k0, r0, dt = order[0], refs[0], each[0]
+    g0 = members[k0][0]
+    cb = dict(value=1.0 / dt, unit="solves/s", cores=1, kind="port", cpu=info)
+    cb["sample"] = ("member %d (%dx%d, the smallest; larger members cost more) solved to the 1e-4 FP64 residual on "
+                    "one thread (beside two other members on their own threads): %d Picard iterations, %.1f s" %
+                    (k0, g0.nx, g0.ny, r0.stats["picard_iterations"], dt))
+    # the reference configurations: 1 thread and every usable core
+    ncores = max(1, min(16, info["usable"]))
+    for name in ("dome", "channel"):
+        if name == "dome":
+            g, c2, st = geometry.dome()
+        else:
+            g, c2, st = geometry.channel()
+        config.set_config(c2, stop_rule=1, nonlinear_rtol=1e-4, DIVA_visc_it_nit=500)
+        t1, _ = _oracle_concurrent([(g, c2, st.copy())])
+        tn, _ = _oracle_concurrent([(g, c2, st.copy()) for _ in range(ncores)])
+        cb[name] = dict(grid="%dx%d" % (g.nx, g.ny), one_thread=1.0 / t1, all_cores=ncores / tn, cores=ncores,
+                        unit="solves/s", stop="FP64 residual <= 1e-4, reference Picard relaxation, banded LU")
+    return order, refs, cb
 
 
 def copy_probe(nbytes):
@@ -322,27 +391,39 @@ def main():
                 continue
             m2 = METHODS[key]
             use_method(cfg, m2)
-            dtp = timed_solves(prob, cfg, 1, 0)
+            # the same warm-up and step count as the headline
+            dtp = timed_solves(prob, cfg, args.steps, args.warmup)
             outp = prob.download()
             name = "plain_fp64" if key == "plain" else "fp64_multigrid"
-            result[name] = dict(value=nsolve / dtp, unit="solves/s", method=m2["text"],
+            result[name] = dict(value=nsolve * args.steps / dtp, unit="solves/s", method=m2["text"],
+                                steps=args.steps, warmup=args.warmup,
                                 picard_max=max(o.stats["picard_iterations"] for o in outp),
                                 max_final_residual=max(o.stats["nonlinear_residual"] for o in outp))
         if "plain_fp64" in result:
             result["speedup_vs_plain_fp64"] = result["value"] / result["plain_fp64"]["value"]
         use_method(cfg, meth, args.relax_max)
     if not args.no_cpu_baseline:
-        k, ref, cb = cpu_baseline(cfg, members)
+        order, refs, cb = cpu_baseline(cfg, members)
         result["cpu_baseline"] = cb
-        # the headline's solve of member k against the oracle's (both stopped at 1e-4)
-        prob.reset_state()
-        if ws is not None:
-            ws.solve(cfg)
-        else:
-            prob.solve(cfg)
-        out = prob.download()
-        u, ur = out[k]["u_vav_cx"], ref["u_vav_cx"]
-        result["err_vs_oracle_at_1e-4"] = float(np.abs(u - ur).max() / np.abs(ur).max())
+        # the headline's solves of those members against the oracle's (both
+        # stopped at 1e-4), and the plain FP64 method's
+        errs = {}
+        for key, m2 in (("headline", meth), ("plain_fp64", METHODS["plain"])):
+            use_method(cfg, m2)
+            prob.reset_state()
+            if key == "headline" and ws is not None:
+                ws.solve(cfg)
+            else:
+                prob.solve(cfg)
+            out = prob.download()
+            errs[key] = {"member %d (%dx%d)" % (k, grids[k].nx, grids[k].ny):
+                         float(max(np.abs(out[k][f] - r[f]).max() / np.abs(r[f]).max() for f in ("u_vav_cx", "v_vav_cy")))
+                         for k, r in zip(order, refs)}
+        use_method(cfg, meth, args.relax_max)
+        result["err_vs_oracle_at_1e-4"] = dict(errs, note="max relative velocity difference to the oracle's own solve; "
+                                                            "both stop at the 1e-4 residual, so this is the spread of "
+                                                            "1e-4-accurate solutions, not the fixed-point error (that is "
+                                                            "tested at 1e-6 / 1e-5 at residual 1e-8)")
     prob.close()
     ctx.close()
     print(json.dumps(result))

@@ -25,6 +25,17 @@
 // tile height of the 3D viscosity kernel (16 and 32 measured slower than 8 on
 // the 64-member ensemble: fewer workgroups in flight)
 #define IMAU_VTI 8
+// rows marched by one wavefront of the row-marching 3D viscosity kernel
+// (nz = 15): each lane keeps its column's viscosity at rows i-1, i, i+1 in an
+// LDS ring, so every level of the previous viscosity is read once, plus two
+// halo rows per tile.  0 selects the IMAU_VTI-row tiled kernel instead, the
+// default (measured 3x faster than 32-row marching on the 64-member ensemble:
+// one 64-lane wavefront per tile leaves too few waves in flight).
+// imau_vel_problem_autotune times the candidates IMAU_MTI_CANDIDATES on the
+// problem itself and keeps the fastest.
+#define IMAU_MTI 0
+#define IMAU_MTI_CANDIDATES {0, 4, 8, 16, 32}
+#define IMAU_MNZ 15
 
 // Anderson acceleration of the Picard loop: up to IMAU_AND_SLOTS stored
 // iterates (IMAU_AND_SLOTS - 1 differences)

@@ -20,6 +20,7 @@
 #include "kernels.h"
 #include "mg.h"
 #include "net.h"
+#include "thickness.h"
 
 namespace {
 
@@ -110,7 +111,9 @@ struct imau_problem {
   DBuf<unsigned long long> resmax;
   DBuf<int> act_pic, act_lin;
   DBuf<double> sc;    // per-member Krylov scalars
-  DBuf<double> relax_d;  // per-member Picard relaxation (adaptive mode)
+  DBuf<double> relax_d;  // per-member Picard relaxation (adaptive mode, restarted members)
+  DBuf<int> restart_d;   // per-member flag: reset to the created state (stall restart)
+  DBuf<int> stop_d;      // per-member flag: stopped at this iteration's residual check
   DBuf<double> gcr_dirs; // GCR directions Q_i, Z_i (allocated on first use)
   DBuf<double> and_slots;  // Anderson slots U_s, F_s (allocated on first use)
   DBuf<double> and_w;      // Anderson weights per member
@@ -122,6 +125,8 @@ struct imau_problem {
   DBuf<Tile> mg_ftiles;                      // tiles of the fused smoother
   DBuf<Tile> vtiles;                         // IMAU_VTI-row tiles of the 3D viscosity kernel
   int nvtiles = 0;
+  DBuf<Tile> mtiles;                         // tiles of its row-marching form (mti rows each)
+  int nmtiles = 0, mti = IMAU_MTI;
   imau::MGPlan plan;                         // device view of the levels
 
   Fields F{};
@@ -135,6 +140,7 @@ struct imau_problem {
   int dyn_uploaded = 0;
   // per-member results
   std::vector<int> picard_its;
+  std::vector<int> restarts;
   std::vector<long long> sweeps;
   std::vector<double> resid_uv, nl_resid;
   double seconds = 0.0;
@@ -161,6 +167,27 @@ static std::string g_init_err;
 
 // ---------------------------------------------------------------------------
 // problem construction
+
+// tiles of the row-marching viscosity kernel: `rows` rows x 64 columns (the
+// tile's row count travels in Tile::pad); rows = 0 leaves none, so the tiled
+// kernel runs.  The buffer is sized for the smallest candidate height (4), so
+// retiling never allocates
+static void set_mtiles(imau_problem *P, int rows) {
+  std::vector<Tile> mt;
+  if (rows > 0)
+    for (int g = 0; g < P->ngrids; ++g)
+      for (int i0 = 0; i0 < P->mem[g].nx; i0 += rows)
+        for (int j0 = 0; j0 < P->mem[g].ny; j0 += IMAU_TJ) mt.push_back(Tile{g, i0, j0, rows});
+  if (!P->mtiles.p) {
+    size_t most = 0;
+    for (int g = 0; g < P->ngrids; ++g)
+      most += (size_t)((P->mem[g].nx + 3) / 4) * ((P->mem[g].ny + IMAU_TJ - 1) / IMAU_TJ);
+    P->mtiles.alloc(std::max(most, mt.size()));
+  }
+  if (!mt.empty()) HIPCHK(hipMemcpy(P->mtiles.p, mt.data(), mt.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  P->nmtiles = (int)mt.size();
+  P->mti = rows;
+}
 
 // Coarse levels: every level halves each component's extent (2 x 2
 // aggregates, rounded up).  Member g is coarsened until it fits
@@ -228,6 +255,7 @@ static void build_mg(imau_problem *P) {
   P->vtiles.alloc(vt.size());
   HIPCHK(hipMemcpy(P->vtiles.p, vt.data(), vt.size() * sizeof(Tile), hipMemcpyHostToDevice));
   P->nvtiles = (int)vt.size();
+  set_mtiles(P, IMAU_MTI);
   P->mg_ftiles.alloc(ft.size());
   HIPCHK(hipMemcpy(P->mg_ftiles.p, ft.data(), ft.size() * sizeof(Tile), hipMemcpyHostToDevice));
   P->plan.ftiles = P->mg_ftiles.p;
@@ -324,6 +352,23 @@ void check_cfg(const imau_cfg *cfg) {
          cfg->choice_idealised_sliding_law);
   if (cfg->choice_basal_hydrology < 0 || cfg->choice_basal_hydrology > 2)
     fail(IMAU_ERR_CONFIG, "unknown choice_basal_hydrology %d", cfg->choice_basal_hydrology);
+  if (cfg->choice_basal_roughness < 0 || cfg->choice_basal_roughness > 2)
+    fail(IMAU_ERR_CONFIG, "unknown choice_basal_roughness %d", cfg->choice_basal_roughness);
+  const int law = cfg->choice_sliding_law;
+  if (cfg->choice_basal_roughness == IMAU_ROUGH_PARAMETERISED && law != IMAU_SLID_NO_SLIDING && law != IMAU_SLID_IDEALISED) {
+    const int pr = cfg->choice_param_basal_roughness;
+    if (pr < 0 || pr > 2) fail(IMAU_ERR_CONFIG, "unknown choice_param_basal_roughness %d", pr);
+    // src/basal_conditions_and_sliding_module.f90:789-792
+    if (pr == IMAU_PROUGH_MARTIN2011 && law != IMAU_SLID_COULOMB && law != IMAU_SLID_COULOMB_REGULARISED &&
+        law != IMAU_SLID_ZOET_IVERSON)
+      fail(IMAU_ERR_CONFIG, "Martin2011 roughness is only applicable when choice_sliding_law = Coulomb, "
+                            "Coulomb_regularised or Zoet-Iverson");
+    // :896-898
+    if (pr == IMAU_PROUGH_MISMIPPLUS && law != IMAU_SLID_WEERTMAN && law != IMAU_SLID_TSAI2015 &&
+        law != IMAU_SLID_SCHOOF2005)
+      fail(IMAU_ERR_CONFIG, "MISMIP+ roughness is only defined when choice_sliding_law = Weertman, Tsai2015 or "
+                            "Schoof2005");
+  }
   const int *bcs = &cfg->DIVA_boundary_BC_u_west;
   for (int q = 0; q < 8; ++q)
     if (bcs[q] < 0 || bcs[q] > 2) fail(IMAU_ERR_CONFIG, "unknown DIVA boundary condition %d", bcs[q]);
@@ -354,6 +399,12 @@ void check_cfg(const imau_cfg *cfg) {
     fail(IMAU_ERR_CONFIG, "ksp_type must be 0 (BiCGStab) or 1 (GCR), got %d", cfg->ksp_type);
   if (cfg->mg_smoother != 0 && cfg->mg_smoother != 1)
     fail(IMAU_ERR_CONFIG, "mg_smoother must be 0 (fused sweep) or 1 (one launch per colour), got %d", cfg->mg_smoother);
+  if (cfg->stall_window < 0 || cfg->stall_window > 1000)
+    fail(IMAU_ERR_CONFIG, "stall_window must lie in [0, 1000], got %d", cfg->stall_window);
+  if (cfg->stall_window > 0 && !(cfg->stall_drop > 1.0))
+    fail(IMAU_ERR_CONFIG, "stall_drop must be > 1, got %g", cfg->stall_drop);
+  if (!(cfg->stall_relax >= 0.0 && cfg->stall_relax <= 1.0))
+    fail(IMAU_ERR_CONFIG, "stall_relax must lie in [0, 1], got %g", cfg->stall_relax);
   if (cfg->mg_levels != 0 && cfg->mg_levels != 1)
     fail(IMAU_ERR_CONFIG, "mg_levels must be 0 (multigrid, automatic depth) or 1 (no multigrid), got %d",
          cfg->mg_levels);
@@ -597,6 +648,9 @@ void imau_vel_default_config(imau_cfg *c) {
   c->anderson_beta = 1.0;
   c->anderson_restart = 1.5;
   c->warm_column_iters = 0;
+  c->stall_window = 0;
+  c->stall_drop = 10.0;
+  c->stall_relax = 0.0;
 }
 
 int imau_vel_init(const imau_grid *grid, const imau_cfg *cfg, imau_ctx **out) {
@@ -681,6 +735,7 @@ int imau_vel_problem_create(imau_ctx *ctx, int32_t ngrids, const imau_grid *grid
   P->na = off;
   P->sheet_any.assign(ngrids, 0);
   P->picard_its.assign(ngrids, 0);
+  P->restarts.assign(ngrids, 0);
   P->sweeps.assign(ngrids, 0);
   P->resid_uv.assign(ngrids, 0.0);
   P->nl_resid.assign(ngrids, NAN);
@@ -701,6 +756,8 @@ int imau_vel_problem_create(imau_ctx *ctx, int32_t ngrids, const imau_grid *grid
   P->act_lin.alloc(ngrids);
   P->sc.alloc((size_t)8 * ngrids);
   P->relax_d.alloc(ngrids);
+  P->restart_d.alloc(ngrids);
+  P->stop_d.alloc(ngrids);
   HIPCHK(hipMemcpy(P->mem_d.p, P->mem.data(), ngrids * sizeof(Member), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(P->tiles_d.p, P->tiles.data(), P->tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(P->tstart_d.p, P->tstart.data(), P->tstart.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -722,6 +779,12 @@ int imau_vel_problem_create(imau_ctx *ctx, int32_t ngrids, const imau_grid *grid
 }
 
 void imau_vel_problem_destroy(imau_problem *P) { delete P; }
+
+int imau_vel_problem_restarts(imau_ctx *ctx, const imau_problem *P, int32_t *restarts) {
+  if (!ctx || !P || !restarts) return IMAU_ERR_ARG;
+  for (int g = 0; g < P->ngrids; ++g) restarts[g] = P->restarts.empty() ? 0 : P->restarts[g];
+  return IMAU_OK;
+}
 
 int imau_vel_problem_stats(imau_ctx *ctx, const imau_problem *P, int32_t *picard_iterations,
                            double *nonlinear_residual) {
@@ -794,7 +857,7 @@ struct Solver {
     HIPCHK(hipMemcpyAsync(dev, stage.data(), stage.size() * sizeof(int), hipMemcpyHostToDevice, c->s));
   }
   void kcheck() { HIPCHK(hipGetLastError()); }
-  imau::Launch LV() const { return imau::Launch{c->s, P->nvtiles, P->mem_d.p, P->vtiles.p}; }
+  imau::Launch LV() const { return imau::Launch{c->s, P->nvtiles, P->mem_d.p, P->vtiles.p, P->nmtiles, P->mtiles.p}; }
 
   std::vector<double> r_abs, b_abs;  // last residual evaluation: ||r||, ||b|| per member
 
@@ -1195,7 +1258,7 @@ struct Solver {
       }
       if (!mixed) {  // relaxation step u_k + rel (G(u_k) - u_k)
         W[slot] = 1.0;
-        W[IMAU_AND_SLOTS + slot] = rel;
+        W[IMAU_AND_SLOTS + slot] = relaxed_ ? relax_h_[g] : rel;
         and_since_[g] = and_count_;
       }
     }
@@ -1215,12 +1278,70 @@ struct Solver {
   std::vector<double> relax_h_;
   long long mg_solves_ = 0;  // multigrid-preconditioned linear solves in this solve
 
+  // stall restarts (imau_cfg.stall_window): called with the residuals r of the
+  // current iterate, before the convergence check of this Picard iteration
+  std::vector<std::vector<double>> stall_hist_;
+  std::vector<double> *prev_r_ = nullptr;
+  std::vector<int> restart_flag_, stop_flag_;
+  bool relaxed_ = false;  // per-member relaxation factors in use (adaptive mode or a restart)
+  template <class VB>
+  void stall_restarts(std::vector<double> &r, bool diva, VB &visc_beta, float *cur32) {
+    const int G = P->ngrids, Wn = cfg->stall_window;
+    std::vector<int> rs(G, 0);
+    bool any = false;
+    for (int g = 0; g < G; ++g) {
+      if (!act_pic[g] || !(r[g] > cfg->nonlinear_rtol)) continue;
+      std::vector<double> &h = stall_hist_[g];
+      h.push_back(r[g]);
+      if (P->restarts[g] > 0) continue;  // a member restarts at most once
+      const int n = (int)h.size();
+      // the first residual of zero velocities is 1: a start no better than that is dropped at once
+      const bool stall = (n == 1 && !(r[g] < 1.0)) || (n > Wn && r[g] > h[n - 1 - Wn] / cfg->stall_drop);
+      if (stall) { rs[g] = 1; any = true; }
+    }
+    if (!any) return;
+    HIPCHK(hipStreamSynchronize(c->s));  // restart_flag_ is reused below
+    restart_flag_ = rs;
+    HIPCHK(hipMemcpyAsync(P->restart_d.p, restart_flag_.data(), G * sizeof(int), hipMemcpyHostToDevice, c->s));
+    imau::reset_members(L, kp, P->F, imau::ResetSrc{P->u0, P->v0, P->tbx0, P->tby0, P->visc0}, P->visc_cur, cur32,
+                        P->restart_d.p);
+    // coefficients and residual of the restarted members at their new state; the
+    // other members keep theirs (the second evaluation leaves them inactive)
+    const std::vector<double> r_keep = r, ra_keep = r_abs, rb_keep = b_abs, l0_keep = lin_r0_;
+    const bool ready = r0_ready_;
+    push_active(P->act_pic.p, rs);
+    visc_beta(false);
+    imau::coef(L, kp, P->F, diva, P->act_pic.p);
+    kcheck();
+    std::vector<double> r2;
+    residual(r2);
+    for (int g = 0; g < G; ++g) {
+      if (rs[g]) {
+        r[g] = r2[g];
+        P->restarts[g] = 1;
+        stall_hist_[g].assign(1, r2[g]);
+        relax_h_[g] = cfg->stall_relax > 0.0 ? cfg->stall_relax : cfg->DIVA_visc_it_relax;
+        and_since_[g] = and_count_;
+        (*prev_r_)[g] = -1.0;
+      } else {
+        r[g] = r_keep[g];
+        r_abs[g] = ra_keep[g];
+        b_abs[g] = rb_keep[g];
+        if (!l0_keep.empty()) lin_r0_[g] = l0_keep[g];
+      }
+    }
+    r0_ready_ = ready;
+    relaxed_ = true;
+    push_active(P->act_pic.p, act_pic);
+  }
+
   void picard(bool diva) {
     const int G = P->ngrids;
     act_pic.assign(G, 0);
     for (int g = 0; g < G; ++g) act_pic[g] = P->sheet_any[g];
     if (!diva && cfg->choice_sliding_law == IMAU_SLID_NO_SLIDING) std::fill(act_pic.begin(), act_pic.end(), 0);
     std::fill(P->picard_its.begin(), P->picard_its.end(), 0);
+    std::fill(P->restarts.begin(), P->restarts.end(), 0);
     std::fill(P->sweeps.begin(), P->sweeps.end(), 0);
     std::fill(P->nl_resid.begin(), P->nl_resid.end(), NAN);
     imau::setup1(L, kp, P->F, P->yg.p);
@@ -1231,6 +1352,11 @@ struct Solver {
     const bool adaptive = cfg->picard_relax_max > 0.0 && cfg->stop_rule == IMAU_STOP_RESIDUAL;
     relax_h_.assign(G, cfg->DIVA_visc_it_relax);
     std::vector<double> prev_r(G, -1.0);
+    prev_r_ = &prev_r;
+    // stall restarts: the FP64 residual history of every member since its start
+    const bool stall_on = cfg->stop_rule == IMAU_STOP_RESIDUAL && cfg->stall_window > 0;
+    stall_hist_.assign(G, std::vector<double>());
+    relaxed_ = adaptive;
     // Anderson acceleration state
     const int and_m = (cfg->stop_rule == IMAU_STOP_RESIDUAL) ? cfg->anderson_m : 0;
     AMix mix{};
@@ -1289,12 +1415,19 @@ struct Solver {
       kcheck();
       if (cfg->stop_rule == IMAU_STOP_RESIDUAL) {
         residual(r);
-        bool left = false;
+        if (stall_on) stall_restarts(r, diva, visc_beta, v32 ? cur32 : nullptr);
+        bool left = false, stopped = false;
+        std::vector<int> stop(G, 0);
         for (int g = 0; g < G; ++g) {
           if (!act_pic[g]) continue;
           P->nl_resid[g] = r[g];
-          if (r[g] <= cfg->nonlinear_rtol || it > cfg->DIVA_visc_it_nit) act_pic[g] = 0;
-          else left = true;
+          if (r[g] <= cfg->nonlinear_rtol || it > cfg->DIVA_visc_it_nit) {
+            act_pic[g] = 0;
+            stop[g] = 1;
+            stopped = true;
+          } else {
+            left = true;
+          }
           if (adaptive) {
             if (prev_r[g] >= 0.0)
               relax_h_[g] = (r[g] < prev_r[g]) ? std::min(cfg->picard_relax_max, 1.25 * relax_h_[g])
@@ -1306,14 +1439,26 @@ struct Solver {
             prev_r[g] = r[g];
           }
         }
+        if (stopped) {
+          // a member that stops returns the state whose residual was evaluated:
+          // its velocities and basal stress with the viscosity they were
+          // computed with (the check's viscosity is that one's next iterate)
+          HIPCHK(hipStreamSynchronize(c->s));  // stop_flag_ is reused below
+          stop_flag_ = stop;
+          HIPCHK(hipMemcpyAsync(P->stop_d.p, stop_flag_.data(), G * sizeof(int), hipMemcpyHostToDevice, c->s));
+          const imau::Launch L3 = L;
+          if (v32) imau::copy3_members(L3, P->nz, oth32, cur32, P->stop_d.p);
+          else imau::copy3_members(L3, P->nz, P->visc_oth, P->visc_cur, P->stop_d.p);
+          kcheck();
+        }
         if (!left) break;
         push_active(P->act_pic.p, act_pic);
       }
       linear_solve();
-      if (adaptive)  // relax_h_ is not modified before the next stream synchronisation
+      if (relaxed_)  // relax_h_ is not modified before the next stream synchronisation
         HIPCHK(hipMemcpyAsync(P->relax_d.p, relax_h_.data(), G * sizeof(double), hipMemcpyHostToDevice, c->s));
       if (and_m > 0) and_step(mix, and_m);
-      imau::post(L, kp, P->F, P->F.xu0, P->F.xv0, P->act_pic.p, diva, P->partial.p, adaptive ? P->relax_d.p : nullptr,
+      imau::post(L, kp, P->F, P->F.xu0, P->F.xv0, P->act_pic.p, diva, P->partial.p, relaxed_ ? P->relax_d.p : nullptr,
                  mix);
       imau::member_sum3(c->s, G, P->tstart_d.p, P->partial.p, P->msum.p);
       kcheck();
@@ -1552,6 +1697,10 @@ int imau_vel_solve_diva_batch(imau_ctx *ctx, int32_t ngrids, const imau_grid *gr
 int imau_vel_nonlinear_residual(imau_ctx *ctx, const imau_grid *grid, const imau_cfg *cfg, imau_ice_io *io,
                                 double *residual) {
   imau_problem *P = nullptr;
+  struct Release {
+    imau_problem *&p;
+    ~Release() { imau_vel_problem_destroy(p); }
+  } release{P};
   ABI_TRY
   if (!ctx || !grid || !cfg || !io || !residual) fail(IMAU_ERR_ARG, "NULL argument");
   int rc = imau_vel_problem_create(ctx, 1, grid, cfg, io, &P);
@@ -1570,14 +1719,304 @@ int imau_vel_nonlinear_residual(imau_ctx *ctx, const imau_grid *grid, const imau
   std::vector<double> r;
   S.residual(r);
   *residual = r[0];
-  imau_vel_problem_destroy(P);
   return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+int imau_vel_problem_autotune(imau_ctx *ctx, imau_problem *P, const imau_cfg *cfg, int32_t *tile_rows) {
+  ABI_TRY
+  if (!ctx || !P || !cfg) fail(IMAU_ERR_ARG, "NULL argument");
+  check_cfg(cfg);
+  if (cfg->nz != P->nz) fail(IMAU_ERR_CONFIG, "cfg nz=%d differs from the problem's nz=%d", cfg->nz, P->nz);
+  const bool diva = cfg->choice_ice_dynamics == IMAU_DYN_DIVA;
+  if (!diva || cfg->nz != IMAU_MNZ) {  // only the DIVA viscosity at nz = 15 has a tunable tile
+    if (tile_rows) *tile_rows = P->mti;
+    return IMAU_OK;
   }
-  catch (const Err &e) {
-    ctx->err = e.msg;
-    imau_vel_problem_destroy(P);
-    return e.code;
+  Solver S(ctx, P, cfg);
+  const bool v32 = cfg->precision == IMAU_PREC_MIXED_VISC;
+  const long long n3 = P->na * (long long)P->nz;
+  if (v32 && !P->visc32.p) P->visc32.alloc((size_t)3 * n3);
+  if (v32) {
+    imau::to_f32(ctx->s, P->F.B3, P->visc32.p + 2 * n3, n3);
+    imau::to_f32(ctx->s, P->visc_cur, P->visc32.p, n3);
   }
+  imau::setup1(S.L, S.kp, P->F, P->yg.p);
+  imau::fill_i32(ctx->s, P->act_pic.p, P->ngrids, 1);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const int cand[] = IMAU_MTI_CANDIDATES;
+  int best = P->mti;
+  float best_ms = 0.0f;
+  for (int rows : cand) {
+    set_mtiles(P, rows);
+    auto run = [&]() {
+      if (v32) imau::visc3d32(S.L, S.LV(), S.kp, P->F, true, P->visc32.p, P->visc32.p + n3, P->visc32.p + 2 * n3,
+                              P->act_pic.p);
+      else imau::visc3d(S.L, S.LV(), S.kp, P->F, true, P->visc_cur, P->visc_oth, P->act_pic.p);
+    };
+    run();  // warm-up
+    HIPCHK(hipEventRecord(e0, ctx->s));
+    for (int rep = 0; rep < 3; ++rep) run();
+    HIPCHK(hipEventRecord(e1, ctx->s));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (best_ms == 0.0f || ms < best_ms) { best_ms = ms; best = rows; }
+  }
+  HIPCHK(hipEventDestroy(e0));
+  HIPCHK(hipEventDestroy(e1));
+  set_mtiles(P, best);
+  HIPCHK(hipGetLastError());
+  if (tile_rows) *tile_rows = best;
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+int imau_vel_problem_set_tile_rows(imau_ctx *ctx, imau_problem *P, int32_t tile_rows) {
+  ABI_TRY
+  if (!ctx || !P) fail(IMAU_ERR_ARG, "NULL argument");
+  const int cand[] = IMAU_MTI_CANDIDATES;
+  bool ok = false;
+  for (int rows : cand) ok = ok || rows == tile_rows;
+  if (!ok) fail(IMAU_ERR_ARG, "tile_rows=%d is not one of the viscosity kernel's tile heights", (int)tile_rows);
+  set_mtiles(P, tile_rows);
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// the ice-thickness update (include/imau_vel.h: imau_thickness_*)
+// ---------------------------------------------------------------------------
+namespace {
+
+void check_thickness_cfg(const imau_thickness_cfg *c) {
+  if (!c) fail(IMAU_ERR_ARG, "thickness cfg is NULL");
+  if (c->choice_ice_integration_method < IMAU_DHI_NONE || c->choice_ice_integration_method > IMAU_DHI_SEMIIMPLICIT)
+    fail(IMAU_ERR_CONFIG, "unknown choice_ice_integration_method %d", c->choice_ice_integration_method);
+  const int *bcs = &c->ice_thickness_west_BC;
+  for (int q = 0; q < 4; ++q)
+    if (bcs[q] < IMAU_HBC_NONE || bcs[q] > IMAU_HBC_FIXED) fail(IMAU_ERR_CONFIG, "unknown ice_thickness BC %d", bcs[q]);
+  if (c->dHi_choice_matrix_solver != IMAU_SOLVER_SOR && c->dHi_choice_matrix_solver != IMAU_SOLVER_PETSC)
+    fail(IMAU_ERR_CONFIG, "unknown dHi_choice_matrix_solver %d", c->dHi_choice_matrix_solver);
+}
+
+// device buffers of one grid (a-grid-shaped slabs, offset 0)
+struct ThkDev {
+  DBuf<double> f;    // 8 inputs, 2 outputs, 11 work vectors, 6 rows, x
+  DBuf<int32_t> m;   // 4 masks
+  DBuf<double> partial, sums;
+  imau::ThkIn in{};
+  imau::ThkWork w{};
+  imau::ThkRows R{};
+  double *dHi = nullptr, *Ht = nullptr, *x = nullptr, *sc = nullptr;
+  long long na = 0;
+  int nblocks = 0;
+  void alloc(const Member &mb) {
+    na = (long long)mb.nx * mb.ny;
+    nblocks = imau::thk_blocks(mb);
+    f.alloc((size_t)28 * na);
+    m.alloc((size_t)4 * na);
+    partial.alloc((size_t)2 * nblocks + 8);
+    sums.alloc(16);
+    double *q = f.p;
+    auto take = [&]() { double *r = q; q += na; return r; };
+    double *Hi = take(), *Hb = take(), *SL = take(), *u = take(), *v = take(), *SMB = take(), *BMB = take(),
+           *fmf = take();
+    in = imau::ThkIn{Hi, Hb, SL, u, v, SMB, BMB, fmf, m.p, m.p + na, m.p + 2 * na, m.p + 3 * na};
+    dHi = take();
+    Ht = take();
+    w.Qx = take(); w.Qy = take(); w.t0 = take(); w.t1 = take();
+    w.r = take(); w.rhat = take(); w.p = take(); w.phat = take(); w.v = take(); w.shat = take(); w.t = take();
+    R.d = take(); R.cw = take(); R.ce = take(); R.cs = take(); R.cn = take(); R.b = take();
+    x = take();
+    w.partial = partial.p;
+    sc = sums.p + 8;
+  }
+};
+
+// the update on the device buffers, then the download into io (Qx/Qy of the explicit scheme)
+void thickness_run(imau_ctx *ctx, const Member &mb, ThkDev &D, const imau_thickness_cfg *c, imau_thickness_io *io,
+                   double dt) {
+  hipStream_t s = ctx->s;
+  const size_t d8 = sizeof(double);
+  const long long na = D.na;
+  const int method = c->choice_ice_integration_method;
+  if (method == IMAU_DHI_NONE) {
+    HIPCHK(hipMemsetAsync(D.dHi, 0, na * d8, s));
+    HIPCHK(hipMemcpyAsync(D.Ht, D.in.Hi, na * d8, hipMemcpyDeviceToDevice, s));
+  } else if (method == IMAU_DHI_EXPLICIT) {
+    imau::thk_explicit(s, mb, D.in, dt, D.w, D.dHi, D.Ht);
+  } else {
+    imau::thk_rows(s, mb, D.in, dt, D.R);
+    HIPCHK(hipMemcpyAsync(D.x, D.in.Hi, na * d8, hipMemcpyDeviceToDevice, s));
+    double h[4];
+    auto fetch = [&](int n) {
+      HIPCHK(hipMemcpyAsync(h, D.sums.p, n * d8, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+    };
+    double scal[4] = {1.0, 1.0, 1.0, 0.0};  // rho_old, alpha, omega, rho
+    auto push = [&]() {
+      HIPCHK(hipMemcpyAsync(D.sc, scal, sizeof scal, hipMemcpyHostToDevice, s));
+      HIPCHK(hipStreamSynchronize(s));  // scal is reused on the host
+    };
+    imau::thk_r0(s, mb, D.R, D.x, D.w, D.sums.p);
+    fetch(2);
+    const bool petsc = c->dHi_choice_matrix_solver == IMAU_SOLVER_PETSC;
+    const double target = std::max(c->dHi_PETSc_rtol * std::sqrt(h[1]), c->dHi_PETSc_abstol);
+    const int maxit = petsc ? 10000 : std::max(1, c->dHi_SOR_nit);
+    bool done = petsc && std::sqrt(h[0]) <= target;
+    scal[3] = h[0];  // (rhat, r) = |r|^2 at the start
+    for (int it = 0; it < maxit && !done; ++it) {
+      push();
+      imau::thk_bicg_p(s, mb, D.R, D.w, D.sc, D.sums.p);
+      fetch(1);
+      if (!(std::fabs(h[0]) > 0.0)) break;  // breakdown: the residual is already zero in exact arithmetic
+      scal[1] = scal[3] / h[0];
+      push();
+      imau::thk_bicg_s(s, mb, D.R, D.w, D.sc, D.sums.p);
+      fetch(2);
+      scal[2] = h[1] > 0.0 ? h[0] / h[1] : 0.0;
+      push();
+      imau::thk_bicg_x(s, mb, D.x, D.w, D.sc, D.sums.p);
+      fetch(3);
+      if (!std::isfinite(h[0])) fail(IMAU_ERR_NAN, "non-finite residual in the semi-implicit thickness solve");
+      scal[0] = scal[3];
+      scal[3] = h[1];
+      done = petsc ? std::sqrt(h[0]) <= target : h[2] <= c->dHi_SOR_tol;
+      if (scal[2] == 0.0) break;
+    }
+    imau::thk_finish(s, mb, D.in.Hi, D.x, dt, D.dHi, D.Ht);
+  }
+  imau::thk_bc(s, mb, D.in, &c->ice_thickness_west_BC, c->choice_ice_dynamics == IMAU_DYN_DIVA, dt, D.dHi, D.Ht);
+  HIPCHK(hipGetLastError());
+  const int nx = mb.nx, ny = mb.ny;
+  if (io->dHi_dt_a) HIPCHK(hipMemcpyAsync(io->dHi_dt_a, D.dHi, na * d8, hipMemcpyDeviceToHost, s));
+  if (io->Hi_tplusdt_a) HIPCHK(hipMemcpyAsync(io->Hi_tplusdt_a, D.Ht, na * d8, hipMemcpyDeviceToHost, s));
+  if (method == IMAU_DHI_EXPLICIT) {
+    if (io->Qx_cx) HIPCHK(hipMemcpyAsync(io->Qx_cx, D.w.Qx, (size_t)ny * (nx - 1) * d8, hipMemcpyDeviceToHost, s));
+    if (io->Qy_cy)
+      HIPCHK(hipMemcpy2DAsync(io->Qy_cy, (size_t)(ny - 1) * d8, D.w.Qy, (size_t)ny * d8, (size_t)(ny - 1) * d8, nx,
+                              hipMemcpyDeviceToHost, s));
+  }
+  HIPCHK(hipStreamSynchronize(s));
+}
+
+void thickness_upload_extra(imau_ctx *ctx, ThkDev &D, const imau_thickness_io *io, int nx, int ny) {
+  hipStream_t s = ctx->s;
+  const size_t na = (size_t)nx * ny, d8 = sizeof(double), i4 = sizeof(int32_t);
+  if (!io->SMB_year || !io->BMB || !io->float_margin_frac_a || !io->mask_shelf_a || !io->mask_cf_a)
+    fail(IMAU_ERR_ARG, "thickness update: SMB_year, BMB, float_margin_frac_a, mask_shelf_a and mask_cf_a are required");
+  HIPCHK(hipMemcpyAsync((void *)D.in.SMB, io->SMB_year, na * d8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync((void *)D.in.BMB, io->BMB, na * d8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync((void *)D.in.fmf, io->float_margin_frac_a, na * d8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync((void *)D.in.m_shelf, io->mask_shelf_a, na * i4, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync((void *)D.in.m_cf, io->mask_cf_a, na * i4, hipMemcpyHostToDevice, s));
+}
+
+}  // namespace
+
+extern "C" {
+
+void imau_thickness_default_config(imau_thickness_cfg *c) {
+  std::memset(c, 0, sizeof *c);
+  c->choice_ice_integration_method = IMAU_DHI_EXPLICIT;
+  c->choice_ice_dynamics = IMAU_DYN_DIVA;
+  c->ice_thickness_west_BC = c->ice_thickness_east_BC = IMAU_HBC_ZERO;
+  c->ice_thickness_south_BC = c->ice_thickness_north_BC = IMAU_HBC_ZERO;
+  c->dHi_choice_matrix_solver = IMAU_SOLVER_SOR;
+  c->dHi_SOR_nit = 3000;
+  c->dHi_SOR_tol = 2.5;
+  c->dHi_SOR_omega = 1.3;
+  c->dHi_PETSc_rtol = 0.001;
+  c->dHi_PETSc_abstol = 0.001;
+  c->dt_max = 10.0;
+}
+
+int imau_thickness_update(imau_ctx *ctx, const imau_grid *grid, const imau_thickness_cfg *cfg, imau_thickness_io *io,
+                          double dt) {
+  ABI_TRY
+  if (!ctx || !grid || !cfg || !io) fail(IMAU_ERR_ARG, "NULL argument");
+  check_thickness_cfg(cfg);
+  if (grid->nx < 4 || grid->ny < 4) fail(IMAU_ERR_ARG, "grid must be at least 4x4");
+  if (!(dt > 0.0)) fail(IMAU_ERR_ARG, "dt must be > 0, got %g", dt);
+  if (!io->Hi_a || !io->Hb_a || !io->SL_a || !io->u_vav_cx || !io->v_vav_cy || !io->mask_ice_a || !io->mask_ocean_a)
+    fail(IMAU_ERR_ARG, "thickness update: a required input is NULL");
+  const int nx = grid->nx, ny = grid->ny;
+  const Member mb{nx, ny, grid->dx, 0, 0, 0, 0};
+  ThkDev D;
+  D.alloc(mb);
+  hipStream_t s = ctx->s;
+  const size_t na = (size_t)nx * ny, d8 = sizeof(double), i4 = sizeof(int32_t);
+  HIPCHK(hipMemcpyAsync((void *)D.in.Hi, io->Hi_a, na * d8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync((void *)D.in.Hb, io->Hb_a, na * d8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync((void *)D.in.SL, io->SL_a, na * d8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync((void *)D.in.u, io->u_vav_cx, (size_t)ny * (nx - 1) * d8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpy2DAsync((void *)D.in.v, (size_t)ny * d8, io->v_vav_cy, (size_t)(ny - 1) * d8, (size_t)(ny - 1) * d8,
+                          nx, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync((void *)D.in.m_ice, io->mask_ice_a, na * i4, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync((void *)D.in.m_ocean, io->mask_ocean_a, na * i4, hipMemcpyHostToDevice, s));
+  thickness_upload_extra(ctx, D, io, nx, ny);
+  thickness_run(ctx, mb, D, cfg, io, dt);
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+int imau_vel_problem_thickness(imau_ctx *ctx, imau_problem *P, int32_t g, const imau_thickness_cfg *cfg,
+                               imau_thickness_io *io, double dt) {
+  ABI_TRY
+  if (!ctx || !P || !cfg || !io) fail(IMAU_ERR_ARG, "NULL argument");
+  check_thickness_cfg(cfg);
+  if (g < 0 || g >= P->ngrids) fail(IMAU_ERR_ARG, "member %d outside [0, %d)", g, P->ngrids);
+  if (!(dt > 0.0)) fail(IMAU_ERR_ARG, "dt must be > 0, got %g", dt);
+  const Member &pm = P->mem[g];
+  const int nx = pm.nx, ny = pm.ny;
+  const Member mb{nx, ny, pm.dx, 0, 0, 0, 0};
+  ThkDev D;
+  D.alloc(mb);
+  hipStream_t s = ctx->s;
+  const size_t na = (size_t)nx * ny, d8 = sizeof(double), i4 = sizeof(int32_t);
+  // the member's slabs, device to device: the solved velocities stay on the GPU
+  HIPCHK(hipMemcpyAsync((void *)D.in.Hi, P->F.Hi + pm.off, na * d8, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync((void *)D.in.Hb, P->F.Hb + pm.off, na * d8, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync((void *)D.in.SL, P->F.SL + pm.off, na * d8, hipMemcpyDeviceToDevice, s));
+  const double *u = (P->dyn_uploaded == IMAU_DYN_DIVA) ? P->F.u : P->O.uvav;
+  const double *v = (P->dyn_uploaded == IMAU_DYN_DIVA) ? P->F.v : P->O.vvav;
+  HIPCHK(hipMemcpyAsync((void *)D.in.u, u + pm.off, na * d8, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync((void *)D.in.v, v + pm.off, na * d8, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync((void *)D.in.m_ice, P->F.m_ice + pm.off, na * i4, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync((void *)D.in.m_ocean, P->F.m_ocean + pm.off, na * i4, hipMemcpyDeviceToDevice, s));
+  thickness_upload_extra(ctx, D, io, nx, ny);
+  thickness_run(ctx, mb, D, cfg, io, dt);
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+int imau_critical_timestep_adv(imau_ctx *ctx, const imau_grid *grid, const double *u_vav_cx, const double *v_vav_cy,
+                               double dt_max, double *dt_crit_adv) {
+  ABI_TRY
+  if (!ctx || !grid || !u_vav_cx || !v_vav_cy || !dt_crit_adv) fail(IMAU_ERR_ARG, "NULL argument");
+  const int nx = grid->nx, ny = grid->ny;
+  const Member mb{nx, ny, grid->dx, 0, 0, 0, 0};
+  const size_t na = (size_t)nx * ny, d8 = sizeof(double);
+  const int nb = imau::thk_blocks(mb);
+  DBuf<double> buf;
+  buf.alloc(2 * na + nb + 1);
+  hipStream_t s = ctx->s;
+  HIPCHK(hipMemcpyAsync(buf.p, u_vav_cx, (size_t)ny * (nx - 1) * d8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpy2DAsync(buf.p + na, (size_t)ny * d8, v_vav_cy, (size_t)(ny - 1) * d8, (size_t)(ny - 1) * d8, nx,
+                          hipMemcpyHostToDevice, s));
+  imau::cfl(s, mb, buf.p, buf.p + na, buf.p + 2 * na, buf.p + 2 * na + nb);
+  HIPCHK(hipGetLastError());
+  double m = 0.0;
+  HIPCHK(hipMemcpyAsync(&m, buf.p + 2 * na + nb, d8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  *dt_crit_adv = std::min(dt_max, std::min(2.0 * dt_max, m) * 0.9);
+  return IMAU_OK;
+  ABI_CATCH(ctx)
 }
 
 }  // extern "C"

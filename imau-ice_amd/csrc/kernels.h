@@ -9,6 +9,10 @@ struct Launch {
   int ntiles;
   const Member *mem;
   const Tile *tiles;
+  // the 3D viscosity launch also carries the tiles of its row-marching
+  // kernel (one wavefront per tile); none: the tiled kernel runs
+  int nmtiles = 0;
+  const Tile *mtiles = nullptr;
 };
 
 // coefficient pointers of the matrix-free operator
@@ -60,6 +64,17 @@ void resid_init(const Launch &L, const KP &kp, const Coef &c, const double *xu, 
                 double *x0v, double *ru, double *rv, const int *active, double *partial);
 void member_sum2(hipStream_t s, int ngrids, const int *tstart, const double *partial, double *out);
 void fill_u64(hipStream_t s, unsigned long long *p, int n, unsigned long long v);
+// dst = src on the 3D slabs of the members with flag[g] != 0
+void copy3_members(const Launch &L, int nz, const double *src, double *dst, const int *flag);
+void copy3_members(const Launch &L, int nz, const float *src, float *dst, const int *flag);
+// the state a problem was created with (velocities, basal stress, 3D viscosity)
+struct ResetSrc {
+  const double *u, *v, *tbx, *tby, *visc;
+};
+// reset the members with flag[g] != 0 to that state (visc32: also the FP32 viscosity, or null)
+void reset_members(const Launch &L, const KP &kp, const Fields &f, const ResetSrc &src, double *visc, float *visc32,
+                   const int *flag);
+void fill_i32(hipStream_t s, int *p, int n, int v);
 void F1(const Launch &L, const KP &kp, const Fields &f, const double *visc, const OutF &o);
 void uv3d(const Launch &L, const KP &kp, const Fields &f, const OutF &o);
 void compose(const Launch &L, const KP &kp, const Fields &f, const OutF &o, int mode);

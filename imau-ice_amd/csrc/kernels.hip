@@ -197,8 +197,9 @@ __global__ void __launch_bounds__(IMAU_WG) k_setup1(const Member *__restrict__ m
       } else if (kp.prough == 1) {
         f.tauc[p] = schoof_tauc(fabs(f.dHs_dx_a[p]), Hi, kp.ssa_L, kp.ssa_m, yg[m.offy + j]);
       } else {
+        // MISMIP+ (:846-899): Weertman sets beta_sq only, Tsai2015 / Schoof2005 both
         f.beta_sq[p] = 1.0e4;
-        f.alpha_sq[p] = 0.5;
+        if (kp.slid != 4) f.alpha_sq[p] = 0.5;
       }
     }
     // grounded-fraction quadrants from the edge-clamped TAF field, :420-520
@@ -484,6 +485,121 @@ __global__ void __launch_bounds__(IMAU_WG) k_visc(const Member *__restrict__ mem
     }
     // F2 integral (calc_F_integral, :960-1013)
     if (DIVA) f.F2[p] = (mi == 1) ? fmax(kp.F_int_min, (double)F2int) : ((kp.margin == 0) ? 0.0 : kp.F_int_min);
+  }
+}
+
+// The DIVA viscosity kernel for nz = IMAU_MNZ, marching along i: one
+// wavefront per tile of t.pad rows x 64 columns, lane = column j.  The
+// previous viscosity of rows i-1, i and i+1 (all levels, plus the columns
+// j0-1 and j0+64 loaded by the edge lanes) sits in an LDS ring of three rows,
+// and the tile advances one row per step, loading only row i+1: each level of
+// visc_in is read once per row plus two halo rows per tile, instead of five
+// times (the centre and four neighbours) as in k_visc.  Arithmetic and results
+// are those of k_visc<true, ...> exactly.
+template <bool THIRD, class T>
+__global__ void __launch_bounds__(64) k_visc_march(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                   const KP kp, const Fields f, const T *__restrict__ visc_in,
+                                                   T *__restrict__ visc_out, const T *__restrict__ B3,
+                                                   const int *__restrict__ active) {
+  constexpr int NZ = IMAU_MNZ, W = 66;
+  __shared__ T ring[3][NZ][W];
+  const Tile t = tiles[blockIdx.x];
+  const Member m = mem[t.g];
+  const int lane = (int)threadIdx.x;
+  const int j = t.j0 + lane;
+  const int nx = m.nx, ny = m.ny;
+  const bool jin = j < ny;
+  const int i1 = min(t.i0 + t.pad, nx);  // t.pad: the tile's rows (the autotuned tile height)
+  if (!active[t.g]) {  // finished member: keep its viscosity in the buffer that becomes current
+    if (!jin) return;
+    for (int i = t.i0; i < i1; ++i)
+      for (int k = 0; k < NZ; ++k) visc_out[p3(m, k, j, i)] = visc_in[p3(m, k, j, i)];
+    return;
+  }
+  // row r of visc_in into ring slot s: own column (0 beyond the grid) and the halo columns
+  auto load_row = [&](int r, int sl) {
+    const bool rin = r >= 0 && r < nx;
+    for (int k = 0; k < NZ; ++k) {
+      ring[sl][k][lane + 1] = (rin && jin) ? visc_in[p3(m, k, j, r)] : (T)0.0;
+      if (lane == 0) ring[sl][k][0] = (rin && j > 0) ? visc_in[p3(m, k, j - 1, r)] : (T)0.0;
+      if (lane == 63) ring[sl][k][W - 1] = (rin && j + 1 < ny) ? visc_in[p3(m, k, j + 1, r)] : (T)0.0;
+    }
+  };
+  int sp = 0, sc = 1, sn = 2;
+  load_row(t.i0 - 1, sp);
+  load_row(t.i0, sc);
+  const T expo = (T)((1.0 - kp.n_flow) / (2.0 * kp.n_flow));
+  const T vmin = (T)kp.visc_min;
+#pragma unroll 1
+  for (int i = t.i0; i < i1; ++i) {
+    load_row(i + 1, sn);
+    __syncthreads();
+    if (jin) {
+      const long long p = p2(m, j, i);
+      const double Hi = f.Hi[p];
+      const int mi = f.m_ice[p];
+      const double eps_h = f.eps_h[p];
+      const bool bc_skip = (kp.margin == 0 && mi == 0);
+      // neighbour data for the vertical shear terms (:691-692, :704-705, :775-788)
+      const int icl = (i == 0) ? 0 : (i == nx - 1 ? nx - 2 : i - 1);
+      const int icr = (i == 0) ? 0 : (i == nx - 1 ? nx - 2 : i);
+      const int jcl = (j == 0) ? 0 : (j == ny - 1 ? ny - 2 : j - 1);
+      const int jcr = (j == 0) ? 0 : (j == ny - 1 ? ny - 2 : j);
+      const bool use_avg_x = (i > 0 && i < nx - 1), use_avg_y = (j > 0 && j < ny - 1);
+      const T tbl = (T)f.taub_cx[p2(m, j, icl)], tbr = (T)f.taub_cx[p2(m, j, icr)];
+      const T tcl = (T)f.taub_cy[p2(m, jcl, i)], tcr = (T)f.taub_cy[p2(m, jcr, i)];
+      const double Hl0 = f.Hi[p2(m, j, icl)], Hl1 = f.Hi[p2(m, j, icl + 1)];
+      const double Hr0 = f.Hi[p2(m, j, icr)], Hr1 = f.Hi[p2(m, j, icr + 1)];
+      const double Hb0 = f.Hi[p2(m, jcl, i)], Hb1 = f.Hi[p2(m, jcl + 1, i)];
+      const double Ht0 = f.Hi[p2(m, jcr, i)], Ht1 = f.Hi[p2(m, jcr + 1, i)];
+      const bool noice_l = f.m_ice[p2(m, j, icl)] == 0 && f.m_ice[p2(m, j, icl + 1)] == 0;
+      const bool noice_r = f.m_ice[p2(m, j, icr)] == 0 && f.m_ice[p2(m, j, icr + 1)] == 0;
+      const bool noice_b = f.m_ice[p2(m, jcl, i)] == 0 && f.m_ice[p2(m, jcl + 1, i)] == 0;
+      const bool noice_t = f.m_ice[p2(m, jcr, i)] == 0 && f.m_ice[p2(m, jcr + 1, i)] == 0;
+      const double Hxl = use_avg_x ? f.Hi[p2(m, j, i - 1)] : 0.0;
+      const double Hyl = use_avg_y ? f.Hi[p2(m, j - 1, i)] : 0.0;
+      const int sl = stag_sel(Hl0, Hl1), sr = stag_sel(Hr0, Hr1), sb = stag_sel(Hb0, Hb1), st = stag_sel(Ht0, Ht1);
+      const int sxa = stag_sel(Hxl, Hi), sya = stag_sel(Hyl, Hi);
+      const T HiT = (T)Hi, eps_hT = (T)eps_h;
+      T vint = (T)0.0, F2int = (T)0.0, eta_prev_k = (T)0.0, g_prev_k = (T)0.0;
+#pragma unroll 1
+      for (int k = 0; k < NZ; ++k) {
+        const T vc = ring[sc][k][lane + 1], vw = ring[sp][k][lane + 1], ve = ring[sn][k][lane + 1];
+        const T vs = ring[sc][k][lane], vn = ring[sc][k][lane + 2];
+        const T z = (T)kp.zeta[k];
+        T dl = (T)0.0, dr = (T)0.0, db = (T)0.0, dt = (T)0.0;
+        if (!noice_l) dl = (tbl / stag_pick<T>((i == 0) ? vc : vw, (i == 0) ? ve : vc, sl)) * z;
+        if (!noice_r) dr = (tbr / stag_pick<T>((i == nx - 1) ? vw : vc, (i == nx - 1) ? vc : ve, sr)) * z;
+        if (!noice_b) db = (tcl / stag_pick<T>((j == 0) ? vc : vs, (j == 0) ? vn : vc, sb)) * z;
+        if (!noice_t) dt = (tcr / stag_pick<T>((j == ny - 1) ? vs : vc, (j == ny - 1) ? vc : vn, st)) * z;
+        const T du_dz_a = use_avg_x ? stag_pick<T>(dl, dr, sxa) : dl;
+        const T dv_dz_a = use_avg_y ? stag_pick<T>(db, dt, sya) : db;
+        const T eps_sq = eps_hT + (T)0.25 * (du_dz_a * du_dz_a + dv_dz_a * dv_dz_a);
+        const T B = B3[p3(m, k, j, i)];
+        T eta = THIRD ? B / cbrt(eps_sq) : B * pow(eps_sq, expo);
+        eta = fmax(eta, vmin);  // NaN (cold start 0/0) -> visc_eff_min
+        if (bc_skip) eta = (T)0.0;
+        visc_out[p3(m, k, j, i)] = eta;
+        const T gk = (HiT / eta) * (z * z);
+        if (k > 0) {
+          const T dz = z - (T)kp.zeta[k - 1];
+          vint += (T)0.5 * (eta + eta_prev_k) * dz;
+          F2int += (T)0.5 * (gk + g_prev_k) * dz;
+        }
+        eta_prev_k = eta;
+        g_prev_k = gk;
+      }
+      if (!bc_skip) {
+        f.visc_int[p] = (double)vint;
+        f.N_a[p] = (double)vint * ((kp.margin == 0) ? Hi : fmax(0.1, Hi));
+      }
+      f.F2[p] = (mi == 1) ? fmax(kp.F_int_min, (double)F2int) : ((kp.margin == 0) ? 0.0 : kp.F_int_min);
+    }
+    __syncthreads();  // slot sp is overwritten by the next step's load
+    const int s0 = sp;
+    sp = sc;
+    sc = sn;
+    sn = s0;
   }
 }
 
@@ -866,6 +982,46 @@ __global__ void __launch_bounds__(IMAU_WG) k_taub(const Member *__restrict__ mem
   }
 }
 
+// reset the flagged members to the state the problem was created with (the
+// stall restart of a warm-started Picard loop): velocities, basal stress and
+// the 3D viscosity (also its FP32 copy when the viscosity runs in FP32)
+__global__ void __launch_bounds__(IMAU_WG) k_reset_members(const Member *__restrict__ mem,
+                                                           const Tile *__restrict__ tiles, const Fields f,
+                                                           imau::ResetSrc src, double *__restrict__ visc,
+                                                           float *__restrict__ visc32, int nz,
+                                                           const int *__restrict__ flag) {
+  TILE_SETUP
+  if (j >= m.ny || !flag[t.g]) return;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= m.nx) break;
+    const long long p = p2(m, j, i);
+    f.u[p] = src.u[p];
+    f.v[p] = src.v[p];
+    f.taub_cx[p] = src.tbx[p];
+    f.taub_cy[p] = src.tby[p];
+    for (int k = 0; k < nz; ++k) {
+      const long long q = p3(m, k, j, i);
+      visc[q] = src.visc[q];
+      if (visc32) visc32[q] = (float)src.visc[q];
+    }
+  }
+}
+
+// dst = src on the 3D slabs of the members with flag[g] != 0
+template <class T>
+__global__ void __launch_bounds__(IMAU_WG) k_copy3_members(const Member *__restrict__ mem,
+                                                           const Tile *__restrict__ tiles, const T *__restrict__ src,
+                                                           T *__restrict__ dst, int nz, const int *__restrict__ flag) {
+  TILE_SETUP
+  if (j >= m.ny || !flag[t.g]) return;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= m.nx) break;
+    for (int k = 0; k < nz; ++k) dst[p3(m, k, j, i)] = src[p3(m, k, j, i)];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // warm-start network I/O: image g of the network is member g's a-grid with
 // rows i (x) and columns j (y), pixel off[g] + i ny + j
@@ -1114,6 +1270,10 @@ __global__ void __launch_bounds__(256) k_member_sum(const int *__restrict__ tsta
 }
 
 __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) p[q] = v;
+}
+__global__ void k_fill_i32(int *p, int n, int v) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q < n) p[q] = v;
 }
@@ -1435,6 +1595,13 @@ void visc3d(const Launch &L, const Launch &LV, const KP &kp, const Fields &f, bo
             double *visc_out, const int *active) {
   // n = 3 (every shipped configuration) takes the cube-root form of the viscosity
   const bool third = kp.n_flow == 3.0;
+  if (diva && kp.nz == IMAU_MNZ && LV.nmtiles > 0) {
+    hipLaunchKernelGGL(k_strain, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, active);
+    auto km = third ? k_visc_march<true, double> : k_visc_march<false, double>;
+    hipLaunchKernelGGL(km, dim3(LV.nmtiles), dim3(64), 0, L.s, L.mem, LV.mtiles, kp, f, visc_in, visc_out, f.B3,
+                       active);
+    return;
+  }
   auto kv = diva ? (third ? k_visc<true, true, double> : k_visc<true, false, double>)
                  : (third ? k_visc<false, true, double> : k_visc<false, false, double>);
   hipLaunchKernelGGL(k_strain, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, active);
@@ -1443,6 +1610,12 @@ void visc3d(const Launch &L, const Launch &LV, const KP &kp, const Fields &f, bo
 void visc3d32(const Launch &L, const Launch &LV, const KP &kp, const Fields &f, bool diva, const float *visc_in,
               float *visc_out, const float *B3, const int *active) {
   const bool third = kp.n_flow == 3.0;
+  if (diva && kp.nz == IMAU_MNZ && LV.nmtiles > 0) {
+    hipLaunchKernelGGL(k_strain, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, active);
+    auto km = third ? k_visc_march<true, float> : k_visc_march<false, float>;
+    hipLaunchKernelGGL(km, dim3(LV.nmtiles), dim3(64), 0, L.s, L.mem, LV.mtiles, kp, f, visc_in, visc_out, B3, active);
+    return;
+  }
   auto kv = diva ? (third ? k_visc<true, true, float> : k_visc<true, false, float>)
                  : (third ? k_visc<false, true, float> : k_visc<false, false, float>);
   hipLaunchKernelGGL(k_strain, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, active);
@@ -1521,6 +1694,19 @@ void member_sum2(hipStream_t s, int ngrids, const int *tstart, const double *par
 }
 void fill_u64(hipStream_t s, unsigned long long *p, int n, unsigned long long v) {
   hipLaunchKernelGGL(k_fill_u64, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+}
+void reset_members(const Launch &L, const KP &kp, const Fields &f, const ResetSrc &src, double *visc,
+                   float *visc32, const int *flag) {
+  hipLaunchKernelGGL(k_reset_members, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, f, src, visc, visc32, kp.nz, flag);
+}
+void copy3_members(const Launch &L, int nz, const double *src, double *dst, const int *flag) {
+  hipLaunchKernelGGL(k_copy3_members<double>, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, src, dst, nz, flag);
+}
+void copy3_members(const Launch &L, int nz, const float *src, float *dst, const int *flag) {
+  hipLaunchKernelGGL(k_copy3_members<float>, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, src, dst, nz, flag);
+}
+void fill_i32(hipStream_t s, int *p, int n, int v) {
+  hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
 }
 void F1(const Launch &L, const KP &kp, const Fields &f, const double *visc, const OutF &o) {
   hipLaunchKernelGGL(k_F1, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, visc, o);

@@ -92,6 +92,14 @@ class ImauIceIO(ctypes.Structure):
     _fields_ = _struct_fields(_TXT, "imau_ice_io", DEFINES)
 
 
+class ImauThicknessCfg(ctypes.Structure):
+    _fields_ = _struct_fields(_TXT, "imau_thickness_cfg", DEFINES)
+
+
+class ImauThicknessIO(ctypes.Structure):
+    _fields_ = _struct_fields(_TXT, "imau_thickness_io", DEFINES)
+
+
 IO_POINTER_FIELDS = [n for n, t in ImauIceIO._fields_ if hasattr(t, "contents")]
 
 # export the enum constants at module level (IMAU_OK, IMAU_DYN_DIVA, ...)

@@ -96,6 +96,9 @@ def default_config(**overrides) -> ImauCfg:
     c.anderson_beta = 1.0
     c.anderson_restart = 1.5
     c.warm_column_iters = 0
+    c.stall_window = 0
+    c.stall_drop = 10.0
+    c.stall_relax = 0.0
     set_config(c, **overrides)
     return c
 
@@ -131,3 +134,36 @@ def set_config(c: ImauCfg, **overrides) -> ImauCfg:
 
 def zeta_of(c: ImauCfg):
     return [c.zeta[k] for k in range(c.nz)]
+
+
+INTEGRATION = {"none": abi.IMAU_DHI_NONE, "explicit": abi.IMAU_DHI_EXPLICIT, "semi-implicit": abi.IMAU_DHI_SEMIIMPLICIT}
+THICKNESS_BC = {"none": abi.IMAU_HBC_NONE, "zero": abi.IMAU_HBC_ZERO, "infinite": abi.IMAU_HBC_INFINITE,
+                "periodic": abi.IMAU_HBC_PERIODIC, "fixed": abi.IMAU_HBC_FIXED}
+
+
+def thickness_config(**overrides):
+    """Reference defaults of the ice-thickness update (src/configuration_module.f90:53,
+    308-314, 325-328); keyword overrides accept enum ints or the reference strings."""
+    c = abi.ImauThicknessCfg()
+    c.choice_ice_integration_method = INTEGRATION["explicit"]
+    c.choice_ice_dynamics = DYNAMICS["DIVA"]
+    for side in ("west", "east", "south", "north"):
+        setattr(c, "ice_thickness_%s_BC" % side, THICKNESS_BC["zero"])
+    c.dHi_choice_matrix_solver = SOLVER["SOR"]
+    c.dHi_SOR_nit = 3000
+    c.dHi_SOR_tol = 2.5
+    c.dHi_SOR_omega = 1.3
+    c.dHi_PETSc_rtol = 0.001
+    c.dHi_PETSc_abstol = 0.001
+    c.dt_max = 10.0
+    for k, v in overrides.items():
+        if isinstance(v, str):
+            table = {"choice_ice_integration_method": INTEGRATION, "choice_ice_dynamics": DYNAMICS,
+                     "dHi_choice_matrix_solver": SOLVER}.get(k, THICKNESS_BC if k.startswith("ice_thickness_") else None)
+            if table is None or v not in table:
+                raise ValueError("unknown choice %r for %s" % (v, k))
+            v = table[v]
+        if not hasattr(c, k):
+            raise AttributeError("imau_thickness_cfg has no field %r" % k)
+        setattr(c, k, v)
+    return c

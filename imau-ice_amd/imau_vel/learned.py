@@ -346,15 +346,25 @@ def forward_flops(plan):
 
 
 class WarmStartSolver:
-    """Learned warm start of a device-resident Problem, then its solve; the
-    cold solve is the fallback whenever a member's FP64 residual stays above
-    cfg.nonlinear_rtol (a stalled or diverged warm-started Picard loop).
+    """Learned warm start of a device-resident Problem, then its solve.
+
+    The plain solve is the fallback at two levels:
+    * inside the Picard loop, per member (imau_cfg.stall_window): a member
+      whose FP64 residual stalls -- it does not fall by stall_drop over
+      stall_window iterations, or the warm start is no better than zero
+      velocities -- restarts from the cold state with the cold solve's
+      relaxation, and its iterations are counted in full;
+    * after the loop: if any member still ends above cfg.nonlinear_rtol, the
+      whole problem is solved again from the cold state.
+    If that cold solve too ends above the target, solve() raises ImauError
+    with IMAU_ERR_STALLED (the result is left in the problem).
 
     Call problem.reset_state() before solve() when the problem holds a solved
     state (the warm start replaces the velocities, the completion passes the
     viscosity and basal stress)."""
 
-    def __init__(self, ctx, problem, net=None, warm_iters=0, column_iters=10, relax=0.7):
+    def __init__(self, ctx, problem, net=None, warm_iters=0, column_iters=10, relax=0.7, stall_window=4,
+                 stall_drop=10.0):
         self.net = net if net is not None else Net.load(ctx)
         self.problem = problem
         self.io = ProblemIO(self.net, problem)
@@ -363,27 +373,41 @@ class WarmStartSolver:
         # relaxation of the unmixed Picard steps (the first, and any after an
         # Anderson restart) from the warm start; the cold fallback keeps cfg's
         self.relax = relax
-        self.fallbacks = 0
+        self.stall_window, self.stall_drop = stall_window, stall_drop
+        self.fallbacks = 0        # whole-problem cold solves after the loop
+        self.restarts = 0         # per-member cold restarts inside the loop
 
     def warm_start(self, cfg):
         self.io.warm_start(cfg)
 
     def solve(self, cfg):
-        saved = cfg.warm_visc_iters, cfg.warm_column_iters, cfg.DIVA_visc_it_relax
+        fields = ("warm_visc_iters", "warm_column_iters", "DIVA_visc_it_relax", "stall_window", "stall_drop",
+                  "stall_relax")
+        saved = {k: getattr(cfg, k) for k in fields}
         cfg.warm_visc_iters, cfg.warm_column_iters = self.warm_iters, self.column_iters
+        if cfg.stop_rule == 1 and self.stall_window > 0:
+            cfg.stall_window, cfg.stall_drop = self.stall_window, self.stall_drop
+            cfg.stall_relax = saved["DIVA_visc_it_relax"]   # a restarted member solves as the cold start does
         if self.relax is not None:
             cfg.DIVA_visc_it_relax = self.relax
         try:
             self.io.warm_start(cfg)
             self.problem.solve(cfg)
         finally:
-            cfg.warm_visc_iters, cfg.warm_column_iters, cfg.DIVA_visc_it_relax = saved
+            for k, v in saved.items():
+                setattr(cfg, k, v)
         its, res = self.problem.stats()
+        self.restarts += sum(self.problem.restarts())
         if cfg.stop_rule == 1 and any(not r <= cfg.nonlinear_rtol for r in res):
             self.fallbacks += 1
             self.problem.reset_state()
             self.problem.solve(cfg)
             its, res = self.problem.stats()
+            bad = [g for g, r in enumerate(res) if not r <= cfg.nonlinear_rtol]
+            if bad:
+                raise solver.ImauError(
+                    -5, "FP64 residual above %g after the cold fallback for member(s) %s (residuals %s)" %
+                    (cfg.nonlinear_rtol, bad, ["%.3e" % res[g] for g in bad]))
         return its, res
 
     def close(self):

@@ -23,7 +23,10 @@ EXPORTED = (
     "imau_vel_problem_create", "imau_vel_problem_solve", "imau_vel_problem_reset_state",
     "imau_vel_problem_download", "imau_vel_problem_destroy", "imau_vel_solve_diva_batch",
     "imau_vel_kernel_timing", "imau_vel_problem_stats",
+    "imau_vel_problem_restarts", "imau_vel_problem_thickness", "imau_vel_problem_autotune", "imau_vel_problem_set_tile_rows",
 )
+# the ice-thickness update that consumes the velocities
+THICKNESS_EXPORTED = ("imau_thickness_default_config", "imau_thickness_update", "imau_critical_timestep_adv")
 
 
 class ImauError(RuntimeError):
@@ -64,6 +67,16 @@ def load_library(path: str = LIB_PATH):
     L.imau_vel_solve_diva_batch.argtypes = [vp, ctypes.c_int32, P(ImauGrid), P(ImauCfg), P(ImauIceIO)]
     L.imau_vel_kernel_timing.argtypes = [vp, ctypes.c_int32, P(ctypes.c_int64), P(ctypes.c_double), P(ctypes.c_int64)]
     L.imau_vel_problem_stats.argtypes = [vp, vp, P(ctypes.c_int32), P(ctypes.c_double)]
+    L.imau_vel_problem_restarts.argtypes = [vp, vp, P(ctypes.c_int32)]
+    L.imau_vel_problem_autotune.argtypes = [vp, vp, P(ImauCfg), P(ctypes.c_int32)]
+    L.imau_vel_problem_set_tile_rows.argtypes = [vp, vp, ctypes.c_int32]
+    L.imau_vel_problem_thickness.argtypes = [vp, vp, ctypes.c_int32, P(abi.ImauThicknessCfg), P(abi.ImauThicknessIO),
+                                             ctypes.c_double]
+    L.imau_thickness_default_config.argtypes = [P(abi.ImauThicknessCfg)]
+    L.imau_thickness_default_config.restype = None
+    L.imau_thickness_update.argtypes = [vp, P(ImauGrid), P(abi.ImauThicknessCfg), P(abi.ImauThicknessIO), ctypes.c_double]
+    L.imau_critical_timestep_adv.argtypes = [vp, P(ImauGrid), P(ctypes.c_double), P(ctypes.c_double), ctypes.c_double,
+                                             P(ctypes.c_double)]
     _lib = L
     return L
 
@@ -117,6 +130,28 @@ class Context:
                                                        ctypes.byref(r)))
         return r.value
 
+    def thickness_update(self, grid, tcfg, ts, dt):
+        """calc_dHi_dt (src/ice_thickness_module.f90:25-58) over one step dt on
+        ThicknessState ts: dHi_dt_a, Hi_tplusdt_a (and the explicit fluxes)."""
+        g = grid.struct()
+        io = ts.io()
+        self._check(self.L.imau_thickness_update(self.h, ctypes.byref(g), ctypes.byref(tcfg), ctypes.byref(io),
+                                                 ctypes.c_double(dt)))
+        return ts
+
+    def critical_timestep_adv(self, grid, u, v, dt_max):
+        """calc_critical_timestep_adv (src/ice_dynamics_module.f90:726-762)."""
+        import numpy as np
+        g = grid.struct()
+        u = np.asfortranarray(u, dtype=np.float64)
+        v = np.asfortranarray(v, dtype=np.float64)
+        out = ctypes.c_double()
+        dp = ctypes.POINTER(ctypes.c_double)
+        self._check(self.L.imau_critical_timestep_adv(self.h, ctypes.byref(g), u.ctypes.data_as(dp),
+                                                      v.ctypes.data_as(dp), ctypes.c_double(dt_max),
+                                                      ctypes.byref(out)))
+        return out.value
+
     def kernel_timing(self, what):
         """(launches, summed ms, summed cells of the processed members) timed since
         the last call; then times `what` next: False/"off", True/"smoother" (one
@@ -165,6 +200,34 @@ class Problem:
         res = (ctypes.c_double * n)()
         self.ctx._check(self.ctx.L.imau_vel_problem_stats(self.ctx.h, self.h, its, res))
         return list(its), list(res)
+
+    def thickness(self, g, tcfg, ts, dt):
+        """The thickness update of member g with the velocities of the last
+        solve, read on the device; ts supplies the other inputs."""
+        io = ts.io()
+        self.ctx._check(self.ctx.L.imau_vel_problem_thickness(self.ctx.h, self.h, g, ctypes.byref(tcfg),
+                                                              ctypes.byref(io), ctypes.c_double(dt)))
+        return ts
+
+    def autotune(self, cfg=None):
+        """Pick the tile height of the 3D viscosity kernel by timing the
+        candidates on this problem; returns the chosen height."""
+        rows = ctypes.c_int32()
+        self.ctx._check(self.ctx.L.imau_vel_problem_autotune(self.ctx.h, self.h, ctypes.byref(cfg or self.cfg),
+                                                             ctypes.byref(rows)))
+        return rows.value
+
+    def set_tile_rows(self, rows):
+        """Fix the tile height of the 3D viscosity kernel (0: tiled kernel;
+        4, 8, 16, 32: row-marching kernel)."""
+        self.ctx._check(self.ctx.L.imau_vel_problem_set_tile_rows(self.ctx.h, self.h, rows))
+
+    def restarts(self):
+        """Stall restarts per member of the last solve (imau_cfg.stall_window)."""
+        n = len(self.grids)
+        out = (ctypes.c_int32 * n)()
+        self.ctx._check(self.ctx.L.imau_vel_problem_restarts(self.ctx.h, self.h, out))
+        return list(out)
 
     def close(self):
         if self.h:

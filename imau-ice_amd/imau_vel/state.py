@@ -142,3 +142,39 @@ _INPUTS = set(list(FIELDS)[: list(FIELDS).index("u_vav_cx")]) | set(STATE_FIELDS
 
 def _is_input(name: str) -> bool:
     return name in _INPUTS
+
+
+THICKNESS_FIELDS = {
+    "Hi_a": ("a", _F64), "Hb_a": ("a", _F64), "SL_a": ("a", _F64), "u_vav_cx": ("cx", _F64), "v_vav_cy": ("cy", _F64),
+    "SMB_year": ("a", _F64), "BMB": ("a", _F64), "float_margin_frac_a": ("a", _F64),
+    "mask_ice_a": ("a", _I32), "mask_shelf_a": ("a", _I32), "mask_ocean_a": ("a", _I32), "mask_cf_a": ("a", _I32),
+    "dHi_dt_a": ("a", _F64), "Hi_tplusdt_a": ("a", _F64), "Qx_cx": ("cx", _F64), "Qy_cy": ("cy", _F64),
+}
+
+
+class ThicknessState:
+    """The fields of one ice-thickness update (imau_thickness_io), Fortran order."""
+
+    def __init__(self, grid: Grid):
+        self.grid = grid
+        self.f = {name: np.zeros(grid_shape(kind, grid.nx, grid.ny), dtype=dt, order="F")
+                  for name, (kind, dt) in THICKNESS_FIELDS.items()}
+
+    def __getitem__(self, k):
+        return self.f[k]
+
+    def __setitem__(self, k, v):
+        self.f[k][...] = v
+
+    def copy(self) -> "ThicknessState":
+        s = ThicknessState(self.grid)
+        for k, v in self.f.items():
+            s.f[k][...] = v
+        return s
+
+    def io(self):
+        io = abi.ImauThicknessIO()
+        for name, arr in self.f.items():
+            ct = ctypes.c_int32 if arr.dtype == np.int32 else ctypes.c_double
+            setattr(io, name, arr.ctypes.data_as(ctypes.POINTER(ct)))
+        return io

@@ -175,6 +175,18 @@ typedef struct imau_cfg {
                               viscosity, F2, beta_eff and |tau_b| iterated
                               this many times in one kernel with each
                               column's own basal stress (0 = none)          */
+  int32_t stall_window;    /* IMAU_STOP_RESIDUAL: a member whose FP64 residual
+                              did not fall by the factor stall_drop over the
+                              last stall_window Picard iterations, or whose
+                              first residual is not below 1 (the residual of
+                              zero velocities), restarts once from the state
+                              the problem was created with (the cold start)
+                              inside the same Picard loop; its iteration count
+                              keeps running.  For a warm-started solve.  0 =
+                              off                                           */
+  double  stall_drop;      /* > 1                                            */
+  double  stall_relax;     /* DIVA_visc_it_relax of a restarted member's
+                              unmixed Picard steps (0 = DIVA_visc_it_relax)  */
 } imau_cfg;
 
 /* The fields of type_ice_model (src/data_types_module.f90:56-325) that the
@@ -286,7 +298,72 @@ int  imau_vel_problem_download(imau_ctx *ctx, imau_problem *prob, imau_ice_io *i
  * nonlinear residual (NaN under IMAU_STOP_PICARD); either pointer may be NULL */
 int  imau_vel_problem_stats(imau_ctx *ctx, const imau_problem *prob, int32_t *picard_iterations,
                             double *nonlinear_residual);
+/* per member of the last solve: the number of stall restarts (0 or 1, see
+ * imau_cfg.stall_window) */
+int  imau_vel_problem_restarts(imau_ctx *ctx, const imau_problem *prob, int32_t *restarts);
 void imau_vel_problem_destroy(imau_problem *prob);
+
+/* No reference counterpart: the project's tile autotuner.  Times the 3D
+ * viscosity kernel of DIVA at nz = 15 on this problem's own grids and state
+ * with HIP events, over its tile heights (0: the tiled kernel, 8 rows per
+ * 256-thread workgroup; 4, 8, 16, 32: the row-marching kernel, that many rows
+ * per 64-lane wavefront), keeps the fastest for the problem's later solves and
+ * returns it in *tile_rows (may be NULL).  The state is not changed; other
+ * configurations keep their fixed tiles.  Every height gives the same
+ * viscosity bit for bit. */
+int  imau_vel_problem_autotune(imau_ctx *ctx, imau_problem *prob, const imau_cfg *cfg, int32_t *tile_rows);
+/* No reference counterpart: sets the tile height the autotuner would choose
+ * (one of 0, 4, 8, 16, 32; IMAU_ERR_ARG otherwise). */
+int  imau_vel_problem_set_tile_rows(imau_ctx *ctx, imau_problem *prob, int32_t tile_rows);
+
+/* ---- ice-thickness update: the consumer of u_vav_cx / v_vav_cy ----
+ * choice_ice_integration_method (src/configuration_module.f90:308) */
+enum { IMAU_DHI_NONE = 0, IMAU_DHI_EXPLICIT = 1, IMAU_DHI_SEMIIMPLICIT = 2 };
+/* ice_thickness_*_BC (src/configuration_module.f90:325-328) */
+enum { IMAU_HBC_NONE = 0, IMAU_HBC_ZERO = 1, IMAU_HBC_INFINITE = 2, IMAU_HBC_PERIODIC = 3, IMAU_HBC_FIXED = 4 };
+
+/* The thickness part of the reference's config struct C
+ * (src/configuration_module.f90:308-314, 325-328, dt_max) */
+typedef struct imau_thickness_cfg {
+  int32_t choice_ice_integration_method;
+  int32_t choice_ice_dynamics;      /* 'zero' boundaries clear two rows under DIVA */
+  int32_t ice_thickness_west_BC, ice_thickness_east_BC, ice_thickness_south_BC, ice_thickness_north_BC;
+  int32_t dHi_choice_matrix_solver; /* IMAU_SOLVER_SOR or IMAU_SOLVER_PETSC (semi-implicit scheme) */
+  int32_t dHi_SOR_nit;
+  double  dHi_SOR_tol, dHi_SOR_omega;
+  double  dHi_PETSc_rtol, dHi_PETSc_abstol;
+  double  dt_max;
+} imau_thickness_cfg;
+
+/* The fields calc_dHi_dt reads and writes (type_ice_model, type_SMB_model,
+ * type_BMB_model; src/data_types_module.f90), host arrays in the layouts above. */
+typedef struct imau_thickness_io {
+  const double  *Hi_a, *Hb_a, *SL_a;                        /* (ny,nx) */
+  const double  *u_vav_cx, *v_vav_cy;                       /* (ny,nx-1), (ny-1,nx) */
+  const double  *SMB_year, *BMB, *float_margin_frac_a;      /* (ny,nx) */
+  const int32_t *mask_ice_a, *mask_shelf_a, *mask_ocean_a, *mask_cf_a;
+  double *dHi_dt_a, *Hi_tplusdt_a;                          /* (ny,nx) out */
+  double *Qx_cx, *Qy_cy;                                    /* explicit scheme's fluxes (m^3 per step), out, may be NULL */
+} imau_thickness_io;
+
+void imau_thickness_default_config(imau_thickness_cfg *cfg);
+
+/* Replaces calc_dHi_dt (src/ice_thickness_module.f90:25-58: the explicit
+ * scheme :59-312 or the semi-implicit one :313-718, then
+ * apply_ice_thickness_BC :719-909) over one time step dt [yr]. */
+int  imau_thickness_update(imau_ctx *ctx, const imau_grid *grid, const imau_thickness_cfg *cfg,
+                           imau_thickness_io *io, double dt);
+
+/* The same with the velocities of a device-resident problem's member g (the
+ * result of its last solve) instead of io->u_vav_cx / v_vav_cy, so they need
+ * not leave the device; io holds the other fields of member g. */
+int  imau_vel_problem_thickness(imau_ctx *ctx, imau_problem *prob, int32_t g, const imau_thickness_cfg *cfg,
+                                imau_thickness_io *io, double dt);
+
+/* Replaces calc_critical_timestep_adv (src/ice_dynamics_module.f90:726-762):
+ * min(dt_max, 0.9 min over interior points of dx / (|u| + |v|)). */
+int  imau_critical_timestep_adv(imau_ctx *ctx, const imau_grid *grid, const double *u_vav_cx,
+                                const double *v_vav_cy, double dt_max, double *dt_crit_adv);
 
 /* ---- ensembles (no reference counterpart: the reference solves one region
  *      grid per call, src/IMAU_ICE_program.f90:247-250) ----

@@ -846,8 +846,9 @@ static void calc_basal_conditions(S *s)
         orc_ssa_schoof2006(C->n_flow, C->SSA_icestream_L, C->SSA_icestream_m, fabs(io->dHs_dx_a[p]), io->Hi_a[p],
                            io->A_flow_vav_a[p], g->y[j], &U, &s->tauc[p]);
       } else {
+        /* src/basal_conditions_and_sliding_module.f90:846-899: Weertman sets beta_sq only */
         s->beta_sq[p] = 1.0e4;
-        s->alpha_sq[p] = 0.5;
+        if (C->choice_sliding_law != IMAU_SLID_WEERTMAN) s->alpha_sq[p] = 0.5;
       }
     }
 }
@@ -1537,10 +1538,27 @@ static int sum_sheet(const S *s)
   return t > 0;
 }
 
+/* the configuration checks of calc_bed_roughness_Martin2011 / _MISMIPplus
+ * (src/basal_conditions_and_sliding_module.f90:789-792, 896-898) */
+static int check_roughness(const imau_cfg *c)
+{
+  int law = c->choice_sliding_law;
+  if (c->choice_basal_roughness != IMAU_ROUGH_PARAMETERISED || law == IMAU_SLID_NO_SLIDING || law == IMAU_SLID_IDEALISED)
+    return 0;
+  if (c->choice_param_basal_roughness == IMAU_PROUGH_MARTIN2011 && law != IMAU_SLID_COULOMB &&
+      law != IMAU_SLID_COULOMB_REGULARISED && law != IMAU_SLID_ZOET_IVERSON)
+    return fail(IMAU_ERR_CONFIG, "Martin2011 roughness needs Coulomb, Coulomb_regularised or Zoet-Iverson sliding");
+  if (c->choice_param_basal_roughness == IMAU_PROUGH_MISMIPPLUS && law != IMAU_SLID_WEERTMAN &&
+      law != IMAU_SLID_TSAI2015 && law != IMAU_SLID_SCHOOF2005)
+    return fail(IMAU_ERR_CONFIG, "MISMIP+ roughness needs Weertman, Tsai2015 or Schoof2005 sliding");
+  return 0;
+}
+
 /* src/ice_velocity_module.f90:249-374 */
 int orc_solve_diva(const imau_grid *gr, const imau_cfg *c, imau_ice_io *io, int lin)
 {
   if (c->choice_ice_dynamics != IMAU_DYN_DIVA) return fail(IMAU_ERR_CONFIG, "solve_DIVA: choice_ice_dynamics is not DIVA");
+  if (check_roughness(c)) return IMAU_ERR_CONFIG;
   S s;
   st_init(&s, gr, c, io, 0);
   const G *g = &s.g;
@@ -1557,8 +1575,13 @@ int orc_solve_diva(const imau_grid *gr, const imau_cfg *c, imau_ice_io *io, int 
   determine_grounded_fractions(g, io->TAF_a, s.fg_a, s.fg_cx, s.fg_cy, s.fg_b);
   int it = 0, rc = 0;
   double resid = 0.0;
+  /* under the residual rule the state returned is the one whose residual was
+   * evaluated: the velocities, basal stress and the viscosity they were
+   * computed with (the viscosity of the stop check is its next iterate) */
+  double *visc_keep = (c->stop_rule == IMAU_STOP_RESIDUAL) ? dalloc(g->nz * s.na) : NULL;
   for (;;) {
     it++;
+    if (visc_keep) memcpy(visc_keep, s.visc3, g->nz * s.na * sizeof(double));
     calc_vertical_shear_strain_rates(&s);
     calc_effective_viscosity(&s, s.u, s.v);
     calc_sliding_term_beta(&s, s.u, s.v);
@@ -1567,7 +1590,11 @@ int orc_solve_diva(const imau_grid *gr, const imau_cfg *c, imau_ice_io *io, int 
     if (c->stop_rule == IMAU_STOP_RESIDUAL) {
       double r = csr_rel_residual(&s, s.u, s.v);
       io->stat_nonlinear_residual = r;
-      if (r <= c->nonlinear_rtol || it > c->DIVA_visc_it_nit) { it--; break; }
+      if (r <= c->nonlinear_rtol || it > c->DIVA_visc_it_nit) {
+        memcpy(s.visc3, visc_keep, g->nz * s.na * sizeof(double));
+        it--;
+        break;
+      }
     }
     memcpy(s.u_prev, s.u, s.ncx * sizeof(double));
     memcpy(s.v_prev, s.v, s.ncy * sizeof(double));
@@ -1585,6 +1612,7 @@ int orc_solve_diva(const imau_grid *gr, const imau_cfg *c, imau_ice_io *io, int 
     calc_basal_velocities(&s);
     if (conv) break;
   }
+  free(visc_keep);
   io->stat_picard_iterations = it;
   io->stat_resid_UV = resid;
   if (rc) { st_free(&s); return rc; }
@@ -1610,6 +1638,7 @@ int orc_solve_ssa(const imau_grid *gr, const imau_cfg *c, imau_ice_io *io, int l
 {
   if (c->choice_ice_dynamics != IMAU_DYN_SSA && c->choice_ice_dynamics != IMAU_DYN_SIA_SSA)
     return fail(IMAU_ERR_CONFIG, "solve_SSA: choice_ice_dynamics is not SSA or SIA/SSA");
+  if (check_roughness(c)) return IMAU_ERR_CONFIG;
   S s;
   st_init(&s, gr, c, io, 1);
   const G *g = &s.g;
@@ -1626,8 +1655,10 @@ int orc_solve_ssa(const imau_grid *gr, const imau_cfg *c, imau_ice_io *io, int l
   /* SSA: no vertical shear (du_dz stays zero) */
   int it = 0, rc = 0;
   double resid = 0.0;
+  double *visc_keep = (c->stop_rule == IMAU_STOP_RESIDUAL) ? dalloc(g->nz * s.na) : NULL;
   for (;;) {
     it++;
+    if (visc_keep) memcpy(visc_keep, s.visc3, g->nz * s.na * sizeof(double));
     calc_effective_viscosity(&s, s.u, s.v);
     calc_sliding_term_beta(&s, s.u, s.v);
     memcpy(s.beta_eff_a, s.beta_a, s.na * sizeof(double));
@@ -1638,7 +1669,11 @@ int orc_solve_ssa(const imau_grid *gr, const imau_cfg *c, imau_ice_io *io, int l
     if (c->stop_rule == IMAU_STOP_RESIDUAL) {
       double r = csr_rel_residual(&s, s.u, s.v);
       io->stat_nonlinear_residual = r;
-      if (r <= c->nonlinear_rtol || it > c->DIVA_visc_it_nit) { it--; break; }
+      if (r <= c->nonlinear_rtol || it > c->DIVA_visc_it_nit) {
+        memcpy(s.visc3, visc_keep, g->nz * s.na * sizeof(double));
+        it--;
+        break;
+      }
     }
     memcpy(s.u_prev, s.u, s.ncx * sizeof(double));
     memcpy(s.v_prev, s.v, s.ncy * sizeof(double));
@@ -1649,6 +1684,7 @@ int orc_solve_ssa(const imau_grid *gr, const imau_cfg *c, imau_ice_io *io, int l
     resid = visc_iter_resid(&s, s.u, s.v);
     if (c->stop_rule == IMAU_STOP_PICARD && (resid < c->DIVA_visc_it_norm_dUV_tol || it >= c->DIVA_visc_it_nit)) break;
   }
+  free(visc_keep);
   io->stat_picard_iterations = it;
   io->stat_resid_UV = resid;
   if (rc) { st_free(&s); return rc; }
@@ -1790,4 +1826,248 @@ void orc_abi_layout(int64_t *out)
   out[5] = (int64_t)offsetof(imau_ice_io, u_vav_cx);
   out[6] = (int64_t)offsetof(imau_ice_io, DIVA_SOR_nit);
   out[7] = (int64_t)offsetof(imau_ice_io, stat_seconds);
+}
+
+/* ------------------------------------------------------------------------ */
+/* ice-thickness update, src/ice_thickness_module.f90:25-909 and            */
+/* calc_critical_timestep_adv, src/ice_dynamics_module.f90:726-762          */
+/* ------------------------------------------------------------------------ */
+static double hi_from_hb_and_hs(double Hb, double Hs, double SL)
+{ /* src/utilities_module.f90:160-183 */
+  double Hs_float = Hb + (SL - Hb) * SEAWATER_DENSITY / ICE_DENSITY;
+  if (Hs > Hs_float) return Hs - Hb;
+  return (Hs - SL) / (1.0 - ICE_DENSITY / SEAWATER_DENSITY);
+}
+
+/* net mass balance of a cell (m/yr): only the floating fraction at the
+ * calving front, none on ice-free ocean (:219-227, :393-401) */
+static double mb_net(const imau_thickness_io *io, size_t p)
+{
+  double mb = io->SMB_year[p] + io->BMB[p];
+  if (io->mask_cf_a[p] == 1 && io->mask_shelf_a[p] == 1) return mb * io->float_margin_frac_a[p];
+  if (io->mask_ocean_a[p] == 1 && io->mask_shelf_a[p] == 0) return 0.0;
+  return mb;
+}
+
+/* src/ice_thickness_module.f90:59-312 */
+static void dhi_explicit(const imau_grid *g, const imau_thickness_io *io, double dt, double *Qx, double *Qy,
+                         double *dHi, double *Ht)
+{
+  const int nx = g->nx, ny = g->ny;
+  const double dx = g->dx;
+  size_t na = (size_t)nx * ny;
+  for (int i = 0; i < nx - 1; ++i)
+    for (int j = 0; j < ny; ++j) {
+      double u = io->u_vav_cx[(size_t)i * ny + j];
+      Qx[(size_t)i * ny + j] = u * io->Hi_a[(size_t)(u > 0.0 ? i : i + 1) * ny + j] * dx * dt;
+    }
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny - 1; ++j) {
+      double v = io->v_vav_cy[(size_t)i * (ny - 1) + j];
+      Qy[(size_t)i * (ny - 1) + j] = v * io->Hi_a[(size_t)i * ny + (v > 0.0 ? j : j + 1)] * dx * dt;
+    }
+  double *dVmb = dalloc(na);
+#define QX(jj, ii) Qx[(size_t)(ii) * ny + (jj)]
+#define QY(jj, ii) Qy[(size_t)(ii) * (ny - 1) + (jj)]
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      size_t p = (size_t)i * ny + j;
+      dVmb[p] = mb_net(io, p) * dx * dx * dt;
+      double Vav = io->Hi_a[p] * dx * dx, out = 0.0;
+      if (i > 0 && QX(j, i - 1) < 0.0) out -= QX(j, i - 1);
+      if (i < nx - 1 && QX(j, i) > 0.0) out += QX(j, i);
+      if (j > 0 && QY(j - 1, i) < 0.0) out -= QY(j - 1, i);
+      if (j < ny - 1 && QY(j, i) > 0.0) out += QY(j, i);
+      double rescale = 1.0;
+      if (-dVmb[p] >= Vav) { dVmb[p] = -Vav; rescale = 0.0; }
+      if (out > Vav + dVmb[p]) rescale = (Vav + dVmb[p]) / out;
+      if (rescale < 1.0) {
+        if (i > 0 && QX(j, i - 1) < 0.0) QX(j, i - 1) *= rescale;
+        if (i < nx - 1 && QX(j, i) > 0.0) QX(j, i) *= rescale;
+        if (j > 0 && QY(j - 1, i) < 0.0) QY(j - 1, i) *= rescale;
+        if (j < ny - 1 && QY(j, i) > 0.0) QY(j, i) *= rescale;
+      }
+    }
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      size_t p = (size_t)i * ny + j;
+      double a = dx * dx * dt, d = dVmb[p] / a;
+      if (i > 0) d += QX(j, i - 1) / a;
+      if (i < nx - 1) d -= QX(j, i) / a;
+      if (j > 0) d += QY(j - 1, i) / a;
+      if (j < ny - 1) d -= QY(j, i) / a;
+      d = fmax(d, -io->Hi_a[p] / dt);
+      dHi[p] = d;
+      Ht[p] = io->Hi_a[p] + d * dt;
+    }
+#undef QX
+#undef QY
+  free(dVmb);
+}
+
+/* velocity through a cell face, zero across a calving front whose floating
+ * cell is not yet full (:474-540, :574-640) */
+static double face_vel(const imau_grid *g, const imau_thickness_io *io, int j, int i, int jn, int in, double vel)
+{
+  const int ny = g->ny;
+  size_t p = (size_t)i * ny + j, q = (size_t)in * ny + jn;
+  if (io->mask_shelf_a[p] == 1 && io->mask_ocean_a[q] == 1 && io->mask_ice_a[q] == 0) {
+    if (io->float_margin_frac_a[p] < 1.0) return 0.0;
+  } else if (io->mask_shelf_a[q] == 1 && io->mask_ocean_a[p] == 1 && io->mask_ice_a[p] == 0) {
+    if (io->float_margin_frac_a[q] < 1.0) return 0.0;
+  }
+  return vel;
+}
+
+/* src/ice_thickness_module.f90:313-718: CSR in the (i outer, j inner) order,
+ * fs = 2.5; solved with the exact banded LU (standing in for SOR / PETSc) */
+static int dhi_semiimplicit(const imau_grid *g, const imau_thickness_io *io, double dt, double *dHi, double *Ht)
+{
+  const int nx = g->nx, ny = g->ny;
+  const double dx = g->dx, fs = 2.5;
+  const int m = nx * ny;
+  int32_t *ptr = ialloc(m + 1), *idx = ialloc(5 * (size_t)m);
+  double *val = dalloc(5 * (size_t)m), *b = dalloc(m), *x = dalloc(m);
+  int k = 0;
+  const int32_t *mi = io->mask_ice_a, *ms = io->mask_shelf_a, *mo = io->mask_ocean_a;
+  const double *H = io->Hi_a;
+#define P2(jj, ii) ((size_t)(ii) * ny + (jj))
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      int n = (int)P2(j, i);
+      ptr[n] = k;
+      int im1 = i > 0 ? i - 1 : 0, ip1 = i < nx - 1 ? i + 1 : nx - 1;
+      int jm1 = j > 0 ? j - 1 : 0, jp1 = j < ny - 1 ? j + 1 : ny - 1;
+      int scheme; /* 0 MB-only, 1 explicit, 2 semi-implicit (:353-385) */
+      if (mi[P2(j, i)] == 0 && mi[P2(j, im1)] == 0 && mi[P2(j, ip1)] == 0 && mi[P2(jm1, i)] == 0 && mi[P2(jp1, i)] == 0)
+        scheme = 0;
+      else if (ms[P2(j, i)] == 1 &&
+               ((mo[P2(j, im1)] == 1 && mi[P2(j, im1)] == 0) || (mo[P2(j, ip1)] == 1 && mi[P2(j, ip1)] == 0) ||
+                (mo[P2(jm1, i)] == 1 && mi[P2(jm1, i)] == 0) || (mo[P2(jp1, i)] == 1 && mi[P2(jp1, i)] == 0)))
+        scheme = 1;
+      else if (mo[P2(j, i)] == 1 && mi[P2(j, i)] == 0 &&
+               (ms[P2(j, im1)] == 1 || ms[P2(j, ip1)] == 1 || ms[P2(jm1, i)] == 1 || ms[P2(jp1, i)] == 1))
+        scheme = 1;
+      else
+        scheme = 2;
+      double mb = mb_net(io, P2(j, i));
+      double uw = 0.0, ue = 0.0, vs = 0.0, vn = 0.0;
+      if (i > 0) uw = face_vel(g, io, j, i, j, i - 1, io->u_vav_cx[P2(j, i - 1)]);
+      if (i < nx - 1) ue = face_vel(g, io, j, i, j, i + 1, io->u_vav_cx[P2(j, i)]);
+      if (j > 0) vs = face_vel(g, io, j, i, j - 1, i, io->v_vav_cy[(size_t)i * (ny - 1) + j - 1]);
+      if (j < ny - 1) vn = face_vel(g, io, j, i, j + 1, i, io->v_vav_cy[(size_t)i * (ny - 1) + j]);
+      if (scheme == 0) {
+        double h = fmax(0.0, H[P2(j, i)] + mb * dt);
+        idx[k] = n; val[k++] = 1.0; b[n] = h; x[n] = h;
+      } else if (scheme == 1) {
+        /* upwind fluxes (m/yr) with the face velocities (:441-560) */
+        double Qw = 0.0, Qe = 0.0, Qs = 0.0, Qn = 0.0;
+        if (i > 0) Qw = uw * H[P2(j, uw > 0.0 ? i - 1 : i)] / dx;
+        if (i < nx - 1) Qe = -ue * H[P2(j, ue > 0.0 ? i : i + 1)] / dx;
+        if (j > 0) Qs = vs * H[P2(vs > 0.0 ? j - 1 : j, i)] / dx;
+        if (j < ny - 1) Qn = -vn * H[P2(vn > 0.0 ? j : j + 1, i)] / dx;
+        double h = fmax(0.0, H[P2(j, i)] + dt * (mb + Qw + Qe + Qs + Qn));
+        idx[k] = n; val[k++] = 1.0; b[n] = h; x[n] = h;
+      } else {
+        double d = 1.0 / dt;
+        if (i > 0) d -= (fs / dx) * fmin(uw, 0.0);
+        if (i < nx - 1) d += (fs / dx) * fmax(ue, 0.0);
+        if (j > 0) d -= (fs / dx) * fmin(vs, 0.0);
+        if (j < ny - 1) d += (fs / dx) * fmax(vn, 0.0);
+        idx[k] = n; val[k++] = d;
+        if (i > 0) { idx[k] = (int)P2(j, i - 1); val[k++] = (-fs / dx) * fmax(uw, 0.0); }
+        if (i < nx - 1) { idx[k] = (int)P2(j, i + 1); val[k++] = (fs / dx) * fmin(ue, 0.0); }
+        if (j > 0) { idx[k] = (int)P2(j - 1, i); val[k++] = (-fs / dx) * fmax(vs, 0.0); }
+        if (j < ny - 1) { idx[k] = (int)P2(j + 1, i); val[k++] = (fs / dx) * fmin(vn, 0.0); }
+        double bb = mb + H[P2(j, i)] / dt;
+        if (i > 0) bb += H[P2(j, i)] * (1.0 - fs) / dx * fmin(uw, 0.0) + H[P2(j, i - 1)] * (1.0 - fs) / dx * fmax(uw, 0.0);
+        if (i < nx - 1) bb -= H[P2(j, i)] * (1.0 - fs) / dx * fmax(ue, 0.0) + H[P2(j, i + 1)] * (1.0 - fs) / dx * fmin(ue, 0.0);
+        if (j > 0) bb += H[P2(j, i)] * (1.0 - fs) / dx * fmin(vs, 0.0) + H[P2(j - 1, i)] * (1.0 - fs) / dx * fmax(vs, 0.0);
+        if (j < ny - 1) bb -= H[P2(j, i)] * (1.0 - fs) / dx * fmax(vn, 0.0) + H[P2(j + 1, i)] * (1.0 - fs) / dx * fmin(vn, 0.0);
+        b[n] = bb;
+        x[n] = H[P2(j, i)];
+      }
+    }
+  ptr[m] = k;
+#undef P2
+  int rc = orc_banded_lu_solve(m, ptr, idx, val, b, x);
+  if (!rc)
+    for (int n = 0; n < m; ++n) {
+      Ht[n] = fmax(0.0, x[n]);
+      dHi[n] = (Ht[n] - H[n]) / dt;
+    }
+  free(ptr); free(idx); free(val); free(b); free(x);
+  return rc;
+}
+
+/* src/ice_thickness_module.f90:719-909, sides in the reference's order */
+static void thickness_bc(const imau_grid *g, const imau_thickness_cfg *c, const imau_thickness_io *io, double dt,
+                         double *dHi, double *Ht)
+{
+  const int nx = g->nx, ny = g->ny;
+  const int diva = c->choice_ice_dynamics == IMAU_DYN_DIVA;
+  const double *H = io->Hi_a, *Hb = io->Hb_a, *SL = io->SL_a;
+#define P2(jj, ii) ((size_t)(ii) * ny + (jj))
+  for (int side = 0; side < 4; ++side) {
+    int bc = (&c->ice_thickness_west_BC)[side];
+    int along = (side < 2) ? ny : nx;
+    for (int q = 0; q < along; ++q) {
+      /* cell on the boundary, its inner neighbour, the second row in, the periodic source */
+      int jb, ib, jn, in, j2, i2, jp, ip;
+      if (side == 0) { jb = q; ib = 0; jn = q; in = 1; j2 = q; i2 = 1; jp = q; ip = nx - 2; }
+      else if (side == 1) { jb = q; ib = nx - 1; jn = q; in = nx - 2; j2 = q; i2 = nx - 2; jp = q; ip = 1; }
+      else if (side == 2) { jb = 0; ib = q; jn = 1; in = q; j2 = 1; i2 = q; jp = ny - 2; ip = q; }
+      else { jb = ny - 1; ib = q; jn = ny - 2; in = q; j2 = ny - 2; i2 = q; jp = 1; ip = q; }
+      size_t p = P2(jb, ib);
+      if (bc == IMAU_HBC_ZERO) {
+        Ht[p] = 0.0; dHi[p] = -H[p] / dt;
+        if (diva) { size_t p2 = P2(j2, i2); Ht[p2] = 0.0; dHi[p2] = -H[p2] / dt; }
+      } else if (bc == IMAU_HBC_INFINITE) {
+        size_t pn = P2(jn, in);
+        double Hs = surface_elevation(Ht[pn], Hb[pn], SL[pn]);
+        Ht[p] = hi_from_hb_and_hs(Hb[p], Hs, SL[p]);
+        dHi[p] = (Ht[p] - H[p]) / dt;
+      } else if (bc == IMAU_HBC_PERIODIC) {
+        Ht[p] = H[P2(jp, ip)];
+        dHi[p] = (Ht[p] - H[p]) / dt;
+      } else if (bc == IMAU_HBC_FIXED) {
+        Ht[p] = H[p]; dHi[p] = 0.0;
+      }
+    }
+  }
+#undef P2
+}
+
+int orc_thickness_update(const imau_grid *g, const imau_thickness_cfg *c, imau_thickness_io *io, double dt)
+{
+  const int nx = g->nx, ny = g->ny;
+  size_t na = (size_t)nx * ny;
+  double *Qx = dalloc((size_t)ny * (nx - 1)), *Qy = dalloc((size_t)(ny - 1) * nx);
+  int rc = 0;
+  if (c->choice_ice_integration_method == IMAU_DHI_NONE) {
+    for (size_t p = 0; p < na; ++p) { io->dHi_dt_a[p] = 0.0; io->Hi_tplusdt_a[p] = io->Hi_a[p]; }
+  } else if (c->choice_ice_integration_method == IMAU_DHI_EXPLICIT) {
+    dhi_explicit(g, io, dt, Qx, Qy, io->dHi_dt_a, io->Hi_tplusdt_a);
+    cp_out(io->Qx_cx, Qx, (size_t)ny * (nx - 1));
+    cp_out(io->Qy_cy, Qy, (size_t)(ny - 1) * nx);
+  } else if (c->choice_ice_integration_method == IMAU_DHI_SEMIIMPLICIT) {
+    rc = dhi_semiimplicit(g, io, dt, io->dHi_dt_a, io->Hi_tplusdt_a);
+  } else {
+    rc = fail(IMAU_ERR_CONFIG, "unknown choice_ice_integration_method");
+  }
+  if (!rc) thickness_bc(g, c, io, dt, io->dHi_dt_a, io->Hi_tplusdt_a);
+  free(Qx); free(Qy);
+  return rc;
+}
+
+double orc_critical_timestep_adv(const imau_grid *g, const double *u_cx, const double *v_cy, double dt_max)
+{
+  const int nx = g->nx, ny = g->ny;
+  double dt_crit = 2.0 * dt_max;
+  for (int i = 1; i < nx - 1; ++i)
+    for (int j = 1; j < ny - 1; ++j) {
+      double dt = g->dx / (fabs(u_cx[(size_t)i * ny + j]) + fabs(v_cy[(size_t)i * (ny - 1) + j]));
+      dt_crit = fmin(dt_crit, dt);
+    }
+  return fmin(dt_max, dt_crit * 0.9);
 }

@@ -68,6 +68,10 @@ int    orc_sor(int m, const int32_t *ptr, const int32_t *idx, const double *val,
                double *x, int nit, double tol, double omega, int *iterations);
 int    orc_banded_lu_solve(int m, const int32_t *ptr, const int32_t *idx, const double *val,
                            const double *b, double *x);
+/* ice-thickness update (src/ice_thickness_module.f90:25-909) and the
+ * advective critical time step (src/ice_dynamics_module.f90:726-762) */
+int    orc_thickness_update(const imau_grid *g, const imau_thickness_cfg *c, imau_thickness_io *io, double dt);
+double orc_critical_timestep_adv(const imau_grid *g, const double *u_cx, const double *v_cy, double dt_max);
 /* Schoof (2006) ice-stream closed form, src/utilities_module.f90:836-877 */
 void   orc_ssa_schoof2006(double n_flow, double L, double m, double tantheta, double h0,
                           double A_flow, double y, double *U, double *tauc);

@@ -116,3 +116,23 @@ def banded_lu(ptr, idx, val, b):
     x = np.zeros(len(b))
     rc = lib().orc_banded_lu_solve(len(b), _ptr(ptr), _ptr(idx), _ptr(val), _ptr(b), _ptr(x))
     return rc, x
+
+
+def thickness_update(grid, tcfg, ts, dt):
+    """The oracle's calc_dHi_dt on ThicknessState ts in place."""
+    L = lib()
+    g = grid.struct()
+    io = ts.io()
+    rc = L.orc_thickness_update(ctypes.byref(g), ctypes.byref(tcfg), ctypes.byref(io), ctypes.c_double(dt))
+    if rc != 0:
+        raise RuntimeError("oracle thickness update failed (%d): %s" % (rc, L.orc_last_error().decode()))
+    return ts
+
+
+def critical_timestep_adv(grid, u, v, dt_max):
+    L = lib()
+    L.orc_critical_timestep_adv.restype = ctypes.c_double
+    g = grid.struct()
+    u = np.asfortranarray(u, dtype=np.float64)
+    v = np.asfortranarray(v, dtype=np.float64)
+    return L.orc_critical_timestep_adv(ctypes.byref(g), _ptr(u), _ptr(v), ctypes.c_double(dt_max))

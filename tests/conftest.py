@@ -10,3 +10,9 @@ for p in (PKG, os.path.dirname(__file__)):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu)")
+    # torch's HIP runtime must initialise before libimauvel.so's: a process
+    # in which the library touched the GPU first cannot initialise torch's
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass

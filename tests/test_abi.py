@@ -36,6 +36,25 @@ def test_library_exports_every_declared_entry_point():
     assert declared == set(solver.EXPORTED)
 
 
+def test_library_exports_every_thickness_entry_point():
+    L = solver.load_library()
+    txt = open(abi.HEADER).read()
+    declared = set(re.findall(r"\b(imau_(?:thickness|critical)_\w+)\s*\(", txt))
+    assert declared == set(solver.THICKNESS_EXPORTED)
+    for name in sorted(declared):
+        assert hasattr(L, name), name
+
+
+def test_thickness_defaults_match_the_python_mirror():
+    from imau_vel import config
+    L = solver.load_library()
+    c = abi.ImauThicknessCfg()
+    L.imau_thickness_default_config(ctypes.byref(c))
+    ref = config.thickness_config()
+    for name, _ in abi.ImauThicknessCfg._fields_:
+        assert getattr(c, name) == getattr(ref, name), name
+
+
 def test_library_exports_every_network_entry_point():
     from imau_vel import learned
     L = solver.load_library()

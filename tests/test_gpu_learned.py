@@ -51,8 +51,9 @@ def test_features_targets_and_warm_mapping_match_oracle(env):
     net = learned.Net(ctx, seed=0)
     io = learned.ProblemIO(net, prob)
     x = unbits(io.features(cfg))
-    t = torch.zeros(io.plan.npix[0] * 2, dtype=torch.float32, device="cuda")
-    w = torch.zeros(io.plan.npix[0], dtype=torch.float32, device="cuda")
+    t = torch.empty(io.plan.npix[0] * 2, dtype=torch.float32, device="cuda")
+    w = torch.empty(io.plan.npix[0], dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()          # the library writes on its own stream
     io.targets(t, w)
     net.sync()
     t, w = t.cpu().numpy().astype(np.float64), w.cpu().numpy().astype(np.float64)
@@ -137,7 +138,8 @@ def test_warm_and_cold_starts_reach_the_same_fixed_point(env):
 def test_fallback_to_the_cold_solve(env):
     """An iteration cap of 2 keeps the warm-started residual above 1e-4: the
     whole problem is solved again from the cold state, and the result equals a
-    direct cold solve bit for bit."""
+    direct cold solve bit for bit.  That cold solve cannot meet the target
+    within the cap either, so the solver reports IMAU_ERR_STALLED."""
     learned, solver, tw, ctx = env
     cfg = tw.solver_config()
     cfg.DIVA_visc_it_nit = 2
@@ -146,11 +148,47 @@ def test_fallback_to_the_cold_solve(env):
     prob.solve(cfg)
     direct = [(o["u_vav_cx"].copy(), o["v_vav_cy"].copy()) for o in prob.download()]
     net = learned.Net(ctx, seed=0)          # untrained: any warm start will do
-    ws = learned.WarmStartSolver(ctx, prob, net=net, warm_iters=2)
+    ws = learned.WarmStartSolver(ctx, prob, net=net, warm_iters=2, stall_window=0)
     prob.reset_state()
-    ws.solve(cfg)
+    with pytest.raises(solver.ImauError) as e:
+        ws.solve(cfg)
+    assert e.value.code == -5
     assert ws.fallbacks == 1
     for (uc, vc), o in zip(direct, prob.download()):
         assert np.array_equal(o["u_vav_cx"], uc) and np.array_equal(o["v_vav_cy"], vc)
+    ws.close()
+    prob.close()
+
+
+def test_stalled_warm_start_restarts_cold_inside_the_loop(env):
+    """The 401 x 41 marine channel (2 km, floating ice) lies outside the
+    network's training distribution: from the learned start the FP64 residual
+    stalls near 0.3 for tens of Picard iterations (93 iterations in all
+    without the restart).  The stall detector restarts the member from the
+    cold state after stall_window iterations, so the learned solve costs at
+    most stall_window + 1 iterations more than the cold solve, and reaches
+    the same target."""
+    from imau_vel import geometry
+    learned, solver, tw, ctx = env
+    if not os.path.exists(learned.WEIGHTS):
+        pytest.skip("no trained weights")
+    g, _, st = geometry.channel()
+    cfg = tw.solver_config()
+    for c in ("u", "v"):
+        setattr(cfg, "DIVA_boundary_BC_%s_west" % c, 1)
+    cfg.DIVA_boundary_BC_v_south = cfg.DIVA_boundary_BC_v_north = 1
+    cfg.DIVA_visc_it_nit = 500
+    states = [st.copy()]
+    tw.prepare(states)
+    prob = solver.Problem(ctx, [g], cfg, states)
+    prob.solve(cfg)
+    its_c, res_c = prob.stats()
+    ws = learned.WarmStartSolver(ctx, prob)
+    prob.reset_state()
+    its_w, res_w = ws.solve(cfg)
+    print("channel: cold %d iterations, learned %d with %d restart(s)" % (its_c[0], its_w[0], ws.restarts))
+    assert res_c[0] <= 1e-4 and res_w[0] <= 1e-4
+    assert ws.restarts == 1 and ws.fallbacks == 0
+    assert its_w[0] <= its_c[0] + ws.stall_window + 1
     ws.close()
     prob.close()

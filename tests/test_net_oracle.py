@@ -99,3 +99,31 @@ def test_package_network_matches_the_oracle_contract():
             assert z["params"].shape == (NO.n_params(),)
             assert np.array_equal(z["layers"], np.array(NO.LAYERS, np.int32))
             assert np.all(np.isfinite(z["params"]))
+
+
+def test_conv_pool_up_match_torch_functional():
+    """The forward operators against torch.nn.functional on ragged images
+    (odd extents, 1-pixel rows and columns): conv = conv2d with zero padding 1
+    (weights w[tap][cout][cin], tap = 3 (dr + 1) + (dc + 1), cross-correlation);
+    pool = avg_pool2d(2) of the image zero-padded to even extents; up =
+    nearest x2 upsampling cropped to the fine image."""
+    import torch
+    import torch.nn.functional as F
+    rng = np.random.default_rng(5)
+    for R, C in [(7, 9), (8, 8), (1, 5), (3, 1), (13, 6)]:
+        cin, cout = 5, 4
+        x = rng.normal(size=(R, C, cin))
+        w = rng.normal(size=(9, cout, cin))
+        b = rng.normal(size=cout)
+        xt = torch.from_numpy(x.transpose(2, 0, 1)[None].copy())
+        wt = torch.from_numpy(w.reshape(3, 3, cout, cin).transpose(2, 3, 0, 1).copy())
+        ref = F.conv2d(xt, wt, torch.from_numpy(b), padding=1)[0].numpy().transpose(1, 2, 0)
+        np.testing.assert_allclose(NO.conv(x, w, b), ref, rtol=1e-12, atol=1e-12)
+        xp = F.pad(xt, (0, C & 1, 0, R & 1))
+        ref = F.avg_pool2d(xp, 2)[0].numpy().transpose(1, 2, 0)
+        np.testing.assert_allclose(NO.pool(x), ref, rtol=1e-12, atol=1e-14)
+        Rc, Cc = (R + 1) // 2, (C + 1) // 2
+        c = rng.normal(size=(Rc, Cc, 3))
+        ref = F.interpolate(torch.from_numpy(c.transpose(2, 0, 1)[None].copy()), scale_factor=2, mode="nearest")
+        ref = ref[0, :, :R, :C].numpy().transpose(1, 2, 0)
+        np.testing.assert_array_equal(NO.up(c, R, C), ref)

@@ -332,3 +332,241 @@ def test_fixed_point_independent_of_relaxation_and_linear_solver():
     b.DIVA_SOR_nit = 500000
     O.solve("diva", g, c2, b, O.LIN_SOR)
     assert np.abs(a["u_vav_cx"] - b["u_vav_cx"]).max() <= 1e-7 * np.abs(a["u_vav_cx"]).max()
+
+
+# ---------------------------------------------------------------- 3D velocities, BC-mode rows, idealised laws
+def _inclined_slab(nz=32, H=2000.0, slope=5e-3, A=1e-16, n=8):
+    zeta = list(np.linspace(0.0, 1.0, nz))
+    cfg = geometry.synthetic_config(zeta=zeta, DIVA_visc_it_nit=800, stop_rule=1, nonlinear_rtol=1e-12)
+    g = Grid(n, n, 10e3)
+    st = IceState(g, nz)
+    st["Hi_a"] = H
+    st["Hb_a"] = -slope * np.meshgrid(g.x, g.y)[0]
+    st["SL_a"] = -1e4
+    st["A_flow_3D_a"] = A
+    geometry.fill_uniform_roughness(st, cfg)
+    geometry.update_general_ice_model_data(st, zeta)
+    O.solve("diva", g, cfg, st, O.LIN_BANDED_LU)
+    return np.array(zeta), cfg, st
+
+
+def test_diva_slab_velocity_profile_closed_form():
+    """u(zeta) = u_b + tau_b F1(zeta) (Lipscomb et al. 2019 Eq. 29): on a uniform
+    slab the shear-only viscosity 1/(2 A (tau zeta)^2) makes F1 the SIA
+    profile, u(zeta) = u_b + 2 A tau^3 H (1 - zeta^4) / 4, and the surface
+    velocity exceeds the depth average by the SIA ratio (5/4 of the
+    deformation part)."""
+    z, cfg, st = _inclined_slab()
+    H, slope, A = 2000.0, 5e-3, 1e-16
+    tau = 910.0 * 9.81 * H * slope
+    ub = st["u_base_cx"][3, 3]
+    expect = ub + 2.0 * A * tau ** 3 * H * (1.0 - z ** 4) / 4.0
+    np.testing.assert_allclose(st["u_3D_cx"][:, 3, 3], expect, rtol=2e-3)
+    # the viscosity column (away from the surface, where the floor of the
+    # strain rate, DIVA_epsilon_sq_0, takes over)
+    eta = st["visc_eff_3D_a"][:, 3, 3]
+    k = z > 0.2
+    np.testing.assert_allclose(eta[k], 1.0 / (2.0 * A * (tau * z[k]) ** 2), rtol=1e-6)
+    # F1 at the surface is F2's counterpart without the zeta^2 weight
+    F1s = st["F1_3D_a"][0, 3, 3]
+    assert F1s == pytest.approx(2.0 * A * tau ** 2 * H / 4.0, rel=2e-3)
+    deform = st["u_surf_a"][3, 3] - ub
+    assert deform == pytest.approx(1.25 * (st["u_vav_cx"][3, 3] - ub), rel=3e-3)
+
+
+def test_slab_vertical_velocity_follows_the_bed():
+    """Mass conservation on an inclined slab of uniform thickness: the flow is
+    parallel to the bed, so w(zeta) = -|s'| u(zeta) at every level.  The
+    reference's calc_3D_vertical_velocities takes the zeta-derivative term
+    from the a-grid velocities of the previous call (u_3D_a is refreshed only
+    afterwards, src/ice_velocity_module.f90:479-492): a first call from zero
+    a-grid velocities gives the basal kinematic value -|s'| u_b at every
+    level, a second call from the converged state the full profile."""
+    z, cfg, st = _inclined_slab(nz=15)
+    slope = 5e-3
+    ub = st["u_base_a"][2:-2, 2:-2]
+    np.testing.assert_allclose(st["w_3D_a"][:, 2:-2, 2:-2], -slope * np.broadcast_to(ub, (15,) + ub.shape),
+                               rtol=1e-9)
+    O.solve("diva", Grid(8, 8, 10e3), cfg, st, O.LIN_BANDED_LU)
+    u3, w3 = st["u_3D_a"], st["w_3D_a"]
+    np.testing.assert_allclose(w3[:, 2:-2, 2:-2], -slope * u3[:, 2:-2, 2:-2], rtol=1e-9)
+    assert abs(w3[0, 3, 3] - w3[-1, 3, 3]) > 0.05 * abs(w3[-1, 3, 3])   # the levels differ
+
+
+def _floating_shelf(kind, A=1e-17, H=400.0, nx=40, ny=6, dx=5e3):
+    """A uniform floating shelf in choice_ice_margin = 'BC', uniform in y: a
+    wall (u = 0) to the west, a grounded strip along it (the reference
+    returns zero velocities when no ice is grounded), the calving front two
+    cells before the eastern edge."""
+    dyn = "SSA" if kind == "ssa" else "DIVA"
+    cfg = geometry.synthetic_config(choice_ice_dynamics=dyn, choice_ice_margin="BC", DIVA_boundary_BC_u_west="zero",
+                                    DIVA_boundary_BC_v_south="zero", DIVA_boundary_BC_v_north="zero",
+                                    stop_rule=1, nonlinear_rtol=1e-11, DIVA_visc_it_nit=2000)
+    g = Grid(nx, ny, dx)
+    st = IceState(g, cfg.nz)
+    Hi = np.full((ny, nx), H)
+    Hi[:, nx - 2:] = 0.0
+    st["Hi_a"] = Hi
+    Hb = np.full((ny, nx), -1000.0)
+    Hb[:, :4] = -340.0            # just grounded: a small surface step at the grounding line
+    st["Hb_a"] = Hb
+    st["SL_a"] = 0.0
+    st["A_flow_3D_a"] = A
+    geometry.fill_uniform_roughness(st, cfg)
+    geometry.update_general_ice_model_data(st, config.zeta_of(cfg))
+    return g, cfg, st
+
+
+@pytest.mark.parametrize("kind", ["diva", "ssa"])
+def test_unconfined_shelf_spreads_at_weertman_rate(kind):
+    """Weertman (1957): a freely floating shelf of uniform thickness in plane
+    flow spreads at du/dx = A (rho_i g H (1 - rho_i/rho_w) / 4)^n.  This pins
+    the calving-front stress rows of choice_ice_margin = 'BC'
+    (src/ice_velocity_module.f90:2192-2341): the front row carries the whole
+    load, the free rows keep the strain rate uniform, and a wrong factor or
+    sign in the front row changes the rate."""
+    A, H = 1e-17, 400.0
+    g, cfg, st = _floating_shelf(kind, A=A, H=H)
+    O.solve(kind, g, cfg, st, O.LIN_BANDED_LU)
+    u = st["u_SSA_cx" if kind == "ssa" else "u_vav_cx"]
+    rate = A * (910.0 * 9.81 * H * (1.0 - 910.0 / 1028.0) / 4.0) ** 3
+    dudx = np.diff(u[2, 8: g.nx - 2]) / g.dx      # the floating part, away from the grounding line
+    np.testing.assert_allclose(dudx, rate, rtol=1e-6)
+    assert np.abs(st["v_SSA_cy" if kind == "ssa" else "v_vav_cy"]).max() <= 1e-9 * np.abs(u).max()
+    # the front row is the stress condition, not the free row
+    assert st["DIVA_isfront_inner"][2, g.nx - 3] == 1 and st["DIVA_isfront_outer"][2, g.nx - 2] == 1
+
+
+@pytest.mark.parametrize("which", ["ISMIP_HOM_C", "ISMIP_HOM_D", "ISMIP_HOM_F"])
+def test_idealised_friction_fields(which):
+    """The ISMIP-HOM friction fields (Pattyn et al. 2008; reference
+    src/basal_conditions_and_sliding_module.f90:1288-1432) evaluated by a
+    solve at the grid's own coordinates: beta = 1000 + 1000 sin(2 pi x/L)
+    sin(2 pi y/L) (C), 1000 + 1000 sin(2 pi x/L) (D), (A_vav 1000)^-1 (F)."""
+    L = 80e3
+    cfg = geometry.synthetic_config(choice_sliding_law="idealised", choice_idealised_sliding_law=which, ISMIP_HOM_L=L,
+                                    DIVA_visc_it_nit=2)
+    g, cfg, st = geometry.random_geometry(31, 27, seed=21, cfg=cfg)
+    O.solve("diva", g, cfg, st, O.LIN_BANDED_LU)
+    X, Y = np.meshgrid(g.x, g.y)
+    if which == "ISMIP_HOM_C":
+        beta = 1000.0 + 1000.0 * np.sin(2 * np.pi * X / L) * np.sin(2 * np.pi * Y / L)
+    elif which == "ISMIP_HOM_D":
+        beta = 1000.0 + 1000.0 * np.sin(2 * np.pi * X / L)
+    else:
+        beta = 1.0 / (st["A_flow_vav_a"] * 1000.0)
+    np.testing.assert_allclose(st["beta_a"], beta, rtol=1e-12, atol=1e-9)
+
+
+def _quad_mc(c, n=600):
+    return _mc_fraction(*c, n=n)
+
+
+@pytest.mark.parametrize("scenario", [1, 2, 3, 4])
+def test_fraction_above_zero_every_scenario_and_rotation(scenario):
+    """Each of the four CISM scenarios (one corner above zero, one below, two
+    adjacent, two opposite) in each of its four rotations, against a brute-force
+    count of the bilinear quad (the rotations of rotate_quad_until_match must
+    not change the area)."""
+    rng = np.random.default_rng(40 + scenario)
+    sign = {1: [1, -1, -1, -1], 2: [-1, 1, 1, 1], 3: [1, 1, -1, -1], 4: [1, -1, 1, -1]}[scenario]
+    worst = 0.0
+    for _ in range(25):
+        mag = rng.uniform(0.2, 1.0, 4) * 10.0 ** rng.uniform(-1, 2)
+        base = np.array(sign) * mag           # NW, NE, SE, SW order around the quad
+        for r in range(4):
+            nw, ne, se, sw = np.roll(base, r)
+            phi = O.fraction_above_zero(nw, ne, sw, se)
+            worst = max(worst, abs(phi - _quad_mc((nw, ne, sw, se))))
+    assert worst < 4e-3, worst
+
+
+def test_fraction_above_zero_near_zero_corners_and_planar_quads():
+    """The ftol perturbation (corners within 1e-4 of zero are moved to
+    +-1e-4 x 1.1/1.21/1.13/0.97) and the |dd| < 1e-4 shift of the SW corner by
+    0.1 (src/general_ice_model_data_module.f90:539-574, 608-680) change the
+    fraction only by their own size: exact zeros, tiny corners and exactly
+    planar quads (dd = 0) of realistic TAF magnitudes (metres to hundreds of
+    metres) against a brute-force count."""
+    rng = np.random.default_rng(9)
+    worst = 0.0
+    for _ in range(60):
+        c = rng.uniform(-1.0, 1.0, 4) * 10.0 ** rng.uniform(1, 2)
+        c[rng.integers(4)] = rng.choice([0.0, 1e-7, -1e-7, 5e-5, -5e-5])
+        phi = O.fraction_above_zero(*c)
+        worst = max(worst, abs(phi - _quad_mc(c)))
+    for _ in range(60):
+        a, b, d = rng.uniform(-1, 1, 3) * np.array([30.0, 80.0, 80.0])
+        f = lambda x, y: a + b * x + d * y      # planar: NE + SW - NW - SE = 0
+        c = (f(0, 1), f(1, 1), f(0, 0), f(1, 0))
+        phi = O.fraction_above_zero(*c)
+        worst = max(worst, abs(phi - _quad_mc(c)))
+    assert worst < 4e-3, worst
+
+
+def _periodic_slab(axis, shift, nx=23, ny=8):
+    """Inclined slab with a thickness wave along `axis` ('x' or 'y'), uniform
+    across it; periodic boundaries along the wave and 'infinite' ones across
+    it.  The reference's periodic rows tie u(1) to u(nx-2) and u(nx-1) to u(2)
+    (1-based), so u is periodic with nx - 3 cells along x and v with ny - 3
+    cells along y (src/ice_velocity_module.f90:1932-2191); the wave has that
+    period.  SSA: in DIVA the boundary cells' one-sided strain rates reach the
+    interior rows through the staggered vertical shear, so only the SSA is
+    exactly periodic."""
+    along = ("west", "east") if axis == "x" else ("south", "north")
+    across = ("south", "north") if axis == "x" else ("west", "east")
+    bcs = {"DIVA_boundary_BC_%s_%s" % (c, s): "periodic" for c in ("u", "v") for s in along}
+    bcs.update({"DIVA_boundary_BC_%s_%s" % (c, s): "infinite" for c in ("u", "v") for s in across})
+    cfg = geometry.synthetic_config(choice_ice_dynamics="SSA", stop_rule=1, nonlinear_rtol=1e-12,
+                                    DIVA_visc_it_nit=2000, **bcs)
+    if axis == "y":
+        nx, ny = ny, nx
+    g = Grid(nx, ny, 5e3)
+    st = IceState(g, cfg.nz)
+    P = (nx if axis == "x" else ny) - 3
+    I, J = np.meshgrid(np.arange(nx), np.arange(ny))
+    k = (I if axis == "x" else J) + shift
+    st["Hi_a"] = 1000.0 + 150.0 * np.sin(2 * np.pi * k / P) + 60.0 * np.cos(6 * np.pi * k / P)
+    X, Y = np.meshgrid(g.x, g.y)
+    st["Hb_a"] = -4e-3 * (X if axis == "x" else Y)
+    st["SL_a"] = -1e4
+    st["A_flow_3D_a"] = 1e-16
+    geometry.fill_uniform_roughness(st, cfg)
+    geometry.update_general_ice_model_data(st, config.zeta_of(cfg))
+    st.DIVA_SOR_tol, st.DIVA_SOR_nit = 1e-9, 200000      # x-periodic rows: the oracle solves with SOR
+    O.solve("ssa", g, cfg, st, O.LIN_BANDED_LU)
+    return P, st
+
+
+@pytest.mark.parametrize("axis", ["x", "y"])
+def test_periodic_rows_are_translation_invariant(axis):
+    """The solution repeats with the period of the boundary rows (to the 1e-12
+    nonlinear residual of the solves), and shifting the input wave by d cells
+    shifts the solution by d cells (to the trace the boundary cells leave); a
+    wrong opposite index breaks both.
+    The corner rows, where a periodic side meets an 'infinite' one, take the
+    'infinite' rule (zero before infinite before periodic,
+    src/ice_velocity_module.f90:1954-1989): they copy the diagonal inner
+    neighbour."""
+    P, a = _periodic_slab(axis, 0)
+    _, b = _periodic_slab(axis, 3)
+    if axis == "x":
+        u0, u3, w = a["u_SSA_cx"], b["u_SSA_cx"], a["v_SSA_cy"]
+    else:     # transpose so the wave runs along the second index
+        u0, u3, w = a["v_SSA_cy"].T, b["v_SSA_cy"].T, a["u_SSA_cx"]
+    tol = 1e-7 * np.abs(u0).max()
+    n = u0.shape[1]
+    inner = slice(1, u0.shape[0] - 1)           # the rows across the wave that are not boundary rows
+    np.testing.assert_allclose(u0[inner, P:], u0[inner, :n - P], rtol=0, atol=tol)
+    # the boundary cells' one-sided strain rates and the corner rows leave a
+    # trace of a few 1e-5 next to the periodic edges; a shift by one cell in
+    # the opposite index would change the solution by about 10%
+    loose = 1e-4 * np.abs(u0).max()
+    np.testing.assert_allclose(u3[inner, : n - 3], u0[inner, 3:], rtol=0, atol=loose)
+    assert np.abs(u0[inner] - u0[1]).max() <= loose                   # uniform across the wave
+    np.testing.assert_allclose(u0[0, 1:-1], u0[1, 1:-1], rtol=0, atol=tol)   # 'infinite' across
+    for (r, c), (ri, ci) in (((0, 0), (1, 1)), ((0, n - 1), (1, n - 2)), ((-1, 0), (-2, 1)), ((-1, n - 1), (-2, n - 2))):
+        assert abs(u0[r, c] - u0[ri, ci]) <= tol, (r, c)               # corners
+    assert abs(u0[1, 0] - u0[1, 1]) > 10 * tol                        # ... which periodic rows would not give
+    assert np.abs(w).max() <= 1e-4 * np.abs(u0).max()
+    assert np.ptp(u0) > 0.05 * np.abs(u0).max()                      # the wave matters
