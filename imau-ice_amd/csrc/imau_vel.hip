@@ -10,7 +10,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -152,6 +154,7 @@ struct imau_problem {
 struct imau_ctx {
   int device = 0;
   hipStream_t s = nullptr;
+  double *scratch = nullptr;   // this device's copy of `scratch` (imau_vel_problem_residual_noalloc)
   std::string err;
   imau_problem *cached = nullptr;
   // optional HIP-event timing of the sweep kernel on this context's stream
@@ -164,6 +167,33 @@ struct imau_ctx {
 };
 
 static std::string g_init_err;
+
+// ---------------------------------------------------------------------------
+// Scratch of imau_vel_problem_residual_noalloc, the entry point for callers
+// that cannot allocate device memory: per-tile partial sums, per-member sums
+// and activity flags.  Every GPU the library runs on holds its own copy until
+// the process ends.  Host code reaches it only through the address
+// hipGetSymbolAddress returns with that device current (scratch_address),
+// never through the array's name.
+constexpr size_t kScratchDoubles = IMAU_NOALLOC_SCRATCH_BYTES / sizeof(double);
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+static std::mutex g_scratch_mu;
+static std::map<int, double *> g_scratch_addr;
+
+// the address of `scratch` on `device`, which must be the current device;
+// looked up once per device
+static double *scratch_address(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_addr.find(device);
+  if (it != g_scratch_addr.end()) return it->second;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    fail(IMAU_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+  g_scratch_addr[device] = static_cast<double *>(p);
+  return static_cast<double *>(p);
+}
 
 // ---------------------------------------------------------------------------
 // problem construction
@@ -665,6 +695,7 @@ int imau_vel_init(const imau_grid *grid, const imau_cfg *cfg, imau_ctx **out) {
   ctx = new imau_ctx();
   HIPCHK(hipGetDevice(&ctx->device));
   HIPCHK(hipStreamCreateWithFlags(&ctx->s, hipStreamNonBlocking));
+  ctx->scratch = scratch_address(ctx->device);
   *out = ctx;
   return IMAU_OK;
   }
@@ -1782,6 +1813,51 @@ int imau_vel_problem_set_tile_rows(imau_ctx *ctx, imau_problem *P, int32_t tile_
   for (int rows : cand) ok = ok || rows == tile_rows;
   if (!ok) fail(IMAU_ERR_ARG, "tile_rows=%d is not one of the viscosity kernel's tile heights", (int)tile_rows);
   set_mtiles(P, tile_rows);
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+int imau_vel_problem_residual_noalloc(imau_ctx *ctx, imau_problem *P, const imau_cfg *cfg, double *residual_dev) {
+  ABI_TRY
+  if (!ctx || !P || !cfg || !residual_dev) fail(IMAU_ERR_ARG, "NULL argument");
+  check_cfg(cfg);
+  if (cfg->nz != P->nz) fail(IMAU_ERR_CONFIG, "cfg nz=%d differs from the problem's nz=%d", cfg->nz, P->nz);
+  const int dyn = cfg->choice_ice_dynamics;
+  if (dyn != IMAU_DYN_DIVA && !uses_ssa_field(dyn))
+    fail(IMAU_ERR_CONFIG, "the nonlinear residual is defined for DIVA, SSA and SIA/SSA only");
+  if (cfg->include_basal_freezing && !P->has_ti) fail(IMAU_ERR_CONFIG, "include_basal_freezing needs Ti_a/Ti_pmp_a at create");
+  if (!ctx->scratch) fail(IMAU_ERR_DEVICE, "no scratch address on device %d", ctx->device);
+  // layout of the scratch: partials (2 per tile), member sums (2 per member), flags (1 int per member)
+  const size_t ntiles = P->tiles.size(), G = (size_t)P->ngrids;
+  const size_t need = 2 * ntiles + 2 * G + (G + 1) / 2;
+  if (need > kScratchDoubles)
+    fail(IMAU_ERR_ARG,
+         "problem too large for the no-allocation residual: %zu tiles and %zu members need %zu bytes of scratch, "
+         "the maximum is %zu bytes",
+         ntiles, G, need * sizeof(double), kScratchDoubles * sizeof(double));
+  double *partial = ctx->scratch;
+  double *sums = partial + 2 * ntiles;
+  int *active = reinterpret_cast<int *>(sums + 2 * G);
+  const KP kp = make_kp(cfg);
+  const imau::Launch L{ctx->s, (int)ntiles, P->mem_d.p, P->tiles_d.p};
+  const imau::Launch LV{ctx->s, P->nvtiles, P->mem_d.p, P->vtiles.p, P->nmtiles, P->mtiles.p};
+  const bool diva = dyn == IMAU_DYN_DIVA;
+  imau::Coef cf;
+  cf.Na = P->F.N_a; cf.Nb = P->F.N_b; cf.Ncx = P->F.N_cx; cf.Ncy = P->F.N_cy;
+  cf.bcx = P->F.beta_eff_cx; cf.bcy = P->F.beta_eff_cy; cf.taudx = P->F.taudx; cf.taudy = P->F.taudy;
+  cf.frhs = P->F.front_rhs; cf.rt_u = P->F.rt_u; cf.rt_v = P->F.rt_v;
+  // the same kernels as imau_vel_nonlinear_residual; the viscosity of the
+  // state is kept (the recomputed one goes to the problem's second buffer)
+  imau::fill_i32(ctx->s, active, (int)G, 1);
+  imau::setup1(L, kp, P->F, P->yg.p);
+  imau::setup2(L, kp, P->F);
+  imau::visc3d(L, LV, kp, P->F, diva, P->visc_cur, P->visc_oth, active);
+  imau::beta(L, kp, P->F, diva, P->xg.p, P->yg.p, active);
+  imau::coef(L, kp, P->F, diva, active);
+  imau::resid(L, kp, cf, P->F.u, P->F.v, partial);
+  imau::member_sum2(ctx->s, (int)G, P->tstart_d.p, partial, sums);
+  imau::resid_ratio(ctx->s, (int)G, sums, residual_dev);
+  HIPCHK(hipGetLastError());
   return IMAU_OK;
   ABI_CATCH(ctx)
 }

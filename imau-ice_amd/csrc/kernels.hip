@@ -1277,6 +1277,13 @@ __global__ void k_fill_i32(int *p, int n, int v) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q < n) p[q] = v;
 }
+// relative residual per member from the member sums (|Au-b|^2, |b|^2)
+__global__ void k_resid_ratio(int ngrids, const double *__restrict__ sums, double *__restrict__ r) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngrids) return;
+  const double rr = sums[2 * g], bb = sums[2 * g + 1];
+  r[g] = bb > 0.0 ? sqrt(rr) / sqrt(bb) : sqrt(rr);
+}
 
 // copy the Picard state into the linear-solve iterate (initial guess, the
 // reference's x(n) = u(j,i); zero rows start from 0)
@@ -1707,6 +1714,9 @@ void copy3_members(const Launch &L, int nz, const float *src, float *dst, const 
 }
 void fill_i32(hipStream_t s, int *p, int n, int v) {
   hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+}
+void resid_ratio(hipStream_t s, int ngrids, const double *sums, double *r) {
+  hipLaunchKernelGGL(k_resid_ratio, dim3((ngrids + 255) / 256), dim3(256), 0, s, ngrids, sums, r);
 }
 void F1(const Launch &L, const KP &kp, const Fields &f, const double *visc, const OutF &o) {
   hipLaunchKernelGGL(k_F1, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, visc, o);

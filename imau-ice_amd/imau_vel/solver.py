@@ -22,7 +22,7 @@ EXPORTED = (
     "imau_vel_nonlinear_residual", "imau_vel_finalize", "imau_vel_last_error", "imau_vel_default_config",
     "imau_vel_problem_create", "imau_vel_problem_solve", "imau_vel_problem_reset_state",
     "imau_vel_problem_download", "imau_vel_problem_destroy", "imau_vel_solve_diva_batch",
-    "imau_vel_kernel_timing", "imau_vel_problem_stats",
+    "imau_vel_kernel_timing", "imau_vel_problem_stats", "imau_vel_problem_residual_noalloc",
     "imau_vel_problem_restarts", "imau_vel_problem_thickness", "imau_vel_problem_autotune", "imau_vel_problem_set_tile_rows",
 )
 # the ice-thickness update that consumes the velocities
@@ -77,6 +77,7 @@ def load_library(path: str = LIB_PATH):
     L.imau_thickness_update.argtypes = [vp, P(ImauGrid), P(abi.ImauThicknessCfg), P(abi.ImauThicknessIO), ctypes.c_double]
     L.imau_critical_timestep_adv.argtypes = [vp, P(ImauGrid), P(ctypes.c_double), P(ctypes.c_double), ctypes.c_double,
                                              P(ctypes.c_double)]
+    L.imau_vel_problem_residual_noalloc.argtypes = [vp, vp, P(ImauCfg), vp]
     _lib = L
     return L
 
@@ -228,6 +229,13 @@ class Problem:
         out = (ctypes.c_int32 * n)()
         self.ctx._check(self.ctx.L.imau_vel_problem_restarts(self.ctx.h, self.h, out))
         return list(out)
+
+    def residual_noalloc(self, out_ptr, cfg=None):
+        """FP64 relative nonlinear residual of every member at the current state,
+        written asynchronously to the device array at out_ptr (ngrids doubles)
+        by the entry point that allocates nothing."""
+        self.ctx._check(self.ctx.L.imau_vel_problem_residual_noalloc(self.ctx.h, self.h, ctypes.byref(cfg or self.cfg),
+                                                                     ctypes.c_void_p(out_ptr)))
 
     def close(self):
         if self.h:

@@ -316,6 +316,24 @@ int  imau_vel_problem_autotune(imau_ctx *ctx, imau_problem *prob, const imau_cfg
  * (one of 0, 4, 8, 16, 32; IMAU_ERR_ARG otherwise). */
 int  imau_vel_problem_set_tile_rows(imau_ctx *ctx, imau_problem *prob, int32_t tile_rows);
 
+/* No reference counterpart.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the FP64 relative nonlinear residual of every member of a
+ * device-resident problem at its current state (u, taub, visc_eff_3D_a), the
+ * same value imau_vel_nonlinear_residual gives for that member's state, written
+ * to residual_dev[ngrids] in DEVICE memory, asynchronously on the context's
+ * stream (no host synchronisation, no allocation, no memset).  Its reduction
+ * partials, member sums and flags live in a static device array of
+ * IMAU_NOALLOC_SCRATCH_BYTES per GPU: problems needing more (2 doubles per
+ * 8 x 64 tile plus 2.5 per member, about 33 M cells) are rejected with
+ * IMAU_ERR_ARG.  All calls on one GPU share that array: calls that overlap on
+ * one GPU (several contexts or threads) race.  The problem's coefficient
+ * diagnostics (N, beta, beta_eff, the second viscosity buffer) are
+ * overwritten; its state is not. */
+#define IMAU_NOALLOC_SCRATCH_BYTES (1 << 20)
+int  imau_vel_problem_residual_noalloc(imau_ctx *ctx, imau_problem *prob, const imau_cfg *cfg,
+                                       double *residual_dev);
+
 /* ---- ice-thickness update: the consumer of u_vav_cx / v_vav_cy ----
  * choice_ice_integration_method (src/configuration_module.f90:308) */
 enum { IMAU_DHI_NONE = 0, IMAU_DHI_EXPLICIT = 1, IMAU_DHI_SEMIIMPLICIT = 2 };
