@@ -260,6 +260,9 @@ def main():
                     help="DIVA_visc_it_relax of the headline (its unmixed Picard steps)")
     ap.add_argument("--lin-rtol", type=float, default=0.3,
                     help="relative tolerance of each linear solve (inexact Picard; DIVA_PETSc_rtol)")
+    ap.add_argument("--visc-tile", default="auto",
+                    help="tile height of the DIVA 3D viscosity kernel: auto (timed on the problem before the timed "
+                         "region) or 0, 4, 8, 16, 32 (0: tiled kernel, else row-marching)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run 5 plain device-to-device copies of BYTES each (counter calibration)")
     args = ap.parse_args()
@@ -298,6 +301,11 @@ def main():
     use_method(cfg, meth, args.relax_max)
     if args.relax is not None:
         cfg.DIVA_visc_it_relax = args.relax
+    if args.visc_tile == "auto":
+        visc_tile = prob.autotune(cfg)   # set-up, outside the timed region
+    else:
+        visc_tile = int(args.visc_tile)
+        prob.set_tile_rows(visc_tile)
     dt = timed_solves(prob, cfg, args.steps, args.warmup, ws)
     out = prob.download()
     its = [o.stats["picard_iterations"] for o in out]
@@ -327,9 +335,11 @@ def main():
     measured = (args.workload == "ensemble" and args.seed == 0 and args.method in ("learned", "mixed", "mixed64visc") and args.ksp is None and args.relax_max is None
                 and args.lin_rtol == 0.3 and args.mg_rebuild is None and args.anderson_m is None
                 and args.relax is None)
-    visc_traffic = VISC_TRAFFIC_BYTES_PER_CELL.get(meth["precision"]) if measured else None
+    # the traffic figures were measured with the tiled kernel (height 0)
+    visc_traffic = VISC_TRAFFIC_BYTES_PER_CELL.get(meth["precision"]) if measured and visc_tile == 0 else None
     sweep_traffic = SWEEP_TRAFFIC_BYTES_PER_CELL.get(meth["precision"]) if measured else None
-    roof_visc = roof("visc", "k_strain + k_visc (strain rates, 3D viscosity, N, F2: once per Picard iteration)",
+    vk = "k_visc" if visc_tile == 0 else "k_visc_march (%d rows)" % visc_tile
+    roof_visc = roof("visc", "k_strain + %s (strain rates, 3D viscosity, N, F2: once per Picard iteration)" % vk,
                      VISC_BYTES_PER_CELL(nz, 4 if meth["precision"] == "mixed_visc" else 8), visc_traffic)
     roof_sweep = roof("smoother", "k_smooth_fused (one fine-level smoother sweep)" if meth["mg_levels"] == 0
                       else "k_prec_tile (one tile Gauss-Seidel application)",
@@ -345,7 +355,8 @@ def main():
         ms_per_step=1e3 * dt / args.steps, higher_is_better=True, scaling="weak", vs_baseline=None,
         dtype=dtype, data="synthetic",
         config=dict(workload=desc, members=nsolve, cells=cells, method=meth["text"],
-                    stop="FP64 nonlinear residual <= 1e-4 for every member", learned_warm_start=ws is not None),
+                    stop="FP64 nonlinear residual <= 1e-4 for every member", learned_warm_start=ws is not None,
+                    visc_tile_rows=visc_tile),
         picard_iterations=dict(min=min(its), max=max(its), mean=float(np.mean(its))),
         preconditioner_applications=dict(min=min(apps), max=max(apps)),
         max_final_residual=max(res),

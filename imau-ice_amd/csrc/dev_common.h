@@ -37,6 +37,19 @@
 #define IMAU_MTI_CANDIDATES {0, 4, 8, 16, 32}
 #define IMAU_MNZ 15
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b on XCD
+// b % 8), each with its own L2.  Tile kernels run tile imau_xcd_tile() rather
+// than tile b: XCD x takes the x-th contiguous eighth of the spatially ordered
+// tile list, so the halo rows and columns that neighbouring tiles share are
+// fetched into one L2 rather than two.  A bijection of [0, gridDim.x); per-tile
+// partial sums are stored at the tile's index, so each member's partials stay
+// contiguous.
+__device__ __forceinline__ int imau_xcd_tile() {
+  const int n = (int)gridDim.x, b = (int)blockIdx.x;
+  const int q = n >> 3, r = n & 7, x = b & 7;
+  return x * q + min(x, r) + (b >> 3);
+}
+
 // Anderson acceleration of the Picard loop: up to IMAU_AND_SLOTS stored
 // iterates (IMAU_AND_SLOTS - 1 differences)
 #define IMAU_AND_SLOTS 4

@@ -9,7 +9,8 @@
 #include "stencil.h"
 
 #define TILE_SETUP                                  \
-  const Tile t = tiles[blockIdx.x];                 \
+  const int tb = imau_xcd_tile();                   \
+  const Tile t = tiles[tb];                         \
   const Member m = mem[t.g];                        \
   const int j = t.j0 + (int)threadIdx.x;
 
@@ -445,19 +446,46 @@ __global__ void __launch_bounds__(IMAU_WG) k_visc(const Member *__restrict__ mem
       const T expo = (T)((1.0 - kp.n_flow) / (2.0 * kp.n_flow));
       const T HiT = (T)Hi, eps_hT = (T)eps_h, vmin = (T)kp.visc_min;
       T eta_prev_k = (T)0.0, g_prev_k = (T)0.0;
+      // the previous viscosity at the centre and its four neighbours and the
+      // rate-factor term, loaded one level ahead so that the loads of level
+      // k + 1 are in flight while level k is computed
+      const long long plane = (long long)nx * ny;
+      const long long pc = p3(m, 0, j, i);
+      const long long ow = (i > 0) ? -(long long)ny : 0, oe = (i < nx - 1) ? (long long)ny : 0;
+      const long long os = (j > 0) ? -1 : 0, on = (j < ny - 1) ? 1 : 0;
+      T nc = (T)0.0, nw = (T)0.0, ne = (T)0.0, ns = (T)0.0, nn = (T)0.0;
+      if (DIVA) {
+        nc = visc_in[pc];
+        nw = visc_in[pc + ow];
+        ne = visc_in[pc + oe];
+        ns = visc_in[pc + os];
+        nn = visc_in[pc + on];
+      }
+      T nB = B3[pc];
 #pragma unroll 1
       for (int k = 0; k < nz; ++k) {
         const T z = (T)kp.zeta[k];
+        // out-of-grid neighbours read the centre (an in-bounds address) and are
+        // replaced by 0, as the reference's rows never use them
+        const T vc = nc, vw = (i > 0) ? nw : (T)0.0, ve = (i < nx - 1) ? ne : (T)0.0;
+        const T vs = (j > 0) ? ns : (T)0.0, vn = (j < ny - 1) ? nn : (T)0.0;
+        const T B = nB;
+        if (k + 1 < nz) {
+          const long long q = pc + (k + 1) * plane;
+          if (DIVA) {
+            nc = visc_in[q];
+            nw = visc_in[q + ow];
+            ne = visc_in[q + oe];
+            ns = visc_in[q + os];
+            nn = visc_in[q + on];
+          }
+          nB = B3[q];
+        }
         T du_dz_a = (T)0.0, dv_dz_a = (T)0.0;
         if (DIVA) {
           // du/dz on the two cx and cy neighbours (calc_vertical_shear_strain_rates);
           // the pairs (icl, icl+1), (icr, icr+1), (jcl, jcl+1), (jcr, jcr+1) are
           // drawn from the centre and its four neighbours
-          const T vc = visc_in[p3(m, k, j, i)];
-          const T vw = (i > 0) ? visc_in[p3(m, k, j, i - 1)] : (T)0.0;
-          const T ve = (i < nx - 1) ? visc_in[p3(m, k, j, i + 1)] : (T)0.0;
-          const T vs = (j > 0) ? visc_in[p3(m, k, j - 1, i)] : (T)0.0;
-          const T vn = (j < ny - 1) ? visc_in[p3(m, k, j + 1, i)] : (T)0.0;
           T dl = (T)0.0, dr = (T)0.0, db = (T)0.0, dt = (T)0.0;
           if (!noice_l) dl = (tbl / stag_pick<T>((i == 0) ? vc : vw, (i == 0) ? ve : vc, sl)) * z;
           if (!noice_r) dr = (tbr / stag_pick<T>((i == nx - 1) ? vw : vc, (i == nx - 1) ? vc : ve, sr)) * z;
@@ -467,10 +495,9 @@ __global__ void __launch_bounds__(IMAU_WG) k_visc(const Member *__restrict__ mem
           dv_dz_a = use_avg_y ? stag_pick<T>(db, dt, sya) : db;
         }
         const T eps_sq = eps_hT + (T)0.25 * (du_dz_a * du_dz_a + dv_dz_a * dv_dz_a);
-        const T B = B3[p3(m, k, j, i)];
         T eta = THIRD ? B / cbrt(eps_sq) : B * pow(eps_sq, expo);
         eta = fmax(eta, vmin);  // NaN (cold start 0/0) -> visc_eff_min
-        visc_out[p3(m, k, j, i)] = eta;
+        visc_out[pc + k * plane] = eta;
         const T gk = (HiT / eta) * (z * z);
         if (k > 0) {
           const T dz = z - (T)kp.zeta[k - 1];
@@ -503,7 +530,7 @@ __global__ void __launch_bounds__(64) k_visc_march(const Member *__restrict__ me
                                                    const int *__restrict__ active) {
   constexpr int NZ = IMAU_MNZ, W = 66;
   __shared__ T ring[3][NZ][W];
-  const Tile t = tiles[blockIdx.x];
+  const Tile t = tiles[imau_xcd_tile()];
   const Member m = mem[t.g];
   const int lane = (int)threadIdx.x;
   const int j = t.j0 + lane;
@@ -962,7 +989,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_post(const Member *__restrict__ mem
       }
     }
   }
-  block_sum_store<3>(acc, partial + 3 * (long long)blockIdx.x);
+  block_sum_store<3>(acc, partial + 3 * (long long)tb);
 }
 
 // Basal stress at the current velocities, tau_b = beta_eff u (:1087-1120),
@@ -1159,7 +1186,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_resid(const Member *__restrict__ me
       }
     }
   }
-  block_sum_store<2>(acc, partial + 2 * (long long)blockIdx.x);
+  block_sum_store<2>(acc, partial + 2 * (long long)tb);
 }
 
 // The FP64 nonlinear residual at the current state (as k_resid) fused with the
@@ -1238,7 +1265,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_resid_init(const Member *__restrict
       }
     }
   }
-  block_sum_store<3>(acc, partial + 3 * (long long)blockIdx.x);
+  block_sum_store<3>(acc, partial + 3 * (long long)tb);
 }
 
 // Sum NV partials per member over its tile range [tstart[g], tstart[g+1])
@@ -1792,7 +1819,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_apply(const Member *__restrict__ me
       }
     }
   }
-  if (partial) block_sum_store<2>(acc, partial + 2 * (long long)blockIdx.x);
+  if (partial) block_sum_store<2>(acc, partial + 2 * (long long)tb);
 }
 
 // r = b - A x, rhat = r, p = v = 0; partials (r, r), (b, b)
@@ -1825,7 +1852,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_bicg_init(const Member *__restrict_
       }
     }
   }
-  block_sum_store<2>(acc, partial + 2 * (long long)blockIdx.x);
+  block_sum_store<2>(acc, partial + 2 * (long long)tb);
 }
 
 #define VEC_LOOP(...)                                               \
@@ -1883,7 +1910,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_bicg_xr(const Member *__restrict__ 
       acc[0] += rh.v[p] * rv; acc[1] += rv * rv;
     }
   })
-  block_sum_store<2>(acc, partial + 2 * (long long)blockIdx.x);
+  block_sum_store<2>(acc, partial + 2 * (long long)tb);
 }
 
 // per-member reduction of two partials and the BiCGStab scalar update.
@@ -1945,7 +1972,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_prec_tile(const Member *__restrict_
                                                        const int *__restrict__ active, int nsweeps) {
   __shared__ double red_unused;
   (void)red_unused;
-  const Tile t = tiles[blockIdx.x];
+  const Tile t = tiles[imau_xcd_tile()];
   const Member m = mem[t.g];
   const int nx = m.nx, ny = m.ny;
   const int tid = threadIdx.y * IMAU_TJ + threadIdx.x;
@@ -2182,7 +2209,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_gcr_dots(const Member *__restrict__
       }
     }
   }
-  block_sum_store<gcr::NV>(acc, partial + gcr::NV * (long long)blockIdx.x);
+  block_sum_store<gcr::NV>(acc, partial + gcr::NV * (long long)tb);
 }
 
 // per member: beta_i = (q,Q_i), q' = q - sum beta_i Q_i, |q'| from the Gram
@@ -2263,7 +2290,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_gcr_update(const Member *__restrict
       }
     }
   }
-  block_sum_store<2>(acc, partial + 2 * (long long)blockIdx.x);
+  block_sum_store<2>(acc, partial + 2 * (long long)tb);
 }
 
 namespace imau {
@@ -2338,7 +2365,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_and_prep(const Member *__restrict__
       }
     }
   }
-  block_sum_store<IMAU_AND_SLOTS>(acc, partial + IMAU_AND_SLOTS * (long long)blockIdx.x);
+  block_sum_store<IMAU_AND_SLOTS>(acc, partial + IMAU_AND_SLOTS * (long long)tb);
 }
 
 namespace imau {

@@ -19,12 +19,14 @@
 #include "stencil.h"
 
 #define TILE_SETUP                                  \
-  const Tile t = tiles[blockIdx.x];                 \
+  const int tb = imau_xcd_tile();                   \
+  const Tile t = tiles[tb];                         \
   const Member m = mem[t.g];                        \
   const int j = t.j0 + (int)threadIdx.x;
 
 #define CTILE_SETUP                                  \
-  const Tile t = L.tiles[blockIdx.x];                \
+  const int tb = imau_xcd_tile();                    \
+  const Tile t = L.tiles[tb];                        \
   const CMember cm = L.mem[t.g];                     \
   const int J = t.j0 + (int)threadIdx.x;
 
@@ -286,7 +288,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_cgs_fused(CLevelT<T> L, const T *__
                                                        T *__restrict__ xov, int reverse,
                                                        const int *__restrict__ active) {
   __shared__ T sx[2][CRI * CRJ];
-  const Tile t = L.tiles[blockIdx.x];
+  const Tile t = L.tiles[imau_xcd_tile()];
   if (!active[t.g] || L.depth[t.g] <= L.lev) return;
   const CMember cm = L.mem[t.g];
   const int Ib = t.i0 - CH, Jb = t.j0 - CH;
@@ -307,12 +309,15 @@ __global__ void __launch_bounds__(IMAU_WG) k_cgs_fused(CLevelT<T> L, const T *__
     const int color = reverse ? 3 - ph : ph;
     const int comp = color >> 1;
     const int r = CH - 1 - ph;
-    const int RW = IMAU_TJ + 2 * r, n = (IMAU_TI + 2 * r) * RW;
+    // lanes take the RW / 2 cells of this colour in each row of the region
+    // (same-component couplings are 5-point, so cells of one colour do not couple)
+    const int RW = IMAU_TJ + 2 * r, RH = RW / 2, n = (IMAU_TI + 2 * r) * RH;
     const T *cf = comp == 0 ? L.cu : L.cv;
     for (int q = tid; q < n; q += IMAU_WG) {
-      const int li = CH - r + q / RW, lj = CH - r + q % RW;
+      const int li = CH - r + q / RH;
+      const int lj = CH - r + 2 * (q % RH) + ((color ^ (Ib + li + Jb + CH - r)) & 1);
       const int I = Ib + li, J = Jb + lj;
-      if (((I + J) & 1) != (color & 1) || !cvalid(cm, comp, J, I)) continue;
+      if (!cvalid(cm, comp, J, I)) continue;
       const long long p = cp2(cm, J, I);
       const T D = cf[p];
       T S = (T)0.0;
@@ -365,7 +370,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_crestrict(CLevelT<T> Fl, CLevelT<T>
 // x(l) += P x(l+1); one thread per fine cell
 template <class T>
 __global__ void __launch_bounds__(IMAU_WG) k_cprolong(CLevelT<T> Fl, CLevelT<T> L, const int *__restrict__ active) {
-  const Tile t = Fl.tiles[blockIdx.x];
+  const Tile t = Fl.tiles[imau_xcd_tile()];
   const CMember fm = Fl.mem[t.g];
   const int j = t.j0 + (int)threadIdx.x;
   if (j >= fm.ny || !active[t.g] || Fl.depth[t.g] <= Fl.lev) return;
@@ -521,7 +526,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_smooth_fused(const Member *__restri
                                                           const KP kp, const CoefT<T> c, CVecT<T> zin, VecT<T> zout,
                                                           CVecT<T> y, int reverse, const int *__restrict__ active) {
   __shared__ T su[FRI * FRJ], sv[FRI * FRJ];
-  const Tile t = ftiles[blockIdx.x];
+  const Tile t = ftiles[imau_xcd_tile()];
   if (!active[t.g]) return;
   const Member m = mem[t.g];
   const int nx = m.nx, ny = m.ny;
@@ -546,12 +551,15 @@ __global__ void __launch_bounds__(IMAU_WG) k_smooth_fused(const Member *__restri
     const bool isu = color < 2;
     const int r = IMAU_FH - 1 - ph;
     const int li0 = IMAU_FH - r, lj0 = IMAU_FH - r;
-    const int RW = IMAU_FTJ + 2 * r, n = (IMAU_FTI + 2 * r) * RW;
+    // the region has an even width, so every row holds RW / 2 cells of this
+    // colour: lanes take those cells only (cells of one colour do not couple,
+    // so the order is free)
+    const int RW = IMAU_FTJ + 2 * r, RH = RW / 2, n = (IMAU_FTI + 2 * r) * RH;
     for (int q = tid; q < n; q += IMAU_WG) {
-      const int li = li0 + q / RW, lj = lj0 + q % RW;
+      const int li = li0 + q / RH;
+      const int lj = lj0 + 2 * (q % RH) + ((color ^ (ib + li + jb + lj0)) & 1);
       const int i = ib + li, j = jb + lj;
       if (i < 0 || j < 0 || i >= nx || j >= ny) continue;
-      if (((i + j) & 1) != (color & 1)) continue;
       if (isu && i >= nx - 1) continue;
       if (!isu && j >= ny - 1) continue;
       const long long p = p2(m, j, i);
