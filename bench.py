@@ -71,12 +71,12 @@ def VISC_BYTES_PER_CELL(nz, word):
 
 
 # HBM bytes per processed cell measured with rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE on the ensemble workload, keyed by the viscosity precision of the
-# method (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0: the factors a 1 GiB
-# device copy reads; profiles/ensemble_learned_kernel_stats.md).  Re-measure
-# after changing either kernel.
-VISC_TRAFFIC_BYTES_PER_CELL = {"mixed_visc": 411.2, "mixed": 620.5}
-SWEEP_TRAFFIC_BYTES_PER_CELL = {"mixed_visc": 165.2, "mixed": 165.2}
+# WRITE_SIZE on the ensemble workload (FETCH_SIZE scaled by 2.0, WRITE_SIZE by
+# 1.0: the factors a 1 GiB device copy reads; profiles/ensemble_learned_kernel_stats.md),
+# for the headline's FP32 viscosity; other precisions are not measured.
+# Re-measure after changing either kernel.
+VISC_TRAFFIC_BYTES_PER_CELL = {"mixed_visc": 311.9}
+SWEEP_TRAFFIC_BYTES_PER_CELL = {"mixed_visc": 45.7}
 METHODS = {
     "learned": dict(precision="mixed_visc", mg_levels=0, picard_relax_max=0.0, ksp_type=1, anderson_m=3, learned=True,
                     text="learned warm start (BF16 conv net on the matrix cores, viscosity passes), then GCR in FP64 "

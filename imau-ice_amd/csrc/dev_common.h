@@ -25,6 +25,10 @@
 // tile height of the 3D viscosity kernel (16 and 32 measured slower than 8 on
 // the 64-member ensemble: fewer workgroups in flight)
 #define IMAU_VTI 8
+// levels of the previous viscosity the 3D viscosity kernel loads ahead (1, 2,
+// 3, 4 measured 561, 610, 700, 692 us on the 64-member ensemble: deeper
+// prefetch costs occupancy)
+#define IMAU_VISC_PF 1
 // rows marched by one wavefront of the row-marching 3D viscosity kernel
 // (nz = 15): each lane keeps its column's viscosity at rows i-1, i, i+1 in an
 // LDS ring, so every level of the previous viscosity is read once, plus two

@@ -22,6 +22,7 @@
 #include "kernels.h"
 #include "mg.h"
 #include "net.h"
+#include "producers.h"
 #include "thickness.h"
 
 namespace {
@@ -116,6 +117,7 @@ struct imau_problem {
   DBuf<double> relax_d;  // per-member Picard relaxation (adaptive mode, restarted members)
   DBuf<int> restart_d;   // per-member flag: reset to the created state (stall restart)
   DBuf<int> stop_d;      // per-member flag: stopped at this iteration's residual check
+  DBuf<int> lin_d;       // per-member flag of the linear solve's best-iterate copies
   DBuf<double> gcr_dirs; // GCR directions Q_i, Z_i (allocated on first use)
   DBuf<double> and_slots;  // Anderson slots U_s, F_s (allocated on first use)
   DBuf<double> and_w;      // Anderson weights per member
@@ -309,7 +311,7 @@ enum F2 {
   F_uabs_vav, F_uabs_surf, F_uabs_base, F_R_shear, F_w_surf, F_w_base,
   F_u0, F_v0, F_tbx0, F_tby0,
   F_ru, F_rv, F_rhu, F_rhv, F_pu, F_pv, F_vu, F_vv, F_phu, F_phv, F_shu, F_shv, F_tu, F_tv,
-  F_eps_h,
+  F_eps_h, F_xbu, F_xbv,
   F2_COUNT
 };
 enum I2 { I_ice, I_sheet, I_ocean, I_land, I_shelf, I_combi, I_fin, I_fout, I2_COUNT };
@@ -789,6 +791,7 @@ int imau_vel_problem_create(imau_ctx *ctx, int32_t ngrids, const imau_grid *grid
   P->relax_d.alloc(ngrids);
   P->restart_d.alloc(ngrids);
   P->stop_d.alloc(ngrids);
+  P->lin_d.alloc(ngrids);
   HIPCHK(hipMemcpy(P->mem_d.p, P->mem.data(), ngrids * sizeof(Member), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(P->tiles_d.p, P->tiles.data(), P->tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(P->tstart_d.p, P->tstart.data(), P->tstart.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -1065,6 +1068,26 @@ struct Solver {
         time_end();
       }
     };
+    // Each member keeps its best iterate: BiCGStab's residual is not monotone,
+    // and on nearly singular systems (the ice-free rows of choice_ice_margin
+    // = 'BC') it can run away.  A member whose residual grows beyond
+    // kRunaway times its best restarts from that iterate (r recomputed, the
+    // BiCGStab scalars reset), at most kRestarts times; a member that ends
+    // unconverged returns its best iterate, never one worse than its start.
+    constexpr double kRunaway = 1e4;
+    constexpr int kRestarts = 10;
+    double *XB[2] = {f2p(P, F_xbu), f2p(P, F_xbv)};
+    std::vector<double> best(G, INFINITY);
+    std::vector<int> nrestart(G, 0);
+    auto push_lin = [&](const std::vector<int> &which) {
+      HIPCHK(hipStreamSynchronize(c->s));  // lin_flag_ is reused below
+      lin_flag_ = which;
+      HIPCHK(hipMemcpyAsync(P->lin_d.p, lin_flag_.data(), G * sizeof(int), hipMemcpyHostToDevice, c->s));
+    };
+    for (int g = 0; g < G; ++g)
+      if (act_lin[g]) best[g] = std::sqrt(out[2 * g]);
+    push_lin(act_lin);
+    imau::copy2_members(L, X[0], X[1], XB[0], XB[1], P->lin_d.p);
     const int maxit = sor_nit > 0 ? sor_nit : 10000;
     for (int it = 0; it < maxit; ++it) {
       imau::bicg_p(L, R[0], R[1], Pp[0], Pp[1], Vv[0], Vv[1], P->sc.p, P->act_lin.p);
@@ -1081,16 +1104,51 @@ struct Solver {
       kcheck();
       fetch();
       any = false;
+      bool improved = false, runaway = false;
+      std::vector<int> imp(G, 0), rst(G, 0);
       for (int g = 0; g < G; ++g) {
         if (!act_lin[g]) continue;
         P->sweeps[g] += 2;
         const double rn = std::sqrt(out[2 * g + 1]);
-        if (!std::isfinite(rn)) fail(IMAU_ERR_NAN, "member %d: non-finite linear residual", g);
-        if (rn <= target[g]) act_lin[g] = 0;
-        else any = true;
+        if (!std::isfinite(rn) && nrestart[g] >= kRestarts)
+          fail(IMAU_ERR_NAN, "member %d: non-finite linear residual", g);
+        if (rn <= target[g]) {
+          act_lin[g] = 0;
+          continue;
+        }
+        any = true;
+        if (rn < best[g]) {
+          best[g] = rn;
+          imp[g] = 1;
+          improved = true;
+        } else if (!(rn <= kRunaway * best[g]) && nrestart[g] < kRestarts) {
+          rst[g] = 1;
+          ++nrestart[g];
+          runaway = true;
+        }
       }
+      if (improved) {
+        push_lin(imp);
+        imau::copy2_members(L, X[0], X[1], XB[0], XB[1], P->lin_d.p);
+      }
+      if (runaway) {
+        push_lin(rst);
+        imau::copy2_members(L, XB[0], XB[1], X[0], X[1], P->lin_d.p);
+        imau::bicg_init(L, kp, cf, X[0], X[1], R[0], R[1], RH[0], RH[1], Pp[0], Pp[1], Vv[0], Vv[1], P->lin_d.p,
+                        P->partial.p);
+        imau::bicg_scalar(c->s, G, P->tstart_d.p, P->partial.p, P->sc.p, P->msum.p, P->lin_d.p, 0);
+      }
+      kcheck();
       if (!any) break;
       push_active(P->act_lin.p, act_lin);
+    }
+    // members that end unconverged return their best iterate
+    bool left = false;
+    for (int g = 0; g < G; ++g) left = left || act_lin[g];
+    if (left) {
+      push_lin(act_lin);
+      imau::copy2_members(L, XB[0], XB[1], X[0], X[1], P->lin_d.p);
+      kcheck();
     }
   }
 
@@ -1313,7 +1371,7 @@ struct Solver {
   // current iterate, before the convergence check of this Picard iteration
   std::vector<std::vector<double>> stall_hist_;
   std::vector<double> *prev_r_ = nullptr;
-  std::vector<int> restart_flag_, stop_flag_;
+  std::vector<int> restart_flag_, stop_flag_, lin_flag_;
   bool relaxed_ = false;  // per-member relaxation factors in use (adaptive mode or a restart)
   template <class VB>
   void stall_restarts(std::vector<double> &r, bool diva, VB &visc_beta, float *cur32) {
@@ -2366,6 +2424,222 @@ int imau_net_sync(imau_ctx *ctx) {
   ABI_TRY
   if (!ctx) fail(IMAU_ERR_ARG, "NULL argument");
   HIPCHK(hipStreamSynchronize(ctx->s));
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// input producers: update_general_ice_model_data and calc_ice_rheology on
+// host arrays (one grid per call)
+// ---------------------------------------------------------------------------
+namespace {
+
+void check_producer_cfg(const imau_producer_cfg *c) {
+  if (c->nz < 2 || c->nz > IMAU_NZ_MAX) fail(IMAU_ERR_CONFIG, "nz=%d outside [2, %d]", c->nz, IMAU_NZ_MAX);
+  if (c->choice_ice_rheology < IMAU_RHEO_UNIFORM || c->choice_ice_rheology > IMAU_RHEO_MISMIP_MOD)
+    fail(IMAU_ERR_CONFIG, "unknown choice_ice_rheology %d", c->choice_ice_rheology);
+  if (c->choice_ice_margin != IMAU_MARGIN_BC && c->choice_ice_margin != IMAU_MARGIN_INFINITE_SLAB)
+    fail(IMAU_ERR_CONFIG, "unknown choice_ice_margin %d", c->choice_ice_margin);
+}
+
+// device buffers of one grid: 2D fields in a-grid-shaped slabs (offset 0),
+// 3D fields in the host's (nz, ny, nx) order
+struct ProdDev {
+  DBuf<double> f, f3;
+  DBuf<int32_t> m;
+  DBuf<int> flag;
+  imau::ProdIn in{};
+  imau::Prod2 o{};
+  imau::ProdMasks mk{};
+  double *Av = nullptr, *Avcx = nullptr, *Avcy = nullptr;
+  double *A3 = nullptr, *A3cx = nullptr, *A3cy = nullptr, *Ti = nullptr;
+  long long na = 0;
+  int nz = 0;
+  void alloc(const Member &mb, int nz_, bool three_d) {
+    na = (long long)mb.nx * mb.ny;
+    nz = nz_;
+    f.alloc((size_t)22 * na);
+    m.alloc((size_t)10 * na);
+    flag.alloc(1);
+    double *q = f.p;
+    auto take = [&]() { double *r = q; q += na; return r; };
+    double *Hi = take(), *Hb = take(), *SL = take(), *dHi = take(), *dHb = take();
+    in = imau::ProdIn{Hi, Hb, SL, dHi, dHb};
+    o.Hs = take(); o.TAF = take(); o.dHs_dt = take(); o.Hi_cx = take(); o.Hi_cy = take(); o.Hi_b = take();
+    o.dHi_dx = take(); o.dHi_dy = take(); o.dHs_dx = take(); o.dHs_dy = take();
+    o.dHs_dx_cx = take(); o.dHs_dy_cx = take(); o.dHs_dx_cy = take(); o.dHs_dy_cy = take();
+    Av = take(); Avcx = take(); Avcy = take();
+    int32_t *w = m.p;
+    auto mt = [&]() { int32_t *r = w; w += na; return r; };
+    mk.land = mt(); mk.ocean = mt(); mk.ice = mt(); mk.sheet = mt(); mk.shelf = mt();
+    mk.coast = mt(); mk.margin = mt(); mk.gl = mt(); mk.cf = mt(); mk.total = mt();
+    if (three_d) {
+      f3.alloc((size_t)4 * na * nz);
+      A3 = f3.p; A3cx = A3 + na * nz; A3cy = A3cx + na * nz; Ti = A3cy + na * nz;
+    }
+  }
+};
+
+// element strides of a host 3D field with nj points along j
+imau::Str3 host3(int nz, int nj) { return imau::Str3{1, (long long)nz * nj, (long long)nz}; }
+
+// host <-> device copies of 2D fields: grid kind 'a', 'x' (cx), 'y' (cy), 'b'
+template <class T>
+void up2(hipStream_t s, T *dev, const T *host, int nx, int ny, char kind) {
+  const int ni = (kind == 'x' || kind == 'b') ? nx - 1 : nx, nj = (kind == 'y' || kind == 'b') ? ny - 1 : ny;
+  HIPCHK(hipMemcpy2DAsync(dev, (size_t)ny * sizeof(T), host, (size_t)nj * sizeof(T), (size_t)nj * sizeof(T), ni,
+                          hipMemcpyHostToDevice, s));
+}
+template <class T>
+void down2(hipStream_t s, T *host, const T *dev, int nx, int ny, char kind) {
+  if (!host) return;
+  const int ni = (kind == 'x' || kind == 'b') ? nx - 1 : nx, nj = (kind == 'y' || kind == 'b') ? ny - 1 : ny;
+  HIPCHK(hipMemcpy2DAsync(host, (size_t)nj * sizeof(T), dev, (size_t)ny * sizeof(T), (size_t)nj * sizeof(T), ni,
+                          hipMemcpyDeviceToHost, s));
+}
+
+void download_masks(hipStream_t s, const ProdDev &D, imau_producer_io *io, int nx, int ny) {
+  down2(s, io->mask_land_a, D.mk.land, nx, ny, 'a');
+  down2(s, io->mask_ocean_a, D.mk.ocean, nx, ny, 'a');
+  down2(s, io->mask_ice_a, D.mk.ice, nx, ny, 'a');
+  down2(s, io->mask_sheet_a, D.mk.sheet, nx, ny, 'a');
+  down2(s, io->mask_shelf_a, D.mk.shelf, nx, ny, 'a');
+  down2(s, io->mask_coast_a, D.mk.coast, nx, ny, 'a');
+  down2(s, io->mask_margin_a, D.mk.margin, nx, ny, 'a');
+  down2(s, io->mask_gl_a, D.mk.gl, nx, ny, 'a');
+  down2(s, io->mask_cf_a, D.mk.cf, nx, ny, 'a');
+  down2(s, io->mask_a, D.mk.total, nx, ny, 'a');
+}
+
+}  // namespace
+
+extern "C" {
+
+void imau_producer_default_config(imau_producer_cfg *c) {
+  std::memset(c, 0, sizeof *c);
+  const double zeta[15] = {0.00, 0.10, 0.20, 0.30, 0.40, 0.50, 0.60, 0.70, 0.80, 0.90, 0.925, 0.95, 0.975, 0.99, 1.00};
+  c->nz = 15;
+  for (int k = 0; k < 15; ++k) c->zeta[k] = zeta[k];
+  c->choice_ice_margin = IMAU_MARGIN_INFINITE_SLAB;
+  c->do_ocean_floodfill = 1;
+  c->choice_ice_rheology = IMAU_RHEO_HUYBRECHTS1992;
+  c->uniform_flow_factor = 1e-16;
+  c->m_enh_sheet = 1.0;
+  c->m_enh_shelf = 1.0;
+}
+
+int imau_update_general_ice_model_data(imau_ctx *ctx, const imau_grid *grid, const imau_producer_cfg *cfg,
+                                       imau_producer_io *io) {
+  ABI_TRY
+  if (!ctx || !grid || !cfg || !io) fail(IMAU_ERR_ARG, "NULL argument");
+  check_producer_cfg(cfg);
+  if (grid->nx < 4 || grid->ny < 4) fail(IMAU_ERR_ARG, "grid must be at least 4x4");
+  if (!io->Hi_a || !io->Hb_a || !io->SL_a) fail(IMAU_ERR_ARG, "Hi_a, Hb_a and SL_a are required");
+  const int nx = grid->nx, ny = grid->ny, nz = cfg->nz;
+  const Member mb{nx, ny, grid->dx, 0, 0, 0, 0};
+  const bool three_d = io->A_flow_3D_a && (io->A_flow_3D_cx || io->A_flow_3D_cy);
+  ProdDev D;
+  D.alloc(mb, nz, three_d);
+  hipStream_t s = ctx->s;
+  up2(s, (double *)D.in.Hi, io->Hi_a, nx, ny, 'a');
+  up2(s, (double *)D.in.Hb, io->Hb_a, nx, ny, 'a');
+  up2(s, (double *)D.in.SL, io->SL_a, nx, ny, 'a');
+  imau::ProdIn in = D.in;
+  const bool rates = io->dHi_dt_a && io->dHb_dt_a && io->dHs_dt_a && io->mask_land_a;
+  if (rates) {
+    up2(s, (double *)in.dHi_dt, io->dHi_dt_a, nx, ny, 'a');
+    up2(s, (double *)in.dHb_dt, io->dHb_dt_a, nx, ny, 'a');
+    up2(s, D.mk.land, io->mask_land_a, nx, ny, 'a');  // the entry mask dHs/dt reads
+  } else {
+    in.dHi_dt = in.dHb_dt = nullptr;
+  }
+  imau::prod_point(s, mb, in, D.o, D.mk, cfg->do_ocean_floodfill ? 1 : 0);
+  if (cfg->do_ocean_floodfill) {
+    // rounds of tile-local growth until a round changes nothing; every round
+    // that changes something adds a cell, so at most na rounds
+    int changed = 1;
+    for (long long round = 0; changed && round <= D.na; ++round) {
+      HIPCHK(hipMemsetAsync(D.flag.p, 0, sizeof(int), s));
+      imau::prod_flood_round(s, mb, in, D.mk, D.flag.p);
+      HIPCHK(hipMemcpyAsync(&changed, D.flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+    }
+    imau::prod_land(s, mb, D.mk);
+  }
+  imau::prod_masks_maps(s, mb, in, D.o, D.mk);
+  if (io->A_flow_vav_a) up2(s, D.Av, io->A_flow_vav_a, nx, ny, 'a');
+  if (three_d)
+    HIPCHK(hipMemcpyAsync(D.A3, io->A_flow_3D_a, (size_t)D.na * nz * sizeof(double), hipMemcpyHostToDevice, s));
+  imau::prod_map_A(s, mb, nz, three_d ? D.A3 : nullptr, host3(nz, ny), D.A3cx, host3(nz, ny), D.A3cy, host3(nz, ny - 1),
+                   io->A_flow_vav_a ? D.Av : nullptr, D.Avcx, D.Avcy);
+  HIPCHK(hipGetLastError());
+  down2(s, io->Hs_a, D.o.Hs, nx, ny, 'a');
+  down2(s, io->TAF_a, D.o.TAF, nx, ny, 'a');
+  if (rates) down2(s, io->dHs_dt_a, D.o.dHs_dt, nx, ny, 'a');
+  download_masks(s, D, io, nx, ny);
+  down2(s, io->Hi_cx, D.o.Hi_cx, nx, ny, 'x');
+  down2(s, io->Hi_cy, D.o.Hi_cy, nx, ny, 'y');
+  down2(s, io->Hi_b, D.o.Hi_b, nx, ny, 'b');
+  down2(s, io->dHi_dx_a, D.o.dHi_dx, nx, ny, 'a');
+  down2(s, io->dHi_dy_a, D.o.dHi_dy, nx, ny, 'a');
+  down2(s, io->dHs_dx_a, D.o.dHs_dx, nx, ny, 'a');
+  down2(s, io->dHs_dy_a, D.o.dHs_dy, nx, ny, 'a');
+  down2(s, io->dHs_dx_cx, D.o.dHs_dx_cx, nx, ny, 'x');
+  down2(s, io->dHs_dy_cx, D.o.dHs_dy_cx, nx, ny, 'x');
+  down2(s, io->dHs_dx_cy, D.o.dHs_dx_cy, nx, ny, 'y');
+  down2(s, io->dHs_dy_cy, D.o.dHs_dy_cy, nx, ny, 'y');
+  if (io->A_flow_vav_a) {
+    down2(s, io->A_flow_vav_cx, D.Avcx, nx, ny, 'x');
+    down2(s, io->A_flow_vav_cy, D.Avcy, nx, ny, 'y');
+  }
+  if (three_d) {
+    if (io->A_flow_3D_cx)
+      HIPCHK(hipMemcpyAsync(io->A_flow_3D_cx, D.A3cx, (size_t)(nx - 1) * ny * nz * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+    if (io->A_flow_3D_cy)
+      HIPCHK(hipMemcpyAsync(io->A_flow_3D_cy, D.A3cy, (size_t)nx * (ny - 1) * nz * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+int imau_calc_ice_rheology(imau_ctx *ctx, const imau_grid *grid, const imau_producer_cfg *cfg, imau_producer_io *io,
+                           double time) {
+  ABI_TRY
+  if (!ctx || !grid || !cfg || !io) fail(IMAU_ERR_ARG, "NULL argument");
+  check_producer_cfg(cfg);
+  if (grid->nx < 4 || grid->ny < 4) fail(IMAU_ERR_ARG, "grid must be at least 4x4");
+  if (!io->A_flow_3D_a || !io->A_flow_vav_a) fail(IMAU_ERR_ARG, "calc_ice_rheology writes A_flow_3D_a and A_flow_vav_a");
+  if (!io->mask_ice_a || !io->mask_sheet_a || !io->mask_shelf_a)
+    fail(IMAU_ERR_ARG, "calc_ice_rheology reads mask_ice_a, mask_sheet_a and mask_shelf_a");
+  const bool huyb = cfg->choice_ice_rheology == IMAU_RHEO_HUYBRECHTS1992;
+  if (huyb && !io->Ti_a) fail(IMAU_ERR_ARG, "Huybrechts1992 reads Ti_a");
+  const int nx = grid->nx, ny = grid->ny, nz = cfg->nz;
+  const Member mb{nx, ny, grid->dx, 0, 0, 0, 0};
+  ProdDev D;
+  D.alloc(mb, nz, true);
+  hipStream_t s = ctx->s;
+  up2(s, D.mk.ice, io->mask_ice_a, nx, ny, 'a');
+  up2(s, D.mk.sheet, io->mask_sheet_a, nx, ny, 'a');
+  up2(s, D.mk.shelf, io->mask_shelf_a, nx, ny, 'a');
+  if (huyb) HIPCHK(hipMemcpyAsync(D.Ti, io->Ti_a, (size_t)D.na * nz * sizeof(double), hipMemcpyHostToDevice, s));
+  imau::prod_rheology(s, mb, *cfg, time, D.Ti, host3(nz, ny), D.mk, D.A3, host3(nz, ny), D.Av);
+  imau::prod_map_A(s, mb, nz, D.A3, host3(nz, ny), D.A3cx, host3(nz, ny), D.A3cy, host3(nz, ny - 1), D.Av, D.Avcx,
+                   D.Avcy);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(io->A_flow_3D_a, D.A3, (size_t)D.na * nz * sizeof(double), hipMemcpyDeviceToHost, s));
+  down2(s, io->A_flow_vav_a, D.Av, nx, ny, 'a');
+  if (io->A_flow_3D_cx)
+    HIPCHK(hipMemcpyAsync(io->A_flow_3D_cx, D.A3cx, (size_t)(nx - 1) * ny * nz * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (io->A_flow_3D_cy)
+    HIPCHK(hipMemcpyAsync(io->A_flow_3D_cy, D.A3cy, (size_t)nx * (ny - 1) * nz * sizeof(double), hipMemcpyDeviceToHost, s));
+  down2(s, io->A_flow_vav_cx, D.Avcx, nx, ny, 'x');
+  down2(s, io->A_flow_vav_cy, D.Avcy, nx, ny, 'y');
+  HIPCHK(hipStreamSynchronize(s));
   return IMAU_OK;
   ABI_CATCH(ctx)
 }

@@ -66,6 +66,8 @@ void member_sum2(hipStream_t s, int ngrids, const int *tstart, const double *par
 void fill_u64(hipStream_t s, unsigned long long *p, int n, unsigned long long v);
 // dst = src on the 3D slabs of the members with flag[g] != 0
 void copy3_members(const Launch &L, int nz, const double *src, double *dst, const int *flag);
+// (du, dv) = (su, sv) on the 2D slabs of the members with flag[g] != 0
+void copy2_members(const Launch &L, const double *su, const double *sv, double *du, double *dv, const int *flag);
 void copy3_members(const Launch &L, int nz, const float *src, float *dst, const int *flag);
 // the state a problem was created with (velocities, basal stress, 3D viscosity)
 struct ResetSrc {

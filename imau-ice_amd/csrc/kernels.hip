@@ -385,6 +385,91 @@ __global__ void __launch_bounds__(IMAU_WG) k_strain(const Member *__restrict__ m
   }
 }
 
+// Per-column constants of the 3D viscosity, shared by both viscosity kernels:
+// the basal stresses and calc_staggered_margin choices of the vertical shear
+// terms (:691-692, :704-705, :775-788) and the column's strain invariant.
+template <class T>
+struct ViscCol {
+  T tbl, tbr, tcl, tcr, eps_h, HiT, vmin, expo;
+  int sl, sr, sb, st, sxa, sya;
+  bool noice_l, noice_r, noice_b, noice_t, use_avg_x, use_avg_y, i0, ie, j0, je;
+};
+
+template <bool DIVA, class T>
+__device__ __forceinline__ ViscCol<T> visc_col(const Member &m, const KP &kp, const Fields &f, int j, int i,
+                                               double Hi, double eps_h) {
+  const int nx = m.nx, ny = m.ny;
+  ViscCol<T> c;
+  const int icl = (i == 0) ? 0 : (i == nx - 1 ? nx - 2 : i - 1);  // left cx index used
+  const int icr = (i == 0) ? 0 : (i == nx - 1 ? nx - 2 : i);      // right cx index used
+  const int jcl = (j == 0) ? 0 : (j == ny - 1 ? ny - 2 : j - 1);
+  const int jcr = (j == 0) ? 0 : (j == ny - 1 ? ny - 2 : j);
+  c.use_avg_x = (i > 0 && i < nx - 1);
+  c.use_avg_y = (j > 0 && j < ny - 1);
+  c.i0 = i == 0; c.ie = i == nx - 1; c.j0 = j == 0; c.je = j == ny - 1;
+  c.tbl = c.tbr = c.tcl = c.tcr = (T)0.0;
+  double Hl0 = 0, Hl1 = 0, Hr0 = 0, Hr1 = 0, Hb0 = 0, Hb1 = 0, Ht0 = 0, Ht1 = 0;
+  c.noice_l = c.noice_r = c.noice_b = c.noice_t = true;
+  if (DIVA) {
+    c.tbl = (T)f.taub_cx[p2(m, j, icl)]; c.tbr = (T)f.taub_cx[p2(m, j, icr)];
+    c.tcl = (T)f.taub_cy[p2(m, jcl, i)]; c.tcr = (T)f.taub_cy[p2(m, jcr, i)];
+    Hl0 = f.Hi[p2(m, j, icl)]; Hl1 = f.Hi[p2(m, j, icl + 1)];
+    Hr0 = f.Hi[p2(m, j, icr)]; Hr1 = f.Hi[p2(m, j, icr + 1)];
+    Hb0 = f.Hi[p2(m, jcl, i)]; Hb1 = f.Hi[p2(m, jcl + 1, i)];
+    Ht0 = f.Hi[p2(m, jcr, i)]; Ht1 = f.Hi[p2(m, jcr + 1, i)];
+    c.noice_l = f.m_ice[p2(m, j, icl)] == 0 && f.m_ice[p2(m, j, icl + 1)] == 0;
+    c.noice_r = f.m_ice[p2(m, j, icr)] == 0 && f.m_ice[p2(m, j, icr + 1)] == 0;
+    c.noice_b = f.m_ice[p2(m, jcl, i)] == 0 && f.m_ice[p2(m, jcl + 1, i)] == 0;
+    c.noice_t = f.m_ice[p2(m, jcr, i)] == 0 && f.m_ice[p2(m, jcr + 1, i)] == 0;
+  }
+  const double Hxl = c.use_avg_x ? f.Hi[p2(m, j, i - 1)] : 0.0;
+  const double Hyl = c.use_avg_y ? f.Hi[p2(m, j - 1, i)] : 0.0;
+  // calc_staggered_margin choices, fixed per column: 0 first, 1 second, 2 mean
+  c.sl = stag_sel(Hl0, Hl1); c.sr = stag_sel(Hr0, Hr1); c.sb = stag_sel(Hb0, Hb1); c.st = stag_sel(Ht0, Ht1);
+  c.sxa = stag_sel(Hxl, Hi); c.sya = stag_sel(Hyl, Hi);
+  c.expo = (T)((1.0 - kp.n_flow) / (2.0 * kp.n_flow));
+  c.HiT = (T)Hi; c.eps_h = (T)eps_h; c.vmin = (T)kp.visc_min;
+  return c;
+}
+
+// The viscosity of one level from the previous viscosity at the centre and
+// its four neighbours (out-of-grid neighbours 0) and the rate-factor term B
+// (:664-721, :775-806).  Multiply-adds are not contracted, so every kernel
+// that calls it rounds alike whatever the surrounding code.
+template <bool DIVA, bool THIRD, class T>
+__device__ __forceinline__ T visc_level(const ViscCol<T> &c, T z, T vc, T vw, T ve, T vs, T vn, T B) {
+#pragma clang fp contract(off)
+  T du_dz_a = (T)0.0, dv_dz_a = (T)0.0;
+  if (DIVA) {
+    T dl = (T)0.0, dr = (T)0.0, db = (T)0.0, dt = (T)0.0;
+    if (!c.noice_l) dl = (c.tbl / stag_pick<T>(c.i0 ? vc : vw, c.i0 ? ve : vc, c.sl)) * z;
+    if (!c.noice_r) dr = (c.tbr / stag_pick<T>(c.ie ? vw : vc, c.ie ? vc : ve, c.sr)) * z;
+    if (!c.noice_b) db = (c.tcl / stag_pick<T>(c.j0 ? vc : vs, c.j0 ? vn : vc, c.sb)) * z;
+    if (!c.noice_t) dt = (c.tcr / stag_pick<T>(c.je ? vs : vc, c.je ? vc : vn, c.st)) * z;
+    du_dz_a = c.use_avg_x ? stag_pick<T>(dl, dr, c.sxa) : dl;
+    dv_dz_a = c.use_avg_y ? stag_pick<T>(db, dt, c.sya) : db;
+  }
+  const T eps_sq = c.eps_h + (T)0.25 * (du_dz_a * du_dz_a + dv_dz_a * dv_dz_a);
+  const T eta = THIRD ? B / cbrt(eps_sq) : B * pow(eps_sq, c.expo);
+  return fmax(eta, c.vmin);  // NaN (cold start 0/0) -> visc_eff_min
+}
+
+// trapezoid steps of the viscosity integral and the F2 integrand (H/eta) z^2
+// (vertical_integrate, src/utilities_module.f90:73-96; :960-1013)
+template <class T>
+__device__ __forceinline__ void visc_trap(int k, T z, T zprev, T eta, T HiT, T &vint, T &F2int, T &eta_prev,
+                                          T &g_prev) {
+#pragma clang fp contract(off)
+  const T gk = (HiT / eta) * (z * z);
+  if (k > 0) {
+    const T dz = z - zprev;
+    vint += (T)0.5 * (eta + eta_prev) * dz;
+    F2int += (T)0.5 * (gk + g_prev) * dz;
+  }
+  eta_prev = eta;
+  g_prev = gk;
+}
+
 template <bool DIVA, bool THIRD, class T>
 __global__ void __launch_bounds__(IMAU_WG) k_visc(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
                                                   const KP kp, const Fields f, const T *__restrict__ visc_in,
@@ -401,9 +486,6 @@ __global__ void __launch_bounds__(IMAU_WG) k_visc(const Member *__restrict__ mem
     }
     return;
   }
-  const double dx = m.dx;
-  const double *__restrict__ u = f.u;
-  const double *__restrict__ v = f.v;
   FOR_VTILE_ROWS {
     const int i = t.i0 + ii;
     if (i >= nx) break;
@@ -417,96 +499,55 @@ __global__ void __launch_bounds__(IMAU_WG) k_visc(const Member *__restrict__ mem
     if (bc_skip) {
       for (int k = 0; k < nz; ++k) visc_out[p3(m, k, j, i)] = (T)0.0;
     } else {
-      // neighbour data for the vertical shear terms (:691-692, :704-705, :775-788)
-      const int icl = (i == 0) ? 0 : (i == nx - 1 ? nx - 2 : i - 1);  // left cx index used
-      const int icr = (i == 0) ? 0 : (i == nx - 1 ? nx - 2 : i);      // right cx index used
-      const int jcl = (j == 0) ? 0 : (j == ny - 1 ? ny - 2 : j - 1);
-      const int jcr = (j == 0) ? 0 : (j == ny - 1 ? ny - 2 : j);
-      const bool use_avg_x = (i > 0 && i < nx - 1), use_avg_y = (j > 0 && j < ny - 1);
-      T tbl = 0, tbr = 0, tcl = 0, tcr = 0;
-      double Hl0 = 0, Hl1 = 0, Hr0 = 0, Hr1 = 0, Hb0 = 0, Hb1 = 0, Ht0 = 0, Ht1 = 0;
-      bool noice_l = true, noice_r = true, noice_b = true, noice_t = true;
-      if (DIVA) {
-        tbl = (T)f.taub_cx[p2(m, j, icl)]; tbr = (T)f.taub_cx[p2(m, j, icr)];
-        tcl = (T)f.taub_cy[p2(m, jcl, i)]; tcr = (T)f.taub_cy[p2(m, jcr, i)];
-        Hl0 = f.Hi[p2(m, j, icl)]; Hl1 = f.Hi[p2(m, j, icl + 1)];
-        Hr0 = f.Hi[p2(m, j, icr)]; Hr1 = f.Hi[p2(m, j, icr + 1)];
-        Hb0 = f.Hi[p2(m, jcl, i)]; Hb1 = f.Hi[p2(m, jcl + 1, i)];
-        Ht0 = f.Hi[p2(m, jcr, i)]; Ht1 = f.Hi[p2(m, jcr + 1, i)];
-        noice_l = f.m_ice[p2(m, j, icl)] == 0 && f.m_ice[p2(m, j, icl + 1)] == 0;
-        noice_r = f.m_ice[p2(m, j, icr)] == 0 && f.m_ice[p2(m, j, icr + 1)] == 0;
-        noice_b = f.m_ice[p2(m, jcl, i)] == 0 && f.m_ice[p2(m, jcl + 1, i)] == 0;
-        noice_t = f.m_ice[p2(m, jcr, i)] == 0 && f.m_ice[p2(m, jcr + 1, i)] == 0;
-      }
-      const double Hxl = use_avg_x ? f.Hi[p2(m, j, i - 1)] : 0.0;
-      const double Hyl = use_avg_y ? f.Hi[p2(m, j - 1, i)] : 0.0;
-      // calc_staggered_margin choices, fixed per column: 0 first, 1 second, 2 mean
-      const int sl = stag_sel(Hl0, Hl1), sr = stag_sel(Hr0, Hr1), sb = stag_sel(Hb0, Hb1), st = stag_sel(Ht0, Ht1);
-      const int sxa = stag_sel(Hxl, Hi), sya = stag_sel(Hyl, Hi);
-      const T expo = (T)((1.0 - kp.n_flow) / (2.0 * kp.n_flow));
-      const T HiT = (T)Hi, eps_hT = (T)eps_h, vmin = (T)kp.visc_min;
+      const ViscCol<T> cc = visc_col<DIVA, T>(m, kp, f, j, i, Hi, eps_h);
       T eta_prev_k = (T)0.0, g_prev_k = (T)0.0;
       // the previous viscosity at the centre and its four neighbours and the
-      // rate-factor term, loaded one level ahead so that the loads of level
-      // k + 1 are in flight while level k is computed
+      // rate-factor term, loaded IMAU_VISC_PF levels ahead into a ring of
+      // registers, so that the loads of the next levels are in flight while
+      // level k is computed
+      constexpr int PF = IMAU_VISC_PF;
       const long long plane = (long long)nx * ny;
       const long long pc = p3(m, 0, j, i);
       const long long ow = (i > 0) ? -(long long)ny : 0, oe = (i < nx - 1) ? (long long)ny : 0;
       const long long os = (j > 0) ? -1 : 0, on = (j < ny - 1) ? 1 : 0;
-      T nc = (T)0.0, nw = (T)0.0, ne = (T)0.0, ns = (T)0.0, nn = (T)0.0;
-      if (DIVA) {
-        nc = visc_in[pc];
-        nw = visc_in[pc + ow];
-        ne = visc_in[pc + oe];
-        ns = visc_in[pc + os];
-        nn = visc_in[pc + on];
-      }
-      T nB = B3[pc];
-#pragma unroll 1
-      for (int k = 0; k < nz; ++k) {
+      T rc[PF], rw[PF], re[PF], rs[PF], rn[PF], rB[PF];
+      auto load = [&](int k, int sl) {
+        const long long q = pc + k * plane;
+        if (DIVA) {
+          rc[sl] = visc_in[q];
+          rw[sl] = visc_in[q + ow];
+          re[sl] = visc_in[q + oe];
+          rs[sl] = visc_in[q + os];
+          rn[sl] = visc_in[q + on];
+        }
+        rB[sl] = B3[q];
+      };
+#pragma unroll
+      for (int sl = 0; sl < PF; ++sl)
+        if (sl < nz) load(sl, sl);
+      auto level = [&](int k, int sl) {
         const T z = (T)kp.zeta[k];
         // out-of-grid neighbours read the centre (an in-bounds address) and are
         // replaced by 0, as the reference's rows never use them
-        const T vc = nc, vw = (i > 0) ? nw : (T)0.0, ve = (i < nx - 1) ? ne : (T)0.0;
-        const T vs = (j > 0) ? ns : (T)0.0, vn = (j < ny - 1) ? nn : (T)0.0;
-        const T B = nB;
-        if (k + 1 < nz) {
-          const long long q = pc + (k + 1) * plane;
-          if (DIVA) {
-            nc = visc_in[q];
-            nw = visc_in[q + ow];
-            ne = visc_in[q + oe];
-            ns = visc_in[q + os];
-            nn = visc_in[q + on];
-          }
-          nB = B3[q];
-        }
-        T du_dz_a = (T)0.0, dv_dz_a = (T)0.0;
+        T vc = (T)0.0, vw = (T)0.0, ve = (T)0.0, vs = (T)0.0, vn = (T)0.0;
         if (DIVA) {
-          // du/dz on the two cx and cy neighbours (calc_vertical_shear_strain_rates);
-          // the pairs (icl, icl+1), (icr, icr+1), (jcl, jcl+1), (jcr, jcr+1) are
-          // drawn from the centre and its four neighbours
-          T dl = (T)0.0, dr = (T)0.0, db = (T)0.0, dt = (T)0.0;
-          if (!noice_l) dl = (tbl / stag_pick<T>((i == 0) ? vc : vw, (i == 0) ? ve : vc, sl)) * z;
-          if (!noice_r) dr = (tbr / stag_pick<T>((i == nx - 1) ? vw : vc, (i == nx - 1) ? vc : ve, sr)) * z;
-          if (!noice_b) db = (tcl / stag_pick<T>((j == 0) ? vc : vs, (j == 0) ? vn : vc, sb)) * z;
-          if (!noice_t) dt = (tcr / stag_pick<T>((j == ny - 1) ? vs : vc, (j == ny - 1) ? vc : vn, st)) * z;
-          du_dz_a = use_avg_x ? stag_pick<T>(dl, dr, sxa) : dl;
-          dv_dz_a = use_avg_y ? stag_pick<T>(db, dt, sya) : db;
+          vc = rc[sl];
+          vw = (i > 0) ? rw[sl] : (T)0.0;
+          ve = (i < nx - 1) ? re[sl] : (T)0.0;
+          vs = (j > 0) ? rs[sl] : (T)0.0;
+          vn = (j < ny - 1) ? rn[sl] : (T)0.0;
         }
-        const T eps_sq = eps_hT + (T)0.25 * (du_dz_a * du_dz_a + dv_dz_a * dv_dz_a);
-        T eta = THIRD ? B / cbrt(eps_sq) : B * pow(eps_sq, expo);
-        eta = fmax(eta, vmin);  // NaN (cold start 0/0) -> visc_eff_min
+        const T B = rB[sl];
+        if (k + PF < nz) load(k + PF, sl);
+        const T eta = visc_level<DIVA, THIRD, T>(cc, z, vc, vw, ve, vs, vn, B);
         visc_out[pc + k * plane] = eta;
-        const T gk = (HiT / eta) * (z * z);
-        if (k > 0) {
-          const T dz = z - (T)kp.zeta[k - 1];
-          vint += (T)0.5 * (eta + eta_prev_k) * dz;
-          F2int += (T)0.5 * (gk + g_prev_k) * dz;
-        }
-        eta_prev_k = eta;
-        g_prev_k = gk;
-      }
+        visc_trap<T>(k, z, k > 0 ? (T)kp.zeta[k - 1] : (T)0.0, eta, cc.HiT, vint, F2int, eta_prev_k, g_prev_k);
+      };
+#pragma unroll 1
+      for (int k0 = 0; k0 < nz; k0 += PF)
+#pragma unroll
+        for (int sl = 0; sl < PF; ++sl)
+          if (k0 + sl < nz) level(k0 + sl, sl);
       f.visc_int[p] = (double)vint;
       f.N_a[p] = (double)vint * ((kp.margin == 0) ? Hi : fmax(0.1, Hi));
     }
@@ -521,8 +562,9 @@ __global__ void __launch_bounds__(IMAU_WG) k_visc(const Member *__restrict__ mem
 // j0-1 and j0+64 loaded by the edge lanes) sits in an LDS ring of three rows,
 // and the tile advances one row per step, loading only row i+1: each level of
 // visc_in is read once per row plus two halo rows per tile, instead of five
-// times (the centre and four neighbours) as in k_visc.  Arithmetic and results
-// are those of k_visc<true, ...> exactly.
+// times (the centre and four neighbours) as in k_visc.  Both kernels compute
+// each level with visc_level and visc_trap, so the results are the same bit
+// for bit.
 template <bool THIRD, class T>
 __global__ void __launch_bounds__(64) k_visc_march(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
                                                    const KP kp, const Fields f, const T *__restrict__ visc_in,
@@ -555,8 +597,6 @@ __global__ void __launch_bounds__(64) k_visc_march(const Member *__restrict__ me
   int sp = 0, sc = 1, sn = 2;
   load_row(t.i0 - 1, sp);
   load_row(t.i0, sc);
-  const T expo = (T)((1.0 - kp.n_flow) / (2.0 * kp.n_flow));
-  const T vmin = (T)kp.visc_min;
 #pragma unroll 1
   for (int i = t.i0; i < i1; ++i) {
     load_row(i + 1, sn);
@@ -567,54 +607,17 @@ __global__ void __launch_bounds__(64) k_visc_march(const Member *__restrict__ me
       const int mi = f.m_ice[p];
       const double eps_h = f.eps_h[p];
       const bool bc_skip = (kp.margin == 0 && mi == 0);
-      // neighbour data for the vertical shear terms (:691-692, :704-705, :775-788)
-      const int icl = (i == 0) ? 0 : (i == nx - 1 ? nx - 2 : i - 1);
-      const int icr = (i == 0) ? 0 : (i == nx - 1 ? nx - 2 : i);
-      const int jcl = (j == 0) ? 0 : (j == ny - 1 ? ny - 2 : j - 1);
-      const int jcr = (j == 0) ? 0 : (j == ny - 1 ? ny - 2 : j);
-      const bool use_avg_x = (i > 0 && i < nx - 1), use_avg_y = (j > 0 && j < ny - 1);
-      const T tbl = (T)f.taub_cx[p2(m, j, icl)], tbr = (T)f.taub_cx[p2(m, j, icr)];
-      const T tcl = (T)f.taub_cy[p2(m, jcl, i)], tcr = (T)f.taub_cy[p2(m, jcr, i)];
-      const double Hl0 = f.Hi[p2(m, j, icl)], Hl1 = f.Hi[p2(m, j, icl + 1)];
-      const double Hr0 = f.Hi[p2(m, j, icr)], Hr1 = f.Hi[p2(m, j, icr + 1)];
-      const double Hb0 = f.Hi[p2(m, jcl, i)], Hb1 = f.Hi[p2(m, jcl + 1, i)];
-      const double Ht0 = f.Hi[p2(m, jcr, i)], Ht1 = f.Hi[p2(m, jcr + 1, i)];
-      const bool noice_l = f.m_ice[p2(m, j, icl)] == 0 && f.m_ice[p2(m, j, icl + 1)] == 0;
-      const bool noice_r = f.m_ice[p2(m, j, icr)] == 0 && f.m_ice[p2(m, j, icr + 1)] == 0;
-      const bool noice_b = f.m_ice[p2(m, jcl, i)] == 0 && f.m_ice[p2(m, jcl + 1, i)] == 0;
-      const bool noice_t = f.m_ice[p2(m, jcr, i)] == 0 && f.m_ice[p2(m, jcr + 1, i)] == 0;
-      const double Hxl = use_avg_x ? f.Hi[p2(m, j, i - 1)] : 0.0;
-      const double Hyl = use_avg_y ? f.Hi[p2(m, j - 1, i)] : 0.0;
-      const int sl = stag_sel(Hl0, Hl1), sr = stag_sel(Hr0, Hr1), sb = stag_sel(Hb0, Hb1), st = stag_sel(Ht0, Ht1);
-      const int sxa = stag_sel(Hxl, Hi), sya = stag_sel(Hyl, Hi);
-      const T HiT = (T)Hi, eps_hT = (T)eps_h;
+      const ViscCol<T> cc = visc_col<true, T>(m, kp, f, j, i, Hi, eps_h);
       T vint = (T)0.0, F2int = (T)0.0, eta_prev_k = (T)0.0, g_prev_k = (T)0.0;
 #pragma unroll 1
       for (int k = 0; k < NZ; ++k) {
         const T vc = ring[sc][k][lane + 1], vw = ring[sp][k][lane + 1], ve = ring[sn][k][lane + 1];
         const T vs = ring[sc][k][lane], vn = ring[sc][k][lane + 2];
         const T z = (T)kp.zeta[k];
-        T dl = (T)0.0, dr = (T)0.0, db = (T)0.0, dt = (T)0.0;
-        if (!noice_l) dl = (tbl / stag_pick<T>((i == 0) ? vc : vw, (i == 0) ? ve : vc, sl)) * z;
-        if (!noice_r) dr = (tbr / stag_pick<T>((i == nx - 1) ? vw : vc, (i == nx - 1) ? vc : ve, sr)) * z;
-        if (!noice_b) db = (tcl / stag_pick<T>((j == 0) ? vc : vs, (j == 0) ? vn : vc, sb)) * z;
-        if (!noice_t) dt = (tcr / stag_pick<T>((j == ny - 1) ? vs : vc, (j == ny - 1) ? vc : vn, st)) * z;
-        const T du_dz_a = use_avg_x ? stag_pick<T>(dl, dr, sxa) : dl;
-        const T dv_dz_a = use_avg_y ? stag_pick<T>(db, dt, sya) : db;
-        const T eps_sq = eps_hT + (T)0.25 * (du_dz_a * du_dz_a + dv_dz_a * dv_dz_a);
-        const T B = B3[p3(m, k, j, i)];
-        T eta = THIRD ? B / cbrt(eps_sq) : B * pow(eps_sq, expo);
-        eta = fmax(eta, vmin);  // NaN (cold start 0/0) -> visc_eff_min
+        T eta = visc_level<true, THIRD, T>(cc, z, vc, vw, ve, vs, vn, B3[p3(m, k, j, i)]);
         if (bc_skip) eta = (T)0.0;
         visc_out[p3(m, k, j, i)] = eta;
-        const T gk = (HiT / eta) * (z * z);
-        if (k > 0) {
-          const T dz = z - (T)kp.zeta[k - 1];
-          vint += (T)0.5 * (eta + eta_prev_k) * dz;
-          F2int += (T)0.5 * (gk + g_prev_k) * dz;
-        }
-        eta_prev_k = eta;
-        g_prev_k = gk;
+        visc_trap<T>(k, z, k > 0 ? (T)kp.zeta[k - 1] : (T)0.0, eta, cc.HiT, vint, F2int, eta_prev_k, g_prev_k);
       }
       if (!bc_skip) {
         f.visc_int[p] = (double)vint;
@@ -698,7 +701,12 @@ __global__ void __launch_bounds__(IMAU_WG) k_beta(const Member *__restrict__ mem
 // column's own values; its first iteration restores them.  Writes the
 // viscosity (in place), visc_int, N_a, F2, beta_a and beta_eff_a.
 // ---------------------------------------------------------------------------
-template <bool THIRD, class T>
+// SLID >= 0: the kernel carries only that sliding law.  NZ > 0: the level
+// count is fixed at compile time and the column's viscosity is kept in LDS
+// across the iterations (one slot per level and thread, level-major, so the
+// lanes of a wavefront touch consecutive words); NZ = 0 reads kp.nz and keeps
+// it in a private array.
+template <bool THIRD, class T, int SLID, int NZ>
 __global__ void __launch_bounds__(IMAU_WG) k_visc_column(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
                                                          const KP kp, const Fields f, T *__restrict__ visc,
                                                          const T *__restrict__ B3, const double *__restrict__ xg,
@@ -706,7 +714,13 @@ __global__ void __launch_bounds__(IMAU_WG) k_visc_column(const Member *__restric
                                                          const int *__restrict__ active) {
   TILE_SETUP
   if (j >= m.ny || !active[t.g]) return;
-  const int nx = m.nx, ny = m.ny, nz = kp.nz;
+  const int nx = m.nx, ny = m.ny;
+  const int nz = NZ > 0 ? NZ : kp.nz;
+  const int slid = SLID >= 0 ? SLID : kp.slid;
+  __shared__ T s_vk[NZ > 0 ? NZ : 1][IMAU_WG];
+  T p_vk[NZ > 0 ? 1 : IMAU_NZMAX];
+  const int tid = threadIdx.y * IMAU_TJ + threadIdx.x;
+  auto vk = [&](int k) -> T & { return NZ > 0 ? s_vk[NZ > 0 ? k : 0][tid] : p_vk[NZ > 0 ? 0 : k]; };
   FOR_VTILE_ROWS {
     const int i = t.i0 + ii;
     if (i >= nx) break;
@@ -718,9 +732,9 @@ __global__ void __launch_bounds__(IMAU_WG) k_visc_column(const Member *__restric
     const double va = (j == 0) ? f.v[p2(m, 0, i)]
                       : (j == ny - 1 ? f.v[p2(m, ny - 2, i)] : 0.5 * (f.v[p2(m, j - 1, i)] + f.v[p]));
     double tauc = f.tauc[p];
-    if (kp.slid == 3 || (kp.slid == 2 && !(kp.rough == 1 && kp.prough == 1)))
+    if (slid == 3 || (slid == 2 && !(kp.rough == 1 && kp.prough == 1)))
       tauc = tan((D_PI / 180.0) * f.phi_fric[p]) * f.Neff[p];
-    double beta = sliding_beta(kp, kp.slid, ua, va, f.beta_sq[p], f.alpha_sq[p], f.phi_fric[p], tauc, f.Neff[p],
+    double beta = sliding_beta(kp, slid, ua, va, f.beta_sq[p], f.alpha_sq[p], f.phi_fric[p], tauc, f.Neff[p],
                                xg[m.offx + i], yg[m.offy + j], f.A_vav ? f.A_vav[p] : 0.0);
     beta = fmin(beta, kp.beta_max);
     if (!kp.glsub && f.m_ocean[p] == 1) beta = 0.0;
@@ -733,15 +747,16 @@ __global__ void __launch_bounds__(IMAU_WG) k_visc_column(const Member *__restric
     const bool bc_skip = (kp.margin == 0 && mi == 0);
     const T expo = (T)((1.0 - kp.n_flow) / (2.0 * kp.n_flow));
     const T HiT = (T)Hi, eps_h = (T)f.eps_h[p], vmin = (T)kp.visc_min;
-    T vk[IMAU_NZMAX];
     double F2 = 0.0, be = 0.0, vint_out = 0.0;
     T tb = (T)0.0;
+#pragma unroll 1
     for (int it = 0; it < iters; ++it) {
       T vint = (T)0.0, F2int = (T)0.0, eta_prev = (T)0.0, g_prev = (T)0.0;
+#pragma unroll 1
       for (int k = 0; k < nz; ++k) {
         const T z = (T)kp.zeta[k];
-        if (bc_skip) { vk[k] = (T)0.0; continue; }
-        const T du = (it == 0) ? (T)0.0 : tb * z / vk[k];
+        if (bc_skip) { vk(k) = (T)0.0; continue; }
+        const T du = (it == 0) ? (T)0.0 : tb * z / vk(k);
         const T eps_sq = eps_h + (T)0.25 * du * du;
         const T B = B3[p3(m, k, j, i)];
         T eta = THIRD ? B / cbrt(eps_sq) : B * pow(eps_sq, expo);
@@ -754,14 +769,14 @@ __global__ void __launch_bounds__(IMAU_WG) k_visc_column(const Member *__restric
         }
         eta_prev = eta;
         g_prev = gk;
-        vk[k] = eta;
+        vk(k) = eta;
       }
       vint_out = (double)vint;
       F2 = (mi == 1) ? fmax(kp.F_int_min, (double)F2int) : ((kp.margin == 0) ? 0.0 : kp.F_int_min);
-      be = (kp.slid == 0 || frozen) ? 1.0 / F2 : beta / (1.0 + beta * F2);
+      be = (slid == 0 || frozen) ? 1.0 / F2 : beta / (1.0 + beta * F2);
       tb = (T)(be * speed);
     }
-    for (int k = 0; k < nz; ++k) visc[p3(m, k, j, i)] = vk[k];
+    for (int k = 0; k < nz; ++k) visc[p3(m, k, j, i)] = vk(k);
     if (!bc_skip) {
       f.visc_int[p] = vint_out;
       f.N_a[p] = vint_out * ((kp.margin == 0) ? Hi : fmax(0.1, Hi));
@@ -847,13 +862,15 @@ struct LAcc {
   __device__ __forceinline__ double operator()(int jj, int ii) const { return g_sx[COMP][ii - i0 + 1][jj - j0 + 1]; }
 };
 
-template <class AU, class AV>
-__device__ __forceinline__ void u_row(const Member &m, const KP &kp, const CoefP &c, int j, int i, uint8_t rt,
-                                      const AU &U, const AV &V, double &S, double &D, double &b) {
-  u_sd<double>(m, kp, c, j, i, rt, U, V, S, D);
+// CROSS: 1 / 0 fixes the cross-term choice at compile time, -1 reads kp.cross
+template <int CROSS, class AU, class AV>
+__device__ __forceinline__ void u_row_x(const Member &m, const KP &kp, const CoefP &c, int j, int i, uint8_t rt,
+                                        const AU &U, const AV &V, double &S, double &D, double &b) {
+  u_sd_x<CROSS, double>(m, kp, c, j, i, rt, U, V, S, D);
   if (rt == RT_FREE) {
     const double dx2 = m.dx * m.dx;
-    b = kp.cross ? -c.taudx[p2(m, j, i)] * dx2 : -c.taudx[p2(m, j, i)] * dx2 / c.Ncx[p2(m, j, i)];
+    const bool cross = CROSS >= 0 ? CROSS != 0 : kp.cross != 0;
+    b = cross ? -c.taudx[p2(m, j, i)] * dx2 : -c.taudx[p2(m, j, i)] * dx2 / c.Ncx[p2(m, j, i)];
   } else if (rt == RT_MARGIN_I || rt == RT_MARGIN_O) {
     b = c.frhs[p2(m, j, (rt == RT_MARGIN_I) ? i : i + 1)];
   } else {
@@ -861,18 +878,30 @@ __device__ __forceinline__ void u_row(const Member &m, const KP &kp, const CoefP
   }
 }
 
-template <class AU, class AV>
-__device__ __forceinline__ void v_row(const Member &m, const KP &kp, const CoefP &c, int j, int i, uint8_t rt,
-                                      const AU &U, const AV &V, double &S, double &D, double &b) {
-  v_sd<double>(m, kp, c, j, i, rt, U, V, S, D);
+template <int CROSS, class AU, class AV>
+__device__ __forceinline__ void v_row_x(const Member &m, const KP &kp, const CoefP &c, int j, int i, uint8_t rt,
+                                        const AU &U, const AV &V, double &S, double &D, double &b) {
+  v_sd_x<CROSS, double>(m, kp, c, j, i, rt, U, V, S, D);
   if (rt == RT_FREE) {
     const double dx2 = m.dx * m.dx;
-    b = kp.cross ? -c.taudy[p2(m, j, i)] * dx2 : -c.taudy[p2(m, j, i)] * dx2 / c.Ncy[p2(m, j, i)];
+    const bool cross = CROSS >= 0 ? CROSS != 0 : kp.cross != 0;
+    b = cross ? -c.taudy[p2(m, j, i)] * dx2 : -c.taudy[p2(m, j, i)] * dx2 / c.Ncy[p2(m, j, i)];
   } else if (rt == RT_MARGIN_I || rt == RT_MARGIN_O) {
     b = c.frhs[p2(m, (rt == RT_MARGIN_I) ? j : j + 1, i)];
   } else {
     b = 0.0;
   }
+}
+
+template <class AU, class AV>
+__device__ __forceinline__ void u_row(const Member &m, const KP &kp, const CoefP &c, int j, int i, uint8_t rt,
+                                      const AU &U, const AV &V, double &S, double &D, double &b) {
+  u_row_x<-1>(m, kp, c, j, i, rt, U, V, S, D, b);
+}
+template <class AU, class AV>
+__device__ __forceinline__ void v_row(const Member &m, const KP &kp, const CoefP &c, int j, int i, uint8_t rt,
+                                      const AU &U, const AV &V, double &S, double &D, double &b) {
+  v_row_x<-1>(m, kp, c, j, i, rt, U, V, S, D, b);
 }
 
 
@@ -1049,6 +1078,21 @@ __global__ void __launch_bounds__(IMAU_WG) k_copy3_members(const Member *__restr
   }
 }
 
+// dst = src (u and v slabs) on the members with flag[g] != 0
+__global__ void __launch_bounds__(IMAU_WG) k_copy2_members(const Member *__restrict__ mem,
+                                                           const Tile *__restrict__ tiles, CVec2 src, Vec2 dst,
+                                                           const int *__restrict__ flag) {
+  TILE_SETUP
+  if (j >= m.ny || !flag[t.g]) return;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= m.nx) break;
+    const long long p = p2(m, j, i);
+    dst.u[p] = src.u[p];
+    dst.v[p] = src.v[p];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // warm-start network I/O: image g of the network is member g's a-grid with
 // rows i (x) and columns j (y), pixel off[g] + i ny + j
@@ -1189,35 +1233,59 @@ __global__ void __launch_bounds__(IMAU_WG) k_resid(const Member *__restrict__ me
   block_sum_store<2>(acc, partial + 2 * (long long)tb);
 }
 
-// The FP64 nonlinear residual at the current state (as k_resid) fused with the
-// start of the linear solve: x0 = u, v with the zero rows set to 0 (k_init_x)
-// and r = b - A x0.  Partial sums per tile: |A u - b|^2, |b|^2, |r|^2.
-struct X0Acc {  // the start vector seen through the state: zero rows read 0
-  const double *x;
-  const uint8_t *rt;
-  Member m;
-  __device__ __forceinline__ double operator()(int jj, int ii) const {
-    const long long p = p2(m, jj, ii);
-    const uint8_t t = rt[p];
-    return (t == RT_ZERO || t == RT_BND_ZERO) ? 0.0 : x[p];
-  }
-};
-
-// the state seen through the row, noting whether it read a zero row's value
-// (where the start vector x0 differs from the state)
-struct ZAcc {
+// The state seen through a row.  Pass 0 reads the state and notes whether the
+// row read a zero row's unknown (where the start vector x0 differs from the
+// state); pass 1 reads x0, which is the state with the zero rows set to 0.
+struct RAcc {
   const double *x;
   const uint8_t *rt;
   Member m;
   bool *hit;
+  bool x0;
   __device__ __forceinline__ double operator()(int jj, int ii) const {
     const long long p = p2(m, jj, ii);
     const uint8_t t = rt[p];
-    if (t == RT_ZERO || t == RT_BND_ZERO) *hit = true;
-    return x[p];
+    const bool z = (t == RT_ZERO || t == RT_BND_ZERO);
+    if (z) *hit = true;
+    return (z && x0) ? 0.0 : x[p];
   }
 };
 
+// one row of k_resid_init: adds |row(u) - b|^2, b^2 and |b - row(x0)|^2 to acc,
+// writes x0 and r = b - row(x0); the row is evaluated a second time only when
+// it reads a zero row
+template <int CROSS, bool ISU>
+__device__ __forceinline__ void resid_init_row(const Member &m, const KP &kp, const CoefP &c, int j, int i,
+                                               long long p, const double *xs, RAcc &U, RAcc &V, double *x0,
+                                               double *r, double acc[3]) {
+  const uint8_t rt = ISU ? c.rt_u[p] : c.rt_v[p];
+  const double x = (rt == RT_ZERO || rt == RT_BND_ZERO) ? 0.0 : xs[p];
+  bool hit = false;
+  U.hit = V.hit = &hit;
+  double rl = 0.0;
+#pragma unroll 1
+  for (int pass = 0; pass < 2; ++pass) {
+    U.x0 = V.x0 = pass == 1;
+    double S, D, b;
+    if (ISU) u_row_x<CROSS>(m, kp, c, j, i, rt, U, V, S, D, b);
+    else v_row_x<CROSS>(m, kp, c, j, i, rt, U, V, S, D, b);
+    if (pass == 0) {
+      const double rn = S + D * xs[p] - b;
+      acc[0] += rn * rn;
+      acc[1] += b * b;
+    }
+    rl = b - (S + D * x);
+    if (!hit) break;
+  }
+  x0[p] = x;
+  r[p] = rl;
+  acc[2] += rl * rl;
+}
+
+// The FP64 nonlinear residual at the current state (as k_resid) fused with the
+// start of the linear solve: x0 = u, v with the zero rows set to 0 (k_init_x)
+// and r = b - A x0.  Partial sums per tile: |A u - b|^2, |b|^2, |r|^2.
+template <int CROSS>
 __global__ void __launch_bounds__(IMAU_WG) k_resid_init(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
                                                         const KP kp, const CoefP c, const double *__restrict__ xu,
                                                         const double *__restrict__ xv, Vec2 x0, Vec2 r,
@@ -1226,43 +1294,14 @@ __global__ void __launch_bounds__(IMAU_WG) k_resid_init(const Member *__restrict
   double acc[3] = {0.0, 0.0, 0.0};
   const int nx = m.nx, ny = m.ny;
   bool hit = false;
-  const ZAcc U{xu, c.rt_u, m, &hit}, V{xv, c.rt_v, m, &hit};
-  const X0Acc U0{xu, c.rt_u, m}, V0{xv, c.rt_v, m};
+  RAcc U{xu, c.rt_u, m, &hit, false}, V{xv, c.rt_v, m, &hit, false};
   if (j < ny && active[t.g]) {
     FOR_TILE_ROWS {
       const int i = t.i0 + ii;
       if (i >= nx) break;
       const long long p = p2(m, j, i);
-      double S, D, b;
-      if (i < nx - 1) {
-        const uint8_t rt = c.rt_u[p];
-        hit = false;
-        u_row(m, kp, c, j, i, rt, U, V, S, D, b);
-        const double rn = S + D * xu[p] - b;
-        acc[0] += rn * rn;
-        acc[1] += b * b;
-        const double x = (rt == RT_ZERO || rt == RT_BND_ZERO) ? 0.0 : xu[p];
-        // the row at x0 equals the row at the state unless it reads a zero row
-        if (hit) u_row(m, kp, c, j, i, rt, U0, V0, S, D, b);
-        const double rl = b - (S + D * x);
-        x0.u[p] = x;
-        r.u[p] = rl;
-        acc[2] += rl * rl;
-      }
-      if (j < ny - 1) {
-        const uint8_t rt = c.rt_v[p];
-        hit = false;
-        v_row(m, kp, c, j, i, rt, U, V, S, D, b);
-        const double rn = S + D * xv[p] - b;
-        acc[0] += rn * rn;
-        acc[1] += b * b;
-        const double x = (rt == RT_ZERO || rt == RT_BND_ZERO) ? 0.0 : xv[p];
-        if (hit) v_row(m, kp, c, j, i, rt, U0, V0, S, D, b);
-        const double rl = b - (S + D * x);
-        x0.v[p] = x;
-        r.v[p] = rl;
-        acc[2] += rl * rl;
-      }
+      if (i < nx - 1) resid_init_row<CROSS, true>(m, kp, c, j, i, p, xu, U, V, x0.u, r.u, acc);
+      if (j < ny - 1) resid_init_row<CROSS, false>(m, kp, c, j, i, p, xv, U, V, x0.v, r.v, acc);
     }
   }
   block_sum_store<3>(acc, partial + 3 * (long long)tb);
@@ -1689,17 +1728,29 @@ void net_warm(const Launch &L, const KP &kp, const Fields &f, const long long *p
               const int *active) {
   hipLaunchKernelGGL(k_net_warm, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, poff, o, active);
 }
+// the k_visc_column specialisation for kp: n = 3, Weertman sliding and
+// nz = IMAU_MNZ each get their own
+template <class T>
+static auto column_kernel(const KP &kp) {
+  const bool third = kp.n_flow == 3.0, weert = kp.slid == 4, fixed = kp.nz == IMAU_MNZ;
+  if (third) {
+    if (weert) return fixed ? k_visc_column<true, T, 4, IMAU_MNZ> : k_visc_column<true, T, 4, 0>;
+    return fixed ? k_visc_column<true, T, -1, IMAU_MNZ> : k_visc_column<true, T, -1, 0>;
+  }
+  if (weert) return fixed ? k_visc_column<false, T, 4, IMAU_MNZ> : k_visc_column<false, T, 4, 0>;
+  return fixed ? k_visc_column<false, T, -1, IMAU_MNZ> : k_visc_column<false, T, -1, 0>;
+}
 void visc_column(const Launch &L, const Launch &LV, const KP &kp, const Fields &f, double *visc, const double *xg,
                  const double *yg, int iters, const int *active) {
   hipLaunchKernelGGL(k_strain, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, active);
-  auto kv = kp.n_flow == 3.0 ? k_visc_column<true, double> : k_visc_column<false, double>;
-  hipLaunchKernelGGL(kv, dim3(LV.ntiles), wg(), 0, L.s, L.mem, LV.tiles, kp, f, visc, f.B3, xg, yg, iters, active);
+  hipLaunchKernelGGL(column_kernel<double>(kp), dim3(LV.ntiles), wg(), 0, L.s, L.mem, LV.tiles, kp, f, visc, f.B3, xg,
+                     yg, iters, active);
 }
 void visc_column32(const Launch &L, const Launch &LV, const KP &kp, const Fields &f, float *visc, const float *B3,
                    const double *xg, const double *yg, int iters, const int *active) {
   hipLaunchKernelGGL(k_strain, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, active);
-  auto kv = kp.n_flow == 3.0 ? k_visc_column<true, float> : k_visc_column<false, float>;
-  hipLaunchKernelGGL(kv, dim3(LV.ntiles), wg(), 0, L.s, L.mem, LV.tiles, kp, f, visc, B3, xg, yg, iters, active);
+  hipLaunchKernelGGL(column_kernel<float>(kp), dim3(LV.ntiles), wg(), 0, L.s, L.mem, LV.tiles, kp, f, visc, B3, xg, yg,
+                     iters, active);
 }
 void taub(const Launch &L, const Fields &f, const int *active) {
   hipLaunchKernelGGL(k_taub, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, f, active);
@@ -1720,7 +1771,8 @@ void member_sum3(hipStream_t s, int ngrids, const int *tstart, const double *par
 }
 void resid_init(const Launch &L, const KP &kp, const Coef &c, const double *xu, const double *xv, double *x0u,
                 double *x0v, double *ru, double *rv, const int *active, double *partial) {
-  hipLaunchKernelGGL(k_resid_init, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, to_p(c), xu, xv, Vec2{x0u, x0v},
+  auto k = kp.cross ? k_resid_init<1> : k_resid_init<0>;
+  hipLaunchKernelGGL(k, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, to_p(c), xu, xv, Vec2{x0u, x0v},
                      Vec2{ru, rv}, active, partial);
 }
 void member_sum2(hipStream_t s, int ngrids, const int *tstart, const double *partial, double *out) {
@@ -1732,6 +1784,9 @@ void fill_u64(hipStream_t s, unsigned long long *p, int n, unsigned long long v)
 void reset_members(const Launch &L, const KP &kp, const Fields &f, const ResetSrc &src, double *visc,
                    float *visc32, const int *flag) {
   hipLaunchKernelGGL(k_reset_members, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, f, src, visc, visc32, kp.nz, flag);
+}
+void copy2_members(const Launch &L, const double *su, const double *sv, double *du, double *dv, const int *flag) {
+  hipLaunchKernelGGL(k_copy2_members, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, CVec2{su, sv}, Vec2{du, dv}, flag);
 }
 void copy3_members(const Launch &L, int nz, const double *src, double *dst, const int *flag) {
   hipLaunchKernelGGL(k_copy3_members<double>, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, src, dst, nz, flag);

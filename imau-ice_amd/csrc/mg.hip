@@ -91,53 +91,57 @@ __device__ __forceinline__ void slot_pos(int rowc, int k, int J, int I, int &cc,
   }
 }
 
-// nonzeros of the fine u row (interior or calving front) at (j,i): f(comp, jj, ii, value)
-template <class F>
+// nonzeros of the fine u row (interior or calving front) at (j,i):
+// f(comp, dj, di, value) for the unknown (comp, j + dj, i + di).  The offsets
+// of an interior row are constants, so a caller that unrolls over the rows of
+// an aggregate resolves every coarse slot at compile time.
+// CROSS: 1 / 0 fixes the cross-term choice at compile time, -1 reads kp.cross
+template <int CROSS, class F>
 __device__ __forceinline__ void u_couplings(const Member &m, const KP &kp, const CoefT<double> &c, int j, int i,
                                             uint8_t rt, F &&f) {
   const double dx2 = m.dx * m.dx;
   if (rt == RT_FREE) {
-    if (kp.cross) {
+    if (CROSS >= 0 ? CROSS != 0 : kp.cross != 0) {
       const double Na0 = c.Na[p2(m, j, i)], Na1 = c.Na[p2(m, j, i + 1)];
       const double Nb0 = c.Nb[p2(m, j, i)], Nbm = c.Nb[p2(m, j - 1, i)];
-      f(0, j, i + 1, 4.0 * Na1); f(0, j, i - 1, 4.0 * Na0); f(0, j + 1, i, Nb0); f(0, j - 1, i, Nbm);
-      f(0, j, i, -4.0 * (Na1 + Na0) + -1.0 * (Nb0 + Nbm) - c.bcx[p2(m, j, i)] * dx2);
-      f(1, j, i, -2.0 * Na0 - Nb0); f(1, j, i + 1, 2.0 * Na1 + Nb0);
-      f(1, j - 1, i, 2.0 * Na0 + Nbm); f(1, j - 1, i + 1, -2.0 * Na1 - Nbm);
+      f(0, 0, 1, 4.0 * Na1); f(0, 0, -1, 4.0 * Na0); f(0, 1, 0, Nb0); f(0, -1, 0, Nbm);
+      f(0, 0, 0, -4.0 * (Na1 + Na0) + -1.0 * (Nb0 + Nbm) - c.bcx[p2(m, j, i)] * dx2);
+      f(1, 0, 0, -2.0 * Na0 - Nb0); f(1, 0, 1, 2.0 * Na1 + Nb0);
+      f(1, -1, 0, 2.0 * Na0 + Nbm); f(1, -1, 1, -2.0 * Na1 - Nbm);
     } else {
       const double Ncx = c.Ncx[p2(m, j, i)];
-      f(0, j, i + 1, 4.0); f(0, j, i - 1, 4.0); f(0, j + 1, i, 1.0); f(0, j - 1, i, 1.0);
-      f(0, j, i, -10.0 - dx2 * c.bcx[p2(m, j, i)] / Ncx);
-      f(1, j, i, -3.0); f(1, j, i + 1, 3.0); f(1, j - 1, i, 3.0); f(1, j - 1, i + 1, -3.0);
+      f(0, 0, 1, 4.0); f(0, 0, -1, 4.0); f(0, 1, 0, 1.0); f(0, -1, 0, 1.0);
+      f(0, 0, 0, -10.0 - dx2 * c.bcx[p2(m, j, i)] / Ncx);
+      f(1, 0, 0, -3.0); f(1, 0, 1, 3.0); f(1, -1, 0, 3.0); f(1, -1, 1, -3.0);
     }
   } else {  // RT_MARGIN_I / RT_MARGIN_O
-    const int i1 = (rt == RT_MARGIN_I) ? i : i + 1;
-    const double Na = c.Na[p2(m, j, i1)] / m.dx;
-    f(0, j, i1 - 1, -4.0 * Na); f(0, j, i1, 4.0 * Na); f(1, j - 1, i1, -2.0 * Na); f(1, j, i1, 2.0 * Na);
+    const int d1 = (rt == RT_MARGIN_I) ? 0 : 1;  // column i1 = i + d1
+    const double Na = c.Na[p2(m, j, i + d1)] / m.dx;
+    f(0, 0, d1 - 1, -4.0 * Na); f(0, 0, d1, 4.0 * Na); f(1, -1, d1, -2.0 * Na); f(1, 0, d1, 2.0 * Na);
   }
 }
-template <class F>
+template <int CROSS, class F>
 __device__ __forceinline__ void v_couplings(const Member &m, const KP &kp, const CoefT<double> &c, int j, int i,
                                             uint8_t rt, F &&f) {
   const double dx2 = m.dx * m.dx;
   if (rt == RT_FREE) {
-    if (kp.cross) {
+    if (CROSS >= 0 ? CROSS != 0 : kp.cross != 0) {
       const double Na0 = c.Na[p2(m, j, i)], Na1 = c.Na[p2(m, j + 1, i)];
       const double Nb0 = c.Nb[p2(m, j, i)], Nbm = c.Nb[p2(m, j, i - 1)];
-      f(1, j + 1, i, 4.0 * Na1); f(1, j - 1, i, 4.0 * Na0); f(1, j, i + 1, Nb0); f(1, j, i - 1, Nbm);
-      f(1, j, i, -4.0 * (Na1 + Na0) + -1.0 * (Nb0 + Nbm) - c.bcy[p2(m, j, i)] * dx2);
-      f(0, j + 1, i - 1, -2.0 * Na1 - Nbm); f(0, j + 1, i, 2.0 * Na1 + Nb0);
-      f(0, j, i - 1, 2.0 * Na0 + Nbm); f(0, j, i, -2.0 * Na0 - Nb0);
+      f(1, 1, 0, 4.0 * Na1); f(1, -1, 0, 4.0 * Na0); f(1, 0, 1, Nb0); f(1, 0, -1, Nbm);
+      f(1, 0, 0, -4.0 * (Na1 + Na0) + -1.0 * (Nb0 + Nbm) - c.bcy[p2(m, j, i)] * dx2);
+      f(0, 1, -1, -2.0 * Na1 - Nbm); f(0, 1, 0, 2.0 * Na1 + Nb0);
+      f(0, 0, -1, 2.0 * Na0 + Nbm); f(0, 0, 0, -2.0 * Na0 - Nb0);
     } else {
       const double Ncy = c.Ncy[p2(m, j, i)];
-      f(1, j + 1, i, 4.0); f(1, j - 1, i, 4.0); f(1, j, i + 1, 1.0); f(1, j, i - 1, 1.0);
-      f(1, j, i, -10.0 - dx2 * c.bcy[p2(m, j, i)] / Ncy);
-      f(0, j + 1, i - 1, -3.0); f(0, j + 1, i, 3.0); f(0, j, i - 1, 3.0); f(0, j, i, -3.0);
+      f(1, 1, 0, 4.0); f(1, -1, 0, 4.0); f(1, 0, 1, 1.0); f(1, 0, -1, 1.0);
+      f(1, 0, 0, -10.0 - dx2 * c.bcy[p2(m, j, i)] / Ncy);
+      f(0, 1, -1, -3.0); f(0, 1, 0, 3.0); f(0, 0, -1, 3.0); f(0, 0, 0, -3.0);
     }
   } else {
-    const int j1 = (rt == RT_MARGIN_I) ? j : j + 1;
-    const double Na = c.Na[p2(m, j1, i)] / m.dx;
-    f(1, j1 - 1, i, -4.0 * Na); f(1, j1, i, 4.0 * Na); f(0, j1, i - 1, -2.0 * Na); f(0, j1, i, 2.0 * Na);
+    const int d1 = (rt == RT_MARGIN_I) ? 0 : 1;  // row j1 = j + d1
+    const double Na = c.Na[p2(m, j + d1, i)] / m.dx;
+    f(1, d1 - 1, 0, -4.0 * Na); f(1, d1, 0, 4.0 * Na); f(0, d1, -1, -2.0 * Na); f(0, d1, 0, 2.0 * Na);
   }
 }
 
@@ -166,7 +170,7 @@ __device__ __forceinline__ bool fine_valid(const Member &m, int comp, int j, int
 // Galerkin products
 // ---------------------------------------------------------------------------
 // fine (level 0, FP64 coefficients) -> level 1
-template <class T>
+template <class T, int CROSS>
 __global__ void __launch_bounds__(IMAU_WG) k_gal0(const Member *__restrict__ mem, const KP kp, const CoefT<double> c,
                                                   CLevelT<T> L, const int *__restrict__ active) {
   CTILE_SETUP
@@ -185,14 +189,14 @@ __global__ void __launch_bounds__(IMAU_WG) k_gal0(const Member *__restrict__ mem
           if (!fine_valid(m, comp, j, i)) continue;
           const uint8_t rt = comp == 0 ? c.rt_u[p2(m, j, i)] : c.rt_v[p2(m, j, i)];
           if (!in_row(rt)) continue;
-          auto acc = [&](int cc, int jj, int i2, double val) {
+          auto acc = [&](int cc, int oj, int oi, double val) {
             int Jc, Ic;
-            if (!fine_agg(m, c.rt_u, c.rt_v, cc, jj, i2, Jc, Ic)) return;
+            if (!fine_agg(m, c.rt_u, c.rt_v, cc, j + oj, i + oi, Jc, Ic)) return;
             const int s = agg_slot(comp, cc, Jc - J, Ic - I);
             if (s >= 0) cf[s] += val;
           };
-          if (comp == 0) u_couplings(m, kp, c, j, i, rt, acc);
-          else v_couplings(m, kp, c, j, i, rt, acc);
+          if (comp == 0) u_couplings<CROSS>(m, kp, c, j, i, rt, acc);
+          else v_couplings<CROSS>(m, kp, c, j, i, rt, acc);
         }
       T *dst = comp == 0 ? L.cu : L.cv;
 #pragma unroll
@@ -779,7 +783,8 @@ void fill(hipStream_t s, T *p, long long n) {
 template <class T>
 void setup_t(const imau::Launch &F, const KP &kp, const imau::Coef &c, const imau::MGPlan &P, const int *act) {
   const CLevelT<T> L1 = view<T>(P, 0);
-  hipLaunchKernelGGL(k_gal0<T>, dim3(L1.ntiles), wg(), 0, F.s, F.mem, kp, coef64(c), L1, act);
+  auto kg = kp.cross ? k_gal0<T, 1> : k_gal0<T, 0>;
+  hipLaunchKernelGGL(kg, dim3(L1.ntiles), wg(), 0, F.s, F.mem, kp, coef64(c), L1, act);
   for (size_t l = 1; l < P.levels.size(); ++l) {
     const CLevelT<T> Fl = view<T>(P, (int)l - 1), L = view<T>(P, (int)l);
     hipLaunchKernelGGL(k_galc<T>, dim3(L.ntiles), wg(), 0, F.s, Fl, L, act);

@@ -1,0 +1,49 @@
+// producers.h -- launchers of the input-producer kernels (producers.hip):
+// update_general_ice_model_data (src/general_ice_model_data_module.f90:30-346,
+// ocean flood fill :868-986) and calc_ice_rheology
+// (src/thermodynamics_module.f90:641-753).
+#pragma once
+#include "../../include/imau_vel.h"
+#include "dev_common.h"
+
+namespace imau {
+
+// strides of a 3D field: element (k, j, i) at k*sk + i*si + j*sj
+struct Str3 {
+  long long sk, si, sj;
+};
+
+// 2D fields of one member in a-grid-shaped slabs (leading dimension ny, offset m.off)
+struct ProdIn {
+  const double *Hi, *Hb, *SL, *dHi_dt, *dHb_dt;  // the rates may be null
+};
+struct ProdMasks {
+  int32_t *land, *ocean, *ice, *sheet, *shelf, *coast, *margin, *gl, *cf, *total;
+};
+struct Prod2 {
+  double *Hs, *TAF, *dHs_dt, *Hi_cx, *Hi_cy, *Hi_b;
+  double *dHi_dx, *dHi_dy, *dHs_dx, *dHs_dy, *dHs_dx_cx, *dHs_dy_cx, *dHs_dx_cy, *dHs_dy_cy;
+};
+
+// Hs, TAF, dHs/dt (reading the land mask as it is on entry), then the
+// land/ocean masks from flotation (floodfill = 0) or the edge cells of the
+// flood fill (floodfill = 1)
+void prod_point(hipStream_t s, const Member &m, const ProdIn &in, const Prod2 &o, const ProdMasks &mk, int floodfill);
+// one round of the flood fill: grows the ocean through the 8 neighbours of
+// floating cells, tile by tile to a local fixed point; *changed (device) is
+// set to 1 when a cell changed
+void prod_flood_round(hipStream_t s, const Member &m, const ProdIn &in, const ProdMasks &mk, int *changed);
+// land = 1 - ocean after the flood fill
+void prod_land(hipStream_t s, const Member &m, const ProdMasks &mk);
+// ice / sheet / shelf, the transition masks, the total mask, H on cx / cy / b
+// and the slopes of H and Hs
+void prod_masks_maps(hipStream_t s, const Member &m, const ProdIn &in, const Prod2 &o, const ProdMasks &mk);
+// A on cx / cy: the vertical average (2D, may be null) and the 3D field
+// (null: skipped); nz levels with the given strides
+void prod_map_A(hipStream_t s, const Member &m, int nz, const double *A3, Str3 sa, double *A3cx, Str3 sx, double *A3cy,
+                Str3 sy, const double *Av, double *Avcx, double *Avcy);
+// calc_ice_rheology's flow factor, enhancement factors and vertical average
+void prod_rheology(hipStream_t s, const Member &m, const imau_producer_cfg &c, double time, const double *Ti, Str3 st,
+                   const ProdMasks &mk, double *A3, Str3 sa, double *Av);
+
+}  // namespace imau

@@ -100,6 +100,14 @@ class ImauThicknessIO(ctypes.Structure):
     _fields_ = _struct_fields(_TXT, "imau_thickness_io", DEFINES)
 
 
+class ImauProducerCfg(ctypes.Structure):
+    _fields_ = _struct_fields(_TXT, "imau_producer_cfg", DEFINES)
+
+
+class ImauProducerIO(ctypes.Structure):
+    _fields_ = _struct_fields(_TXT, "imau_producer_io", DEFINES)
+
+
 IO_POINTER_FIELDS = [n for n, t in ImauIceIO._fields_ if hasattr(t, "contents")]
 
 # export the enum constants at module level (IMAU_OK, IMAU_DYN_DIVA, ...)

@@ -167,3 +167,37 @@ def thickness_config(**overrides):
             raise AttributeError("imau_thickness_cfg has no field %r" % k)
         setattr(c, k, v)
     return c
+
+
+RHEOLOGY = {"uniform": abi.IMAU_RHEO_UNIFORM, "Huybrechts1992": abi.IMAU_RHEO_HUYBRECHTS1992,
+            "MISMIP_mod": abi.IMAU_RHEO_MISMIP_MOD}
+
+
+def producer_config(**overrides):
+    """Reference defaults of the input producers (src/configuration_module.f90:168-171,
+    273-274, 425-426, 702); keyword overrides accept enum ints or the reference strings."""
+    c = abi.ImauProducerCfg()
+    c.nz = len(DEFAULT_ZETA)
+    for k, z in enumerate(DEFAULT_ZETA):
+        c.zeta[k] = z
+    c.choice_ice_margin = MARGIN["infinite_slab"]
+    c.do_ocean_floodfill = 1
+    c.choice_ice_rheology = RHEOLOGY["Huybrechts1992"]
+    c.uniform_flow_factor = 1e-16
+    c.m_enh_sheet = 1.0
+    c.m_enh_shelf = 1.0
+    for k, v in overrides.items():
+        if k == "zeta":
+            c.nz = len(v)
+            for i, z in enumerate(v):
+                c.zeta[i] = z
+            continue
+        if isinstance(v, str):
+            table = {"choice_ice_margin": MARGIN, "choice_ice_rheology": RHEOLOGY}.get(k)
+            if table is None or v not in table:
+                raise ValueError("unknown choice %r for %s" % (v, k))
+            v = table[v]
+        if not hasattr(c, k):
+            raise AttributeError("imau_producer_cfg has no field %r" % k)
+        setattr(c, k, v)
+    return c

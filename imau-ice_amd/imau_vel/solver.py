@@ -27,6 +27,8 @@ EXPORTED = (
 )
 # the ice-thickness update that consumes the velocities
 THICKNESS_EXPORTED = ("imau_thickness_default_config", "imau_thickness_update", "imau_critical_timestep_adv")
+# the input producers the solve reads
+PRODUCER_EXPORTED = ("imau_producer_default_config", "imau_update_general_ice_model_data", "imau_calc_ice_rheology")
 
 
 class ImauError(RuntimeError):
@@ -78,6 +80,11 @@ def load_library(path: str = LIB_PATH):
     L.imau_critical_timestep_adv.argtypes = [vp, P(ImauGrid), P(ctypes.c_double), P(ctypes.c_double), ctypes.c_double,
                                              P(ctypes.c_double)]
     L.imau_vel_problem_residual_noalloc.argtypes = [vp, vp, P(ImauCfg), vp]
+    L.imau_producer_default_config.argtypes = [P(abi.ImauProducerCfg)]
+    L.imau_producer_default_config.restype = None
+    L.imau_update_general_ice_model_data.argtypes = [vp, P(ImauGrid), P(abi.ImauProducerCfg), P(abi.ImauProducerIO)]
+    L.imau_calc_ice_rheology.argtypes = [vp, P(ImauGrid), P(abi.ImauProducerCfg), P(abi.ImauProducerIO),
+                                         ctypes.c_double]
     _lib = L
     return L
 
@@ -139,6 +146,23 @@ class Context:
         self._check(self.L.imau_thickness_update(self.h, ctypes.byref(g), ctypes.byref(tcfg), ctypes.byref(io),
                                                  ctypes.c_double(dt)))
         return ts
+
+    def update_general_ice_model_data(self, grid, pcfg, ps, omit=()):
+        """update_general_ice_model_data (src/general_ice_model_data_module.f90:30-84)
+        on ProducerState ps; fields named in `omit` are passed as NULL."""
+        g = grid.struct()
+        io = ps.io(omit)
+        self._check(self.L.imau_update_general_ice_model_data(self.h, ctypes.byref(g), ctypes.byref(pcfg),
+                                                              ctypes.byref(io)))
+        return ps
+
+    def calc_ice_rheology(self, grid, pcfg, ps, time=0.0, omit=()):
+        """calc_ice_rheology (src/thermodynamics_module.f90:641-753) on ProducerState ps."""
+        g = grid.struct()
+        io = ps.io(omit)
+        self._check(self.L.imau_calc_ice_rheology(self.h, ctypes.byref(g), ctypes.byref(pcfg), ctypes.byref(io),
+                                                  ctypes.c_double(time)))
+        return ps
 
     def critical_timestep_adv(self, grid, u, v, dt_max):
         """calc_critical_timestep_adv (src/ice_dynamics_module.f90:726-762)."""

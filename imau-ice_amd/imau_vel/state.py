@@ -178,3 +178,56 @@ class ThicknessState:
             ct = ctypes.c_int32 if arr.dtype == np.int32 else ctypes.c_double
             setattr(io, name, arr.ctypes.data_as(ctypes.POINTER(ct)))
         return io
+
+
+# (grid, 3D, dtype) of every imau_producer_io field
+PRODUCER_FIELDS = {
+    "Hi_a": ("a", 0, _F64), "Hb_a": ("a", 0, _F64), "SL_a": ("a", 0, _F64),
+    "dHi_dt_a": ("a", 0, _F64), "dHb_dt_a": ("a", 0, _F64), "Ti_a": ("a", 1, _F64),
+    "Hs_a": ("a", 0, _F64), "TAF_a": ("a", 0, _F64), "dHs_dt_a": ("a", 0, _F64),
+    "mask_land_a": ("a", 0, _I32), "mask_ocean_a": ("a", 0, _I32), "mask_ice_a": ("a", 0, _I32),
+    "mask_sheet_a": ("a", 0, _I32), "mask_shelf_a": ("a", 0, _I32), "mask_coast_a": ("a", 0, _I32),
+    "mask_margin_a": ("a", 0, _I32), "mask_gl_a": ("a", 0, _I32), "mask_cf_a": ("a", 0, _I32),
+    "mask_a": ("a", 0, _I32),
+    "Hi_cx": ("cx", 0, _F64), "Hi_cy": ("cy", 0, _F64), "Hi_b": ("b", 0, _F64),
+    "A_flow_3D_a": ("a", 1, _F64), "A_flow_vav_a": ("a", 0, _F64),
+    "A_flow_3D_cx": ("cx", 1, _F64), "A_flow_3D_cy": ("cy", 1, _F64),
+    "A_flow_vav_cx": ("cx", 0, _F64), "A_flow_vav_cy": ("cy", 0, _F64),
+    "dHi_dx_a": ("a", 0, _F64), "dHi_dy_a": ("a", 0, _F64), "dHs_dx_a": ("a", 0, _F64), "dHs_dy_a": ("a", 0, _F64),
+    "dHs_dx_cx": ("cx", 0, _F64), "dHs_dy_cx": ("cx", 0, _F64), "dHs_dx_cy": ("cy", 0, _F64),
+    "dHs_dy_cy": ("cy", 0, _F64),
+}
+
+
+class ProducerState:
+    """The fields of update_general_ice_model_data / calc_ice_rheology
+    (imau_producer_io), Fortran order; 3D fields are (nz, ny, nx)."""
+
+    def __init__(self, grid: Grid, nz: int):
+        self.grid, self.nz = grid, nz
+        self.f = {}
+        for name, (kind, is3d, dt) in PRODUCER_FIELDS.items():
+            shape = grid_shape(kind, grid.nx, grid.ny)
+            self.f[name] = np.zeros(((nz,) + shape) if is3d else shape, dtype=dt, order="F")
+
+    def __getitem__(self, k):
+        return self.f[k]
+
+    def __setitem__(self, k, v):
+        self.f[k][...] = v
+
+    def copy(self) -> "ProducerState":
+        s = ProducerState(self.grid, self.nz)
+        for k, v in self.f.items():
+            s.f[k][...] = v
+        return s
+
+    def io(self, omit=()):
+        """imau_producer_io over these arrays; fields in `omit` are passed as NULL."""
+        io = abi.ImauProducerIO()
+        for name, arr in self.f.items():
+            if name in omit:
+                continue
+            ct = ctypes.c_int32 if arr.dtype == np.int32 else ctypes.c_double
+            setattr(io, name, arr.ctypes.data_as(ctypes.POINTER(ct)))
+        return io

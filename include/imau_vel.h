@@ -383,6 +383,65 @@ int  imau_vel_problem_thickness(imau_ctx *ctx, imau_problem *prob, int32_t g, co
 int  imau_critical_timestep_adv(imau_ctx *ctx, const imau_grid *grid, const double *u_vav_cx,
                                 const double *v_vav_cy, double dt_max, double *dt_crit_adv);
 
+/* ---- input producers: the fields the velocity solve reads, from the
+ *      geometry and the temperature ----
+ * choice_ice_rheology (src/configuration_module.f90:425) */
+enum { IMAU_RHEO_UNIFORM = 0, IMAU_RHEO_HUYBRECHTS1992 = 1, IMAU_RHEO_MISMIP_MOD = 2 };
+/* mask_a values (C%type_*, src/configuration_module.f90:3171-3179) */
+enum { IMAU_TYPE_LAND = 0, IMAU_TYPE_OCEAN = 1, IMAU_TYPE_LAKE = 2, IMAU_TYPE_SHEET = 3, IMAU_TYPE_SHELF = 4,
+       IMAU_TYPE_COAST = 5, IMAU_TYPE_MARGIN = 6, IMAU_TYPE_GROUNDINGLINE = 7, IMAU_TYPE_CALVINGFRONT = 8 };
+
+/* The C% fields the producers read (src/configuration_module.f90:168-171,
+ * 273-274, 425-426, 702). */
+typedef struct imau_producer_cfg {
+  int32_t nz;
+  double  zeta[IMAU_NZ_MAX];
+  int32_t choice_ice_margin;        /* the flow factor of ice-free cells under Huybrechts1992 */
+  int32_t do_ocean_floodfill;       /* 1: ocean = floating cells connected to the domain edge */
+  int32_t choice_ice_rheology;
+  double  uniform_flow_factor;      /* [Pa^-3 yr^-1] */
+  double  m_enh_sheet, m_enh_shelf;
+} imau_producer_cfg;
+
+/* The fields update_general_ice_model_data and calc_ice_rheology read and
+ * write (type_ice_model, src/data_types_module.f90:56-325), host arrays in
+ * the layouts above (2D (ny,nx) a-grid unless noted, 3D (nz,ny,nx)).  An
+ * output pointer may be NULL (not written). */
+typedef struct imau_producer_io {
+  const double *Hi_a, *Hb_a, *SL_a;
+  const double *dHi_dt_a, *dHb_dt_a;        /* NULL: dHs_dt_a is not computed; it reads
+                                               mask_land_a as it is on entry (the previous
+                                               call's), as the reference does */
+  const double *Ti_a;                       /* 3D; Huybrechts1992 only */
+  double  *Hs_a, *TAF_a, *dHs_dt_a;
+  int32_t *mask_land_a, *mask_ocean_a, *mask_ice_a, *mask_sheet_a, *mask_shelf_a;
+  int32_t *mask_coast_a, *mask_margin_a, *mask_gl_a, *mask_cf_a, *mask_a;
+  double  *Hi_cx, *Hi_cy, *Hi_b;            /* (ny,nx-1), (ny-1,nx), (ny-1,nx-1) */
+  double  *A_flow_3D_a, *A_flow_vav_a;      /* in for update_general_ice_model_data, out for calc_ice_rheology */
+  double  *A_flow_3D_cx, *A_flow_3D_cy;     /* (nz,ny,nx-1), (nz,ny-1,nx) */
+  double  *A_flow_vav_cx, *A_flow_vav_cy;
+  double  *dHi_dx_a, *dHi_dy_a, *dHs_dx_a, *dHs_dy_a;
+  double  *dHs_dx_cx, *dHs_dy_cx, *dHs_dx_cy, *dHs_dy_cy;
+} imau_producer_io;
+
+void imau_producer_default_config(imau_producer_cfg *cfg);
+
+/* Replaces update_general_ice_model_data (src/general_ice_model_data_module.f90:30-84):
+ * Hs, TAF and dHs/dt, the masks (determine_masks :87-346, with the ocean
+ * flood fill :868-986 when do_ocean_floodfill), H on cx/cy/b, the flow factor
+ * in io->A_flow_3D_a / A_flow_vav_a mapped to cx/cy, and the surface and
+ * thickness slopes.  The masks read by calc_ice_rheology are written first. */
+int  imau_update_general_ice_model_data(imau_ctx *ctx, const imau_grid *grid, const imau_producer_cfg *cfg,
+                                        imau_producer_io *io);
+
+/* Replaces calc_ice_rheology (src/thermodynamics_module.f90:641-753) at model
+ * time `time` [yr]: the 3D flow factor (uniform, Huybrechts (1992) Arrhenius
+ * law of Ti_a, or MISMIP_mod's step function of time) times the sheet/shelf
+ * enhancement factors (io->mask_ice_a, mask_sheet_a, mask_shelf_a are read),
+ * its vertical average, and both mapped to cx/cy. */
+int  imau_calc_ice_rheology(imau_ctx *ctx, const imau_grid *grid, const imau_producer_cfg *cfg,
+                            imau_producer_io *io, double time);
+
 /* ---- ensembles (no reference counterpart: the reference solves one region
  *      grid per call, src/IMAU_ICE_program.f90:247-250) ----
  * Solves ngrids independent DIVA problems of possibly different sizes in one

@@ -324,6 +324,198 @@ void orc_update_general_ice_model_data(const imau_grid *gr, const imau_cfg *c,
 }
 
 /* ------------------------------------------------------------------------ */
+/* input producers with the full masks: update_general_ice_model_data        */
+/* (src/general_ice_model_data_module.f90:30-84), determine_masks (:87-346), */
+/* ocean_floodfill (:868-986); calc_ice_rheology                             */
+/* (src/thermodynamics_module.f90:641-753)                                   */
+/* ------------------------------------------------------------------------ */
+#define SETI(ptr, p, v) do { if (ptr) (ptr)[p] = (v); } while (0)
+
+/* ocean_floodfill, :868-986: a depth-first fill from every edge cell through
+ * the 8 neighbours of floating cells, with the reference's stack order */
+static void ocean_floodfill(const G *g, const double *Hi, const double *Hb, const double *SL, int32_t *land,
+                            int32_t *ocean)
+{
+  const int nx = g->nx, ny = g->ny;
+  int32_t *map = ialloc((size_t)nx * ny);
+  int32_t *stack = ialloc(2 * (size_t)nx * ny + 4 * (size_t)(nx + ny));
+  long n = 0;
+  for (size_t p = 0; p < (size_t)nx * ny; ++p) { land[p] = 1; ocean[p] = 0; }
+  for (int i = 0; i < nx; ++i) {        /* :905-914 */
+    int j = 0;
+    stack[2 * n] = i; stack[2 * n + 1] = j; ++n; map[A2(j, i)] = 1;
+    j = ny - 1;
+    stack[2 * n] = i; stack[2 * n + 1] = j; ++n; map[A2(j, i)] = 1;
+  }
+  for (int j = 0; j < ny; ++j) {        /* :915-924 */
+    int i = 0;
+    stack[2 * n] = i; stack[2 * n + 1] = j; ++n; map[A2(j, i)] = 1;
+    i = nx - 1;
+    stack[2 * n] = i; stack[2 * n + 1] = j; ++n; map[A2(j, i)] = 1;
+  }
+  while (n > 0) {                       /* :927-968 */
+    --n;
+    const int i = stack[2 * n], j = stack[2 * n + 1];
+    if (!is_floating(Hi[A2(j, i)], Hb[A2(j, i)], SL[A2(j, i)])) continue;
+    map[A2(j, i)] = 2;
+    ocean[A2(j, i)] = 1;
+    land[A2(j, i)] = 0;
+    for (int ii = i - 1; ii <= i + 1; ++ii)
+      for (int jj = j - 1; jj <= j + 1; ++jj) {
+        if (ii < 0 || ii >= nx || jj < 0 || jj >= ny || (ii == i && jj == j)) continue;
+        if (map[A2(jj, ii)] == 0) {
+          map[A2(jj, ii)] = 1;
+          stack[2 * n] = ii; stack[2 * n + 1] = jj; ++n;
+        }
+      }
+  }
+  free(map);
+  free(stack);
+}
+
+int orc_producers_update(const imau_grid *gr, const imau_producer_cfg *c, imau_producer_io *io)
+{
+  G gg = {gr->nx, gr->ny, c->nz, gr->dx, gr->x, gr->y, NULL};
+  const G *g = &gg;
+  const int nx = g->nx, ny = g->ny;
+  const size_t na = (size_t)nx * ny;
+  const double *Hi = io->Hi_a, *Hb = io->Hb_a, *SL = io->SL_a;
+  double *Hs = io->Hs_a ? io->Hs_a : dalloc(na);
+  int32_t *land = ialloc(na), *ocean = ialloc(na), *ice = ialloc(na), *sheet = ialloc(na), *shelf = ialloc(na);
+  /* :45-56 */
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      const size_t p = A2(j, i);
+      Hs[p] = surface_elevation(Hi[p], Hb[p], SL[p]);
+      if (io->TAF_a) io->TAF_a[p] = thickness_above_floatation(Hi[p], Hb[p], SL[p]);
+    }
+  /* :47-52, before the masks are updated: the land mask as it is on entry
+   * (the previous call's) */
+  if (io->dHi_dt_a && io->dHb_dt_a && io->dHs_dt_a && io->mask_land_a)
+    for (size_t p = 0; p < na; ++p)
+      io->dHs_dt_a[p] = io->mask_land_a[p] == 1 ? io->dHb_dt_a[p] + io->dHi_dt_a[p]
+                                                : io->dHi_dt_a[p] * (1.0 - 910.0 / 1028.0);
+  /* determine_masks_landocean, :109-151 */
+  if (c->do_ocean_floodfill) ocean_floodfill(g, Hi, Hb, SL, land, ocean);
+  else
+    for (size_t p = 0; p < na; ++p) {
+      const int flt = is_floating(Hi[p], Hb[p], SL[p]);
+      land[p] = flt ? 0 : 1;
+      ocean[p] = flt ? 1 : 0;
+    }
+  /* determine_masks_ice, :152-201 */
+  for (size_t p = 0; p < na; ++p) {
+    ice[p] = Hi[p] > 2.2250738585072014e-308 ? 1 : 0;
+    sheet[p] = (ice[p] == 1 && land[p] == 1) ? 1 : 0;
+    shelf[p] = (ice[p] == 1 && ocean[p] == 1) ? 1 : 0;
+  }
+  /* determine_masks_transitions, :202-281 (interior cells only) */
+  for (size_t p = 0; p < na; ++p) {
+    SETI(io->mask_coast_a, p, 0); SETI(io->mask_margin_a, p, 0);
+    SETI(io->mask_gl_a, p, 0); SETI(io->mask_cf_a, p, 0);
+  }
+  int32_t *coast = ialloc(na), *margin = ialloc(na), *gl = ialloc(na), *cf = ialloc(na);
+  for (int i = 1; i < nx - 1; ++i)
+    for (int j = 1; j < ny - 1; ++j) {
+      const size_t p = A2(j, i), s = A2(j - 1, i), n = A2(j + 1, i), w = A2(j, i - 1), e = A2(j, i + 1);
+      if (land[p] == 1 && ice[p] == 0 && (ocean[s] == 1 || ocean[n] == 1 || ocean[w] == 1 || ocean[e] == 1))
+        coast[p] = 1;
+      if (ice[p] == 1 && (ice[s] == 0 || ice[n] == 0 || ice[w] == 0 || ice[e] == 0)) margin[p] = 1;
+      if (sheet[p] == 1 && (shelf[s] == 1 || shelf[n] == 1 || shelf[w] == 1 || shelf[e] == 1)) gl[p] = 1;
+      if (ice[p] == 1 && ((ocean[s] == 1 && ice[s] == 0) || (ocean[n] == 1 && ice[n] == 0) ||
+                          (ocean[w] == 1 && ice[w] == 0) || (ocean[e] == 1 && ice[e] == 0)))
+        cf[p] = 1;
+    }
+  /* determine_masks_total, :282-346 */
+  for (size_t p = 0; p < na; ++p) {
+    int32_t t = land[p] == 1 ? IMAU_TYPE_LAND : IMAU_TYPE_OCEAN;
+    if (coast[p] == 1) t = IMAU_TYPE_COAST;
+    if (sheet[p] == 1) t = IMAU_TYPE_SHEET;
+    else if (shelf[p] == 1) t = IMAU_TYPE_SHELF;
+    if (margin[p] == 1) t = IMAU_TYPE_MARGIN;
+    if (gl[p] == 1) t = IMAU_TYPE_GROUNDINGLINE;
+    if (cf[p] == 1) t = IMAU_TYPE_CALVINGFRONT;
+    SETI(io->mask_a, p, t);
+    SETI(io->mask_land_a, p, land[p]); SETI(io->mask_ocean_a, p, ocean[p]);
+    SETI(io->mask_ice_a, p, ice[p]); SETI(io->mask_sheet_a, p, sheet[p]); SETI(io->mask_shelf_a, p, shelf[p]);
+    SETI(io->mask_coast_a, p, coast[p]); SETI(io->mask_margin_a, p, margin[p]);
+    SETI(io->mask_gl_a, p, gl[p]); SETI(io->mask_cf_a, p, cf[p]);
+  }
+  /* :62-80 */
+  if (io->Hi_cx) map_a_to_cx(g, Hi, io->Hi_cx);
+  if (io->Hi_cy) map_a_to_cy(g, Hi, io->Hi_cy);
+  if (io->Hi_b) map_a_to_b(g, Hi, io->Hi_b);
+  if (io->A_flow_3D_a && io->A_flow_3D_cx) map_a_to_cx_3d(g, io->A_flow_3D_a, io->A_flow_3D_cx);
+  if (io->A_flow_3D_a && io->A_flow_3D_cy) map_a_to_cy_3d(g, io->A_flow_3D_a, io->A_flow_3D_cy);
+  if (io->A_flow_vav_a && io->A_flow_vav_cx) map_a_to_cx(g, io->A_flow_vav_a, io->A_flow_vav_cx);
+  if (io->A_flow_vav_a && io->A_flow_vav_cy) map_a_to_cy(g, io->A_flow_vav_a, io->A_flow_vav_cy);
+  if (io->dHi_dx_a) ddx_a_to_a(g, Hi, io->dHi_dx_a);
+  if (io->dHi_dy_a) ddy_a_to_a(g, Hi, io->dHi_dy_a);
+  if (io->dHs_dx_a) ddx_a_to_a(g, Hs, io->dHs_dx_a);
+  if (io->dHs_dy_a) ddy_a_to_a(g, Hs, io->dHs_dy_a);
+  if (io->dHs_dx_cx) ddx_a_to_cx(g, Hs, io->dHs_dx_cx);
+  if (io->dHs_dy_cx) ddy_a_to_cx(g, Hs, io->dHs_dy_cx);
+  if (io->dHs_dx_cy) ddx_a_to_cy(g, Hs, io->dHs_dx_cy);
+  if (io->dHs_dy_cy) ddy_a_to_cy(g, Hs, io->dHs_dy_cy);
+  if (!io->Hs_a) free(Hs);
+  free(land); free(ocean); free(ice); free(sheet); free(shelf);
+  free(coast); free(margin); free(gl); free(cf);
+  return 0;
+}
+
+int orc_calc_ice_rheology(const imau_grid *gr, const imau_producer_cfg *c, imau_producer_io *io, double time)
+{
+  G gg = {gr->nx, gr->ny, c->nz, gr->dx, gr->x, gr->y, NULL};
+  const G *g = &gg;
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  const size_t na = (size_t)nx * ny;
+  /* :654-658 */
+  const double A_low = 1.14e-05, A_high = 5.47e+10, Q_low = 6.0e+04, Q_high = 13.9e+04, R_gas = 8.314;
+  double *A3 = io->A_flow_3D_a, *Av = io->A_flow_vav_a;
+  if (!A3 || !Av) { snprintf(g_err, sizeof g_err, "calc_ice_rheology needs A_flow_3D_a and A_flow_vav_a"); return -1; }
+  if (c->choice_ice_rheology == IMAU_RHEO_UNIFORM) {          /* :664-669 */
+    for (size_t q = 0; q < na * nz; ++q) A3[q] = c->uniform_flow_factor;
+  } else if (c->choice_ice_rheology == IMAU_RHEO_HUYBRECHTS1992) {  /* :670-699 */
+    if (!io->Ti_a || !io->mask_ice_a) { snprintf(g_err, sizeof g_err, "Huybrechts1992 needs Ti_a and mask_ice_a"); return -1; }
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < nz; ++k) {
+          const size_t q = A3I(k, j, i);
+          if (io->mask_ice_a[A2(j, i)] == 1) {
+            const double T = io->Ti_a[q];
+            A3[q] = T < 263.15 ? A_low * exp(-Q_low / (R_gas * T)) : A_high * exp(-Q_high / (R_gas * T));
+          } else {
+            A3[q] = c->choice_ice_margin == IMAU_MARGIN_BC ? 0.0 : A_low * exp(-Q_low / (R_gas * 263.15));
+          }
+        }
+  } else if (c->choice_ice_rheology == IMAU_RHEO_MISMIP_MOD) {     /* :701-713 */
+    double A = 1.0e-16;
+    if (time < 15000.0) A = 1.0e-16;
+    else if (time < 30000.0) A = 1.0e-17;
+    else if (time < 45000.0) A = 1.0e-16;
+    for (size_t q = 0; q < na * nz; ++q) A3[q] = A;
+  } else {
+    snprintf(g_err, sizeof g_err, "unknown choice_ice_rheology %d", c->choice_ice_rheology);
+    return -1;
+  }
+  /* :719-729 enhancement factors */
+  if (io->mask_sheet_a && io->mask_shelf_a)
+    for (size_t p = 0; p < na; ++p) {
+      if (io->mask_sheet_a[p] != 1 && io->mask_shelf_a[p] != 1) continue;
+      const double f = io->mask_sheet_a[p] == 1 ? c->m_enh_sheet : c->m_enh_shelf;
+      for (int k = 0; k < nz; ++k) A3[p * nz + k] *= f;
+    }
+  /* :731-737 */
+  for (size_t p = 0; p < na; ++p) Av[p] = orc_vertical_average(nz, c->zeta, A3 + p * nz);
+  /* :739-743 */
+  if (io->A_flow_3D_cx) map_a_to_cx_3d(g, A3, io->A_flow_3D_cx);
+  if (io->A_flow_3D_cy) map_a_to_cy_3d(g, A3, io->A_flow_3D_cy);
+  if (io->A_flow_vav_cx) map_a_to_cx(g, Av, io->A_flow_vav_cx);
+  if (io->A_flow_vav_cy) map_a_to_cy(g, Av, io->A_flow_vav_cy);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* grounded fractions, src/general_ice_model_data_module.f90:349-781         */
 /* ------------------------------------------------------------------------ */
 static void rotate_quad(double *NW, double *NE, double *SW, double *SE)

@@ -70,6 +70,10 @@ int    orc_banded_lu_solve(int m, const int32_t *ptr, const int32_t *idx, const 
                            const double *b, double *x);
 /* ice-thickness update (src/ice_thickness_module.f90:25-909) and the
  * advective critical time step (src/ice_dynamics_module.f90:726-762) */
+/* update_general_ice_model_data with the full masks (and the ocean flood
+ * fill) and calc_ice_rheology, with the library's producer structs */
+int    orc_producers_update(const imau_grid *g, const imau_producer_cfg *c, imau_producer_io *io);
+int    orc_calc_ice_rheology(const imau_grid *g, const imau_producer_cfg *c, imau_producer_io *io, double time);
 int    orc_thickness_update(const imau_grid *g, const imau_thickness_cfg *c, imau_thickness_io *io, double dt);
 double orc_critical_timestep_adv(const imau_grid *g, const double *u_cx, const double *v_cy, double dt_max);
 /* Schoof (2006) ice-stream closed form, src/utilities_module.f90:836-877 */

@@ -136,3 +136,24 @@ def critical_timestep_adv(grid, u, v, dt_max):
     u = np.asfortranarray(u, dtype=np.float64)
     v = np.asfortranarray(v, dtype=np.float64)
     return L.orc_critical_timestep_adv(ctypes.byref(g), _ptr(u), _ptr(v), ctypes.c_double(dt_max))
+
+
+def producers_update(grid, pcfg, ps, omit=()):
+    """The oracle's update_general_ice_model_data (full masks) on ProducerState ps in place."""
+    L = lib()
+    g = grid.struct()
+    io = ps.io(omit)
+    rc = L.orc_producers_update(ctypes.byref(g), ctypes.byref(pcfg), ctypes.byref(io))
+    if rc != 0:
+        raise RuntimeError("oracle producers failed (%d): %s" % (rc, L.orc_last_error().decode()))
+    return ps
+
+
+def calc_ice_rheology(grid, pcfg, ps, time=0.0, omit=()):
+    L = lib()
+    g = grid.struct()
+    io = ps.io(omit)
+    rc = L.orc_calc_ice_rheology(ctypes.byref(g), ctypes.byref(pcfg), ctypes.byref(io), ctypes.c_double(time))
+    if rc != 0:
+        raise RuntimeError("oracle rheology failed (%d): %s" % (rc, L.orc_last_error().decode()))
+    return ps

@@ -45,6 +45,28 @@ def test_library_exports_every_thickness_entry_point():
         assert hasattr(L, name), name
 
 
+def test_library_exports_every_producer_entry_point():
+    L = solver.load_library()
+    txt = open(abi.HEADER).read()
+    declared = set(re.findall(r"\b(imau_(?:producer|update|calc)_\w+)\s*\(", txt))
+    assert declared == set(solver.PRODUCER_EXPORTED)
+    for name in sorted(declared):
+        assert hasattr(L, name), name
+
+
+def test_producer_defaults_match_the_python_mirror():
+    from imau_vel import config
+    L = solver.load_library()
+    c = abi.ImauProducerCfg()
+    L.imau_producer_default_config(ctypes.byref(c))
+    ref = config.producer_config()
+    for name, t in abi.ImauProducerCfg._fields_:
+        if hasattr(t, "_length_"):
+            assert list(getattr(c, name)) == list(getattr(ref, name)), name
+        else:
+            assert getattr(c, name) == getattr(ref, name), name
+
+
 def test_thickness_defaults_match_the_python_mirror():
     from imau_vel import config
     L = solver.load_library()
