@@ -1972,9 +1972,8 @@ struct ThkDev {
   }
 };
 
-// the update on the device buffers, then the download into io (Qx/Qy of the explicit scheme)
-void thickness_run(imau_ctx *ctx, const Member &mb, ThkDev &D, const imau_thickness_cfg *c, imau_thickness_io *io,
-                   double dt) {
+// the update on the device buffers: D.dHi, D.Ht (and the explicit scheme's fluxes)
+void thickness_compute(imau_ctx *ctx, const Member &mb, ThkDev &D, const imau_thickness_cfg *c, double dt) {
   hipStream_t s = ctx->s;
   const size_t d8 = sizeof(double);
   const long long na = D.na;
@@ -2027,6 +2026,16 @@ void thickness_run(imau_ctx *ctx, const Member &mb, ThkDev &D, const imau_thickn
   }
   imau::thk_bc(s, mb, D.in, &c->ice_thickness_west_BC, c->choice_ice_dynamics == IMAU_DYN_DIVA, dt, D.dHi, D.Ht);
   HIPCHK(hipGetLastError());
+}
+
+// the update on the device buffers, then the download into io (Qx/Qy of the explicit scheme)
+void thickness_run(imau_ctx *ctx, const Member &mb, ThkDev &D, const imau_thickness_cfg *c, imau_thickness_io *io,
+                   double dt) {
+  thickness_compute(ctx, mb, D, c, dt);
+  hipStream_t s = ctx->s;
+  const size_t d8 = sizeof(double);
+  const long long na = D.na;
+  const int method = c->choice_ice_integration_method;
   const int nx = mb.nx, ny = mb.ny;
   if (io->dHi_dt_a) HIPCHK(hipMemcpyAsync(io->dHi_dt_a, D.dHi, na * d8, hipMemcpyDeviceToHost, s));
   if (io->Hi_tplusdt_a) HIPCHK(hipMemcpyAsync(io->Hi_tplusdt_a, D.Ht, na * d8, hipMemcpyDeviceToHost, s));
@@ -2640,6 +2649,234 @@ int imau_calc_ice_rheology(imau_ctx *ctx, const imau_grid *grid, const imau_prod
   down2(s, io->A_flow_vav_cx, D.Avcx, nx, ny, 'x');
   down2(s, io->A_flow_vav_cy, D.Avcy, nx, ny, 'y');
   HIPCHK(hipStreamSynchronize(s));
+  return IMAU_OK;
+  ABI_CATCH(ctx)
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// time stepping with choice_timestepping = 'direct' on one grid, everything
+// device-resident between steps: the velocity problem's own fields are the
+// geometry the producers update in place
+// ---------------------------------------------------------------------------
+namespace {
+
+void check_timestep_cfg(const imau_timestep_cfg *c) {
+  if (!(c->dt_startup_phase > 0.0)) fail(IMAU_ERR_CONFIG, "dt_startup_phase must be > 0");
+  if (!(c->dt_min > 0.0) || c->dt_max < c->dt_min) fail(IMAU_ERR_CONFIG, "need 0 < dt_min <= dt_max");
+}
+
+struct LoopDev {
+  DBuf<double> f;       // SMB, BMB, float margin fraction, dHs/dt, A_vav on cx / cy, zero
+  DBuf<int32_t> m;      // coast, margin, grounding line, calving front, total, shelf keep
+  DBuf<int> flag;
+  DBuf<double> red;     // reduction partials and the result
+  double *SMB, *BMB, *fmf, *dHs_dt, *Avcx, *Avcy, *zero;
+  int32_t *keep;
+  int nb = 0;
+};
+
+}  // namespace
+
+extern "C" {
+
+int imau_run_ice_dynamics_direct(imau_ctx *ctx, const imau_grid *grid, const imau_cfg *cfg,
+                                 const imau_thickness_cfg *tcfg, const imau_producer_cfg *pcfg,
+                                 const imau_timestep_cfg *scfg, imau_ice_io *io, const double *SMB_year,
+                                 const double *BMB, double *Hi_end, imau_producer_io *geo, imau_timestep_state *st,
+                                 double t_end) {
+  ABI_TRY
+  if (!ctx || !grid || !cfg || !tcfg || !pcfg || !scfg || !io || !SMB_year || !BMB || !Hi_end || !st)
+    fail(IMAU_ERR_ARG, "NULL argument");
+  const int dyn = cfg->choice_ice_dynamics;
+  if (dyn != IMAU_DYN_NONE && dyn != IMAU_DYN_SIA && dyn != IMAU_DYN_SSA && dyn != IMAU_DYN_SIA_SSA)
+    fail(IMAU_ERR_CONFIG, "direct time stepping works with 'none', 'SIA', 'SSA' or 'SIA/SSA' only");
+  check_cfg(cfg);
+  check_thickness_cfg(tcfg);
+  check_producer_cfg(pcfg);
+  check_timestep_cfg(scfg);
+  if (pcfg->nz != cfg->nz) fail(IMAU_ERR_CONFIG, "producer nz=%d differs from the solver's nz=%d", pcfg->nz, cfg->nz);
+  imau_cfg vcfg = *cfg;
+  if (dyn == IMAU_DYN_NONE) vcfg.choice_ice_dynamics = IMAU_DYN_SIA;  // the problem only carries the geometry
+  imau_problem *P = nullptr;
+  int rc = imau_vel_problem_create(ctx, 1, grid, &vcfg, io, &P);
+  if (rc) return rc;
+  struct Release {
+    imau_problem *&p;
+    ~Release() { imau_vel_problem_destroy(p); }
+  } release{P};
+  hipStream_t s = ctx->s;
+  const Member mb = P->mem[0];
+  const int nx = mb.nx, ny = mb.ny, nz = cfg->nz;
+  const long long na = (long long)nx * ny;
+  const size_t d8 = sizeof(double), i4 = sizeof(int32_t);
+  LoopDev X;
+  X.f.alloc((size_t)7 * na);
+  X.m.alloc((size_t)6 * na);
+  X.flag.alloc(1);
+  X.nb = imau::thk_blocks(mb);
+  X.red.alloc((size_t)X.nb + 1);
+  {
+    double *q = X.f.p;
+    X.SMB = q; X.BMB = q + na; X.fmf = q + 2 * na; X.dHs_dt = q + 3 * na; X.Avcx = q + 4 * na; X.Avcy = q + 5 * na;
+    X.zero = q + 6 * na;
+  }
+  HIPCHK(hipMemcpyAsync(X.SMB, SMB_year, na * d8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(X.BMB, BMB, na * d8, hipMemcpyHostToDevice, s));
+  // the problem's fields are the geometry the producers update
+  double *Hi = f2p(P, F_Hi);
+  const imau::ProdMasks mk{i2p(P, I_land), i2p(P, I_ocean), i2p(P, I_ice), i2p(P, I_sheet), i2p(P, I_shelf),
+                           X.m.p, X.m.p + na, X.m.p + 2 * na, X.m.p + 3 * na, X.m.p + 4 * na};
+  X.keep = X.m.p + 5 * na;
+  imau::Prod2 o{};
+  o.Hs = f2p(P, F_Hs); o.TAF = f2p(P, F_TAF); o.dHs_dt = X.dHs_dt;
+  o.Hi_cx = f2p(P, F_Hi_cx); o.Hi_cy = f2p(P, F_Hi_cy); o.Hi_b = f2p(P, F_Hi_b);
+  o.dHi_dx = f2p(P, F_dHi_dx_a); o.dHi_dy = f2p(P, F_dHi_dy_a); o.dHs_dx = f2p(P, F_dHs_dx_a);
+  o.dHs_dy = f2p(P, F_dHs_dy_a); o.dHs_dx_cx = f2p(P, F_dHs_dx_cx); o.dHs_dy_cx = f2p(P, F_dHs_dy_cx);
+  o.dHs_dx_cy = f2p(P, F_dHs_dx_cy); o.dHs_dy_cy = f2p(P, F_dHs_dy_cy);
+  const imau::Str3 s3{na, (long long)ny, 1};  // k-plane major
+  ThkDev T;
+  T.alloc(mb);
+  T.in = imau::ThkIn{Hi, f2p(P, F_Hb), f2p(P, F_SL), P->O.uvav, P->O.vvav, X.SMB, X.BMB, X.fmf,
+                     mk.ice, mk.shelf, mk.ocean, mk.cf};
+  auto flood = [&](auto round) {  // rounds of tile-local growth until one changes nothing
+    int changed = 1;
+    for (long long r = 0; changed && r <= na; ++r) {
+      HIPCHK(hipMemsetAsync(X.flag.p, 0, sizeof(int), s));
+      round();
+      HIPCHK(hipMemcpyAsync(&changed, X.flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+    }
+  };
+  // update_general_ice_model_data (with the rates of the last step, or none)
+  auto producers = [&](const double *dHi) {
+    const imau::ProdIn in{Hi, f2p(P, F_Hb), f2p(P, F_SL), dHi, dHi ? f2p(P, F_dHb_dt) : nullptr};
+    imau::prod_point(s, mb, in, o, mk, pcfg->do_ocean_floodfill ? 1 : 0);
+    if (pcfg->do_ocean_floodfill) {
+      flood([&]() { imau::prod_flood_round(s, mb, in, mk, X.flag.p); });
+      imau::prod_land(s, mb, mk);
+    }
+    imau::prod_masks_maps(s, mb, in, o, mk);
+    imau::prod_map_A(s, mb, nz, f3p(P, G_A3), s3, P->Acx3, s3, P->Acy3, s3, f2p(P, F_A_vav), X.Avcx, X.Avcy);
+  };
+  auto scalar = [&](double *dev) {
+    double v = 0.0;
+    HIPCHK(hipMemcpyAsync(&v, dev, d8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return v;
+  };
+  // the derived fields of the state at entry, and its floating-margin fraction
+  producers(nullptr);
+  imau::prod_float_margin(s, mb, Hi, mk, X.fmf);
+  imau::Zeta zeta{};
+  for (int k = 0; k < nz; ++k) zeta.z[k] = cfg->zeta[k];
+  const int32_t nit0 = P->sor_nit_;
+  const double tol0 = P->sor_tol_, rtol0 = P->petsc_rtol_, atol0 = P->petsc_abstol_;
+  double time = st->time;
+  st->nsteps = 0;
+  while (time < t_end) {
+    // :88-99: tighter velocity solves in the start-up phase (r floored at
+    // 1e-3: the reference's r = 0 at the very start asks for unbounded iterations)
+    double r = 1.0;
+    if (time <= scfg->start_time_of_run + scfg->dt_startup_phase)
+      r = std::max(1e-3, 0.01 * 0.99 * (time - scfg->start_time_of_run) / scfg->dt_startup_phase);
+    P->sor_nit_ = nit0 * (int32_t)std::ceil(1.0 / r);
+    P->sor_tol_ = tol0 * r;
+    P->petsc_rtol_ = rtol0 * r;
+    P->petsc_abstol_ = atol0 * r;
+    // :101-108
+    double dt_max;
+    if (time <= scfg->start_time_of_run + scfg->dt_startup_phase)
+      dt_max = scfg->dt_min + (scfg->dt_max - scfg->dt_min) *
+                                  std::pow((time - scfg->start_time_of_run) / scfg->dt_startup_phase, 2);
+    else if (time >= scfg->end_time_of_run - scfg->dt_startup_phase)
+      dt_max = scfg->dt_min + (scfg->dt_max - scfg->dt_min) *
+                                  std::pow((scfg->end_time_of_run - time) / scfg->dt_startup_phase, 2);
+    else
+      dt_max = scfg->dt_max;
+    if ((dyn == IMAU_DYN_SIA || dyn == IMAU_DYN_SIA_SSA) && time == st->t_next_SIA) {
+      if ((rc = solve_kind(ctx, P, cfg, SOLVE_SIA))) return rc;
+      imau::crit_sia(s, mb, nz, zeta, P->O.u3sia, P->O.v3sia, P->O.u3, P->O.v3, o.Hi_cx, o.Hi_cy, o.dHs_dx_cx,
+                     o.dHs_dy_cy, mk.sheet, X.red.p, X.red.p + X.nb);
+      HIPCHK(hipGetLastError());
+      double d = 0.9 * std::min(scfg->dt_max, scalar(X.red.p + X.nb));
+      d = std::max(scfg->dt_min, std::min(dt_max, d));
+      st->dt_crit_SIA = d;
+      st->t_next_SIA = time + d;
+    }
+    if ((dyn == IMAU_DYN_SSA || dyn == IMAU_DYN_SIA_SSA) && time == st->t_next_SSA) {
+      if ((rc = solve_kind(ctx, P, cfg, SOLVE_SSA))) return rc;
+      imau::cfl(s, mb, P->O.uvav, P->O.vvav, X.red.p, X.red.p + X.nb);
+      HIPCHK(hipGetLastError());
+      double d = std::min(scfg->dt_max, std::min(2.0 * scfg->dt_max, scalar(X.red.p + X.nb)) * 0.9);
+      d = std::max(scfg->dt_min, std::min(dt_max, d));
+      st->dt_crit_SSA = d;
+      st->t_next_SSA = time + d;
+    }
+    // determine_timesteps_and_actions, :838-856 and :933
+    double t_next = std::min(t_end, time + scfg->dt_max);
+    if (dyn == IMAU_DYN_SIA || dyn == IMAU_DYN_SIA_SSA) t_next = std::min(t_next, st->t_next_SIA);
+    if (dyn == IMAU_DYN_SSA || dyn == IMAU_DYN_SIA_SSA) t_next = std::min(t_next, st->t_next_SSA);
+    const double dt = t_next - time;
+    st->dt = dt;
+    // :224-236
+    if (dyn == IMAU_DYN_NONE) {
+      HIPCHK(hipMemsetAsync(T.dHi, 0, na * d8, s));
+      HIPCHK(hipMemcpyAsync(T.Ht, Hi, na * d8, hipMemcpyDeviceToDevice, s));
+    } else {
+      thickness_compute(ctx, mb, T, tcfg, dt);
+      imau::thk_advance(s, mb, Hi, T.dHi, dt, T.Ht);
+    }
+    // update_ice_thickness, :457-645 (calving law 'none', no optional removals)
+    imau::thk_clip(s, mb, T.Ht, Hi);
+    producers(nullptr);                                   // determine_masks_ice / _transitions
+    imau::prod_float_margin(s, mb, Hi, mk, X.fmf);
+    imau::prod_shelf_seed(s, mb, mk, X.keep);             // remove_unconnected_shelves
+    flood([&]() { imau::prod_shelf_round(s, mb, mk, X.keep, X.flag.p); });
+    imau::prod_shelf_remove(s, mb, mk, X.keep, Hi);
+    producers(T.dHi);                                     // update_general_ice_model_data
+    HIPCHK(hipGetLastError());
+    time += dt;
+    ++st->nsteps;
+  }
+  st->time = time;
+  P->sor_nit_ = nit0;
+  P->sor_tol_ = tol0;
+  P->petsc_rtol_ = rtol0;
+  P->petsc_abstol_ = atol0;
+  // outputs: velocities through the problem's download, the geometry here
+  if (dyn != IMAU_DYN_NONE && (rc = imau_vel_problem_download(ctx, P, io))) return rc;
+  HIPCHK(hipMemcpyAsync(Hi_end, Hi, na * d8, hipMemcpyDeviceToHost, s));
+  if (geo) {
+    down2(s, geo->Hs_a, o.Hs, nx, ny, 'a');
+    down2(s, geo->TAF_a, o.TAF, nx, ny, 'a');
+    if (st->nsteps > 0) down2(s, geo->dHs_dt_a, o.dHs_dt, nx, ny, 'a');
+    const int32_t *msrc[10] = {mk.land, mk.ocean, mk.ice, mk.sheet, mk.shelf, mk.coast, mk.margin, mk.gl, mk.cf,
+                               mk.total};
+    int32_t *mdst[10] = {geo->mask_land_a, geo->mask_ocean_a, geo->mask_ice_a, geo->mask_sheet_a, geo->mask_shelf_a,
+                         geo->mask_coast_a, geo->mask_margin_a, geo->mask_gl_a, geo->mask_cf_a, geo->mask_a};
+    for (int q = 0; q < 10; ++q) down2(s, mdst[q], msrc[q], nx, ny, 'a');
+    down2(s, geo->Hi_cx, o.Hi_cx, nx, ny, 'x');
+    down2(s, geo->Hi_cy, o.Hi_cy, nx, ny, 'y');
+    down2(s, geo->Hi_b, o.Hi_b, nx, ny, 'b');
+    down2(s, geo->dHi_dx_a, o.dHi_dx, nx, ny, 'a');
+    down2(s, geo->dHi_dy_a, o.dHi_dy, nx, ny, 'a');
+    down2(s, geo->dHs_dx_a, o.dHs_dx, nx, ny, 'a');
+    down2(s, geo->dHs_dy_a, o.dHs_dy, nx, ny, 'a');
+    down2(s, geo->dHs_dx_cx, o.dHs_dx_cx, nx, ny, 'x');
+    down2(s, geo->dHs_dy_cx, o.dHs_dy_cx, nx, ny, 'x');
+    down2(s, geo->dHs_dx_cy, o.dHs_dx_cy, nx, ny, 'y');
+    down2(s, geo->dHs_dy_cy, o.dHs_dy_cy, nx, ny, 'y');
+    down2(s, geo->A_flow_vav_a, f2p(P, F_A_vav), nx, ny, 'a');
+    down2(s, geo->A_flow_vav_cx, X.Avcx, nx, ny, 'x');
+    down2(s, geo->A_flow_vav_cy, X.Avcy, nx, ny, 'y');
+    down3(ctx, P, 0, KA, geo->A_flow_3D_a, f3p(P, G_A3));
+    down3(ctx, P, 0, KCX, geo->A_flow_3D_cx, P->Acx3);
+    down3(ctx, P, 0, KCY, geo->A_flow_3D_cy, P->Acy3);
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  (void)i4;
   return IMAU_OK;
   ABI_CATCH(ctx)
 }

@@ -42,6 +42,13 @@ void prod_masks_maps(hipStream_t s, const Member &m, const ProdIn &in, const Pro
 // (null: skipped); nz levels with the given strides
 void prod_map_A(hipStream_t s, const Member &m, int nz, const double *A3, Str3 sa, double *A3cx, Str3 sx, double *A3cy,
                 Str3 sy, const double *Av, double *Avcx, double *Avcy);
+// determine_floating_margin_fraction (reads mk.ice, mk.cf, mk.shelf)
+void prod_float_margin(hipStream_t s, const Member &m, const double *Hi, const ProdMasks &mk, double *fmf);
+// remove_unconnected_shelves: seeds into keep, rounds of growth through the
+// 4 neighbours of shelf cells (*changed set when a cell changed), removal
+void prod_shelf_seed(hipStream_t s, const Member &m, const ProdMasks &mk, int32_t *keep);
+void prod_shelf_round(hipStream_t s, const Member &m, const ProdMasks &mk, int32_t *keep, int *changed);
+void prod_shelf_remove(hipStream_t s, const Member &m, const ProdMasks &mk, const int32_t *keep, double *Hi);
 // calc_ice_rheology's flow factor, enhancement factors and vertical average
 void prod_rheology(hipStream_t s, const Member &m, const imau_producer_cfg &c, double time, const double *Ti, Str3 st,
                    const ProdMasks &mk, double *A3, Str3 sa, double *Av);

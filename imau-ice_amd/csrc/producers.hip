@@ -21,6 +21,12 @@
 #include "dev_common.h"
 #include "producers.h"
 
+// Multiply-adds are not contracted in this file: the thickness and mask
+// arithmetic rounds as the reference's does, so that a thickness driven to
+// zero (H + dt (-H / dt)) is zero and not a residue of a fused multiply-add
+// that would count as ice.
+#pragma clang fp contract(off)
+
 namespace {
 
 constexpr int TB = 256;
@@ -40,17 +46,13 @@ __device__ __forceinline__ bool cell_of(const Member &m, int q, Cell &c) {
 }
 
 // src/utilities_module.f90:126-158, with the reference's order of operations
-// and no contraction into fused multiply-adds
 __device__ __forceinline__ bool is_floating(double Hi, double Hb, double SL) {
-#pragma clang fp contract(off)
   return Hi < (SL - Hb) * RHO_W / RHO_I;
 }
 __device__ __forceinline__ double surface_elevation(double Hi, double Hb, double SL) {
-#pragma clang fp contract(off)
   return Hi + fmax(SL - RHO_I / RHO_W * Hi, Hb);
 }
 __device__ __forceinline__ double thickness_above_floatation(double Hi, double Hb, double SL) {
-#pragma clang fp contract(off)
   return Hi - fmax(0.0, (SL - Hb) * (RHO_W / RHO_I));
 }
 
@@ -64,7 +66,6 @@ __global__ void __launch_bounds__(TB) k_prod_point(const Member m, const imau::P
   if (o.TAF) o.TAF[p] = thickness_above_floatation(Hi, Hb, SL);
   // :47-52 reads the land mask before determine_masks updates it
   if (in.dHi_dt && in.dHb_dt && o.dHs_dt) {
-#pragma clang fp contract(off)
     o.dHs_dt[p] = mk.land[p] == 1 ? in.dHb_dt[p] + in.dHi_dt[p] : in.dHi_dt[p] * (1.0 - RHO_I / RHO_W);
   }
   const bool flt = is_floating(Hi, Hb, SL);
@@ -283,6 +284,108 @@ __global__ void __launch_bounds__(TB) k_prod_rheology(const Member m, const imau
   Av[c.p] = s;
 }
 
+// determine_floating_margin_fraction (src/general_ice_model_data_module.f90:803-858):
+// rows 1 .. ny-2 only, as the reference
+__global__ void __launch_bounds__(TB) k_float_margin(const Member m, const double *__restrict__ Hi,
+                                                    const imau::ProdMasks mk, double *__restrict__ fmf) {
+  Cell c;
+  if (!cell_of(m, blockIdx.x * TB + threadIdx.x, c)) return;
+  const int nx = m.nx, ny = m.ny, i = c.i, j = c.j;
+  if (j < 1 || j > ny - 2) return;
+  const long long p = c.p;
+  double f = mk.ice[p] == 1 ? 1.0 : 0.0;
+  if (mk.cf[p] == 1 && mk.shelf[p] == 1) {
+    bool noncf = false;
+    double hmax = 0.0;
+    for (int ii = max(0, i - 1); ii <= min(nx - 1, i + 1); ++ii)
+      for (int jj = j - 1; jj <= j + 1; ++jj) {
+        const long long q = m.off + (long long)ii * ny + jj;
+        if (mk.ice[q] == 1 && mk.cf[q] == 0) {
+          noncf = true;
+          hmax = fmax(hmax, Hi[q]);
+        }
+      }
+    f = (!noncf || hmax < Hi[p]) ? 1.0 : Hi[p] / hmax;
+  }
+  fmf[p] = f;
+}
+
+// remove_unconnected_shelves (src/ice_thickness_module.f90:932-1044), what its
+// stack walk returns: the shelf cells 4-connected through shelf cells to an
+// interior shelf cell that has a 4-neighbour sheet cell keep their ice.
+// Seeds, then rounds of tile-local growth (as the ocean flood fill), then the removal.
+__global__ void __launch_bounds__(TB) k_shelf_seed(const Member m, const imau::ProdMasks mk, int32_t *__restrict__ keep) {
+  Cell c;
+  if (!cell_of(m, blockIdx.x * TB + threadIdx.x, c)) return;
+  const int nx = m.nx, ny = m.ny, i = c.i, j = c.j;
+  const long long p = c.p;
+  int k = 0;
+  if (i > 0 && i < nx - 1 && j > 0 && j < ny - 1 && mk.shelf[p] == 1)
+    k = (mk.sheet[p + ny] == 1 || mk.sheet[p - ny] == 1 || mk.sheet[p + 1] == 1 || mk.sheet[p - 1] == 1) ? 1 : 0;
+  keep[p] = k;
+}
+
+__global__ void __launch_bounds__(TB) k_shelf_flood(const Member m, const int32_t *__restrict__ shelf,
+                                                   int32_t *__restrict__ keep, int *__restrict__ changed) {
+  __shared__ uint8_t al[FI + 2][FJ + 2], kp[FI + 2][FJ + 2];
+  __shared__ int any;
+  const int ntj = (m.ny + FJ - 1) / FJ;
+  const int i0 = (blockIdx.x / ntj) * FI, j0 = (blockIdx.x % ntj) * FJ;
+  const int tid = threadIdx.x;
+  for (int q = tid; q < (FI + 2) * (FJ + 2); q += TB) {
+    const int li = q / (FJ + 2), lj = q - li * (FJ + 2);
+    const int i = i0 + li - 1, j = j0 + lj - 1;
+    uint8_t a = 0, k = 0;
+    if (i >= 0 && j >= 0 && i < m.nx && j < m.ny) {
+      const long long p = m.off + (long long)i * m.ny + j;
+      a = shelf[p] == 1 ? 1 : 0;
+      k = keep[p] ? 1 : 0;
+    }
+    al[li][lj] = a;
+    kp[li][lj] = k;
+  }
+  bool grew = false;
+  for (;;) {
+    if (tid == 0) any = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int q = tid; q < FI * FJ; q += TB) {
+      const int li = 1 + q / FJ, lj = 1 + q % FJ;
+      if (!al[li][lj] || kp[li][lj]) continue;
+      if (kp[li - 1][lj] | kp[li + 1][lj] | kp[li][lj - 1] | kp[li][lj + 1]) {
+        kp[li][lj] = 1;
+        mine = 1;
+      }
+    }
+    if (mine) any = 1;
+    __syncthreads();
+    const int a = any;
+    __syncthreads();  // every thread has read `any` before it is cleared again
+    if (!a) break;
+    grew = true;
+  }
+  if (!grew) return;
+  int wrote = 0;
+  for (int q = tid; q < FI * FJ; q += TB) {
+    const int li = 1 + q / FJ, lj = 1 + q % FJ;
+    const int i = i0 + li - 1, j = j0 + lj - 1;
+    if (i >= m.nx || j >= m.ny || !kp[li][lj]) continue;
+    const long long p = m.off + (long long)i * m.ny + j;
+    if (!keep[p]) {
+      keep[p] = 1;
+      wrote = 1;
+    }
+  }
+  if (wrote) *changed = 1;
+}
+
+__global__ void __launch_bounds__(TB) k_shelf_remove(const Member m, const int32_t *__restrict__ shelf,
+                                                    const int32_t *__restrict__ keep, double *__restrict__ Hi) {
+  Cell c;
+  if (!cell_of(m, blockIdx.x * TB + threadIdx.x, c)) return;
+  if (shelf[c.p] == 1 && !keep[c.p]) Hi[c.p] = 0.0;
+}
+
 int blocks(const Member &m) { return (m.nx * m.ny + TB - 1) / TB; }
 
 }  // namespace
@@ -307,6 +410,19 @@ void prod_map_A(hipStream_t s, const Member &m, int nz, const double *A3, Str3 s
   const long long n = (long long)m.nx * m.ny * (nz + 1);
   hipLaunchKernelGGL(k_prod_map_A, dim3((unsigned)((n + TB - 1) / TB)), dim3(TB), 0, s, m, nz, A3, sa, A3cx, sx, A3cy,
                      sy, Av, Avcx, Avcy);
+}
+void prod_float_margin(hipStream_t s, const Member &m, const double *Hi, const ProdMasks &mk, double *fmf) {
+  hipLaunchKernelGGL(k_float_margin, dim3(blocks(m)), dim3(TB), 0, s, m, Hi, mk, fmf);
+}
+void prod_shelf_seed(hipStream_t s, const Member &m, const ProdMasks &mk, int32_t *keep) {
+  hipLaunchKernelGGL(k_shelf_seed, dim3(blocks(m)), dim3(TB), 0, s, m, mk, keep);
+}
+void prod_shelf_round(hipStream_t s, const Member &m, const ProdMasks &mk, int32_t *keep, int *changed) {
+  const int nt = ((m.nx + FI - 1) / FI) * ((m.ny + FJ - 1) / FJ);
+  hipLaunchKernelGGL(k_shelf_flood, dim3(nt), dim3(TB), 0, s, m, mk.shelf, keep, changed);
+}
+void prod_shelf_remove(hipStream_t s, const Member &m, const ProdMasks &mk, const int32_t *keep, double *Hi) {
+  hipLaunchKernelGGL(k_shelf_remove, dim3(blocks(m)), dim3(TB), 0, s, m, mk.shelf, keep, Hi);
 }
 void prod_rheology(hipStream_t s, const Member &m, const imau_producer_cfg &c, double time, const double *Ti, Str3 st,
                    const ProdMasks &mk, double *A3, Str3 sa, double *Av) {

@@ -38,6 +38,18 @@ void thk_finish(hipStream_t s, const Member &m, const double *Hi, const double *
 // apply_ice_thickness_BC, sides west, east, south, north in that order
 void thk_bc(hipStream_t s, const Member &m, const ThkIn &in, const int *bcs, int diva, double dt, double *dHi,
             double *Ht);
+// the zeta levels of a column (a kernel argument)
+struct Zeta {
+  double z[IMAU_NZ_MAX];
+};
+// calc_critical_timestep_SIA's minimum over cells (before the 0.9 and the
+// dt_max cap) into *out (device); 3D fields k-plane major in the member's slab
+void crit_sia(hipStream_t s, const Member &m, int nz, const Zeta &zeta, const double *u3s, const double *v3s,
+              const double *u3, const double *v3, const double *Hi_cx, const double *Hi_cy, const double *dHs_dx_cx,
+              const double *dHs_dy_cy, const int32_t *sheet, double *partial, double *out);
+// Ht = Hi + dt dHi; Hi = max(0, Ht)
+void thk_advance(hipStream_t s, const Member &m, const double *Hi, const double *dHi, double dt, double *Ht);
+void thk_clip(hipStream_t s, const Member &m, const double *Ht, double *Hi);
 // min over interior cells of dx / (|u| + |v|) into *out (device)
 void cfl(hipStream_t s, const Member &m, const double *u, const double *v, double *partial, double *out);
 

@@ -16,6 +16,12 @@
 #include "dev_common.h"
 #include "thickness.h"
 
+// Multiply-adds are not contracted in this file: the thickness and mask
+// arithmetic rounds as the reference's does, so that a thickness driven to
+// zero (H + dt (-H / dt)) is zero and not a residue of a fused multiply-add
+// that would count as ice.
+#pragma clang fp contract(off)
+
 namespace {
 
 constexpr int TB = 256;
@@ -438,6 +444,70 @@ __global__ void __launch_bounds__(TB) k_min(const double *__restrict__ v, int n,
   if (threadIdx.x == 0) *out = mn[0];
 }
 
+// calc_critical_timestep_SIA (src/ice_dynamics_module.f90:670-721): per cell
+// dx^2 / (6 D_SIA) with D_SIA = max(1e-9, |u_vav H / dHs_dx| on the sheet's cx
+// and cy faces), and dx / (u_a + v_a) of every level of the 3D velocity;
+// block minima into partial (the 0.9 and the dt_max cap are the caller's)
+__global__ void __launch_bounds__(TB) k_crit_sia(const Member m, int nz, const imau::Zeta zeta,
+                                                const double *__restrict__ u3s, const double *__restrict__ v3s,
+                                                const double *__restrict__ u3, const double *__restrict__ v3,
+                                                const double *__restrict__ Hi_cx, const double *__restrict__ Hi_cy,
+                                                const double *__restrict__ dHs_dx_cx,
+                                                const double *__restrict__ dHs_dy_cy,
+                                                const int32_t *__restrict__ sheet, double *__restrict__ partial) {
+  Cell c;
+  double acc = INFINITY;
+  if (cell_of(m, blockIdx.x * TB + threadIdx.x, c)) {
+    const int nx = m.nx, ny = m.ny, i = c.i, j = c.j;
+    const long long plane = (long long)nx * ny;
+    auto q3 = [&](int k, int jj, int ii) { return m.off3 + k * plane + (long long)ii * ny + jj; };
+    auto vav = [&](const double *f, int jj, int ii) {  // vertical_average, src/utilities_module.f90:97-123
+      double s = 0.0;
+      for (int k = 0; k < nz - 1; ++k) s += 0.5 * (f[q3(k + 1, jj, ii)] + f[q3(k, jj, ii)]) * (zeta.z[k + 1] - zeta.z[k]);
+      return s;
+    };
+    double D = 1e-9;
+    if (i < nx - 1 && sheet[c.p] == 1 && sheet[c.p + ny] == 1)
+      D = fmax(D, fabs(vav(u3s, j, i) * Hi_cx[c.p] / dHs_dx_cx[c.p]));
+    if (j < ny - 1 && sheet[c.p] == 1 && sheet[c.p + 1] == 1)
+      D = fmax(D, fabs(vav(v3s, j, i) * Hi_cy[c.p] / dHs_dy_cy[c.p]));
+    acc = m.dx * m.dx / (6.0 * D);
+    for (int k = 0; k < nz; ++k) {
+      double ua, va;
+      if (i == 0) ua = fabs(u3[q3(k, j, 0)]);
+      else if (i == nx - 1) ua = fabs(u3[q3(k, j, nx - 2)]);
+      else ua = fmax(fabs(u3[q3(k, j, i - 1)]), fabs(u3[q3(k, j, i)]));
+      if (j == 0) va = fabs(v3[q3(k, 0, i)]);
+      else if (j == ny - 1) va = fabs(v3[q3(k, ny - 2, i)]);
+      else va = fmax(fabs(v3[q3(k, j - 1, i)]), fabs(v3[q3(k, j, i)]));
+      acc = fmin(acc, m.dx / (ua + va));
+    }
+  }
+  __shared__ double mn[TB];
+  mn[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = TB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) mn[threadIdx.x] = fmin(mn[threadIdx.x], mn[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = mn[0];
+}
+
+// Hi_tplusdt = Hi + dt dHi_dt (src/ice_dynamics_module.f90:233)
+__global__ void __launch_bounds__(TB) k_thk_advance(const Member m, const double *__restrict__ Hi,
+                                                   const double *__restrict__ dHi, double dt, double *__restrict__ Ht) {
+  Cell c;
+  if (!cell_of(m, blockIdx.x * TB + threadIdx.x, c)) return;
+  Ht[c.p] = Hi[c.p] + dt * dHi[c.p];
+}
+
+// Hi = max(0, Hi_tplusdt) (src/ice_dynamics_module.f90:559)
+__global__ void __launch_bounds__(TB) k_thk_clip(const Member m, const double *__restrict__ Ht, double *__restrict__ Hi) {
+  Cell c;
+  if (!cell_of(m, blockIdx.x * TB + threadIdx.x, c)) return;
+  Hi[c.p] = fmax(0.0, Ht[c.p]);
+}
+
 }  // namespace
 
 namespace imau {
@@ -502,6 +572,20 @@ void thk_bc(hipStream_t s, const Member &m, const ThkIn &in, const int *bcs, int
   }
 }
 
+void crit_sia(hipStream_t s, const Member &m, int nz, const Zeta &zeta, const double *u3s, const double *v3s,
+              const double *u3, const double *v3, const double *Hi_cx, const double *Hi_cy, const double *dHs_dx_cx,
+              const double *dHs_dy_cy, const int32_t *sheet, double *partial, double *out) {
+  const int nb = thk_blocks(m);
+  hipLaunchKernelGGL(k_crit_sia, dim3(nb), dim3(TB), 0, s, m, nz, zeta, u3s, v3s, u3, v3, Hi_cx, Hi_cy, dHs_dx_cx,
+                     dHs_dy_cy, sheet, partial);
+  hipLaunchKernelGGL(k_min, dim3(1), dim3(TB), 0, s, partial, nb, out);
+}
+void thk_advance(hipStream_t s, const Member &m, const double *Hi, const double *dHi, double dt, double *Ht) {
+  hipLaunchKernelGGL(k_thk_advance, dim3(thk_blocks(m)), dim3(TB), 0, s, m, Hi, dHi, dt, Ht);
+}
+void thk_clip(hipStream_t s, const Member &m, const double *Ht, double *Hi) {
+  hipLaunchKernelGGL(k_thk_clip, dim3(thk_blocks(m)), dim3(TB), 0, s, m, Ht, Hi);
+}
 void cfl(hipStream_t s, const Member &m, const double *u, const double *v, double *partial, double *out) {
   const int nb = thk_blocks(m);
   hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(TB), 0, s, m, u, v, partial);

@@ -108,6 +108,14 @@ class ImauProducerIO(ctypes.Structure):
     _fields_ = _struct_fields(_TXT, "imau_producer_io", DEFINES)
 
 
+class ImauTimestepCfg(ctypes.Structure):
+    _fields_ = _struct_fields(_TXT, "imau_timestep_cfg", DEFINES)
+
+
+class ImauTimestepState(ctypes.Structure):
+    _fields_ = _struct_fields(_TXT, "imau_timestep_state", DEFINES)
+
+
 IO_POINTER_FIELDS = [n for n, t in ImauIceIO._fields_ if hasattr(t, "contents")]
 
 # export the enum constants at module level (IMAU_OK, IMAU_DYN_DIVA, ...)

@@ -201,3 +201,27 @@ def producer_config(**overrides):
             raise AttributeError("imau_producer_cfg has no field %r" % k)
         setattr(c, k, v)
     return c
+
+
+def timestep_config(**overrides):
+    """Reference defaults of the time loop (src/configuration_module.f90:50-55)."""
+    c = abi.ImauTimestepCfg()
+    c.start_time_of_run = 0.0
+    c.end_time_of_run = 50000.0
+    c.dt_max = 10.0
+    c.dt_min = 0.01
+    c.dt_startup_phase = 10.0
+    for k, v in overrides.items():
+        if not hasattr(c, k):
+            raise AttributeError("imau_timestep_cfg has no field %r" % k)
+        setattr(c, k, v)
+    return c
+
+
+def timestep_state(time, t_next_SIA=None, t_next_SSA=None):
+    """region%time and the dynamics timers; a timer left None is due at `time`."""
+    s = abi.ImauTimestepState()
+    s.time = time
+    s.t_next_SIA = time if t_next_SIA is None else t_next_SIA
+    s.t_next_SSA = time if t_next_SSA is None else t_next_SSA
+    return s

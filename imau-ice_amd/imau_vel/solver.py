@@ -29,6 +29,8 @@ EXPORTED = (
 THICKNESS_EXPORTED = ("imau_thickness_default_config", "imau_thickness_update", "imau_critical_timestep_adv")
 # the input producers the solve reads
 PRODUCER_EXPORTED = ("imau_producer_default_config", "imau_update_general_ice_model_data", "imau_calc_ice_rheology")
+# the time loop that calls them
+TIMELOOP_EXPORTED = ("imau_run_ice_dynamics_direct",)
 
 
 class ImauError(RuntimeError):
@@ -85,6 +87,9 @@ def load_library(path: str = LIB_PATH):
     L.imau_update_general_ice_model_data.argtypes = [vp, P(ImauGrid), P(abi.ImauProducerCfg), P(abi.ImauProducerIO)]
     L.imau_calc_ice_rheology.argtypes = [vp, P(ImauGrid), P(abi.ImauProducerCfg), P(abi.ImauProducerIO),
                                          ctypes.c_double]
+    L.imau_run_ice_dynamics_direct.argtypes = [vp, P(ImauGrid), P(ImauCfg), P(abi.ImauThicknessCfg),
+                                               P(abi.ImauProducerCfg), P(abi.ImauTimestepCfg), P(ImauIceIO), vp, vp,
+                                               vp, vp, P(abi.ImauTimestepState), ctypes.c_double]
     _lib = L
     return L
 
@@ -163,6 +168,28 @@ class Context:
         self._check(self.L.imau_calc_ice_rheology(self.h, ctypes.byref(g), ctypes.byref(pcfg), ctypes.byref(io),
                                                   ctypes.c_double(time)))
         return ps
+
+    def run_ice_dynamics_direct(self, grid, cfg, tcfg, pcfg, scfg, st, SMB, BMB, tstate, t_end, geo=None):
+        """The 'direct' time loop (run_ice_dynamics_direct with
+        update_ice_thickness) from tstate.time to t_end on IceState st, whose
+        velocity outputs receive the last solve; returns the final thickness.
+        geo: a ProducerState for the final derived fields, or None."""
+        import numpy as np
+        g = grid.struct()
+        io = st.io()
+        shape = (grid.ny, grid.nx)
+        SMB = np.asfortranarray(np.broadcast_to(SMB, shape), dtype=np.float64)
+        BMB = np.asfortranarray(np.broadcast_to(BMB, shape), dtype=np.float64)
+        Hi = np.zeros(shape, order="F")
+        gio = geo.io(omit=("Hi_a", "Hb_a", "SL_a", "dHi_dt_a", "dHb_dt_a", "Ti_a")) if geo is not None else None
+        vp = ctypes.c_void_p
+        self._check(self.L.imau_run_ice_dynamics_direct(
+            self.h, ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(tcfg), ctypes.byref(pcfg), ctypes.byref(scfg),
+            ctypes.byref(io), vp(SMB.ctypes.data), vp(BMB.ctypes.data), vp(Hi.ctypes.data),
+            ctypes.cast(ctypes.pointer(gio), vp) if gio is not None else None, ctypes.byref(tstate),
+            ctypes.c_double(t_end)))
+        st.take_stats(io)
+        return Hi
 
     def critical_timestep_adv(self, grid, u, v, dt_max):
         """calc_critical_timestep_adv (src/ice_dynamics_module.f90:726-762)."""

@@ -442,6 +442,45 @@ int  imau_update_general_ice_model_data(imau_ctx *ctx, const imau_grid *grid, co
 int  imau_calc_ice_rheology(imau_ctx *ctx, const imau_grid *grid, const imau_producer_cfg *cfg,
                             imau_producer_io *io, double time);
 
+/* ---- time stepping: choice_timestepping = 'direct' ----
+ * The C% fields of the time loop (src/configuration_module.f90:50-55). */
+typedef struct imau_timestep_cfg {
+  double start_time_of_run, end_time_of_run;   /* [yr] */
+  double dt_min, dt_max, dt_startup_phase;     /* [yr] */
+} imau_timestep_cfg;
+
+/* The timers of type_model_region the loop reads and writes
+ * (src/data_types_module.f90: region%time, t_next_SIA, t_next_SSA, dt_crit_*, dt). */
+typedef struct imau_timestep_state {
+  double time, t_next_SIA, t_next_SSA;          /* in / out */
+  double dt_crit_SIA, dt_crit_SSA, dt;          /* out: the last values */
+  int32_t nsteps;                               /* out: time steps taken by the call */
+} imau_timestep_state;
+
+/* Replaces the ice part of the main time loop (src/IMAU_ICE_main_model.f90:90-214)
+ * with choice_timestepping = 'direct' and no other model component:
+ * while time < t_end, run_ice_dynamics_direct (src/ice_dynamics_module.f90:63-242:
+ * the SIA and SSA solves when their timers are due, each with its critical
+ * time step calc_critical_timestep_SIA :651-725 / _adv :726-762, the time
+ * step of determine_timesteps_and_actions :817-941 limited to the dynamics,
+ * then calc_dHi_dt), update_ice_thickness (:435-648 with choice_calving_law =
+ * 'none': H = max(0, H_t+dt), the masks, determine_floating_margin_fraction
+ * src/general_ice_model_data_module.f90:784-865, remove_unconnected_shelves
+ * src/ice_thickness_module.f90:932-1044, update_general_ice_model_data) and
+ * time += dt.  Everything stays on the device between steps.
+ * cfg->choice_ice_dynamics: 'none', 'SIA', 'SSA' or 'SIA/SSA' (DIVA needs
+ * the predictor-corrector loop, as in the reference).  io holds the state at
+ * st->time (geometry and its derived fields consistent, as after
+ * update_general_ice_model_data); its tolerances are the C%DIVA_* values the
+ * start-up phase scales.  SMB_year and BMB (ny,nx) are held fixed.  On
+ * return: io's velocity outputs, the final thickness in Hi_end (ny,nx) and
+ * the fields of geo that are not NULL; geo's input pointers are ignored. */
+int  imau_run_ice_dynamics_direct(imau_ctx *ctx, const imau_grid *grid, const imau_cfg *cfg,
+                                  const imau_thickness_cfg *tcfg, const imau_producer_cfg *pcfg,
+                                  const imau_timestep_cfg *scfg, imau_ice_io *io, const double *SMB_year,
+                                  const double *BMB, double *Hi_end, imau_producer_io *geo,
+                                  imau_timestep_state *st, double t_end);
+
 /* ---- ensembles (no reference counterpart: the reference solves one region
  *      grid per call, src/IMAU_ICE_program.f90:247-250) ----
  * Solves ngrids independent DIVA problems of possibly different sizes in one

@@ -2263,3 +2263,318 @@ double orc_critical_timestep_adv(const imau_grid *g, const double *u_cx, const d
     }
   return fmin(dt_max, dt_crit * 0.9);
 }
+
+/* working arrays of the producer fields (one grid) */
+typedef struct {
+  double *Hs, *TAF, *dHs_dt, *Hi_cx, *Hi_cy, *Hi_b, *A3cx, *A3cy, *Avcx, *Avcy;
+  double *dHi_dx_a, *dHi_dy_a, *dHs_dx_a, *dHs_dy_a, *dHs_dx_cx, *dHs_dy_cx, *dHs_dx_cy, *dHs_dy_cy;
+  int32_t *land, *ocean, *ice, *sheet, *shelf, *coast, *margin, *gl, *cf, *total;
+} ProducerBufs_;
+
+static void producer_bufs_alloc(ProducerBufs_ *b, size_t na, size_t ncx, size_t ncy, int nz)
+{
+  b->Hs = dalloc(na); b->TAF = dalloc(na); b->dHs_dt = dalloc(na);
+  b->Hi_cx = dalloc(ncx); b->Hi_cy = dalloc(ncy); b->Hi_b = dalloc(na);
+  b->A3cx = dalloc(ncx * nz); b->A3cy = dalloc(ncy * nz); b->Avcx = dalloc(ncx); b->Avcy = dalloc(ncy);
+  b->dHi_dx_a = dalloc(na); b->dHi_dy_a = dalloc(na); b->dHs_dx_a = dalloc(na); b->dHs_dy_a = dalloc(na);
+  b->dHs_dx_cx = dalloc(ncx); b->dHs_dy_cx = dalloc(ncx); b->dHs_dx_cy = dalloc(ncy); b->dHs_dy_cy = dalloc(ncy);
+  b->land = ialloc(na); b->ocean = ialloc(na); b->ice = ialloc(na); b->sheet = ialloc(na); b->shelf = ialloc(na);
+  b->coast = ialloc(na); b->margin = ialloc(na); b->gl = ialloc(na); b->cf = ialloc(na); b->total = ialloc(na);
+}
+static void producer_bufs_free(ProducerBufs_ *b)
+{
+  double *d[] = {b->Hs, b->TAF, b->dHs_dt, b->Hi_cx, b->Hi_cy, b->Hi_b, b->A3cx, b->A3cy, b->Avcx, b->Avcy,
+                 b->dHi_dx_a, b->dHi_dy_a, b->dHs_dx_a, b->dHs_dy_a, b->dHs_dx_cx, b->dHs_dy_cx, b->dHs_dx_cy,
+                 b->dHs_dy_cy};
+  int32_t *m[] = {b->land, b->ocean, b->ice, b->sheet, b->shelf, b->coast, b->margin, b->gl, b->cf, b->total};
+  for (size_t q = 0; q < sizeof d / sizeof d[0]; ++q) free(d[q]);
+  for (size_t q = 0; q < sizeof m / sizeof m[0]; ++q) free(m[q]);
+}
+static void producer_io_bind(imau_producer_io *o, const ProducerBufs_ *b, const double *Hi, const double *Hb,
+                             const double *SL, double *A3, double *Av)
+{
+  memset(o, 0, sizeof *o);
+  o->Hi_a = Hi; o->Hb_a = Hb; o->SL_a = SL;
+  o->Hs_a = b->Hs; o->TAF_a = b->TAF; o->dHs_dt_a = b->dHs_dt;
+  o->mask_land_a = b->land; o->mask_ocean_a = b->ocean; o->mask_ice_a = b->ice; o->mask_sheet_a = b->sheet;
+  o->mask_shelf_a = b->shelf; o->mask_coast_a = b->coast; o->mask_margin_a = b->margin; o->mask_gl_a = b->gl;
+  o->mask_cf_a = b->cf; o->mask_a = b->total;
+  o->Hi_cx = b->Hi_cx; o->Hi_cy = b->Hi_cy; o->Hi_b = b->Hi_b;
+  o->A_flow_3D_a = A3; o->A_flow_vav_a = Av; o->A_flow_3D_cx = b->A3cx; o->A_flow_3D_cy = b->A3cy;
+  o->A_flow_vav_cx = b->Avcx; o->A_flow_vav_cy = b->Avcy;
+  o->dHi_dx_a = b->dHi_dx_a; o->dHi_dy_a = b->dHi_dy_a; o->dHs_dx_a = b->dHs_dx_a; o->dHs_dy_a = b->dHs_dy_a;
+  o->dHs_dx_cx = b->dHs_dx_cx; o->dHs_dy_cx = b->dHs_dy_cx; o->dHs_dx_cy = b->dHs_dx_cy; o->dHs_dy_cy = b->dHs_dy_cy;
+}
+/* the fields of geo that are not NULL, from the working set */
+static void producer_io_copy_out(imau_producer_io *geo, const imau_producer_io *w, size_t na, size_t ncx, size_t ncy,
+                                 size_t nb, int nz)
+{
+  const size_t d = sizeof(double), q = sizeof(int32_t);
+#define CPO(f, n, s) if (geo->f && w->f) memcpy(geo->f, w->f, (n) * (s))
+  CPO(Hs_a, na, d); CPO(TAF_a, na, d); CPO(dHs_dt_a, na, d);
+  CPO(mask_land_a, na, q); CPO(mask_ocean_a, na, q); CPO(mask_ice_a, na, q); CPO(mask_sheet_a, na, q);
+  CPO(mask_shelf_a, na, q); CPO(mask_coast_a, na, q); CPO(mask_margin_a, na, q); CPO(mask_gl_a, na, q);
+  CPO(mask_cf_a, na, q); CPO(mask_a, na, q);
+  CPO(Hi_cx, ncx, d); CPO(Hi_cy, ncy, d); CPO(Hi_b, nb, d);
+  CPO(A_flow_3D_a, na * nz, d); CPO(A_flow_vav_a, na, d); CPO(A_flow_3D_cx, ncx * nz, d);
+  CPO(A_flow_3D_cy, ncy * nz, d); CPO(A_flow_vav_cx, ncx, d); CPO(A_flow_vav_cy, ncy, d);
+  CPO(dHi_dx_a, na, d); CPO(dHi_dy_a, na, d); CPO(dHs_dx_a, na, d); CPO(dHs_dy_a, na, d);
+  CPO(dHs_dx_cx, ncx, d); CPO(dHs_dy_cx, ncx, d); CPO(dHs_dx_cy, ncy, d); CPO(dHs_dy_cy, ncy, d);
+#undef CPO
+}
+
+/* ------------------------------------------------------------------------ */
+/* time stepping with choice_timestepping = 'direct':                       */
+/* run_ice_dynamics_direct (src/ice_dynamics_module.f90:63-242) and         */
+/* update_ice_thickness (:435-648, calving law 'none')                      */
+/* ------------------------------------------------------------------------ */
+/* calc_critical_timestep_SIA, :651-725 */
+static double crit_sia(const G *g, const double *dt_max, const double *u3sia, const double *v3sia, const double *u3,
+                       const double *v3, const double *Hi_cx, const double *Hi_cy, const double *dHs_dx_cx,
+                       const double *dHs_dy_cy, const int32_t *sheet, const double *zeta)
+{
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  double dtc = *dt_max;
+  double *col = dalloc(nz);
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      double D = 1e-9;
+      if (i < nx - 1 && sheet[A2(j, i)] == 1 && sheet[A2(j, i + 1)] == 1) {
+        for (int k = 0; k < nz; ++k) col[k] = u3sia[A3I(k, j, i)];
+        const double u = orc_vertical_average(nz, zeta, col);
+        D = fmax(D, fabs(u * Hi_cx[A2(j, i)] / dHs_dx_cx[A2(j, i)]));
+      }
+      if (j < ny - 1 && sheet[A2(j, i)] == 1 && sheet[A2(j + 1, i)] == 1) {
+        for (int k = 0; k < nz; ++k) col[k] = v3sia[CY3(k, j, i)];
+        const double v = orc_vertical_average(nz, zeta, col);
+        D = fmax(D, fabs(v * Hi_cy[CY(j, i)] / dHs_dy_cy[CY(j, i)]));
+      }
+      dtc = fmin(dtc, g->dx * g->dx / (6.0 * D));
+      for (int k = 0; k < nz; ++k) {
+        double ua, va;
+        if (i == 0) ua = fabs(u3[A3I(k, j, 0)]);
+        else if (i == nx - 1) ua = fabs(u3[A3I(k, j, nx - 2)]);
+        else ua = fmax(fabs(u3[A3I(k, j, i - 1)]), fabs(u3[A3I(k, j, i)]));
+        if (j == 0) va = fabs(v3[CY3(k, 0, i)]);
+        else if (j == ny - 1) va = fabs(v3[CY3(k, ny - 2, i)]);
+        else va = fmax(fabs(v3[CY3(k, j - 1, i)]), fabs(v3[CY3(k, j, i)]));
+        dtc = fmin(dtc, g->dx / (ua + va));
+      }
+    }
+  free(col);
+  return dtc * 0.9;
+}
+
+/* determine_floating_margin_fraction, src/general_ice_model_data_module.f90:784-865 */
+static void floating_margin_fraction(const G *g, const double *Hi, const int32_t *ice, const int32_t *cf,
+                                     const int32_t *shelf, double *fmf)
+{
+  const int nx = g->nx, ny = g->ny;
+  for (int i = 0; i < nx; ++i)
+    for (int j = 1; j < ny - 1; ++j) {
+      const size_t p = A2(j, i);
+      fmf[p] = ice[p] == 1 ? 1.0 : 0.0;
+      if (!(cf[p] == 1 && shelf[p] == 1)) continue;
+      int noncf = 0;
+      double hmax = 0.0;
+      for (int ii = (i > 0 ? i - 1 : 0); ii <= (i < nx - 1 ? i + 1 : nx - 1); ++ii)
+        for (int jj = j - 1; jj <= j + 1; ++jj)
+          if (ice[A2(jj, ii)] == 1 && cf[A2(jj, ii)] == 0) {
+            noncf = 1;
+            hmax = fmax(hmax, Hi[A2(jj, ii)]);
+          }
+      if (!noncf) { fmf[p] = 1.0; continue; }
+      if (hmax < Hi[p]) { fmf[p] = 1.0; continue; }
+      fmf[p] = Hi[p] / hmax;
+    }
+}
+
+/* remove_unconnected_shelves, src/ice_thickness_module.f90:932-1044: a
+ * depth-first fill through the 4 neighbours from the shelf cells next to the
+ * sheet; shelf cells it does not reach lose their ice */
+static void remove_unconnected_shelves(const G *g, double *Hi, const int32_t *shelf, const int32_t *sheet)
+{
+  const int nx = g->nx, ny = g->ny;
+  int32_t *map = ialloc((size_t)nx * ny);
+  int32_t *stack = ialloc(2 * (size_t)nx * ny + 8);
+  long n = 0;
+  for (int i = 1; i < nx - 1; ++i)
+    for (int j = 1; j < ny - 1; ++j)
+      if (shelf[A2(j, i)] == 1 && (sheet[A2(j, i + 1)] == 1 || sheet[A2(j, i - 1)] == 1 || sheet[A2(j + 1, i)] == 1 ||
+                                   sheet[A2(j - 1, i)] == 1)) {
+        stack[2 * n] = i; stack[2 * n + 1] = j; ++n;
+        map[A2(j, i)] = 1;
+      }
+  while (n > 0) {
+    --n;
+    const int i = stack[2 * n], j = stack[2 * n + 1];
+    if (shelf[A2(j, i)] != 1) continue;
+    map[A2(j, i)] = 2;
+    const int ni[4] = {i - 1, i + 1, i, i}, nj[4] = {j, j, j - 1, j + 1};
+    for (int q = 0; q < 4; ++q) {
+      if (ni[q] < 0 || ni[q] >= nx || nj[q] < 0 || nj[q] >= ny) continue;
+      if (map[A2(nj[q], ni[q])] == 0) {
+        map[A2(nj[q], ni[q])] = 1;
+        stack[2 * n] = ni[q]; stack[2 * n + 1] = nj[q]; ++n;
+      }
+    }
+  }
+  for (size_t p = 0; p < (size_t)nx * ny; ++p)
+    if (shelf[p] == 1 && map[p] == 0) Hi[p] = 0.0;
+  free(map);
+  free(stack);
+}
+
+int orc_run_ice_dynamics_direct(const imau_grid *gr, const imau_cfg *c, const imau_thickness_cfg *tc,
+                                const imau_producer_cfg *pc, const imau_timestep_cfg *sc, imau_ice_io *io,
+                                const double *SMB, const double *BMB, double *Hi_end, imau_producer_io *geo,
+                                imau_timestep_state *st, double t_end)
+{
+  const int dyn = c->choice_ice_dynamics;
+  if (dyn != IMAU_DYN_NONE && dyn != IMAU_DYN_SIA && dyn != IMAU_DYN_SSA && dyn != IMAU_DYN_SIA_SSA)
+    return fail(-2, "direct time stepping works with 'none', 'SIA', 'SSA' or 'SIA/SSA' only");
+  if (!(sc->dt_startup_phase > 0.0) || !(sc->dt_min > 0.0) || sc->dt_max < sc->dt_min)
+    return fail(-1, "need dt_startup_phase > 0 and 0 < dt_min <= dt_max");
+  G gg = {gr->nx, gr->ny, c->nz, gr->dx, gr->x, gr->y, c};
+  const G *g = &gg;
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  const size_t na = (size_t)nx * ny, ncx = (size_t)ny * (nx - 1), ncy = (size_t)(ny - 1) * nx;
+  /* working geometry and derived fields */
+  double *Hi = dalloc(na), *fmf = dalloc(na), *dHi = dalloc(na), *Ht = dalloc(na), *zero = dalloc(na);
+  double *A3 = dalloc(na * nz), *Av = dalloc(na);
+  memcpy(Hi, io->Hi_a, na * sizeof(double));
+  memcpy(A3, io->A_flow_3D_a, na * nz * sizeof(double));
+  memcpy(Av, io->A_flow_vav_a, na * sizeof(double));
+  ProducerBufs_ PB;
+  producer_bufs_alloc(&PB, na, ncx, ncy, nz);
+  imau_producer_io pio;
+  producer_io_bind(&pio, &PB, Hi, io->Hb_a, io->SL_a, A3, Av);
+  /* velocity outputs the loop reads, where the caller did not ask for them */
+  double *u_vav = io->u_vav_cx ? io->u_vav_cx : dalloc(ncx), *v_vav = io->v_vav_cy ? io->v_vav_cy : dalloc(ncy);
+  double *u3 = io->u_3D_cx ? io->u_3D_cx : dalloc(ncx * nz), *v3 = io->v_3D_cy ? io->v_3D_cy : dalloc(ncy * nz);
+  double *u3s = io->u_3D_SIA_cx ? io->u_3D_SIA_cx : dalloc(ncx * nz);
+  double *v3s = io->v_3D_SIA_cy ? io->v_3D_SIA_cy : dalloc(ncy * nz);
+  double *uS = io->u_SSA_cx ? io->u_SSA_cx : dalloc(ncx), *vS = io->v_SSA_cy ? io->v_SSA_cy : dalloc(ncy);
+  imau_ice_io w = *io;
+  w.Hi_a = Hi; w.Hs_a = PB.Hs; w.TAF_a = PB.TAF;
+  w.dHs_dx_a = PB.dHs_dx_a; w.dHs_dy_a = PB.dHs_dy_a; w.dHi_dx_a = PB.dHi_dx_a; w.dHi_dy_a = PB.dHi_dy_a;
+  w.A_flow_vav_a = Av;
+  w.mask_land_a = PB.land; w.mask_ocean_a = PB.ocean; w.mask_ice_a = PB.ice; w.mask_sheet_a = PB.sheet;
+  w.mask_shelf_a = PB.shelf;
+  w.Hi_cx = PB.Hi_cx; w.dHs_dx_cx = PB.dHs_dx_cx; w.dHs_dy_cx = PB.dHs_dy_cx;
+  w.Hi_cy = PB.Hi_cy; w.dHs_dx_cy = PB.dHs_dx_cy; w.dHs_dy_cy = PB.dHs_dy_cy; w.Hi_b = PB.Hi_b;
+  w.A_flow_3D_a = A3; w.A_flow_3D_cx = PB.A3cx; w.A_flow_3D_cy = PB.A3cy;
+  w.u_vav_cx = u_vav; w.v_vav_cy = v_vav; w.u_3D_cx = u3; w.v_3D_cy = v3; w.u_3D_SIA_cx = u3s; w.v_3D_SIA_cy = v3s;
+  w.u_SSA_cx = uS; w.v_SSA_cy = vS;
+  /* the derived fields of the state at entry, and its floating-margin fraction */
+  pio.dHi_dt_a = NULL;
+  orc_producers_update(gr, pc, &pio);
+  floating_margin_fraction(g, Hi, PB.ice, PB.cf, PB.shelf, fmf);
+  const int32_t nit0 = io->DIVA_SOR_nit;
+  const double tol0 = io->DIVA_SOR_tol, rtol0 = io->DIVA_PETSc_rtol, atol0 = io->DIVA_PETSc_abstol;
+  double time = st->time;
+  int rc = 0;
+  st->nsteps = 0;
+  while (time < t_end) {
+    /* :88-99: tighter velocity solves in the start-up phase; the reference's
+     * r = 0 at the very start would ask for unbounded iterations, so r is
+     * floored at 1e-3 */
+    double r = 1.0;
+    if (time <= sc->start_time_of_run + sc->dt_startup_phase)
+      r = fmax(1e-3, 0.01 * 0.99 * (time - sc->start_time_of_run) / sc->dt_startup_phase);
+    w.DIVA_SOR_nit = nit0 * (int32_t)ceil(1.0 / r);
+    w.DIVA_SOR_tol = tol0 * r;
+    w.DIVA_PETSc_rtol = rtol0 * r;
+    w.DIVA_PETSc_abstol = atol0 * r;
+    /* :101-108 */
+    double dt_max;
+    if (time <= sc->start_time_of_run + sc->dt_startup_phase)
+      dt_max = sc->dt_min + (sc->dt_max - sc->dt_min) * pow((time - sc->start_time_of_run) / sc->dt_startup_phase, 2);
+    else if (time >= sc->end_time_of_run - sc->dt_startup_phase)
+      dt_max = sc->dt_min + (sc->dt_max - sc->dt_min) * pow((sc->end_time_of_run - time) / sc->dt_startup_phase, 2);
+    else
+      dt_max = sc->dt_max;
+    if ((dyn == IMAU_DYN_SIA || dyn == IMAU_DYN_SIA_SSA) && time == st->t_next_SIA) {
+      if ((rc = orc_solve_sia(gr, c, &w))) break;
+      double d = crit_sia(g, &sc->dt_max, u3s, v3s, u3, v3, PB.Hi_cx, PB.Hi_cy, PB.dHs_dx_cx, PB.dHs_dy_cy, PB.sheet,
+                          c->zeta);
+      d = fmax(sc->dt_min, fmin(dt_max, d));
+      st->dt_crit_SIA = d;
+      st->t_next_SIA = time + d;
+    }
+    if ((dyn == IMAU_DYN_SSA || dyn == IMAU_DYN_SIA_SSA) && time == st->t_next_SSA) {
+      if ((rc = orc_solve_ssa(gr, c, &w, ORC_LIN_BANDED_LU))) break;
+      double d = orc_critical_timestep_adv(gr, u_vav, v_vav, sc->dt_max);
+      d = fmax(sc->dt_min, fmin(dt_max, d));
+      st->dt_crit_SSA = d;
+      st->t_next_SSA = time + d;
+    }
+    /* determine_timesteps_and_actions, :838-856 and :933 */
+    double t_next = fmin(t_end, time + sc->dt_max);
+    if (dyn == IMAU_DYN_SIA || dyn == IMAU_DYN_SIA_SSA) t_next = fmin(t_next, st->t_next_SIA);
+    if (dyn == IMAU_DYN_SSA || dyn == IMAU_DYN_SIA_SSA) t_next = fmin(t_next, st->t_next_SSA);
+    const double dt = t_next - time;
+    st->dt = dt;
+    /* :224-236 */
+    if (dyn == IMAU_DYN_NONE) {
+      for (size_t p = 0; p < na; ++p) { dHi[p] = 0.0; Ht[p] = Hi[p]; }
+    } else {
+      imau_thickness_io t;
+      memset(&t, 0, sizeof t);
+      t.Hi_a = Hi; t.Hb_a = io->Hb_a; t.SL_a = io->SL_a; t.u_vav_cx = u_vav; t.v_vav_cy = v_vav;
+      t.SMB_year = SMB; t.BMB = BMB; t.float_margin_frac_a = fmf;
+      t.mask_ice_a = PB.ice; t.mask_shelf_a = PB.shelf; t.mask_ocean_a = PB.ocean; t.mask_cf_a = PB.cf;
+      t.dHi_dt_a = dHi; t.Hi_tplusdt_a = Ht;
+      if ((rc = orc_thickness_update(gr, tc, &t, dt))) break;
+      for (size_t p = 0; p < na; ++p) Ht[p] = Hi[p] + dt * dHi[p];
+    }
+    /* update_ice_thickness, :457-645 (calving law 'none', no optional removals) */
+    for (size_t p = 0; p < na; ++p) Hi[p] = fmax(0.0, Ht[p]);
+    pio.dHi_dt_a = NULL;
+    orc_producers_update(gr, pc, &pio);                         /* determine_masks_ice / _transitions */
+    floating_margin_fraction(g, Hi, PB.ice, PB.cf, PB.shelf, fmf);
+    remove_unconnected_shelves(g, Hi, PB.shelf, PB.sheet);
+    pio.dHi_dt_a = dHi;                                          /* update_general_ice_model_data */
+    pio.dHb_dt_a = io->dHb_dt_a ? io->dHb_dt_a : zero;
+    orc_producers_update(gr, pc, &pio);
+    time += dt;
+    ++st->nsteps;
+  }
+  st->time = time;
+  if (!rc) {
+    memcpy(Hi_end, Hi, na * sizeof(double));
+    if (geo) producer_io_copy_out(geo, &pio, na, ncx, ncy, (size_t)(ny - 1) * (nx - 1), nz);
+  }
+  if (!io->u_vav_cx) free(u_vav);
+  if (!io->v_vav_cy) free(v_vav);
+  if (!io->u_3D_cx) free(u3);
+  if (!io->v_3D_cy) free(v3);
+  if (!io->u_3D_SIA_cx) free(u3s);
+  if (!io->v_3D_SIA_cy) free(v3s);
+  if (!io->u_SSA_cx) free(uS);
+  if (!io->v_SSA_cy) free(vS);
+  producer_bufs_free(&PB);
+  free(Hi); free(fmf); free(dHi); free(Ht); free(zero); free(A3); free(Av);
+  return rc;
+}
+
+/* building blocks of the loop, exposed for unit tests */
+void orc_floating_margin_fraction(const imau_grid *gr, const double *Hi, const int32_t *ice, const int32_t *cf,
+                                  const int32_t *shelf, double *fmf)
+{
+  G gg = {gr->nx, gr->ny, 1, gr->dx, gr->x, gr->y, NULL};
+  floating_margin_fraction(&gg, Hi, ice, cf, shelf, fmf);
+}
+void orc_remove_unconnected_shelves(const imau_grid *gr, double *Hi, const int32_t *shelf, const int32_t *sheet)
+{
+  G gg = {gr->nx, gr->ny, 1, gr->dx, gr->x, gr->y, NULL};
+  remove_unconnected_shelves(&gg, Hi, shelf, sheet);
+}
+double orc_critical_timestep_sia(const imau_grid *gr, int nz, const double *zeta, double dt_max, const double *u3sia,
+                                 const double *v3sia, const double *u3, const double *v3, const double *Hi_cx,
+                                 const double *Hi_cy, const double *dHs_dx_cx, const double *dHs_dy_cy,
+                                 const int32_t *sheet)
+{
+  G gg = {gr->nx, gr->ny, nz, gr->dx, gr->x, gr->y, NULL};
+  return crit_sia(&gg, &dt_max, u3sia, v3sia, u3, v3, Hi_cx, Hi_cy, dHs_dx_cx, dHs_dy_cy, sheet, zeta);
+}

@@ -74,6 +74,19 @@ int    orc_banded_lu_solve(int m, const int32_t *ptr, const int32_t *idx, const 
  * fill) and calc_ice_rheology, with the library's producer structs */
 int    orc_producers_update(const imau_grid *g, const imau_producer_cfg *c, imau_producer_io *io);
 int    orc_calc_ice_rheology(const imau_grid *g, const imau_producer_cfg *c, imau_producer_io *io, double time);
+/* the 'direct' time loop with update_ice_thickness, same arguments as
+ * imau_run_ice_dynamics_direct (include/imau_vel.h) without the context */
+int    orc_run_ice_dynamics_direct(const imau_grid *g, const imau_cfg *c, const imau_thickness_cfg *tc,
+                                   const imau_producer_cfg *pc, const imau_timestep_cfg *sc, imau_ice_io *io,
+                                   const double *SMB, const double *BMB, double *Hi_end, imau_producer_io *geo,
+                                   imau_timestep_state *st, double t_end);
+void   orc_floating_margin_fraction(const imau_grid *g, const double *Hi, const int32_t *ice, const int32_t *cf,
+                                    const int32_t *shelf, double *fmf);
+void   orc_remove_unconnected_shelves(const imau_grid *g, double *Hi, const int32_t *shelf, const int32_t *sheet);
+double orc_critical_timestep_sia(const imau_grid *g, int nz, const double *zeta, double dt_max, const double *u3sia,
+                                 const double *v3sia, const double *u3, const double *v3, const double *Hi_cx,
+                                 const double *Hi_cy, const double *dHs_dx_cx, const double *dHs_dy_cy,
+                                 const int32_t *sheet);
 int    orc_thickness_update(const imau_grid *g, const imau_thickness_cfg *c, imau_thickness_io *io, double dt);
 double orc_critical_timestep_adv(const imau_grid *g, const double *u_cx, const double *v_cy, double dt_max);
 /* Schoof (2006) ice-stream closed form, src/utilities_module.f90:836-877 */

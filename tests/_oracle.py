@@ -157,3 +157,57 @@ def calc_ice_rheology(grid, pcfg, ps, time=0.0, omit=()):
     if rc != 0:
         raise RuntimeError("oracle rheology failed (%d): %s" % (rc, L.orc_last_error().decode()))
     return ps
+
+
+def run_ice_dynamics_direct(grid, cfg, tcfg, pcfg, scfg, st, SMB, BMB, tstate, t_end, geo=None):
+    """The oracle's 'direct' time loop on IceState st (velocity outputs written
+    into st); returns the final thickness.  geo: a ProducerState for the final
+    derived fields, or None."""
+    L = lib()
+    g = grid.struct()
+    io = st.io()
+    SMB = np.asfortranarray(np.broadcast_to(SMB, (grid.ny, grid.nx)), dtype=np.float64)
+    BMB = np.asfortranarray(np.broadcast_to(BMB, (grid.ny, grid.nx)), dtype=np.float64)
+    Hi = np.zeros((grid.ny, grid.nx), order="F")
+    gio = geo.io(omit=("Hi_a", "Hb_a", "SL_a", "dHi_dt_a", "dHb_dt_a", "Ti_a")) if geo is not None else None
+    rc = L.orc_run_ice_dynamics_direct(ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(tcfg), ctypes.byref(pcfg),
+                                       ctypes.byref(scfg), ctypes.byref(io), _ptr(SMB), _ptr(BMB), _ptr(Hi),
+                                       ctypes.byref(gio) if gio is not None else None, ctypes.byref(tstate),
+                                       ctypes.c_double(t_end))
+    if rc != 0:
+        raise RuntimeError("oracle time loop failed (%d): %s" % (rc, L.orc_last_error().decode()))
+    st.take_stats(io)
+    return Hi
+
+
+def floating_margin_fraction(grid, Hi, ice, cf, shelf, fmf):
+    L = lib()
+    g = grid.struct()
+    i32 = lambda a: np.asfortranarray(a, dtype=np.int32)
+    ice, cf, shelf = i32(ice), i32(cf), i32(shelf)
+    p32 = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    Hi = np.asfortranarray(Hi, dtype=np.float64)
+    L.orc_floating_margin_fraction(ctypes.byref(g), _ptr(Hi), p32(ice), p32(cf), p32(shelf), _ptr(fmf))
+    return fmf
+
+
+def remove_unconnected_shelves(grid, Hi, shelf, sheet):
+    L = lib()
+    g = grid.struct()
+    i32 = lambda a: np.asfortranarray(a, dtype=np.int32)
+    shelf, sheet = i32(shelf), i32(sheet)
+    p32 = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    L.orc_remove_unconnected_shelves(ctypes.byref(g), _ptr(Hi), p32(shelf), p32(sheet))
+    return Hi
+
+
+def critical_timestep_sia(grid, zeta, dt_max, st):
+    L = lib()
+    L.orc_critical_timestep_sia.restype = ctypes.c_double
+    g = grid.struct()
+    z = np.ascontiguousarray(zeta, dtype=np.float64)
+    p32 = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    return L.orc_critical_timestep_sia(ctypes.byref(g), len(z), _ptr(z), ctypes.c_double(dt_max),
+                                       _ptr(st["u_3D_SIA_cx"]), _ptr(st["v_3D_SIA_cy"]), _ptr(st["u_3D_cx"]),
+                                       _ptr(st["v_3D_cy"]), _ptr(st["Hi_cx"]), _ptr(st["Hi_cy"]),
+                                       _ptr(st["dHs_dx_cx"]), _ptr(st["dHs_dy_cy"]), p32(st["mask_sheet_a"]))

@@ -54,6 +54,15 @@ def test_library_exports_every_producer_entry_point():
         assert hasattr(L, name), name
 
 
+def test_library_exports_the_time_loop():
+    L = solver.load_library()
+    txt = open(abi.HEADER).read()
+    declared = set(re.findall(r"\b(imau_run_\w+)\s*\(", txt))
+    assert declared == set(solver.TIMELOOP_EXPORTED)
+    for name in sorted(declared):
+        assert hasattr(L, name), name
+
+
 def test_producer_defaults_match_the_python_mirror():
     from imau_vel import config
     L = solver.load_library()
