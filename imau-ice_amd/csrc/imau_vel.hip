@@ -2662,17 +2662,25 @@ int imau_calc_ice_rheology(imau_ctx *ctx, const imau_grid *grid, const imau_prod
 // ---------------------------------------------------------------------------
 namespace {
 
-void check_timestep_cfg(const imau_timestep_cfg *c) {
+void check_timestep_cfg(const imau_timestep_cfg *c, const imau_refgeo_io *ref) {
   if (!(c->dt_startup_phase > 0.0)) fail(IMAU_ERR_CONFIG, "dt_startup_phase must be > 0");
   if (!(c->dt_min > 0.0) || c->dt_max < c->dt_min) fail(IMAU_ERR_CONFIG, "need 0 < dt_min <= dt_max");
+  if (c->choice_calving_law != IMAU_CALVING_NONE && c->choice_calving_law != IMAU_CALVING_THRESHOLD_THICKNESS)
+    fail(IMAU_ERR_CONFIG, "unknown choice_calving_law %d", c->choice_calving_law);
+  if (c->remove_shelves_larger_than_PD && !(ref && ref->Hi_PD && ref->Hb_PD))
+    fail(IMAU_ERR_ARG, "remove_shelves_larger_than_PD needs ref->Hi_PD and ref->Hb_PD");
+  if (c->continental_shelf_calving && !(ref && ref->Hi_GIAeq && ref->Hb_GIAeq))
+    fail(IMAU_ERR_ARG, "continental_shelf_calving needs ref->Hi_GIAeq and ref->Hb_GIAeq");
 }
 
 struct LoopDev {
-  DBuf<double> f;       // SMB, BMB, float margin fraction, dHs/dt, A_vav on cx / cy, zero
+  DBuf<double> f;       // SMB, BMB, float margin fraction, dHs/dt, A_vav on cx / cy, zero, Hi_eff_cf
   DBuf<int32_t> m;      // coast, margin, grounding line, calving front, total, shelf keep
   DBuf<int> flag;
   DBuf<double> red;     // reduction partials and the result
-  double *SMB, *BMB, *fmf, *dHs_dt, *Avcx, *Avcy, *zero;
+  DBuf<double> rg;      // reference geometries of the removal options
+  DBuf<int32_t> noice;
+  double *SMB, *BMB, *fmf, *dHs_dt, *Avcx, *Avcy, *zero, *heff;
   int32_t *keep;
   int nb = 0;
 };
@@ -2683,9 +2691,9 @@ extern "C" {
 
 int imau_run_ice_dynamics_direct(imau_ctx *ctx, const imau_grid *grid, const imau_cfg *cfg,
                                  const imau_thickness_cfg *tcfg, const imau_producer_cfg *pcfg,
-                                 const imau_timestep_cfg *scfg, imau_ice_io *io, const double *SMB_year,
-                                 const double *BMB, double *Hi_end, imau_producer_io *geo, imau_timestep_state *st,
-                                 double t_end) {
+                                 const imau_timestep_cfg *scfg, imau_ice_io *io, const imau_refgeo_io *ref,
+                                 const double *SMB_year, const double *BMB, double *Hi_end, imau_producer_io *geo,
+                                 imau_timestep_state *st, double t_end) {
   ABI_TRY
   if (!ctx || !grid || !cfg || !tcfg || !pcfg || !scfg || !io || !SMB_year || !BMB || !Hi_end || !st)
     fail(IMAU_ERR_ARG, "NULL argument");
@@ -2695,7 +2703,7 @@ int imau_run_ice_dynamics_direct(imau_ctx *ctx, const imau_grid *grid, const ima
   check_cfg(cfg);
   check_thickness_cfg(tcfg);
   check_producer_cfg(pcfg);
-  check_timestep_cfg(scfg);
+  check_timestep_cfg(scfg, ref);
   if (pcfg->nz != cfg->nz) fail(IMAU_ERR_CONFIG, "producer nz=%d differs from the solver's nz=%d", pcfg->nz, cfg->nz);
   imau_cfg vcfg = *cfg;
   if (dyn == IMAU_DYN_NONE) vcfg.choice_ice_dynamics = IMAU_DYN_SIA;  // the problem only carries the geometry
@@ -2712,7 +2720,7 @@ int imau_run_ice_dynamics_direct(imau_ctx *ctx, const imau_grid *grid, const ima
   const long long na = (long long)nx * ny;
   const size_t d8 = sizeof(double), i4 = sizeof(int32_t);
   LoopDev X;
-  X.f.alloc((size_t)7 * na);
+  X.f.alloc((size_t)8 * na);
   X.m.alloc((size_t)6 * na);
   X.flag.alloc(1);
   X.nb = imau::thk_blocks(mb);
@@ -2720,10 +2728,37 @@ int imau_run_ice_dynamics_direct(imau_ctx *ctx, const imau_grid *grid, const ima
   {
     double *q = X.f.p;
     X.SMB = q; X.BMB = q + na; X.fmf = q + 2 * na; X.dHs_dt = q + 3 * na; X.Avcx = q + 4 * na; X.Avcy = q + 5 * na;
-    X.zero = q + 6 * na;
+    X.zero = q + 6 * na; X.heff = q + 7 * na;
   }
   HIPCHK(hipMemcpyAsync(X.SMB, SMB_year, na * d8, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(X.BMB, BMB, na * d8, hipMemcpyHostToDevice, s));
+  // the removal rules of update_ice_thickness and their fixed inputs
+  imau::Removal rem{};
+  rem.shelves = scfg->do_remove_shelves ? 1 : 0;
+  rem.min_height = scfg->continental_shelf_min_height;
+  if (scfg->remove_shelves_larger_than_PD || scfg->continental_shelf_calving) {
+    X.rg.alloc((size_t)4 * na);
+    if (scfg->remove_shelves_larger_than_PD) {
+      rem.Hi_PD = X.rg.p;
+      rem.Hb_PD = X.rg.p + na;
+      HIPCHK(hipMemcpyAsync(X.rg.p, ref->Hi_PD, na * d8, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(X.rg.p + na, ref->Hb_PD, na * d8, hipMemcpyHostToDevice, s));
+    }
+    if (scfg->continental_shelf_calving) {
+      rem.Hi_GIA = X.rg.p + 2 * na;
+      rem.Hb_GIA = X.rg.p + 3 * na;
+      HIPCHK(hipMemcpyAsync(X.rg.p + 2 * na, ref->Hi_GIAeq, na * d8, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(X.rg.p + 3 * na, ref->Hb_GIAeq, na * d8, hipMemcpyHostToDevice, s));
+    }
+  }
+  if (ref && ref->mask_noice_a) {
+    X.noice.alloc((size_t)na);
+    rem.noice = X.noice.p;
+    HIPCHK(hipMemcpyAsync(X.noice.p, ref->mask_noice_a, na * i4, hipMemcpyHostToDevice, s));
+  }
+  const bool removal = rem.noice || rem.shelves || rem.Hi_PD || rem.Hi_GIA;
+  const int fixed = (scfg->fixed_shelf_geometry ? 1 : 0) | (scfg->fixed_sheet_geometry ? 2 : 0) |
+                    (scfg->fixed_grounding_line ? 4 : 0);
   // the problem's fields are the geometry the producers update
   double *Hi = f2p(P, F_Hi);
   const imau::ProdMasks mk{i2p(P, I_land), i2p(P, I_ocean), i2p(P, I_ice), i2p(P, I_sheet), i2p(P, I_shelf),
@@ -2768,7 +2803,7 @@ int imau_run_ice_dynamics_direct(imau_ctx *ctx, const imau_grid *grid, const ima
   };
   // the derived fields of the state at entry, and its floating-margin fraction
   producers(nullptr);
-  imau::prod_float_margin(s, mb, Hi, mk, X.fmf);
+  imau::prod_float_margin(s, mb, Hi, mk, X.fmf, X.heff);
   imau::Zeta zeta{};
   for (int k = 0; k < nz; ++k) zeta.z[k] = cfg->zeta[k];
   const int32_t nit0 = P->sor_nit_;
@@ -2828,13 +2863,22 @@ int imau_run_ice_dynamics_direct(imau_ctx *ctx, const imau_grid *grid, const ima
       thickness_compute(ctx, mb, T, tcfg, dt);
       imau::thk_advance(s, mb, Hi, T.dHi, dt, T.Ht);
     }
-    // update_ice_thickness, :457-645 (calving law 'none', no optional removals)
+    // update_ice_thickness, :457-645
+    if (fixed) imau::prod_fix_geometry(s, mb, fixed, Hi, mk, T.Ht);
     imau::thk_clip(s, mb, T.Ht, Hi);
     producers(nullptr);                                   // determine_masks_ice / _transitions
-    imau::prod_float_margin(s, mb, Hi, mk, X.fmf);
+    imau::prod_float_margin(s, mb, Hi, mk, X.fmf, X.heff);
+    if (scfg->choice_calving_law == IMAU_CALVING_THRESHOLD_THICKNESS) {  // :574-588, twice
+      imau::prod_calve(s, mb, mk, X.heff, scfg->calving_threshold_thickness, Hi);
+      producers(nullptr);
+      imau::prod_float_margin(s, mb, Hi, mk, X.fmf, X.heff);
+      imau::prod_calve(s, mb, mk, X.heff, scfg->calving_threshold_thickness, Hi);
+      producers(nullptr);
+    }
     imau::prod_shelf_seed(s, mb, mk, X.keep);             // remove_unconnected_shelves
     flood([&]() { imau::prod_shelf_round(s, mb, mk, X.keep, X.flag.p); });
     imau::prod_shelf_remove(s, mb, mk, X.keep, Hi);
+    if (removal) imau::prod_remove_ice(s, mb, rem, f2p(P, F_Hb), f2p(P, F_SL), Hi);
     producers(T.dHi);                                     // update_general_ice_model_data
     HIPCHK(hipGetLastError());
     time += dt;

@@ -42,8 +42,24 @@ void prod_masks_maps(hipStream_t s, const Member &m, const ProdIn &in, const Pro
 // (null: skipped); nz levels with the given strides
 void prod_map_A(hipStream_t s, const Member &m, int nz, const double *A3, Str3 sa, double *A3cx, Str3 sx, double *A3cy,
                 Str3 sy, const double *Av, double *Avcx, double *Avcy);
-// determine_floating_margin_fraction (reads mk.ice, mk.cf, mk.shelf)
-void prod_float_margin(hipStream_t s, const Member &m, const double *Hi, const ProdMasks &mk, double *fmf);
+// determine_floating_margin_fraction (reads mk.ice, mk.cf, mk.shelf); heff
+// (may be null) receives the effective calving-front thickness Hi_eff_cf_a
+void prod_float_margin(hipStream_t s, const Member &m, const double *Hi, const ProdMasks &mk, double *fmf,
+                       double *heff);
+// update_ice_thickness's fixed-geometry switches on H(t+dt): which = bit 0
+// fixed_shelf_geometry, bit 1 fixed_sheet_geometry, bit 2 fixed_grounding_line
+void prod_fix_geometry(hipStream_t s, const Member &m, int which, const double *Hi, const ProdMasks &mk, double *Ht);
+// threshold_thickness_calving: calving-front cells with heff < thr lose their ice
+void prod_calve(hipStream_t s, const Member &m, const ProdMasks &mk, const double *heff, double thr, double *Hi);
+// the removals after the calving; a null field switches its rule off
+struct Removal {
+  const int32_t *noice;                 // mask_noice
+  int shelves;                          // do_remove_shelves
+  const double *Hi_PD, *Hb_PD;          // remove_shelves_larger_than_PD
+  const double *Hi_GIA, *Hb_GIA;        // continental_shelf_calving
+  double min_height;                    // continental_shelf_min_height
+};
+void prod_remove_ice(hipStream_t s, const Member &m, const Removal &r, const double *Hb, const double *SL, double *Hi);
 // remove_unconnected_shelves: seeds into keep, rounds of growth through the
 // 4 neighbours of shelf cells (*changed set when a cell changed), removal
 void prod_shelf_seed(hipStream_t s, const Member &m, const ProdMasks &mk, int32_t *keep);

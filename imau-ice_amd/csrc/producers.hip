@@ -285,15 +285,17 @@ __global__ void __launch_bounds__(TB) k_prod_rheology(const Member m, const imau
 }
 
 // determine_floating_margin_fraction (src/general_ice_model_data_module.f90:803-858):
-// rows 1 .. ny-2 only, as the reference
+// rows 1 .. ny-2 only, as the reference; heff (may be null) = Hi_eff_cf_a
 __global__ void __launch_bounds__(TB) k_float_margin(const Member m, const double *__restrict__ Hi,
-                                                    const imau::ProdMasks mk, double *__restrict__ fmf) {
+                                                    const imau::ProdMasks mk, double *__restrict__ fmf,
+                                                    double *__restrict__ heff) {
   Cell c;
   if (!cell_of(m, blockIdx.x * TB + threadIdx.x, c)) return;
   const int nx = m.nx, ny = m.ny, i = c.i, j = c.j;
   if (j < 1 || j > ny - 2) return;
   const long long p = c.p;
-  double f = mk.ice[p] == 1 ? 1.0 : 0.0;
+  const bool ice = mk.ice[p] == 1;
+  double f = ice ? 1.0 : 0.0, h = ice ? Hi[p] : 0.0;
   if (mk.cf[p] == 1 && mk.shelf[p] == 1) {
     bool noncf = false;
     double hmax = 0.0;
@@ -305,9 +307,56 @@ __global__ void __launch_bounds__(TB) k_float_margin(const Member m, const doubl
           hmax = fmax(hmax, Hi[q]);
         }
       }
-    f = (!noncf || hmax < Hi[p]) ? 1.0 : Hi[p] / hmax;
+    const bool whole = !noncf || hmax < Hi[p];
+    f = whole ? 1.0 : Hi[p] / hmax;
+    h = whole ? Hi[p] : hmax;
   }
   fmf[p] = f;
+  if (heff) heff[p] = h;
+}
+
+// the fixed-geometry switches of update_ice_thickness
+// (src/ice_dynamics_module.f90:460-559), with the masks of the state at t:
+// H(t+dt) = H(t) on the shelf and the grounding line next to it (bit 0), the
+// sheet and the grounding line next to it (bit 1), the grounding line (bit 2)
+__device__ __forceinline__ bool any3(const Member &m, const int32_t *__restrict__ mask, int i, int j) {
+  for (int ii = max(0, i - 1); ii <= min(m.nx - 1, i + 1); ++ii)
+    for (int jj = max(0, j - 1); jj <= min(m.ny - 1, j + 1); ++jj)
+      if (mask[m.off + (long long)ii * m.ny + jj] == 1) return true;
+  return false;
+}
+__global__ void __launch_bounds__(TB) k_fix_geometry(const Member m, int which, const double *__restrict__ Hi,
+                                                    const int32_t *__restrict__ sheet,
+                                                    const int32_t *__restrict__ shelf, double *__restrict__ Ht) {
+  Cell c;
+  if (!cell_of(m, blockIdx.x * TB + threadIdx.x, c)) return;
+  const bool sh = shelf[c.p] == 1, gr = sheet[c.p] == 1;
+  const bool gl_sheet = gr && any3(m, shelf, c.i, c.j), gl_shelf = sh && any3(m, sheet, c.i, c.j);
+  const bool fix = ((which & 1) && (sh || gl_sheet)) || ((which & 2) && (gr || gl_shelf)) ||
+                   ((which & 4) && (gl_sheet || gl_shelf));
+  if (fix) Ht[c.p] = Hi[c.p];
+}
+
+// threshold_thickness_calving (src/calving_module.f90:62-91)
+__global__ void __launch_bounds__(TB) k_calve(const Member m, const int32_t *__restrict__ cf,
+                                             const double *__restrict__ heff, double thr, double *__restrict__ Hi) {
+  Cell c;
+  if (!cell_of(m, blockIdx.x * TB + threadIdx.x, c)) return;
+  if (cf[c.p] == 1 && heff[c.p] < thr) Hi[c.p] = 0.0;
+}
+
+// the ice removals after the calving (src/ice_dynamics_module.f90:590-635)
+__global__ void __launch_bounds__(TB) k_remove_ice(const Member m, const imau::Removal r, const double *__restrict__ Hb,
+                                                  const double *__restrict__ SL, double *__restrict__ Hi) {
+  Cell c;
+  if (!cell_of(m, blockIdx.x * TB + threadIdx.x, c)) return;
+  const long long p = c.p;
+  double h = Hi[p];
+  if (r.noice && r.noice[p] == 1) h = 0.0;
+  if (r.shelves && is_floating(h, Hb[p], SL[p])) h = 0.0;
+  if (r.Hi_PD && r.Hi_PD[p] == 0.0 && r.Hb_PD[p] < 0.0) h = 0.0;
+  if (r.Hi_GIA && r.Hi_GIA[p] == 0.0 && r.Hb_GIA[p] < r.min_height) h = 0.0;
+  Hi[p] = h;
 }
 
 // remove_unconnected_shelves (src/ice_thickness_module.f90:932-1044), what its
@@ -411,8 +460,18 @@ void prod_map_A(hipStream_t s, const Member &m, int nz, const double *A3, Str3 s
   hipLaunchKernelGGL(k_prod_map_A, dim3((unsigned)((n + TB - 1) / TB)), dim3(TB), 0, s, m, nz, A3, sa, A3cx, sx, A3cy,
                      sy, Av, Avcx, Avcy);
 }
-void prod_float_margin(hipStream_t s, const Member &m, const double *Hi, const ProdMasks &mk, double *fmf) {
-  hipLaunchKernelGGL(k_float_margin, dim3(blocks(m)), dim3(TB), 0, s, m, Hi, mk, fmf);
+void prod_float_margin(hipStream_t s, const Member &m, const double *Hi, const ProdMasks &mk, double *fmf,
+                       double *heff) {
+  hipLaunchKernelGGL(k_float_margin, dim3(blocks(m)), dim3(TB), 0, s, m, Hi, mk, fmf, heff);
+}
+void prod_fix_geometry(hipStream_t s, const Member &m, int which, const double *Hi, const ProdMasks &mk, double *Ht) {
+  hipLaunchKernelGGL(k_fix_geometry, dim3(blocks(m)), dim3(TB), 0, s, m, which, Hi, mk.sheet, mk.shelf, Ht);
+}
+void prod_calve(hipStream_t s, const Member &m, const ProdMasks &mk, const double *heff, double thr, double *Hi) {
+  hipLaunchKernelGGL(k_calve, dim3(blocks(m)), dim3(TB), 0, s, m, mk.cf, heff, thr, Hi);
+}
+void prod_remove_ice(hipStream_t s, const Member &m, const Removal &r, const double *Hb, const double *SL, double *Hi) {
+  hipLaunchKernelGGL(k_remove_ice, dim3(blocks(m)), dim3(TB), 0, s, m, r, Hb, SL, Hi);
 }
 void prod_shelf_seed(hipStream_t s, const Member &m, const ProdMasks &mk, int32_t *keep) {
   hipLaunchKernelGGL(k_shelf_seed, dim3(blocks(m)), dim3(TB), 0, s, m, mk, keep);

@@ -116,6 +116,10 @@ class ImauTimestepState(ctypes.Structure):
     _fields_ = _struct_fields(_TXT, "imau_timestep_state", DEFINES)
 
 
+class ImauRefgeoIO(ctypes.Structure):
+    _fields_ = _struct_fields(_TXT, "imau_refgeo_io", DEFINES)
+
+
 IO_POINTER_FIELDS = [n for n, t in ImauIceIO._fields_ if hasattr(t, "contents")]
 
 # export the enum constants at module level (IMAU_OK, IMAU_DYN_DIVA, ...)

@@ -203,14 +203,25 @@ def producer_config(**overrides):
     return c
 
 
+CALVING = {"none": abi.DEFINES["IMAU_CALVING_NONE"],
+           "threshold_thickness": abi.DEFINES["IMAU_CALVING_THRESHOLD_THICKNESS"]}
+
+
 def timestep_config(**overrides):
-    """Reference defaults of the time loop (src/configuration_module.f90:50-55)."""
+    """Reference defaults of the time loop (src/configuration_module.f90:50-55)
+    and of update_ice_thickness (:335-337, :412-417); choice_calving_law may
+    be given by its reference name."""
     c = abi.ImauTimestepCfg()
     c.start_time_of_run = 0.0
     c.end_time_of_run = 50000.0
     c.dt_max = 10.0
     c.dt_min = 0.01
     c.dt_startup_phase = 10.0
+    c.choice_calving_law = CALVING["threshold_thickness"]
+    c.calving_threshold_thickness = 200.0
+    c.continental_shelf_min_height = -2000.0
+    if isinstance(overrides.get("choice_calving_law"), str):
+        overrides["choice_calving_law"] = CALVING[overrides["choice_calving_law"]]
     for k, v in overrides.items():
         if not hasattr(c, k):
             raise AttributeError("imau_timestep_cfg has no field %r" % k)

@@ -89,7 +89,7 @@ def load_library(path: str = LIB_PATH):
                                          ctypes.c_double]
     L.imau_run_ice_dynamics_direct.argtypes = [vp, P(ImauGrid), P(ImauCfg), P(abi.ImauThicknessCfg),
                                                P(abi.ImauProducerCfg), P(abi.ImauTimestepCfg), P(ImauIceIO), vp, vp,
-                                               vp, vp, P(abi.ImauTimestepState), ctypes.c_double]
+                                               vp, vp, vp, P(abi.ImauTimestepState), ctypes.c_double]
     _lib = L
     return L
 
@@ -169,11 +169,12 @@ class Context:
                                                   ctypes.c_double(time)))
         return ps
 
-    def run_ice_dynamics_direct(self, grid, cfg, tcfg, pcfg, scfg, st, SMB, BMB, tstate, t_end, geo=None):
+    def run_ice_dynamics_direct(self, grid, cfg, tcfg, pcfg, scfg, st, SMB, BMB, tstate, t_end, geo=None, ref=None):
         """The 'direct' time loop (run_ice_dynamics_direct with
         update_ice_thickness) from tstate.time to t_end on IceState st, whose
         velocity outputs receive the last solve; returns the final thickness.
-        geo: a ProducerState for the final derived fields, or None."""
+        geo: a ProducerState for the final derived fields, or None; ref: a
+        state.RefGeo with the removal options' inputs, or None."""
         import numpy as np
         g = grid.struct()
         io = st.io()
@@ -183,9 +184,11 @@ class Context:
         Hi = np.zeros(shape, order="F")
         gio = geo.io(omit=("Hi_a", "Hb_a", "SL_a", "dHi_dt_a", "dHb_dt_a", "Ti_a")) if geo is not None else None
         vp = ctypes.c_void_p
+        rio = ref.io() if ref is not None else None
         self._check(self.L.imau_run_ice_dynamics_direct(
             self.h, ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(tcfg), ctypes.byref(pcfg), ctypes.byref(scfg),
-            ctypes.byref(io), vp(SMB.ctypes.data), vp(BMB.ctypes.data), vp(Hi.ctypes.data),
+            ctypes.byref(io), ctypes.cast(ctypes.pointer(rio), vp) if rio is not None else None,
+            vp(SMB.ctypes.data), vp(BMB.ctypes.data), vp(Hi.ctypes.data),
             ctypes.cast(ctypes.pointer(gio), vp) if gio is not None else None, ctypes.byref(tstate),
             ctypes.c_double(t_end)))
         st.take_stats(io)

@@ -231,3 +231,28 @@ class ProducerState:
             ct = ctypes.c_int32 if arr.dtype == np.int32 else ctypes.c_double
             setattr(io, name, arr.ctypes.data_as(ctypes.POINTER(ct)))
         return io
+
+
+class RefGeo:
+    """The fixed inputs of update_ice_thickness (src/ice_dynamics_module.f90:435-445):
+    mask_noice and the present-day / GIA-equilibrium reference geometries,
+    each (ny, nx) or None."""
+
+    NAMES = ("mask_noice_a", "Hi_PD", "Hb_PD", "Hi_GIAeq", "Hb_GIAeq")
+
+    def __init__(self, grid: Grid, **fields):
+        self.f = {}
+        for name, arr in fields.items():
+            if name not in self.NAMES:
+                raise KeyError("imau_refgeo_io has no field %r" % name)
+            if arr is None:
+                continue
+            dt = np.int32 if name == "mask_noice_a" else np.float64
+            self.f[name] = np.asfortranarray(np.broadcast_to(arr, (grid.ny, grid.nx)), dtype=dt)
+
+    def io(self):
+        io = abi.ImauRefgeoIO()
+        for name, arr in self.f.items():
+            ct = ctypes.c_int32 if arr.dtype == np.int32 else ctypes.c_double
+            setattr(io, name, arr.ctypes.data_as(ctypes.POINTER(ct)))
+        return io

@@ -443,11 +443,33 @@ int  imau_calc_ice_rheology(imau_ctx *ctx, const imau_grid *grid, const imau_pro
                             imau_producer_io *io, double time);
 
 /* ---- time stepping: choice_timestepping = 'direct' ----
- * The C% fields of the time loop (src/configuration_module.f90:50-55). */
+ * C%choice_calving_law (src/configuration_module.f90:412, calving_module.f90:30-91) */
+#define IMAU_CALVING_NONE                0
+#define IMAU_CALVING_THRESHOLD_THICKNESS 1
+
+/* The C% fields of the time loop (src/configuration_module.f90:50-55) and of
+ * update_ice_thickness (:335-337, :412-417); flags are 0 / 1. */
 typedef struct imau_timestep_cfg {
   double start_time_of_run, end_time_of_run;   /* [yr] */
   double dt_min, dt_max, dt_startup_phase;     /* [yr] */
+  int32_t choice_calving_law;                  /* IMAU_CALVING_* */
+  int32_t fixed_shelf_geometry, fixed_sheet_geometry, fixed_grounding_line;
+  int32_t do_remove_shelves, remove_shelves_larger_than_PD, continental_shelf_calving;
+  int32_t pad0_;
+  double calving_threshold_thickness;          /* [m] */
+  double continental_shelf_min_height;         /* [m] */
 } imau_timestep_cfg;
+
+/* The fixed inputs update_ice_thickness reads beside the ice model
+ * (src/ice_dynamics_module.f90:435-445), all (ny,nx): mask_noice (1 = no ice
+ * allowed; NULL = none), and the reference geometries that
+ * remove_shelves_larger_than_PD (refgeo_PD) and continental_shelf_calving
+ * (refgeo_GIAeq) read; NULL where the option is off. */
+typedef struct imau_refgeo_io {
+  const int32_t *mask_noice_a;
+  const double *Hi_PD, *Hb_PD;
+  const double *Hi_GIAeq, *Hb_GIAeq;
+} imau_refgeo_io;
 
 /* The timers of type_model_region the loop reads and writes
  * (src/data_types_module.f90: region%time, t_next_SIA, t_next_SSA, dt_crit_*, dt). */
@@ -463,11 +485,15 @@ typedef struct imau_timestep_state {
  * the SIA and SSA solves when their timers are due, each with its critical
  * time step calc_critical_timestep_SIA :651-725 / _adv :726-762, the time
  * step of determine_timesteps_and_actions :817-941 limited to the dynamics,
- * then calc_dHi_dt), update_ice_thickness (:435-648 with choice_calving_law =
- * 'none': H = max(0, H_t+dt), the masks, determine_floating_margin_fraction
- * src/general_ice_model_data_module.f90:784-865, remove_unconnected_shelves
- * src/ice_thickness_module.f90:932-1044, update_general_ice_model_data) and
- * time += dt.  Everything stays on the device between steps.
+ * then calc_dHi_dt), update_ice_thickness (:435-648: the fixed-geometry
+ * switches, H = max(0, H_t+dt), the masks, determine_floating_margin_fraction
+ * src/general_ice_model_data_module.f90:784-865, twice the calving law,
+ * remove_unconnected_shelves src/ice_thickness_module.f90:932-1044, the
+ * mask_noice / shelf / PD / continental-shelf removals, then
+ * update_general_ice_model_data) and time += dt.  Everything stays on the
+ * device between steps.  do_apply_prescribed_retreat_mask (a time series
+ * read from NetCDF) is not an option here.  ref may be NULL when no option
+ * that reads it is on.
  * cfg->choice_ice_dynamics: 'none', 'SIA', 'SSA' or 'SIA/SSA' (DIVA needs
  * the predictor-corrector loop, as in the reference).  io holds the state at
  * st->time (geometry and its derived fields consistent, as after
@@ -477,9 +503,9 @@ typedef struct imau_timestep_state {
  * the fields of geo that are not NULL; geo's input pointers are ignored. */
 int  imau_run_ice_dynamics_direct(imau_ctx *ctx, const imau_grid *grid, const imau_cfg *cfg,
                                   const imau_thickness_cfg *tcfg, const imau_producer_cfg *pcfg,
-                                  const imau_timestep_cfg *scfg, imau_ice_io *io, const double *SMB_year,
-                                  const double *BMB, double *Hi_end, imau_producer_io *geo,
-                                  imau_timestep_state *st, double t_end);
+                                  const imau_timestep_cfg *scfg, imau_ice_io *io, const imau_refgeo_io *ref,
+                                  const double *SMB_year, const double *BMB, double *Hi_end,
+                                  imau_producer_io *geo, imau_timestep_state *st, double t_end);
 
 /* ---- ensembles (no reference counterpart: the reference solves one region
  *      grid per call, src/IMAU_ICE_program.f90:247-250) ----

@@ -1045,6 +1045,22 @@ static void calc_basal_conditions(S *s)
     }
 }
 
+/* calc_basal_conditions alone, for unit tests: the pressures and Neff into
+ * io's outputs, the friction angle and till yield stress into phi / tauc */
+int orc_calc_basal_conditions(const imau_grid *gr, const imau_cfg *c, imau_ice_io *io, double *phi, double *tauc)
+{
+  S s;
+  st_init(&s, gr, c, io, 0);
+  calc_basal_conditions(&s);
+  cp_out(io->overburden_pressure_a, s.overb, s.na);
+  cp_out(io->pore_water_pressure_a, s.pore, s.na);
+  cp_out(io->Neff_a, s.Neff, s.na);
+  cp_out(phi, s.phi_fric, s.na);
+  cp_out(tauc, s.tauc, s.na);
+  st_free(&s);
+  return 0;
+}
+
 /* src/ice_velocity_module.f90:1322-1431 */
 static void set_sico_masks(S *s)
 {
@@ -2020,6 +2036,23 @@ void orc_abi_layout(int64_t *out)
   out[7] = (int64_t)offsetof(imau_ice_io, stat_seconds);
 }
 
+/* sizes of the other ABI structs and the offset of their last member, in the
+ * order thickness cfg / io, producer cfg / io, time-step cfg / state, refgeo io */
+void orc_abi_layout2(int64_t *out)
+{
+  out[0] = (int64_t)sizeof(imau_thickness_cfg);
+  out[1] = (int64_t)sizeof(imau_thickness_io);
+  out[2] = (int64_t)sizeof(imau_producer_cfg);
+  out[3] = (int64_t)sizeof(imau_producer_io);
+  out[4] = (int64_t)sizeof(imau_timestep_cfg);
+  out[5] = (int64_t)sizeof(imau_timestep_state);
+  out[6] = (int64_t)sizeof(imau_refgeo_io);
+  out[7] = (int64_t)offsetof(imau_timestep_cfg, continental_shelf_min_height);
+  out[8] = (int64_t)offsetof(imau_timestep_state, nsteps);
+  out[9] = (int64_t)offsetof(imau_refgeo_io, Hb_GIAeq);
+  out[10] = (int64_t)offsetof(imau_timestep_cfg, choice_calving_law);
+}
+
 /* ------------------------------------------------------------------------ */
 /* ice-thickness update, src/ice_thickness_module.f90:25-909 and            */
 /* calc_critical_timestep_adv, src/ice_dynamics_module.f90:726-762          */
@@ -2326,7 +2359,7 @@ static void producer_io_copy_out(imau_producer_io *geo, const imau_producer_io *
 /* ------------------------------------------------------------------------ */
 /* time stepping with choice_timestepping = 'direct':                       */
 /* run_ice_dynamics_direct (src/ice_dynamics_module.f90:63-242) and         */
-/* update_ice_thickness (:435-648, calving law 'none')                      */
+/* update_ice_thickness (:435-648)                                          */
 /* ------------------------------------------------------------------------ */
 /* calc_critical_timestep_SIA, :651-725 */
 static double crit_sia(const G *g, const double *dt_max, const double *u3sia, const double *v3sia, const double *u3,
@@ -2366,27 +2399,83 @@ static double crit_sia(const G *g, const double *dt_max, const double *u3sia, co
 }
 
 /* determine_floating_margin_fraction, src/general_ice_model_data_module.f90:784-865 */
+/* (with the effective front thickness Hi_eff_cf_a when heff is not NULL) */
 static void floating_margin_fraction(const G *g, const double *Hi, const int32_t *ice, const int32_t *cf,
-                                     const int32_t *shelf, double *fmf)
+                                     const int32_t *shelf, double *fmf, double *heff)
 {
   const int nx = g->nx, ny = g->ny;
   for (int i = 0; i < nx; ++i)
     for (int j = 1; j < ny - 1; ++j) {
       const size_t p = A2(j, i);
-      fmf[p] = ice[p] == 1 ? 1.0 : 0.0;
-      if (!(cf[p] == 1 && shelf[p] == 1)) continue;
-      int noncf = 0;
-      double hmax = 0.0;
-      for (int ii = (i > 0 ? i - 1 : 0); ii <= (i < nx - 1 ? i + 1 : nx - 1); ++ii)
-        for (int jj = j - 1; jj <= j + 1; ++jj)
-          if (ice[A2(jj, ii)] == 1 && cf[A2(jj, ii)] == 0) {
-            noncf = 1;
-            hmax = fmax(hmax, Hi[A2(jj, ii)]);
-          }
-      if (!noncf) { fmf[p] = 1.0; continue; }
-      if (hmax < Hi[p]) { fmf[p] = 1.0; continue; }
-      fmf[p] = Hi[p] / hmax;
+      double f = ice[p] == 1 ? 1.0 : 0.0, h = ice[p] == 1 ? Hi[p] : 0.0;
+      if (cf[p] == 1 && shelf[p] == 1) {
+        int noncf = 0;
+        double hmax = 0.0;
+        for (int ii = (i > 0 ? i - 1 : 0); ii <= (i < nx - 1 ? i + 1 : nx - 1); ++ii)
+          for (int jj = j - 1; jj <= j + 1; ++jj)
+            if (ice[A2(jj, ii)] == 1 && cf[A2(jj, ii)] == 0) {
+              noncf = 1;
+              hmax = fmax(hmax, Hi[A2(jj, ii)]);
+            }
+        if (!noncf || hmax < Hi[p]) {
+          f = 1.0;
+          h = Hi[p];
+        } else {
+          f = Hi[p] / hmax;
+          h = hmax;
+        }
+      }
+      fmf[p] = f;
+      if (heff) heff[p] = h;
     }
+}
+
+/* the fixed-geometry switches of update_ice_thickness,
+ * src/ice_dynamics_module.f90:460-559: H(t+dt) = H(t) on the shelf and its
+ * grounding line, on the sheet and its grounding line, or on the grounding
+ * line only (masks of the state at t) */
+static int any3(const G *g, const int32_t *m, int j, int i)
+{
+  for (int ii = (i > 0 ? i - 1 : 0); ii <= (i < g->nx - 1 ? i + 1 : g->nx - 1); ++ii)
+    for (int jj = (j > 0 ? j - 1 : 0); jj <= (j < g->ny - 1 ? j + 1 : g->ny - 1); ++jj)
+      if (m[(size_t)ii * g->ny + jj] == 1) return 1;
+  return 0;
+}
+static void fix_geometry(const G *g, const imau_timestep_cfg *sc, const double *Hi, const int32_t *sheet,
+                         const int32_t *shelf, double *Ht)
+{
+  for (int i = 0; i < g->nx; ++i)
+    for (int j = 0; j < g->ny; ++j) {
+      const size_t p = (size_t)i * g->ny + j;
+      const int gl_sheet = sheet[p] == 1 && any3(g, shelf, j, i);  /* sheet cell next to the shelf */
+      const int gl_shelf = shelf[p] == 1 && any3(g, sheet, j, i);  /* shelf cell next to the sheet */
+      int fix = 0;
+      if (sc->fixed_shelf_geometry && (shelf[p] == 1 || gl_sheet)) fix = 1;
+      if (sc->fixed_sheet_geometry && (sheet[p] == 1 || gl_shelf)) fix = 1;
+      if (sc->fixed_grounding_line && (gl_sheet || gl_shelf)) fix = 1;
+      if (fix) Ht[p] = Hi[p];
+    }
+}
+
+/* threshold_thickness_calving, src/calving_module.f90:62-91 */
+static void threshold_calving(size_t na, double *Hi, const int32_t *cf, const double *heff, double thr)
+{
+  for (size_t p = 0; p < na; ++p)
+    if (cf[p] == 1 && heff[p] < thr) Hi[p] = 0.0;
+}
+
+/* the ice removals of update_ice_thickness after the calving,
+ * src/ice_dynamics_module.f90:590-635 */
+static void remove_ice(size_t na, const imau_timestep_cfg *sc, const imau_refgeo_io *rg, const double *Hb,
+                       const double *SL, double *Hi)
+{
+  for (size_t p = 0; p < na; ++p) {
+    if (rg && rg->mask_noice_a && rg->mask_noice_a[p] == 1) Hi[p] = 0.0;
+    if (sc->do_remove_shelves && is_floating(Hi[p], Hb[p], SL[p])) Hi[p] = 0.0;
+    if (sc->remove_shelves_larger_than_PD && rg->Hi_PD[p] == 0.0 && rg->Hb_PD[p] < 0.0) Hi[p] = 0.0;
+    if (sc->continental_shelf_calving && rg->Hi_GIAeq[p] == 0.0 && rg->Hb_GIAeq[p] < sc->continental_shelf_min_height)
+      Hi[p] = 0.0;
+  }
 }
 
 /* remove_unconnected_shelves, src/ice_thickness_module.f90:932-1044: a
@@ -2427,20 +2516,26 @@ static void remove_unconnected_shelves(const G *g, double *Hi, const int32_t *sh
 
 int orc_run_ice_dynamics_direct(const imau_grid *gr, const imau_cfg *c, const imau_thickness_cfg *tc,
                                 const imau_producer_cfg *pc, const imau_timestep_cfg *sc, imau_ice_io *io,
-                                const double *SMB, const double *BMB, double *Hi_end, imau_producer_io *geo,
-                                imau_timestep_state *st, double t_end)
+                                const imau_refgeo_io *rg, const double *SMB, const double *BMB, double *Hi_end,
+                                imau_producer_io *geo, imau_timestep_state *st, double t_end)
 {
   const int dyn = c->choice_ice_dynamics;
   if (dyn != IMAU_DYN_NONE && dyn != IMAU_DYN_SIA && dyn != IMAU_DYN_SSA && dyn != IMAU_DYN_SIA_SSA)
     return fail(-2, "direct time stepping works with 'none', 'SIA', 'SSA' or 'SIA/SSA' only");
   if (!(sc->dt_startup_phase > 0.0) || !(sc->dt_min > 0.0) || sc->dt_max < sc->dt_min)
     return fail(-1, "need dt_startup_phase > 0 and 0 < dt_min <= dt_max");
+  if (sc->choice_calving_law != IMAU_CALVING_NONE && sc->choice_calving_law != IMAU_CALVING_THRESHOLD_THICKNESS)
+    return fail(-2, "unknown choice_calving_law");
+  if ((sc->remove_shelves_larger_than_PD && !(rg && rg->Hi_PD && rg->Hb_PD)) ||
+      (sc->continental_shelf_calving && !(rg && rg->Hi_GIAeq && rg->Hb_GIAeq)))
+    return fail(-1, "the reference geometry of a removal option is missing");
   G gg = {gr->nx, gr->ny, c->nz, gr->dx, gr->x, gr->y, c};
   const G *g = &gg;
   const int nx = g->nx, ny = g->ny, nz = g->nz;
   const size_t na = (size_t)nx * ny, ncx = (size_t)ny * (nx - 1), ncy = (size_t)(ny - 1) * nx;
   /* working geometry and derived fields */
   double *Hi = dalloc(na), *fmf = dalloc(na), *dHi = dalloc(na), *Ht = dalloc(na), *zero = dalloc(na);
+  double *heff = dalloc(na);
   double *A3 = dalloc(na * nz), *Av = dalloc(na);
   memcpy(Hi, io->Hi_a, na * sizeof(double));
   memcpy(A3, io->A_flow_3D_a, na * nz * sizeof(double));
@@ -2469,7 +2564,7 @@ int orc_run_ice_dynamics_direct(const imau_grid *gr, const imau_cfg *c, const im
   /* the derived fields of the state at entry, and its floating-margin fraction */
   pio.dHi_dt_a = NULL;
   orc_producers_update(gr, pc, &pio);
-  floating_margin_fraction(g, Hi, PB.ice, PB.cf, PB.shelf, fmf);
+  floating_margin_fraction(g, Hi, PB.ice, PB.cf, PB.shelf, fmf, heff);
   const int32_t nit0 = io->DIVA_SOR_nit;
   const double tol0 = io->DIVA_SOR_tol, rtol0 = io->DIVA_PETSc_rtol, atol0 = io->DIVA_PETSc_abstol;
   double time = st->time;
@@ -2528,12 +2623,22 @@ int orc_run_ice_dynamics_direct(const imau_grid *gr, const imau_cfg *c, const im
       if ((rc = orc_thickness_update(gr, tc, &t, dt))) break;
       for (size_t p = 0; p < na; ++p) Ht[p] = Hi[p] + dt * dHi[p];
     }
-    /* update_ice_thickness, :457-645 (calving law 'none', no optional removals) */
+    /* update_ice_thickness, :457-645 */
+    if (sc->fixed_shelf_geometry || sc->fixed_sheet_geometry || sc->fixed_grounding_line)
+      fix_geometry(g, sc, Hi, PB.sheet, PB.shelf, Ht);
     for (size_t p = 0; p < na; ++p) Hi[p] = fmax(0.0, Ht[p]);
     pio.dHi_dt_a = NULL;
     orc_producers_update(gr, pc, &pio);                         /* determine_masks_ice / _transitions */
-    floating_margin_fraction(g, Hi, PB.ice, PB.cf, PB.shelf, fmf);
+    floating_margin_fraction(g, Hi, PB.ice, PB.cf, PB.shelf, fmf, heff);
+    if (sc->choice_calving_law == IMAU_CALVING_THRESHOLD_THICKNESS) {  /* :574-588, twice */
+      threshold_calving(na, Hi, PB.cf, heff, sc->calving_threshold_thickness);
+      orc_producers_update(gr, pc, &pio);
+      floating_margin_fraction(g, Hi, PB.ice, PB.cf, PB.shelf, fmf, heff);
+      threshold_calving(na, Hi, PB.cf, heff, sc->calving_threshold_thickness);
+      orc_producers_update(gr, pc, &pio);
+    }
     remove_unconnected_shelves(g, Hi, PB.shelf, PB.sheet);
+    remove_ice(na, sc, rg, io->Hb_a, io->SL_a, Hi);
     pio.dHi_dt_a = dHi;                                          /* update_general_ice_model_data */
     pio.dHb_dt_a = io->dHb_dt_a ? io->dHb_dt_a : zero;
     orc_producers_update(gr, pc, &pio);
@@ -2541,6 +2646,12 @@ int orc_run_ice_dynamics_direct(const imau_grid *gr, const imau_cfg *c, const im
     ++st->nsteps;
   }
   st->time = time;
+  /* the statistics of the last velocity solve */
+  io->stat_picard_iterations = w.stat_picard_iterations;
+  io->stat_linear_sweeps = w.stat_linear_sweeps;
+  io->stat_resid_UV = w.stat_resid_UV;
+  io->stat_nonlinear_residual = w.stat_nonlinear_residual;
+  io->stat_seconds = w.stat_seconds;
   if (!rc) {
     memcpy(Hi_end, Hi, na * sizeof(double));
     if (geo) producer_io_copy_out(geo, &pio, na, ncx, ncy, (size_t)(ny - 1) * (nx - 1), nz);
@@ -2554,16 +2665,22 @@ int orc_run_ice_dynamics_direct(const imau_grid *gr, const imau_cfg *c, const im
   if (!io->u_SSA_cx) free(uS);
   if (!io->v_SSA_cy) free(vS);
   producer_bufs_free(&PB);
-  free(Hi); free(fmf); free(dHi); free(Ht); free(zero); free(A3); free(Av);
+  free(Hi); free(fmf); free(dHi); free(Ht); free(zero); free(A3); free(Av); free(heff);
   return rc;
 }
 
 /* building blocks of the loop, exposed for unit tests */
 void orc_floating_margin_fraction(const imau_grid *gr, const double *Hi, const int32_t *ice, const int32_t *cf,
-                                  const int32_t *shelf, double *fmf)
+                                  const int32_t *shelf, double *fmf, double *heff)
 {
   G gg = {gr->nx, gr->ny, 1, gr->dx, gr->x, gr->y, NULL};
-  floating_margin_fraction(&gg, Hi, ice, cf, shelf, fmf);
+  floating_margin_fraction(&gg, Hi, ice, cf, shelf, fmf, heff);
+}
+void orc_fix_geometry(const imau_grid *gr, const imau_timestep_cfg *sc, const double *Hi, const int32_t *sheet,
+                      const int32_t *shelf, double *Ht)
+{
+  G gg = {gr->nx, gr->ny, 1, gr->dx, gr->x, gr->y, NULL};
+  fix_geometry(&gg, sc, Hi, sheet, shelf, Ht);
 }
 void orc_remove_unconnected_shelves(const imau_grid *gr, double *Hi, const int32_t *shelf, const int32_t *sheet)
 {

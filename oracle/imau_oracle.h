@@ -78,10 +78,13 @@ int    orc_calc_ice_rheology(const imau_grid *g, const imau_producer_cfg *c, ima
  * imau_run_ice_dynamics_direct (include/imau_vel.h) without the context */
 int    orc_run_ice_dynamics_direct(const imau_grid *g, const imau_cfg *c, const imau_thickness_cfg *tc,
                                    const imau_producer_cfg *pc, const imau_timestep_cfg *sc, imau_ice_io *io,
-                                   const double *SMB, const double *BMB, double *Hi_end, imau_producer_io *geo,
-                                   imau_timestep_state *st, double t_end);
+                                   const imau_refgeo_io *rg, const double *SMB, const double *BMB, double *Hi_end,
+                                   imau_producer_io *geo, imau_timestep_state *st, double t_end);
+int    orc_calc_basal_conditions(const imau_grid *g, const imau_cfg *c, imau_ice_io *io, double *phi, double *tauc);
 void   orc_floating_margin_fraction(const imau_grid *g, const double *Hi, const int32_t *ice, const int32_t *cf,
-                                    const int32_t *shelf, double *fmf);
+                                    const int32_t *shelf, double *fmf, double *heff);
+void   orc_fix_geometry(const imau_grid *g, const imau_timestep_cfg *sc, const double *Hi, const int32_t *sheet,
+                        const int32_t *shelf, double *Ht);
 void   orc_remove_unconnected_shelves(const imau_grid *g, double *Hi, const int32_t *shelf, const int32_t *sheet);
 double orc_critical_timestep_sia(const imau_grid *g, int nz, const double *zeta, double dt_max, const double *u3sia,
                                  const double *v3sia, const double *u3, const double *v3, const double *Hi_cx,

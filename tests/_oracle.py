@@ -159,10 +159,10 @@ def calc_ice_rheology(grid, pcfg, ps, time=0.0, omit=()):
     return ps
 
 
-def run_ice_dynamics_direct(grid, cfg, tcfg, pcfg, scfg, st, SMB, BMB, tstate, t_end, geo=None):
+def run_ice_dynamics_direct(grid, cfg, tcfg, pcfg, scfg, st, SMB, BMB, tstate, t_end, geo=None, ref=None):
     """The oracle's 'direct' time loop on IceState st (velocity outputs written
     into st); returns the final thickness.  geo: a ProducerState for the final
-    derived fields, or None."""
+    derived fields, or None; ref: a RefGeo, or None."""
     L = lib()
     g = grid.struct()
     io = st.io()
@@ -170,8 +170,10 @@ def run_ice_dynamics_direct(grid, cfg, tcfg, pcfg, scfg, st, SMB, BMB, tstate, t
     BMB = np.asfortranarray(np.broadcast_to(BMB, (grid.ny, grid.nx)), dtype=np.float64)
     Hi = np.zeros((grid.ny, grid.nx), order="F")
     gio = geo.io(omit=("Hi_a", "Hb_a", "SL_a", "dHi_dt_a", "dHb_dt_a", "Ti_a")) if geo is not None else None
+    rio = ref.io() if ref is not None else None
     rc = L.orc_run_ice_dynamics_direct(ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(tcfg), ctypes.byref(pcfg),
-                                       ctypes.byref(scfg), ctypes.byref(io), _ptr(SMB), _ptr(BMB), _ptr(Hi),
+                                       ctypes.byref(scfg), ctypes.byref(io),
+                                       ctypes.byref(rio) if rio is not None else None, _ptr(SMB), _ptr(BMB), _ptr(Hi),
                                        ctypes.byref(gio) if gio is not None else None, ctypes.byref(tstate),
                                        ctypes.c_double(t_end))
     if rc != 0:
@@ -180,15 +182,27 @@ def run_ice_dynamics_direct(grid, cfg, tcfg, pcfg, scfg, st, SMB, BMB, tstate, t
     return Hi
 
 
-def floating_margin_fraction(grid, Hi, ice, cf, shelf, fmf):
+def floating_margin_fraction(grid, Hi, ice, cf, shelf, fmf, heff=None):
     L = lib()
     g = grid.struct()
     i32 = lambda a: np.asfortranarray(a, dtype=np.int32)
     ice, cf, shelf = i32(ice), i32(cf), i32(shelf)
     p32 = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
     Hi = np.asfortranarray(Hi, dtype=np.float64)
-    L.orc_floating_margin_fraction(ctypes.byref(g), _ptr(Hi), p32(ice), p32(cf), p32(shelf), _ptr(fmf))
+    L.orc_floating_margin_fraction(ctypes.byref(g), _ptr(Hi), p32(ice), p32(cf), p32(shelf), _ptr(fmf),
+                                   _ptr(heff) if heff is not None else None)
     return fmf
+
+
+def fix_geometry(grid, scfg, Hi, sheet, shelf, Ht):
+    L = lib()
+    g = grid.struct()
+    i32 = lambda a: np.asfortranarray(a, dtype=np.int32)
+    sheet, shelf = i32(sheet), i32(shelf)
+    p32 = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    Hi = np.asfortranarray(Hi, dtype=np.float64)
+    L.orc_fix_geometry(ctypes.byref(g), ctypes.byref(scfg), _ptr(Hi), p32(sheet), p32(shelf), _ptr(Ht))
+    return Ht
 
 
 def remove_unconnected_shelves(grid, Hi, shelf, sheet):

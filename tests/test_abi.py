@@ -24,6 +24,16 @@ def test_python_structs_match_c_layout():
     assert out[5] == abi.ImauIceIO.u_vav_cx.offset
     assert out[6] == abi.ImauIceIO.DIVA_SOR_nit.offset
     assert out[7] == abi.ImauIceIO.stat_seconds.offset
+    out = np.zeros(11, np.int64)
+    O.lib().orc_abi_layout2(O._ptr(out))
+    sizes = [ctypes.sizeof(c) for c in (abi.ImauThicknessCfg, abi.ImauThicknessIO, abi.ImauProducerCfg,
+                                        abi.ImauProducerIO, abi.ImauTimestepCfg, abi.ImauTimestepState,
+                                        abi.ImauRefgeoIO)]
+    assert list(out[:7]) == sizes
+    assert out[7] == abi.ImauTimestepCfg.continental_shelf_min_height.offset
+    assert out[8] == abi.ImauTimestepState.nsteps.offset
+    assert out[9] == abi.ImauRefgeoIO.Hb_GIAeq.offset
+    assert out[10] == abi.ImauTimestepCfg.choice_calving_law.offset
 
 
 def test_library_exports_every_declared_entry_point():

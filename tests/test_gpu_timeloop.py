@@ -1,14 +1,15 @@
 """GPU parity of the 'direct' time loop (run_ice_dynamics_direct,
 src/ice_dynamics_module.f90:63-242, with update_ice_thickness :435-648)
 against the oracle's loop: the same number of steps and timers, the final
-thickness, masks and velocities.  The velocity solves are converged tightly
+thickness, masks and velocities, with threshold calving (the reference
+default) and with the fixed-geometry and ice-removal options.  The velocity solves are converged tightly
 (FP64 residual 1e-10, Krylov 1e-12) so the two loops follow the same path."""
 import numpy as np
 import pytest
 
 import _oracle as O
 from imau_vel import config, geometry
-from imau_vel.state import ProducerState
+from imau_vel.state import ProducerState, RefGeo
 
 pytestmark = pytest.mark.gpu
 
@@ -38,23 +39,48 @@ def case(name):
         tight(cfg, st)
         return g, cfg, st, 0.2, 30.0, "none"
     g, cfg, st = geometry.channel(nx=61, ny=21, dx=12e3)      # marine: floating front at the east edge
-    config.set_config(cfg, choice_ice_dynamics="SSA" if name == "ssa_channel" else "SIA/SSA")
+    config.set_config(cfg, choice_ice_dynamics="SIA/SSA" if name == "hybrid_channel" else "SSA")
     tight(cfg, st)
     return g, cfg, st, 0.1, 20.0, "zero"
 
 
-@pytest.mark.parametrize("name", ["sia_dome", "hybrid_dome", "ssa_channel", "hybrid_channel"])
+def removal_inputs(g, st, name):
+    """update_ice_thickness options of the channel cases and their inputs."""
+    if name == "calving_channel":          # a threshold that the thinner part of the shelf front falls below
+        return {"calving_threshold_thickness": 450.0}, None
+    if name == "fixed_gl_pd_channel":      # grounding line held, a present-day front, a no-ice strip before it
+        PD = st["Hi_a"].copy()
+        PD[:, -4:] = 0.0
+        noice = np.zeros(PD.shape, dtype=np.int32)
+        noice[:, -6:-4] = 1
+        return ({"fixed_grounding_line": 1, "remove_shelves_larger_than_PD": 1},
+                RefGeo(g, mask_noice_a=noice, Hi_PD=PD, Hb_PD=st["Hb_a"]))
+    if name == "shelf_removal_channel":    # all floating ice removed; the sheet held; a continental-shelf edge
+        GIA = np.where(g.x[None, :] > g.x[-8], -3000.0, st["Hb_a"])
+        return ({"do_remove_shelves": 1, "fixed_sheet_geometry": 1, "continental_shelf_calving": 1},
+                RefGeo(g, Hi_GIAeq=np.zeros((g.ny, g.nx)), Hb_GIAeq=GIA))
+    return {}, None
+
+
+@pytest.mark.parametrize("name", ["sia_dome", "hybrid_dome", "ssa_channel", "hybrid_channel", "calving_channel",
+                                  "fixed_gl_pd_channel", "shelf_removal_channel"])
 def test_time_loop_matches_oracle(ctx, name):
     g, cfg, st, smb, t_end, bc = case(name)
     tcfg = config.thickness_config(choice_ice_dynamics=cfg.choice_ice_dynamics, ice_thickness_west_BC=bc,
                                    ice_thickness_east_BC=bc, ice_thickness_south_BC=bc, ice_thickness_north_BC=bc)
     pcfg = config.producer_config()
-    scfg = config.timestep_config(start_time_of_run=-1000.0, dt_max=5.0)
+    opts, rgeo = removal_inputs(g, st, name)
+    scfg = config.timestep_config(start_time_of_run=-1000.0, dt_max=5.0, **opts)
     gpu, ref = st.copy(), st.copy()
     ts_g, ts_r = config.timestep_state(0.0), config.timestep_state(0.0)
     geo_g, geo_r = ProducerState(g, cfg.nz), ProducerState(g, cfg.nz)
-    Hg = ctx.run_ice_dynamics_direct(g, cfg, tcfg, pcfg, scfg, gpu, smb, 0.0, ts_g, t_end, geo=geo_g)
-    Hr = O.run_ice_dynamics_direct(g, cfg, tcfg, pcfg, scfg, ref, smb, 0.0, ts_r, t_end, geo=geo_r)
+    Hg = ctx.run_ice_dynamics_direct(g, cfg, tcfg, pcfg, scfg, gpu, smb, 0.0, ts_g, t_end, geo=geo_g, ref=rgeo)
+    Hr = O.run_ice_dynamics_direct(g, cfg, tcfg, pcfg, scfg, ref, smb, 0.0, ts_r, t_end, geo=geo_r, ref=rgeo)
+    if opts:  # the options bite: more ice-free cells than the same loop without them
+        plain = config.timestep_config(start_time_of_run=-1000.0, dt_max=5.0)
+        Hp = O.run_ice_dynamics_direct(g, cfg, tcfg, pcfg, plain, st.copy(), smb, 0.0, config.timestep_state(0.0),
+                                       t_end)
+        assert (Hr == 0).sum() > (Hp == 0).sum(), ((Hr == 0).sum(), (Hp == 0).sum())
     assert ts_g.nsteps == ts_r.nsteps and ts_g.nsteps >= 4, (ts_g.nsteps, ts_r.nsteps)
     assert ts_g.time == ts_r.time == t_end
     for f in ("t_next_SIA", "t_next_SSA", "dt_crit_SIA", "dt_crit_SSA", "dt"):
@@ -63,5 +89,7 @@ def test_time_loop_matches_oracle(ctx, name):
     assert np.abs(Hg - Hr).max() <= 1e-8 * scale, np.abs(Hg - Hr).max() / scale
     for m in ("mask_ice_a", "mask_sheet_a", "mask_shelf_a", "mask_cf_a", "mask_a"):
         np.testing.assert_array_equal(geo_g[m], geo_r[m], err_msg=m)
+    if gpu.stats.get("picard_iterations", 0) > 0:   # the last solve converged (a capped loop is no parity case)
+        assert gpu.stats["nonlinear_residual"] <= 1e-9, gpu.stats
     u = np.abs(ref["u_vav_cx"]).max()
     assert np.abs(gpu["u_vav_cx"] - ref["u_vav_cx"]).max() <= 1e-6 * u

@@ -3,14 +3,16 @@ src/ice_dynamics_module.f90:63-242, with update_ice_thickness :435-648) and
 its building blocks to facts they do not restate: mass conservation over many
 steps, the timers of the loop, the SIA critical time step against the
 closed-form SIA column, the floating-margin fraction of hand-built fronts and
-the unconnected-shelf removal against connected components."""
+the unconnected-shelf removal against connected components, the threshold
+calving of hand-built shelves, the fixed-geometry switches against
+morphological dilation, and the ice removals."""
 import numpy as np
 import pytest
 from scipy import ndimage
 
 import _oracle as O
 from imau_vel import config, geometry
-from imau_vel.state import Grid, IceState, ProducerState
+from imau_vel.state import Grid, IceState, ProducerState, RefGeo
 
 CROSS = ndimage.generate_binary_structure(2, 1)
 
@@ -26,6 +28,7 @@ def loop_cfgs(dyn="SIA", **ts):
                                    ice_thickness_south_BC="none", ice_thickness_north_BC="none",
                                    choice_ice_dynamics=dyn)
     pcfg = config.producer_config(do_ocean_floodfill=0)
+    ts.setdefault("choice_calving_law", "none")
     ts.setdefault("start_time_of_run", -1000.0)
     ts.setdefault("end_time_of_run", 1e6)
     scfg = config.timestep_config(**ts)
@@ -153,3 +156,112 @@ def test_final_geometry_is_consistent():
     O.producers_update(g, pcfg, ref)
     for f in ("Hs_a", "mask_ice_a", "mask_sheet_a", "Hi_cx", "dHs_dx_cx", "dHs_dy_cy"):
         np.testing.assert_array_equal(geo[f], ref[f], err_msg=f)
+
+
+def shelf_bay(profile, nx=14, ny=7):
+    """Land walls in rows 0 and ny-1 and columns 0-2 (bed +100 m), a sheet of
+    500 m on the land between the walls, a shelf over a 1000 m deep bay in
+    columns 3 .. 3+len(profile)-1 with the given thickness per column, open
+    ocean beyond; the shelf's only calving front faces the open ocean."""
+    g = Grid(nx, ny, 10e3)
+    cfg = geometry.synthetic_config()
+    config.set_config(cfg, choice_ice_dynamics="none")
+    st = IceState(g, cfg.nz)
+    Hb = np.full((ny, nx), -1000.0)
+    Hb[:, :3] = 100.0
+    Hb[0, :] = Hb[-1, :] = 100.0
+    H = np.zeros((ny, nx))
+    H[1:-1, :3] = 500.0
+    H[1:-1, 3:3 + len(profile)] = profile
+    st["Hb_a"], st["Hi_a"] = Hb, H
+    st["A_flow_3D_a"] = 1e-16
+    st["A_flow_vav_a"] = 1e-16
+    return g, cfg, st
+
+
+def one_update(g, cfg, st, ref=None, **ts):
+    """One step of the loop with fixed dynamics (choice_ice_dynamics = 'none'):
+    the thickness only changes through update_ice_thickness."""
+    tcfg, pcfg, _ = loop_cfgs("none")
+    ts.setdefault("choice_calving_law", "none")
+    scfg = config.timestep_config(start_time_of_run=-1000.0, dt_max=5.0, **ts)
+    tstate = config.timestep_state(0.0)
+    H = O.run_ice_dynamics_direct(g, cfg, tcfg, pcfg, scfg, st, 0.0, 0.0, tstate, 5.0, ref=ref)
+    assert tstate.nsteps == 1
+    return H
+
+
+@pytest.mark.parametrize("thr, profile, calved", [
+    (200.0, [150.0] * 8, [9, 10]),                       # thin uniform shelf: one front column per pass, two passes
+    (200.0, [250.0] * 8, []),                            # thicker than the threshold: nothing
+    (200.0, [300 - 25 * k for k in range(8)], [9, 10]),  # thinning shelf: front 125 (eff. 150), then 150 (eff. 175)
+    (170.0, [300 - 25 * k for k in range(8)], [10]),     # ... the second front's effective 175 m survives 170 m
+])
+def test_threshold_calving_of_a_bay_shelf(thr, profile, calved):
+    """threshold_thickness_calving applied twice per update
+    (src/ice_dynamics_module.f90:574-588): a front cell goes when its effective
+    thickness (the thickest non-front ice neighbour, or its own when that is
+    thinner) is below the threshold; the second pass sees the new front."""
+    g, cfg, st = shelf_bay(profile)
+    H0 = st["Hi_a"].copy()
+    H = one_update(g, cfg, st, choice_calving_law="threshold_thickness", calving_threshold_thickness=thr)
+    expect = H0.copy()
+    expect[:, calved] = 0.0
+    np.testing.assert_array_equal(H, expect)
+    Hn = one_update(g, cfg, st, choice_calving_law="none")
+    np.testing.assert_array_equal(Hn, H0)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_fixed_geometry_switches_are_dilations(seed):
+    """fixed_shelf_geometry keeps H on the shelf and on sheet cells with a
+    shelf cell in their 3x3 neighbourhood, fixed_sheet_geometry the converse,
+    fixed_grounding_line both grounding-line sets
+    (src/ice_dynamics_module.f90:460-559): morphological dilations."""
+    rng = np.random.default_rng(seed)
+    ny, nx = 23, 27
+    g = Grid(nx, ny, 1e3)
+    sheet = (ndimage.gaussian_filter(rng.standard_normal((ny, nx)), 2) > 0.0).astype(np.int32)
+    shelf = ((rng.uniform(size=(ny, nx)) < 0.5) & (sheet == 0)).astype(np.int32)
+    Hi = rng.uniform(10, 100, (ny, nx))
+    Ht0 = rng.uniform(200, 300, (ny, nx))
+    box = np.ones((3, 3), bool)
+    gl_sheet = (sheet == 1) & ndimage.binary_dilation(shelf == 1, structure=box)
+    gl_shelf = (shelf == 1) & ndimage.binary_dilation(sheet == 1, structure=box)
+    for flags, fixed in (({"fixed_shelf_geometry": 1}, (shelf == 1) | gl_sheet),
+                         ({"fixed_sheet_geometry": 1}, (sheet == 1) | gl_shelf),
+                         ({"fixed_grounding_line": 1}, gl_sheet | gl_shelf)):
+        sc = config.timestep_config(**flags)
+        Ht = np.asfortranarray(Ht0.copy())
+        O.fix_geometry(g, sc, Hi, sheet, shelf, Ht)
+        np.testing.assert_array_equal(Ht, np.where(fixed, Hi, Ht0), err_msg=str(flags))
+        assert fixed.any() and (~fixed).any()
+
+
+def test_ice_removals():
+    """mask_noice, do_remove_shelves, remove_shelves_larger_than_PD and
+    continental_shelf_calving (src/ice_dynamics_module.f90:590-635): each
+    removes exactly the cells its rule names, and nothing else changes."""
+    g, cfg, st = shelf_bay([400.0] * 8)
+    H0 = st["Hi_a"].copy()
+    noice = np.zeros_like(H0, dtype=np.int32)
+    noice[2:4, 1] = 1
+    H = one_update(g, cfg, st.copy(), ref=RefGeo(g, mask_noice_a=noice))
+    np.testing.assert_array_equal(H, np.where(noice == 1, 0.0, H0))
+    H = one_update(g, cfg, st.copy(), do_remove_shelves=1)
+    np.testing.assert_array_equal(H, np.where(st["Hb_a"] < 0, 0.0, H0))
+    PD_Hi = H0.copy()
+    PD_Hi[:, 7:] = 0.0                                    # present-day front at column 7
+    H = one_update(g, cfg, st.copy(), ref=RefGeo(g, Hi_PD=PD_Hi, Hb_PD=st["Hb_a"]), remove_shelves_larger_than_PD=1)
+    expect = H0.copy()
+    expect[:, 7:] = 0.0
+    np.testing.assert_array_equal(H, expect)
+    GIA_Hb = np.where(np.arange(g.nx)[None, :] >= 9, -2500.0, -1000.0) * np.ones((g.ny, 1))
+    GIA_Hb[:, :3] = 100.0
+    H = one_update(g, cfg, st.copy(), ref=RefGeo(g, Hi_GIAeq=np.zeros_like(H0), Hb_GIAeq=GIA_Hb),
+                   continental_shelf_calving=1, continental_shelf_min_height=-2000.0)
+    expect = H0.copy()
+    expect[:, 9:] = 0.0
+    np.testing.assert_array_equal(H, expect)
+    with pytest.raises(RuntimeError):
+        one_update(g, cfg, st.copy(), remove_shelves_larger_than_PD=1)
