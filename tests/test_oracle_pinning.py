@@ -6,6 +6,7 @@ function is therefore checked against something independent of its own
 formula: closed forms, physical limits, brute force, a library routine
 (scipy's sparse direct solver), symmetries and invariants.
 """
+import ctypes
 import math
 
 import numpy as np
@@ -570,3 +571,77 @@ def test_periodic_rows_are_translation_invariant(axis):
     assert abs(u0[1, 0] - u0[1, 1]) > 10 * tol                        # ... which periodic rows would not give
     assert np.abs(w).max() <= 1e-4 * np.abs(u0).max()
     assert np.ptp(u0) > 0.05 * np.abs(u0).max()                      # the wave matters
+
+
+def test_martin2011_hydrology_and_roughness():
+    """Martin et al. (2011, The Cryosphere 5, 727-740) as configured by the
+    reference (src/basal_conditions_and_sliding_module.f90:90-294, defaults
+    src/configuration_module.f90:362-377): pore water pressure 0.96 x the
+    overburden times lambda, lambda = 1 for a bed at or below sea level + 0 m,
+    falling linearly to 0 at 1000 m above it; till friction angle 5 deg at or
+    below -1000 m bed elevation, 20 deg at or above 0 m, linear between.  The
+    hydrology reads the bed relative to sea level, the roughness the bed
+    itself: a sea level of 50 m tells them apart.  Values by hand."""
+    g = Grid(8, 5, 1e3)
+    cfg = geometry.synthetic_config()
+    config.set_config(cfg, choice_basal_hydrology="Martin2011", choice_basal_roughness="parameterised",
+                      choice_param_basal_roughness="Martin2011", choice_sliding_law="Coulomb_regularised")
+    st = IceState(g, cfg.nz)
+    Hb = np.array([-1500.0, -1000.0, -500.0, -250.0, 0.0, 300.0, 1050.0, 2000.0])
+    st["Hb_a"] = Hb[None, :]
+    st["Hi_a"] = 1000.0
+    st["SL_a"] = 50.0
+    io = st.io()
+    phi = np.zeros((g.ny, g.nx), order="F")
+    tauc = np.zeros((g.ny, g.nx), order="F")
+    assert O.lib().orc_calc_basal_conditions(ctypes.byref(g.struct()), ctypes.byref(cfg), ctypes.byref(io),
+                                             O._ptr(phi), O._ptr(tauc)) == 0
+    po = 910.0 * 9.81 * 1000.0
+    # Hb - SL = -1550, -1050, -550, -300, -50, 250, 1000, 1950 -> lambda 1, 1, 1, 1, 1, 0.75, 0, 0
+    lam = np.array([1.0, 1.0, 1.0, 1.0, 1.0, 0.75, 0.0, 0.0])
+    np.testing.assert_allclose(st["overburden_pressure_a"], po, rtol=1e-15)
+    np.testing.assert_allclose(st["pore_water_pressure_a"][2], 0.96 * po * lam, rtol=1e-14)
+    np.testing.assert_allclose(st["Neff_a"][2] / po, [0.04] * 5 + [0.28, 1.0, 1.0], rtol=1e-13)
+    np.testing.assert_allclose(phi[2], [5.0, 5.0, 12.5, 16.25, 20.0, 20.0, 20.0, 20.0], rtol=1e-15)
+    assert np.all(phi == phi[2]) and np.all(st["Neff_a"] == st["Neff_a"][2])
+
+
+def test_shelf_land_zero_rows_of_bc_mode():
+    """choice_ice_margin = 'BC': a velocity between a shelf cell and an
+    ice-free land cell is a zero row (src/ice_velocity_module.f90:1787-1796),
+    in either order of the two cells.  An ice-free island in the middle of a
+    y-uniform shelf: its four faces carry exactly zero velocity, the faces
+    next to them do not, and the flow stays mirror-symmetric about the
+    island's row (a rule applied to one orientation only would break both)."""
+    g, cfg, st = _floating_shelf("ssa", ny=9)
+    jc, ic = 4, 20
+    Hb, Hi = st["Hb_a"].copy(), st["Hi_a"].copy()
+    Hb[jc, ic], Hi[jc, ic] = 100.0, 0.0
+    st["Hb_a"], st["Hi_a"] = Hb, Hi
+    geometry.update_general_ice_model_data(st, config.zeta_of(cfg))
+    O.solve("ssa", g, cfg, st, O.LIN_BANDED_LU)
+    u, v = st["u_SSA_cx"], st["v_SSA_cy"]
+    assert u[jc, ic - 1] == 0.0 and u[jc, ic] == 0.0          # shelf|land and land|shelf
+    assert v[jc - 1, ic] == 0.0 and v[jc, ic] == 0.0
+    scale = np.abs(u).max()
+    assert abs(u[jc, ic - 2]) > 1e-3 * scale and abs(u[jc, ic + 1]) > 1e-3 * scale
+    assert abs(u[jc - 1, ic - 1]) > 1e-3 * scale and abs(v[jc - 1, ic - 1]) > 1e-6 * scale
+    np.testing.assert_allclose(u[1:jc], u[-2:jc:-1], rtol=0, atol=1e-7 * scale)      # mirror in y
+    np.testing.assert_allclose(v[1:jc - 1], -v[-2:jc:-1], rtol=0, atol=1e-7 * scale)
+
+
+def test_zero_boundary_wins_the_corner():
+    """Domain-boundary corners take 'zero' when either side is 'zero', else
+    'infinite' (src/ice_velocity_module.f90:1954-1989): with u_south = zero
+    and u_east = infinite the south-east u corner is exactly zero, the
+    north-east one (north and east infinite) copies its diagonal inner
+    neighbour, and the east column between them copies its west neighbour."""
+    g, cfg, st = _floating_shelf("ssa", ny=7)
+    config.set_config(cfg, DIVA_boundary_BC_u_south="zero")
+    O.solve("ssa", g, cfg, st, O.LIN_BANDED_LU)
+    u = st["u_SSA_cx"]
+    e = g.nx - 2                                  # the last u column
+    assert u[0, e] == 0.0 and np.all(u[0] == 0.0)
+    assert u[-1, e] == u[-2, e - 1]
+    np.testing.assert_array_equal(u[1:-1, e], u[1:-1, e - 1])
+    assert np.abs(u[-1]).max() > 0.0
