@@ -125,6 +125,9 @@ struct imau_problem {
   std::vector<std::unique_ptr<MGLevel>> mg;  // coarse levels 1.. of the multigrid preconditioner
   DBuf<int> mg_depth;                        // per member: its coarsest level
   DBuf<float> mg_f32;                        // FP32 fine-level coefficients, rhs and iterates
+  DBuf<uint8_t> mg_plain;                    // level-1 plain-aggregate flags
+  DBuf<int2> mg_list;                        // level-1 cells that are not plain
+  DBuf<int> mg_nlist;
   DBuf<double> mg_z64;                       // FP64 scratch iterate of the fused smoother
   DBuf<Tile> mg_ftiles;                      // tiles of the fused smoother
   DBuf<Tile> vtiles;                         // IMAU_VTI-row tiles of the 3D viscosity kernel
@@ -267,6 +270,13 @@ static void build_mg(imau_problem *P) {
   for (int d : depth) P->plan.levels[d - 1].ends = true;
   P->plan.depth = P->mg_depth.p;
   P->plan.ngrids = P->ngrids;
+  P->mg_plain.alloc((size_t)P->plan.levels[0].na);
+  P->plan.plain = P->mg_plain.p;
+  P->mg_list.alloc((size_t)P->plan.levels[0].na);
+  P->mg_nlist.alloc(1);
+  P->plan.list = P->mg_list.p;
+  P->plan.nlist_d = P->mg_nlist.p;
+  P->plan.nlist = 0;
   P->plan.coarsest_sweeps = 10;
   const long long na = P->na;
   P->mg_f32.alloc((size_t)12 * na);
@@ -411,6 +421,7 @@ void check_cfg(const imau_cfg *cfg) {
   if (cfg->picard_relax_max > 0.0 && cfg->stop_rule != IMAU_STOP_RESIDUAL)
     fail(IMAU_ERR_CONFIG, "adaptive Picard relaxation (picard_relax_max > 0) needs stop_rule IMAU_STOP_RESIDUAL");
   if (cfg->mg_rebuild < 1) fail(IMAU_ERR_CONFIG, "mg_rebuild must be >= 1, got %d", cfg->mg_rebuild);
+  if (cfg->mg_graph != 0 && cfg->mg_graph != 1) fail(IMAU_ERR_CONFIG, "mg_graph must be 0 or 1, got %d", cfg->mg_graph);
   if (!(cfg->anderson_beta > 0.0 && cfg->anderson_beta <= 1.0))
     fail(IMAU_ERR_CONFIG, "anderson_beta must lie in (0, 1], got %g", cfg->anderson_beta);
   if (!(cfg->anderson_restart >= 1.0))
@@ -676,6 +687,7 @@ void imau_vel_default_config(imau_cfg *c) {
   c->ksp_type = 0;
   c->anderson_m = 0;
   c->mg_rebuild = 1;
+  c->mg_graph = 1;
   c->warm_visc_iters = 0;
   c->anderson_beta = 1.0;
   c->anderson_restart = 1.5;
@@ -1153,8 +1165,42 @@ struct Solver {
   }
 
   // preconditioner application z = M^-1 y shared by the Krylov methods
+  // V-cycles captured as HIP graphs, one per (operands, precision, smoother
+  // form) seen in this solve: the kernel parameters, coefficients and levels
+  // are fixed for the solve, the activity flags are read from device memory
+  // at replay.  Released with the solver.
+  struct VGraph {
+    const double *yu, *yv;
+    double *zu, *zv;
+    bool fp32, fused, periodic;
+    hipGraphExec_t exec;
+  };
+  std::vector<VGraph> vgraphs_;
+  ~Solver() {
+    for (VGraph &g : vgraphs_) (void)hipGraphExecDestroy(g.exec);
+  }
   void precondition(const imau::Coef &cf, bool use_mg, bool fp32, const double *yu, const double *yv, double *zu,
                     double *zv) {
+    if (use_mg && cfg->mg_graph && c->ktime != IMAU_TIME_SMOOTHER) {  // (the smoother timing records events)
+      for (VGraph &g : vgraphs_)
+        if (g.yu == yu && g.yv == yv && g.zu == zu && g.zv == zv && g.fp32 == fp32 && g.fused == P->plan.fused &&
+            g.periodic == P->plan.periodic) {
+          HIPCHK(hipGraphLaunch(g.exec, c->s));
+          return;
+        }
+      hipGraph_t graph = nullptr;
+      HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+      imau::mg_vcycle(L, kp, cf, P->plan, fp32, yu, yv, zu, zv, P->act_lin.p, nullptr, nullptr);
+      const hipError_t ec = hipStreamEndCapture(c->s, &graph);
+      HIPCHK(ec);
+      VGraph g{yu, yv, zu, zv, fp32, P->plan.fused, P->plan.periodic, nullptr};
+      const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIPCHK(ei);
+      vgraphs_.push_back(g);
+      HIPCHK(hipGraphLaunch(g.exec, c->s));
+      return;
+    }
     if (use_mg) {
       imau::mg_vcycle(L, kp, cf, P->plan, fp32, yu, yv, zu, zv, P->act_lin.p, &Solver::mg_mark, this);
     } else {
@@ -1435,7 +1481,12 @@ struct Solver {
     std::fill(P->nl_resid.begin(), P->nl_resid.end(), NAN);
     imau::setup1(L, kp, P->F, P->yg.p);
     imau::setup2(L, kp, P->F);
+    imau::mg_plain_flags(L, P->F.rt_u, P->F.rt_v, P->plan);
     kcheck();
+    if (!P->plan.levels.empty()) {  // the length of the non-plain list, once per solve
+      HIPCHK(hipMemcpyAsync(&P->plan.nlist, P->plan.nlist_d, sizeof(int), hipMemcpyDeviceToHost, c->s));
+      HIPCHK(hipStreamSynchronize(c->s));
+    }
     std::vector<double> r;
     // adaptive relaxation: per-member factor, raised while the FP64 residual falls
     const bool adaptive = cfg->picard_relax_max > 0.0 && cfg->stop_rule == IMAU_STOP_RESIDUAL;
@@ -1573,12 +1624,10 @@ struct Solver {
     if (dyn == IMAU_DYN_DIVA) {
       imau::F1(L, kp, P->F, P->visc_cur, P->O);
       imau::uv3d(L, kp, P->F, P->O);
-      imau::w3d(L, kp, P->F, P->O);
-      imau::secondary(L, kp, P->F, P->O, dyn, P->F.u, P->F.v);
+      imau::final_a(L, kp, P->F, P->O, dyn, P->F.u, P->F.v, true);
     } else {
       imau::compose(L, kp, P->F, P->O, dyn == IMAU_DYN_SIA ? 1 : (dyn == IMAU_DYN_SSA ? 2 : 3));
-      if (dyn != IMAU_DYN_SSA) imau::w3d(L, kp, P->F, P->O);
-      imau::secondary(L, kp, P->F, P->O, dyn, P->O.uvav, P->O.vvav);
+      imau::final_a(L, kp, P->F, P->O, dyn, P->O.uvav, P->O.vvav, dyn != IMAU_DYN_SSA);
     }
     kcheck();
   }
@@ -2912,6 +2961,8 @@ int imau_run_ice_dynamics_direct(imau_ctx *ctx, const imau_grid *grid, const ima
     down2(s, geo->dHs_dy_cx, o.dHs_dy_cx, nx, ny, 'x');
     down2(s, geo->dHs_dx_cy, o.dHs_dx_cy, nx, ny, 'y');
     down2(s, geo->dHs_dy_cy, o.dHs_dy_cy, nx, ny, 'y');
+    down2(s, geo->float_margin_frac_a, X.fmf, nx, ny, 'a');
+    down2(s, geo->Hi_eff_cf_a, X.heff, nx, ny, 'a');
     down2(s, geo->A_flow_vav_a, f2p(P, F_A_vav), nx, ny, 'a');
     down2(s, geo->A_flow_vav_cx, X.Avcx, nx, ny, 'x');
     down2(s, geo->A_flow_vav_cy, X.Avcy, nx, ny, 'y');

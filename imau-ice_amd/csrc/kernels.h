@@ -82,9 +82,9 @@ void resid_ratio(hipStream_t s, int ngrids, const double *sums, double *r);
 void F1(const Launch &L, const KP &kp, const Fields &f, const double *visc, const OutF &o);
 void uv3d(const Launch &L, const KP &kp, const Fields &f, const OutF &o);
 void compose(const Launch &L, const KP &kp, const Fields &f, const OutF &o, int mode);
-void w3d(const Launch &L, const KP &kp, const Fields &f, const OutF &o);
-void secondary(const Launch &L, const KP &kp, const Fields &f, const OutF &o, int dyn, const double *uvav,
-               const double *vvav);
+// w_3D_a (with_w), the a-grid 3D velocities and the secondary 2D fields in one pass
+void final_a(const Launch &L, const KP &kp, const Fields &f, const OutF &o, int dyn, const double *uvav,
+             const double *vvav, bool with_w);
 void sia(const Launch &L, const KP &kp, const Fields &f, const double *Acx, const double *Acy, const OutF &o);
 void h2d_3d(hipStream_t s, const Member *mem, int g, const double *src, double *dst, int ni, int nj, int ldj, int nz);
 void d2h_3d(hipStream_t s, const Member *mem, int g, const double *src, double *dst, int ni, int nj, int ldj, int nz);

@@ -1467,10 +1467,46 @@ __global__ void __launch_bounds__(IMAU_WG) k_compose(const Member *__restrict__ 
   }
 }
 
-// w_3D_a by mass conservation (:534-661); u3a/v3a hold the a-grid 3D
-// velocities of the previous call, as in the reference
-__global__ void __launch_bounds__(IMAU_WG) k_w3d(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
-                                                 const KP kp, const Fields f, OutF o) {
+// selection of map_velocity_from_cx_to_a / _cy_to_a (:2592-2811) at a-cell
+// (j,i) from the ice mask: 0 -> 0, 1 -> the mean of the two staggered values,
+// 2 -> the upper one (index i or j), 3 -> the lower one (index i-1 or j-1);
+// at the first / last a cell only the edge value exists (2 / 3)
+__device__ __forceinline__ int mv_sel(const Member &m, const Fields &f, int j, int i, bool ydir) {
+  const int n = ydir ? m.ny : m.nx, c = ydir ? j : i;
+  const int mi = f.m_ice[p2(m, j, i)];
+  if (c == 0) return mi == 1 ? 2 : 0;
+  if (c == n - 1) return mi == 1 ? 3 : 0;
+  const int a0 = ydir ? f.m_ice[p2(m, j - 1, i)] : f.m_ice[p2(m, j, i - 1)];
+  const int a2 = ydir ? f.m_ice[p2(m, j + 1, i)] : f.m_ice[p2(m, j, i + 1)];
+  if (mi != 1) return 0;
+  if (a0 == 1 && a2 == 1) return 1;
+  if (a0 == 0 && a2 == 1) return 2;
+  if (a0 == 1 && a2 == 0) return 3;
+  return 0;
+}
+__device__ __forceinline__ double mv_pick(int sel, double lo, double hi) {
+  return sel == 1 ? 0.5 * (lo + hi) : (sel == 2 ? hi : (sel == 3 ? lo : 0.0));
+}
+// the same map of a 2D staggered field (n_stag = n - 1 values along the direction)
+__device__ __forceinline__ double mv_2d(const double *a, const Member &m, int sel, int j, int i, bool ydir) {
+  const int n = ydir ? m.ny : m.nx, c = ydir ? j : i;
+  const double lo = c > 0 ? a[ydir ? p2(m, j - 1, i) : p2(m, j, i - 1)] : 0.0;
+  const double hi = c < n - 1 ? a[p2(m, j, i)] : 0.0;
+  return mv_pick(sel, lo, hi);
+}
+
+// One pass over each column, base to surface: w_3D_a by mass conservation
+// (:534-661, when W; u3a/v3a hold the a-grid 3D velocities of the previous
+// call as the reference reads them, each read before it is replaced), the
+// a-grid copies of the 3D velocities, the depth-averaged, surface and basal
+// a-grid velocities, |u|, the shear ratio and the surface and basal w
+// (:490-533, :642-650).  The staggered values a level needs are loaded once
+// and serve both the divergence and the a-grid map.
+template <bool W>
+__global__ void __launch_bounds__(IMAU_WG) k_final_a(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
+                                                     const KP kp, const Fields f, OutF o, int dyn,
+                                                     const double *__restrict__ uvav,
+                                                     const double *__restrict__ vvav) {
   TILE_SETUP
   if (j >= m.ny) return;
   const int nx = m.nx, ny = m.ny, nz = kp.nz;
@@ -1479,93 +1515,72 @@ __global__ void __launch_bounds__(IMAU_WG) k_w3d(const Member *__restrict__ mem,
     const int i = t.i0 + ii;
     if (i >= nx) break;
     const long long p = p2(m, j, i);
-    // local ddx_a_to_a / ddy_a_to_a of Hs and Hi (:563-566)
-    double dHs_dx, dHs_dy, dHi_dx, dHi_dy;
-    if (i == 0) { dHs_dx = (f.Hs[p2(m, j, 1)] - f.Hs[p]) / dx; dHi_dx = (f.Hi[p2(m, j, 1)] - f.Hi[p]) / dx; }
-    else if (i == nx - 1) { dHs_dx = (f.Hs[p] - f.Hs[p2(m, j, nx - 2)]) / dx; dHi_dx = (f.Hi[p] - f.Hi[p2(m, j, nx - 2)]) / dx; }
-    else { dHs_dx = (f.Hs[p2(m, j, i + 1)] - f.Hs[p2(m, j, i - 1)]) / (2 * dx); dHi_dx = (f.Hi[p2(m, j, i + 1)] - f.Hi[p2(m, j, i - 1)]) / (2 * dx); }
-    if (j == 0) { dHs_dy = (f.Hs[p2(m, 1, i)] - f.Hs[p]) / dx; dHi_dy = (f.Hi[p2(m, 1, i)] - f.Hi[p]) / dx; }
-    else if (j == ny - 1) { dHs_dy = (f.Hs[p] - f.Hs[p2(m, ny - 2, i)]) / dx; dHi_dy = (f.Hi[p] - f.Hi[p2(m, ny - 2, i)]) / dx; }
-    else { dHs_dy = (f.Hs[p2(m, j + 1, i)] - f.Hs[p2(m, j - 1, i)]) / (2 * dx); dHi_dy = (f.Hi[p2(m, j + 1, i)] - f.Hi[p2(m, j - 1, i)]) / (2 * dx); }
-    const double dHbase_dx = dHs_dx - dHi_dx, dHbase_dy = dHs_dy - dHi_dy;
+    // the staggered columns read: iL/iR (jL/jR) of the divergence, which
+    // also hold the a-grid map's lower / upper values
     const int iL = (i == 0) ? 0 : (i == nx - 1 ? nx - 3 : i - 1);
     const int iR = (i == 0) ? 1 : (i == nx - 1 ? nx - 2 : i);
     const int jL = (j == 0) ? 0 : (j == ny - 1 ? ny - 3 : j - 1);
     const int jR = (j == 0) ? 1 : (j == ny - 1 ? ny - 2 : j);
+    const int sx = mv_sel(m, f, j, i, false), sy = mv_sel(m, f, j, i, true);
+    auto ulo = [&](double uL, double uR) { return i == nx - 1 ? uR : uL; };
+    auto uhi = [&](double uL, double uR) { return i == 0 ? uL : uR; };
+    auto vlo = [&](double vL, double vR) { return j == ny - 1 ? vR : vL; };
+    auto vhi = [&](double vL, double vR) { return j == 0 ? vL : vR; };
     const int kb = nz - 1;
-    double ub, vb;
-    if (i == 0) ub = o.u3[p3(m, kb, j, 0)];
-    else if (i == nx - 1) ub = o.u3[p3(m, kb, j, nx - 2)];
-    else ub = 0.5 * (o.u3[p3(m, kb, j, i - 1)] + o.u3[p3(m, kb, j, i)]);
-    if (j == 0) vb = o.v3[p3(m, kb, 0, i)];
-    else if (j == ny - 1) vb = o.v3[p3(m, kb, ny - 2, i)];
-    else vb = 0.5 * (o.v3[p3(m, kb, j - 1, i)] + o.v3[p3(m, kb, j, i)]);
-    double w = ub * dHbase_dx + vb * dHbase_dy;
-    if (f.m_sheet[p] == 1 && f.dHb_dt) w += f.dHb_dt[p];
-    o.w3[p3(m, kb, j, i)] = w;
-    const double Hm = fmax(0.1, f.Hi[p]);
-    const double dHsx = f.dHs_dx_a[p], dHix = f.dHi_dx_a[p], dHsy = f.dHs_dy_a[p], dHiy = f.dHi_dy_a[p];
+    double uL1 = o.u3[p3(m, kb, j, iL)], uR1 = o.u3[p3(m, kb, j, iR)];
+    double vL1 = o.v3[p3(m, kb, jL, i)], vR1 = o.v3[p3(m, kb, jR, i)];
+    double w = 0.0, wb = 0.0, ua1 = 0.0, va1 = 0.0;
+    double dHsx = 0.0, dHix = 0.0, dHsy = 0.0, dHiy = 0.0, Hm = 0.0, Hi = 0.0;
+    if (W) {
+      // local ddx_a_to_a / ddy_a_to_a of Hs and Hi (:563-566)
+      double dHs_dx, dHs_dy, dHi_dx, dHi_dy;
+      if (i == 0) { dHs_dx = (f.Hs[p2(m, j, 1)] - f.Hs[p]) / dx; dHi_dx = (f.Hi[p2(m, j, 1)] - f.Hi[p]) / dx; }
+      else if (i == nx - 1) { dHs_dx = (f.Hs[p] - f.Hs[p2(m, j, nx - 2)]) / dx; dHi_dx = (f.Hi[p] - f.Hi[p2(m, j, nx - 2)]) / dx; }
+      else { dHs_dx = (f.Hs[p2(m, j, i + 1)] - f.Hs[p2(m, j, i - 1)]) / (2 * dx); dHi_dx = (f.Hi[p2(m, j, i + 1)] - f.Hi[p2(m, j, i - 1)]) / (2 * dx); }
+      if (j == 0) { dHs_dy = (f.Hs[p2(m, 1, i)] - f.Hs[p]) / dx; dHi_dy = (f.Hi[p2(m, 1, i)] - f.Hi[p]) / dx; }
+      else if (j == ny - 1) { dHs_dy = (f.Hs[p] - f.Hs[p2(m, ny - 2, i)]) / dx; dHi_dy = (f.Hi[p] - f.Hi[p2(m, ny - 2, i)]) / dx; }
+      else { dHs_dy = (f.Hs[p2(m, j + 1, i)] - f.Hs[p2(m, j - 1, i)]) / (2 * dx); dHi_dy = (f.Hi[p2(m, j + 1, i)] - f.Hi[p2(m, j - 1, i)]) / (2 * dx); }
+      const double dHbase_dx = dHs_dx - dHi_dx, dHbase_dy = dHs_dy - dHi_dy;
+      const double ub = (i == 0) ? uL1 : (i == nx - 1 ? uR1 : 0.5 * (uL1 + uR1));
+      const double vb = (j == 0) ? vL1 : (j == ny - 1 ? vR1 : 0.5 * (vL1 + vR1));
+      w = ub * dHbase_dx + vb * dHbase_dy;
+      if (f.m_sheet[p] == 1 && f.dHb_dt) w += f.dHb_dt[p];
+      o.w3[p3(m, kb, j, i)] = w;
+      wb = w;
+      Hi = f.Hi[p];
+      Hm = fmax(0.1, Hi);
+      dHsx = f.dHs_dx_a[p]; dHix = f.dHi_dx_a[p]; dHsy = f.dHs_dy_a[p]; dHiy = f.dHi_dy_a[p];
+      ua1 = o.u3a[p3(m, kb, j, i)];
+      va1 = o.v3a[p3(m, kb, j, i)];
+    }
+    double usa = mv_pick(sx, ulo(uL1, uR1), uhi(uL1, uR1)), vsa = mv_pick(sy, vlo(vL1, vR1), vhi(vL1, vR1));
+    o.u3a[p3(m, kb, j, i)] = usa;
+    o.v3a[p3(m, kb, j, i)] = vsa;
     for (int k = nz - 2; k >= 0; --k) {
-      const double dudx_k = (o.u3[p3(m, k, j, iR)] - o.u3[p3(m, k, j, iL)]) / dx;
-      const double dudx_k1 = (o.u3[p3(m, k + 1, j, iR)] - o.u3[p3(m, k + 1, j, iL)]) / dx;
-      const double dvdy_k = (o.v3[p3(m, k, jR, i)] - o.v3[p3(m, k, jL, i)]) / dx;
-      const double dvdy_k1 = (o.v3[p3(m, k + 1, jR, i)] - o.v3[p3(m, k + 1, jL, i)]) / dx;
-      const double w1 = (dudx_k + dudx_k1) / 2.0, w2 = (dvdy_k + dvdy_k1) / 2.0;
-      const double zm = 0.5 * (kp.zeta[k + 1] + kp.zeta[k]), dz = kp.zeta[k + 1] - kp.zeta[k];
-      const double w3 = ((dHsx - zm * dHix) / Hm) * ((o.u3a[p3(m, k + 1, j, i)] - o.u3a[p3(m, k, j, i)]) / dz);
-      const double w4 = ((dHsy - zm * dHiy) / Hm) * ((o.v3a[p3(m, k + 1, j, i)] - o.v3a[p3(m, k, j, i)]) / dz);
-      w = w - f.Hi[p] * (w1 + w2 + w3 + w4) * dz;
-      o.w3[p3(m, k, j, i)] = w;
+      const double uL = o.u3[p3(m, k, j, iL)], uR = o.u3[p3(m, k, j, iR)];
+      const double vL = o.v3[p3(m, k, jL, i)], vR = o.v3[p3(m, k, jR, i)];
+      if (W) {
+        const double ua = o.u3a[p3(m, k, j, i)], va = o.v3a[p3(m, k, j, i)];
+        const double dudx_k = (uR - uL) / dx, dudx_k1 = (uR1 - uL1) / dx;
+        const double dvdy_k = (vR - vL) / dx, dvdy_k1 = (vR1 - vL1) / dx;
+        const double w1 = (dudx_k + dudx_k1) / 2.0, w2 = (dvdy_k + dvdy_k1) / 2.0;
+        const double zm = 0.5 * (kp.zeta[k + 1] + kp.zeta[k]), dz = kp.zeta[k + 1] - kp.zeta[k];
+        const double w3 = ((dHsx - zm * dHix) / Hm) * ((ua1 - ua) / dz);
+        const double w4 = ((dHsy - zm * dHiy) / Hm) * ((va1 - va) / dz);
+        w = w - Hi * (w1 + w2 + w3 + w4) * dz;
+        o.w3[p3(m, k, j, i)] = w;
+        ua1 = ua;
+        va1 = va;
+      }
+      usa = mv_pick(sx, ulo(uL, uR), uhi(uL, uR));
+      vsa = mv_pick(sy, vlo(vL, vR), vhi(vL, vR));
+      o.u3a[p3(m, k, j, i)] = usa;
+      o.v3a[p3(m, k, j, i)] = vsa;
+      uL1 = uL; uR1 = uR; vL1 = vL; vR1 = vR;
     }
-  }
-}
-
-// map_velocity_from_cx/cy_to_a (:2592-2811) value at a-cell (j,i) from a cx/cy field
-__device__ __forceinline__ double mapvel_x(const double *a, const Member &m, const Fields &f, int j, int i, int k, bool is3d) {
-  const int nx = m.nx;
-#define AT(jj, ii_) (is3d ? a[p3(m, k, (jj), (ii_))] : a[p2(m, (jj), (ii_))])
-  const int mi = f.m_ice[p2(m, j, i)];
-  if (i == 0) return mi == 1 ? AT(j, 0) : 0.0;
-  if (i == nx - 1) return mi == 1 ? AT(j, nx - 2) : 0.0;
-  const int a0 = f.m_ice[p2(m, j, i - 1)], a2 = f.m_ice[p2(m, j, i + 1)];
-  if (a0 == 1 && mi == 1 && a2 == 1) return 0.5 * (AT(j, i - 1) + AT(j, i));
-  if (a0 == 0 && mi == 1 && a2 == 1) return AT(j, i);
-  if (a0 == 1 && mi == 1 && a2 == 0) return AT(j, i - 1);
-  return 0.0;
-#undef AT
-}
-__device__ __forceinline__ double mapvel_y(const double *a, const Member &m, const Fields &f, int j, int i, int k, bool is3d) {
-  const int ny = m.ny;
-#define AT(jj, ii_) (is3d ? a[p3(m, k, (jj), (ii_))] : a[p2(m, (jj), (ii_))])
-  const int mi = f.m_ice[p2(m, j, i)];
-  if (j == 0) return mi == 1 ? AT(0, i) : 0.0;
-  if (j == ny - 1) return mi == 1 ? AT(ny - 2, i) : 0.0;
-  const int a0 = f.m_ice[p2(m, j - 1, i)], a2 = f.m_ice[p2(m, j + 1, i)];
-  if (a0 == 1 && mi == 1 && a2 == 1) return 0.5 * (AT(j - 1, i) + AT(j, i));
-  if (a0 == 0 && mi == 1 && a2 == 1) return AT(j, i);
-  if (a0 == 1 && mi == 1 && a2 == 0) return AT(j - 1, i);
-  return 0.0;
-#undef AT
-}
-
-// a-grid copies, |u|, shear ratio, surface and basal w (:490-533, :642-650)
-__global__ void __launch_bounds__(IMAU_WG) k_secondary(const Member *__restrict__ mem, const Tile *__restrict__ tiles,
-                                                       const KP kp, const Fields f, OutF o, int dyn,
-                                                       const double *__restrict__ uvav, const double *__restrict__ vvav) {
-  TILE_SETUP
-  if (j >= m.ny) return;
-  const int nz = kp.nz;
-  FOR_TILE_ROWS {
-    const int i = t.i0 + ii;
-    if (i >= m.nx) break;
-    const long long p = p2(m, j, i);
-    for (int k = 0; k < nz; ++k) {
-      o.u3a[p3(m, k, j, i)] = mapvel_x(o.u3, m, f, j, i, k, true);
-      o.v3a[p3(m, k, j, i)] = mapvel_y(o.v3, m, f, j, i, k, true);
-    }
-    const double uva = mapvel_x(uvav, m, f, j, i, 0, false), vva = mapvel_y(vvav, m, f, j, i, 0, false);
-    const double usa = mapvel_x(o.u3, m, f, j, i, 0, true), vsa = mapvel_y(o.v3, m, f, j, i, 0, true);
-    const double uba = mapvel_x(f.u_base, m, f, j, i, 0, false), vba = mapvel_y(f.v_base, m, f, j, i, 0, false);
+    // usa / vsa now hold the surface (k = 0) values
+    const double uva = mv_2d(uvav, m, sx, j, i, false), vva = mv_2d(vvav, m, sy, j, i, true);
+    const double uba = mv_2d(f.u_base, m, sx, j, i, false), vba = mv_2d(f.v_base, m, sy, j, i, true);
     o.u_vav_a[p] = uva; o.v_vav_a[p] = vva;
     o.u_surf_a[p] = usa; o.v_surf_a[p] = vsa;
     o.u_base_a[p] = uba; o.v_base_a[p] = vba;
@@ -1576,8 +1591,8 @@ __global__ void __launch_bounds__(IMAU_WG) k_secondary(const Member *__restrict_
     else if (dyn == 2) R = 0.0;
     else R = (f.m_ice[p] == 1) ? fmax(0.0, fmin(1.0, 1.0 - ab / as)) : 0.0;
     o.R_shear[p] = R;
-    o.w_surf[p] = o.w3[p3(m, 0, j, i)];
-    o.w_base[p] = o.w3[p3(m, nz - 1, j, i)];
+    o.w_surf[p] = W ? w : o.w3[p3(m, 0, j, i)];
+    o.w_base[p] = W ? wb : o.w3[p3(m, kb, j, i)];
   }
 }
 
@@ -1809,12 +1824,12 @@ void uv3d(const Launch &L, const KP &kp, const Fields &f, const OutF &o) {
 void compose(const Launch &L, const KP &kp, const Fields &f, const OutF &o, int mode) {
   hipLaunchKernelGGL(k_compose, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, o, mode);
 }
-void w3d(const Launch &L, const KP &kp, const Fields &f, const OutF &o) {
-  hipLaunchKernelGGL(k_w3d, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, o);
-}
-void secondary(const Launch &L, const KP &kp, const Fields &f, const OutF &o, int dyn, const double *uvav,
-               const double *vvav) {
-  hipLaunchKernelGGL(k_secondary, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, o, dyn, uvav, vvav);
+void final_a(const Launch &L, const KP &kp, const Fields &f, const OutF &o, int dyn, const double *uvav,
+             const double *vvav, bool with_w) {
+  if (with_w)
+    hipLaunchKernelGGL(k_final_a<true>, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, o, dyn, uvav, vvav);
+  else
+    hipLaunchKernelGGL(k_final_a<false>, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, o, dyn, uvav, vvav);
 }
 void sia(const Launch &L, const KP &kp, const Fields &f, const double *Acx, const double *Acy, const OutF &o) {
   hipLaunchKernelGGL(k_sia, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, kp, f, Acx, Acy, o);

@@ -39,6 +39,10 @@ struct MGPlan {
   const Tile *ftiles;    // IMAU_FTI x IMAU_FTJ tiles of the fused smoother
   int nftiles;
   int coarsest_sweeps;
+  uint8_t *plain;        // per level-1 cell: bit c when the component-c aggregate is plain (mg.hip)
+  int2 *list;            // the level-1 cells that are not plain: (member, index in the member's level)
+  int *nlist_d;          // their number on the device, and on the host after mg_plain_flags
+  int nlist;
   bool fused;            // one fused launch per fine-level sweep
   bool periodic;         // some domain boundary is periodic (needs a separate boundary-row pass)
 };
@@ -50,6 +54,10 @@ typedef void (*MGMark)(void *ctx, int which);
 // Galerkin coarse operators (once per linear solve, or every few solves)
 void mg_setup(const Launch &F, const KP &kp, const Coef &c, const MGPlan &plan, bool fp32, bool galerkin,
               const int *active);
+// once per solve, after the row types are set: marks the level-1 aggregates
+// whose Galerkin sums take the straight-line path
+// (the caller copies *plan.nlist_d into plan.nlist before the next mg_setup)
+void mg_plain_flags(const Launch &F, const uint8_t *rt_u, const uint8_t *rt_v, const MGPlan &plan);
 // z = V-cycle(y) for every active member
 void mg_vcycle(const Launch &F, const KP &kp, const Coef &c, const MGPlan &plan, bool fp32, const double *yu,
                const double *yv, double *zu, double *zv, const int *active, MGMark mark, void *mark_ctx);

@@ -170,9 +170,142 @@ __device__ __forceinline__ bool fine_valid(const Member &m, int comp, int j, int
 // Galerkin products
 // ---------------------------------------------------------------------------
 // fine (level 0, FP64 coefficients) -> level 1
+// One level-1 cell, the components not marked plain: every coupling's
+// unknown is mapped to its aggregate (fine_agg) and slot at run time.
 template <class T, int CROSS>
-__global__ void __launch_bounds__(IMAU_WG) k_gal0(const Member *__restrict__ mem, const KP kp, const CoefT<double> c,
-                                                  CLevelT<T> L, const int *__restrict__ active) {
+__device__ __forceinline__ void gal0_cell(const Member &m, const KP &kp, const CoefT<double> &c, const CLevelT<T> &L,
+                                          const CMember &cm, int J, int I, long long pc, int fl) {
+  for (int comp = 0; comp < 2; ++comp) {
+    if (!cvalid(cm, comp, J, I) || ((fl >> comp) & 1)) continue;
+    double cf[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int dj = 0; dj < 2; ++dj)
+      for (int di = 0; di < 2; ++di) {
+        const int j = 2 * J + dj, i = 2 * I + di;
+        if (!fine_valid(m, comp, j, i)) continue;
+        const uint8_t rt = comp == 0 ? c.rt_u[p2(m, j, i)] : c.rt_v[p2(m, j, i)];
+        if (!in_row(rt)) continue;
+        auto acc = [&](int cc, int oj, int oi, double val) {
+          int Jc, Ic;
+          if (!fine_agg(m, c.rt_u, c.rt_v, cc, j + oj, i + oi, Jc, Ic)) return;
+          const int s = agg_slot(comp, cc, Jc - J, Ic - I);
+          if (s >= 0) cf[s] += val;
+        };
+        if (comp == 0) u_couplings<CROSS>(m, kp, c, j, i, rt, acc);
+        else v_couplings<CROSS>(m, kp, c, j, i, rt, acc);
+      }
+    T *dst = comp == 0 ? L.cu : L.cv;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) dst[k * L.na + pc] = (T)cf[k];
+  }
+}
+
+// fine -> level 1 on the cells that are not plain (boundary rows, calving
+// fronts, zero rows and their neighbours): a compact list built with the
+// flags, so no wavefront of the interior waits for one such lane
+template <class T, int CROSS>
+__global__ void __launch_bounds__(256) k_gal0_list(const Member *__restrict__ mem, const KP kp, const CoefT<double> c,
+                                                   CLevelT<T> L, const int *__restrict__ active,
+                                                   const uint8_t *__restrict__ plain, const int2 *__restrict__ list,
+                                                   int n) {
+  const int q = (int)(blockIdx.x * 256 + threadIdx.x);
+  if (q >= n) return;
+  const int2 e = list[q];
+  if (!active[e.x]) return;
+  const CMember cm = L.mem[e.x];
+  const int I = e.y / cm.ny, J = e.y - I * cm.ny;
+  const long long pc = cm.off + e.y;
+  gal0_cell<T, CROSS>(mem[e.x], kp, c, L, cm, J, I, pc, plain[pc]);
+}
+
+// Plain aggregates: all four fine rows of a level-1 aggregate of component
+// comp are interior rows (RT_FREE) and every unknown their couplings reach is
+// an interior or calving-front unknown.  The aggregate of each such unknown
+// is then (j >> 1, i >> 1), so a coupling's coarse slot follows from its
+// offsets alone.  Bit comp of plain[level-1 cell]; once per solve (the row
+// types are fixed for a solve).
+__device__ __forceinline__ bool plain_row(const Member &m, const uint8_t *rt_u, const uint8_t *rt_v, int comp, int j,
+                                          int i) {
+  if (!fine_valid(m, comp, j, i)) return false;
+  if ((comp == 0 ? rt_u[p2(m, j, i)] : rt_v[p2(m, j, i)]) != RT_FREE) return false;
+  // the unknowns of u_couplings / v_couplings for an interior row
+  bool ok;
+  if (comp == 0)
+    ok = in_row(rt_u[p2(m, j, i + 1)]) && in_row(rt_u[p2(m, j, i - 1)]) && in_row(rt_u[p2(m, j + 1, i)]) &&
+         in_row(rt_u[p2(m, j - 1, i)]) && in_row(rt_v[p2(m, j, i)]) && in_row(rt_v[p2(m, j, i + 1)]) &&
+         in_row(rt_v[p2(m, j - 1, i)]) && in_row(rt_v[p2(m, j - 1, i + 1)]);
+  else
+    ok = in_row(rt_v[p2(m, j + 1, i)]) && in_row(rt_v[p2(m, j - 1, i)]) && in_row(rt_v[p2(m, j, i + 1)]) &&
+         in_row(rt_v[p2(m, j, i - 1)]) && in_row(rt_u[p2(m, j + 1, i - 1)]) && in_row(rt_u[p2(m, j + 1, i)]) &&
+         in_row(rt_u[p2(m, j, i - 1)]) && in_row(rt_u[p2(m, j, i)]);
+  return ok;
+}
+// The cells with a valid component that is not plain are appended to list
+// (member, index within the member's level), counted in *n (zeroed first).
+__global__ void __launch_bounds__(IMAU_WG) k_gal_plain_flags(const Member *__restrict__ mem,
+                                                             const uint8_t *__restrict__ rt_u,
+                                                             const uint8_t *__restrict__ rt_v, CLevelT<float> L,
+                                                             uint8_t *__restrict__ plain, int2 *__restrict__ list,
+                                                             int *__restrict__ n) {
+  CTILE_SETUP
+  if (J >= cm.ny) return;
+  const Member m = mem[t.g];
+  FOR_TILE_ROWS {
+    const int I = t.i0 + ii;
+    if (I >= cm.nx) break;
+    int fl = 0, valid = 0;
+    for (int comp = 0; comp < 2; ++comp) {
+      bool ok = cvalid(cm, comp, J, I);
+      if (ok) valid |= 1 << comp;
+      for (int q = 0; q < 4 && ok; ++q) ok = plain_row(m, rt_u, rt_v, comp, 2 * J + (q >> 1), 2 * I + (q & 1));
+      if (ok) fl |= 1 << comp;
+    }
+    plain[cp2(cm, J, I)] = (uint8_t)fl;
+    // one atomic per wavefront: the lanes with a cell to append take
+    // consecutive slots after the wavefront's base
+    const bool add = (valid & ~fl) != 0;
+    const unsigned long long mask = __ballot(add);
+    if (mask) {
+      const int lane = (int)__lane_id(), lead = __ffsll((long long)mask) - 1;
+      int base = 0;
+      if (lane == lead) base = atomicAdd(n, __popcll(mask));
+      base = __shfl(base, lead);
+      if (add) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = make_int2(t.g, I * cm.ny + J);
+    }
+  }
+}
+
+// one fine row (2J+DJ, 2I+DI) of a plain aggregate: every coupling offset is a
+// literal after inlining, so the slot is a constant and cf stays in registers
+template <int CROSS, int COMP, int DJ, int DI>
+__device__ __forceinline__ void gal_plain_row(const Member &m, const KP &kp, const CoefT<double> &c, int J, int I,
+                                              double (&cf)[9]) {
+  auto acc = [&](int cc, int oj, int oi, double val) {
+    const int s = agg_slot(COMP, cc, (DJ + oj) >> 1, (DI + oi) >> 1);
+    if (s >= 0) cf[s] += val;
+  };
+  if (COMP == 0) u_couplings<CROSS>(m, kp, c, 2 * J + DJ, 2 * I + DI, RT_FREE, acc);
+  else v_couplings<CROSS>(m, kp, c, 2 * J + DJ, 2 * I + DI, RT_FREE, acc);
+}
+
+// fine -> level 1 on the plain aggregates: the same sums in the same order as
+// k_gal0, with straight-line loads and no aggregate look-ups
+template <class T, int CROSS, int COMP>
+__device__ __forceinline__ void gal_plain(const Member &m, const KP &kp, const CoefT<double> &c, const CLevelT<T> &L,
+                                          int J, int I, long long pc) {
+  double cf[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  gal_plain_row<CROSS, COMP, 0, 0>(m, kp, c, J, I, cf);
+  gal_plain_row<CROSS, COMP, 0, 1>(m, kp, c, J, I, cf);
+  gal_plain_row<CROSS, COMP, 1, 0>(m, kp, c, J, I, cf);
+  gal_plain_row<CROSS, COMP, 1, 1>(m, kp, c, J, I, cf);
+  T *dst = COMP == 0 ? L.cu : L.cv;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) dst[k * L.na + pc] = (T)cf[k];
+}
+template <class T, int CROSS>
+__global__ void __launch_bounds__(IMAU_WG) k_gal0_plain(const Member *__restrict__ mem, const KP kp,
+                                                        const CoefT<double> c, CLevelT<T> L,
+                                                        const int *__restrict__ active,
+                                                        const uint8_t *__restrict__ plain) {
   CTILE_SETUP
   if (J >= cm.ny || !active[t.g]) return;
   const Member m = mem[t.g];
@@ -180,32 +313,31 @@ __global__ void __launch_bounds__(IMAU_WG) k_gal0(const Member *__restrict__ mem
     const int I = t.i0 + ii;
     if (I >= cm.nx) break;
     const long long pc = cp2(cm, J, I);
-    for (int comp = 0; comp < 2; ++comp) {
-      if (!cvalid(cm, comp, J, I)) continue;
-      double cf[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-      for (int dj = 0; dj < 2; ++dj)
-        for (int di = 0; di < 2; ++di) {
-          const int j = 2 * J + dj, i = 2 * I + di;
-          if (!fine_valid(m, comp, j, i)) continue;
-          const uint8_t rt = comp == 0 ? c.rt_u[p2(m, j, i)] : c.rt_v[p2(m, j, i)];
-          if (!in_row(rt)) continue;
-          auto acc = [&](int cc, int oj, int oi, double val) {
-            int Jc, Ic;
-            if (!fine_agg(m, c.rt_u, c.rt_v, cc, j + oj, i + oi, Jc, Ic)) return;
-            const int s = agg_slot(comp, cc, Jc - J, Ic - I);
-            if (s >= 0) cf[s] += val;
-          };
-          if (comp == 0) u_couplings<CROSS>(m, kp, c, j, i, rt, acc);
-          else v_couplings<CROSS>(m, kp, c, j, i, rt, acc);
-        }
-      T *dst = comp == 0 ? L.cu : L.cv;
-#pragma unroll
-      for (int k = 0; k < 9; ++k) dst[k * L.na + pc] = (T)cf[k];
-    }
+    const int fl = plain[pc];
+    if (fl & 1) gal_plain<T, CROSS, 0>(m, kp, c, L, J, I, pc);
+    if (fl & 2) gal_plain<T, CROSS, 1>(m, kp, c, L, J, I, pc);
   }
 }
 
-// level l (Fl) -> level l+1 (L), accumulated in FP64
+// one fine cell (2J+DJ, 2I+DI) of a coarse cell whose fine cells and
+// couplings all lie inside the level: slot and target are constants
+template <class T, int COMP, int DJ, int DI>
+__device__ __forceinline__ void galc_plain(const CLevelT<T> &Fl, const CMember &fm, int J, int I, double (&cf)[9]) {
+  const long long pf = cp2(fm, 2 * J + DJ, 2 * I + DI);
+  const T *src = COMP == 0 ? Fl.cu : Fl.cv;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    int cc, jj, i2;
+    slot_pos(COMP, k, DJ, DI, cc, jj, i2);  // offsets from the coarse cell's first fine cell
+    const int s = agg_slot(COMP, cc, jj >> 1, i2 >> 1);
+    if (s >= 0) cf[s] += (double)src[k * Fl.na + pf];
+  }
+}
+
+// level l (Fl) -> level l+1 (L), accumulated in FP64.  Away from the
+// level's edges (the test below: every fine cell and every unknown its row
+// reaches is valid) the sums run with constant slots, in the same order as
+// the general loop.
 template <class T>
 __global__ void __launch_bounds__(IMAU_WG) k_galc(CLevelT<T> Fl, CLevelT<T> L, const int *__restrict__ active) {
   CTILE_SETUP
@@ -215,8 +347,29 @@ __global__ void __launch_bounds__(IMAU_WG) k_galc(CLevelT<T> Fl, CLevelT<T> L, c
     const int I = t.i0 + ii;
     if (I >= cm.nx) break;
     const long long pc = cp2(cm, J, I);
+    const bool inner = J >= 1 && I >= 1;
+    const bool plain_u = inner && 2 * J + 2 < fm.nyu && 2 * I + 2 < fm.nxu && 2 * J + 1 < fm.nyv && 2 * I + 2 < fm.nxv;
+    const bool plain_v = inner && 2 * J + 2 < fm.nyv && 2 * I + 2 < fm.nxv && 2 * J + 2 < fm.nyu && 2 * I + 1 < fm.nxu;
+    if (plain_u && cvalid(cm, 0, J, I)) {
+      double cf[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      galc_plain<T, 0, 0, 0>(Fl, fm, J, I, cf);
+      galc_plain<T, 0, 0, 1>(Fl, fm, J, I, cf);
+      galc_plain<T, 0, 1, 0>(Fl, fm, J, I, cf);
+      galc_plain<T, 0, 1, 1>(Fl, fm, J, I, cf);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) L.cu[k * L.na + pc] = (T)cf[k];
+    }
+    if (plain_v && cvalid(cm, 1, J, I)) {
+      double cf[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      galc_plain<T, 1, 0, 0>(Fl, fm, J, I, cf);
+      galc_plain<T, 1, 0, 1>(Fl, fm, J, I, cf);
+      galc_plain<T, 1, 1, 0>(Fl, fm, J, I, cf);
+      galc_plain<T, 1, 1, 1>(Fl, fm, J, I, cf);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) L.cv[k * L.na + pc] = (T)cf[k];
+    }
     for (int comp = 0; comp < 2; ++comp) {
-      if (!cvalid(cm, comp, J, I)) continue;
+      if (!cvalid(cm, comp, J, I) || (comp == 0 ? plain_u : plain_v)) continue;
       double cf[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
       const T *src = comp == 0 ? Fl.cu : Fl.cv;
       for (int dj = 0; dj < 2; ++dj)
@@ -783,8 +936,13 @@ void fill(hipStream_t s, T *p, long long n) {
 template <class T>
 void setup_t(const imau::Launch &F, const KP &kp, const imau::Coef &c, const imau::MGPlan &P, const int *act) {
   const CLevelT<T> L1 = view<T>(P, 0);
-  auto kg = kp.cross ? k_gal0<T, 1> : k_gal0<T, 0>;
-  hipLaunchKernelGGL(kg, dim3(L1.ntiles), wg(), 0, F.s, F.mem, kp, coef64(c), L1, act);
+  auto kp0 = kp.cross ? k_gal0_plain<T, 1> : k_gal0_plain<T, 0>;
+  hipLaunchKernelGGL(kp0, dim3(L1.ntiles), wg(), 0, F.s, F.mem, kp, coef64(c), L1, act, P.plain);
+  if (P.nlist > 0) {
+    auto kg = kp.cross ? k_gal0_list<T, 1> : k_gal0_list<T, 0>;
+    hipLaunchKernelGGL(kg, dim3((P.nlist + 255) / 256), dim3(256), 0, F.s, F.mem, kp, coef64(c), L1, act, P.plain,
+                       P.list, P.nlist);
+  }
   for (size_t l = 1; l < P.levels.size(); ++l) {
     const CLevelT<T> Fl = view<T>(P, (int)l - 1), L = view<T>(P, (int)l);
     hipLaunchKernelGGL(k_galc<T>, dim3(L.ntiles), wg(), 0, F.s, Fl, L, act);
@@ -880,6 +1038,14 @@ void vcycle_fused_t(const imau::Launch &F, const KP &kp, const CoefT<T> &cf, con
 }  // namespace
 
 namespace imau {
+
+void mg_plain_flags(const Launch &F, const uint8_t *rt_u, const uint8_t *rt_v, const MGPlan &P) {
+  if (P.levels.empty()) return;
+  const CLevelT<float> L1 = view<float>(P, 0);
+  fill_i32(F.s, P.nlist_d, 1, 0);
+  hipLaunchKernelGGL(k_gal_plain_flags, dim3(L1.ntiles), wg(), 0, F.s, F.mem, rt_u, rt_v, L1, P.plain, P.list,
+                     P.nlist_d);
+}
 
 void mg_setup(const Launch &F, const KP &kp, const Coef &c, const MGPlan &P, bool fp32, bool galerkin,
               const int *active) {

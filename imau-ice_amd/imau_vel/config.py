@@ -92,6 +92,7 @@ def default_config(**overrides) -> ImauCfg:
     c.ksp_type = 0
     c.anderson_m = 0
     c.mg_rebuild = 1
+    c.mg_graph = 1
     c.warm_visc_iters = 0
     c.anderson_beta = 1.0
     c.anderson_restart = 1.5

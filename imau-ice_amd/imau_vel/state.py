@@ -196,6 +196,7 @@ PRODUCER_FIELDS = {
     "dHi_dx_a": ("a", 0, _F64), "dHi_dy_a": ("a", 0, _F64), "dHs_dx_a": ("a", 0, _F64), "dHs_dy_a": ("a", 0, _F64),
     "dHs_dx_cx": ("cx", 0, _F64), "dHs_dy_cx": ("cx", 0, _F64), "dHs_dx_cy": ("cy", 0, _F64),
     "dHs_dy_cy": ("cy", 0, _F64),
+    "float_margin_frac_a": ("a", 0, _F64), "Hi_eff_cf_a": ("a", 0, _F64),
 }
 
 

@@ -187,6 +187,10 @@ typedef struct imau_cfg {
   double  stall_drop;      /* > 1                                            */
   double  stall_relax;     /* DIVA_visc_it_relax of a restarted member's
                               unmixed Picard steps (0 = DIVA_visc_it_relax)  */
+  int32_t mg_graph;        /* 1: each multigrid V-cycle is replayed from a HIP
+                              graph captured on its first use in a solve (one
+                              launch instead of one per kernel); 0: launched
+                              kernel by kernel.  Identical results          */
 } imau_cfg;
 
 /* The fields of type_ice_model (src/data_types_module.f90:56-325) that the
@@ -422,6 +426,9 @@ typedef struct imau_producer_io {
   double  *A_flow_vav_cx, *A_flow_vav_cy;
   double  *dHi_dx_a, *dHi_dy_a, *dHs_dx_a, *dHs_dy_a;
   double  *dHs_dx_cx, *dHs_dy_cx, *dHs_dx_cy, *dHs_dy_cy;
+  double  *float_margin_frac_a, *Hi_eff_cf_a;  /* time loop only: ice%float_margin_frac_a /
+                                                  Hi_eff_cf_a as the last update_ice_thickness
+                                                  left them (rows 1 .. ny-2) */
 } imau_producer_io;
 
 void imau_producer_default_config(imau_producer_cfg *cfg);

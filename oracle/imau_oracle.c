@@ -2654,7 +2654,11 @@ int orc_run_ice_dynamics_direct(const imau_grid *gr, const imau_cfg *c, const im
   io->stat_seconds = w.stat_seconds;
   if (!rc) {
     memcpy(Hi_end, Hi, na * sizeof(double));
-    if (geo) producer_io_copy_out(geo, &pio, na, ncx, ncy, (size_t)(ny - 1) * (nx - 1), nz);
+    if (geo) {
+      producer_io_copy_out(geo, &pio, na, ncx, ncy, (size_t)(ny - 1) * (nx - 1), nz);
+      if (geo->float_margin_frac_a) memcpy(geo->float_margin_frac_a, fmf, na * sizeof(double));
+      if (geo->Hi_eff_cf_a) memcpy(geo->Hi_eff_cf_a, heff, na * sizeof(double));
+    }
   }
   if (!io->u_vav_cx) free(u_vav);
   if (!io->v_vav_cy) free(v_vav);

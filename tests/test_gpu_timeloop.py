@@ -41,7 +41,10 @@ def case(name):
     g, cfg, st = geometry.channel(nx=61, ny=21, dx=12e3)      # marine: floating front at the east edge
     config.set_config(cfg, choice_ice_dynamics="SIA/SSA" if name == "hybrid_channel" else "SSA")
     tight(cfg, st)
-    return g, cfg, st, 0.1, 20.0, "zero"
+    # 'zero' thickness boundaries leave H + dt (-H / dt) = +-1 ulp of ice where H was removed, and the
+    # sign of that residue (a mask bit) follows the last bits of dt: where a calving front meets such a
+    # boundary row, 1e-13 differences in dt flip the front's mask.  That case keeps 'infinite' ones.
+    return g, cfg, st, 0.1, 20.0, "infinite" if name == "fixed_gl_pd_channel" else "zero"
 
 
 def removal_inputs(g, st, name):
@@ -89,6 +92,8 @@ def test_time_loop_matches_oracle(ctx, name):
     assert np.abs(Hg - Hr).max() <= 1e-8 * scale, np.abs(Hg - Hr).max() / scale
     for m in ("mask_ice_a", "mask_sheet_a", "mask_shelf_a", "mask_cf_a", "mask_a"):
         np.testing.assert_array_equal(geo_g[m], geo_r[m], err_msg=m)
+    for f in ("float_margin_frac_a", "Hi_eff_cf_a"):
+        np.testing.assert_allclose(geo_g[f][1:-1], geo_r[f][1:-1], rtol=1e-8, atol=1e-8 * scale, err_msg=f)
     if gpu.stats.get("picard_iterations", 0) > 0:   # the last solve converged (a capped loop is no parity case)
         assert gpu.stats["nonlinear_residual"] <= 1e-9, gpu.stats
     u = np.abs(ref["u_vav_cx"]).max()
