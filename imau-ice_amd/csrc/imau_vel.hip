@@ -118,6 +118,11 @@ struct imau_problem {
   DBuf<int> restart_d;   // per-member flag: reset to the created state (stall restart)
   DBuf<int> stop_d;      // per-member flag: stopped at this iteration's residual check
   DBuf<int> lin_d;       // per-member flag of the linear solve's best-iterate copies
+  // DIVA_err_cx / cy (when requested): two slots [u (na)][v (na)]; Picard
+  // iteration it writes slot it & 1, slot 0 starts at 1e5, and err_slot[g]
+  // is the slot of member g's last estimate kept as output
+  DBuf<double> err;
+  std::vector<int> err_slot;
   DBuf<double> gcr_dirs; // GCR directions Q_i, Z_i (allocated on first use)
   DBuf<double> and_slots;  // Anderson slots U_s, F_s (allocated on first use)
   DBuf<double> and_w;      // Anderson weights per member
@@ -586,6 +591,10 @@ void upload(imau_ctx *c, imau_problem *P, const imau_cfg *cfg, const imau_ice_io
     const bool ssa = uses_ssa_field(cfg->choice_ice_dynamics);
     up2(c, P, g, KCX, ssa ? io.u_SSA_cx : io.u_vav_cx, P->u0, d);
     up2(c, P, g, KCY, ssa ? io.v_SSA_cy : io.v_vav_cy, P->v0, d);
+    if (ssa && P->err.p) {  // ice%u_vav_cx of the state: read by the SSA error estimate (:1626)
+      up2(c, P, g, KCX, io.u_vav_cx, P->O.uvav, d);
+      up2(c, P, g, KCY, io.v_vav_cy, P->O.vvav, d);
+    }
     up2(c, P, g, KCX, io.taub_cx, P->tbx0, d);
     up2(c, P, g, KCY, io.taub_cy, P->tby0, d);
     up2(c, P, g, KA, io.N_a, f2p(P, F_N_a), d);              // BC margin keeps stale N_a of ice-free cells
@@ -811,6 +820,15 @@ int imau_vel_problem_create(imau_ctx *ctx, int32_t ngrids, const imau_grid *grid
   HIPCHK(hipMemcpy(P->yg.p, ys.data(), ys.size() * sizeof(double), hipMemcpyHostToDevice));
   build_mg(P);
   wire(P);
+  // estimate_visc_iter_UV_errors is computed when some member asks for it
+  bool want_err = false;
+  for (int g = 0; g < ngrids; ++g) want_err = want_err || ios[g].DIVA_err_cx || ios[g].DIVA_err_cy;
+  if (want_err) {
+    P->err.alloc((size_t)4 * P->na);
+    P->err_slot.assign(ngrids, 0);
+    P->F.chk_u = P->O.uvav;
+    P->F.chk_v = P->O.vvav;
+  }
   upload(ctx, P, cfg, ios);
   P->dyn_uploaded = cfg->choice_ice_dynamics;
   P->has_ti = cfg->include_basal_freezing != 0;
@@ -1177,6 +1195,9 @@ struct Solver {
   };
   std::vector<VGraph> vgraphs_;
   ~Solver() {
+    // copies from the host flag vectors below and graph launches may still be
+    // queued when a solve ends on an error
+    (void)hipStreamSynchronize(c->s);
     for (VGraph &g : vgraphs_) (void)hipGraphExecDestroy(g.exec);
   }
   void precondition(const imau::Coef &cf, bool use_mg, bool fp32, const double *yu, const double *yv, double *zu,
@@ -1482,6 +1503,10 @@ struct Solver {
     imau::setup1(L, kp, P->F, P->yg.p);
     imau::setup2(L, kp, P->F);
     imau::mg_plain_flags(L, P->F.rt_u, P->F.rt_v, P->plan);
+    if (P->err.p) {  // :176-177 / :303-304
+      imau::fill_f64(c->s, P->err.p, 2 * P->na, 1e5);
+      std::fill(P->err_slot.begin(), P->err_slot.end(), 0);
+    }
     kcheck();
     if (!P->plan.levels.empty()) {  // the length of the non-plain list, once per solve
       HIPCHK(hipMemcpyAsync(&P->plan.nlist, P->plan.nlist_d, sizeof(int), hipMemcpyDeviceToHost, c->s));
@@ -1598,12 +1623,17 @@ struct Solver {
       if (relaxed_)  // relax_h_ is not modified before the next stream synchronisation
         HIPCHK(hipMemcpyAsync(P->relax_d.p, relax_h_.data(), G * sizeof(double), hipMemcpyHostToDevice, c->s));
       if (and_m > 0) and_step(mix, and_m);
+      if (P->err.p) {  // this iteration's error estimate goes to slot it & 1
+        P->F.errt_u = P->err.p + (size_t)(it & 1) * 2 * P->na;
+        P->F.errt_v = P->F.errt_u + P->na;
+      }
       imau::post(L, kp, P->F, P->F.xu0, P->F.xv0, P->act_pic.p, diva, P->partial.p, relaxed_ ? P->relax_d.p : nullptr,
                  mix);
       imau::member_sum3(c->s, G, P->tstart_d.p, P->partial.p, P->msum.p);
       kcheck();
       HIPCHK(hipMemcpyAsync(host4.data(), P->msum.p, 3 * G * sizeof(double), hipMemcpyDeviceToHost, c->s));
       HIPCHK(hipStreamSynchronize(c->s));
+      const std::vector<int> ran = act_pic;
       for (int g = 0; g < G; ++g) {
         if (!act_pic[g]) continue;
         P->picard_its[g] = it;
@@ -1615,6 +1645,9 @@ struct Solver {
         if (cfg->stop_rule == IMAU_STOP_PICARD && (res < cfg->DIVA_visc_it_norm_dUV_tol || it >= cfg->DIVA_visc_it_nit))
           act_pic[g] = 0;
       }
+      if (P->err.p)  // :355: the estimate of an iteration that does not end the loop is kept
+        for (int g = 0; g < G; ++g)
+          if (ran[g] && (cfg->stop_rule != IMAU_STOP_PICARD || act_pic[g])) P->err_slot[g] = it & 1;
     }
     if (v32) imau::to_f64(c->s, cur32, P->visc_cur, n3);  // the solve's viscosity output in FP64
   }
@@ -1783,6 +1816,11 @@ int imau_vel_problem_download(imau_ctx *ctx, imau_problem *P, imau_ice_io *ios) 
       down2(ctx, P, g, KA, io.DIVA_isfront_inner, P->F.fin, q);
       down2(ctx, P, g, KA, io.DIVA_isfront_outer, P->F.fout, q);
       down2(ctx, P, g, KCX, io.u_cx_prev, P->F.u_prev, d);
+      if (P->err.p) {
+        const double *e = P->err.p + (size_t)P->err_slot[g] * 2 * P->na;
+        down2(ctx, P, g, KCX, io.DIVA_err_cx, e, d);
+        down2(ctx, P, g, KCY, io.DIVA_err_cy, e + P->na, d);
+      }
       down2(ctx, P, g, KCY, io.v_cy_prev, P->F.v_prev, d);
     }
     io.stat_picard_iterations = P->picard_its[g];

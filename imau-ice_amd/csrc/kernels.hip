@@ -955,6 +955,8 @@ __global__ void __launch_bounds__(IMAU_WG) k_post(const Member *__restrict__ mem
         }
         f.u[p] = un;
         f.u_prev[p] = uo;
+        if (f.errt_u)  // estimate_visc_iter_UV_errors, :1626-1631
+          f.errt_u[p] = fabs(diva ? un : f.chk_u[p]) > 1e-6 ? 2.0 * fabs(un - uo) / fabs(un + uo + 1e-5) : 0.0;
         if (fabs(un) > 1e-6) {
           acc[0] += (un - uo) * (un - uo);
           acc[1] += (un + uo) * (un + uo);
@@ -995,6 +997,8 @@ __global__ void __launch_bounds__(IMAU_WG) k_post(const Member *__restrict__ mem
         }
         f.v[p] = vn;
         f.v_prev[p] = vo;
+        if (f.errt_v)  // :1633-1638
+          f.errt_v[p] = fabs(diva ? vn : f.chk_v[p]) > 1e-6 ? 2.0 * fabs(vn - vo) / fabs(vn + vo + 1e-5) : 0.0;
         if (fabs(vn) > 1e-6) {
           acc[0] += (vn - vo) * (vn - vo);
           acc[1] += (vn + vo) * (vn + vo);
@@ -1637,6 +1641,11 @@ __global__ void __launch_bounds__(IMAU_WG) k_sia(const Member *__restrict__ mem,
   }
 }
 
+__global__ void k_fill_f64(double *p, long long n, double v) {
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
+    p[q] = v;
+}
+
 // layout transposes between the host (k fastest, leading dimension ldj)
 // and the device (k-plane major, leading dimension ny) for ni x nj columns
 __global__ void k_h2d_3d(const Member *__restrict__ mem, int g, const double *__restrict__ src, double *__restrict__ dst,
@@ -1808,6 +1817,10 @@ void copy3_members(const Launch &L, int nz, const double *src, double *dst, cons
 }
 void copy3_members(const Launch &L, int nz, const float *src, float *dst, const int *flag) {
   hipLaunchKernelGGL(k_copy3_members<float>, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, src, dst, nz, flag);
+}
+void fill_f64(hipStream_t s, double *p, long long n, double v) {
+  const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_f64, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, s, p, n, v);
 }
 void fill_i32(hipStream_t s, int *p, int n, int v) {
   hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);

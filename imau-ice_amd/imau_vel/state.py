@@ -50,6 +50,7 @@ FIELDS = {
     "overburden_pressure_a": ("a", 0, _F64), "pore_water_pressure_a": ("a", 0, _F64), "Neff_a": ("a", 0, _F64),
     "DIVA_mask_combi": ("a", 0, _I32), "DIVA_isfront_inner": ("a", 0, _I32), "DIVA_isfront_outer": ("a", 0, _I32),
     "u_cx_prev": ("cx", 0, _F64), "v_cy_prev": ("cy", 0, _F64),
+    "DIVA_err_cx": ("cx", 0, _F64), "DIVA_err_cy": ("cy", 0, _F64),
 }
 
 STATE_FIELDS = ("u_vav_cx", "v_vav_cy", "u_SSA_cx", "v_SSA_cy", "taub_cx", "taub_cy", "visc_eff_3D_a")

@@ -228,6 +228,10 @@ typedef struct imau_ice_io {
   double *overburden_pressure_a, *pore_water_pressure_a, *Neff_a;
   int32_t *DIVA_mask_combi, *DIVA_isfront_inner, *DIVA_isfront_outer;
   double *u_cx_prev, *v_cy_prev;
+  double *DIVA_err_cx, *DIVA_err_cy;   /* estimate_visc_iter_UV_errors (src/ice_velocity_module.f90:1602-1652):
+                                          1e5 at the start of a solve, then after every Picard iteration
+                                          that does not end the loop 2|u-u_prev|/|u+u_prev+1e-5| where
+                                          |u_vav| > 1e-6, else 0.  Computed only when requested at create */
   /* ---- per-call solver tolerances (ice%DIVA_SOR_*, ice%DIVA_PETSc_*,
    *      src/data_types_module.f90:256-260; set by the caller as in
    *      src/ice_dynamics_module.f90:94-99) ---- */

@@ -1763,6 +1763,26 @@ static int check_roughness(const imau_cfg *c)
 }
 
 /* src/ice_velocity_module.f90:249-374 */
+/* estimate_visc_iter_UV_errors, src/ice_velocity_module.f90:1602-1652: the
+ * relative change of this iteration where the velocity chk (ice%u_vav_cx)
+ * exceeds 1e-6 m/yr */
+static void visc_iter_errors(const S *s, const double *chk_u, const double *chk_v, imau_ice_io *io)
+{
+  if (io->DIVA_err_cx)
+    for (size_t p = 0; p < s->ncx; ++p)
+      io->DIVA_err_cx[p] = fabs(chk_u ? chk_u[p] : 0.0) > 1e-6
+                               ? 2.0 * fabs(s->u[p] - s->u_prev[p]) / fabs(s->u[p] + s->u_prev[p] + 1e-5) : 0.0;
+  if (io->DIVA_err_cy)
+    for (size_t p = 0; p < s->ncy; ++p)
+      io->DIVA_err_cy[p] = fabs(chk_v ? chk_v[p] : 0.0) > 1e-6
+                               ? 2.0 * fabs(s->v[p] - s->v_prev[p]) / fabs(s->v[p] + s->v_prev[p] + 1e-5) : 0.0;
+}
+static void visc_iter_errors_init(const S *s, imau_ice_io *io)  /* :176-177, :303-304 */
+{
+  if (io->DIVA_err_cx) for (size_t p = 0; p < s->ncx; ++p) io->DIVA_err_cx[p] = 1e5;
+  if (io->DIVA_err_cy) for (size_t p = 0; p < s->ncy; ++p) io->DIVA_err_cy[p] = 1e5;
+}
+
 int orc_solve_diva(const imau_grid *gr, const imau_cfg *c, imau_ice_io *io, int lin)
 {
   if (c->choice_ice_dynamics != IMAU_DYN_DIVA) return fail(IMAU_ERR_CONFIG, "solve_DIVA: choice_ice_dynamics is not DIVA");
@@ -1781,6 +1801,7 @@ int orc_solve_diva(const imau_grid *gr, const imau_cfg *c, imau_ice_io *io, int 
   calc_taud(&s);
   calc_basal_conditions(&s);
   determine_grounded_fractions(g, io->TAF_a, s.fg_a, s.fg_cx, s.fg_cy, s.fg_b);
+  visc_iter_errors_init(&s, io);
   int it = 0, rc = 0;
   double resid = 0.0;
   /* under the residual rule the state returned is the one whose residual was
@@ -1816,6 +1837,7 @@ int orc_solve_diva(const imau_grid *gr, const imau_cfg *c, imau_ice_io *io, int 
       if (resid < c->DIVA_visc_it_norm_dUV_tol) conv = 1;
       else if (it >= c->DIVA_visc_it_nit) conv = 1;
     }
+    if (!conv) visc_iter_errors(&s, s.u, s.v, io);  /* :355 */
     calc_basal_stress(&s);
     calc_basal_velocities(&s);
     if (conv) break;
@@ -1861,6 +1883,7 @@ int orc_solve_ssa(const imau_grid *gr, const imau_cfg *c, imau_ice_io *io, int l
   calc_basal_conditions(&s);
   determine_grounded_fractions(g, io->TAF_a, s.fg_a, s.fg_cx, s.fg_cy, s.fg_b);
   /* SSA: no vertical shear (du_dz stays zero) */
+  visc_iter_errors_init(&s, io);
   int it = 0, rc = 0;
   double resid = 0.0;
   double *visc_keep = (c->stop_rule == IMAU_STOP_RESIDUAL) ? dalloc(g->nz * s.na) : NULL;
@@ -1891,6 +1914,7 @@ int orc_solve_ssa(const imau_grid *gr, const imau_cfg *c, imau_ice_io *io, int l
     relax(&s, s.u, s.v, c->DIVA_visc_it_relax);
     resid = visc_iter_resid(&s, s.u, s.v);
     if (c->stop_rule == IMAU_STOP_PICARD && (resid < c->DIVA_visc_it_norm_dUV_tol || it >= c->DIVA_visc_it_nit)) break;
+    visc_iter_errors(&s, io->u_vav_cx, io->v_vav_cy, io);  /* :238, the test on the state's u_vav */
   }
   free(visc_keep);
   io->stat_picard_iterations = it;

@@ -303,3 +303,84 @@ def test_periodic_slab_against_oracle_and_its_period(ctx, axis):
     u = st["u_SSA_cx"] if axis == "x" else st["v_SSA_cy"].T
     n = u.shape[1]
     assert np.abs(u[1:-1, Pn:] - u[1:-1, :n - Pn]).max() <= 1e-7 * np.abs(u).max()
+
+
+def _visc_errs(u_new, u_old, chk):
+    """estimate_visc_iter_UV_errors (src/ice_velocity_module.f90:1602-1652)."""
+    return np.where(np.abs(chk) > 1e-6, 2.0 * np.abs(u_new - u_old) / np.abs(u_new + u_old + 1e-5), 0.0)
+
+
+@pytest.mark.parametrize("name", ["no_sliding", "schoof_mismip", "ssa_dome"])
+def test_visc_iter_error_estimate(ctx, name):
+    """DIVA_err_cx / cy after K fixed Picard iterations of the reference loop
+    hold iterate K-1 against K-2 (:355; :238 for solve_SSA, whose tolerance
+    test reads the state's u_vav_cx).  Against the GPU's own run of K-1
+    iterations (which ends on those two iterates) to rounding; against the
+    oracle cell by cell within the bound the formula propagates from the 1e-6
+    velocity bar delta: |d err| <= 2 delta (2 + err) / |u_new + u_old + 1e-5|."""
+    kind, g, cfg, st = B.build(name)
+    K = 6
+    tight_linear(st)
+    u, v = vel_fields(kind)
+    if kind == "ssa":     # the state's vertical average: zero on half of each grid
+        st["u_vav_cx"] = 1.0
+        st["u_vav_cx"][: g.ny // 2] = 0.0
+        st["v_vav_cy"] = 1.0
+        st["v_vav_cy"][:, : g.nx // 2] = 0.0
+    runs = {}
+    for k in (K - 1, K):
+        config.set_config(cfg, stop_rule=0, DIVA_visc_it_nit=k, DIVA_visc_it_norm_dUV_tol=0.0)
+        gpu, ref = st.copy(), st.copy()
+        gpu_solve(ctx, kind, g, cfg, gpu)
+        O.solve(kind, g, cfg, ref, O.LIN_BANDED_LU)
+        assert gpu.stats["picard_iterations"] == ref.stats["picard_iterations"] == k
+        runs[k] = gpu, ref
+    (g1, r1), (gK, rK) = runs[K - 1], runs[K]
+    delta = 1e-6 * max(np.abs(r1[u]).max(), np.abs(r1[v]).max())
+    for e, f, p, chk in (("DIVA_err_cx", u, "u_cx_prev", "u_vav_cx"), ("DIVA_err_cy", v, "v_cy_prev", "v_vav_cy")):
+        cg = g1[f] if kind == "diva" else st[chk]
+        np.testing.assert_allclose(gK[e], _visc_errs(g1[f], g1[p], cg), rtol=1e-13, atol=0.0, err_msg=e)
+        cr = r1[f] if kind == "diva" else st[chk]
+        np.testing.assert_allclose(rK[e], _visc_errs(r1[f], r1[p], cr), rtol=1e-13, atol=0.0, err_msg=e)
+        s = np.abs(r1[f] + r1[p] + 1e-5)
+        bound = 2.0 * delta * (2.0 + rK[e]) / np.maximum(s, 1e-300)
+        away = np.abs(np.abs(cr) - 1e-6) > (delta if kind == "diva" else 0.0)   # off the test's threshold
+        bad = away & (np.abs(gK[e] - rK[e]) > bound)
+        assert not bad.any(), (e, np.argwhere(bad)[:5], gK[e][bad][:5], rK[e][bad][:5])
+        assert (rK[e][away] > 0).any() and (rK[e][away] < 1e5).all()
+        if kind == "ssa":
+            assert (rK[e][np.abs(cr) <= 1e-6] == 0).all() and (np.abs(r1[f][np.abs(cr) <= 1e-6]) > 1e-3).any()
+
+
+def test_visc_iter_error_estimate_of_a_batch(ctx):
+    """Members of one batch that leave the reference Picard loop at different
+    iterations keep the estimate of their own last continuing iteration: each
+    equals its single solve; under the residual rule the estimate is the
+    final iterate against u_cx_prev."""
+    from imau_vel import geometry, solver
+    cases = [geometry.dome(), geometry.random_geometry(45, 37, seed=4)]
+    cfg = cases[0][1]
+    config.set_config(cfg, stop_rule=0, DIVA_visc_it_norm_dUV_tol=1e-2, DIVA_visc_it_nit=50)
+    singles = []
+    for g, _, st in cases:
+        s = st.copy()
+        ctx.solve_diva(g, cfg, s)
+        singles.append(s)
+    assert singles[0].stats["picard_iterations"] != singles[1].stats["picard_iterations"]
+    batch = [st.copy() for _, _, st in cases]
+    prob = solver.Problem(ctx, [c[0] for c in cases], cfg, batch)
+    try:
+        prob.solve()
+        prob.download()
+    finally:
+        prob.close()
+    for s1, sb in zip(singles, batch):
+        assert s1.stats["picard_iterations"] == sb.stats["picard_iterations"]
+        for e in ("DIVA_err_cx", "DIVA_err_cy"):
+            np.testing.assert_array_equal(s1[e], sb[e], err_msg=e)
+            assert (s1[e] < 1e5).all()
+    config.set_config(cfg, stop_rule=1, nonlinear_rtol=1e-6, DIVA_visc_it_nit=300)
+    g, _, st = cases[0]
+    ctx.solve_diva(g, cfg, st)
+    for e, f, p in (("DIVA_err_cx", "u_vav_cx", "u_cx_prev"), ("DIVA_err_cy", "v_vav_cy", "v_cy_prev")):
+        np.testing.assert_allclose(st[e], _visc_errs(st[f], st[p], st[f]), rtol=1e-13, atol=0.0, err_msg=e)

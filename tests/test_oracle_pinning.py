@@ -645,3 +645,69 @@ def test_zero_boundary_wins_the_corner():
     assert u[-1, e] == u[-2, e - 1]
     np.testing.assert_array_equal(u[1:-1, e], u[1:-1, e - 1])
     assert np.abs(u[-1]).max() > 0.0
+
+
+# ---------------------------------------------------------------- Picard error estimate
+def _visc_errs(u_new, u_old, chk):
+    """2 |u_new - u_old| / |u_new + u_old + 1e-5| where |chk| > 1e-6, else 0."""
+    return np.where(np.abs(chk) > 1e-6, 2.0 * np.abs(u_new - u_old) / np.abs(u_new + u_old + 1e-5), 0.0)
+
+
+def test_visc_iter_errors_pair_consecutive_iterates():
+    """estimate_visc_iter_UV_errors (src/ice_velocity_module.f90:1602-1652) runs
+    after every Picard iteration that does not end the loop (:355): the
+    estimate left by a run of K iterations compares iterate K-1 with K-2.
+    The iterates come from independent runs of 1, 2 and 3 iterations of the
+    same deterministic loop; a run of one iteration leaves the 1e5 set at the
+    solve's start (:176-177)."""
+    g, cfg, st = geometry.dome(nx=17, ny=15, dx=50e3)
+    st["u_vav_cx"] = 1e-3 * np.sin(np.arange(st["u_vav_cx"].size)).reshape(st["u_vav_cx"].shape, order="F")
+    runs = {}
+    for K in (1, 2, 3):
+        config.set_config(cfg, stop_rule=0, DIVA_visc_it_norm_dUV_tol=0.0, DIVA_visc_it_nit=K)
+        runs[K] = O.solve("diva", g, cfg, st.copy())
+        assert runs[K].stats["picard_iterations"] == K
+    assert np.all(runs[1]["DIVA_err_cx"] == 1e5) and np.all(runs[1]["DIVA_err_cy"] == 1e5)
+    iterate = {0: st, 1: runs[1], 2: runs[2]}
+    for K in (2, 3):
+        new, old = iterate[K - 1], iterate[K - 2]
+        np.testing.assert_array_equal(runs[K]["u_cx_prev"], new["u_vav_cx"])
+        for e, u in (("DIVA_err_cx", "u_vav_cx"), ("DIVA_err_cy", "v_vav_cy")):
+            want = _visc_errs(new[u], old[u], new[u])
+            np.testing.assert_allclose(runs[K][e], want, rtol=1e-14, atol=0.0, err_msg="%s K=%d" % (e, K))
+    assert (runs[3]["DIVA_err_cx"] > 0.0).all() and (runs[3]["DIVA_err_cx"] < 1e5).all()
+
+
+def test_visc_iter_errors_of_ssa_test_the_stale_vertical_average():
+    """In solve_SSA the tolerance test reads ice%u_vav_cx, which the SSA loop
+    does not update (:238, the estimate of u_SSA_cx): cells where the state's
+    u_vav_cx is below 1e-6 get 0 whatever the SSA velocity does."""
+    import branch_cases as B
+    kind, g, cfg, st = B.build("ssa_dome")
+    half = g.ny // 2
+    st["u_vav_cx"] = 1.0
+    st["u_vav_cx"][:half] = 0.0
+    st["v_vav_cy"] = 1.0
+    st["v_vav_cy"][:, : g.nx // 2] = 0.0
+    runs = {}
+    for K in (1, 2):
+        config.set_config(cfg, stop_rule=0, DIVA_visc_it_norm_dUV_tol=0.0, DIVA_visc_it_nit=K)
+        runs[K] = O.solve("ssa", g, cfg, st.copy())
+    new = runs[1]
+    for e, u, chk in (("DIVA_err_cx", "u_SSA_cx", st["u_vav_cx"]), ("DIVA_err_cy", "v_SSA_cy", st["v_vav_cy"])):
+        np.testing.assert_allclose(runs[2][e], _visc_errs(new[u], st[u], chk), rtol=1e-14, atol=0.0, err_msg=e)
+    assert np.all(runs[2]["DIVA_err_cx"][:half] == 0.0)
+    assert (np.abs(new["u_SSA_cx"][:half]) > 1e-3).any()         # moving ice in the masked half
+    assert (runs[2]["DIVA_err_cx"][half:] > 0.0).any()
+
+
+def test_visc_iter_errors_under_the_residual_rule():
+    """The residual stop rule ends the loop on the nonlinear residual, not on
+    the reference's has_converged: every iteration, the last one included,
+    leaves its estimate (final iterate against u_cx_prev)."""
+    g, cfg, st = geometry.dome(nx=17, ny=15, dx=50e3)
+    config.set_config(cfg, stop_rule=1, nonlinear_rtol=1e-6, DIVA_visc_it_nit=200)
+    O.solve("diva", g, cfg, st)
+    assert st.stats["picard_iterations"] > 2
+    for e, u, p in (("DIVA_err_cx", "u_vav_cx", "u_cx_prev"), ("DIVA_err_cy", "v_vav_cy", "v_cy_prev")):
+        np.testing.assert_allclose(st[e], _visc_errs(st[u], st[p], st[u]), rtol=1e-14, atol=0.0, err_msg=e)
