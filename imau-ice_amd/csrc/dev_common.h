@@ -147,6 +147,7 @@ struct Fields {
   // linear solve
   double *xu0, *xv0, *xu1, *xv1;       // double-buffered iterate
   double *u_prev, *v_prev, *u_base, *v_base;
+  const double *ismip_e;               // ISMIP-HOM E beta per global column (offx + i): 0 or 1e30
   double *errt_u, *errt_v;             // this iteration's error estimate (null: not requested)
   const double *chk_u, *chk_v;         // the velocities its |u| > 1e-6 test reads (SSA: u_vav of the state)
   double *eps_h;                       // horizontal strain-rate invariant on the a-grid (k_strain -> k_visc)

@@ -104,6 +104,8 @@ struct imau_problem {
   DBuf<Tile> tiles_d;
   DBuf<int> tstart_d;
   DBuf<double> xg, yg;
+  DBuf<double> ismip_e;          // ISMIP-HOM E beta per global column (offx + i)
+  std::vector<char> has_slides;  // member g was created with ISMIP_HOM_E_slides
   DBuf<double> f2;    // all double 2D fields (arena)
   DBuf<int32_t> i2;   // int 2D fields
   DBuf<uint8_t> rt;   // row types
@@ -393,10 +395,9 @@ void check_cfg(const imau_cfg *cfg) {
     fail(IMAU_ERR_CONFIG, "unknown choice_ice_margin %d", cfg->choice_ice_margin);
   if (cfg->choice_sliding_law < 0 || cfg->choice_sliding_law > 7)
     fail(IMAU_ERR_CONFIG, "unknown choice_sliding_law %d", cfg->choice_sliding_law);
-  if (cfg->choice_sliding_law == IMAU_SLID_IDEALISED && cfg->choice_idealised_sliding_law != IMAU_IDEAL_ISMIP_HOM_C &&
-      cfg->choice_idealised_sliding_law != IMAU_IDEAL_ISMIP_HOM_D && cfg->choice_idealised_sliding_law != IMAU_IDEAL_ISMIP_HOM_F)
-    fail(IMAU_ERR_CONFIG, "unsupported choice_idealised_sliding_law %d (ISMIP_HOM_E reads a data file)",
-         cfg->choice_idealised_sliding_law);
+  if (cfg->choice_sliding_law == IMAU_SLID_IDEALISED &&
+      (cfg->choice_idealised_sliding_law < IMAU_IDEAL_ISMIP_HOM_C || cfg->choice_idealised_sliding_law > IMAU_IDEAL_ISMIP_HOM_F))
+    fail(IMAU_ERR_CONFIG, "unknown choice_idealised_sliding_law %d", cfg->choice_idealised_sliding_law);
   if (cfg->choice_basal_hydrology < 0 || cfg->choice_basal_hydrology > 2)
     fail(IMAU_ERR_CONFIG, "unknown choice_basal_hydrology %d", cfg->choice_basal_hydrology);
   if (cfg->choice_basal_roughness < 0 || cfg->choice_basal_roughness > 2)
@@ -549,6 +550,20 @@ void wire(imau_problem *P) {
 }
 
 bool uses_ssa_field(int dyn) { return dyn == IMAU_DYN_SSA || dyn == IMAU_DYN_SIA_SSA; }
+
+// ISMIP-HOM E reads its slip columns from the 51-line Arolla file
+// (src/basal_conditions_and_sliding_module.f90:1372-1389)
+void check_ismip_e(const imau_problem *P, const imau_cfg *cfg) {
+  if (cfg->choice_sliding_law != IMAU_SLID_IDEALISED || cfg->choice_idealised_sliding_law != IMAU_IDEAL_ISMIP_HOM_E)
+    return;
+  for (int g = 0; g < P->ngrids; ++g) {
+    if (!P->has_slides[g])
+      fail(IMAU_ERR_ARG, "member %d: ISMIP_HOM_E needs ISMIP_HOM_E_slides at create", g);
+    if (P->mem[g].nx != 51)
+      fail(IMAU_ERR_CONFIG, "member %d: ISMIP_HOM_E: the Arolla file has 51 lines, one per grid column (nx=%d)", g,
+           P->mem[g].nx);
+  }
+}
 
 void upload(imau_ctx *c, imau_problem *P, const imau_cfg *cfg, const imau_ice_io *ios) {
   const int nz = P->nz;
@@ -797,6 +812,17 @@ int imau_vel_problem_create(imau_ctx *ctx, int32_t ngrids, const imau_grid *grid
   P->tiles_d.alloc(P->tiles.size());
   P->tstart_d.alloc(P->tstart.size());
   P->xg.alloc(xs.size());
+  {
+    std::vector<double> eb(xs.size(), 1e30);
+    P->has_slides.assign(ngrids, 0);
+    for (int g = 0; g < ngrids; ++g)
+      if (const int32_t *sl = ios[g].ISMIP_HOM_E_slides) {
+        P->has_slides[g] = 1;
+        for (int i = 0; i < P->mem[g].nx; ++i) eb[P->mem[g].offx + i] = sl[i] == 1 ? 0.0 : 1e30;
+      }
+    P->ismip_e.alloc(eb.size());
+    HIPCHK(hipMemcpy(P->ismip_e.p, eb.data(), eb.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   P->yg.alloc(ys.size());
   P->f2.alloc((size_t)F2_COUNT * P->na);
   P->i2.alloc((size_t)I2_COUNT * P->na);
@@ -820,6 +846,7 @@ int imau_vel_problem_create(imau_ctx *ctx, int32_t ngrids, const imau_grid *grid
   HIPCHK(hipMemcpy(P->yg.p, ys.data(), ys.size() * sizeof(double), hipMemcpyHostToDevice));
   build_mg(P);
   wire(P);
+  P->F.ismip_e = P->ismip_e.p;
   // estimate_visc_iter_UV_errors is computed when some member asks for it
   bool want_err = false;
   for (int g = 0; g < ngrids; ++g) want_err = want_err || ios[g].DIVA_err_cx || ios[g].DIVA_err_cy;
@@ -830,6 +857,12 @@ int imau_vel_problem_create(imau_ctx *ctx, int32_t ngrids, const imau_grid *grid
     P->F.chk_v = P->O.vvav;
   }
   upload(ctx, P, cfg, ios);
+  if (want_err)  // slot 0 starts with the caller's values (kept by a member that never enters its loop)
+    for (int g = 0; g < ngrids; ++g) {
+      up2(ctx, P, g, KCX, ios[g].DIVA_err_cx, P->err.p, sizeof(double));
+      up2(ctx, P, g, KCY, ios[g].DIVA_err_cy, P->err.p + P->na, sizeof(double));
+    }
+  HIPCHK(hipStreamSynchronize(ctx->s));  // the caller's arrays may go once create returns
   P->dyn_uploaded = cfg->choice_ice_dynamics;
   P->has_ti = cfg->include_basal_freezing != 0;
   *prob = P;
@@ -1503,9 +1536,11 @@ struct Solver {
     imau::setup1(L, kp, P->F, P->yg.p);
     imau::setup2(L, kp, P->F);
     imau::mg_plain_flags(L, P->F.rt_u, P->F.rt_v, P->plan);
-    if (P->err.p) {  // :176-177 / :303-304
-      imau::fill_f64(c->s, P->err.p, 2 * P->na, 1e5);
-      std::fill(P->err_slot.begin(), P->err_slot.end(), 0);
+    if (P->err.p) {  // :176-177 / :303-304, after the early return of a member without grounded ice
+      push_active(P->act_pic.p, act_pic);
+      imau::fill2_members(L, P->err.p, P->err.p + P->na, 1e5, P->act_pic.p);
+      for (int g = 0; g < G; ++g)
+        if (act_pic[g]) P->err_slot[g] = 0;
     }
     kcheck();
     if (!P->plan.levels.empty()) {  // the length of the non-plain list, once per solve
@@ -1676,6 +1711,7 @@ int solve_kind(imau_ctx *ctx, imau_problem *P, const imau_cfg *cfg, int kinds) {
   if (!ctx || !P) fail(IMAU_ERR_ARG, "NULL argument");
   check_cfg(cfg);
   if (cfg->nz != P->nz) fail(IMAU_ERR_CONFIG, "cfg nz=%d differs from the problem's nz=%d", cfg->nz, P->nz);
+  check_ismip_e(P, cfg);
   if (cfg->include_basal_freezing && !P->has_ti) fail(IMAU_ERR_CONFIG, "include_basal_freezing needs Ti_a/Ti_pmp_a at create");
   const int dyn = cfg->choice_ice_dynamics;
   auto t0 = std::chrono::steady_clock::now();
@@ -1904,6 +1940,7 @@ int imau_vel_problem_autotune(imau_ctx *ctx, imau_problem *P, const imau_cfg *cf
   if (!ctx || !P || !cfg) fail(IMAU_ERR_ARG, "NULL argument");
   check_cfg(cfg);
   if (cfg->nz != P->nz) fail(IMAU_ERR_CONFIG, "cfg nz=%d differs from the problem's nz=%d", cfg->nz, P->nz);
+  check_ismip_e(P, cfg);
   const bool diva = cfg->choice_ice_dynamics == IMAU_DYN_DIVA;
   if (!diva || cfg->nz != IMAU_MNZ) {  // only the DIVA viscosity at nz = 15 has a tunable tile
     if (tile_rows) *tile_rows = P->mti;
@@ -1967,6 +2004,7 @@ int imau_vel_problem_residual_noalloc(imau_ctx *ctx, imau_problem *P, const imau
   if (!ctx || !P || !cfg || !residual_dev) fail(IMAU_ERR_ARG, "NULL argument");
   check_cfg(cfg);
   if (cfg->nz != P->nz) fail(IMAU_ERR_CONFIG, "cfg nz=%d differs from the problem's nz=%d", cfg->nz, P->nz);
+  check_ismip_e(P, cfg);
   const int dyn = cfg->choice_ice_dynamics;
   if (dyn != IMAU_DYN_DIVA && !uses_ssa_field(dyn))
     fail(IMAU_ERR_CONFIG, "the nonlinear residual is defined for DIVA, SSA and SIA/SSA only");

@@ -77,7 +77,8 @@ struct ResetSrc {
 void reset_members(const Launch &L, const KP &kp, const Fields &f, const ResetSrc &src, double *visc, float *visc32,
                    const int *flag);
 void fill_i32(hipStream_t s, int *p, int n, int v);
-void fill_f64(hipStream_t s, double *p, long long n, double v);
+// (du, dv) = v on the 2D slabs of the members with flag[g] != 0
+void fill2_members(const Launch &L, double *du, double *dv, double v, const int *flag);
 // r[g] = sqrt(sums[2g]) / sqrt(sums[2g+1]) (or sqrt(sums[2g]) when the second is 0)
 void resid_ratio(hipStream_t s, int ngrids, const double *sums, double *r);
 void F1(const Launch &L, const KP &kp, const Fields &f, const double *visc, const OutF &o);

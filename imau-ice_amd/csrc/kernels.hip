@@ -56,7 +56,7 @@ __device__ void block_sum_store(double (&v)[NV], double *out) {
 // specialised on it carries only that law
 __device__ __forceinline__ double sliding_beta(const KP &kp, int slid, double ua, double va, double beta_sq,
                                                double alpha_sq, double phi, double tauc, double Neff, double x,
-                                               double y, double A_vav) {
+                                               double y, double A_vav, double beta_e) {
   const double uabs = sqrt(kp.delta_v * kp.delta_v + ua * ua + va * va);
   switch (slid) {
     case 0: return 0.0;
@@ -74,6 +74,7 @@ __device__ __forceinline__ double sliding_beta(const KP &kp, int slid, double ua
     case 1:                                                                                   // idealised :1288-1432
       if (kp.ideal == 0) return 1000.0 + 1000.0 * sin(2.0 * D_PI * x / kp.ismip_L) * sin(2.0 * D_PI * y / kp.ismip_L);
       if (kp.ideal == 1) return 1000.0 + 1000.0 * sin(2.0 * D_PI * x / kp.ismip_L);
+      if (kp.ideal == 2) return beta_e;                                                      // :1352-1399
       if (kp.ideal == 3) return pow(A_vav * 1000.0, -1.0);
       return __builtin_nan("");
     default: return __builtin_nan("");
@@ -669,7 +670,8 @@ __global__ void __launch_bounds__(IMAU_WG) k_beta(const Member *__restrict__ mem
       f.tauc[p] = tauc;
     }
     double beta = sliding_beta(kp, slid, ua, va, f.beta_sq[p], f.alpha_sq[p], f.phi_fric[p], tauc, f.Neff[p],
-                               xg[m.offx + i], yg[m.offy + j], f.A_vav ? f.A_vav[p] : 0.0);
+                               xg[m.offx + i], yg[m.offy + j], f.A_vav ? f.A_vav[p] : 0.0,
+                               (slid == 1 && kp.ideal == 2) ? f.ismip_e[m.offx + i] : 0.0);
     beta = fmin(beta, kp.beta_max);
     if (!kp.glsub && f.m_ocean[p] == 1) beta = 0.0;
     f.beta_a[p] = beta;
@@ -735,7 +737,8 @@ __global__ void __launch_bounds__(IMAU_WG) k_visc_column(const Member *__restric
     if (slid == 3 || (slid == 2 && !(kp.rough == 1 && kp.prough == 1)))
       tauc = tan((D_PI / 180.0) * f.phi_fric[p]) * f.Neff[p];
     double beta = sliding_beta(kp, slid, ua, va, f.beta_sq[p], f.alpha_sq[p], f.phi_fric[p], tauc, f.Neff[p],
-                               xg[m.offx + i], yg[m.offy + j], f.A_vav ? f.A_vav[p] : 0.0);
+                               xg[m.offx + i], yg[m.offy + j], f.A_vav ? f.A_vav[p] : 0.0,
+                               (slid == 1 && kp.ideal == 2) ? f.ismip_e[m.offx + i] : 0.0);
     beta = fmin(beta, kp.beta_max);
     if (!kp.glsub && f.m_ocean[p] == 1) beta = 0.0;
     bool frozen = false;
@@ -1641,9 +1644,19 @@ __global__ void __launch_bounds__(IMAU_WG) k_sia(const Member *__restrict__ mem,
   }
 }
 
-__global__ void k_fill_f64(double *p, long long n, double v) {
-  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
-    p[q] = v;
+// (du, dv) = v on the 2D slabs of the members with flag[g] != 0
+__global__ void __launch_bounds__(IMAU_WG) k_fill2_members(const Member *__restrict__ mem,
+                                                           const Tile *__restrict__ tiles, Vec2 dst, double v,
+                                                           const int *__restrict__ flag) {
+  TILE_SETUP
+  if (j >= m.ny || !flag[t.g]) return;
+  FOR_TILE_ROWS {
+    const int i = t.i0 + ii;
+    if (i >= m.nx) break;
+    const long long p = p2(m, j, i);
+    dst.u[p] = v;
+    dst.v[p] = v;
+  }
 }
 
 // layout transposes between the host (k fastest, leading dimension ldj)
@@ -1818,9 +1831,8 @@ void copy3_members(const Launch &L, int nz, const double *src, double *dst, cons
 void copy3_members(const Launch &L, int nz, const float *src, float *dst, const int *flag) {
   hipLaunchKernelGGL(k_copy3_members<float>, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, src, dst, nz, flag);
 }
-void fill_f64(hipStream_t s, double *p, long long n, double v) {
-  const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_fill_f64, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, s, p, n, v);
+void fill2_members(const Launch &L, double *du, double *dv, double v, const int *flag) {
+  hipLaunchKernelGGL(k_fill2_members, dim3(L.ntiles), wg(), 0, L.s, L.mem, L.tiles, Vec2{du, dv}, v, flag);
 }
 void fill_i32(hipStream_t s, int *p, int n, int v) {
   hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);

@@ -20,7 +20,7 @@ SLIDING = {"no_sliding": abi.IMAU_SLID_NO_SLIDING, "idealised": abi.IMAU_SLID_ID
            "Weertman": abi.IMAU_SLID_WEERTMAN, "Tsai2015": abi.IMAU_SLID_TSAI2015,
            "Schoof2005": abi.IMAU_SLID_SCHOOF2005, "Zoet-Iverson": abi.IMAU_SLID_ZOET_IVERSON}
 IDEALISED = {"ISMIP_HOM_C": abi.IMAU_IDEAL_ISMIP_HOM_C, "ISMIP_HOM_D": abi.IMAU_IDEAL_ISMIP_HOM_D,
-             "ISMIP_HOM_F": abi.IMAU_IDEAL_ISMIP_HOM_F}
+             "ISMIP_HOM_E": abi.IMAU_IDEAL_ISMIP_HOM_E, "ISMIP_HOM_F": abi.IMAU_IDEAL_ISMIP_HOM_F}
 HYDROLOGY = {"dry": abi.IMAU_HYDRO_DRY, "saturated": abi.IMAU_HYDRO_SATURATED,
              "Martin2011": abi.IMAU_HYDRO_MARTIN2011}
 ROUGHNESS = {"uniform": abi.IMAU_ROUGH_UNIFORM, "parameterised": abi.IMAU_ROUGH_PARAMETERISED,

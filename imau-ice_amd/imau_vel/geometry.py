@@ -11,6 +11,7 @@ the velocity path is measured on:
   channel   401x41 at 2 km marine channel with Gaussian side walls
   random    n_x x n_y random geometries (Matern bed, dome + fractal noise)
   ensemble  uneven-size members drawn from `random`
+  ismip_hom_e  the ISMIP-HOM E flowline from a file in the Arolla format
 """
 from __future__ import annotations
 
@@ -182,6 +183,57 @@ def channel(nx: int = 401, ny: int = 41, dx: float = 2e3, seed: int = 0, A: floa
         prof = prof + a * np.sin(k * np.pi * xi)
     st["Hi_a"] = np.maximum(0.0, prof[None, :] - wall)
     st["SL_a"] = 0.0
+    return g, cfg, _finish(st, cfg, A)
+
+
+class ArollaFileError(ValueError):
+    """A file the reference's ISMIP-HOM E readers reject (they crash)."""
+
+
+def read_ismip_hom_e(path):
+    """The ISMIP-HOM E input file (C%ISMIP_HOM_E_Arolla_filename): 51 records
+    of `x, Hb, Hs, slides`, read list-directed as in
+    initialise_reference_geometry_idealised_ISMIP_HOM_E
+    (src/reference_fields_module.f90:751-762) and
+    calc_sliding_law_idealised_ISMIP_HOM_E
+    (src/basal_conditions_and_sliding_module.f90:1373-1389); records past
+    the 51st are not read.  Returns (x, Hb, Hs, slides) arrays of 51."""
+    rows = []
+    with open(path) as f:
+        for line in f:
+            tok = line.replace(",", " ").split()
+            if not tok:
+                continue
+            if len(rows) == 51:
+                break
+            if len(tok) < 4:
+                raise ArollaFileError('length of text file "%s" should be 51 lines!' % path)
+            rows.append((float(tok[0]), float(tok[1]), float(tok[2]), int(float(tok[3]))))
+    if len(rows) != 51:
+        raise ArollaFileError('length of text file "%s" should be 51 lines!' % path)
+    a = np.array(rows, dtype=np.float64)
+    return a[:, 0].copy(), a[:, 1].copy(), a[:, 2].copy(), a[:, 3].astype(np.int32)
+
+
+def ismip_hom_e(source, ny: int = 5, A: float = 1e-16, cfg=None):
+    """ISMIP-HOM experiment E: the flowline of `source` (a path to a file in
+    the Arolla format, or its (x, Hb, Hs, slides) arrays) on nx = 51 columns
+    (the reference forces nsx = 25, src/IMAU_ICE_main_model.f90:704) and ny
+    identical rows: Hb and Hi = Hs - Hb per column
+    (src/reference_fields_module.f90:753-758), DIVA with the ISMIP_HOM_E
+    sliding law, whose slip columns are the file's fourth column."""
+    x, Hb, Hs, slides = read_ismip_hom_e(source) if isinstance(source, (str, bytes)) or hasattr(source, "__fspath__") \
+        else (np.asarray(a) for a in source)
+    if len(x) != 51:
+        raise ArollaFileError("an ISMIP-HOM E flowline has 51 columns")
+    cfg = cfg or synthetic_config(choice_sliding_law="idealised", choice_idealised_sliding_law="ISMIP_HOM_E")
+    dx = float(x[1] - x[0])
+    g = Grid(51, ny, dx, xmin=float(x[0]), ymin=-(ny - 1) // 2 * dx)
+    st = IceState(g, cfg.nz)
+    st["Hb_a"] = np.broadcast_to(Hb[None, :], (ny, 51))
+    st["Hi_a"] = np.broadcast_to((Hs - Hb)[None, :], (ny, 51))
+    st["SL_a"] = 0.0
+    st["ISMIP_HOM_E_slides"] = slides
     return g, cfg, _finish(st, cfg, A)
 
 

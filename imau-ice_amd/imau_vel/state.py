@@ -28,7 +28,7 @@ FIELDS = {
     "Hi_cy": ("cy", 0, _F64), "dHs_dx_cy": ("cy", 0, _F64), "dHs_dy_cy": ("cy", 0, _F64),
     "Hi_b": ("b", 0, _F64),
     "A_flow_3D_a": ("a", 1, _F64), "A_flow_3D_cx": ("cx", 1, _F64), "A_flow_3D_cy": ("cy", 1, _F64),
-    "Ti_a": ("a", 1, _F64), "Ti_pmp_a": ("a", 1, _F64),
+    "Ti_a": ("a", 1, _F64), "Ti_pmp_a": ("a", 1, _F64), "ISMIP_HOM_E_slides": ("col", 0, _I32),
     "beta_sq_a": ("a", 0, _F64), "alpha_sq_a": ("a", 0, _F64), "phi_fric_a": ("a", 0, _F64), "tauc_a": ("a", 0, _F64),
     # in/out state
     "u_vav_cx": ("cx", 0, _F64), "v_vav_cy": ("cy", 0, _F64), "u_SSA_cx": ("cx", 0, _F64), "v_SSA_cy": ("cy", 0, _F64),
@@ -57,7 +57,7 @@ STATE_FIELDS = ("u_vav_cx", "v_vav_cy", "u_SSA_cx", "v_SSA_cy", "taub_cx", "taub
 
 
 def grid_shape(kind: str, nx: int, ny: int):
-    return {"a": (ny, nx), "cx": (ny, nx - 1), "cy": (ny - 1, nx), "b": (ny - 1, nx - 1)}[kind]
+    return {"a": (ny, nx), "cx": (ny, nx - 1), "cy": (ny - 1, nx), "b": (ny - 1, nx - 1), "col": (nx,)}[kind]
 
 
 class Grid:

@@ -60,7 +60,7 @@ enum {
   IMAU_SLID_SCHOOF2005 = 6, IMAU_SLID_ZOET_IVERSON = 7
 };
 /* choice_idealised_sliding_law (src/configuration_module.f90:350) */
-enum { IMAU_IDEAL_ISMIP_HOM_C = 0, IMAU_IDEAL_ISMIP_HOM_D = 1, IMAU_IDEAL_ISMIP_HOM_F = 3 };
+enum { IMAU_IDEAL_ISMIP_HOM_C = 0, IMAU_IDEAL_ISMIP_HOM_D = 1, IMAU_IDEAL_ISMIP_HOM_E = 2, IMAU_IDEAL_ISMIP_HOM_F = 3 };
 /* choice_basal_hydrology (src/configuration_module.f90:361) */
 enum { IMAU_HYDRO_DRY = 0, IMAU_HYDRO_SATURATED = 1, IMAU_HYDRO_MARTIN2011 = 2 };
 /* choice_basal_roughness / choice_param_basal_roughness (src/configuration_module.f90:366,373) */
@@ -207,6 +207,11 @@ typedef struct imau_ice_io {
   const double  *A_flow_3D_a;                              /* (nz,ny,nx) */
   const double  *A_flow_3D_cx, *A_flow_3D_cy;              /* SIA only */
   const double  *Ti_a, *Ti_pmp_a;                          /* only if include_basal_freezing */
+  const int32_t *ISMIP_HOM_E_slides;  /* (nx) ISMIP-HOM E: the fourth column ("slides") of the
+                                         C%ISMIP_HOM_E_Arolla_filename file, line i for grid column i
+                                         (src/basal_conditions_and_sliding_module.f90:1352-1399): beta is
+                                         0 where it is 1, 1e30 elsewhere.  The file has 51 lines, so the
+                                         grid has nx = 51.  Read once when the problem is created */
   /* ---- bed roughness (in; updated when choice_basal_roughness=parameterised) ---- */
   double *beta_sq_a, *alpha_sq_a, *phi_fric_a, *tauc_a;    /* (ny,nx) */
   /* ---- in/out state (warm start and result) ---- */
@@ -231,7 +236,9 @@ typedef struct imau_ice_io {
   double *DIVA_err_cx, *DIVA_err_cy;   /* estimate_visc_iter_UV_errors (src/ice_velocity_module.f90:1602-1652):
                                           1e5 at the start of a solve, then after every Picard iteration
                                           that does not end the loop 2|u-u_prev|/|u+u_prev+1e-5| where
-                                          |u_vav| > 1e-6, else 0.  Computed only when requested at create */
+                                          |u_vav| > 1e-6, else 0.  Computed only when requested at create;
+                                          a member whose solve returns before its Picard loop (no grounded
+                                          ice) keeps the values it had, the caller's at create */
   /* ---- per-call solver tolerances (ice%DIVA_SOR_*, ice%DIVA_PETSc_*,
    *      src/data_types_module.f90:256-260; set by the caller as in
    *      src/ice_dynamics_module.f90:94-99) ---- */

@@ -917,8 +917,12 @@ static void calc_sliding_term_beta(S *s, const double *u_cx, const double *v_cy)
         tauc = tan((PI_D / 180.0) * s->phi_fric[p]) * s->Neff[p];
         s->tauc[p] = tauc;
       }
-      double b = orc_sliding_beta(C, ua[p], va[p], s->beta_sq[p], s->alpha_sq[p], s->phi_fric[p], tauc, s->Neff[p],
-                                  g->x[i], g->y[j], s->io->A_flow_vav_a ? s->io->A_flow_vav_a[p] : 0.0);
+      double b;
+      if (C->choice_sliding_law == IMAU_SLID_IDEALISED && C->choice_idealised_sliding_law == IMAU_IDEAL_ISMIP_HOM_E)
+        b = s->io->ISMIP_HOM_E_slides[i] == 1 ? 0.0 : 1.0e30;  /* line i of the Arolla file, :1372-1384 */
+      else
+        b = orc_sliding_beta(C, ua[p], va[p], s->beta_sq[p], s->alpha_sq[p], s->phi_fric[p], tauc, s->Neff[p],
+                             g->x[i], g->y[j], s->io->A_flow_vav_a ? s->io->A_flow_vav_a[p] : 0.0);
       b = fmin(b, C->DIVA_beta_max);
       if (!C->do_GL_subgrid_friction && s->io->mask_ocean_a[p] == 1) b = 0.0;
       s->beta_a[p] = b;
@@ -1748,6 +1752,16 @@ static int sum_sheet(const S *s)
 
 /* the configuration checks of calc_bed_roughness_Martin2011 / _MISMIPplus
  * (src/basal_conditions_and_sliding_module.f90:789-792, 896-898) */
+/* ISMIP-HOM E: the slip columns of the 51-line Arolla file, one per grid
+ * column (src/basal_conditions_and_sliding_module.f90:1372-1389) */
+static int check_ismip_e(const imau_grid *gr, const imau_cfg *c, const imau_ice_io *io)
+{
+  if (c->choice_sliding_law != IMAU_SLID_IDEALISED || c->choice_idealised_sliding_law != IMAU_IDEAL_ISMIP_HOM_E) return 0;
+  if (!io->ISMIP_HOM_E_slides) return fail(IMAU_ERR_ARG, "ISMIP_HOM_E needs ISMIP_HOM_E_slides");
+  if (gr->nx != 51) return fail(IMAU_ERR_CONFIG, "ISMIP_HOM_E: the Arolla file has 51 lines, one per grid column");
+  return 0;
+}
+
 static int check_roughness(const imau_cfg *c)
 {
   int law = c->choice_sliding_law;
@@ -1787,6 +1801,7 @@ int orc_solve_diva(const imau_grid *gr, const imau_cfg *c, imau_ice_io *io, int 
 {
   if (c->choice_ice_dynamics != IMAU_DYN_DIVA) return fail(IMAU_ERR_CONFIG, "solve_DIVA: choice_ice_dynamics is not DIVA");
   if (check_roughness(c)) return IMAU_ERR_CONFIG;
+  { int e = check_ismip_e(gr, c, io); if (e) return e; }
   S s;
   st_init(&s, gr, c, io, 0);
   const G *g = &s.g;
@@ -1869,6 +1884,7 @@ int orc_solve_ssa(const imau_grid *gr, const imau_cfg *c, imau_ice_io *io, int l
   if (c->choice_ice_dynamics != IMAU_DYN_SSA && c->choice_ice_dynamics != IMAU_DYN_SIA_SSA)
     return fail(IMAU_ERR_CONFIG, "solve_SSA: choice_ice_dynamics is not SSA or SIA/SSA");
   if (check_roughness(c)) return IMAU_ERR_CONFIG;
+  { int e = check_ismip_e(gr, c, io); if (e) return e; }
   S s;
   st_init(&s, gr, c, io, 1);
   const G *g = &s.g;

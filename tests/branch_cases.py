@@ -59,6 +59,25 @@ def idealised_case(which: str):
     return "diva", g, cfg, st
 
 
+def arolla_like_rows():
+    """51 records in the ISMIP-HOM E file format (x, Hb, Hs, slides): a
+    synthetic valley-glacier flowline 5 km long at 100 m spacing, thickness
+    20-300 m, with a slip zone for 2200 <= x <= 2500 m.  The shape of the
+    Haut Glacier d'Arolla input, not its data."""
+    x = np.arange(51) * 100.0
+    Hb = 2600.0 - 0.12 * x + 40.0 * np.sin(2.0 * np.pi * x / 2500.0)
+    Hs = Hb + 20.0 + 280.0 * np.sin(np.pi * x / 5000.0)
+    slides = ((x >= 2200.0) & (x <= 2500.0)).astype(np.int32)
+    return x, Hb, Hs, slides
+
+
+def ismip_e_case():
+    """ISMIP-HOM E (:1352-1399): beta 0 in the slip columns, 1e30 (clamped to
+    DIVA_beta_max) elsewhere, on the 51-column flowline."""
+    g, cfg, st = geometry.ismip_hom_e(arolla_like_rows(), ny=5)
+    return "diva", g, cfg, st
+
+
 def freezing_case():
     """include_basal_freezing: the western half of the bed is frozen (basal
     temperature 5 K below the pressure-melting point), the eastern half at
@@ -202,6 +221,8 @@ SLIDING_CASES = {
 def build(name: str):
     if name in SLIDING_CASES:
         return sliding_case(**SLIDING_CASES[name])
+    if name == "ismip_e":
+        return ismip_e_case()
     if name.startswith("ismip_"):
         return idealised_case("ISMIP_HOM_" + name[-1].upper())
     table = {
@@ -214,7 +235,7 @@ def build(name: str):
     return table[name]()
 
 
-DIVA_CASES = sorted(SLIDING_CASES) + ["ismip_c", "ismip_d", "ismip_f", "freezing", "bc_margin", "sans_random",
+DIVA_CASES = sorted(SLIDING_CASES) + ["ismip_c", "ismip_d", "ismip_e", "ismip_f", "freezing", "bc_margin", "sans_random",
                                       "sans_channel", "periodic"]
 SSA_CASES = ["ssa_dome", "ssa_icestream", "ssa_channel", "bc_margin_ssa"]
 SIA_CASES = ["sia_dome", "sia_random"]

@@ -37,7 +37,7 @@ MASKS = ("DIVA_mask_combi", "DIVA_isfront_inner", "DIVA_isfront_outer")
 
 # cases whose reference Picard loop converges to an FP64 residual of 1e-11
 FIXED_POINT = ["no_sliding", "coulomb_saturated", "tsai_saturated", "schoof_mismip", "weertman_mismip",
-               "zoet_iverson_dry", "ismip_c", "ismip_f", "freezing", "bc_margin", "sans_random", "sans_channel",
+               "zoet_iverson_dry", "ismip_c", "ismip_e", "ismip_f", "freezing", "bc_margin", "sans_random", "sans_channel",
                "periodic", "ssa_dome", "bc_margin_ssa"]
 # cases that cycle (plastic beds, a zero-friction strip) or are held by the limiter
 TRAJECTORY = ["coulomb_reg_martin", "zoet_iverson_martin", "ismip_d", "limiter", "ssa_channel", "ssa_icestream"]
@@ -60,7 +60,7 @@ def relerr(a, b, scale=None):
 # fields compared against the magnitude of the velocity field rather than
 # their own: the transverse component where it vanishes by symmetry, and the
 # basal velocity where a nearly frozen bed makes it a difference of close numbers
-VEL_SCALED = ("v_SSA_cy", "v_vav_cy", "u_SSA_cx", "u_vav_cx", "u_base_cx", "v_base_cy", "taub_cy")
+VEL_SCALED = ("v_SSA_cy", "v_vav_cy", "v_3D_cy", "u_SSA_cx", "u_vav_cx", "u_base_cx", "v_base_cy", "taub_cy")
 
 
 def ice_mask(st, f):
@@ -358,8 +358,15 @@ def test_visc_iter_error_estimate_of_a_batch(ctx):
     equals its single solve; under the residual rule the estimate is the
     final iterate against u_cx_prev."""
     from imau_vel import geometry, solver
-    cases = [geometry.dome(), geometry.random_geometry(45, 37, seed=4)]
+    cases = [geometry.dome(), geometry.random_geometry(45, 37, seed=4), geometry.dome(nx=21, ny=17)]
     cfg = cases[0][1]
+    # the third has no ice: its solve returns before the loop (:273-279) and
+    # leaves DIVA_err as the caller passed it
+    empty = cases[2][2]
+    empty["Hi_a"] = 0.0
+    geometry.update_general_ice_model_data(empty, config.zeta_of(cfg))
+    empty["DIVA_err_cx"] = 7.0
+    empty["DIVA_err_cy"] = 8.0
     config.set_config(cfg, stop_rule=0, DIVA_visc_it_norm_dUV_tol=1e-2, DIVA_visc_it_nit=50)
     singles = []
     for g, _, st in cases:
@@ -379,8 +386,22 @@ def test_visc_iter_error_estimate_of_a_batch(ctx):
         for e in ("DIVA_err_cx", "DIVA_err_cy"):
             np.testing.assert_array_equal(s1[e], sb[e], err_msg=e)
             assert (s1[e] < 1e5).all()
+    assert (batch[2]["DIVA_err_cx"] == 7.0).all() and (batch[2]["DIVA_err_cy"] == 8.0).all()
+    ref = empty.copy()
+    O.solve("diva", cases[2][0], cfg, ref)
+    assert (ref["DIVA_err_cx"] == 7.0).all() and (ref["u_vav_cx"] == 0.0).all()
     config.set_config(cfg, stop_rule=1, nonlinear_rtol=1e-6, DIVA_visc_it_nit=300)
     g, _, st = cases[0]
     ctx.solve_diva(g, cfg, st)
     for e, f, p in (("DIVA_err_cx", "u_vav_cx", "u_cx_prev"), ("DIVA_err_cy", "v_vav_cy", "v_cy_prev")):
         np.testing.assert_allclose(st[e], _visc_errs(st[f], st[p], st[f]), rtol=1e-13, atol=0.0, err_msg=e)
+
+
+def test_ismip_hom_e_needs_51_columns(ctx):
+    """The Arolla file has 51 records, one per grid column; another grid is
+    rejected (IMAU_ERR_CONFIG) as by the oracle."""
+    from imau_vel import geometry
+    g, cfg, st = geometry.random_geometry(41, 37, seed=11, cfg=geometry.synthetic_config(
+        choice_sliding_law="idealised", choice_idealised_sliding_law="ISMIP_HOM_E"))
+    with pytest.raises(RuntimeError, match="51 lines"):
+        ctx.solve_diva(g, cfg, st)

@@ -711,3 +711,80 @@ def test_visc_iter_errors_under_the_residual_rule():
     assert st.stats["picard_iterations"] > 2
     for e, u, p in (("DIVA_err_cx", "u_vav_cx", "u_cx_prev"), ("DIVA_err_cy", "v_vav_cy", "v_cy_prev")):
         np.testing.assert_allclose(st[e], _visc_errs(st[u], st[p], st[u]), rtol=1e-14, atol=0.0, err_msg=e)
+
+
+# ---------------------------------------------------------------- ISMIP-HOM E
+def _arolla_file(path, rows, extra=0, drop=0):
+    x, Hb, Hs, sl = rows
+    with open(path, "w") as f:
+        for k in range(51 - drop):
+            f.write("%.1f %.6f %.6f %d\n" % (x[k], Hb[k], Hs[k], sl[k]))
+        for k in range(extra):
+            f.write("9999.0 0.0 0.0 1\n")
+    return path
+
+
+def test_ismip_hom_e_file_and_geometry(tmp_path):
+    """The Arolla-format file: 51 list-directed records (x, Hb, Hs, slides),
+    later ones not read, fewer rejected with the reference's message; every
+    row of the grid holds the file's column profile with Hi = Hs - Hb
+    (src/reference_fields_module.f90:751-762)."""
+    import branch_cases as B
+    rows = B.arolla_like_rows()
+    p = _arolla_file(tmp_path / "a.dat", rows, extra=3)
+    x, Hb, Hs, sl = geometry.read_ismip_hom_e(p)
+    np.testing.assert_allclose(Hb, rows[1], atol=1e-6)
+    np.testing.assert_array_equal(sl, rows[3])
+    g, cfg, st = geometry.ismip_hom_e(p, ny=6)
+    assert (g.nx, g.ny, g.dx) == (51, 6, 100.0)
+    for j in range(6):
+        np.testing.assert_allclose(st["Hi_a"][j], rows[2] - rows[1], atol=2e-6)
+    np.testing.assert_array_equal(st["ISMIP_HOM_E_slides"], rows[3])
+    with pytest.raises(geometry.ArollaFileError, match="should be 51 lines"):
+        geometry.read_ismip_hom_e(_arolla_file(tmp_path / "b.dat", rows, drop=1))
+
+
+def test_ismip_hom_e_sliding_law():
+    """calc_sliding_law_idealised_ISMIP_HOM_E (:1352-1399): record i of the
+    file sets column i, beta = 0 where slides = 1 and 1e30 (clamped to
+    DIVA_beta_max) elsewhere, on every row.  The solve equals, bit for bit,
+    Weertman sliding with m = 1 (beta = beta_sq exactly) and beta_sq = 0 / 1e30
+    on those columns, an independent law fed the intended pattern (a
+    transposed or inverted pattern differs); with no slip column it equals
+    the no-sliding law (beta_eff = 1/F2) to the Picard tolerance."""
+    import branch_cases as B
+    kind, g, cfg, st = B.build("ismip_e")
+    config.set_config(cfg, stop_rule=1, nonlinear_rtol=1e-11, DIVA_visc_it_nit=500)
+    e = O.solve("diva", g, cfg, st.copy())
+    sl = st["ISMIP_HOM_E_slides"]
+    assert 0 < sl.sum() < 51
+    want = np.where(sl[None, :] == 1, 0.0, cfg.DIVA_beta_max) * np.ones((g.ny, 1))
+    np.testing.assert_array_equal(e["beta_a"], want)
+    cw = config.default_config()
+    ctypes.memmove(ctypes.addressof(cw), ctypes.addressof(cfg), ctypes.sizeof(cfg))
+    config.set_config(cw, choice_sliding_law="Weertman", slid_Weertman_m=1.0)
+    w = st.copy()
+    w["beta_sq_a"] = np.where(sl[None, :] == 1, 0.0, 1e30) * np.ones((g.ny, 1))
+    O.solve("diva", g, cw, w)
+    for k in ("u_vav_cx", "v_vav_cy", "beta_a", "u_base_a", "w_3D_a"):
+        np.testing.assert_array_equal(e[k], w[k], err_msg=k)
+    # no slip column: the no-sliding law
+    st0 = st.copy()
+    st0["ISMIP_HOM_E_slides"] = 0
+    e0 = O.solve("diva", g, cfg, st0)
+    cn = config.default_config()
+    ctypes.memmove(ctypes.addressof(cn), ctypes.addressof(cfg), ctypes.sizeof(cfg))
+    config.set_config(cn, choice_sliding_law="no_sliding")
+    n0 = O.solve("diva", g, cn, st.copy())
+    scale = np.abs(n0["u_vav_cx"]).max()
+    assert np.abs(e0["u_vav_cx"] - n0["u_vav_cx"]).max() <= 1e-8 * scale
+    # the slip columns speed the flow up there
+    cols = np.where(sl == 1)[0][:-1]          # cx faces between two slip columns
+    assert (np.abs(e["u_vav_cx"][:, cols]) > 1.5 * np.abs(n0["u_vav_cx"][:, cols])).all()
+
+
+def test_ismip_hom_e_needs_51_columns():
+    g, cfg, st = geometry.random_geometry(41, 37, seed=11, cfg=geometry.synthetic_config(
+        choice_sliding_law="idealised", choice_idealised_sliding_law="ISMIP_HOM_E"))
+    with pytest.raises(RuntimeError, match="51 lines"):
+        O.solve("diva", g, cfg, st)
