@@ -307,7 +307,10 @@ def main():
         visc_tile = int(args.visc_tile)
         prob.set_tile_rows(visc_tile)
     dt = timed_solves(prob, cfg, args.steps, args.warmup, ws)
+    t_dl = time.perf_counter()
     out = prob.download()
+    t_dl = time.perf_counter() - t_dl
+    dl_bytes = sum(a.nbytes for o in out for a in o.f.values())
     its = [o.stats["picard_iterations"] for o in out]
     apps = [o.stats["linear_sweeps"] for o in out]
     res = [o.stats["nonlinear_residual"] for o in out]
@@ -388,6 +391,8 @@ def main():
                                     fallbacks=ws.fallbacks,
                                     fallback="cold solve of the whole ensemble when any member's FP64 residual "
                                              "stays above 1e-4")
+    result["download"] = dict(ms=1e3 * t_dl, bytes=dl_bytes, note="every field of every member into the host "
+                              "IceState arrays after the timed solves (outside the timed region)")
     if ws is not None and not args.no_cold:
         # the same solver from the cold start
         dtc = timed_solves(prob, cfg, args.steps, 0)

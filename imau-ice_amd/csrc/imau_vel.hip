@@ -163,9 +163,22 @@ struct imau_problem {
   double sor_tol_ = 2.5, sor_omega_ = 1.3, petsc_rtol_ = 0.01, petsc_abstol_ = 2.5;
 };
 
+// a 2D download collected by imau_vel_problem_download (flush_down2)
+struct Down2Req {
+  int g;
+  int kind;
+  void *dst;
+  const char *src_base;
+  size_t esz;
+};
+
 struct imau_ctx {
   int device = 0;
   hipStream_t s = nullptr;
+  std::vector<Down2Req> *batch = nullptr;  // non-null: down2 collects instead of copying
+  void *pin[2] = {nullptr, nullptr};       // pinned staging of batched downloads, one field each
+  size_t pin_bytes = 0;
+  hipEvent_t pin_ev[2] = {nullptr, nullptr};
   double *scratch = nullptr;   // this device's copy of `scratch` (imau_vel_problem_residual_noalloc)
   std::string err;
   imau_problem *cached = nullptr;
@@ -357,12 +370,69 @@ void up2(imau_ctx *c, imau_problem *P, int g, Kind k, const void *src, void *dst
 }
 void down2(imau_ctx *c, imau_problem *P, int g, Kind k, void *dst, const void *src_base, size_t esz) {
   if (!dst) return;
+  if (c->batch) {
+    c->batch->push_back(Down2Req{g, (int)k, dst, (const char *)src_base, esz});
+    return;
+  }
   const Member &m = P->mem[g];
   int ni = m.nx, nj = m.ny;
   if (k == KCX || k == KB) ni = m.nx - 1;
   if (k == KCY || k == KB) nj = m.ny - 1;
   const char *src = (const char *)src_base + (size_t)m.off * esz;
   HIPCHK(hipMemcpy2DAsync(dst, (size_t)nj * esz, src, (size_t)m.ny * esz, (size_t)nj * esz, ni, hipMemcpyDeviceToHost, c->s));
+}
+
+// The collected 2D downloads, one device-to-host copy per field: the field's
+// whole arena (every member, padding included) goes into pinned memory and the
+// host copies each member's part into its array, instead of one pageable
+// copy per member and field.  Two pinned buffers let the next field's copy
+// run while the host copies out the previous one.
+void flush_down2(imau_ctx *c, imau_problem *P, const std::vector<Down2Req> &reqs) {
+  std::vector<const char *> srcs;
+  std::vector<size_t> eszs;
+  for (const Down2Req &r : reqs)
+    if (std::find(srcs.begin(), srcs.end(), r.src_base) == srcs.end()) {
+      srcs.push_back(r.src_base);
+      eszs.push_back(r.esz);
+    }
+  if (srcs.empty()) return;
+  const size_t need = (size_t)P->na * sizeof(double);
+  if (c->pin_bytes < need) {
+    HIPCHK(hipStreamSynchronize(c->s));
+    for (void *&p : c->pin) {
+      if (p) HIPCHK(hipHostFree(p));
+      p = nullptr;
+    }
+    c->pin_bytes = 0;
+    for (void *&p : c->pin) HIPCHK(hipHostMalloc(&p, need, hipHostMallocDefault));
+    c->pin_bytes = need;
+  }
+  for (hipEvent_t &e : c->pin_ev)
+    if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  auto issue = [&](size_t f) {
+    HIPCHK(hipMemcpyAsync(c->pin[f & 1], srcs[f], (size_t)P->na * eszs[f], hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipEventRecord(c->pin_ev[f & 1], c->s));
+  };
+  issue(0);
+  for (size_t f = 0; f < srcs.size(); ++f) {
+    // buffer (f + 1) & 1 was copied out in the previous pass of this loop
+    if (f + 1 < srcs.size()) issue(f + 1);
+    HIPCHK(hipEventSynchronize(c->pin_ev[f & 1]));
+    const char *h = (const char *)c->pin[f & 1];
+    for (const Down2Req &r : reqs) {
+      if (r.src_base != srcs[f]) continue;
+      const Member &m = P->mem[r.g];
+      const int ni = (r.kind == KCX || r.kind == KB) ? m.nx - 1 : m.nx;
+      const int nj = (r.kind == KCY || r.kind == KB) ? m.ny - 1 : m.ny;
+      const char *src = h + (size_t)m.off * r.esz;
+      char *dst = (char *)r.dst;
+      if (nj == m.ny)
+        std::memcpy(dst, src, (size_t)ni * nj * r.esz);
+      else
+        for (int i = 0; i < ni; ++i)
+          std::memcpy(dst + (size_t)i * nj * r.esz, src + (size_t)i * m.ny * r.esz, (size_t)nj * r.esz);
+    }
+  }
 }
 void up3(imau_ctx *c, imau_problem *P, int g, Kind k, const double *src, double *dst) {
   if (!src) return;
@@ -550,6 +620,21 @@ void wire(imau_problem *P) {
 }
 
 bool uses_ssa_field(int dyn) { return dyn == IMAU_DYN_SSA || dyn == IMAU_DYN_SIA_SSA; }
+
+// buffers that only some configurations use (GCR directions, Anderson
+// slots, the FP32 viscosity), allocated before a solve's first kernel since
+// DBuf::alloc synchronises the device: at create for the create-time
+// configuration, and at the start of a solve whose configuration needs more
+void reserve_buffers(imau_problem *P, const imau_cfg *cfg) {
+  const long long na = P->na, n3 = na * (long long)P->nz;
+  if (cfg->DIVA_choice_matrix_solver != IMAU_SOLVER_SOR && cfg->ksp_type == 1 && !P->gcr_dirs.p)
+    P->gcr_dirs.alloc((size_t)4 * IMAU_GCR_M * na);
+  if (cfg->stop_rule == IMAU_STOP_RESIDUAL && cfg->anderson_m > 0) {
+    if (!P->and_slots.p) P->and_slots.alloc((size_t)4 * IMAU_AND_SLOTS * na);
+    if (!P->and_w.p) P->and_w.alloc((size_t)2 * IMAU_AND_SLOTS * P->ngrids);
+  }
+  if (cfg->precision == IMAU_PREC_MIXED_VISC && !P->visc32.p) P->visc32.alloc((size_t)3 * n3);
+}
 
 // ISMIP-HOM E reads its slip columns from the 51-line Arolla file
 // (src/basal_conditions_and_sliding_module.f90:1372-1389)
@@ -746,7 +831,12 @@ int imau_vel_init(const imau_grid *grid, const imau_cfg *cfg, imau_ctx **out) {
 
 void imau_vel_finalize(imau_ctx *ctx) {
   if (!ctx) return;
+  if (ctx->s) (void)hipStreamSynchronize(ctx->s);
   for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
+  for (hipEvent_t e : ctx->pin_ev)
+    if (e) (void)hipEventDestroy(e);
+  for (void *p : ctx->pin)
+    if (p) (void)hipHostFree(p);
   imau_vel_problem_destroy(ctx->cached);
   if (ctx->s) (void)hipStreamDestroy(ctx->s);
   delete ctx;
@@ -856,6 +946,7 @@ int imau_vel_problem_create(imau_ctx *ctx, int32_t ngrids, const imau_grid *grid
     P->F.chk_u = P->O.uvav;
     P->F.chk_v = P->O.vvav;
   }
+  reserve_buffers(P, cfg);
   upload(ctx, P, cfg, ios);
   if (want_err)  // slot 0 starts with the caller's values (kept by a member that never enters its loop)
     for (int g = 0; g < ngrids; ++g) {
@@ -1529,6 +1620,7 @@ struct Solver {
     act_pic.assign(G, 0);
     for (int g = 0; g < G; ++g) act_pic[g] = P->sheet_any[g];
     if (!diva && cfg->choice_sliding_law == IMAU_SLID_NO_SLIDING) std::fill(act_pic.begin(), act_pic.end(), 0);
+    reserve_buffers(P, cfg);
     std::fill(P->picard_its.begin(), P->picard_its.end(), 0);
     std::fill(P->restarts.begin(), P->restarts.end(), 0);
     std::fill(P->sweeps.begin(), P->sweeps.end(), 0);
@@ -1778,6 +1870,12 @@ int imau_vel_problem_download(imau_ctx *ctx, imau_problem *P, imau_ice_io *ios) 
   ABI_TRY
   if (!ctx || !P || !ios) fail(IMAU_ERR_ARG, "NULL argument");
   const size_t d = sizeof(double), q = sizeof(int32_t);
+  std::vector<Down2Req> batch;
+  struct Collect {  // the 2D fields below are collected, then copied by flush_down2
+    imau_ctx *c;
+    ~Collect() { c->batch = nullptr; }
+  } collect{ctx};
+  ctx->batch = &batch;
   const int dyn = P->dyn_uploaded;
   const bool ssa = uses_ssa_field(dyn);
   for (int g = 0; g < P->ngrids; ++g) {
@@ -1865,6 +1963,8 @@ int imau_vel_problem_download(imau_ctx *ctx, imau_problem *P, imau_ice_io *ios) 
     io.stat_nonlinear_residual = P->nl_resid[g];
     io.stat_seconds = P->seconds;
   }
+  ctx->batch = nullptr;
+  flush_down2(ctx, P, batch);
   HIPCHK(hipStreamSynchronize(ctx->s));
   // the reference sets u, v = 0 and returns for grids without grounded ice (:276-285)
   for (int g = 0; g < P->ngrids; ++g) {
