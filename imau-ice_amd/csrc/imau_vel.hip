@@ -2914,17 +2914,23 @@ struct LoopDev {
 
 extern "C" {
 
-int imau_run_ice_dynamics_direct(imau_ctx *ctx, const imau_grid *grid, const imau_cfg *cfg,
-                                 const imau_thickness_cfg *tcfg, const imau_producer_cfg *pcfg,
-                                 const imau_timestep_cfg *scfg, imau_ice_io *io, const imau_refgeo_io *ref,
-                                 const double *SMB_year, const double *BMB, double *Hi_end, imau_producer_io *geo,
-                                 imau_timestep_state *st, double t_end) {
-  ABI_TRY
+namespace {
+// the time loop of run_ice_dynamics_direct (pc == nullptr) or _pc
+int run_loop(imau_ctx *ctx, const imau_grid *grid, const imau_cfg *cfg, const imau_thickness_cfg *tcfg,
+             const imau_producer_cfg *pcfg, const imau_timestep_cfg *scfg, imau_ice_io *io,
+             const imau_refgeo_io *ref, const double *SMB_year, const double *BMB, double *Hi_end,
+             imau_producer_io *geo, imau_timestep_state *st, imau_pc_state *pc, double t_end) {
   if (!ctx || !grid || !cfg || !tcfg || !pcfg || !scfg || !io || !SMB_year || !BMB || !Hi_end || !st)
     fail(IMAU_ERR_ARG, "NULL argument");
   const int dyn = cfg->choice_ice_dynamics;
-  if (dyn != IMAU_DYN_NONE && dyn != IMAU_DYN_SIA && dyn != IMAU_DYN_SSA && dyn != IMAU_DYN_SIA_SSA)
+  if (!pc && dyn != IMAU_DYN_NONE && dyn != IMAU_DYN_SIA && dyn != IMAU_DYN_SSA && dyn != IMAU_DYN_SIA_SSA)
     fail(IMAU_ERR_CONFIG, "direct time stepping works with 'none', 'SIA', 'SSA' or 'SIA/SSA' only");
+  if (pc && (!pc->dHidt_Hn_un || !pc->dHi_dt_a)) fail(IMAU_ERR_ARG, "pc state fields are NULL");
+  if (pc && pc->t_next < st->time) fail(IMAU_ERR_CONFIG, "pc t_next %g is before the current time %g", pc->t_next, st->time);
+  if (pc && !(scfg->pc_epsilon > 0.0 && scfg->pc_eta_min > 0.0 && pc->pc_eta > 0.0 && pc->pc_eta_prev > 0.0 &&
+              pc->dt_crit_ice > 0.0 && pc->dt > 0.0))
+    fail(IMAU_ERR_CONFIG, "predictor-corrector: pc_epsilon, pc_eta_min, pc_eta, pc_eta_prev, dt_crit_ice and dt "
+                          "must be positive");
   check_cfg(cfg);
   check_thickness_cfg(tcfg);
   check_producer_cfg(pcfg);
@@ -2998,8 +3004,20 @@ int imau_run_ice_dynamics_direct(imau_ctx *ctx, const imau_grid *grid, const ima
   const imau::Str3 s3{na, (long long)ny, 1};  // k-plane major
   ThkDev T;
   T.alloc(mb);
-  T.in = imau::ThkIn{Hi, f2p(P, F_Hb), f2p(P, F_SL), P->O.uvav, P->O.vvav, X.SMB, X.BMB, X.fmf,
+  // the current velocities: ice%u_vav_cx (DIVA solves it in place; SIA and SSA compose it)
+  const double *uvel = dyn == IMAU_DYN_DIVA ? P->F.u : P->O.uvav, *vvel = dyn == IMAU_DYN_DIVA ? P->F.v : P->O.vvav;
+  T.in = imau::ThkIn{Hi, f2p(P, F_Hb), f2p(P, F_SL), uvel, vvel, X.SMB, X.BMB, X.fmf,
                      mk.ice, mk.shelf, mk.ocean, mk.cf};
+  // predictor-corrector fields: dHi/dt_n, dHi/dt_n-1, dHi/dt*, Hi_old, Hi_pred, dHi_dt_a
+  DBuf<double> pcf;
+  double *pHn = nullptr, *pHnm1 = nullptr, *pHstar = nullptr, *pHold = nullptr, *pHpred = nullptr, *pdHi = nullptr;
+  if (pc) {
+    pcf.alloc((size_t)6 * na);
+    pHn = pcf.p; pHnm1 = pcf.p + na; pHstar = pcf.p + 2 * na; pHold = pcf.p + 3 * na; pHpred = pcf.p + 4 * na;
+    pdHi = pcf.p + 5 * na;
+    HIPCHK(hipMemcpyAsync(pHn, pc->dHidt_Hn_un, na * d8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(pdHi, pc->dHi_dt_a, na * d8, hipMemcpyHostToDevice, s));
+  }
   auto flood = [&](auto round) {  // rounds of tile-local growth until one changes nothing
     int changed = 1;
     for (long long r = 0; changed && r <= na; ++r) {
@@ -3055,7 +3073,41 @@ int imau_run_ice_dynamics_direct(imau_ctx *ctx, const imau_grid *grid, const ima
                                   std::pow((scfg->end_time_of_run - time) / scfg->dt_startup_phase, 2);
     else
       dt_max = scfg->dt_max;
-    if ((dyn == IMAU_DYN_SIA || dyn == IMAU_DYN_SIA_SSA) && time == st->t_next_SIA) {
+    if (pc && dyn != IMAU_DYN_NONE && time == pc->t_next) {  // :312-415
+      imau::cfl(s, mb, uvel, vvel, X.red.p, X.red.p + X.nb);
+      HIPCHK(hipGetLastError());
+      const double dt_adv = std::min(scfg->dt_max, scalar(X.red.p + X.nb) * 0.9);
+      pc->dt_crit_ice_prev = pc->dt_crit_ice;
+      const double dt_from_pc = std::pow(scfg->pc_epsilon / pc->pc_eta, scfg->pc_k_I + scfg->pc_k_p) *
+                                std::pow(scfg->pc_epsilon / pc->pc_eta_prev, -scfg->pc_k_p) * pc->dt;
+      double dc = std::min(std::min(scfg->dt_max, 2.0 * pc->dt_crit_ice_prev), std::min(dt_adv, dt_from_pc));
+      dc = std::max(scfg->dt_min, std::max(0.5 * pc->dt_crit_ice_prev, dc));
+      dc = std::max(scfg->dt_min, std::min(dt_max, dc));
+      pc->dt_crit_ice = dc;
+      pc->pc_zeta = dc / pc->dt_crit_ice_prev;
+      std::swap(pHn, pHnm1);                                       // dHidt_Hnm1_unm1 = dHidt_Hn_un
+      thickness_compute(ctx, mb, T, tcfg, dc);                     // at (H_n, u_n)
+      HIPCHK(hipMemcpyAsync(pHn, T.dHi, na * d8, hipMemcpyDeviceToDevice, s));
+      imau::pc_pred(s, mb, Hi, pHold, pHpred, pHn, pHnm1, dc, pc->pc_zeta);
+      producers(T.dHi);
+      if (dyn == IMAU_DYN_SIA || dyn == IMAU_DYN_SIA_SSA)
+        if ((rc = solve_kind(ctx, P, cfg, SOLVE_SIA))) return rc;
+      if (dyn == IMAU_DYN_SSA || dyn == IMAU_DYN_SIA_SSA)
+        if ((rc = solve_kind(ctx, P, cfg, SOLVE_SSA))) return rc;
+      if (dyn == IMAU_DYN_DIVA)
+        if ((rc = solve_kind(ctx, P, cfg, SOLVE_DIVA))) return rc;
+      pc->t_next = time + dc;
+      thickness_compute(ctx, mb, T, tcfg, dc);                     // at (H*_n+1, u_n+1)
+      HIPCHK(hipMemcpyAsync(pHstar, T.dHi, na * d8, hipMemcpyDeviceToDevice, s));
+      HIPCHK(hipMemcpyAsync(Hi, pHold, na * d8, hipMemcpyDeviceToDevice, s));
+      producers(T.dHi);
+      imau::pc_corr(s, mb, Hi, pHpred, pHn, pHstar, dc, pc->pc_zeta, mk.sheet, mk.gl, scfg->pc_eta_min, pdHi,
+                    X.red.p, X.red.p + X.nb);
+      HIPCHK(hipGetLastError());
+      pc->pc_eta_prev = pc->pc_eta;
+      pc->pc_eta = scalar(X.red.p + X.nb);
+    }
+    if (!pc && (dyn == IMAU_DYN_SIA || dyn == IMAU_DYN_SIA_SSA) && time == st->t_next_SIA) {
       if ((rc = solve_kind(ctx, P, cfg, SOLVE_SIA))) return rc;
       imau::crit_sia(s, mb, nz, zeta, P->O.u3sia, P->O.v3sia, P->O.u3, P->O.v3, o.Hi_cx, o.Hi_cy, o.dHs_dx_cx,
                      o.dHs_dy_cy, mk.sheet, X.red.p, X.red.p + X.nb);
@@ -3065,7 +3117,7 @@ int imau_run_ice_dynamics_direct(imau_ctx *ctx, const imau_grid *grid, const ima
       st->dt_crit_SIA = d;
       st->t_next_SIA = time + d;
     }
-    if ((dyn == IMAU_DYN_SSA || dyn == IMAU_DYN_SIA_SSA) && time == st->t_next_SSA) {
+    if (!pc && (dyn == IMAU_DYN_SSA || dyn == IMAU_DYN_SIA_SSA) && time == st->t_next_SSA) {
       if ((rc = solve_kind(ctx, P, cfg, SOLVE_SSA))) return rc;
       imau::cfl(s, mb, P->O.uvav, P->O.vvav, X.red.p, X.red.p + X.nb);
       HIPCHK(hipGetLastError());
@@ -3076,12 +3128,21 @@ int imau_run_ice_dynamics_direct(imau_ctx *ctx, const imau_grid *grid, const ima
     }
     // determine_timesteps_and_actions, :838-856 and :933
     double t_next = std::min(t_end, time + scfg->dt_max);
-    if (dyn == IMAU_DYN_SIA || dyn == IMAU_DYN_SIA_SSA) t_next = std::min(t_next, st->t_next_SIA);
-    if (dyn == IMAU_DYN_SSA || dyn == IMAU_DYN_SIA_SSA) t_next = std::min(t_next, st->t_next_SSA);
+    if (pc) {
+      if (dyn != IMAU_DYN_NONE) t_next = std::min(t_next, pc->t_next);
+    } else {
+      if (dyn == IMAU_DYN_SIA || dyn == IMAU_DYN_SIA_SSA) t_next = std::min(t_next, st->t_next_SIA);
+      if (dyn == IMAU_DYN_SSA || dyn == IMAU_DYN_SIA_SSA) t_next = std::min(t_next, st->t_next_SSA);
+    }
     const double dt = t_next - time;
     st->dt = dt;
-    // :224-236
-    if (dyn == IMAU_DYN_NONE) {
+    if (pc) {
+      pc->dt = dt;
+      // :257-260 for 'none'; :428 Hi_tplusdt = max(0, Hi + dt dHi_dt_a) (the clip follows)
+      if (dyn == IMAU_DYN_NONE) HIPCHK(hipMemsetAsync(pdHi, 0, na * d8, s));
+      HIPCHK(hipMemcpyAsync(T.dHi, pdHi, na * d8, hipMemcpyDeviceToDevice, s));
+      imau::thk_advance(s, mb, Hi, T.dHi, dt, T.Ht);
+    } else if (dyn == IMAU_DYN_NONE) {  // :224-236
       HIPCHK(hipMemsetAsync(T.dHi, 0, na * d8, s));
       HIPCHK(hipMemcpyAsync(T.Ht, Hi, na * d8, hipMemcpyDeviceToDevice, s));
     } else {
@@ -3146,9 +3207,34 @@ int imau_run_ice_dynamics_direct(imau_ctx *ctx, const imau_grid *grid, const ima
     down3(ctx, P, 0, KCX, geo->A_flow_3D_cx, P->Acx3);
     down3(ctx, P, 0, KCY, geo->A_flow_3D_cy, P->Acy3);
   }
+  if (pc) {
+    HIPCHK(hipMemcpyAsync(pc->dHidt_Hn_un, pHn, na * d8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(pc->dHi_dt_a, pdHi, na * d8, hipMemcpyDeviceToHost, s));
+  }
   HIPCHK(hipStreamSynchronize(s));
   (void)i4;
   return IMAU_OK;
+}
+}  // namespace
+
+int imau_run_ice_dynamics_direct(imau_ctx *ctx, const imau_grid *grid, const imau_cfg *cfg,
+                                 const imau_thickness_cfg *tcfg, const imau_producer_cfg *pcfg,
+                                 const imau_timestep_cfg *scfg, imau_ice_io *io, const imau_refgeo_io *ref,
+                                 const double *SMB_year, const double *BMB, double *Hi_end, imau_producer_io *geo,
+                                 imau_timestep_state *st, double t_end) {
+  ABI_TRY
+  return run_loop(ctx, grid, cfg, tcfg, pcfg, scfg, io, ref, SMB_year, BMB, Hi_end, geo, st, nullptr, t_end);
+  ABI_CATCH(ctx)
+}
+
+int imau_run_ice_dynamics_pc(imau_ctx *ctx, const imau_grid *grid, const imau_cfg *cfg,
+                             const imau_thickness_cfg *tcfg, const imau_producer_cfg *pcfg,
+                             const imau_timestep_cfg *scfg, imau_ice_io *io, const imau_refgeo_io *ref,
+                             const double *SMB_year, const double *BMB, double *Hi_end, imau_producer_io *geo,
+                             imau_timestep_state *st, imau_pc_state *pc, double t_end) {
+  ABI_TRY
+  if (!pc) fail(IMAU_ERR_ARG, "NULL argument");
+  return run_loop(ctx, grid, cfg, tcfg, pcfg, scfg, io, ref, SMB_year, BMB, Hi_end, geo, st, pc, t_end);
   ABI_CATCH(ctx)
 }
 

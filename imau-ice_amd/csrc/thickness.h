@@ -50,6 +50,13 @@ void crit_sia(hipStream_t s, const Member &m, int nz, const Zeta &zeta, const do
 // Ht = Hi + dt dHi; Hi = max(0, Ht)
 void thk_advance(hipStream_t s, const Member &m, const double *Hi, const double *dHi, double dt, double *Ht);
 void thk_clip(hipStream_t s, const Member &m, const double *Ht, double *Hi);
+// predictor-corrector steps of run_ice_dynamics_pc; pc_corr writes dHi and the
+// truncation error pc_eta into *eta (device); partial holds thk_blocks(m) doubles
+void pc_pred(hipStream_t s, const Member &m, double *Hi, double *Hold, double *Hpred, const double *dHn,
+             const double *dHnm1, double dt, double zeta);
+void pc_corr(hipStream_t s, const Member &m, const double *Hi, const double *Hpred, const double *dHn,
+             const double *dHstar, double dt, double zeta, const int32_t *sheet, const int32_t *gl, double eta_min,
+             double *dHi, double *partial, double *eta);
 // min over interior cells of dx / (|u| + |v|) into *out (device)
 void cfl(hipStream_t s, const Member &m, const double *u, const double *v, double *partial, double *out);
 

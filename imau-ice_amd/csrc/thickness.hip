@@ -580,6 +580,74 @@ void crit_sia(hipStream_t s, const Member &m, int nz, const Zeta &zeta, const do
                      dHs_dy_cy, sheet, partial);
   hipLaunchKernelGGL(k_min, dim3(1), dim3(TB), 0, s, partial, nb, out);
 }
+// predictor of run_ice_dynamics_pc (src/ice_dynamics_module.f90:333-340):
+// Hi_pred = max(0, Hi + dt ((1 + zeta/2) dHn - (zeta/2) dHnm1)), Hi_old = Hi, Hi = Hi_pred
+__global__ void __launch_bounds__(TB) k_pc_pred(const Member m, double *__restrict__ Hi, double *__restrict__ Hold,
+                                                double *__restrict__ Hpred, const double *__restrict__ dHn,
+                                                const double *__restrict__ dHnm1, double dt, double zeta) {
+  Cell c;
+  if (!cell_of(m, blockIdx.x * TB + threadIdx.x, c)) return;
+  const double h = Hi[c.p];
+  const double hp = fmax(0.0, h + dt * ((1.0 + zeta / 2.0) * dHn[c.p] - (zeta / 2.0) * dHnm1[c.p]));
+  Hold[c.p] = h;
+  Hpred[c.p] = hp;
+  Hi[c.p] = hp;
+}
+// corrector (:404-414) at the restored Hi = Hi_old: Hi_corr = max(0, Hi + dt/2 (dHn +
+// dHstar)), dHi = (Hi_corr - Hi) / dt; and calc_pc_truncation_error (:763-816):
+// the block maximum of tau = |zeta (Hi_corr - Hi_pred) / ((3 zeta + 3) dt)| over the
+// interior sheet cells with no grounding line in their 3 x 3 neighbourhood
+__global__ void __launch_bounds__(TB) k_pc_corr(const Member m, const double *__restrict__ Hi,
+                                                const double *__restrict__ Hpred, const double *__restrict__ dHn,
+                                                const double *__restrict__ dHstar, double dt, double zeta,
+                                                const int32_t *__restrict__ sheet, const int32_t *__restrict__ gl,
+                                                double eta_min, double *__restrict__ dHi, double *__restrict__ partial) {
+  Cell c;
+  double acc = eta_min;
+  if (cell_of(m, blockIdx.x * TB + threadIdx.x, c)) {
+    const double hc = fmax(0.0, Hi[c.p] + 0.5 * dt * (dHn[c.p] + dHstar[c.p]));
+    dHi[c.p] = (hc - Hi[c.p]) / dt;
+    if (c.i > 0 && c.i < m.nx - 1 && c.j > 0 && c.j < m.ny - 1 && sheet[c.p] == 1) {
+      bool no_gl = true;
+      for (int di = -1; di <= 1; ++di)
+        for (int dj = -1; dj <= 1; ++dj) no_gl = no_gl && gl[c.p + (long long)di * m.ny + dj] == 0;
+      if (no_gl) acc = fmax(acc, fabs(zeta * (hc - Hpred[c.p]) / ((3.0 * zeta + 3.0) * dt)));
+    }
+  }
+  __shared__ double mx[TB];
+  mx[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = TB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) mx[threadIdx.x] = fmax(mx[threadIdx.x], mx[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = mx[0];
+}
+__global__ void __launch_bounds__(TB) k_max(const double *__restrict__ v, int n, double *__restrict__ out) {
+  __shared__ double mx[TB];
+  double acc = -INFINITY;
+  for (int q = threadIdx.x; q < n; q += TB) acc = fmax(acc, v[q]);
+  mx[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = TB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) mx[threadIdx.x] = fmax(mx[threadIdx.x], mx[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = mx[0];
+}
+
+void pc_pred(hipStream_t s, const Member &m, double *Hi, double *Hold, double *Hpred, const double *dHn,
+             const double *dHnm1, double dt, double zeta) {
+  hipLaunchKernelGGL(k_pc_pred, dim3(thk_blocks(m)), dim3(TB), 0, s, m, Hi, Hold, Hpred, dHn, dHnm1, dt, zeta);
+}
+void pc_corr(hipStream_t s, const Member &m, const double *Hi, const double *Hpred, const double *dHn,
+             const double *dHstar, double dt, double zeta, const int32_t *sheet, const int32_t *gl, double eta_min,
+             double *dHi, double *partial, double *eta) {
+  const int nb = thk_blocks(m);
+  hipLaunchKernelGGL(k_pc_corr, dim3(nb), dim3(TB), 0, s, m, Hi, Hpred, dHn, dHstar, dt, zeta, sheet, gl, eta_min,
+                     dHi, partial);
+  hipLaunchKernelGGL(k_max, dim3(1), dim3(TB), 0, s, partial, nb, eta);
+}
 void thk_advance(hipStream_t s, const Member &m, const double *Hi, const double *dHi, double dt, double *Ht) {
   hipLaunchKernelGGL(k_thk_advance, dim3(thk_blocks(m)), dim3(TB), 0, s, m, Hi, dHi, dt, Ht);
 }

@@ -116,6 +116,10 @@ class ImauTimestepState(ctypes.Structure):
     _fields_ = _struct_fields(_TXT, "imau_timestep_state", DEFINES)
 
 
+class ImauPcState(ctypes.Structure):
+    _fields_ = _struct_fields(_TXT, "imau_pc_state", DEFINES)
+
+
 class ImauRefgeoIO(ctypes.Structure):
     _fields_ = _struct_fields(_TXT, "imau_refgeo_io", DEFINES)
 

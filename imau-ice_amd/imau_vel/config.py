@@ -209,8 +209,9 @@ CALVING = {"none": abi.DEFINES["IMAU_CALVING_NONE"],
 
 
 def timestep_config(**overrides):
-    """Reference defaults of the time loop (src/configuration_module.f90:50-55)
-    and of update_ice_thickness (:335-337, :412-417); choice_calving_law may
+    """Reference defaults of the time loop (src/configuration_module.f90:50-55),
+    of the predictor-corrector step (:317-320) and of update_ice_thickness
+    (:335-337, :412-417); choice_calving_law may
     be given by its reference name."""
     c = abi.ImauTimestepCfg()
     c.start_time_of_run = 0.0
@@ -221,6 +222,7 @@ def timestep_config(**overrides):
     c.choice_calving_law = CALVING["threshold_thickness"]
     c.calving_threshold_thickness = 200.0
     c.continental_shelf_min_height = -2000.0
+    c.pc_epsilon, c.pc_k_I, c.pc_k_p, c.pc_eta_min = 0.5, 0.2, 0.2, 1e-8   # :317-320
     if isinstance(overrides.get("choice_calving_law"), str):
         overrides["choice_calving_law"] = CALVING[overrides["choice_calving_law"]]
     for k, v in overrides.items():

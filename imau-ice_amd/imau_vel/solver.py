@@ -30,7 +30,7 @@ THICKNESS_EXPORTED = ("imau_thickness_default_config", "imau_thickness_update", 
 # the input producers the solve reads
 PRODUCER_EXPORTED = ("imau_producer_default_config", "imau_update_general_ice_model_data", "imau_calc_ice_rheology")
 # the time loop that calls them
-TIMELOOP_EXPORTED = ("imau_run_ice_dynamics_direct",)
+TIMELOOP_EXPORTED = ("imau_run_ice_dynamics_direct", "imau_run_ice_dynamics_pc")
 
 
 class ImauError(RuntimeError):
@@ -90,6 +90,9 @@ def load_library(path: str = LIB_PATH):
     L.imau_run_ice_dynamics_direct.argtypes = [vp, P(ImauGrid), P(ImauCfg), P(abi.ImauThicknessCfg),
                                                P(abi.ImauProducerCfg), P(abi.ImauTimestepCfg), P(ImauIceIO), vp, vp,
                                                vp, vp, vp, P(abi.ImauTimestepState), ctypes.c_double]
+    L.imau_run_ice_dynamics_pc.argtypes = [vp, P(ImauGrid), P(ImauCfg), P(abi.ImauThicknessCfg),
+                                           P(abi.ImauProducerCfg), P(abi.ImauTimestepCfg), P(ImauIceIO), vp, vp,
+                                           vp, vp, vp, P(abi.ImauTimestepState), P(abi.ImauPcState), ctypes.c_double]
     _lib = L
     return L
 
@@ -169,12 +172,15 @@ class Context:
                                                   ctypes.c_double(time)))
         return ps
 
-    def run_ice_dynamics_direct(self, grid, cfg, tcfg, pcfg, scfg, st, SMB, BMB, tstate, t_end, geo=None, ref=None):
+    def run_ice_dynamics_direct(self, grid, cfg, tcfg, pcfg, scfg, st, SMB, BMB, tstate, t_end, geo=None, ref=None,
+                                pc=None):
         """The 'direct' time loop (run_ice_dynamics_direct with
         update_ice_thickness) from tstate.time to t_end on IceState st, whose
         velocity outputs receive the last solve; returns the final thickness.
         geo: a ProducerState for the final derived fields, or None; ref: a
-        state.RefGeo with the removal options' inputs, or None."""
+        state.RefGeo with the removal options' inputs, or None.  pc: a
+        state.PcState runs the predictor-corrector loop instead
+        (run_ice_dynamics_pc) and is updated in place."""
         import numpy as np
         g = grid.struct()
         io = st.io()
@@ -185,12 +191,14 @@ class Context:
         gio = geo.io(omit=("Hi_a", "Hb_a", "SL_a", "dHi_dt_a", "dHb_dt_a", "Ti_a")) if geo is not None else None
         vp = ctypes.c_void_p
         rio = ref.io() if ref is not None else None
-        self._check(self.L.imau_run_ice_dynamics_direct(
-            self.h, ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(tcfg), ctypes.byref(pcfg), ctypes.byref(scfg),
-            ctypes.byref(io), ctypes.cast(ctypes.pointer(rio), vp) if rio is not None else None,
-            vp(SMB.ctypes.data), vp(BMB.ctypes.data), vp(Hi.ctypes.data),
-            ctypes.cast(ctypes.pointer(gio), vp) if gio is not None else None, ctypes.byref(tstate),
-            ctypes.c_double(t_end)))
+        args = (self.h, ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(tcfg), ctypes.byref(pcfg), ctypes.byref(scfg),
+                ctypes.byref(io), ctypes.cast(ctypes.pointer(rio), vp) if rio is not None else None,
+                vp(SMB.ctypes.data), vp(BMB.ctypes.data), vp(Hi.ctypes.data),
+                ctypes.cast(ctypes.pointer(gio), vp) if gio is not None else None, ctypes.byref(tstate))
+        if pc is None:
+            self._check(self.L.imau_run_ice_dynamics_direct(*args, ctypes.c_double(t_end)))
+        else:
+            self._check(self.L.imau_run_ice_dynamics_pc(*args, ctypes.byref(pc.s), ctypes.c_double(t_end)))
         st.take_stats(io)
         return Hi
 

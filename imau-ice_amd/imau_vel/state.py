@@ -258,3 +258,35 @@ class RefGeo:
             ct = ctypes.c_int32 if arr.dtype == np.int32 else ctypes.c_double
             setattr(io, name, arr.ctypes.data_as(ctypes.POINTER(ct)))
         return io
+
+
+class PcState:
+    """The predictor-corrector state run_ice_dynamics_pc carries between time
+    steps (imau_pc_state), as the reference initialises it at the start of a
+    run (src/IMAU_ICE_main_model.f90:478-541, src/ice_dynamics_module.f90:993-997):
+    pc_zeta = 1, pc_eta = pc_eta_prev = pc_epsilon, the time steps dt_min, the
+    next velocity update at start_time_of_run, both rate fields zero."""
+
+    def __init__(self, grid: Grid, scfg, time=None):
+        self.dHidt_Hn_un = np.zeros((grid.ny, grid.nx), order="F")
+        self.dHi_dt_a = np.zeros((grid.ny, grid.nx), order="F")
+        s = abi.ImauPcState()
+        s.pc_zeta = 1.0
+        s.pc_eta = s.pc_eta_prev = scfg.pc_epsilon
+        s.dt_crit_ice = s.dt_crit_ice_prev = s.dt = scfg.dt_min
+        s.t_next = scfg.start_time_of_run if time is None else time   # the run's first step updates
+        self.s = s
+        self._bind()
+
+    def _bind(self):
+        dp = ctypes.POINTER(ctypes.c_double)
+        self.s.dHidt_Hn_un = self.dHidt_Hn_un.ctypes.data_as(dp)
+        self.s.dHi_dt_a = self.dHi_dt_a.ctypes.data_as(dp)
+
+    def copy(self) -> "PcState":
+        c = PcState.__new__(PcState)
+        c.dHidt_Hn_un, c.dHi_dt_a = self.dHidt_Hn_un.copy(order="F"), self.dHi_dt_a.copy(order="F")
+        c.s = abi.ImauPcState()
+        ctypes.memmove(ctypes.addressof(c.s), ctypes.addressof(self.s), ctypes.sizeof(c.s))
+        c._bind()
+        return c

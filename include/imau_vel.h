@@ -476,6 +476,10 @@ typedef struct imau_timestep_cfg {
   int32_t pad0_;
   double calving_threshold_thickness;          /* [m] */
   double continental_shelf_min_height;         /* [m] */
+  /* choice_timestepping = 'pc' (src/configuration_module.f90:317-320; Robinson
+   * et al. 2020, Eqs. 32-33): target truncation error [m/yr], the two
+   * exponents of the step controller, the floor of the error estimate */
+  double pc_epsilon, pc_k_I, pc_k_p, pc_eta_min;
 } imau_timestep_cfg;
 
 /* The fixed inputs update_ice_thickness reads beside the ice model
@@ -524,6 +528,38 @@ int  imau_run_ice_dynamics_direct(imau_ctx *ctx, const imau_grid *grid, const im
                                   const imau_timestep_cfg *scfg, imau_ice_io *io, const imau_refgeo_io *ref,
                                   const double *SMB_year, const double *BMB, double *Hi_end,
                                   imau_producer_io *geo, imau_timestep_state *st, double t_end);
+
+/* The predictor-corrector state of type_model_region / type_ice_model that
+ * run_ice_dynamics_pc carries from one time step to the next
+ * (src/ice_dynamics_module.f90:243-432).  At the start of a run
+ * (IMAU_ICE_main_model.f90:478-541, ice_dynamics_module.f90:993-997):
+ * pc_zeta = 1, pc_eta = pc_eta_prev = pc_epsilon, dt_crit_ice =
+ * dt_crit_ice_prev = dt = dt_min, t_next = start_time_of_run, and the two
+ * fields zero. */
+typedef struct imau_pc_state {
+  double pc_zeta, pc_eta, pc_eta_prev;     /* in / out */
+  double dt_crit_ice, dt_crit_ice_prev;    /* in / out [yr] */
+  double dt;                               /* in / out: region%dt, the last time step [yr] */
+  double t_next;                           /* in / out: t_next_SIA / _SSA / _DIVA, the next velocity update */
+  double *dHidt_Hn_un;                     /* (ny,nx) in / out: dHi/dt of the last predictor [m/yr] */
+  double *dHi_dt_a;                        /* (ny,nx) in / out: the corrected rate the steps apply [m/yr] */
+} imau_pc_state;
+
+/* The same loop with choice_timestepping = 'pc' (run_ice_dynamics_pc,
+ * src/ice_dynamics_module.f90:243-432), for every choice_ice_dynamics
+ * including DIVA: when time reaches pc->t_next, the step dt_crit_ice from
+ * calc_critical_timestep_adv (:726-762) and the truncation-error controller,
+ * the predictor dHi/dt at (H_n, u_n) and H* = max(0, H + dt ((1 + zeta/2)
+ * dHi/dt_n - zeta/2 dHi/dt_n-1)), the velocity solve at H*, dHi/dt at
+ * (H*, u_n+1), the corrector H = max(0, H_n + dt/2 (both rates)) and
+ * calc_pc_truncation_error (:763-816); every step then applies dHi_dt_a
+ * through update_ice_thickness as the direct loop does.  st->t_next_SIA /
+ * _SSA are not used; st->dt is the last step. */
+int  imau_run_ice_dynamics_pc(imau_ctx *ctx, const imau_grid *grid, const imau_cfg *cfg,
+                              const imau_thickness_cfg *tcfg, const imau_producer_cfg *pcfg,
+                              const imau_timestep_cfg *scfg, imau_ice_io *io, const imau_refgeo_io *ref,
+                              const double *SMB_year, const double *BMB, double *Hi_end, imau_producer_io *geo,
+                              imau_timestep_state *st, imau_pc_state *pc, double t_end);
 
 /* ---- ensembles (no reference counterpart: the reference solves one region
  *      grid per call, src/IMAU_ICE_program.f90:247-250) ----

@@ -2554,13 +2554,15 @@ static void remove_unconnected_shelves(const G *g, double *Hi, const int32_t *sh
   free(stack);
 }
 
-int orc_run_ice_dynamics_direct(const imau_grid *gr, const imau_cfg *c, const imau_thickness_cfg *tc,
-                                const imau_producer_cfg *pc, const imau_timestep_cfg *sc, imau_ice_io *io,
-                                const imau_refgeo_io *rg, const double *SMB, const double *BMB, double *Hi_end,
-                                imau_producer_io *geo, imau_timestep_state *st, double t_end)
+/* the time loop of run_ice_dynamics_direct (pcs == NULL) or run_ice_dynamics_pc
+ * (src/ice_dynamics_module.f90:63-242, :243-432) */
+static int run_loop(const imau_grid *gr, const imau_cfg *c, const imau_thickness_cfg *tc,
+                    const imau_producer_cfg *pc, const imau_timestep_cfg *sc, imau_ice_io *io,
+                    const imau_refgeo_io *rg, const double *SMB, const double *BMB, double *Hi_end,
+                    imau_producer_io *geo, imau_timestep_state *st, imau_pc_state *pcs, double t_end)
 {
   const int dyn = c->choice_ice_dynamics;
-  if (dyn != IMAU_DYN_NONE && dyn != IMAU_DYN_SIA && dyn != IMAU_DYN_SSA && dyn != IMAU_DYN_SIA_SSA)
+  if (!pcs && dyn != IMAU_DYN_NONE && dyn != IMAU_DYN_SIA && dyn != IMAU_DYN_SSA && dyn != IMAU_DYN_SIA_SSA)
     return fail(-2, "direct time stepping works with 'none', 'SIA', 'SSA' or 'SIA/SSA' only");
   if (!(sc->dt_startup_phase > 0.0) || !(sc->dt_min > 0.0) || sc->dt_max < sc->dt_min)
     return fail(-1, "need dt_startup_phase > 0 and 0 < dt_min <= dt_max");
@@ -2605,6 +2607,19 @@ int orc_run_ice_dynamics_direct(const imau_grid *gr, const imau_cfg *c, const im
   pio.dHi_dt_a = NULL;
   orc_producers_update(gr, pc, &pio);
   floating_margin_fraction(g, Hi, PB.ice, PB.cf, PB.shelf, fmf, heff);
+  /* predictor-corrector fields (type_ice_model :293-299) */
+  double *Hn = NULL, *Hnm1 = NULL, *Hstar = NULL, *Hold = NULL, *Hpred = NULL;
+  if (pcs) {
+    Hn = dalloc(na); Hnm1 = dalloc(na); Hstar = dalloc(na); Hold = dalloc(na); Hpred = dalloc(na);
+    memcpy(Hn, pcs->dHidt_Hn_un, na * sizeof(double));
+    memcpy(dHi, pcs->dHi_dt_a, na * sizeof(double));
+  }
+  imau_thickness_io t;  /* calc_dHi_dt at the current geometry and velocities */
+  memset(&t, 0, sizeof t);
+  t.Hi_a = Hi; t.Hb_a = io->Hb_a; t.SL_a = io->SL_a; t.u_vav_cx = u_vav; t.v_vav_cy = v_vav;
+  t.SMB_year = SMB; t.BMB = BMB; t.float_margin_frac_a = fmf;
+  t.mask_ice_a = PB.ice; t.mask_shelf_a = PB.shelf; t.mask_ocean_a = PB.ocean; t.mask_cf_a = PB.cf;
+  t.Hi_tplusdt_a = Ht;
   const int32_t nit0 = io->DIVA_SOR_nit;
   const double tol0 = io->DIVA_SOR_tol, rtol0 = io->DIVA_PETSc_rtol, atol0 = io->DIVA_PETSc_abstol;
   double time = st->time;
@@ -2629,7 +2644,56 @@ int orc_run_ice_dynamics_direct(const imau_grid *gr, const imau_cfg *c, const im
       dt_max = sc->dt_min + (sc->dt_max - sc->dt_min) * pow((sc->end_time_of_run - time) / sc->dt_startup_phase, 2);
     else
       dt_max = sc->dt_max;
-    if ((dyn == IMAU_DYN_SIA || dyn == IMAU_DYN_SIA_SSA) && time == st->t_next_SIA) {
+    if (pcs && dyn != IMAU_DYN_NONE && time == pcs->t_next) {  /* :312-415 */
+      const double dt_adv = orc_critical_timestep_adv(gr, u_vav, v_vav, sc->dt_max);
+      pcs->dt_crit_ice_prev = pcs->dt_crit_ice;
+      const double dt_from_pc = pow(sc->pc_epsilon / pcs->pc_eta, sc->pc_k_I + sc->pc_k_p) *
+                                pow(sc->pc_epsilon / pcs->pc_eta_prev, -sc->pc_k_p) * pcs->dt;
+      double dc = fmin(fmin(sc->dt_max, 2.0 * pcs->dt_crit_ice_prev), fmin(dt_adv, dt_from_pc));
+      dc = fmax(sc->dt_min, fmax(0.5 * pcs->dt_crit_ice_prev, dc));
+      dc = fmax(sc->dt_min, fmin(dt_max, dc));
+      pcs->dt_crit_ice = dc;
+      const double z = dc / pcs->dt_crit_ice_prev;
+      pcs->pc_zeta = z;
+      memcpy(Hnm1, Hn, na * sizeof(double));
+      t.dHi_dt_a = Hn;
+      if ((rc = orc_thickness_update(gr, tc, &t, dc))) break;
+      for (size_t p = 0; p < na; ++p) {
+        Hpred[p] = fmax(0.0, Hi[p] + dc * ((1.0 + z / 2.0) * Hn[p] - (z / 2.0) * Hnm1[p]));
+        Hold[p] = Hi[p];
+        Hi[p] = Hpred[p];
+      }
+      pio.dHi_dt_a = Hn;
+      pio.dHb_dt_a = io->dHb_dt_a ? io->dHb_dt_a : zero;
+      orc_producers_update(gr, pc, &pio);
+      if (dyn == IMAU_DYN_SIA || dyn == IMAU_DYN_SIA_SSA)
+        if ((rc = orc_solve_sia(gr, c, &w))) break;
+      if (dyn == IMAU_DYN_SSA || dyn == IMAU_DYN_SIA_SSA)
+        if ((rc = orc_solve_ssa(gr, c, &w, ORC_LIN_BANDED_LU))) break;
+      if (dyn == IMAU_DYN_DIVA)
+        if ((rc = orc_solve_diva(gr, c, &w, ORC_LIN_BANDED_LU))) break;
+      pcs->t_next = time + dc;
+      t.dHi_dt_a = Hstar;
+      if ((rc = orc_thickness_update(gr, tc, &t, dc))) break;
+      memcpy(Hi, Hold, na * sizeof(double));
+      pio.dHi_dt_a = Hstar;
+      orc_producers_update(gr, pc, &pio);
+      pcs->pc_eta_prev = pcs->pc_eta;
+      double eta = sc->pc_eta_min;
+      for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) {
+          const size_t p = (size_t)i * ny + j;
+          const double hc = fmax(0.0, Hold[p] + 0.5 * dc * (Hn[p] + Hstar[p]));
+          dHi[p] = (hc - Hi[p]) / dc;
+          if (i < 1 || i > nx - 2 || j < 1 || j > ny - 2 || PB.sheet[p] != 1) continue;  /* :786-808 */
+          int gl = 0;
+          for (int di = -1; di <= 1; ++di)
+            for (int dj = -1; dj <= 1; ++dj) gl |= PB.gl[(size_t)(i + di) * ny + (j + dj)];
+          if (!gl) eta = fmax(eta, fabs(z * (hc - Hpred[p]) / ((3.0 * z + 3.0) * dc)));
+        }
+      pcs->pc_eta = eta;
+    }
+    if (!pcs && (dyn == IMAU_DYN_SIA || dyn == IMAU_DYN_SIA_SSA) && time == st->t_next_SIA) {
       if ((rc = orc_solve_sia(gr, c, &w))) break;
       double d = crit_sia(g, &sc->dt_max, u3s, v3s, u3, v3, PB.Hi_cx, PB.Hi_cy, PB.dHs_dx_cx, PB.dHs_dy_cy, PB.sheet,
                           c->zeta);
@@ -2637,7 +2701,7 @@ int orc_run_ice_dynamics_direct(const imau_grid *gr, const imau_cfg *c, const im
       st->dt_crit_SIA = d;
       st->t_next_SIA = time + d;
     }
-    if ((dyn == IMAU_DYN_SSA || dyn == IMAU_DYN_SIA_SSA) && time == st->t_next_SSA) {
+    if (!pcs && (dyn == IMAU_DYN_SSA || dyn == IMAU_DYN_SIA_SSA) && time == st->t_next_SSA) {
       if ((rc = orc_solve_ssa(gr, c, &w, ORC_LIN_BANDED_LU))) break;
       double d = orc_critical_timestep_adv(gr, u_vav, v_vav, sc->dt_max);
       d = fmax(sc->dt_min, fmin(dt_max, d));
@@ -2646,20 +2710,22 @@ int orc_run_ice_dynamics_direct(const imau_grid *gr, const imau_cfg *c, const im
     }
     /* determine_timesteps_and_actions, :838-856 and :933 */
     double t_next = fmin(t_end, time + sc->dt_max);
-    if (dyn == IMAU_DYN_SIA || dyn == IMAU_DYN_SIA_SSA) t_next = fmin(t_next, st->t_next_SIA);
-    if (dyn == IMAU_DYN_SSA || dyn == IMAU_DYN_SIA_SSA) t_next = fmin(t_next, st->t_next_SSA);
+    if (pcs) {
+      if (dyn != IMAU_DYN_NONE) t_next = fmin(t_next, pcs->t_next);
+    } else {
+      if (dyn == IMAU_DYN_SIA || dyn == IMAU_DYN_SIA_SSA) t_next = fmin(t_next, st->t_next_SIA);
+      if (dyn == IMAU_DYN_SSA || dyn == IMAU_DYN_SIA_SSA) t_next = fmin(t_next, st->t_next_SSA);
+    }
     const double dt = t_next - time;
     st->dt = dt;
-    /* :224-236 */
-    if (dyn == IMAU_DYN_NONE) {
+    if (pcs) {  /* :257-260 for 'none'; :428 */
+      pcs->dt = dt;
+      if (dyn == IMAU_DYN_NONE) memset(dHi, 0, na * sizeof(double));
+      for (size_t p = 0; p < na; ++p) Ht[p] = Hi[p] + dt * dHi[p];
+    } else if (dyn == IMAU_DYN_NONE) {  /* :224-236 */
       for (size_t p = 0; p < na; ++p) { dHi[p] = 0.0; Ht[p] = Hi[p]; }
     } else {
-      imau_thickness_io t;
-      memset(&t, 0, sizeof t);
-      t.Hi_a = Hi; t.Hb_a = io->Hb_a; t.SL_a = io->SL_a; t.u_vav_cx = u_vav; t.v_vav_cy = v_vav;
-      t.SMB_year = SMB; t.BMB = BMB; t.float_margin_frac_a = fmf;
-      t.mask_ice_a = PB.ice; t.mask_shelf_a = PB.shelf; t.mask_ocean_a = PB.ocean; t.mask_cf_a = PB.cf;
-      t.dHi_dt_a = dHi; t.Hi_tplusdt_a = Ht;
+      t.dHi_dt_a = dHi;
       if ((rc = orc_thickness_update(gr, tc, &t, dt))) break;
       for (size_t p = 0; p < na; ++p) Ht[p] = Hi[p] + dt * dHi[p];
     }
@@ -2708,9 +2774,34 @@ int orc_run_ice_dynamics_direct(const imau_grid *gr, const imau_cfg *c, const im
   if (!io->v_3D_SIA_cy) free(v3s);
   if (!io->u_SSA_cx) free(uS);
   if (!io->v_SSA_cy) free(vS);
+  if (pcs) {
+    if (!rc) {
+      memcpy(pcs->dHidt_Hn_un, Hn, na * sizeof(double));
+      memcpy(pcs->dHi_dt_a, dHi, na * sizeof(double));
+    }
+    free(Hn); free(Hnm1); free(Hstar); free(Hold); free(Hpred);
+  }
   producer_bufs_free(&PB);
   free(Hi); free(fmf); free(dHi); free(Ht); free(zero); free(A3); free(Av); free(heff);
   return rc;
+}
+
+int orc_run_ice_dynamics_direct(const imau_grid *gr, const imau_cfg *c, const imau_thickness_cfg *tc,
+                                const imau_producer_cfg *pc, const imau_timestep_cfg *sc, imau_ice_io *io,
+                                const imau_refgeo_io *rg, const double *SMB, const double *BMB, double *Hi_end,
+                                imau_producer_io *geo, imau_timestep_state *st, double t_end)
+{
+  return run_loop(gr, c, tc, pc, sc, io, rg, SMB, BMB, Hi_end, geo, st, NULL, t_end);
+}
+
+int orc_run_ice_dynamics_pc(const imau_grid *gr, const imau_cfg *c, const imau_thickness_cfg *tc,
+                            const imau_producer_cfg *pc, const imau_timestep_cfg *sc, imau_ice_io *io,
+                            const imau_refgeo_io *rg, const double *SMB, const double *BMB, double *Hi_end,
+                            imau_producer_io *geo, imau_timestep_state *st, imau_pc_state *pcs, double t_end)
+{
+  if (!pcs || !pcs->dHidt_Hn_un || !pcs->dHi_dt_a) return fail(-1, "pc state fields are NULL");
+  if (pcs->t_next < st->time) return fail(-1, "pc t_next is before the current time");
+  return run_loop(gr, c, tc, pc, sc, io, rg, SMB, BMB, Hi_end, geo, st, pcs, t_end);
 }
 
 /* building blocks of the loop, exposed for unit tests */

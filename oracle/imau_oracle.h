@@ -80,6 +80,10 @@ int    orc_run_ice_dynamics_direct(const imau_grid *g, const imau_cfg *c, const 
                                    const imau_producer_cfg *pc, const imau_timestep_cfg *sc, imau_ice_io *io,
                                    const imau_refgeo_io *rg, const double *SMB, const double *BMB, double *Hi_end,
                                    imau_producer_io *geo, imau_timestep_state *st, double t_end);
+int    orc_run_ice_dynamics_pc    (const imau_grid *g, const imau_cfg *c, const imau_thickness_cfg *tc,
+                                   const imau_producer_cfg *pc, const imau_timestep_cfg *sc, imau_ice_io *io,
+                                   const imau_refgeo_io *rg, const double *SMB, const double *BMB, double *Hi_end,
+                                   imau_producer_io *geo, imau_timestep_state *st, imau_pc_state *pcs, double t_end);
 int    orc_calc_basal_conditions(const imau_grid *g, const imau_cfg *c, imau_ice_io *io, double *phi, double *tauc);
 void   orc_floating_margin_fraction(const imau_grid *g, const double *Hi, const int32_t *ice, const int32_t *cf,
                                     const int32_t *shelf, double *fmf, double *heff);

@@ -159,10 +159,11 @@ def calc_ice_rheology(grid, pcfg, ps, time=0.0, omit=()):
     return ps
 
 
-def run_ice_dynamics_direct(grid, cfg, tcfg, pcfg, scfg, st, SMB, BMB, tstate, t_end, geo=None, ref=None):
+def run_ice_dynamics_direct(grid, cfg, tcfg, pcfg, scfg, st, SMB, BMB, tstate, t_end, geo=None, ref=None, pc=None):
     """The oracle's 'direct' time loop on IceState st (velocity outputs written
     into st); returns the final thickness.  geo: a ProducerState for the final
-    derived fields, or None; ref: a RefGeo, or None."""
+    derived fields, or None; ref: a RefGeo, or None; pc: a PcState runs the
+    predictor-corrector loop (updated in place)."""
     L = lib()
     g = grid.struct()
     io = st.io()
@@ -171,11 +172,13 @@ def run_ice_dynamics_direct(grid, cfg, tcfg, pcfg, scfg, st, SMB, BMB, tstate, t
     Hi = np.zeros((grid.ny, grid.nx), order="F")
     gio = geo.io(omit=("Hi_a", "Hb_a", "SL_a", "dHi_dt_a", "dHb_dt_a", "Ti_a")) if geo is not None else None
     rio = ref.io() if ref is not None else None
-    rc = L.orc_run_ice_dynamics_direct(ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(tcfg), ctypes.byref(pcfg),
-                                       ctypes.byref(scfg), ctypes.byref(io),
-                                       ctypes.byref(rio) if rio is not None else None, _ptr(SMB), _ptr(BMB), _ptr(Hi),
-                                       ctypes.byref(gio) if gio is not None else None, ctypes.byref(tstate),
-                                       ctypes.c_double(t_end))
+    args = (ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(tcfg), ctypes.byref(pcfg), ctypes.byref(scfg),
+            ctypes.byref(io), ctypes.byref(rio) if rio is not None else None, _ptr(SMB), _ptr(BMB), _ptr(Hi),
+            ctypes.byref(gio) if gio is not None else None, ctypes.byref(tstate))
+    if pc is None:
+        rc = L.orc_run_ice_dynamics_direct(*args, ctypes.c_double(t_end))
+    else:
+        rc = L.orc_run_ice_dynamics_pc(*args, ctypes.byref(pc.s), ctypes.c_double(t_end))
     if rc != 0:
         raise RuntimeError("oracle time loop failed (%d): %s" % (rc, L.orc_last_error().decode()))
     st.take_stats(io)

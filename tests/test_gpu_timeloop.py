@@ -98,3 +98,54 @@ def test_time_loop_matches_oracle(ctx, name):
         assert gpu.stats["nonlinear_residual"] <= 1e-9, gpu.stats
     u = np.abs(ref["u_vav_cx"]).max()
     assert np.abs(gpu["u_vav_cx"] - ref["u_vav_cx"]).max() <= 1e-6 * u
+
+
+@pytest.mark.parametrize("name", ["pc_sia_dome", "pc_diva_dome", "pc_ssa_channel"])
+def test_pc_time_loop_matches_oracle(ctx, name):
+    """The predictor-corrector loop (run_ice_dynamics_pc, :243-432) against
+    the oracle's: the DIVA dome (the loop the reference runs DIVA with), the
+    SSA channel with its floating front and threshold calving, and the SIA
+    dome with the truncation-error step controller active.  The DIVA and SSA
+    cases keep the controller off (a large pc_epsilon): its step follows
+    pc_eta^0.4, a difference of two thickness estimates that rounding in the
+    solves moves by more than the 1e-8 thickness bar."""
+    from imau_vel.state import PcState
+    if name == "pc_sia_dome":
+        g, cfg, st, smb, t_end, bc = case("sia_dome")
+        eps = 2e-4
+    elif name == "pc_diva_dome":
+        g, cfg, st = geometry.dome(nx=21, ny=19, dx=60e3)
+        tight(cfg, st)
+        smb, t_end, bc, eps = 0.2, 30.0, "none", 1e30
+    else:
+        g, cfg, st, smb, t_end, bc = case("ssa_channel")
+        eps = 1e30
+    tcfg = config.thickness_config(choice_ice_dynamics=cfg.choice_ice_dynamics, ice_thickness_west_BC=bc,
+                                   ice_thickness_east_BC=bc, ice_thickness_south_BC=bc, ice_thickness_north_BC=bc)
+    pcfg = config.producer_config()
+    scfg = config.timestep_config(start_time_of_run=-1000.0, dt_max=5.0, pc_epsilon=eps)
+    gpu, ref = st.copy(), st.copy()
+    ts_g, ts_r = config.timestep_state(0.0), config.timestep_state(0.0)
+    pc_g, pc_r = PcState(g, scfg, time=0.0), PcState(g, scfg, time=0.0)
+    geo_g, geo_r = ProducerState(g, cfg.nz), ProducerState(g, cfg.nz)
+    Hg = ctx.run_ice_dynamics_direct(g, cfg, tcfg, pcfg, scfg, gpu, smb, 0.0, ts_g, t_end, geo=geo_g, pc=pc_g)
+    Hr = O.run_ice_dynamics_direct(g, cfg, tcfg, pcfg, scfg, ref, smb, 0.0, ts_r, t_end, geo=geo_r, pc=pc_r)
+    print(name, "steps", ts_g.nsteps, ts_r.nsteps, "dt_crit_ice", pc_g.s.dt_crit_ice, pc_r.s.dt_crit_ice,
+          "pc_eta", pc_g.s.pc_eta, pc_r.s.pc_eta)
+    assert ts_g.nsteps == ts_r.nsteps and ts_g.nsteps >= 4, (ts_g.nsteps, ts_r.nsteps)
+    assert ts_g.time == ts_r.time == t_end
+    for f in ("dt_crit_ice", "dt_crit_ice_prev", "pc_zeta", "t_next", "dt"):
+        assert getattr(pc_g.s, f) == pytest.approx(getattr(pc_r.s, f), rel=1e-7, abs=1e-12), f
+    assert pc_g.s.pc_eta == pytest.approx(pc_r.s.pc_eta, rel=1e-3)
+    scale = np.abs(Hr).max()
+    assert np.abs(Hg - Hr).max() <= 1e-8 * scale, np.abs(Hg - Hr).max() / scale
+    rate = max(np.abs(pc_r.dHi_dt_a).max(), 1e-30)
+    assert np.abs(pc_g.dHi_dt_a - pc_r.dHi_dt_a).max() <= 1e-6 * rate
+    assert np.abs(pc_g.dHidt_Hn_un - pc_r.dHidt_Hn_un).max() <= 1e-6 * max(np.abs(pc_r.dHidt_Hn_un).max(), 1e-30)
+    for m in ("mask_ice_a", "mask_sheet_a", "mask_shelf_a", "mask_cf_a", "mask_a"):
+        np.testing.assert_array_equal(geo_g[m], geo_r[m], err_msg=m)
+    if gpu.stats.get("picard_iterations", 0) > 0:
+        assert gpu.stats["nonlinear_residual"] <= 1e-9, gpu.stats
+    uf = "u_vav_cx"
+    u = np.abs(ref[uf]).max()
+    assert np.abs(gpu[uf] - ref[uf]).max() <= 1e-6 * u

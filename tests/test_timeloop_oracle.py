@@ -265,3 +265,101 @@ def test_ice_removals():
     np.testing.assert_array_equal(H, expect)
     with pytest.raises(RuntimeError):
         one_update(g, cfg, st.copy(), remove_shelves_larger_than_PD=1)
+
+
+# ---------------------------------------------------------------- predictor-corrector ('pc')
+def _pc_run(dt, pc, t_end=100.0, smb=0.0, **ts):
+    """The SIA dome from velocities solved at its geometry, steps fixed at dt
+    (dt_min = dt_max = dt; a huge pc_epsilon keeps the controller off)."""
+    from imau_vel.state import PcState
+    g, cfg, st = sia_dome()
+    O.solve("sia", g, cfg, st)
+    ts.setdefault("pc_epsilon", 1e30)
+    tcfg, pcfg, scfg = loop_cfgs(dt_max=dt, dt_min=dt, **ts)
+    tst = config.timestep_state(0.0)
+    p = PcState(g, scfg, time=0.0) if pc else None
+    H = O.run_ice_dynamics_direct(g, cfg, tcfg, pcfg, scfg, st, smb, 0.0, tst, t_end, pc=p)
+    return H, tst, p, g
+
+
+def test_pc_is_second_order_in_time():
+    """run_ice_dynamics_pc (src/ice_dynamics_module.f90:243-432) integrates
+    dH/dt = F(H, u(H)) with a trapezoidal corrector: halving the step divides
+    the error (against a run at an eighth of the smallest step) by about 4,
+    where the 'direct' loop's forward step divides it by about 2.  A dropped
+    or misweighted corrector term falls back to first order.  The estimate
+    pc_eta ~ |H_corr - H_pred| / dt of calc_pc_truncation_error (:763-816)
+    falls by about 4 as well, which the AB2 predictor's weights
+    (1 + zeta/2, -zeta/2) give and a forward-Euler predictor would not (2)."""
+    ratios = {}
+    for pc in (False, True):
+        ref = _pc_run(0.625, pc)[0]
+        m = ref > 1500.0                      # the interior, away from the moving margin
+        errs, etas = [], []
+        for dt in (10.0, 5.0, 2.5):
+            H, tst, p, _ = _pc_run(dt, pc)
+            assert tst.nsteps == round(100.0 / dt)
+            errs.append(np.abs(H - ref)[m].max())
+            if pc:
+                etas.append(p.s.pc_eta)
+        ratios[pc] = [errs[k] / errs[k + 1] for k in range(2)]
+        if pc:
+            assert all(etas[k] / etas[k + 1] > 3.5 for k in range(2)), etas
+    assert all(r > 3.5 for r in ratios[True]), ratios
+    assert all(1.7 < r < 2.6 for r in ratios[False]), ratios
+
+
+def test_pc_conserves_mass_and_keeps_its_state():
+    """With a uniform SMB and no outflow the corrected rate (the mean of two
+    conservative flux divergences) adds exactly SMB x area x time; the state
+    handed back continues the run: two calls of 50 yr equal one of 100 yr."""
+    smb = 0.3
+    g, cfg, st = sia_dome()
+    V0 = st["Hi_a"].sum()
+    H, tst, p, g = _pc_run(5.0, True, smb=smb)
+    assert H.sum() - V0 == pytest.approx(smb * g.nx * g.ny * 100.0, rel=1e-10)
+    from imau_vel.state import PcState
+    g, cfg, st = sia_dome()
+    O.solve("sia", g, cfg, st)
+    tcfg, pcfg, scfg = loop_cfgs(dt_max=5.0, dt_min=5.0, pc_epsilon=1e30)
+    tst2, p2 = config.timestep_state(0.0), PcState(g, scfg, time=0.0)
+    st["Hi_a"] = O.run_ice_dynamics_direct(g, cfg, tcfg, pcfg, scfg, st, smb, 0.0, tst2, 50.0, pc=p2)
+    H2 = O.run_ice_dynamics_direct(g, cfg, tcfg, pcfg, scfg, st, smb, 0.0, tst2, 100.0, pc=p2)
+    np.testing.assert_array_equal(H2, H)
+    assert p2.s.pc_eta == p.s.pc_eta and tst2.nsteps == 10
+
+
+def test_pc_step_controller_bounds():
+    """The step of each velocity update (:316-323): within [dt_min, dt_max],
+    within [0.5, 2] times the previous one and below the advective limit of
+    the velocities it starts from; with a small pc_epsilon the truncation
+    error, not the advective limit, sets it, and the error estimate stays
+    near pc_epsilon.  Each call below runs just past one update."""
+    from imau_vel.state import PcState
+    g, cfg, st = sia_dome()
+    O.solve("sia", g, cfg, st)
+    tcfg, pcfg, scfg = loop_cfgs(dt_max=50.0, dt_min=0.01, pc_epsilon=2e-4)
+    tst, p = config.timestep_state(0.0), PcState(g, scfg, time=0.0)
+    seq, etas = [p.s.dt_crit_ice], []
+    for k in range(25):
+        adv = O.critical_timestep_adv(g, st["u_vav_cx"], st["v_vav_cy"], scfg.dt_max)
+        st["Hi_a"] = O.run_ice_dynamics_direct(g, cfg, tcfg, pcfg, scfg, st, 0.0, 0.0, tst, p.s.t_next + 1e-6, pc=p)
+        seq.append(p.s.dt_crit_ice)
+        etas.append(p.s.pc_eta)
+        assert scfg.dt_min <= seq[-1] <= scfg.dt_max
+        assert 0.5 * seq[-2] * (1 - 1e-12) <= seq[-1] <= 2.0 * seq[-2] * (1 + 1e-12)
+        assert seq[-1] <= max(adv, scfg.dt_min) * (1 + 1e-12)
+    assert max(seq) < 0.5 * O.critical_timestep_adv(g, st["u_vav_cx"], st["v_vav_cy"], scfg.dt_max)
+    assert 0.2 * scfg.pc_epsilon < np.median(etas[5:]) < 5.0 * scfg.pc_epsilon, etas
+
+
+def test_pc_none_dynamics_keeps_the_geometry():
+    g, cfg, st = sia_dome()
+    config.set_config(cfg, choice_ice_dynamics="none")
+    from imau_vel.state import PcState
+    tcfg, pcfg, scfg = loop_cfgs("none", dt_max=7.0)
+    tst, p = config.timestep_state(0.0), PcState(g, scfg, time=0.0)
+    H0 = st["Hi_a"].copy()
+    H = O.run_ice_dynamics_direct(g, cfg, tcfg, pcfg, scfg, st, 0.5, 0.0, tst, 50.0, pc=p)
+    assert tst.nsteps == 8 and tst.time == 50.0
+    np.testing.assert_array_equal(H, H0)
