@@ -100,7 +100,7 @@ def test_time_loop_matches_oracle(ctx, name):
     assert np.abs(gpu["u_vav_cx"] - ref["u_vav_cx"]).max() <= 1e-6 * u
 
 
-@pytest.mark.parametrize("name", ["pc_sia_dome", "pc_diva_dome", "pc_ssa_channel"])
+@pytest.mark.parametrize("name", ["pc_sia_dome", "pc_diva_dome", "pc_ssa_channel", "pc_hybrid_dome"])
 def test_pc_time_loop_matches_oracle(ctx, name):
     """The predictor-corrector loop (run_ice_dynamics_pc, :243-432) against
     the oracle's: the DIVA dome (the loop the reference runs DIVA with), the
@@ -113,6 +113,9 @@ def test_pc_time_loop_matches_oracle(ctx, name):
     if name == "pc_sia_dome":
         g, cfg, st, smb, t_end, bc = case("sia_dome")
         eps = 2e-4
+    elif name == "pc_hybrid_dome":
+        g, cfg, st, smb, t_end, bc = case("hybrid_dome")
+        eps = 1e30
     elif name == "pc_diva_dome":
         g, cfg, st = geometry.dome(nx=21, ny=19, dx=60e3)
         tight(cfg, st)
