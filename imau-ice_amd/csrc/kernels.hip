@@ -703,6 +703,15 @@ __global__ void __launch_bounds__(IMAU_WG) k_beta(const Member *__restrict__ mem
 // column's own values; its first iteration restores them.  Writes the
 // viscosity (in place), visc_int, N_a, F2, beta_a and beta_eff_a.
 // ---------------------------------------------------------------------------
+// the FP32 column completion computes an initial guess, so it takes the
+// hardware reciprocal, log2 and exp2 (v_rcp_f32, v_log_f32, v_exp_f32: about
+// 1 ulp each) in place of the correctly rounded division and cbrt / pow; the
+// FP64 variant keeps exact arithmetic
+__device__ __forceinline__ float wdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double wdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float wpow(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
+__device__ __forceinline__ double wpow(double x, double y) { return pow(x, y); }
+
 // SLID >= 0: the kernel carries only that sliding law.  NZ > 0: the level
 // count is fixed at compile time and the column's viscosity is kept in LDS
 // across the iterations (one slot per level and thread, level-major, so the
@@ -759,12 +768,12 @@ __global__ void __launch_bounds__(IMAU_WG) k_visc_column(const Member *__restric
       for (int k = 0; k < nz; ++k) {
         const T z = (T)kp.zeta[k];
         if (bc_skip) { vk(k) = (T)0.0; continue; }
-        const T du = (it == 0) ? (T)0.0 : tb * z / vk(k);
+        const T du = (it == 0) ? (T)0.0 : wdiv(tb * z, vk(k));
         const T eps_sq = eps_h + (T)0.25 * du * du;
         const T B = B3[p3(m, k, j, i)];
-        T eta = THIRD ? B / cbrt(eps_sq) : B * pow(eps_sq, expo);
+        T eta = B * wpow(eps_sq, THIRD ? (T)(-1.0 / 3.0) : expo);
         eta = fmax(eta, vmin);
-        const T gk = (HiT / eta) * (z * z);
+        const T gk = wdiv(HiT, eta) * (z * z);
         if (k > 0) {
           const T dz = z - (T)kp.zeta[k - 1];
           vint += (T)0.5 * (eta + eta_prev) * dz;
