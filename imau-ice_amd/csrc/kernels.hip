@@ -437,21 +437,32 @@ __device__ __forceinline__ ViscCol<T> visc_col(const Member &m, const KP &kp, co
 // its four neighbours (out-of-grid neighbours 0) and the rate-factor term B
 // (:664-721, :775-806).  Multiply-adds are not contracted, so every kernel
 // that calls it rounds alike whatever the surrounding code.
+// FP32 viscosity arithmetic (the Picard loop's FP32 viscosity and the warm
+// start's column completion) takes the hardware reciprocal, log2 and exp2
+// (v_rcp_f32, v_log_f32, v_exp_f32: about 1 ulp each) in place of the
+// correctly rounded division and cbrt / pow; FP64 keeps exact arithmetic
+__device__ __forceinline__ float wdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double wdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float wpow(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
+__device__ __forceinline__ double wpow(double x, double y) { return pow(x, y); }
+__device__ __forceinline__ float wcbrt(float x) { return __builtin_amdgcn_exp2f((1.0f / 3.0f) * __builtin_amdgcn_logf(x)); }
+__device__ __forceinline__ double wcbrt(double x) { return cbrt(x); }
+
 template <bool DIVA, bool THIRD, class T>
 __device__ __forceinline__ T visc_level(const ViscCol<T> &c, T z, T vc, T vw, T ve, T vs, T vn, T B) {
 #pragma clang fp contract(off)
   T du_dz_a = (T)0.0, dv_dz_a = (T)0.0;
   if (DIVA) {
     T dl = (T)0.0, dr = (T)0.0, db = (T)0.0, dt = (T)0.0;
-    if (!c.noice_l) dl = (c.tbl / stag_pick<T>(c.i0 ? vc : vw, c.i0 ? ve : vc, c.sl)) * z;
-    if (!c.noice_r) dr = (c.tbr / stag_pick<T>(c.ie ? vw : vc, c.ie ? vc : ve, c.sr)) * z;
-    if (!c.noice_b) db = (c.tcl / stag_pick<T>(c.j0 ? vc : vs, c.j0 ? vn : vc, c.sb)) * z;
-    if (!c.noice_t) dt = (c.tcr / stag_pick<T>(c.je ? vs : vc, c.je ? vc : vn, c.st)) * z;
+    if (!c.noice_l) dl = wdiv(c.tbl, stag_pick<T>(c.i0 ? vc : vw, c.i0 ? ve : vc, c.sl)) * z;
+    if (!c.noice_r) dr = wdiv(c.tbr, stag_pick<T>(c.ie ? vw : vc, c.ie ? vc : ve, c.sr)) * z;
+    if (!c.noice_b) db = wdiv(c.tcl, stag_pick<T>(c.j0 ? vc : vs, c.j0 ? vn : vc, c.sb)) * z;
+    if (!c.noice_t) dt = wdiv(c.tcr, stag_pick<T>(c.je ? vs : vc, c.je ? vc : vn, c.st)) * z;
     du_dz_a = c.use_avg_x ? stag_pick<T>(dl, dr, c.sxa) : dl;
     dv_dz_a = c.use_avg_y ? stag_pick<T>(db, dt, c.sya) : db;
   }
   const T eps_sq = c.eps_h + (T)0.25 * (du_dz_a * du_dz_a + dv_dz_a * dv_dz_a);
-  const T eta = THIRD ? B / cbrt(eps_sq) : B * pow(eps_sq, c.expo);
+  const T eta = THIRD ? wdiv(B, wcbrt(eps_sq)) : B * wpow(eps_sq, c.expo);
   return fmax(eta, c.vmin);  // NaN (cold start 0/0) -> visc_eff_min
 }
 
@@ -461,7 +472,7 @@ template <class T>
 __device__ __forceinline__ void visc_trap(int k, T z, T zprev, T eta, T HiT, T &vint, T &F2int, T &eta_prev,
                                           T &g_prev) {
 #pragma clang fp contract(off)
-  const T gk = (HiT / eta) * (z * z);
+  const T gk = wdiv(HiT, eta) * (z * z);
   if (k > 0) {
     const T dz = z - zprev;
     vint += (T)0.5 * (eta + eta_prev) * dz;
@@ -703,14 +714,6 @@ __global__ void __launch_bounds__(IMAU_WG) k_beta(const Member *__restrict__ mem
 // column's own values; its first iteration restores them.  Writes the
 // viscosity (in place), visc_int, N_a, F2, beta_a and beta_eff_a.
 // ---------------------------------------------------------------------------
-// the FP32 column completion computes an initial guess, so it takes the
-// hardware reciprocal, log2 and exp2 (v_rcp_f32, v_log_f32, v_exp_f32: about
-// 1 ulp each) in place of the correctly rounded division and cbrt / pow; the
-// FP64 variant keeps exact arithmetic
-__device__ __forceinline__ float wdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-__device__ __forceinline__ double wdiv(double a, double b) { return a / b; }
-__device__ __forceinline__ float wpow(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
-__device__ __forceinline__ double wpow(double x, double y) { return pow(x, y); }
 
 // SLID >= 0: the kernel carries only that sliding law.  NZ > 0: the level
 // count is fixed at compile time and the column's viscosity is kept in LDS
