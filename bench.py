@@ -75,8 +75,8 @@ def VISC_BYTES_PER_CELL(nz, word):
 # 1.0: the factors a 1 GiB device copy reads; profiles/ensemble_learned_kernel_stats.md),
 # for the headline's FP32 viscosity; other precisions are not measured.
 # Re-measure after changing either kernel.
-VISC_TRAFFIC_BYTES_PER_CELL = {"mixed_visc": 311.9}
-SWEEP_TRAFFIC_BYTES_PER_CELL = {"mixed_visc": 45.7}
+VISC_TRAFFIC_BYTES_PER_CELL = {"mixed_visc": 309.7}
+SWEEP_TRAFFIC_BYTES_PER_CELL = {"mixed_visc": 45.2}
 METHODS = {
     "learned": dict(precision="mixed_visc", mg_levels=0, picard_relax_max=0.0, ksp_type=1, anderson_m=3, learned=True,
                     text="learned warm start (BF16 conv net on the matrix cores, viscosity passes), then GCR in FP64 "
