@@ -18,6 +18,11 @@
 #include "mg.h"
 #include "stencil.h"
 
+// the FP32 V-cycle (a preconditioner) divides by the diagonal with the
+// hardware reciprocal (v_rcp_f32, about 1 ulp); the FP64 one divides exactly
+__device__ __forceinline__ float mdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double mdiv(double a, double b) { return a / b; }
+
 #define TILE_SETUP                                  \
   const int tb = imau_xcd_tile();                   \
   const Tile t = tiles[tb];                         \
@@ -427,7 +432,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_cgs(CLevelT<T> L, int color, const 
     crow(L, cm, comp, J, I, L.xu, L.xv, S, D);
     const long long p = cp2(cm, J, I);
     const T b = (comp == 0 ? L.bu : L.bv)[p];
-    (comp == 0 ? L.xu : L.xv)[p] = (D != (T)0.0) ? (b - S) / D : (T)0.0;
+    (comp == 0 ? L.xu : L.xv)[p] = (D != (T)0.0) ? mdiv(b - S, D) : (T)0.0;
   }
 }
 
@@ -486,7 +491,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_cgs_fused(CLevelT<T> L, const T *__
         S += cf[k * L.na + p] * sx[cc][(II - Ib) * CRJ + (JJ - Jb)];
       }
       const T bb = (comp == 0 ? L.bu : L.bv)[p];
-      sx[comp][li * CRJ + lj] = (D != (T)0.0) ? (bb - S) / D : (T)0.0;
+      sx[comp][li * CRJ + lj] = (D != (T)0.0) ? mdiv(bb - S, D) : (T)0.0;
     }
     __syncthreads();
   }
@@ -588,12 +593,12 @@ __global__ void __launch_bounds__(1024) k_coarsest(CLevelT<T> L, const int *__re
           T S = (T)0.0;
 #pragma unroll
           for (int k = 1; k < 9; ++k) S += cu[k] * sx[nu[k]];
-          sx[q] = (cu[0] != (T)0.0) ? (bu - S) / cu[0] : (T)0.0;
+          sx[q] = (cu[0] != (T)0.0) ? mdiv(bu - S, cu[0]) : (T)0.0;
         } else if (color >= 2 && vv) {
           T S = (T)0.0;
 #pragma unroll
           for (int k = 1; k < 9; ++k) S += cv[k] * sx[nv[k]];
-          sx[IMAU_CMAX * IMAU_CMAX + q] = (cv[0] != (T)0.0) ? (bv - S) / cv[0] : (T)0.0;
+          sx[IMAU_CMAX * IMAU_CMAX + q] = (cv[0] != (T)0.0) ? mdiv(bv - S, cv[0]) : (T)0.0;
         }
       }
       __syncthreads();
@@ -632,7 +637,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_smooth0(const Member *__restrict__ 
       T S, D;
       if (isu) u_sd<T>(m, kp, c, j, i, rt, U, V, S, D);
       else v_sd<T>(m, kp, c, j, i, rt, U, V, S, D);
-      (isu ? x.u : x.v)[p] = ((isu ? y.u : y.v)[p] - S) / D;
+      (isu ? x.u : x.v)[p] = mdiv((isu ? y.u : y.v)[p] - S, D);
     } else {
       if (i < nx - 1 && is_bnd(c.rt_u[p])) {
         const uint8_t rt = c.rt_u[p];
@@ -725,7 +730,7 @@ __global__ void __launch_bounds__(IMAU_WG) k_smooth_fused(const Member *__restri
       T S, D;
       if (isu) u_sd<T>(m, kp, c, j, i, rt, U, V, S, D);
       else v_sd<T>(m, kp, c, j, i, rt, U, V, S, D);
-      (isu ? su : sv)[li * FRJ + lj] = ((isu ? y.u : y.v)[p] - S) / D;
+      (isu ? su : sv)[li * FRJ + lj] = mdiv((isu ? y.u : y.v)[p] - S, D);
     }
     __syncthreads();
   }
