@@ -246,7 +246,7 @@ def main():
     ap.add_argument("--method", choices=tuple(METHODS), default="learned")
     ap.add_argument("--warm-iters", type=int, default=0,
                     help="viscosity passes completing the learned warm start (warm_visc_iters)")
-    ap.add_argument("--warm-column", type=int, default=10,
+    ap.add_argument("--warm-column", type=int, default=9,
                     help="column-local completion iterations before those passes (warm_column_iters)")
     ap.add_argument("--warm-relax", type=float, default=0.7,
                     help="relaxation of the unmixed Picard steps of a warm-started solve")

@@ -363,7 +363,7 @@ class WarmStartSolver:
     state (the warm start replaces the velocities, the completion passes the
     viscosity and basal stress)."""
 
-    def __init__(self, ctx, problem, net=None, warm_iters=0, column_iters=10, relax=0.7, stall_window=4,
+    def __init__(self, ctx, problem, net=None, warm_iters=0, column_iters=9, relax=0.7, stall_window=4,
                  stall_drop=10.0):
         self.net = net if net is not None else Net.load(ctx)
         self.problem = problem
