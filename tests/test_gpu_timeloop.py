@@ -152,3 +152,23 @@ def test_pc_time_loop_matches_oracle(ctx, name):
     uf = "u_vav_cx"
     u = np.abs(ref[uf]).max()
     assert np.abs(gpu[uf] - ref[uf]).max() <= 1e-6 * u
+
+
+def test_pc_rejects_bad_state(ctx):
+    """The library rejects, before any device work, an update timer behind the
+    clock and a non-positive controller state (IMAU_ERR_CONFIG), as the oracle
+    does for the timer."""
+    from imau_vel.state import PcState
+    g, cfg, st, smb, t_end, bc = case("sia_dome")
+    tcfg = config.thickness_config(choice_ice_dynamics=cfg.choice_ice_dynamics)
+    pcfg = config.producer_config()
+    scfg = config.timestep_config(start_time_of_run=-1000.0, dt_max=5.0)
+    p = PcState(g, scfg, time=-1.0)
+    with pytest.raises(RuntimeError, match="before the current time"):
+        ctx.run_ice_dynamics_direct(g, cfg, tcfg, pcfg, scfg, st.copy(), smb, 0.0, config.timestep_state(0.0), 5.0,
+                                    pc=p)
+    p = PcState(g, scfg, time=0.0)
+    p.s.pc_eta = 0.0
+    with pytest.raises(RuntimeError, match="must be positive"):
+        ctx.run_ice_dynamics_direct(g, cfg, tcfg, pcfg, scfg, st.copy(), smb, 0.0, config.timestep_state(0.0), 5.0,
+                                    pc=p)

@@ -363,3 +363,14 @@ def test_pc_none_dynamics_keeps_the_geometry():
     H = O.run_ice_dynamics_direct(g, cfg, tcfg, pcfg, scfg, st, 0.5, 0.0, tst, 50.0, pc=p)
     assert tst.nsteps == 8 and tst.time == 50.0
     np.testing.assert_array_equal(H, H0)
+
+
+def test_pc_rejects_a_timer_behind_the_clock():
+    """The update timer must not lie before the current time (the loop would
+    step backwards); the state's rate fields must be given."""
+    from imau_vel.state import PcState
+    g, cfg, st = sia_dome()
+    tcfg, pcfg, scfg = loop_cfgs(dt_max=5.0)
+    p = PcState(g, scfg, time=-1.0)
+    with pytest.raises(RuntimeError, match="before the current time"):
+        O.run_ice_dynamics_direct(g, cfg, tcfg, pcfg, scfg, st, 0.0, 0.0, config.timestep_state(0.0), 10.0, pc=p)
